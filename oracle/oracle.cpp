@@ -1,0 +1,455 @@
+// ORACLE — test infrastructure only. Only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg may load this library, and only as the checker
+// (or as the timed CPU baseline). The product path never calls it.
+//
+// CPU restatement of Muscade.jl's element-evaluation-and-assembly path:
+//   dof numbering          src/ModelDescription.jl:195-263   (orc_number_dofs)
+//   asmvec!/asmmat!        src/Assemble.jl:340-357,373-448   (orc_model_prepare_sweepx)
+//   assemble!/addin!       src/Assemble.jl:470-505,559-608, src/SweepX.jl:39-96 (orc_model_assemble_sweepx)
+//   element residuals      toolbox/BeamElement.jl, src/BasicElements.jl, test/SomeElements.jl
+//   rotations              toolbox/Rotations.jl (orc_sinc1, orc_scac_d, orc_rodrigues ...)
+// Parity pins: see the header of each test module in tests/ for the golden
+// literal (reference file:line) each function is checked against.
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <tuple>
+#include <vector>
+#include "beam.hpp"
+#include "elements.hpp"
+
+using namespace orc;
+
+enum Kind { K_BEAM = 0, K_HOLD = 1, K_DOFLOAD = 2, K_ZERO = 3, K_TURBINE = 4, K_ANCHOR = 5 };
+
+struct EType {
+    int kind, nele, nX, nU, nA;
+    int udof;
+    std::vector<int64_t> iX, iU, iA;   // 1-based model dof indices, [iele*n + k]
+    std::vector<double> scaleX;        // per element dof (dis.dis[ieletyp].scale.X)
+    std::vector<EulerBeam3D> beams;
+    std::vector<double> F;             // DofLoad values
+    std::vector<Turbine> turbines;
+    std::vector<AnchorLine> anchors;
+    std::vector<int64_t> asm1, asm2;   // asm[1,ieletyp] (nX×nele), asm[2,ieletyp] (nX²×nele), 1-based, 0 = skip
+};
+struct OModel {
+    int64_t nX, nU, nA;
+    std::vector<EType> types;
+    int64_t nnz = 0;
+    std::vector<int64_t> colptr, rowval;
+};
+
+// --------------------------------------------------------------- elements
+template <class TX, int ND>
+static int elem_residual(const EType& ty, int e, const TX (*X)[12], const double* U, const double* A, TX* R) {
+    switch (ty.kind) {
+    case K_BEAM: {
+        if (ty.udof) {
+            TX Uj[1][3];
+            for (int c = 0; c < 3; ++c) Uj[0][c] = TX(U[c]);
+            beam_residual<TX, ND, 1>(ty.beams[e], X, Uj, R);
+        } else {
+            beam_residual<TX, ND, 1>(ty.beams[e], X, nullptr, R);
+        }
+        return 0;
+    }
+    case K_HOLD: hold_residual<TX>(X[0], R); return 0;
+    case K_DOFLOAD: dofload_residual<TX>(ty.F[e], R); return 0;
+    case K_ZERO: for (int i = 0; i < ty.nX; ++i) R[i] = TX(0.); return 0;
+    case K_TURBINE: turbine_residual<TX>(ty.turbines[e], X[0], A, R); return 0;
+    case K_ANCHOR: anchorline_residual<TX>(ty.anchors[e], X[0], A, R); return 0;
+    }
+    return 1;
+}
+
+static bool has_nan(double v) { return std::isnan(v); }
+template <int N, class T> static bool has_nan(const Dual<N, T>& v) {
+    if (has_nan(v.x)) return true;
+    for (int i = 0; i < N; ++i) if (has_nan(v.dx[i])) return true;
+    return false;
+}
+
+extern "C" {
+
+// ---------------------------------------------------------------- rotations
+double orc_sinc1_k(int k, double x) {
+    switch (k) { case 0: return sinc1_0(x); case 1: return sinc1_1(x); case 2: return sinc1_2(x);
+                 case 3: return sinc1_3(x); case 4: return sinc1_4(x); }
+    return 0.;
+}
+// k-th derivative of scac at x by nested variation (test/TestRotations.jl:31-55)
+double orc_scac_d(int k, double x) {
+    if (k == 0) return scac(x);
+    if (k == 1) { auto y = scac(seed<1, double>(x, 0)); return y.dx[0]; }
+    if (k == 2) { using D1 = Dual<1, double>; Dual<1, D1> v; v.x = seed<1, double>(x, 0); v.dx[0] = D1(1.);
+                  auto y = scac(v); return y.dx[0].dx[0]; }
+    if (k == 3) { using D1 = Dual<1, double>; using D2 = Dual<1, D1>; Dual<1, D2> v;
+                  v.x.x = seed<1, double>(x, 0); v.x.dx[0] = D1(1.); v.dx[0] = D2(1.);
+                  auto y = scac(v); return y.dx[0].dx[0].dx[0]; }
+    if (k == 4) { using D1 = Dual<1, double>; using D2 = Dual<1, D1>; using D3 = Dual<1, D2>; Dual<1, D3> v;
+                  v.x.x.x = seed<1, double>(x, 0); v.x.x.dx[0] = D1(1.); v.x.dx[0] = D2(1.); v.dx[0] = D3(1.);
+                  auto y = scac(v); return y.dx[0].dx[0].dx[0].dx[0]; }
+    return 0.;
+}
+void orc_rodrigues(const double* v, double* M) {
+    V3<double> a; for (int i = 0; i < 3; ++i) a[i] = v[i];
+    M3<double> r = Rodrigues(a); for (int i = 0; i < 9; ++i) M[i] = r.m[i];
+}
+void orc_rodrigues_inv(const double* M, double* v) {
+    M3<double> m; for (int i = 0; i < 9; ++i) m.m[i] = M[i];
+    V3<double> r = Rodrigues_inv(m); for (int i = 0; i < 3; ++i) v[i] = r[i];
+}
+// w = Rodrigues⁻¹(Rodrigues(v)) and its Jacobian (column-major 3×3)
+void orc_rodrigues_roundtrip(const double* v, double* w, double* J) {
+    using D = Dual<3, double>;
+    V3<D> a; for (int i = 0; i < 3; ++i) a[i] = seed<3, double>(v[i], i);
+    V3<D> r = Rodrigues_inv(Rodrigues(a));
+    for (int i = 0; i < 3; ++i) { w[i] = r[i].x; for (int j = 0; j < 3; ++j) J[i + 3 * j] = r[i].dx[j]; }
+}
+void orc_adjust(const double* u, const double* v, double* r) {
+    V3<double> a, b; for (int i = 0; i < 3; ++i) { a[i] = u[i]; b[i] = v[i]; }
+    V3<double> w = adjust(a, b); for (int i = 0; i < 3; ++i) r[i] = w[i];
+}
+
+// ---------------------------------------------------------------- beam
+// Constructor (BeamElement.jl:122-148). out: cm(3) rm(9) zgp(4) znod(2) tgm(3) tge(3)
+// ya yu yv ka ku kv (6×4) L(1) dL(4) = 53 doubles. Returns 1 if orient2 is nearly parallel.
+int orc_beam_construct(const double* c1, const double* c2, const double* orient2, double* out) {
+    BeamCrossSection mat{}; EulerBeam3D o;
+    if (make_beam(c1, c2, orient2, mat, 0, &o)) return 1;
+    double* p = out;
+    for (int i = 0; i < 3; ++i) *p++ = o.cm[i];
+    for (int i = 0; i < 9; ++i) *p++ = o.rm[i];
+    for (int i = 0; i < 4; ++i) *p++ = o.zgp[i];
+    for (int i = 0; i < 2; ++i) *p++ = o.znod[i];
+    for (int i = 0; i < 3; ++i) *p++ = o.tgm[i];
+    for (int i = 0; i < 3; ++i) *p++ = o.tge[i];
+    const double* sh[6] = {o.ya, o.yu, o.yv, o.ka, o.ku, o.kv};
+    for (int s = 0; s < 6; ++s) for (int i = 0; i < 4; ++i) *p++ = sh[s][i];
+    *p++ = o.L;
+    for (int i = 0; i < 4; ++i) *p++ = o.dL[i];
+    return 0;
+}
+
+// diffed_residual of one beam (src/Diagnostic.jl:929-961): R (12), and
+// dR/dX[der] for der < ND (each 12×12 column-major), dR/dU (12×3) if udof.
+// X: ND×12, U: 3 (Udof only). mat: 16 BeamCrossSection fields.
+int orc_beam_diffed_residual(const double* c1, const double* c2, const double* orient2, const double* mat16, int udof,
+                             int ND, const double* X, const double* U, double* R, double* dRdX, double* dRdU) {
+    BeamCrossSection mat; std::memcpy(&mat, mat16, sizeof(mat));
+    EulerBeam3D o;
+    if (make_beam(c1, c2, orient2, mat, udof, &o)) return 1;
+    // revariate{1}((;X,U,A)): partials over ND*12 X entries then U (Diagnostic.jl:951)
+    constexpr int NP = 39;
+    using TX = Dual<NP, double>;
+    TX Xd[3][12];
+    for (int d = 0; d < ND; ++d) for (int i = 0; i < 12; ++i) Xd[d][i] = seed<NP, double>(X[d * 12 + i], d * 12 + i);
+    TX Rr[12];
+    if (udof) {
+        TX Ud[1][3]; for (int c = 0; c < 3; ++c) Ud[0][c] = seed<NP, double>(U[c], ND * 12 + c);
+        if (ND == 1) beam_residual<TX, 1, 1>(o, Xd, Ud, Rr);
+        else if (ND == 2) beam_residual<TX, 2, 1>(o, Xd, Ud, Rr);
+        else beam_residual<TX, 3, 1>(o, Xd, Ud, Rr);
+    } else {
+        if (ND == 1) beam_residual<TX, 1, 1>(o, Xd, nullptr, Rr);
+        else if (ND == 2) beam_residual<TX, 2, 1>(o, Xd, nullptr, Rr);
+        else beam_residual<TX, 3, 1>(o, Xd, nullptr, Rr);
+    }
+    for (int i = 0; i < 12; ++i) {
+        R[i] = Rr[i].x;
+        for (int d = 0; d < ND; ++d) for (int j = 0; j < 12; ++j) dRdX[d * 144 + i + 12 * j] = Rr[i].dx[d * 12 + j];
+        if (udof && dRdU) for (int c = 0; c < 3; ++c) dRdU[i + 12 * c] = Rr[i].dx[ND * 12 + c];
+    }
+    return 0;
+}
+
+// residual with plain-double X (P=1 path, BeamElement.jl:151-174), ND=1..3.
+// Also returns the requestables ε and ♢κ (mid-element curvature).
+int orc_beam_residual_plain(const double* c1, const double* c2, const double* orient2, const double* mat16, int ND,
+                            const double* X, double* R, double* eps, double* kappa_mid) {
+    BeamCrossSection mat; std::memcpy(&mat, mat16, sizeof(mat));
+    EulerBeam3D o;
+    if (make_beam(c1, c2, orient2, mat, 0, &o)) return 1;
+    double Xd[3][12];
+    for (int d = 0; d < ND; ++d) for (int i = 0; i < 12; ++i) Xd[d][i] = X[d * 12 + i];
+    BeamReq<double> rq;
+    if (ND == 1) beam_residual<double, 1, 1>(o, Xd, nullptr, R, &rq);
+    else if (ND == 2) beam_residual<double, 2, 1>(o, Xd, nullptr, R, &rq);
+    else beam_residual<double, 3, 1>(o, Xd, nullptr, R, &rq);
+    if (eps) *eps = rq.eps;
+    if (kappa_mid) for (int c = 0; c < 3; ++c) kappa_mid[c] = rq.kappa_mid[c];
+    return 0;
+}
+
+// ε and ♢κ with their gradients w.r.t. X₀ (12): the strain-gauge requestables
+// (toolbox/StrainGaugeOnBeamElement.jl:70-76) differentiated as by
+// diffed_lagrangian{1} (Diagnostic.jl:851-911). Also R and the first-order
+// residual partials K (12×12). out: eps, deps(12), kap(3), dkap(3×12 row-major by comp)
+int orc_beam_gauge(const double* c1, const double* c2, const double* orient2, const double* mat16, const double* X,
+                   double* R, double* K, double* eps, double* deps, double* kap, double* dkap) {
+    BeamCrossSection mat; std::memcpy(&mat, mat16, sizeof(mat));
+    EulerBeam3D o;
+    if (make_beam(c1, c2, orient2, mat, 0, &o)) return 1;
+    using TX = Dual<12, double>;
+    TX Xd[1][12];
+    for (int i = 0; i < 12; ++i) Xd[0][i] = seed<12, double>(X[i], i);
+    TX Rr[12];
+    BeamReq<TX> rq;
+    beam_residual<TX, 1, 1>(o, Xd, nullptr, Rr, &rq);
+    for (int i = 0; i < 12; ++i) { R[i] = Rr[i].x; for (int j = 0; j < 12; ++j) K[i + 12 * j] = Rr[i].dx[j]; }
+    *eps = rq.eps.x;
+    for (int j = 0; j < 12; ++j) deps[j] = rq.eps.dx[j];
+    for (int c = 0; c < 3; ++c) { kap[c] = rq.kappa_mid[c].x; for (int j = 0; j < 12; ++j) dkap[c * 12 + j] = rq.kappa_mid[c].dx[j]; }
+    return 0;
+}
+
+// ---------------------------------------------------------------- dof numbering
+// addelement! (ModelDescription.jl:195-263) for a sequence of single-element calls.
+// Inputs: nel elements, element e has type key tkey[e] (concrete-type identity),
+// nodes nodes[noff[e] .. noff[e+1]); each type key k has a doflist given by
+// dl_off[k]..dl_off[k+1] entries of (inod 1-based, class 0/1/2, field code).
+// Outputs: ieletyp[e] (1-based, first-appearance order), iele[e] (1-based within type),
+// dofid[doff[e]+k] = class-local 1-based dof number of eledof k (doff = prefix of doflist lengths),
+// ndof[3] counts per class, and per-dof (class-local order) node and field:
+// dofnode_c / doffield_c for c=X,U,A (arrays of capacity cap).
+int orc_number_dofs(int nel, const int* tkey, const int* noff, const int* nodes, int ntkey, const int* dl_off,
+                    const int* dl_inod, const int* dl_class, const int* dl_field, int* ieletyp, int* iele, int* dofid,
+                    int64_t* ndof, int cap, int* dofnode_x, int* doffield_x, int* dofnode_u, int* doffield_u,
+                    int* dofnode_a, int* doffield_a) {
+    std::vector<int> type_of_key(ntkey, 0), count(1, 0);
+    int ntype = 0;
+    std::vector<std::pair<int, int>> doftyp;   // (class, field) in first-appearance order
+    // node → list of (idoftyp, class, class-local id)
+    std::vector<std::vector<std::tuple<int, int, int>>> nodedofs;
+    int64_t n[3] = {0, 0, 0};
+    int* dn[3] = {dofnode_x, dofnode_u, dofnode_a};
+    int* df[3] = {doffield_x, doffield_u, doffield_a};
+    int dpos = 0;
+    for (int e = 0; e < nel; ++e) {
+        int k = tkey[e];
+        if (type_of_key[k] == 0) { type_of_key[k] = ++ntype; count.push_back(0); }
+        int t = type_of_key[k];
+        ieletyp[e] = t;
+        iele[e] = ++count[t];
+        for (int q = dl_off[k]; q < dl_off[k + 1]; ++q) {
+            int nod = nodes[noff[e] + dl_inod[q] - 1];
+            int cls = dl_class[q], fld = dl_field[q];
+            int idt = -1;
+            for (size_t s = 0; s < doftyp.size(); ++s) if (doftyp[s].first == cls && doftyp[s].second == fld) { idt = (int)s; break; }
+            if (idt < 0) { doftyp.push_back({cls, fld}); idt = (int)doftyp.size() - 1; }
+            if ((int)nodedofs.size() <= nod) nodedofs.resize(nod + 1);
+            int id = -1;
+            for (auto& d : nodedofs[nod]) if (std::get<0>(d) == idt) { id = std::get<2>(d); break; }
+            if (id < 0) {
+                id = (int)(++n[cls]);
+                if (id > cap) return 2;
+                nodedofs[nod].push_back({idt, cls, id});
+                if (dn[cls]) { dn[cls][id - 1] = nod; df[cls][id - 1] = fld; }
+            }
+            dofid[dpos++] = id;
+        }
+    }
+    for (int c = 0; c < 3; ++c) ndof[c] = n[c];
+    return 0;
+}
+
+// ---------------------------------------------------------------- model
+void* orc_model_new(int64_t nX, int64_t nU, int64_t nA) {
+    OModel* m = new OModel; m->nX = nX; m->nU = nU; m->nA = nA; return m;
+}
+void orc_model_free(void* h) { delete (OModel*)h; }
+
+// Append an element type (one Muscade eletype). idx*: 1-based model dofs, nele×n (element-major).
+// params per kind:
+//   K_BEAM    : nele × 25 = c1(3) c2(3) orient2(3) mat(16)
+//   K_DOFLOAD : nele × 1  = F
+//   K_TURBINE : nele × 5  = xm(2) z seadrag skydrag
+//   K_ANCHOR  : nele × 10 = xmtop(3) dxmtop(3) xmbot(2) L buoyancy
+int orc_model_add_type(void* h, int kind, int nele, int nX, int nU, int nA, const int64_t* iX, const int64_t* iU,
+                       const int64_t* iA, const double* scaleX, const double* params, int udof) {
+    OModel* m = (OModel*)h;
+    EType t; t.kind = kind; t.nele = nele; t.nX = nX; t.nU = nU; t.nA = nA; t.udof = udof;
+    t.iX.assign(iX, iX + (size_t)nele * nX);
+    if (nU) t.iU.assign(iU, iU + (size_t)nele * nU);
+    if (nA) t.iA.assign(iA, iA + (size_t)nele * nA);
+    t.scaleX.assign(nX, 1.);
+    if (scaleX) t.scaleX.assign(scaleX, scaleX + nX);
+    if (kind == K_BEAM) {
+        t.beams.resize(nele);
+        for (int e = 0; e < nele; ++e) {
+            const double* p = params + 25 * (size_t)e;
+            BeamCrossSection mat; std::memcpy(&mat, p + 9, sizeof(mat));
+            if (make_beam(p, p + 3, p + 6, mat, udof, &t.beams[e])) return 1;
+        }
+    } else if (kind == K_DOFLOAD) {
+        t.F.assign(params, params + nele);
+    } else if (kind == K_TURBINE) {
+        t.turbines.resize(nele);
+        for (int e = 0; e < nele; ++e) { const double* p = params + 5 * (size_t)e;
+            t.turbines[e] = Turbine{{p[0], p[1]}, p[2], p[3], p[4]}; }
+    } else if (kind == K_ANCHOR) {
+        t.anchors.resize(nele);
+        for (int e = 0; e < nele; ++e) { const double* p = params + 10 * (size_t)e;
+            AnchorLine a; for (int i = 0; i < 3; ++i) { a.xmtop[i] = p[i]; a.dxmtop[i] = p[3 + i]; }
+            a.xmbot[0] = p[6]; a.xmbot[1] = p[7]; a.L = p[8]; a.buoyancy = p[9]; t.anchors[e] = a; }
+    }
+    m->types.push_back(std::move(t));
+    return 0;
+}
+int orc_model_set_dofload(void* h, int itype, const double* F) {
+    OModel* m = (OModel*)h; EType& t = m->types[itype];
+    if (t.kind != K_DOFLOAD) return 1;
+    std::copy(F, F + t.nele, t.F.begin());
+    return 0;
+}
+
+// prepare(AssemblySweepX) (SweepX.jl:24-33): asmvec! over allXdofs, then asmmat!.
+// Returns nnz; pairs are sorted lexicographically on (j,i) with ties kept in
+// traversal order, as sortperm does (Assemble.jl:373-448).
+int64_t orc_model_prepare_sweepx(void* h) {
+    OModel* m = (OModel*)h;
+    // asmvec!: with allXdofs, X[idof] = idof, so asm[1] = index.X (Assemble.jl:347-357)
+    for (auto& t : m->types) t.asm1 = t.iX;
+    // asmmat! step 1-2
+    int64_t npair = 0;
+    for (auto& t : m->types) for (int e = 0; e < t.nele; ++e) {
+        int a = 0; for (int k = 0; k < t.nX; ++k) a += t.asm1[(size_t)e * t.nX + k] != 0;
+        npair += (int64_t)a * a;
+    }
+    std::vector<std::pair<int64_t, int64_t>> A(npair);
+    int64_t ip = 0;
+    for (auto& t : m->types) for (int e = 0; e < t.nele; ++e)
+        for (int j = 0; j < t.nX; ++j) for (int i = 0; i < t.nX; ++i) {
+            int64_t ii = t.asm1[(size_t)e * t.nX + i], jj = t.asm1[(size_t)e * t.nX + j];
+            if (ii != 0 && jj != 0) A[ip++] = {jj, ii};
+        }
+    // step 3: sortperm (stable)
+    std::vector<int64_t> I(npair);
+    std::iota(I.begin(), I.end(), 0);
+    std::stable_sort(I.begin(), I.end(), [&](int64_t a, int64_t b) { return A[a] < A[b]; });
+    // step 4
+    int64_t nnz = 0;
+    for (int64_t p = 0; p < npair; ++p) if (p == 0 || A[I[p]] != A[I[p - 1]]) ++nnz;
+    std::vector<int64_t> K(npair);
+    m->colptr.assign(m->nX + 1, 0);
+    m->rowval.assign(nnz, 0);
+    m->colptr[0] = 1;
+    int64_t inz = 0, icol = 1;
+    for (int64_t p = 0; p < npair; ++p) {
+        if (p == 0 || A[I[p]] != A[I[p - 1]]) {
+            ++inz;
+            int64_t j = A[I[p]].first, i = A[I[p]].second;
+            m->rowval[inz - 1] = i;
+            while (j > icol) { ++icol; m->colptr[icol - 1] = inz; }
+        }
+        K[I[p]] = inz;
+    }
+    for (int64_t i = icol + 1; i <= m->nX + 1; ++i) m->colptr[i - 1] = nnz + 1;
+    // step 5
+    ip = 0;
+    for (auto& t : m->types) {
+        t.asm2.assign((size_t)t.nele * t.nX * t.nX, 0);
+        for (int e = 0; e < t.nele; ++e)
+            for (int j = 0; j < t.nX; ++j) for (int i = 0; i < t.nX; ++i) {
+                int64_t ii = t.asm1[(size_t)e * t.nX + i], jj = t.asm1[(size_t)e * t.nX + j];
+                if (ii != 0 && jj != 0) t.asm2[(size_t)e * t.nX * t.nX + i + t.nX * j] = K[ip++];
+            }
+    }
+    m->nnz = nnz;
+    return nnz;
+}
+void orc_model_get_pattern(void* h, int64_t* colptr, int64_t* rowval) {
+    OModel* m = (OModel*)h;
+    std::copy(m->colptr.begin(), m->colptr.end(), colptr);
+    std::copy(m->rowval.begin(), m->rowval.end(), rowval);
+}
+void orc_model_get_asm(void* h, int itype, int64_t* asm1, int64_t* asm2) {
+    OModel* m = (OModel*)h; EType& t = m->types[itype];
+    if (asm1) std::copy(t.asm1.begin(), t.asm1.end(), asm1);
+    if (asm2) std::copy(t.asm2.begin(), t.asm2.end(), asm2);
+}
+
+}  // extern "C"
+
+// assemble!{mission}(AssemblySweepX{OX}) (Assemble.jl:470-505 + SweepX.jl:39-96).
+// c = (a1,a2,a3,b1,b2,b3) Newmark coefficients; mission 0 = :step, 1 = :iter.
+// X0..X2: model X vectors (X1,X2 used for OX>=1,2); U, A model vectors; t time.
+// On a NaN the element is reported through err_type/err_ele (0-based) and 3 is returned.
+template <int OX>
+static int assemble_sweepx(OModel* m, int mission, const double* c, const double* const* Xs, const double* U,
+                           const double* A, double* rhs, double* nzval, int* err_type, int* err_ele) {
+    std::fill(rhs, rhs + m->nX, 0.);
+    std::fill(nzval, nzval + m->nnz, 0.);
+    constexpr int ND = OX + 1;
+    const bool step = (mission == 0) && OX > 0;
+    for (size_t it = 0; it < m->types.size(); ++it) {
+        const EType& t = m->types[it];
+        const int nX = t.nX;
+        if (nX == 0) continue;   // SweepX.jl:39-44
+        for (int e = 0; e < t.nele; ++e) {
+            double x[3][12];
+            for (int d = 0; d < ND; ++d) for (int k = 0; k < nX; ++k) x[d][k] = Xs[d][t.iX[(size_t)e * nX + k] - 1];
+            double Ue[3] = {0, 0, 0}, Ae[4] = {0, 0, 0, 0};
+            for (int k = 0; k < t.nU && k < 3; ++k) Ue[k] = U[t.iU[(size_t)e * t.nU + k] - 1];
+            for (int k = 0; k < t.nA && k < 4; ++k) Ae[k] = A[t.iA[(size_t)e * t.nA + k] - 1];
+            const int64_t* a1 = &t.asm1[(size_t)e * nX];
+            const int64_t* a2 = &t.asm2[(size_t)e * nX * nX];
+            if (step) {
+                // :step — reδ{1}((;X=x,r=0.),(;X=scale.X,r=1.)): 13 partials (SweepX.jl:45-59)
+                using TX = Dual<13, double>;
+                TX Xd[3][12];
+                for (int k = 0; k < nX; ++k) {
+                    TX dX; dX.dx[k] = t.scaleX[k];
+                    TX dr; dr.dx[12] = 1.;
+                    (void)dr;
+                    double a = 0., b = 0.;
+                    if (OX >= 1) a = c[1] * x[1][k] + (OX >= 2 ? c[2] * x[2][k] : 0.);
+                    if (OX >= 2) b = c[4] * x[1][k] + c[5] * x[2][k];
+                    Xd[0][k] = x[0][k] + dX;
+                    if (OX >= 1) { TX v = x[1][k] + c[0] * dX; v.dx[12] = a; Xd[1][k] = v; }
+                    if (OX >= 2) { TX v = x[2][k] + c[3] * dX; v.dx[12] = b; Xd[2][k] = v; }
+                }
+                TX R[12];
+                elem_residual<TX, ND>(t, e, Xd, Ue, Ae, R);
+                for (int i = 0; i < nX; ++i) if (has_nan(R[i])) { *err_type = (int)it; *err_ele = e; return 3; }
+                for (int i = 0; i < nX; ++i) R[i] = R[i] * t.scaleX[i];
+                for (int i = 0; i < nX; ++i) if (a1[i]) rhs[a1[i] - 1] += R[i].x;            // add_value!
+                for (int i = 0; i < nX; ++i) if (a1[i]) rhs[a1[i] - 1] -= R[i].dx[12];       // add_∂!{1,:minus} (Nx+1,)
+                for (int i = 0; i < nX; ++i) for (int j = 0; j < nX; ++j) {
+                    int64_t k = a2[i + nX * j]; if (k) nzval[k - 1] += R[i].dx[j]; }
+            } else {
+                using TX = Dual<12, double>;
+                TX Xd[3][12];
+                for (int k = 0; k < nX; ++k) {
+                    TX dX; dX.dx[k] = t.scaleX[k];   // δ{1,Nx,𝕣}(scale.X)
+                    Xd[0][k] = x[0][k] + dX;
+                    if (OX >= 1) Xd[1][k] = x[1][k] + c[0] * dX;
+                    if (OX >= 2) Xd[2][k] = x[2][k] + c[3] * dX;
+                }
+                TX R[12];
+                elem_residual<TX, ND>(t, e, Xd, Ue, Ae, R);
+                for (int i = 0; i < nX; ++i) if (has_nan(R[i])) { *err_type = (int)it; *err_ele = e; return 3; }
+                for (int i = 0; i < nX; ++i) R[i] = R[i] * t.scaleX[i];
+                for (int i = 0; i < nX; ++i) if (a1[i]) rhs[a1[i] - 1] += R[i].x;
+                for (int i = 0; i < nX; ++i) for (int j = 0; j < nX; ++j) {
+                    int64_t k = a2[i + nX * j]; if (k) nzval[k - 1] += R[i].dx[j]; }
+            }
+        }
+    }
+    return 0;
+}
+extern "C" {
+int orc_model_assemble_sweepx(void* h, int OX, int mission, const double* c, const double* X0, const double* X1,
+                              const double* X2, const double* U, const double* A, double* rhs, double* nzval,
+                              int* err_type, int* err_ele) {
+    OModel* m = (OModel*)h;
+    const double* Xs[3] = {X0, X1, X2};
+    if (OX == 0) return assemble_sweepx<0>(m, mission, c, Xs, U, A, rhs, nzval, err_type, err_ele);
+    if (OX == 1) return assemble_sweepx<1>(m, mission, c, Xs, U, A, rhs, nzval, err_type, err_ele);
+    if (OX == 2) return assemble_sweepx<2>(m, mission, c, Xs, U, A, rhs, nzval, err_type, err_ele);
+    return 1;
+}
+
+}  // extern "C"
