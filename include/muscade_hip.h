@@ -1,0 +1,121 @@
+/* muscade_hip.h — C ABI of the MI355X element-evaluation-and-assembly library.
+ *
+ * The library replaces the per-element loop of Muscade.jl's assembler
+ * (src/Assemble.jl:470-505 `assemble!`/`assemble_!`, with the SweepX `addin!`
+ * methods of src/SweepX.jl:39-96 and the scatter helpers `add_value!`/`add_∂!`
+ * of src/Assemble.jl:559-608) and the one-off index-map construction of
+ * `asmvec!`/`asmmat!` (src/Assemble.jl:340-448) that `prepare` runs
+ * (src/SweepX.jl:24-33).
+ *
+ * Conventions
+ *   - Every function returns an int status: 0 = MH_OK, otherwise an MH_ERR_*
+ *     code; mh_last_error() gives the message. Nothing throws across the ABI.
+ *   - Host pointers are borrowed for the duration of the call only.
+ *   - Dof indices crossing the ABI are 1-based int64 like Muscade's `asm`
+ *     and `Disassembler` arrays; 0 means "not in the dof group" (skip).
+ *   - Arrays described as "n×nele" are column-major with the n entries of
+ *     one element contiguous (Julia's Matrix layout, e.g. asm[1,ieletyp]).
+ *   - A context owns one HIP stream and all device buffers; it is not
+ *     re-entrant (one host thread per context).
+ */
+#ifndef MUSCADE_HIP_H
+#define MUSCADE_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MH_OK = 0,
+    MH_ERR_ARG = 1,     /* invalid argument or call order                      */
+    MH_ERR_HIP = 2,     /* HIP runtime error                                   */
+    MH_ERR_NAN = 3,     /* an element returned NaN (Assemble.jl:625-631)       */
+    MH_ERR_ORIENT = 4,  /* orient2 nearly parallel to a beam (BeamElement.jl:135) */
+    MH_ERR_SIZE = 5     /* model exceeds the 32-bit index range of the device maps */
+};
+
+/* Element kinds with a device kernel. Each Muscade element type (one
+ * `ieletyp`, i.e. one entry of model.eleobj) becomes one group of one kind. */
+enum {
+    MH_KIND_BEAM = 1,    /* EulerBeam3D{BeamCrossSection,false} (toolbox/BeamElement.jl:87-174)        */
+    MH_KIND_BEAM_U = 2,  /* EulerBeam3D{BeamCrossSection,true}: 3 U-dofs, load per length             */
+    MH_KIND_HOLD = 3,    /* Hold = DofConstraint{:X} equal, gap=x (src/BasicElements.jl:426-438,484-488) */
+    MH_KIND_DOFLOAD = 4, /* DofLoad, R = -value(t) evaluated by the host (BasicElements.jl:275-284)      */
+    MH_KIND_ZERO = 5     /* Lagrangian-only element (DofCost, BasicElements.jl:32-50): in SweepX its
+                            X-dof pairs enter the pattern and contribute structural zeros             */
+};
+
+typedef struct mh_ctx mh_ctx;
+
+/* Error detail of the last assembly: the first element (in Muscade's
+ * traversal order) whose residual or tangent contained a NaN. */
+typedef struct mh_err {
+    int32_t code;
+    int32_t igroup; /* 0-based group (= ieletyp-1) */
+    int64_t iele;   /* 0-based element within the group (= iele-1) */
+} mh_err;
+
+/* ---- context ------------------------------------------------------------ */
+int mh_ctx_create(int device, mh_ctx** ctx);
+int mh_ctx_destroy(mh_ctx* ctx);
+const char* mh_last_error(const mh_ctx* ctx);
+const char* mh_strerror(int code);
+
+/* ---- model description (Model + Disassembler, src/ModelDescription.jl,
+ *      src/Assemble.jl:32-99) ------------------------------------------------
+ * Start a model: dof counts per class and node coordinates xyz (3×nnod). */
+int mh_model_begin(mh_ctx* ctx, int64_t nXdof, int64_t nUdof, int64_t nAdof, int64_t nnod, const double* xyz);
+/* Add one element group, in Muscade's eletype order.
+ *   idxX   nX×nele  model X-dof of each element X-dof (dis.dis[ieletyp].index[iele].X)
+ *   scaleX nX       dis.dis[ieletyp].scale.X
+ *   idxU   nU×nele  model U-dof of each element U-dof (MH_KIND_BEAM_U only, else NULL)
+ *   nodes  2×nele   1-based node numbers of each beam (beams only, else NULL)
+ *   orient2 3×nele  first bending axis of each beam (EulerBeam3D's orient2 keyword,
+ *                   or column 2 of its rₘ; beams only)
+ *   mat    16×nele  BeamCrossSection fields in declaration order (beams only)
+ * Returns the 0-based group number in *igroup. */
+int mh_group_add(mh_ctx* ctx, int kind, int64_t nele, int nX, const int64_t* idxX, const double* scaleX, int nU,
+                 const int64_t* idxU, const int64_t* nodes, const double* orient2, const double* mat, int* igroup);
+
+/* ---- prepare(AssemblySweepX) (src/SweepX.jl:24-33) ----------------------
+ * Builds on the device the CSC pattern of Lλx and the sorted
+ * contribution→nzval maps that replace asm[1,:] / asm[2,:]
+ * (asmvec!/asmmat!, src/Assemble.jl:340-448). Returns nnz. */
+int mh_prepare_sweepx(mh_ctx* ctx, int64_t* nnz);
+/* The pattern exactly as Muscade's SparseMatrixCSC: colptr (nX+1), rowval (nnz), 1-based. */
+int mh_get_pattern(mh_ctx* ctx, int64_t* colptr, int64_t* rowval);
+/* asm[1,ieletyp] (nX×nele) and asm[2,ieletyp] (nX²×nele, entry i+nX·(j-1)), 1-based. */
+int mh_get_asm(mh_ctx* ctx, int igroup, int64_t* asm1, int64_t* asm2);
+
+/* ---- state (State, src/Assemble.jl:106-159) -----------------------------
+ * Upload X[ider] (ider 0,1,2 = X, X′, X″), U[0] and A. Device-resident until replaced. */
+int mh_set_X(mh_ctx* ctx, int ider, const double* x);
+int mh_set_U(mh_ctx* ctx, const double* u);
+int mh_set_A(mh_ctx* ctx, const double* a);
+/* Host-evaluated element values (DofLoad: value(t,args...) per element). */
+int mh_set_point_values(mh_ctx* ctx, int igroup, const double* values);
+
+/* ---- assemble!{mission}(::AssemblySweepX{OX}) (src/Assemble.jl:470-505,
+ *      src/SweepX.jl:39-96) ---------------------------------------------------
+ * mission: 0 = :step, 1 = :iter. c6 = Newmark coefficients (a₁,a₂,a₃,b₁,b₂,b₃),
+ * Newmarkβcoefficients{OX} (src/SweepX.jl:3-15). Results stay on the device
+ * (out.Lλ and out.Lλx.nzval); fetch them with mh_get_rhs / mh_get_nzval. */
+int mh_assemble_sweepx(mh_ctx* ctx, int OX, int mission, const double* c6, mh_err* err);
+int mh_get_rhs(mh_ctx* ctx, double* rhs);
+int mh_get_nzval(mh_ctx* ctx, double* nzval);
+/* Device pointers of the assembled arrays (valid until the next prepare). */
+int mh_device_outputs(mh_ctx* ctx, double** d_rhs, double** d_nzval);
+/* Per-element results of the last assembly for one group (for element-level
+ * checks, like diffed_residual): R (nX×nele) and ∂R/∂X·scale (nX²×nele). */
+int mh_get_element_blocks(mh_ctx* ctx, int igroup, double* R, double* K);
+
+/* ---- measurement -----------------------------------------------------------
+ * HIP-event times (ms) of the last assembly, measured on the context's stream:
+ * ms[0] element evaluation (all groups), ms[1] nzval scatter, ms[2] rhs scatter. */
+int mh_last_timings(mh_ctx* ctx, double* ms);
+int mh_sync(mh_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,720 @@
+// MI355X element-evaluation-and-assembly library (implementation of include/muscade_hip.h).
+//
+// Device data layout
+//   node coordinates   xyz[3*n + c]                      (node-major: one 24-byte gather per node)
+//   element dof maps   idx[k*nele + e]   int32, 0-based  (SoA: a wave loads 64 consecutive ints)
+//   beam data          o2[c*nele + e], sec[f*nele + e]   (SoA)
+//   element outputs    Kbuf[kbase + (i + nX*j)*nele + e] (SoA, one coalesced store per entry)
+//                      Rbuf[rbase + i*nele + e]
+//   assembly maps      ksrc[p] = Kbuf index of the p-th contribution in (column,row) order,
+//                      kseg[k] = first contribution of nz k; rsrc/rseg likewise for the rhs.
+//
+// Assembly is a deterministic segmented gather-reduce: nzval[k] = Σ Kbuf[ksrc[p]],
+// p = kseg[k]..kseg[k+1]-1, summed from 0 in increasing p. The radix sort that
+// builds ksrc is stable and its input is in Muscade's traversal order
+// (ieletyp, iele, jeledof, ieledof — src/Assemble.jl:384-395), so each nz is
+// summed in the element order of Muscade's sequential add_∂! loop
+// (src/Assemble.jl:470-505,572-586). No atomics touch any assembled value.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "../../include/muscade_hip.h"
+#include "beam_math.hpp"
+
+namespace {
+
+struct Group {
+    int kind = 0;
+    int64_t nele = 0;
+    int nX = 0, nU = 0;
+    double scaleX[12];
+    int32_t* d_idx = nullptr;    // nX × nele
+    int32_t* d_uidx = nullptr;   // nU × nele
+    int32_t* d_nodes = nullptr;  // 2 × nele
+    double* d_o2 = nullptr;      // 3 × nele
+    double* d_sec = nullptr;     // 16 × nele
+    double* d_vals = nullptr;    // nele (DofLoad values)
+    std::vector<int64_t> h_idx;  // nX × nele, 1-based (for mh_get_asm)
+    int64_t kbase = 0, rbase = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int64_t nX = 0, nU = 0, nA = 0, nnod = 0;
+    double* d_xyz = nullptr;
+    std::vector<Group> groups;
+    double* d_X[3] = {nullptr, nullptr, nullptr};
+    double* d_U = nullptr;
+    double* d_A = nullptr;
+    // prepared
+    bool prepared = false;
+    int64_t nK = 0, nR = 0, nnz = 0;
+    double* d_Kbuf = nullptr;
+    double* d_Rbuf = nullptr;
+    double* d_dRbuf = nullptr;
+    double* d_nzval = nullptr;
+    double* d_rhs = nullptr;
+    int32_t* d_colptr = nullptr;
+    int32_t* d_rowval = nullptr;
+    int32_t* d_kseg = nullptr;
+    int32_t* d_ksrc = nullptr;
+    int32_t* d_rseg = nullptr;
+    int32_t* d_rsrc = nullptr;
+    unsigned long long* d_flag = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool timed = false;
+};
+
+#define HIPCHK(ctx, call)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                    \
+            return MH_ERR_HIP;                                                                  \
+        }                                                                                       \
+    } while (0)
+
+template <class T> void dfree(T*& p) { if (p) { (void)hipFree(p); p = nullptr; } }
+
+void free_prepared(Ctx* c) {
+    dfree(c->d_Kbuf); dfree(c->d_Rbuf); dfree(c->d_dRbuf); dfree(c->d_nzval); dfree(c->d_rhs);
+    dfree(c->d_colptr); dfree(c->d_rowval); dfree(c->d_kseg); dfree(c->d_ksrc); dfree(c->d_rseg); dfree(c->d_rsrc);
+    c->prepared = false;
+}
+void free_model(Ctx* c) {
+    free_prepared(c);
+    for (auto& g : c->groups) { dfree(g.d_idx); dfree(g.d_uidx); dfree(g.d_nodes); dfree(g.d_o2); dfree(g.d_sec); dfree(g.d_vals); }
+    c->groups.clear();
+    dfree(c->d_xyz);
+    for (auto& x : c->d_X) dfree(x);
+    dfree(c->d_U); dfree(c->d_A);
+}
+
+// ------------------------------------------------------------------ kernels
+constexpr int kBlock = 256;
+
+struct BeamArgs {
+    int64_t nele;
+    const int32_t* idx;
+    const int32_t* uidx;
+    const int32_t* nodes;
+    const double* xyz;
+    const double* o2;
+    const double* sec;
+    const double* X;
+    const double* U;
+    double s[12];
+    double* K;
+    double* R;
+    unsigned long long* flag;
+    unsigned long long gtag;
+};
+
+// One element per lane. Static EulerBeam3D residual + tangent (OX=0).
+template <bool UDOF>
+__global__ __launch_bounds__(kBlock) void beam_static_kernel(BeamArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nele) return;
+    const int64_t n = a.nele;
+    double X[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) X[k] = a.X[a.idx[k * n + e]];
+    const int32_t n1 = a.nodes[e], n2 = a.nodes[n + e];
+    mh::v3 c1 = mh::mk(a.xyz[3 * (int64_t)n1], a.xyz[3 * (int64_t)n1 + 1], a.xyz[3 * (int64_t)n1 + 2]);
+    mh::v3 c2 = mh::mk(a.xyz[3 * (int64_t)n2], a.xyz[3 * (int64_t)n2 + 1], a.xyz[3 * (int64_t)n2 + 2]);
+    mh::v3 o2 = mh::mk(a.o2[e], a.o2[n + e], a.o2[2 * n + e]);
+    mh::BeamSection S;
+    S.EA = a.sec[e]; S.EI2 = a.sec[n + e]; S.EI3 = a.sec[2 * n + e]; S.GJ = a.sec[3 * n + e]; S.w = a.sec[6 * n + e];
+    mh::v3 fe = mh::mk(0., 0., S.w);
+    if (UDOF) {
+        fe.x = fe.x - a.U[a.uidx[e]];
+        fe.y = fe.y - a.U[a.uidx[n + e]];
+        fe.z = fe.z - a.U[a.uidx[2 * n + e]];
+    }
+    mh::BeamGeo G = mh::beam_geometry(c1, c2, o2);
+    mh::BeamOut out;
+    mh::beam_static(G, S, fe, X, out);
+#pragma unroll
+    for (int i = 0; i < 12; ++i) a.R[i * n + e] = out.R[i] * a.s[i];
+#pragma unroll
+    for (int i = 0; i < 12; ++i)
+#pragma unroll
+        for (int j = i; j < 12; ++j) {
+            double v = out.K[mh::tri(i, j)];
+            a.K[(i + 12 * j) * n + e] = v * a.s[i] * a.s[j];
+            if (i != j) a.K[(j + 12 * i) * n + e] = v * a.s[j] * a.s[i];
+        }
+    if (out.nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+}
+
+struct PointArgs {
+    int kind;
+    int64_t nele;
+    int nX;
+    const int32_t* idx;
+    const double* X;
+    const double* vals;
+    double s[12];
+    double* K;
+    double* R;
+    double* dR;
+    unsigned long long* flag;
+    unsigned long long gtag;
+};
+
+// Hold, DofLoad and structural-zero elements.
+__global__ __launch_bounds__(kBlock) void point_kernel(PointArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nele) return;
+    const int64_t n = a.nele;
+    const int nX = a.nX;
+    if (a.kind == MH_KIND_HOLD) {
+        // DofConstraint{:X} equal, gap(x)=x: R = (-λ, -x), ∂R = [[0,-1],[-1,0]]
+        double x = a.X[a.idx[e]], lam = a.X[a.idx[n + e]];
+        double R0 = -lam, R1 = -x;
+        a.R[e] = R0 * a.s[0];
+        a.R[n + e] = R1 * a.s[1];
+        a.K[0 * n + e] = 0.;
+        a.K[1 * n + e] = -1. * a.s[0] * a.s[1];   // (i=1,j=0): ∂R₁/∂x
+        a.K[2 * n + e] = -1. * a.s[1] * a.s[0];   // (i=0,j=1): ∂R₀/∂λ
+        a.K[3 * n + e] = 0.;
+        if (x != x || lam != lam) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+    } else if (a.kind == MH_KIND_DOFLOAD) {
+        double F = a.vals[e];
+        a.R[e] = -F * a.s[0];
+        a.K[e] = 0.;
+        if (F != F) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+    } else {
+        for (int i = 0; i < nX; ++i) a.R[i * n + e] = 0.;
+        for (int i = 0; i < nX * nX; ++i) a.K[i * n + e] = 0.;
+    }
+    if (a.dR)
+        for (int i = 0; i < nX; ++i) a.dR[i * n + e] = 0.;
+}
+
+// nzval[k] = Σ Kbuf[src[p]] over the segment of k, summed from zero in p order.
+__global__ __launch_bounds__(kBlock) void scatter_kernel(int64_t nseg, const int32_t* __restrict__ seg,
+                                                         const int32_t* __restrict__ src, const double* __restrict__ buf,
+                                                         double* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nseg) return;
+    const int32_t p0 = seg[k], p1 = seg[k + 1];
+    double acc = 0.;
+    for (int32_t p = p0; p < p1; ++p) acc += buf[src[p]];
+    out[k] = acc;
+}
+// rhs with the :step correction: for each contribution, acc += R then acc -= ∂R/∂δr
+// (add_value! then add_∂!{1,:minus}, src/SweepX.jl:56-57, per element in order).
+__global__ __launch_bounds__(kBlock) void scatter_rhs_step_kernel(int64_t nseg, const int32_t* __restrict__ seg,
+                                                                  const int32_t* __restrict__ src, const double* __restrict__ R,
+                                                                  const double* __restrict__ dR, double* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nseg) return;
+    const int32_t p0 = seg[k], p1 = seg[k + 1];
+    double acc = 0.;
+    for (int32_t p = p0; p < p1; ++p) { acc += R[src[p]]; acc -= dR[src[p]]; }
+    out[k] = acc;
+}
+
+__global__ void fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+// ---- pattern construction ----------------------------------------------------
+// Pairs of one group in traversal order (iele, jeledof, ieledof), key = J·nX + I.
+__global__ __launch_bounds__(kBlock) void gen_pairs_kernel(int64_t nele, int nX, const int32_t* idx, int64_t nXdof,
+                                                           int64_t kbase, unsigned long long* keys, int32_t* vals) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)nX * nX;
+    if (q >= nele * per) return;
+    const int64_t e = q / per;
+    const int r = (int)(q - e * per);
+    const int j = r / nX, i = r - j * nX;
+    const unsigned long long J = (unsigned long long)idx[(int64_t)j * nele + e];
+    const unsigned long long I = (unsigned long long)idx[(int64_t)i * nele + e];
+    keys[q] = J * (unsigned long long)nXdof + I;
+    vals[q] = (int32_t)(kbase + (int64_t)(i + nX * j) * nele + e);
+}
+__global__ __launch_bounds__(kBlock) void gen_rpairs_kernel(int64_t nele, int nX, const int32_t* idx, int64_t rbase,
+                                                            unsigned int* keys, int32_t* vals) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nele * nX) return;
+    const int64_t e = q / nX;
+    const int i = (int)(q - e * nX);
+    keys[q] = (unsigned int)idx[(int64_t)i * nele + e];
+    vals[q] = (int32_t)(rbase + (int64_t)i * nele + e);
+}
+template <class K>
+__global__ __launch_bounds__(kBlock) void head_kernel(int64_t n, const K* keys, int32_t* head) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    head[p] = (p == 0 || keys[p] != keys[p - 1]) ? 1 : 0;
+}
+// For each segment head: seg[id] = p, and (row, col) of the nz from the key.
+__global__ __launch_bounds__(kBlock) void seg_kernel(int64_t n, const unsigned long long* keys, const int32_t* head,
+                                                     const int32_t* id, int64_t nXdof, int32_t* seg, int32_t* rowval,
+                                                     int32_t* colk) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !head[p]) return;
+    const int32_t k = id[p] - 1;
+    seg[k] = (int32_t)p;
+    rowval[k] = (int32_t)(keys[p] % (unsigned long long)nXdof);
+    colk[k] = (int32_t)(keys[p] / (unsigned long long)nXdof);
+}
+// ptr[c] = first segment whose column is >= c  (c = 0..ncol), given sorted col[0..nseg)
+__global__ __launch_bounds__(kBlock) void gapfill_kernel(int64_t nseg, const int32_t* col, int64_t ncol, int32_t* ptr) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nseg) return;
+    const int64_t cprev = k == 0 ? -1 : col[k - 1];
+    const int64_t c = k == nseg ? ncol : col[k];
+    for (int64_t cc = cprev + 1; cc <= c; ++cc) ptr[cc] = (int32_t)k;
+}
+__global__ __launch_bounds__(kBlock) void rseg_kernel(int64_t n, const unsigned int* keys, int64_t ndof, int32_t* seg) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > n) return;
+    const int64_t kprev = p == 0 ? -1 : (int64_t)keys[p - 1];
+    const int64_t k = p == n ? ndof : (int64_t)keys[p];
+    for (int64_t cc = kprev + 1; cc <= k; ++cc) seg[cc] = (int32_t)p;
+}
+// asm[2] dump: for every contribution p of nz k, out[ksrc[p]] = k+1.
+__global__ __launch_bounds__(kBlock) void asm_dump_kernel(int64_t nnz, const int32_t* seg, const int32_t* src, int32_t* out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    for (int32_t p = seg[k]; p < seg[k + 1]; ++p) out[src[p]] = (int32_t)(k + 1);
+}
+
+inline unsigned grid(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+int check_launch(Ctx* c, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { c->err = std::string(what) + ": " + hipGetErrorString(e); return MH_ERR_HIP; }
+    return MH_OK;
+}
+
+}  // namespace
+
+// ============================================================================ C ABI
+struct mh_ctx : Ctx {};
+
+static int upload_idx(Ctx* c, int32_t** dst, const int64_t* src, int n, int64_t nele, int64_t ndof, bool allow_zero) {
+    std::vector<int32_t> h((size_t)n * nele);
+    for (int64_t e = 0; e < nele; ++e)
+        for (int k = 0; k < n; ++k) {
+            int64_t v = src[e * n + k];
+            if (v < (allow_zero ? 0 : 1) || v > ndof) { c->err = "dof index out of range"; return MH_ERR_ARG; }
+            h[(size_t)k * nele + e] = (int32_t)(v - 1);
+        }
+    HIPCHK(c, hipMalloc(dst, sizeof(int32_t) * std::max<size_t>(h.size(), 1)));
+    if (!h.empty()) HIPCHK(c, hipMemcpy(*dst, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
+    return MH_OK;
+}
+template <class T>
+static int upload_soa(Ctx* c, T** dst, const T* src, int n, int64_t nele) {
+    std::vector<T> h((size_t)n * nele);
+    for (int64_t e = 0; e < nele; ++e)
+        for (int k = 0; k < n; ++k) h[(size_t)k * nele + e] = src[e * n + k];
+    HIPCHK(c, hipMalloc(dst, sizeof(T) * std::max<size_t>(h.size(), 1)));
+    if (!h.empty()) HIPCHK(c, hipMemcpy(*dst, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    return MH_OK;
+}
+
+
+extern "C" {
+
+const char* mh_strerror(int code) {
+    switch (code) {
+    case MH_OK: return "ok";
+    case MH_ERR_ARG: return "invalid argument or call order";
+    case MH_ERR_HIP: return "HIP runtime error";
+    case MH_ERR_NAN: return "element returned NaN in R or its derivatives";
+    case MH_ERR_ORIENT: return "orient2 is nearly parallel to a beam element";
+    case MH_ERR_SIZE: return "model exceeds the 32-bit index range of the device maps";
+    }
+    return "unknown error";
+}
+
+const char* mh_last_error(const mh_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int mh_ctx_create(int device, mh_ctx** out) {
+    if (!out) return MH_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return MH_ERR_HIP;
+    mh_ctx* c = new mh_ctx;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c; return MH_ERR_HIP;
+    }
+    for (auto& e : c->ev) if (hipEventCreate(&e) != hipSuccess) { delete c; return MH_ERR_HIP; }
+    if (hipMalloc(&c->d_flag, sizeof(unsigned long long)) != hipSuccess) { delete c; return MH_ERR_HIP; }
+    *out = c;
+    return MH_OK;
+}
+
+int mh_ctx_destroy(mh_ctx* c) {
+    if (!c) return MH_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_model(c);
+    dfree(c->d_flag);
+    for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MH_OK;
+}
+
+int mh_model_begin(mh_ctx* c, int64_t nX, int64_t nU, int64_t nA, int64_t nnod, const double* xyz) {
+    if (!c || nX < 0 || nU < 0 || nA < 0 || nnod < 0 || (nnod > 0 && !xyz)) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_model(c);
+    if (nX >= INT32_MAX) { c->err = "too many X dofs"; return MH_ERR_SIZE; }
+    c->nX = nX; c->nU = nU; c->nA = nA; c->nnod = nnod;
+    if (nnod) {
+        HIPCHK(c, hipMalloc(&c->d_xyz, sizeof(double) * 3 * nnod));
+        HIPCHK(c, hipMemcpy(c->d_xyz, xyz, sizeof(double) * 3 * nnod, hipMemcpyHostToDevice));
+    }
+    for (int d = 0; d < 3; ++d) {
+        HIPCHK(c, hipMalloc(&c->d_X[d], sizeof(double) * std::max<int64_t>(nX, 1)));
+        HIPCHK(c, hipMemset(c->d_X[d], 0, sizeof(double) * std::max<int64_t>(nX, 1)));
+    }
+    HIPCHK(c, hipMalloc(&c->d_U, sizeof(double) * std::max<int64_t>(nU, 1)));
+    HIPCHK(c, hipMemset(c->d_U, 0, sizeof(double) * std::max<int64_t>(nU, 1)));
+    HIPCHK(c, hipMalloc(&c->d_A, sizeof(double) * std::max<int64_t>(nA, 1)));
+    HIPCHK(c, hipMemset(c->d_A, 0, sizeof(double) * std::max<int64_t>(nA, 1)));
+    return MH_OK;
+}
+
+int mh_group_add(mh_ctx* c, int kind, int64_t nele, int nX, const int64_t* idxX, const double* scaleX, int nU,
+                 const int64_t* idxU, const int64_t* nodes, const double* orient2, const double* mat, int* igroup) {
+    if (!c || nele < 0 || nX < 0 || nX > 12 || (nele > 0 && nX > 0 && !idxX)) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    free_prepared(c);
+    const bool beam = kind == MH_KIND_BEAM || kind == MH_KIND_BEAM_U;
+    if (beam && (nX != 12 || !nodes || !orient2 || !mat)) { c->err = "beam group needs 12 X dofs, nodes, orient2 and mat"; return MH_ERR_ARG; }
+    if (kind == MH_KIND_BEAM_U && (nU != 3 || !idxU)) { c->err = "Udof beam group needs 3 U dofs"; return MH_ERR_ARG; }
+    if (kind == MH_KIND_HOLD && nX != 2) { c->err = "Hold group needs 2 X dofs"; return MH_ERR_ARG; }
+    if (kind == MH_KIND_DOFLOAD && nX != 1) { c->err = "DofLoad group needs 1 X dof"; return MH_ERR_ARG; }
+    if (kind < MH_KIND_BEAM || kind > MH_KIND_ZERO) { c->err = "unknown element kind"; return MH_ERR_ARG; }
+    Group g;
+    g.kind = kind; g.nele = nele; g.nX = nX; g.nU = beam ? nU : 0;
+    for (int k = 0; k < 12; ++k) g.scaleX[k] = (scaleX && k < nX) ? scaleX[k] : 1.;
+    int r;
+    if ((r = upload_idx(c, &g.d_idx, idxX, nX, nele, c->nX, false))) return r;
+    g.h_idx.assign(idxX, idxX + (size_t)nX * nele);
+    if (kind == MH_KIND_BEAM_U && (r = upload_idx(c, &g.d_uidx, idxU, 3, nele, c->nU, false))) return r;
+    if (beam) {
+        std::vector<int64_t> nd((size_t)2 * nele);
+        for (int64_t e = 0; e < nele; ++e)
+            for (int k = 0; k < 2; ++k) {
+                int64_t v = nodes[e * 2 + k];
+                if (v < 1 || v > c->nnod) { c->err = "node index out of range"; return MH_ERR_ARG; }
+                nd[e * 2 + k] = v - 1;
+            }
+        // orient2 check of the constructor (BeamElement.jl:132-135), on the host
+        std::vector<double> xyz((size_t)3 * c->nnod);
+        if (c->nnod) HIPCHK(c, hipMemcpy(xyz.data(), c->d_xyz, sizeof(double) * xyz.size(), hipMemcpyDeviceToHost));
+        for (int64_t e = 0; e < nele; ++e) {
+            const double* a = &xyz[3 * nd[e * 2]];
+            const double* b = &xyz[3 * nd[e * 2 + 1]];
+            mh::BeamGeo G = mh::beam_geometry(mh::mk(a[0], a[1], a[2]), mh::mk(b[0], b[1], b[2]),
+                                              mh::mk(orient2[3 * e], orient2[3 * e + 1], orient2[3 * e + 2]));
+            if (G.bad) { c->err = "orient2 nearly parallel to beam " + std::to_string(e + 1); return MH_ERR_ORIENT; }
+        }
+        std::vector<int32_t> nd32(nd.size());
+        for (size_t i = 0; i < nd.size(); ++i) nd32[i] = (int32_t)nd[i];
+        if ((r = upload_soa<int32_t>(c, &g.d_nodes, nd32.data(), 2, nele))) return r;
+        if ((r = upload_soa<double>(c, &g.d_o2, orient2, 3, nele))) return r;
+        if ((r = upload_soa<double>(c, &g.d_sec, mat, 16, nele))) return r;
+    }
+    if (kind == MH_KIND_DOFLOAD) {
+        HIPCHK(c, hipMalloc(&g.d_vals, sizeof(double) * std::max<int64_t>(nele, 1)));
+        HIPCHK(c, hipMemset(g.d_vals, 0, sizeof(double) * std::max<int64_t>(nele, 1)));
+    }
+    c->groups.push_back(std::move(g));
+    if (igroup) *igroup = (int)c->groups.size() - 1;
+    return MH_OK;
+}
+
+int mh_prepare_sweepx(mh_ctx* c, int64_t* nnz_out) {
+    if (!c) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    free_prepared(c);
+    // contribution counts and buffer offsets, in eletype order
+    int64_t nK = 0, nR = 0;
+    for (auto& g : c->groups) {
+        g.kbase = nK; g.rbase = nR;
+        nK += g.nele * g.nX * g.nX;
+        nR += g.nele * g.nX;
+    }
+    if (nK >= INT32_MAX || nR >= INT32_MAX) { c->err = "more than 2^31 contributions"; return MH_ERR_SIZE; }
+    c->nK = nK; c->nR = nR;
+    const int64_t nX = c->nX;
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipMalloc(&c->d_Kbuf, sizeof(double) * std::max<int64_t>(nK, 1)));
+    HIPCHK(c, hipMalloc(&c->d_Rbuf, sizeof(double) * std::max<int64_t>(nR, 1)));
+    HIPCHK(c, hipMalloc(&c->d_dRbuf, sizeof(double) * std::max<int64_t>(nR, 1)));
+    HIPCHK(c, hipMalloc(&c->d_rhs, sizeof(double) * std::max<int64_t>(nX, 1)));
+    HIPCHK(c, hipMalloc(&c->d_colptr, sizeof(int32_t) * (nX + 1)));
+    HIPCHK(c, hipMalloc(&c->d_rseg, sizeof(int32_t) * (nX + 1)));
+    // ---- matrix pairs
+    unsigned long long *kin = nullptr, *kout = nullptr;
+    int32_t *vin = nullptr, *head = nullptr, *id = nullptr, *colk = nullptr;
+    HIPCHK(c, hipMalloc(&kin, sizeof(unsigned long long) * std::max<int64_t>(nK, 1)));
+    HIPCHK(c, hipMalloc(&kout, sizeof(unsigned long long) * std::max<int64_t>(nK, 1)));
+    HIPCHK(c, hipMalloc(&vin, sizeof(int32_t) * std::max<int64_t>(nK, 1)));
+    HIPCHK(c, hipMalloc(&c->d_ksrc, sizeof(int32_t) * std::max<int64_t>(nK, 1)));
+    for (auto& g : c->groups) {
+        if (g.nX == 0 || g.nele == 0) continue;
+        int64_t np = g.nele * g.nX * g.nX;
+        hipLaunchKernelGGL(gen_pairs_kernel, dim3(grid(np)), dim3(kBlock), 0, s, g.nele, g.nX, g.d_idx, nX, g.kbase,
+                           kin + g.kbase, vin + g.kbase);
+    }
+    if (int r = check_launch(c, "gen_pairs")) return r;
+    int end_bit = 1;
+    while (end_bit < 64 && ((unsigned long long)nX * (unsigned long long)nX) > (1ull << end_bit)) ++end_bit;
+    size_t tmp_bytes = 0;
+    void* tmp = nullptr;
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, kin, kout, vin, c->d_ksrc, (int)nK, 0, end_bit, s));
+    HIPCHK(c, hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kin, kout, vin, c->d_ksrc, (int)nK, 0, end_bit, s));
+    dfree(tmp);
+    HIPCHK(c, hipMalloc(&head, sizeof(int32_t) * std::max<int64_t>(nK, 1)));
+    HIPCHK(c, hipMalloc(&id, sizeof(int32_t) * std::max<int64_t>(nK, 1)));
+    hipLaunchKernelGGL(head_kernel<unsigned long long>, dim3(grid(nK)), dim3(kBlock), 0, s, nK, kout, head);
+    tmp_bytes = 0;
+    HIPCHK(c, hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, head, id, (int)nK, s));
+    HIPCHK(c, hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
+    HIPCHK(c, hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, head, id, (int)nK, s));
+    dfree(tmp);
+    int32_t nnz = 0;
+    if (nK > 0) HIPCHK(c, hipMemcpyAsync(&nnz, id + nK - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    c->nnz = nnz;
+    HIPCHK(c, hipMalloc(&c->d_rowval, sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
+    HIPCHK(c, hipMalloc(&c->d_kseg, sizeof(int32_t) * (nnz + 1)));
+    HIPCHK(c, hipMalloc(&colk, sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
+    HIPCHK(c, hipMalloc(&c->d_nzval, sizeof(double) * std::max<int64_t>(nnz, 1)));
+    hipLaunchKernelGGL(seg_kernel, dim3(grid(nK)), dim3(kBlock), 0, s, nK, kout, head, id, nX, c->d_kseg, c->d_rowval, colk);
+    int32_t nk32 = (int32_t)nK;
+    HIPCHK(c, hipMemcpyAsync(c->d_kseg + nnz, &nk32, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(gapfill_kernel, dim3(grid(nnz + 1)), dim3(kBlock), 0, s, (int64_t)nnz, colk, nX, c->d_colptr);
+    if (int r = check_launch(c, "pattern")) return r;
+    HIPCHK(c, hipStreamSynchronize(s));
+    dfree(kin); dfree(kout); dfree(vin); dfree(head); dfree(id); dfree(colk);
+    // ---- vector pairs
+    unsigned int *rkin = nullptr, *rkout = nullptr;
+    int32_t* rvin = nullptr;
+    HIPCHK(c, hipMalloc(&rkin, sizeof(unsigned int) * std::max<int64_t>(nR, 1)));
+    HIPCHK(c, hipMalloc(&rkout, sizeof(unsigned int) * std::max<int64_t>(nR, 1)));
+    HIPCHK(c, hipMalloc(&rvin, sizeof(int32_t) * std::max<int64_t>(nR, 1)));
+    HIPCHK(c, hipMalloc(&c->d_rsrc, sizeof(int32_t) * std::max<int64_t>(nR, 1)));
+    for (auto& g : c->groups) {
+        if (g.nX == 0 || g.nele == 0) continue;
+        int64_t np = g.nele * g.nX;
+        hipLaunchKernelGGL(gen_rpairs_kernel, dim3(grid(np)), dim3(kBlock), 0, s, g.nele, g.nX, g.d_idx, g.rbase,
+                           rkin + g.rbase, rvin + g.rbase);
+    }
+    int rbit = 1;
+    while (rbit < 32 && (unsigned long long)nX > (1ull << rbit)) ++rbit;
+    tmp_bytes = 0;
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, rkin, rkout, rvin, c->d_rsrc, (int)nR, 0, rbit, s));
+    HIPCHK(c, hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, rkin, rkout, rvin, c->d_rsrc, (int)nR, 0, rbit, s));
+    dfree(tmp);
+    hipLaunchKernelGGL(rseg_kernel, dim3(grid(nR + 1)), dim3(kBlock), 0, s, nR, rkout, nX, c->d_rseg);
+    if (int r = check_launch(c, "rhs pattern")) return r;
+    HIPCHK(c, hipStreamSynchronize(s));
+    dfree(rkin); dfree(rkout); dfree(rvin);
+    c->prepared = true;
+    if (nnz_out) *nnz_out = nnz;
+    return MH_OK;
+}
+
+int mh_get_pattern(mh_ctx* c, int64_t* colptr, int64_t* rowval) {
+    if (!c || !c->prepared) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<int32_t> cp(c->nX + 1), rv(c->nnz);
+    HIPCHK(c, hipMemcpy(cp.data(), c->d_colptr, sizeof(int32_t) * cp.size(), hipMemcpyDeviceToHost));
+    if (c->nnz) HIPCHK(c, hipMemcpy(rv.data(), c->d_rowval, sizeof(int32_t) * rv.size(), hipMemcpyDeviceToHost));
+    if (colptr) for (size_t i = 0; i < cp.size(); ++i) colptr[i] = (int64_t)cp[i] + 1;
+    if (rowval) for (size_t i = 0; i < rv.size(); ++i) rowval[i] = (int64_t)rv[i] + 1;
+    return MH_OK;
+}
+
+int mh_get_asm(mh_ctx* c, int igroup, int64_t* asm1, int64_t* asm2) {
+    if (!c || !c->prepared || igroup < 0 || igroup >= (int)c->groups.size()) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    const Group& g = c->groups[igroup];
+    if (asm1) std::copy(g.h_idx.begin(), g.h_idx.end(), asm1);
+    if (asm2 && g.nX > 0 && g.nele > 0) {
+        int32_t* d = nullptr;
+        HIPCHK(c, hipMalloc(&d, sizeof(int32_t) * std::max<int64_t>(c->nK, 1)));
+        hipLaunchKernelGGL(asm_dump_kernel, dim3(grid(c->nnz)), dim3(kBlock), 0, c->stream, c->nnz, c->d_kseg, c->d_ksrc, d);
+        int64_t n = g.nele * g.nX * g.nX;
+        std::vector<int32_t> h(n);
+        HIPCHK(c, hipMemcpyAsync(h.data(), d + g.kbase, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        dfree(d);
+        const int64_t per = (int64_t)g.nX * g.nX;
+        for (int64_t e = 0; e < g.nele; ++e)
+            for (int64_t r = 0; r < per; ++r) asm2[e * per + r] = h[r * g.nele + e];
+    }
+    return MH_OK;
+}
+
+int mh_set_X(mh_ctx* c, int ider, const double* x) {
+    if (!c || ider < 0 || ider > 2 || !x) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->nX) HIPCHK(c, hipMemcpyAsync(c->d_X[ider], x, sizeof(double) * c->nX, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MH_OK;
+}
+int mh_set_U(mh_ctx* c, const double* u) {
+    if (!c || (c->nU && !u)) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->nU) HIPCHK(c, hipMemcpyAsync(c->d_U, u, sizeof(double) * c->nU, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MH_OK;
+}
+int mh_set_A(mh_ctx* c, const double* a) {
+    if (!c || (c->nA && !a)) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->nA) HIPCHK(c, hipMemcpyAsync(c->d_A, a, sizeof(double) * c->nA, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MH_OK;
+}
+int mh_set_point_values(mh_ctx* c, int igroup, const double* v) {
+    if (!c || igroup < 0 || igroup >= (int)c->groups.size() || !v) return MH_ERR_ARG;
+    Group& g = c->groups[igroup];
+    if (g.kind != MH_KIND_DOFLOAD) { c->err = "group has no host-evaluated values"; return MH_ERR_ARG; }
+    HIPCHK(c, hipSetDevice(c->device));
+    if (g.nele) HIPCHK(c, hipMemcpyAsync(g.d_vals, v, sizeof(double) * g.nele, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MH_OK;
+}
+
+int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err* err) {
+    if (err) { err->code = MH_OK; err->igroup = -1; err->iele = -1; }
+    if (!c || !c->prepared || OX < 0 || OX > 2 || mission < 0 || mission > 1) return MH_ERR_ARG;
+    (void)c6;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    for (auto& g : c->groups)
+        if ((g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U) && OX > 0) {
+            c->err = "dynamic (OX>0) beam assembly is not available in this build"; return MH_ERR_ARG;
+        }
+    const bool step = (mission == 0) && OX > 0;
+    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, s, c->d_flag, ~0ull);
+    HIPCHK(c, hipEventRecord(c->ev[0], s));
+    for (size_t gi = 0; gi < c->groups.size(); ++gi) {
+        Group& g = c->groups[gi];
+        if (g.nX == 0 || g.nele == 0) continue;
+        const unsigned long long gtag = (unsigned long long)gi << 40;
+        if (g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U) {
+            BeamArgs a;
+            a.nele = g.nele; a.idx = g.d_idx; a.uidx = g.d_uidx; a.nodes = g.d_nodes; a.xyz = c->d_xyz; a.o2 = g.d_o2;
+            a.sec = g.d_sec; a.X = c->d_X[0]; a.U = c->d_U;
+            for (int k = 0; k < 12; ++k) a.s[k] = g.scaleX[k];
+            a.K = c->d_Kbuf + g.kbase; a.R = c->d_Rbuf + g.rbase; a.flag = c->d_flag; a.gtag = gtag;
+            if (g.kind == MH_KIND_BEAM_U) hipLaunchKernelGGL(beam_static_kernel<true>, dim3(grid(g.nele)), dim3(kBlock), 0, s, a);
+            else hipLaunchKernelGGL(beam_static_kernel<false>, dim3(grid(g.nele)), dim3(kBlock), 0, s, a);
+        } else {
+            PointArgs a;
+            a.kind = g.kind; a.nele = g.nele; a.nX = g.nX; a.idx = g.d_idx; a.X = c->d_X[0]; a.vals = g.d_vals;
+            for (int k = 0; k < 12; ++k) a.s[k] = g.scaleX[k];
+            a.K = c->d_Kbuf + g.kbase; a.R = c->d_Rbuf + g.rbase; a.dR = step ? c->d_dRbuf + g.rbase : nullptr;
+            a.flag = c->d_flag; a.gtag = gtag;
+            hipLaunchKernelGGL(point_kernel, dim3(grid(g.nele)), dim3(kBlock), 0, s, a);
+        }
+    }
+    if (int r = check_launch(c, "element kernels")) return r;
+    HIPCHK(c, hipEventRecord(c->ev[1], s));
+    if (c->nnz)
+        hipLaunchKernelGGL(scatter_kernel, dim3(grid(c->nnz)), dim3(kBlock), 0, s, c->nnz, c->d_kseg, c->d_ksrc, c->d_Kbuf, c->d_nzval);
+    HIPCHK(c, hipEventRecord(c->ev[2], s));
+    if (c->nX) {
+        if (step)
+            hipLaunchKernelGGL(scatter_rhs_step_kernel, dim3(grid(c->nX)), dim3(kBlock), 0, s, c->nX, c->d_rseg, c->d_rsrc,
+                               c->d_Rbuf, c->d_dRbuf, c->d_rhs);
+        else
+            hipLaunchKernelGGL(scatter_kernel, dim3(grid(c->nX)), dim3(kBlock), 0, s, c->nX, c->d_rseg, c->d_rsrc, c->d_Rbuf, c->d_rhs);
+    }
+    HIPCHK(c, hipEventRecord(c->ev[3], s));
+    if (int r = check_launch(c, "scatter kernels")) return r;
+    unsigned long long flag = 0;
+    HIPCHK(c, hipMemcpyAsync(&flag, c->d_flag, sizeof(flag), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    c->timed = true;
+    if (flag != ~0ull) {
+        int gi = (int)(flag >> 40);
+        int64_t e = (int64_t)(flag & ((1ull << 40) - 1));
+        c->err = "element " + std::to_string(e + 1) + " of element type " + std::to_string(gi + 1) + " returned NaN";
+        if (err) { err->code = MH_ERR_NAN; err->igroup = gi; err->iele = e; }
+        return MH_ERR_NAN;
+    }
+    return MH_OK;
+}
+
+int mh_get_rhs(mh_ctx* c, double* rhs) {
+    if (!c || !c->prepared || !rhs) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->nX) HIPCHK(c, hipMemcpyAsync(rhs, c->d_rhs, sizeof(double) * c->nX, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MH_OK;
+}
+int mh_get_nzval(mh_ctx* c, double* nzval) {
+    if (!c || !c->prepared || !nzval) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->nnz) HIPCHK(c, hipMemcpyAsync(nzval, c->d_nzval, sizeof(double) * c->nnz, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MH_OK;
+}
+int mh_device_outputs(mh_ctx* c, double** d_rhs, double** d_nzval) {
+    if (!c || !c->prepared) return MH_ERR_ARG;
+    if (d_rhs) *d_rhs = c->d_rhs;
+    if (d_nzval) *d_nzval = c->d_nzval;
+    return MH_OK;
+}
+int mh_get_element_blocks(mh_ctx* c, int igroup, double* R, double* K) {
+    if (!c || !c->prepared || igroup < 0 || igroup >= (int)c->groups.size()) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    const Group& g = c->groups[igroup];
+    const int64_t n = g.nele, nX = g.nX;
+    if (n == 0 || nX == 0) return MH_OK;
+    std::vector<double> hk(n * nX * nX), hr(n * nX);
+    HIPCHK(c, hipMemcpyAsync(hk.data(), c->d_Kbuf + g.kbase, sizeof(double) * hk.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hr.data(), c->d_Rbuf + g.rbase, sizeof(double) * hr.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int64_t e = 0; e < n; ++e) {
+        if (R) for (int64_t i = 0; i < nX; ++i) R[e * nX + i] = hr[i * n + e];
+        if (K) for (int64_t r = 0; r < nX * nX; ++r) K[e * nX * nX + r] = hk[r * n + e];
+    }
+    return MH_OK;
+}
+
+int mh_last_timings(mh_ctx* c, double* ms) {
+    if (!c || !ms || !c->timed) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    float t = 0.f;
+    for (int i = 0; i < 3; ++i) {
+        HIPCHK(c, hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]));
+        ms[i] = t;
+    }
+    return MH_OK;
+}
+
+int mh_sync(mh_ctx* c) {
+    if (!c) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,15 @@
+"""muscade_amd — MI355X element-evaluation-and-assembly path of Muscade.jl.
+
+Host-side mirror of Muscade's model/element/solver API for this path, calling
+hand-written HIP kernels through the C ABI of include/muscade_hip.h.
+"""
+from ._lib import LIBPATH, lib
+from .model import (Disassembler, Model, MuscadeError, State, addelement, addnode, getdof, initialize, setdof,
+                    setscale)
+from .elements import BeamCrossSection, DofCost, DofLoad, EulerBeam3D, EulerBeam3DU, Hold
+from .assemble import AssemblySweepX, DeviceContext, Newmarkβcoefficients
+from .sweepx import SweepX, solve
+
+__all__ = ["LIBPATH", "lib", "Model", "MuscadeError", "State", "Disassembler", "addnode", "addelement", "initialize",
+           "setscale", "getdof", "setdof", "BeamCrossSection", "EulerBeam3D", "EulerBeam3DU", "Hold", "DofLoad",
+           "DofCost", "AssemblySweepX", "DeviceContext", "Newmarkβcoefficients", "SweepX", "solve"]

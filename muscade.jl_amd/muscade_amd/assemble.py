@@ -1,0 +1,158 @@
+"""AssemblySweepX on the GPU: prepare / zero! / assemble! of src/SweepX.jl:17-96.
+
+The host keeps Muscade's call structure; the element loop, the index maps and
+the scatter run in libmuscade_hip.so (include/muscade_hip.h).
+"""
+import ctypes
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from ._lib import as_f64, as_i64, dptr, i64ptr, lib
+from .model import MuscadeError
+
+
+class Newmarkβcoefficients:
+    """Newmarkβcoefficients{OX}(Δt,β,γ) — src/SweepX.jl:3-15."""
+
+    def __init__(self, OX, Δt=0., β=0.25, γ=0.5):
+        self.OX, self.Δt = OX, Δt
+        if OX == 0 or Δt == 0.:
+            self.a1 = self.a2 = self.a3 = self.b1 = self.b2 = self.b3 = 0.
+        elif OX == 1:
+            self.a1, self.a2, self.a3, self.b1, self.b2, self.b3 = 1 / (γ * Δt), 1 / γ, 0., 0., 0., 0.
+        else:
+            self.a1, self.a2, self.a3 = γ / (β * Δt), γ / β, (γ / (2 * β) - 1) * Δt
+            self.b1, self.b2, self.b3 = 1 / (β * Δt ** 2), 1 / (β * Δt), 1 / (2 * β)
+
+    def array(self):
+        return np.array([self.a1, self.a2, self.a3, self.b1, self.b2, self.b3])
+
+
+class DeviceContext:
+    """Owns one mh_ctx (one HIP stream and the device buffers of one model)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        r = lib.mh_ctx_create(device, ctypes.byref(h))
+        if r != _lib.MH_OK:
+            raise RuntimeError(f"mh_ctx_create failed: {lib.mh_strerror(r).decode()} (no usable GPU {device}?)")
+        self.h = h
+
+    def check(self, r, what):
+        if r != _lib.MH_OK:
+            msg = lib.mh_last_error(self.h).decode()
+            raise MuscadeError(f"{what}: {lib.mh_strerror(r).decode()}: {msg}")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.mh_ctx_destroy(self.h)
+            self.h = None
+
+
+def upload_model(ctx, model, dis):
+    """Describe the model to the device: one group per element type, in eletype order."""
+    xyz = as_f64(model.coords()[:, :3]) if model.nnod else np.zeros((0, 3))
+    ctx.check(lib.mh_model_begin(ctx.h, model.ndof["X"], model.ndof["U"], model.ndof["A"], model.nnod,
+                                 dptr(xyz) if model.nnod else None), "mh_model_begin")
+    groups = []
+    for it, E in enumerate(model.eletypes):
+        d = dis.dis[it]
+        nele = d.X.shape[0]
+        nX = d.X.shape[1]
+        idx = as_i64(d.X)
+        sX = as_f64(d.scaleX) if nX else None
+        nodes = orient = mat = idxU = None
+        nU = 0
+        kind = E.cls.kind
+        if kind in (_lib.MH_KIND_BEAM, _lib.MH_KIND_BEAM_U):
+            nodes = as_i64(E.cat_nodes()[:, :2] + 1)
+            orient = as_f64(np.concatenate([p["orient2"] for p in E.params]))
+            mat = as_f64(np.concatenate([p["mat"] for p in E.params]))
+            if kind == _lib.MH_KIND_BEAM_U:
+                idxU = as_i64(d.U)
+                nU = 3
+        g = ctypes.c_int()
+        ctx.check(lib.mh_group_add(ctx.h, kind, nele, nX, i64ptr(idx) if nX else None, dptr(sX), nU, i64ptr(idxU),
+                                   i64ptr(nodes), dptr(orient), dptr(mat), ctypes.byref(g)), f"element type {it + 1}")
+        groups.append(g.value)
+    return groups
+
+
+class AssemblySweepX:
+    """AssemblySweepX{OX}: prepare(AssemblySweepX{OX},model,dis) + assemble!{mission} (SweepX.jl:17-96)."""
+
+    def __init__(self, OX, model, dis, device=0):
+        self.OX = OX
+        self.model, self.dis = model, dis
+        self.ctx = DeviceContext(device)
+        self.groups = upload_model(self.ctx, model, dis)
+        nnz = ctypes.c_int64()
+        self.ctx.check(lib.mh_prepare_sweepx(self.ctx.h, ctypes.byref(nnz)), "prepare")
+        self.nX = model.ndof["X"]
+        self.nnz = nnz.value
+        self.colptr = np.zeros(self.nX + 1, np.int64)
+        self.rowval = np.zeros(self.nnz, np.int64)
+        self.ctx.check(lib.mh_get_pattern(self.ctx.h, i64ptr(self.colptr), i64ptr(self.rowval)), "pattern")
+        self.c = Newmarkβcoefficients(OX)
+        self.Lλ = np.zeros(self.nX)
+        self.nzval = np.zeros(self.nnz)
+        self._indptr0 = self.colptr - 1
+        self._indices0 = self.rowval - 1
+
+    def asm(self, ieletyp):
+        """asm[1,ieletyp], asm[2,ieletyp] as Muscade builds them (1-based, element-major)."""
+        d = self.dis.dis[ieletyp]
+        nele, nX = d.X.shape
+        a1 = np.zeros((nele, nX), np.int64)
+        a2 = np.zeros((nele, nX * nX), np.int64)
+        self.ctx.check(lib.mh_get_asm(self.ctx.h, self.groups[ieletyp], i64ptr(a1), i64ptr(a2)), "asm")
+        return a1, a2
+
+    def set_state(self, state, t):
+        h = self.ctx.h
+        for ider in range(min(self.OX + 1, len(state.X))):
+            self.ctx.check(lib.mh_set_X(h, ider, dptr(as_f64(state.X[ider]))), "set X")
+        if self.model.ndof["U"]:
+            self.ctx.check(lib.mh_set_U(h, dptr(as_f64(state.U[0]))), "set U")
+        if self.model.ndof["A"]:
+            self.ctx.check(lib.mh_set_A(h, dptr(as_f64(state.A))), "set A")
+        for it, E in enumerate(self.model.eletypes):
+            if E.cls.kind == _lib.MH_KIND_DOFLOAD and E.nele:
+                vals = np.concatenate([np.full(n.shape[0], float(p["value"](t, **p["args"])))
+                                       for n, p in zip(E.nodes, E.params)])
+                self.ctx.check(lib.mh_set_point_values(h, self.groups[it], dptr(as_f64(vals))), "DofLoad values")
+
+    def assemble(self, state, mission, dbg=None, fetch=True):
+        """assemble!{:step|:iter}(out,asm,dis,model,state,Δt,dbg). mission: 'step' or 'iter'."""
+        self.set_state(state, state.time)
+        err = _lib.mh_err()
+        m = 0 if mission == "step" else 1
+        r = lib.mh_assemble_sweepx(self.ctx.h, self.OX, m, dptr(self.c.array()), ctypes.byref(err))
+        if r == _lib.MH_ERR_NAN:
+            E = self.model.eletypes[err.igroup]
+            raise MuscadeError(f"residual({E.key[0]},...) returned NaN in R, FB or derivatives",
+                               dict(dbg or {}, ieletyp=err.igroup + 1, iele=err.iele + 1, t=state.time))
+        self.ctx.check(r, "assemble")
+        if fetch:
+            self.ctx.check(lib.mh_get_rhs(self.ctx.h, dptr(self.Lλ)), "rhs")
+            self.ctx.check(lib.mh_get_nzval(self.ctx.h, dptr(self.nzval)), "nzval")
+        return self.Lλ, self.nzval
+
+    def Lλx(self):
+        """out.Lλx as a scipy CSC matrix sharing the pattern."""
+        return sp.csc_matrix((self.nzval.copy(), self._indices0, self._indptr0), shape=(self.nX, self.nX))
+
+    def element_blocks(self, ieletyp):
+        d = self.dis.dis[ieletyp]
+        nele, nX = d.X.shape
+        R = np.zeros((nele, nX))
+        K = np.zeros((nele, nX * nX))
+        self.ctx.check(lib.mh_get_element_blocks(self.ctx.h, self.groups[ieletyp], dptr(R), dptr(K)), "element blocks")
+        return R, K.reshape(nele, nX, nX).transpose(0, 2, 1)   # K[e][i][j]
+
+    def timings(self):
+        ms = np.zeros(3)
+        self.ctx.check(lib.mh_last_timings(self.ctx.h, dptr(ms)), "timings")
+        return ms

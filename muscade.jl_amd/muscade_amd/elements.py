@@ -1,0 +1,128 @@
+"""Element types of the assembly path.
+
+Each class mirrors one Muscade element: its `doflist` (src/ElementAPI.jl:79-99),
+its constructor's validation, and the identity of its concrete type (which
+decides the element-type grouping of ModelDescription.jl:203-208). The
+residuals themselves are evaluated by the HIP kernels; `kind` names the kernel.
+"""
+import numpy as np
+
+from . import _lib
+from .model import MuscadeError
+
+BEAM_FIELDS = ("EA", "EI₂", "EI₃", "GJ", "μ", "ι₁", "w", "Ca₁", "Cl₁", "Cq₁", "Ca₂", "Cl₂", "Cq₂", "Ca₃", "Cl₃", "Cq₃")
+
+
+class BeamCrossSection:
+    """BeamCrossSection(;EA,EI₂,EI₃,GJ,μ,ι₁,w=0,Ca₁..Cq₃=0) — toolbox/BeamElement.jl:6-25."""
+
+    def __init__(self, EA, EI2, EI3, GJ, mu, iota1, w=0., Ca1=0., Cl1=0., Cq1=0., Ca2=0., Cl2=0., Cq2=0., Ca3=0.,
+                 Cl3=0., Cq3=0.):
+        self.values = np.array([EA, EI2, EI3, GJ, mu, iota1, w, Ca1, Cl1, Cq1, Ca2, Cl2, Cq2, Ca3, Cl3, Cq3], float)
+
+    def __repr__(self):
+        return "BeamCrossSection(" + ", ".join(f"{n}={v:g}" for n, v in zip(BEAM_FIELDS, self.values)) + ")"
+
+
+def _per_element(value, nele, width, name):
+    a = np.asarray(value, dtype=np.float64)
+    if a.ndim == 1:
+        if a.size != width:
+            raise MuscadeError(f"{name} must have {width} entries")
+        return np.broadcast_to(a, (nele, width)).copy()
+    if a.shape != (nele, width):
+        raise MuscadeError(f"{name} must be ({nele},{width}) for per-element values")
+    return a.copy()
+
+
+def check_orient(c1, c2, orient2):
+    """The constructor's orient2 test, vectorized (BeamElement.jl:126-135)."""
+    tg = c2 - c1
+    L = np.linalg.norm(tg, axis=1)
+    t = tg / L[:, None]
+    o = orient2 / np.linalg.norm(orient2, axis=1)[:, None]
+    n = o - t * np.sum(o * t, axis=1)[:, None]
+    nn = np.linalg.norm(n, axis=1)
+    bad = ~(nn > 1e-3)
+    if bad.any():
+        raise MuscadeError("Provide a 'orient' input that is not nearly parallel to the element")
+    return L
+
+
+class EulerBeam3D:
+    """EulerBeam3D{Udof}(nod; mat, orient2=SVector(0,1,0)) — toolbox/BeamElement.jl:87-148.
+
+    `mat` is a BeamCrossSection or an (nele,16) array of per-element sections;
+    `orient2` a 3-vector or an (nele,3) array."""
+    Udof = False
+    kind = _lib.MH_KIND_BEAM
+
+    @classmethod
+    def doflist(cls):   # BeamElement.jl:109-118
+        inod = (1,) * 6 + (2,) * 6
+        field = ("t1", "t2", "t3", "r1", "r2", "r3") * 2
+        dclass = ("X",) * 12
+        if cls.Udof:
+            inod += (3, 3, 3)
+            field += ("t1", "t2", "t3")
+            dclass += ("U",) * 3
+        return inod, dclass, field
+
+    @classmethod
+    def construct(cls, model, nodes0, mat, orient2=(0., 1., 0.)):
+        nele = nodes0.shape[0]
+        m = mat.values if isinstance(mat, BeamCrossSection) else mat
+        m = _per_element(m, nele, 16, "mat")
+        o = _per_element(orient2, nele, 3, "orient2")
+        xyz = model.coords()
+        if nele:
+            check_orient(xyz[nodes0[:, 0]], xyz[nodes0[:, 1]], o)
+        return ("EulerBeam3D", "BeamCrossSection", cls.Udof), cls.doflist(), {"mat": m, "orient2": o}
+
+
+class EulerBeam3DU(EulerBeam3D):
+    """EulerBeam3D{true}: with 3 U-dofs on a third node, a distributed load per length (BeamElement.jl:114-118,167)."""
+    Udof = True
+    kind = _lib.MH_KIND_BEAM_U
+
+
+class Hold:
+    """Hold(nod; field, λfield=Symbol(:λ,field)) — src/BasicElements.jl:484-488.
+    A DofConstraint{:X} with gap(x)=x[1] and mode `equal` (BasicElements.jl:426-438)."""
+    kind = _lib.MH_KIND_HOLD
+
+    @classmethod
+    def construct(cls, model, nodes0, field, λfield=None):
+        λf = λfield if λfield is not None else "λ" + field
+        doflist = ((1, 1), ("X", "X"), (field, λf))
+        return ("DofConstraint", "X", (field,), λf, "equal"), doflist, {}
+
+
+class DofLoad:
+    """DofLoad(nod; field, value, args...) — src/BasicElements.jl:275-284. R = [-value(t, args...)].
+
+    `value` is a host callable; it is evaluated on the host at each step and
+    the values are handed to the device (no device kernel for user functions)."""
+    kind = _lib.MH_KIND_DOFLOAD
+
+    @classmethod
+    def construct(cls, model, nodes0, field, value, **args):
+        name = getattr(value, "__name__", type(value).__name__)
+        return ("DofLoad", field, name, tuple(sorted(args))), ((1,), ("X",), (field,)), {"value": value, "args": args}
+
+
+class DofCost:
+    """DofCost(nod; xinod, xfield, uinod, ufield, ainod, afield, cost, costargs) — src/BasicElements.jl:32-50.
+
+    A Lagrangian-only element: in SweepX its residual ∂L/∂Λ is identically
+    zero, but its X-dof pairs are part of the pattern (structural zeros)."""
+    kind = _lib.MH_KIND_ZERO
+
+    @classmethod
+    def construct(cls, model, nodes0, cost, xinod=(), xfield=(), uinod=(), ufield=(), ainod=(), afield=(), costargs=()):
+        inod = tuple(xinod) + tuple(uinod) + tuple(ainod)
+        dclass = ("X",) * len(xinod) + ("U",) * len(uinod) + ("A",) * len(ainod)
+        field = tuple(xfield) + tuple(ufield) + tuple(afield)
+        name = getattr(cost, "__name__", type(cost).__name__)
+        key = ("DofCost", tuple(xinod), tuple(xfield), tuple(uinod), tuple(ufield), tuple(ainod), tuple(afield), name)
+        return key, (inod, dclass, field), {"cost": cost, "costargs": costargs}

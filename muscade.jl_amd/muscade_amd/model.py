@@ -1,0 +1,325 @@
+"""Model description, dof numbering, Disassembler and State.
+
+Host-side mirror of Muscade's src/ModelDescription.jl and the
+`Disassembler`/`State`/`DofGroup` parts of src/Assemble.jl, written with numpy
+so that models of millions of elements are numbered in seconds.
+
+Numbering rules kept from the reference (these fix the asm maps and the CSC
+pattern bit for bit):
+  * element types are numbered in order of first appearance
+    (getieletyp / addelement!, ModelDescription.jl:203-208);
+  * within a class (:X, :U, :A) a dof is identified by (node, class, field)
+    and receives the next number the first time an element refers to it,
+    elements taken in the order they are added and each element's dofs in
+    doflist order (ModelDescription.jl:226-249);
+  * dof types (class, field) are registered in first-appearance order
+    (ModelDescription.jl:228-233) and carry the scale of setscale!.
+"""
+import math
+
+import numpy as np
+
+CLASSES = ("X", "U", "A")
+_SHIFT = 20   # key = node << _SHIFT | idoftyp
+
+
+class MuscadeError(Exception):
+    """Mirror of MuscadeException (src/Exceptions.jl:1-16): message + location info."""
+
+    def __init__(self, msg, dbg=None):
+        super().__init__(msg)
+        self.msg = msg
+        self.dbg = dict(dbg or {})
+
+    def __str__(self):
+        return self.msg + (f" {self.dbg}" if self.dbg else "")
+
+
+class EleType:
+    """One Muscade element type (an entry of model.eleobj): all elements of one concrete type."""
+
+    def __init__(self, cls, key, doflist):
+        self.cls = cls
+        self.key = key
+        self.inod, self.dofclass, self.field = doflist
+        self.nodes = []     # chunks of (nele, nnod) 0-based node numbers
+        self.dofid = []     # chunks of (nele, ndof) class-local 1-based dof numbers
+        self.params = []    # chunks of per-element parameters (element class defines them)
+
+    @property
+    def nele(self):
+        return sum(n.shape[0] for n in self.nodes)
+
+    def cat_nodes(self):
+        return np.concatenate(self.nodes) if self.nodes else np.zeros((0, max(self.inod, default=0)), np.int64)
+
+    def cat_dofid(self):
+        return np.concatenate(self.dofid) if self.dofid else np.zeros((0, len(self.inod)), np.int64)
+
+    def class_columns(self, c):
+        return [k for k, cc in enumerate(self.dofclass) if cc == c]
+
+
+class Model:
+    """Model(ID=:muscade_model) — ModelDescription.jl:150."""
+
+    def __init__(self, ID="muscade_model"):
+        self.ID = ID
+        self._coords = []            # list of (n, dim) chunks
+        self.nnod = 0
+        self.eletypes = []           # first-appearance order
+        self._typeindex = {}
+        self.doftyp = []             # [class, field, scale]
+        self._doftypindex = {}
+        self.scaleΛ = 1.0
+        self.locked = False
+        self._regkeys = {c: np.zeros(0, np.int64) for c in CLASSES}
+        self._regids = {c: np.zeros(0, np.int64) for c in CLASSES}
+        self._dofnode = {c: [] for c in CLASSES}
+        self._doftypof = {c: [] for c in CLASSES}
+        self.ndof = {c: 0 for c in CLASSES}
+
+    # ------------------------------------------------------------------ nodes
+    def coords(self):
+        if not self._coords:
+            return np.zeros((0, 3))
+        dim = max(c.shape[1] for c in self._coords)
+        out = np.zeros((self.nnod, max(dim, 3)))
+        i = 0
+        for c in self._coords:
+            out[i:i + c.shape[0], :c.shape[1]] = c
+            i += c.shape[0]
+        return out
+
+    def _assert_unlocked(self):
+        if self.locked:
+            raise MuscadeError(f"model {self.ID} is initialized and can no longer be edited")
+
+    # ---------------------------------------------------------------- dofs
+    def _idoftyp(self, cls, field):
+        k = (cls, field)
+        if k not in self._doftypindex:
+            self._doftypindex[k] = len(self.doftyp)
+            self.doftyp.append([cls, field, 1.0])
+        return self._doftypindex[k]
+
+    def _number(self, E, nodes):
+        """First-encounter numbering of the dofs of a batch of elements (ModelDescription.jl:226-249)."""
+        nele = nodes.shape[0]
+        ndof = len(E.inod)
+        idt = np.array([self._idoftyp(c, f) for c, f in zip(E.dofclass, E.field)], np.int64)
+        dofid = np.zeros((nele, ndof), np.int64)
+        for c in CLASSES:
+            ks = E.class_columns(c)
+            if not ks or nele == 0:
+                continue
+            cols = np.array([E.inod[k] - 1 for k in ks])
+            keys = (nodes[:, cols].astype(np.int64) << _SHIFT) | idt[ks][None, :]
+            flat = keys.ravel()   # element-major, doflist order within: traversal order
+            reg, rid = self._regkeys[c], self._regids[c]
+            pos = np.searchsorted(reg, flat)
+            found = np.zeros(flat.size, bool)
+            if reg.size:
+                posc = np.minimum(pos, reg.size - 1)
+                found = reg[posc] == flat
+            ids = np.empty(flat.size, np.int64)
+            ids[found] = rid[pos[found]]
+            new = flat[~found]
+            if new.size:
+                uk, first = np.unique(new, return_index=True)
+                order = np.argsort(first, kind="stable")
+                base = self.ndof[c]
+                newid = np.empty(uk.size, np.int64)
+                newid[order] = base + 1 + np.arange(uk.size)
+                ids[~found] = newid[np.searchsorted(uk, new)]
+                self.ndof[c] += uk.size
+                self._dofnode[c].append(uk[order] >> _SHIFT)
+                self._doftypof[c].append(uk[order] & ((1 << _SHIFT) - 1))
+                allk = np.concatenate([reg, uk])
+                alli = np.concatenate([rid, newid])
+                s = np.argsort(allk, kind="stable")
+                self._regkeys[c], self._regids[c] = allk[s], alli[s]
+            dofid[:, ks] = ids.reshape(nele, len(ks))
+        return dofid
+
+    def dofnode(self, c):
+        return np.concatenate(self._dofnode[c]) if self._dofnode[c] else np.zeros(0, np.int64)
+
+    def doftypof(self, c):
+        return np.concatenate(self._doftypof[c]) if self._doftypof[c] else np.zeros(0, np.int64)
+
+    def dofscale(self, c):
+        sc = np.array([d[2] for d in self.doftyp]) if self.doftyp else np.zeros(0)
+        return sc[self.doftypof(c)] if self.ndof[c] else np.zeros(0)
+
+    def fieldof(self, c):
+        names = [d[1] for d in self.doftyp]
+        return [names[i] for i in self.doftypof(c)]
+
+    def getndof(self, c=None):
+        if c is None:
+            return sum(self.ndof.values())
+        return self.ndof[c]
+
+    def getnele(self):
+        return sum(E.nele for E in self.eletypes)
+
+    def getdofID(self, cls, field, nodID=None):
+        """Model dof numbers (1-based) of class/field, optionally restricted to nodes (ModelDescription.jl:133-138)."""
+        if (cls, field) not in self._doftypindex:
+            raise MuscadeError(f"The model has no dof of class {cls} and field {field}.")
+        it = self._doftypindex[(cls, field)]
+        ids = np.nonzero(self.doftypof(cls) == it)[0] + 1
+        if nodID is not None:
+            nodes = np.atleast_1d(np.asarray(nodID, np.int64)) - 1
+            ids = ids[np.isin(self.dofnode(cls)[ids - 1], nodes)]
+        return ids
+
+
+def addnode(model, coord):
+    """nodID = addnode!(model, coord) — ModelDescription.jl:166-173. Rows of a matrix are nodes.
+    Returns 1-based node numbers (an int for a single vector)."""
+    model._assert_unlocked()
+    c = np.asarray(coord, dtype=np.float64)
+    single = c.ndim == 1
+    c = c.reshape(1, -1) if single else c
+    ids = model.nnod + 1 + np.arange(c.shape[0])
+    model._coords.append(c)
+    model.nnod += c.shape[0]
+    return int(ids[0]) if single else ids
+
+
+def addelement(model, ElType, nodID, **kwargs):
+    """eleID = addelement!(model, ElType, nodID; kwargs...) — ModelDescription.jl:195-263.
+
+    nodID: one element's node numbers (vector) or a matrix with one element per row.
+    kwargs are passed to the element constructor; beam materials and orient2 may
+    also be given per element (arrays with one row per element)."""
+    model._assert_unlocked()
+    nod = np.asarray(nodID, dtype=np.int64)
+    single = nod.ndim == 1
+    nod = nod.reshape(1, -1) if single else nod
+    nodes0 = nod - 1
+    if nodes0.size and (nodes0.min() < 0 or nodes0.max() >= model.nnod):
+        raise MuscadeError("addelement: node number out of range")
+    key, doflist, params = ElType.construct(model, nodes0, **kwargs)
+    nnod = max(doflist[0])
+    if nod.shape[1] != nnod:
+        raise MuscadeError(f"Connecting element of type {ElType.__name__}: Second dimension of inod ({nod.shape[1]}) "
+                           f"must be equal to element's nnod ({nnod})")
+    if key not in model._typeindex:
+        model._typeindex[key] = len(model.eletypes)
+        model.eletypes.append(EleType(ElType, key, doflist))
+    ieletyp = model._typeindex[key]
+    E = model.eletypes[ieletyp]
+    iele0 = E.nele
+    dofid = model._number(E, nodes0)
+    E.nodes.append(nodes0)
+    E.dofid.append(dofid)
+    E.params.append(params)
+    ids = [(ieletyp + 1, iele0 + 1 + i) for i in range(nod.shape[0])]
+    return ids[0] if single else ids
+
+
+def setscale(model, scale=None, Λscale=None):
+    """setscale!(model; scale, Λscale) — ModelDescription.jl:287-299."""
+    model._assert_unlocked()
+    if scale is not None:
+        for d in model.doftyp:
+            if d[0] in scale and d[1] in scale[d[0]]:
+                d[2] = float(scale[d[0]][d[1]])
+    if Λscale is not None:
+        model.scaleΛ = float(Λscale)
+
+
+class EletypDisassembler:
+    def __init__(self, X, U, A, sX, sU, sA, sΛ):
+        self.X, self.U, self.A = X, U, A          # (nele, n) 1-based model dofs
+        self.scaleX, self.scaleU, self.scaleA, self.scaleΛ = sX, sU, sA, sΛ
+
+
+class Disassembler:
+    """Disassembler(model) — src/Assemble.jl:32-99."""
+
+    def __init__(self, model):
+        self.dis = []
+        for E in model.eletypes:
+            d = E.cat_dofid()
+            cols = {c: E.class_columns(c) for c in CLASSES}
+            # scales come from the element type's first element (Assemble.jl:49-63)
+            sc = {c: np.array([model.doftyp[model._doftypindex[(c, E.field[k])]][2] for k in cols[c]]) for c in CLASSES}
+            self.dis.append(EletypDisassembler(d[:, cols["X"]], d[:, cols["U"]], d[:, cols["A"]],
+                                               sc["X"], sc["U"], sc["A"], sc["X"] * model.scaleΛ))
+        self.scaleX = model.dofscale("X")
+        self.scaleΛ = self.scaleX * model.scaleΛ
+        self.scaleU = model.dofscale("U")
+        self.scaleA = model.dofscale("A")
+        self.fieldX = model.fieldof("X")
+        self.fieldU = model.fieldof("U")
+        self.fieldA = model.fieldof("A")
+
+
+class State:
+    """State{nΛder,nXder,nUder} — src/Assemble.jl:106-159. Vectors are numpy arrays."""
+
+    def __init__(self, time, Λ, X, U, A, SP, model, dis):
+        self.time = time
+        self.Λ = list(Λ)
+        self.X = list(X)
+        self.U = list(U)
+        self.A = A
+        self.SP = SP
+        self.model = model
+        self.dis = dis
+
+    def copy(self, time=None):
+        return State(self.time if time is None else time, [a.copy() for a in self.Λ], [a.copy() for a in self.X],
+                     [a.copy() for a in self.U], self.A.copy(), self.SP, self.model, self.dis)
+
+    def with_ders(self, nΛder, nXder, nUder):
+        """State{nΛder,nXder,nUder}(s): shave off or zero-pad derivatives (Assemble.jl:131-144)."""
+        def pad(v, n, size):
+            return [v[i] if i < len(v) else np.zeros(size) for i in range(n)]
+        return State(self.time, pad(self.Λ, nΛder, self.model.ndof["X"]), pad(self.X, nXder, self.model.ndof["X"]),
+                     pad(self.U, nUder, self.model.ndof["U"]), self.A, self.SP, self.model, self.dis)
+
+
+def initialize(model, nΛder=1, nXder=1, nUder=1, time=-math.inf):
+    """initialstate = initialize!(model) — ModelDescription.jl:319-324."""
+    model._assert_unlocked()
+    model.locked = True
+    dis = Disassembler(model)
+    nX, nU, nA = model.ndof["X"], model.ndof["U"], model.ndof["A"]
+    return State(time, [np.zeros(nX) for _ in range(nΛder)], [np.zeros(nX) for _ in range(nXder)],
+                 [np.zeros(nU) for _ in range(nUder)], np.zeros(nA), None, model, dis)
+
+
+def getdof(state, cls="X", field=None, nodID=None, order=0):
+    """getdof(state; class, field, nodID, order) — src/Output.jl:21-54 (state or list of states)."""
+    states = state if isinstance(state, (list, tuple)) else [state]
+    c = "X" if cls == "Λ" else cls
+    ids = states[0].model.getdofID(c, field, nodID)
+    out = np.zeros((ids.size, len(states)))
+    for i, s in enumerate(states):
+        if cls == "A":
+            out[:, i] = s.A[ids - 1]
+            continue
+        sc = {"Λ": s.Λ, "X": s.X, "U": s.U}[cls]
+        if order < len(sc):
+            out[:, i] = sc[order][ids - 1]
+    return out[:, 0] if not isinstance(state, (list, tuple)) else out
+
+
+def setdof(state, value, cls="X", field=None, nodID=None, order=0):
+    """state = setdof!(state, value; class, field, nodID, order) — src/Output.jl:73-128."""
+    c = "X" if cls == "Λ" else cls
+    ids = state.model.getdofID(c, field, nodID)
+    if cls == "A":
+        state.A[ids - 1] = value
+        return state
+    sc = {"Λ": state.Λ, "X": state.X, "U": state.U}[cls]
+    n = state.model.ndof[c]
+    while len(sc) < order + 1:
+        sc.append(np.zeros(n))
+    sc[order][ids - 1] = value
+    return state

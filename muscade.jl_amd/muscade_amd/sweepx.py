@@ -1,0 +1,126 @@
+"""solve(SweepX{OX}; initialstate, time, β, γ, maxiter, maxΔx, maxLλ, saveiter) — src/SweepX.jl:134-226,
+and the solve wrapper of src/Solve.jl:16-35.
+
+The Newton/Newmark loop stays on the host like in the reference; assembly of
+Lλ and Lλx runs on the GPU; the sparse factorization (UMFPACK `lu` in the
+reference, SuperLU here) stays on the host.
+"""
+import math
+import sys
+import time as _time
+
+import numpy as np
+import scipy.sparse.linalg as spla
+
+from .assemble import AssemblySweepX, Newmarkβcoefficients
+from .model import MuscadeError
+
+
+class _SweepXMeta(type):
+    def __getitem__(cls, OX):
+        if OX not in (0, 1, 2):
+            raise MuscadeError("SweepX order must be 0, 1 or 2")
+        return type(f"SweepX{{{OX}}}", (SweepX,), {"OX": OX})
+
+
+class SweepX(metaclass=_SweepXMeta):
+    """SweepX{OX}: SweepX[0] static Newton-Raphson, SweepX[1] first-order, SweepX[2] Newmark-β."""
+    OX = 0
+
+    @classmethod
+    def run(cls, pstate, verbose, dbg, *, time, initialstate, β=0.25, γ=0.5, maxiter=50, maxΔx=1e-5,
+            maxLλ=math.inf, saveiter=False, device=0):
+        OX = cls.OX
+        model, dis = initialstate.model, initialstate.dis
+        out = AssemblySweepX(OX, model, dis, device=device)
+        nX = out.nX
+        scaleX = dis.scaleX
+        cΔx2, cLλ2 = maxΔx ** 2, maxLλ ** 2
+        state = initialstate.copy().with_ders(1, OX + 1, 1)
+        states = [None] * (maxiter if saveiter else len(time))
+        pstate[0] = states
+        citer = 0
+        a = b = None
+        for step, t in enumerate(time, start=1):
+            oldt = state.time
+            state.time = float(t)
+            Δt = state.time - oldt
+            if Δt <= 0 and OX > 0:
+                raise MuscadeError("Time step length not strictly positive at step=%3d" % step)
+            out.c = Newmarkβcoefficients(OX, Δt, β, γ)
+            c = out.c
+            for iiter in range(1, maxiter + 1):
+                citer += 1
+                first = iiter == 1
+                d = dict(dbg, solver="SweepX", step=step, iiter=iiter)
+                Lλ, _ = out.assemble(state, "step" if first else "iter", dbg=d)
+                try:
+                    lu = spla.splu(out.Lλx())
+                    Δx = lu.solve(Lλ)
+                except Exception as e:   # SweepX.jl:207-209
+                    raise MuscadeError("matrix factorization failed at step=%i, iiter=%i" % (step, iiter)) from e
+                if not np.all(np.isfinite(Δx)):
+                    raise MuscadeError("matrix factorization failed at step=%i, iiter=%i" % (step, iiter))
+                Δx2, Lλ2 = float(np.sum(Δx ** 2)), float(np.sum(Lλ ** 2))
+                # Newmarkβdecrement!{OX} (SweepX.jl:98-132) with the allXdofs group
+                if OX == 2:
+                    if first:
+                        x1 = state.X[1] / scaleX
+                        x2 = state.X[2] / scaleX
+                        a = c.a2 * x1 + c.a3 * x2
+                        b = c.b2 * x1 + c.b3 * x2
+                        Δx1 = c.a1 * Δx + a
+                        Δx2_ = c.b1 * Δx + b
+                    else:
+                        Δx1, Δx2_ = c.a1 * Δx, c.b1 * Δx
+                    state.X[0] -= Δx * scaleX
+                    state.X[1] -= Δx1 * scaleX
+                    state.X[2] -= Δx2_ * scaleX
+                elif OX == 1:
+                    if first:
+                        a = c.a2 * (state.X[1] / scaleX)
+                        Δx1 = c.a1 * Δx + a
+                    else:
+                        Δx1 = c.a1 * Δx
+                    state.X[0] -= Δx * scaleX
+                    state.X[1] -= Δx1 * scaleX
+                else:
+                    state.X[0] -= Δx * scaleX
+                if saveiter:
+                    states[iiter - 1] = state.copy()
+                if Δx2 <= cΔx2 and Lλ2 <= cLλ2:
+                    if verbose:
+                        print("    step %3d converged in %3d iterations. |Δx|=%7.1e |Lλ|=%7.1e"
+                              % (step, iiter, math.sqrt(Δx2), math.sqrt(Lλ2)))
+                    if not saveiter:
+                        states[step - 1] = state.copy()
+                    break
+                if iiter == maxiter:
+                    raise MuscadeError("no convergence of step %3d after %3d iterations |Δx|=%g / %g, |Lλ|=%g / %g"
+                                       % (step, iiter, math.sqrt(Δx2), maxΔx, math.sqrt(Lλ2), maxLλ))
+        if verbose:
+            print("\n    nel=%d, ndof=%d, nstep=%d, niter=%d, niter/nstep=%5.2f"
+                  % (model.getnele(), nX, len(time), citer, citer / max(len(time), 1)))
+
+
+def solve(Solver, dbg=None, verbose=True, silenterror=False, catcherror=True, **kwargs):
+    """solve(Solver; dbg, verbose, silenterror, catcherror, kwargs...) — src/Solve.jl:16-35.
+
+    Returns the states computed so far, also when the solver raised."""
+    pstate = [None]
+    dbg = dict(dbg or {})
+    if verbose:
+        print(f"\n\n\nMuscade: {Solver.__name__} solver\n")
+    t0 = _time.perf_counter()
+    if catcherror:
+        try:
+            Solver.run(pstate, verbose, dict(dbg, solver=Solver.__name__), **kwargs)
+        except MuscadeError as e:
+            if not silenterror:
+                print(f"\n\nMuscadeException: {e}\n\nAborting the analysis. Function 'solve' still returns any "
+                      "results obtained before the exception.", file=sys.stderr)
+    else:
+        Solver.run(pstate, verbose, dict(dbg, solver=Solver.__name__), **kwargs)
+    if verbose:
+        print("    %s time: %.3f [s]\nMuscade done.\n\n" % (Solver.__name__, _time.perf_counter() - t0))
+    return pstate[0]

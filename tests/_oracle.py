@@ -1,0 +1,176 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) — test infrastructure only.
+
+The oracle restates Muscade's own algorithms (nested forward-mode duals,
+sequential assembly) in C++; tests use it as the checker for the HIP path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIBPATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+_IP = ctypes.POINTER(ctypes.c_int)
+_V = ctypes.c_void_p
+
+K_BEAM, K_HOLD, K_DOFLOAD, K_ZERO, K_TURBINE, K_ANCHOR = 0, 1, 2, 3, 4, 5
+
+_SIG = {
+    "orc_sinc1_k": (ctypes.c_double, [ctypes.c_int, ctypes.c_double]),
+    "orc_scac_d": (ctypes.c_double, [ctypes.c_int, ctypes.c_double]),
+    "orc_rodrigues": (None, [_D, _D]),
+    "orc_rodrigues_inv": (None, [_D, _D]),
+    "orc_rodrigues_roundtrip": (None, [_D, _D, _D]),
+    "orc_adjust": (None, [_D, _D, _D]),
+    "orc_beam_construct": (ctypes.c_int, [_D, _D, _D, _D]),
+    "orc_beam_diffed_residual": (ctypes.c_int, [_D, _D, _D, _D, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, _D]),
+    "orc_beam_residual_plain": (ctypes.c_int, [_D, _D, _D, _D, ctypes.c_int, _D, _D, _D, _D]),
+    "orc_beam_gauge": (ctypes.c_int, [_D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
+    "orc_number_dofs": (ctypes.c_int, [ctypes.c_int, _IP, _IP, _IP, ctypes.c_int, _IP, _IP, _IP, _IP, _IP, _IP, _IP,
+                                       _I64, ctypes.c_int, _IP, _IP, _IP, _IP, _IP, _IP]),
+    "orc_model_new": (_V, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
+    "orc_model_free": (None, [_V]),
+    "orc_model_add_type": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          _I64, _I64, _I64, _D, _D, ctypes.c_int]),
+    "orc_model_set_dofload": (ctypes.c_int, [_V, ctypes.c_int, _D]),
+    "orc_model_prepare_sweepx": (ctypes.c_int64, [_V]),
+    "orc_model_get_pattern": (None, [_V, _I64, _I64]),
+    "orc_model_get_asm": (None, [_V, ctypes.c_int, _I64, _I64]),
+    "orc_model_assemble_sweepx": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, _D, _D, _D, _D, _IP,
+                                                 _IP]),
+}
+
+lib = ctypes.CDLL(LIBPATH)
+for _n, (_r, _a) in _SIG.items():
+    getattr(lib, _n).restype = _r
+    getattr(lib, _n).argtypes = _a
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def dp(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def ip64(a):
+    return None if a is None else a.ctypes.data_as(_I64)
+
+
+def ip(a):
+    return None if a is None else a.ctypes.data_as(_IP)
+
+
+def beam_construct(c1, c2, orient2):
+    out = np.zeros(53)
+    r = lib.orc_beam_construct(dp(f64(c1)), dp(f64(c2)), dp(f64(orient2)), dp(out))
+    if r:
+        raise ValueError("orient2 nearly parallel")
+    o = {}
+    i = 0
+    for name, n in [("cm", 3), ("rm", 9), ("zgp", 4), ("znod", 2), ("tgm", 3), ("tge", 3), ("ya", 4), ("yu", 4),
+                    ("yv", 4), ("ka", 4), ("ku", 4), ("kv", 4), ("L", 1), ("dL", 4)]:
+        o[name] = out[i:i + n].copy()
+        i += n
+    o["rm"] = o["rm"].reshape(3, 3, order="F")
+    o["L"] = o["L"][0]
+    return o
+
+
+def beam_diffed_residual(c1, c2, orient2, mat16, X, U=None):
+    """diffed_residual of one EulerBeam3D: X is a list of ND 12-vectors. Returns R, [∂R/∂X_der], ∂R/∂U."""
+    ND = len(X)
+    Xa = f64(np.concatenate([np.asarray(x, float) for x in X]))
+    R = np.zeros(12)
+    dX = np.zeros(ND * 144)
+    dU = np.zeros(36)
+    udof = 0 if U is None else 1
+    r = lib.orc_beam_diffed_residual(dp(f64(c1)), dp(f64(c2)), dp(f64(orient2)), dp(f64(mat16)), udof, ND, dp(Xa),
+                                     dp(f64(U) if U is not None else None), dp(R), dp(dX), dp(dU))
+    if r:
+        raise ValueError("orient2 nearly parallel")
+    Ks = [dX[d * 144:(d + 1) * 144].reshape(12, 12, order="F") for d in range(ND)]
+    return R, Ks, dU.reshape(12, 3, order="F")
+
+
+def beam_residual_plain(c1, c2, orient2, mat16, X):
+    ND = len(X)
+    Xa = f64(np.concatenate([np.asarray(x, float) for x in X]))
+    R = np.zeros(12)
+    eps = np.zeros(1)
+    kap = np.zeros(3)
+    lib.orc_beam_residual_plain(dp(f64(c1)), dp(f64(c2)), dp(f64(orient2)), dp(f64(mat16)), ND, dp(Xa), dp(R), dp(eps),
+                                dp(kap))
+    return R, eps[0], kap
+
+
+def beam_gauge(c1, c2, orient2, mat16, X):
+    R = np.zeros(12)
+    K = np.zeros(144)
+    eps = np.zeros(1)
+    deps = np.zeros(12)
+    kap = np.zeros(3)
+    dkap = np.zeros(36)
+    lib.orc_beam_gauge(dp(f64(c1)), dp(f64(c2)), dp(f64(orient2)), dp(f64(mat16)), dp(f64(X)), dp(R), dp(K), dp(eps),
+                       dp(deps), dp(kap), dp(dkap))
+    return R, K.reshape(12, 12, order="F"), eps[0], deps, kap, dkap.reshape(3, 12)
+
+
+class OracleModel:
+    """A model handed to the oracle's sequential SweepX assembler (Assemble.jl:470-505 restated)."""
+
+    def __init__(self, nX, nU=0, nA=0):
+        self.h = lib.orc_model_new(nX, nU, nA)
+        self.nX = nX
+        self.ntypes = 0
+
+    def add_type(self, kind, iX, iU=None, iA=None, scaleX=None, params=None, udof=0):
+        iX = np.ascontiguousarray(iX, np.int64)
+        nele, nX = iX.shape
+        nU = 0 if iU is None else iU.shape[1]
+        nA = 0 if iA is None else iA.shape[1]
+        r = lib.orc_model_add_type(self.h, kind, nele, nX, nU, nA, ip64(iX),
+                                   ip64(np.ascontiguousarray(iU, np.int64) if nU else None),
+                                   ip64(np.ascontiguousarray(iA, np.int64) if nA else None),
+                                   dp(f64(scaleX) if scaleX is not None else None),
+                                   dp(f64(params) if params is not None else None), udof)
+        if r:
+            raise ValueError("orc_model_add_type failed")
+        self.ntypes += 1
+        return self.ntypes - 1
+
+    def set_dofload(self, itype, F):
+        lib.orc_model_set_dofload(self.h, itype, dp(f64(F)))
+
+    def prepare(self):
+        self.nnz = lib.orc_model_prepare_sweepx(self.h)
+        self.colptr = np.zeros(self.nX + 1, np.int64)
+        self.rowval = np.zeros(self.nnz, np.int64)
+        lib.orc_model_get_pattern(self.h, ip64(self.colptr), ip64(self.rowval))
+        return self.nnz
+
+    def asm(self, itype, nele, nX):
+        a1 = np.zeros(nele * nX, np.int64)
+        a2 = np.zeros(nele * nX * nX, np.int64)
+        lib.orc_model_get_asm(self.h, itype, ip64(a1), ip64(a2))
+        return a1.reshape(nele, nX), a2.reshape(nele, nX * nX)
+
+    def assemble(self, OX, mission, c6, X, U=None, A=None):
+        rhs = np.zeros(self.nX)
+        nz = np.zeros(self.nnz)
+        et = np.zeros(1, np.intc)
+        ee = np.zeros(1, np.intc)
+        Xs = [f64(x) for x in X] + [None] * (3 - len(X))
+        r = lib.orc_model_assemble_sweepx(self.h, OX, mission, dp(f64(c6)), dp(Xs[0]), dp(Xs[1]), dp(Xs[2]),
+                                          dp(f64(U) if U is not None else None), dp(f64(A) if A is not None else None),
+                                          dp(rhs), dp(nz), ip(et), ip(ee))
+        return r, rhs, nz, (int(et[0]), int(ee[0]))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_model_free(self.h)
+            self.h = None

@@ -1,0 +1,64 @@
+"""Shared test helpers: build models with the product's host API and hand the
+same model to the CPU oracle (test infrastructure)."""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "muscade.jl_amd")
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
+
+from tests import _oracle as orc  # noqa: E402
+
+
+def model_api():
+    """The host package without loading the HIP library (model description only)."""
+    from muscade_amd import model, elements  # noqa: F401
+    return model, elements
+
+
+def build_lattice(nx, ny, nz, seed=7, disp=1e-3, rot=1e-2, sigma=0.2):
+    from muscade_amd.workloads import space_frame
+    return space_frame(nx, ny, nz, seed=seed, disp=disp, rot=rot, sigma=sigma)
+
+
+KIND_TO_ORACLE = {1: orc.K_BEAM, 3: orc.K_HOLD, 4: orc.K_DOFLOAD, 5: orc.K_ZERO}
+
+
+def oracle_model(model, dis, t=0.0):
+    """The same model for the oracle's sequential assembler."""
+    om = orc.OracleModel(model.ndof["X"], model.ndof["U"], model.ndof["A"])
+    xyz = model.coords()
+    for it, E in enumerate(model.eletypes):
+        d = dis.dis[it]
+        kind = KIND_TO_ORACLE[E.cls.kind]
+        params = None
+        if kind == orc.K_BEAM:
+            nodes = E.cat_nodes()
+            o2 = np.concatenate([p["orient2"] for p in E.params])
+            mat = np.concatenate([p["mat"] for p in E.params])
+            params = np.concatenate([xyz[nodes[:, 0], :3], xyz[nodes[:, 1], :3], o2, mat], axis=1)
+        elif kind == orc.K_DOFLOAD:
+            params = np.concatenate([np.full(n.shape[0], float(p["value"](t, **p["args"])))
+                                     for n, p in zip(E.nodes, E.params)])
+        om.add_type(kind, d.X, d.U if d.U.shape[1] else None, d.A if d.A.shape[1] else None, d.scaleX, params,
+                    udof=int(getattr(E.cls, "Udof", False)))
+    om.prepare()
+    return om
+
+
+def oracle_assemble_sweepx0(model, state, mission=1):
+    om = oracle_model(model, state.dis, t=state.time)
+    r, rhs, nz, err = om.assemble(0, mission, np.zeros(6), [state.X[0]],
+                                  state.U[0] if model.ndof["U"] else None, state.A if model.ndof["A"] else None)
+    assert r == 0, (r, err)
+    return {"rhs": rhs, "nzval": nz, "colptr": om.colptr, "rowval": om.rowval, "om": om}
+
+
+def relerr(a, b):
+    """Normwise relative error max|a-b| / max|b| (the parity metric of the tests)."""
+    a, b = np.asarray(a), np.asarray(b)
+    d = np.abs(b).max()
+    return np.abs(a - b).max() / (d if d > 0 else 1.0)
