@@ -1,0 +1,197 @@
+"""Benchmark: EulerBeam3D elements assembled per second (FP64 residual + tangent).
+
+Workload: static assembly at scale — a 100×100×33-node space frame
+(973,400 EulerBeam3D + 60,000 Hold elements, 2,040,000 X-dofs, seed 1001),
+see muscade_amd/workloads.py. One step = one assemble!{:iter} of
+AssemblySweepX{0}: element evaluation of every element plus the scatter into
+Lλ and Lλx.nzval, from a device-resident state to device-resident outputs.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line. Kernel times come from HIP events recorded by the
+library on the stream it launches on.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "muscade.jl_amd"))
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
+FP64_PEAK = 78.6e12        # MI355X FP64 vector, FLOP/s
+# FP64 operations per beam of the static kernel (add/sub/mul/div/sqrt = 1, fma = 2),
+# from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters of one assembly of this
+# workload divided by the number of beams (profiles/beam_static_pmc.md).
+F_ELEM = None
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(nbeam, nhold, nnz, nX, N_c, N_cR, nKbuf):
+    """Bytes each kernel must move at least (the roofline numerator)."""
+    # eval, per beam: in X 12×8, dof map 12×4, nodes 2×4, coordinates 2×24, orient2 24, section 5×8;
+    #                 out 45 tangent numbers ×8, R 12×8
+    eval_beam = 96 + 48 + 8 + 48 + 24 + 40 + 45 * 8 + 96
+    # Hold: in 2×8 + 2×4, out 4×8 + 2×8
+    eval_b = nbeam * eval_beam + nhold * (16 + 8 + 32 + 16)
+    # nz scatter: every element output read once (nKbuf doubles), one coded source
+    # index per contribution, one segment offset and one write per nz
+    scat_K = 8 * nKbuf + 4 * N_c + 4 * (nnz + 1) + 8 * nnz
+    scat_R = 12 * N_cR + 8 * nX + 4 * (nX + 1)
+    return eval_b, scat_K, scat_R
+
+
+def calib_copy(n=1 << 27):
+    """Time one streaming copy of n doubles (the counter-calibration kernel)."""
+    import muscade_amd as mc
+    from muscade_amd import _lib
+    ctx = mc.DeviceContext(0)
+    ms = np.zeros(1)
+    ctx.check(_lib.lib.mh_calib_copy(ctx.h, n, _lib.dptr(ms)), "calib copy")
+    print(json.dumps({"calib_copy_bytes_read": 8 * n, "calib_copy_bytes_written": 8 * n, "ms": ms[0],
+                      "GB_s": 16 * n / (ms[0] * 1e-3) / 1e9}))
+
+
+def cpu_baseline(model, state, nsample):
+    """The oracle (Muscade's nested-dual algorithm restated, sequential add_value!/add_∂!),
+    timed on the first `nsample` beams of the same model, 1 thread."""
+    sys.path.insert(0, ROOT)
+    from tests import _oracle as orc
+    E = model.eletypes[0]
+    d = state.dis.dis[0]
+    n = min(nsample, d.X.shape[0])
+    xyz = model.coords()
+    nodes = E.cat_nodes()[:n]
+    o2 = np.concatenate([p["orient2"] for p in E.params])[:n]
+    mat = np.concatenate([p["mat"] for p in E.params])[:n]
+    params = np.concatenate([xyz[nodes[:, 0], :3], xyz[nodes[:, 1], :3], o2, mat], axis=1)
+    om = orc.OracleModel(model.ndof["X"])
+    om.add_type(orc.K_BEAM, d.X[:n], None, None, d.scaleX, params)
+    om.prepare()
+    t0 = time.perf_counter()
+    r, rhs, nz, _ = om.assemble(0, 1, np.zeros(6), [state.X[0]])
+    dt = time.perf_counter() - t0
+    assert r == 0
+    return n / dt, n, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=100)
+    ap.add_argument("--ny", type=int, default=100)
+    ap.add_argument("--nz", type=int, default=33)
+    ap.add_argument("--cpu-sample", type=int, default=16000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true", help="run only the counter-calibration copy kernel")
+    args = ap.parse_args()
+    if args.calib_copy:
+        calib_copy()
+        return
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import muscade_amd as mc
+    from muscade_amd import _lib
+    from muscade_amd.workloads import space_frame
+
+    t0 = time.perf_counter()
+    model, state = space_frame(args.nx, args.ny, args.nz, seed=1001)
+    nbeam = model.eletypes[0].nele
+    nhold = sum(E.nele for E in model.eletypes[1:])
+    log(f"model: {nbeam} beams, {nhold} holds, nX={model.ndof['X']} ({time.perf_counter() - t0:.1f}s)")
+    t0 = time.perf_counter()
+    out = mc.AssemblySweepX(0, model, state.dis)
+    t_prep = time.perf_counter() - t0
+    log(f"prepare (device pattern, nnz={out.nnz}): {t_prep:.2f}s")
+    out.set_state(state, 0.0)
+    h = out.ctx.h
+    c6 = out.c.array()
+    err = _lib.mh_err()
+
+    def step():
+        r = _lib.lib.mh_assemble_sweepx(h, 0, 1, _lib.dptr(c6), ctypes.byref(err))
+        if r != 0:
+            raise RuntimeError(_lib.lib.mh_last_error(h).decode())
+
+    for _ in range(args.warmup):
+        step()
+    _lib.lib.mh_sync(h)
+    ms = np.zeros(3)
+    acc = np.zeros(3)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        _lib.lib.mh_last_timings(h, _lib.dptr(ms))
+        acc += ms
+        if (i + 1) % max(1, args.steps // 4) == 0:
+            log(f"  step {i + 1}/{args.steps}")
+    _lib.lib.mh_sync(h)
+    elapsed = time.perf_counter() - t0
+    ms_step = elapsed / args.steps * 1e3
+    value = nbeam * args.steps / elapsed
+    t_eval, t_sk, t_sr = acc / args.steps * 1e-3   # s per launch group
+    N_c = sum(E.nele * len(E.class_columns("X")) ** 2 for E in model.eletypes)
+    N_cR = sum(E.nele * len(E.class_columns("X")) for E in model.eletypes)
+    nKbuf = nbeam * 45 + sum(E.nele * len(E.class_columns("X")) ** 2 for E in model.eletypes[1:])
+    b_eval, b_sk, b_sr = algorithmic_bytes(nbeam, nhold, out.nnz, out.nX, N_c, N_cR, nKbuf)
+    rl_eval_hbm = {"bound": "hbm", "achieved": b_eval / t_eval / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s"}
+    rl_scatter = {"bound": "hbm", "achieved": b_sk / t_sk / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s"}
+    if F_ELEM:
+        rl_eval = {"bound": "fp64", "achieved": F_ELEM * nbeam / t_eval / 1e12, "peak": FP64_PEAK / 1e12,
+                   "unit": "TFLOP/s"}
+    else:
+        rl_eval = rl_eval_hbm
+    for r_ in (rl_eval, rl_scatter, rl_eval_hbm):
+        r_["frac"] = r_["achieved"] / r_["peak"]
+        r_["traffic"] = None
+    dominant = ("eval", rl_eval) if t_eval >= t_sk else ("scatter", rl_scatter)
+    log(f"per step: eval {t_eval * 1e3:.3f} ms, nz scatter {t_sk * 1e3:.3f} ms, rhs scatter {t_sr * 1e3:.3f} ms, "
+        f"wall {ms_step:.3f} ms")
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle, 1 thread)...")
+        eps, n, dt = cpu_baseline(model, state, args.cpu_sample)
+        cpu = {"value": eps, "unit": "elements/s", "cores": 1, "kind": "port",
+               "sample": f"first {n} beams of the same 100x100x33 lattice, one sequential assemble!{{:iter}} "
+                         f"({dt:.1f} s): nested forward-mode duals + sequential add_value!/add_∂!"}
+    line = {
+        "metric": "EulerBeam3D elements assembled/s (FP64 R+K)",
+        "value": value,
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"static space frame {args.nx}x{args.ny}x{args.nz} nodes, {nbeam} EulerBeam3D + "
+                               f"{nhold} Hold, residual+tangent assembly (AssemblySweepX{{0}}, :iter)",
+                   "nbeam": nbeam, "nX": out.nX, "nnz": out.nnz},
+        "roofline": dict(dominant[1], kernel=dominant[0]),
+        "kernels_ms": {"eval": t_eval * 1e3, "scatter_nz": t_sk * 1e3, "scatter_rhs": t_sr * 1e3},
+        "roofline_eval_hbm": rl_eval_hbm,
+        "roofline_scatter": rl_scatter,
+        "prepare_s": t_prep,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

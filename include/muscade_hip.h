@@ -113,6 +113,10 @@ int mh_get_element_blocks(mh_ctx* ctx, int igroup, double* R, double* K);
  * HIP-event times (ms) of the last assembly, measured on the context's stream:
  * ms[0] element evaluation (all groups), ms[1] nzval scatter, ms[2] rhs scatter. */
 int mh_last_timings(mh_ctx* ctx, double* ms);
+/* Measurement utility (no counterpart in the reference): one streaming copy of
+ * n doubles on the context's stream, time in *ms. It is the known-size kernel
+ * that HBM byte counters (FETCH_SIZE, WRITE_SIZE) are calibrated against. */
+int mh_calib_copy(mh_ctx* ctx, int64_t n, double* ms);
 int mh_sync(mh_ctx* ctx);
 
 #ifdef __cplusplus

@@ -228,74 +228,91 @@ MH_HD BeamGeo beam_geometry(v3 c1, v3 c2, v3 o2) {   // BeamElement.jl:122-148
     return g;
 }
 
+
+
+// ------------------------------------------------------------ compact tangent
+// The static tangent of the 2-node beam has an exact block structure: the
+// translations enter Π only through d = ½(u₂-u₁) (internal terms) and
+// c = ½(u₁+u₂) (linearly, external load), so with A = ∂²Π/∂d² (global frame),
+// T = ∂²Π/∂d∂v (3×6 over the rotation dofs v = (v₁,v₂)) and S = ∂²Π/∂v² (6×6):
+//     K[u₁,u₁] = K[u₂,u₂] = ¼A,  K[u₁,u₂] = K[u₂,u₁] = -¼A,
+//     K[u₁,v]  = -½T,            K[u₂,v]  = +½T,          K[v,v] = S,
+// and K is symmetric. These 45 numbers are the element output; the scatter
+// multiplies by the exact powers of two ±1, ±½, ±¼.
+// Slots: A at 0..5 (xx,yy,zz,xy,xz,yz), T at 6..23 (T[a][r] = 6 + 6a + r),
+// S at 24..44 (upper triangle, row-major: 24 + sym6(r,s)).
+constexpr int kCompact = 45;
+MH_HD int sym3(int a, int b) {   // (xx,yy,zz,xy,xz,yz)
+    if (a > b) { int t = a; a = b; b = t; }
+    return a == b ? a : (a == 0 ? 2 + b : 5);
+}
+MH_HD int sym6(int r, int s) {   // upper triangle of 6×6, row-major
+    if (r > s) { int t = r; r = s; s = t; }
+    return r * 6 - (r * (r - 1)) / 2 + (s - r);
+}
+// Compact slot and exact multiplier of element entry K[i][j] (i,j = 0..11,
+// element dof order t1,t2,t3,r1,r2,r3 of node 1 then node 2).
+MH_HD int compact_slot(int i, int j, double* mult) {
+    int ni = i / 6, a = i % 6, nj = j / 6, b = j % 6;
+    bool ri = a >= 3, rj = b >= 3;
+    if (!ri && !rj) { *mult = (ni == nj) ? 0.25 : -0.25; return sym3(a, b); }
+    if (!ri && rj) { *mult = ni == 0 ? -0.5 : 0.5; return 6 + 6 * a + (3 * nj + b - 3); }
+    if (ri && !rj) { *mult = nj == 0 ? -0.5 : 0.5; return 6 + 6 * b + (3 * ni + a - 3); }
+    *mult = 1.;
+    return 24 + sym6(3 * ni + a - 3, 3 * nj + b - 3);
+}
+
 struct BeamSection { double EA, EI2, EI3, GJ, w; };
+struct BeamOut { double R[12]; double Kc[kCompact]; int nan; };
 
-// Output: R[12], and K as the upper triangle (incl. diagonal) in the order
-// K[i][j], i<=j, row-major over the 78 entries; K is symmetric.
-struct BeamOut { double R[12]; double K[78]; int nan; };
-MH_HD int tri(int i, int j) { return i * 12 - (i * (i - 1)) / 2 + (j - i); }   // i <= j
-
-// Static residual and tangent; fe = (0,0,w) - U is the external load per length.
+// Static residual and compact tangent; fe = (0,0,w) - U is the external load per length.
 MH_HD void beam_static(const BeamGeo& G, const BeamSection& S, v3 fe, const double* X, BeamOut& out) {
     const double L = G.L;
     v3 u1 = mk(X[0], X[1], X[2]), v1 = mk(X[3], X[4], X[5]);
     v3 u2 = mk(X[6], X[7], X[8]), v2 = mk(X[9], X[10], X[11]);
-    // ---- rotation core ------------------------------------------------------
+    // ---- rotation core, values --------------------------------------------------
     ExpCoef k1 = exp_coef(norm3(v1)), k2 = exp_coef(norm3(v2));
-    m3 R1 = rodrigues(v1, k1), R2 = rodrigues(v2, k2);
-    m3 T1 = tangent_op(v1, k1), T2 = tangent_op(v2, k2);
-    m3 M = mulABt(R2, R1);
-    // φ = Rodrigues⁻¹(M) = spin⁻¹(M)/scac((tr M - 1)/2), Δv = φ/2
-    double sc = scac((at(M, 0, 0) + at(M, 1, 1) + at(M, 2, 2) - 1.) / 2.);
-    v3 phi = mk((at(M, 2, 1) - at(M, 1, 2)) / 2. / sc, (at(M, 0, 2) - at(M, 2, 0)) / 2. / sc, (at(M, 1, 0) - at(M, 0, 1)) / 2. / sc);
-    v3 dv = 0.5 * phi;
-    ExpCoef kp = exp_coef(norm3(phi)), kh = exp_coef(norm3(dv));
-    m3 Tpi = tangent_inv(phi, kp);
-    m3 H = rodrigues(dv, kh), TH = tangent_op(dv, kh);
-    m3 HR1 = mul(H, R1);
-    m3 rsm = mul(HR1, G.rm);
-    // first-order spin Jacobians, 3×6 stored as 6 columns: [0..2] w.r.t. δv₁, [3..5] w.r.t. δv₂
-    v3 wM[6], Jphi[6], JD[6], wH[6], wsm[6], HT1[3], MT1[3];
-    for (int j = 0; j < 3; ++j) {
-        v3 t1 = col(T1, j), t2 = col(T2, j);
-        MT1[j] = mul(M, t1);
-        HT1[j] = mul(H, t1);
-        wM[j] = -1. * MT1[j];
-        wM[3 + j] = t2;
+    m3 M, H, rsm;
+    v3 phi, dv;
+    ExpCoef kp, kh;
+    {
+        m3 R1 = rodrigues(v1, k1), R2 = rodrigues(v2, k2);
+        M = mulABt(R2, R1);
+        // φ = Rodrigues⁻¹(M) = spin⁻¹(M)/scac((tr M - 1)/2), Δv = φ/2
+        double sc = scac((at(M, 0, 0) + at(M, 1, 1) + at(M, 2, 2) - 1.) / 2.);
+        phi = mk((at(M, 2, 1) - at(M, 1, 2)) / 2. / sc, (at(M, 0, 2) - at(M, 2, 0)) / 2. / sc,
+                 (at(M, 1, 0) - at(M, 0, 1)) / 2. / sc);
+        dv = 0.5 * phi;
+        kp = exp_coef(norm3(phi));
+        kh = exp_coef(norm3(dv));
+        H = rodrigues(dv, kh);
+        rsm = mul(mul(H, R1), G.rm);
     }
-    for (int j = 0; j < 6; ++j) {
-        Jphi[j] = mul(Tpi, wM[j]);
-        JD[j] = 0.5 * Jphi[j];
-        wH[j] = mul(TH, JD[j]);
-        wsm[j] = wH[j];
-    }
-    for (int j = 0; j < 3; ++j) wsm[j] = wsm[j] + HT1[j];
-    // ---- local deformations ---------------------------------------------------
+    // ---- local deformations and the local gradient ---------------------------------
     v3 cs = mk(0.5 * (u1.x + u2.x), 0.5 * (u1.y + u2.y), 0.5 * (u1.z + u2.z));
     v3 d = mk(u2.x + G.tgm.x * 0.5 - cs.x, u2.y + G.tgm.y * 0.5 - cs.y, u2.z + G.tgm.z * 0.5 - cs.z);
     v3 ul = mulT(rsm, d);
     ul.x = ul.x - L * 0.5;
     v3 vl = mulT(rsm, dv);
-    // axial
     v3 w = mk(ul.x + L / 2., ul.y, ul.z);
     double ell = sqrt(w.x * w.x + w.y * w.y + w.z * w.z);
     double eps = ell * 2. / L - 1.;
     double Lsum = ((G.dL[0] + G.dL[1]) + G.dL[2]) + G.dL[3];
-    // f_q (gradient of Π w.r.t. q = (ul, vl)) and D_q (Hessian of the internal part)
     double fq[6] = {0, 0, 0, 0, 0, 0};
-    double Dq[21];   // upper triangle of 6×6, row-major
-    for (int i = 0; i < 21; ++i) Dq[i] = 0.;
-    auto dq = [&](int i, int j) -> double& { return Dq[i * 6 - (i * (i - 1)) / 2 + (j - i)]; };
+    // D_q = ∂²Π_int/∂q² (q = (ul, vl)): dense uu block, u₂v₃ and u₃v₂ couplings, diagonal vv
+    s3 Duu;
+    double D15 = 0., D24 = 0., D33 = 0., D44 = 0., D55 = 0.;
     {
-        double de = 2. / L / ell;                  // ∂ε/∂w = de·w
-        double fa = S.EA * Lsum * eps;             // ∂Π_a/∂ε
+        double de = 2. / L / ell;
+        double fa = S.EA * Lsum * eps;
         fq[0] += fa * de * w.x; fq[1] += fa * de * w.y; fq[2] += fa * de * w.z;
-        double c1 = S.EA * Lsum * de * de;          // EA Lsum (∂ε)(∂ε)ᵀ
-        double c2 = fa * de;                        // EA Lsum ε ∂²ε = fa·de·(I - w wᵀ/ℓ²)
-        double il2 = 1. / (ell * ell);
-        double ww[3] = {w.x, w.y, w.z};
-        for (int i = 0; i < 3; ++i)
-            for (int j = i; j < 3; ++j) dq(i, j) += c1 * ww[i] * ww[j] + c2 * ((i == j ? 1. : 0.) - ww[i] * ww[j] * il2);
+        double c1 = S.EA * Lsum * de * de, c2 = fa * de, il2 = 1. / (ell * ell);
+        Duu.xx = c1 * w.x * w.x + c2 * (1. - w.x * w.x * il2);
+        Duu.yy = c1 * w.y * w.y + c2 * (1. - w.y * w.y * il2);
+        Duu.zz = c1 * w.z * w.z + c2 * (1. - w.z * w.z * il2);
+        Duu.xy = c1 * w.x * w.y - c2 * w.x * w.y * il2;
+        Duu.xz = c1 * w.x * w.z - c2 * w.x * w.z * il2;
+        Duu.yz = c1 * w.y * w.z - c2 * w.y * w.z * il2;
     }
     const double ka = 2. / L, kv = 2. / L;
     for (int g = 0; g < 4; ++g) {
@@ -303,163 +320,156 @@ MH_HD void beam_static(const BeamGeo& G, const BeamSection& S, v3 fe, const doub
         double ku = (-24. * z) / (L * L);
         double k1v = ka * vl.x, k2v = ku * ul.y + kv * vl.z, k3v = ku * ul.z - kv * vl.y;
         double dl = G.dL[g];
-        // mᵢ = (GJ κ₁, EI₃ κ₂, EI₂ κ₃)
         double m1 = S.GJ * k1v, m2 = S.EI3 * k2v, m3v = S.EI2 * k3v;
         fq[3] += dl * m1 * ka;
         fq[1] += dl * m2 * ku; fq[5] += dl * m2 * kv;
         fq[2] += dl * m3v * ku; fq[4] += -dl * m3v * kv;
-        dq(3, 3) += dl * S.GJ * ka * ka;
-        dq(1, 1) += dl * S.EI3 * ku * ku; dq(1, 5) += dl * S.EI3 * ku * kv; dq(5, 5) += dl * S.EI3 * kv * kv;
-        dq(2, 2) += dl * S.EI2 * ku * ku; dq(2, 4) += -dl * S.EI2 * ku * kv; dq(4, 4) += dl * S.EI2 * kv * kv;
+        D33 += dl * S.GJ * ka * ka;
+        Duu.yy += dl * S.EI3 * ku * ku; D15 += dl * S.EI3 * ku * kv; D55 += dl * S.EI3 * kv * kv;
+        Duu.zz += dl * S.EI2 * ku * ku; D24 += -dl * S.EI2 * ku * kv; D44 += dl * S.EI2 * kv * kv;
     }
-    // external load: Π_e = Σ_gp dL fₑ·(rₛₘ p_gp + cₛₘ), p_gp = tgₑ ζ + Y_gp q
-    v3 gphi_e = mk(0, 0, 0);
-    double Hee[6] = {0, 0, 0, 0, 0, 0};   // Σ [sym(e sᵀ) - (e·s) I] as s3 (xx,yy,zz,xy,xz,yz)
-    double Ye[3][6];                       // Ỹ = Σ spin(e) rₛₘ Y_gp (3×6)
-    for (int i = 0; i < 3; ++i) for (int j = 0; j < 6; ++j) Ye[i][j] = 0.;
-    double esum[3] = {0, 0, 0};
-    for (int g = 0; g < 4; ++g) {
-        double z = G.zgp[g];
-        double ya = 2. * z, yu = -4. * (z * z * z) + 3. * z, yv = (z * z - 1. / 4.) * L;
-        v3 y = mk(ya * ul.x, yu * ul.y + yv * vl.z, yu * ul.z - yv * vl.y);
-        v3 p = mk(L * z + y.x, y.y, y.z);
-        v3 s = mul(rsm, p);
-        v3 e = G.dL[g] * fe;
-        esum[0] += e.x; esum[1] += e.y; esum[2] += e.z;
-        gphi_e = gphi_e + cross(s, e);
-        double es = dot(e, s);
-        Hee[0] += e.x * s.x - es; Hee[1] += e.y * s.y - es; Hee[2] += e.z * s.z - es;
-        Hee[3] += 0.5 * (e.x * s.y + s.x * e.y); Hee[4] += 0.5 * (e.x * s.z + s.x * e.z); Hee[5] += 0.5 * (e.y * s.z + s.y * e.z);
-        // f_q^ext = Yᵀ rₛₘᵀ e
-        v3 re = mulT(rsm, e);
-        fq[0] += ya * re.x; fq[1] += yu * re.y; fq[5] += yv * re.y; fq[2] += yu * re.z; fq[4] += -yv * re.z;
-        // spin(e) rₛₘ Y: columns of rₛₘ Y are rₛₘ·(Y column)
+    // external load Π_e = Σ_gp dL fₑ·(rₛₘ p_gp + cₛₘ), p_gp = tgₑ ζ + Y_gp q
+    v3 gphi_e = mk(0, 0, 0), esum = mk(0, 0, 0);
+    s3 Hee; Hee.xx = Hee.yy = Hee.zz = Hee.xy = Hee.xz = Hee.yz = 0.;
+    v3 Ye[6];   // columns of Ỹ = Σ spin(e) rₛₘ Y_gp over q = (u1,u2,u3,v1,v2,v3); column v1 is zero
+    for (int j = 0; j < 6; ++j) Ye[j] = mk(0, 0, 0);
+    {
         v3 r0 = col(rsm, 0), r1 = col(rsm, 1), r2 = col(rsm, 2);
-        v3 yc[6] = {ya * r0, yu * r1, yu * r2, mk(0, 0, 0), -yv * r2, yv * r1};
-        for (int j = 0; j < 6; ++j) { v3 c = cross(e, yc[j]); Ye[0][j] += c.x; Ye[1][j] += c.y; Ye[2][j] += c.z; }
+        for (int g = 0; g < 4; ++g) {
+            double z = G.zgp[g];
+            double ya = 2. * z, yu = -4. * (z * z * z) + 3. * z, yv = (z * z - 1. / 4.) * L;
+            v3 y = mk(ya * ul.x, yu * ul.y + yv * vl.z, yu * ul.z - yv * vl.y);
+            v3 s = mul(rsm, mk(L * z + y.x, y.y, y.z));
+            v3 e = G.dL[g] * fe;
+            esum = esum + e;
+            gphi_e = gphi_e + cross(s, e);
+            double es = dot(e, s);
+            Hee.xx += e.x * s.x - es; Hee.yy += e.y * s.y - es; Hee.zz += e.z * s.z - es;
+            Hee.xy += 0.5 * (e.x * s.y + s.x * e.y); Hee.xz += 0.5 * (e.x * s.z + s.x * e.z); Hee.yz += 0.5 * (e.y * s.z + s.y * e.z);
+            v3 re = mulT(rsm, e);
+            fq[0] += ya * re.x; fq[1] += yu * re.y; fq[5] += yv * re.y; fq[2] += yu * re.z; fq[4] += -yv * re.z;
+            Ye[0] = Ye[0] + cross(e, ya * r0);
+            Ye[1] = Ye[1] + cross(e, yu * r1);
+            Ye[2] = Ye[2] + cross(e, yu * r2);
+            Ye[4] = Ye[4] + cross(e, -yv * r2);
+            Ye[5] = Ye[5] + cross(e, yv * r1);
+        }
     }
-    // ---- gradients w.r.t. local coordinates z = (φ, Δ, d, c) -----------------
     v3 Fu = mul(rsm, mk(fq[0], fq[1], fq[2]));
     v3 Fv = mul(rsm, mk(fq[3], fq[4], fq[5]));
     v3 gphi = cross(Fu, d) + cross(Fv, dv) + gphi_e;
     v3 gD = Fv;
-    // ---- residual R = Aᵀ g_z -----------------------------------------------------
-    for (int i = 0; i < 3; ++i) {
-        double gd = (&Fu.x)[i], gc = esum[i];
-        out.R[i] = -0.5 * gd + 0.5 * gc;
-        out.R[6 + i] = 0.5 * gd + 0.5 * gc;
-    }
-    for (int j = 0; j < 6; ++j) {
-        double r = dot(wsm[j], gphi) + dot(JD[j], gD);
-        out.R[j < 3 ? 3 + j : 6 + j] = r;
-    }
-    // ---- tangent --------------------------------------------------------------
-    // Bq = ∂q/∂X (6×12): δul = rᵀ(δd + spin(d) φ), δvl = rᵀ(Δ + spin(Δv) φ)
-    double Bq[6][12];
-    for (int i = 0; i < 6; ++i) for (int j = 0; j < 12; ++j) Bq[i][j] = 0.;
-    for (int i = 0; i < 3; ++i) {
-        v3 ri = col(rsm, i);   // row i of rₛₘᵀ
-        for (int k = 0; k < 3; ++k) { Bq[i][k] = -0.5 * (&ri.x)[k]; Bq[i][6 + k] = 0.5 * (&ri.x)[k]; }
-        for (int j = 0; j < 6; ++j) {
-            int cj = j < 3 ? 3 + j : 6 + j;
-            Bq[i][cj] = dot(ri, cross(d, wsm[j]));
-            Bq[3 + i][cj] = dot(ri, JD[j] + cross(dv, wsm[j]));
+    // translational residual R = Aᵀ g_z on u₁, u₂
+    out.R[0] = -0.5 * Fu.x + 0.5 * esum.x; out.R[1] = -0.5 * Fu.y + 0.5 * esum.y; out.R[2] = -0.5 * Fu.z + 0.5 * esum.z;
+    out.R[6] = 0.5 * Fu.x + 0.5 * esum.x;  out.R[7] = 0.5 * Fu.y + 0.5 * esum.y;  out.R[8] = 0.5 * Fu.z + 0.5 * esum.z;
+    // ---- rotation core, first-order spin Jacobians (6 columns: δv₁ then δv₂) ----------
+    v3 JD[6], W[6], HT1[3], MT1[3], T2c[3];
+    m3 TH = tangent_op(dv, kh);
+    {
+        m3 T1 = tangent_op(v1, k1), T2 = tangent_op(v2, k2), Tpi = tangent_inv(phi, kp);
+        for (int j = 0; j < 3; ++j) {
+            v3 t1 = col(T1, j);
+            T2c[j] = col(T2, j);
+            MT1[j] = mul(M, t1);
+            HT1[j] = mul(H, t1);
+            JD[j] = mul(Tpi, -0.5 * MT1[j]);
+            JD[3 + j] = mul(Tpi, 0.5 * T2c[j]);
         }
-    }
-    // K1 = Bqᵀ Dq Bq
-    double DB[6][12];
-    for (int i = 0; i < 6; ++i)
-        for (int j = 0; j < 12; ++j) {
-            double s = 0.;
-            for (int k = 0; k < 6; ++k) s += (i <= k ? dq(i, k) : dq(k, i)) * Bq[k][j];
-            DB[i][j] = s;
-        }
-    for (int i = 0; i < 12; ++i)
-        for (int j = i; j < 12; ++j) {
-            double s = 0.;
-            for (int k = 0; k < 6; ++k) s += Bq[k][i] * DB[k][j];
-            out.K[tri(i, j)] = s;
-        }
-    // Hφφ' = sym(Fu dᵀ) - (Fu·d)I + sym(Fv Δvᵀ) - (Fv·Δv)I + Σ[sym(e sᵀ) - (e·s)I]
-    double fud = dot(Fu, d), fvd = dot(Fv, dv);
-    s3 Hpp;
-    Hpp.xx = Fu.x * d.x - fud + Fv.x * dv.x - fvd + Hee[0];
-    Hpp.yy = Fu.y * d.y - fud + Fv.y * dv.y - fvd + Hee[1];
-    Hpp.zz = Fu.z * d.z - fud + Fv.z * dv.z - fvd + Hee[2];
-    Hpp.xy = 0.5 * (Fu.x * d.y + d.x * Fu.y) + 0.5 * (Fv.x * dv.y + dv.x * Fv.y) + Hee[3];
-    Hpp.xz = 0.5 * (Fu.x * d.z + d.x * Fu.z) + 0.5 * (Fv.x * dv.z + dv.x * Fv.z) + Hee[4];
-    Hpp.yz = 0.5 * (Fu.y * d.z + d.y * Fu.z) + 0.5 * (Fv.y * dv.z + dv.y * Fv.z) + Hee[5];
-    // N' = spin(Fv) AΔ + spin(Fu) Ad - Ỹ Bq   (3×12); Aφ = wsm on the rotation columns
-    double Np[3][12];
-    for (int j = 0; j < 12; ++j) {
-        v3 c = mk(0, 0, 0);
-        if (j < 3) c = cross(Fu, mk(j == 0 ? -0.5 : 0., j == 1 ? -0.5 : 0., j == 2 ? -0.5 : 0.));
-        else if (j >= 6 && j < 9) c = cross(Fu, mk(j == 6 ? 0.5 : 0., j == 7 ? 0.5 : 0., j == 8 ? 0.5 : 0.));
-        else { int r = j < 6 ? j - 3 : j - 6; c = cross(Fv, JD[r]); }
-        for (int i = 0; i < 3; ++i) {
-            double s = 0.;
-            for (int k = 0; k < 6; ++k) s += Ye[i][k] * Bq[k][j];
-            Np[i][j] = (&c.x)[i] - s;
-        }
-    }
-    // K += Aφᵀ Hφφ' Aφ + Aφᵀ N' + N'ᵀ Aφ   (Aφ nonzero only on rotation columns)
-    auto rot = [](int r) { return r < 3 ? 3 + r : 6 + r; };
-    for (int a = 0; a < 6; ++a) {
-        v3 hw = smul(Hpp, wsm[a]);
-        for (int b = a; b < 6; ++b) {
-            int i = rot(a), j = rot(b);
-            out.K[tri(i, j)] += dot(wsm[b], hw);
-        }
-    }
-    for (int a = 0; a < 6; ++a) {
-        int i = rot(a);
-        for (int j = 0; j < 12; ++j) {
-            double t = wsm[a].x * Np[0][j] + wsm[a].y * Np[1][j] + wsm[a].z * Np[2][j];
-            if (i <= j) out.K[tri(i, j)] += t;
-            if (j <= i) out.K[tri(j, i)] += t;   // (N'ᵀAφ)[j][i]; the diagonal gets both
-        }
-    }
-    // ---- second-order rotation chain, contracted with the adjoints --------------
-    // μ = ½ T(φ)⁻ᵀ (gΔ + T_Hᵀ gφ);  λ₁ = Hᵀ gφ - Mᵀ μ;  λ₂ = μ
-    v3 lamD = gD + mulT(TH, gphi);
-    v3 mu = 0.5 * mulT(Tpi, lamD);
-    v3 lam1 = mulT(H, gphi) - mulT(M, mu);
-    double Gr[21];
-    for (int i = 0; i < 21; ++i) Gr[i] = 0.;
-    auto gr = [&](int i, int j) -> double& { return Gr[i * 6 - (i * (i - 1)) / 2 + (j - i)]; };
-    {   // (i) λ₁·W₁, (ii) λ₂·W₂
+        for (int j = 0; j < 6; ++j) W[j] = mul(TH, JD[j]);
+        for (int j = 0; j < 3; ++j) W[j] = W[j] + HT1[j];
+        // ---- second-order rotation chain contracted with the adjoints (into S) ------
+        // μ = ½ T(φ)⁻ᵀ (gΔ + T_Hᵀ gφ);  λ₁ = Hᵀ gφ - Mᵀ μ;  λ₂ = μ
+        v3 mu = 0.5 * mulT(Tpi, gD + mulT(TH, gphi));
+        v3 lam1 = mulT(H, gphi) - mulT(M, mu);
+        for (int i = 0; i < 21; ++i) out.Kc[24 + i] = 0.;
+        double* Sm = out.Kc + 24;
         s3 B1 = dT_form(v1, lam1, k1), B2 = dT_form(v2, mu, k2);
-        gr(0, 0) += B1.xx; gr(1, 1) += B1.yy; gr(2, 2) += B1.zz; gr(0, 1) += B1.xy; gr(0, 2) += B1.xz; gr(1, 2) += B1.yz;
-        gr(3, 3) += B2.xx; gr(4, 4) += B2.yy; gr(5, 5) += B2.zz; gr(3, 4) += B2.xy; gr(3, 5) += B2.xz; gr(4, 5) += B2.yz;
-    }
-    {   // (iii) J_Δᵀ sym B(gφ; Δv) J_Δ  and  (iv) -J_φᵀ sym B(μ; φ) J_φ
+        Sm[sym6(0, 0)] += B1.xx; Sm[sym6(1, 1)] += B1.yy; Sm[sym6(2, 2)] += B1.zz;
+        Sm[sym6(0, 1)] += B1.xy; Sm[sym6(0, 2)] += B1.xz; Sm[sym6(1, 2)] += B1.yz;
+        Sm[sym6(3, 3)] += B2.xx; Sm[sym6(4, 4)] += B2.yy; Sm[sym6(5, 5)] += B2.zz;
+        Sm[sym6(3, 4)] += B2.xy; Sm[sym6(3, 5)] += B2.xz; Sm[sym6(4, 5)] += B2.yz;
         s3 BH = dT_form(dv, gphi, kh), BP = dT_form(phi, mu, kp);
         for (int a = 0; a < 6; ++a) {
-            v3 h1 = smul(BH, JD[a]), h2 = smul(BP, Jphi[a]);
-            for (int b = a; b < 6; ++b) gr(a, b) += dot(JD[b], h1) - dot(Jphi[b], h2);
+            // J_φ = 2 J_Δ:  J_Δᵀ B_H J_Δ - J_φᵀ B_P J_φ = J_Δᵀ (B_H - 4 B_P) J_Δ
+            v3 h = smul(BH, JD[a]) - 4. * smul(BP, JD[a]);
+            v3 wHa = a < 3 ? W[a] - HT1[a] : W[a];
+            for (int b = a; b < 6; ++b) {
+                v3 wHb = b < 3 ? W[b] - HT1[b] : W[b];
+                double t = dot(JD[b], h);
+                v3 hwa = a < 3 ? HT1[a] : mk(0, 0, 0), hwb = b < 3 ? HT1[b] : mk(0, 0, 0);
+                t += -0.5 * (dot(wHa, cross(gphi, hwb)) - dot(hwa, cross(gphi, wHb)));
+                v3 w2a = a >= 3 ? T2c[a - 3] : mk(0, 0, 0), w2b = b >= 3 ? T2c[b - 3] : mk(0, 0, 0);
+                v3 mwa = a < 3 ? MT1[a] : mk(0, 0, 0), mwb = b < 3 ? MT1[b] : mk(0, 0, 0);
+                t += 0.5 * (dot(w2a, cross(mu, mwb)) - dot(mwa, cross(mu, w2b)));
+                Sm[sym6(a, b)] += t;
+            }
         }
     }
-    {   // (v) -½(ω_Hᵀ S_g Hω₁ - (Hω₁)ᵀ S_g ω_H), S_g = spin(gφ), Hω₁ = [H T₁, 0]
-        //     as a bilinear form: -½(ω_H a · (gφ × Hω₁ b) - Hω₁ a · (gφ × ω_H b))
-        // (vi) +½(ω₂ᵀ S_μ Mω₁ - (Mω₁)ᵀ S_μ ω₂), ω₂ = [0, T₂], Mω₁ = [M T₁, 0]
-        v3 t2c[3] = {col(T2, 0), col(T2, 1), col(T2, 2)};
-        for (int a = 0; a < 6; ++a)
-            for (int b = a; b < 6; ++b) {
-                v3 hw_a = a < 3 ? HT1[a] : mk(0, 0, 0), hw_b = b < 3 ? HT1[b] : mk(0, 0, 0);
-                double t = -0.5 * (dot(wH[a], cross(gphi, hw_b)) - dot(hw_a, cross(gphi, wH[b])));
-                v3 w2a = a >= 3 ? t2c[a - 3] : mk(0, 0, 0), w2b = b >= 3 ? t2c[b - 3] : mk(0, 0, 0);
-                v3 mw_a = a < 3 ? MT1[a] : mk(0, 0, 0), mw_b = b < 3 ? MT1[b] : mk(0, 0, 0);
-                t += 0.5 * (dot(w2a, cross(mu, mw_b)) - dot(mw_a, cross(mu, w2b)));
-                gr(a, b) += t;
-            }
+    // rotational residual
+    for (int j = 0; j < 6; ++j) out.R[j < 3 ? 3 + j : 6 + j] = dot(W[j], gphi) + dot(JD[j], gD);
+    // ---- A = rₛₘ D_uu rₛₘᵀ ----------------------------------------------------------------
+    {
+        v3 c0 = col(rsm, 0), c1 = col(rsm, 1), c2 = col(rsm, 2);
+        // rows of rₛₘ D_uu: (rₛₘ D_uu)[i][k] = Σ_m rsm[i][m] D[m][k]
+        v3 Dc0 = mk(Duu.xx, Duu.xy, Duu.xz), Dc1 = mk(Duu.xy, Duu.yy, Duu.yz), Dc2 = mk(Duu.xz, Duu.yz, Duu.zz);
+        v3 P0 = mul(rsm, Dc0), P1 = mul(rsm, Dc1), P2 = mul(rsm, Dc2);   // columns of rₛₘ D_uu
+        // A = (rₛₘ D_uu) rₛₘᵀ: A[i][j] = Σ_k P_k[i] rsm[j][k]
+        auto Aij = [&](int i, int j) {
+            return (&P0.x)[i] * at(rsm, j, 0) + (&P1.x)[i] * at(rsm, j, 1) + (&P2.x)[i] * at(rsm, j, 2);
+        };
+        out.Kc[0] = Aij(0, 0); out.Kc[1] = Aij(1, 1); out.Kc[2] = Aij(2, 2);
+        out.Kc[3] = Aij(0, 1); out.Kc[4] = Aij(0, 2); out.Kc[5] = Aij(1, 2);
+        (void)c0; (void)c1; (void)c2;
     }
-    for (int a = 0; a < 6; ++a)
-        for (int b = a; b < 6; ++b) out.K[tri(rot(a), rot(b))] += gr(a, b);
-    // NaN check like getresidual's hasnan (Assemble.jl:625)
+    // ---- Br = ∂q/∂v (6×6), T, and the remaining rotation-rotation terms ---------------------
+    {
+        // Q3 = spin(F_u) - Ỹ_u rₛₘᵀ   (3×3),  T = rₛₘ(D_uu Br_u + D_uv Br_v) + Q3ᵀ W
+        s3 Hpp;   // Hφφ' = sym(Fu dᵀ) - (Fu·d)I + sym(Fv Δvᵀ) - (Fv·Δv)I + Σ[sym(e sᵀ) - (e·s)I]
+        double fud = dot(Fu, d), fvd = dot(Fv, dv);
+        Hpp.xx = Fu.x * d.x - fud + Fv.x * dv.x - fvd + Hee.xx;
+        Hpp.yy = Fu.y * d.y - fud + Fv.y * dv.y - fvd + Hee.yy;
+        Hpp.zz = Fu.z * d.z - fud + Fv.z * dv.z - fvd + Hee.zz;
+        Hpp.xy = 0.5 * (Fu.x * d.y + d.x * Fu.y) + 0.5 * (Fv.x * dv.y + dv.x * Fv.y) + Hee.xy;
+        Hpp.xz = 0.5 * (Fu.x * d.z + d.x * Fu.z) + 0.5 * (Fv.x * dv.z + dv.x * Fv.z) + Hee.xz;
+        Hpp.yz = 0.5 * (Fu.y * d.z + d.y * Fu.z) + 0.5 * (Fv.y * dv.z + dv.y * Fv.z) + Hee.yz;
+        double* Sm = out.Kc + 24;
+        v3 Bu[6], Bv[6], Np[6];
+        for (int j = 0; j < 6; ++j) {
+            Bu[j] = mulT(rsm, cross(d, W[j]));
+            Bv[j] = mulT(rsm, JD[j] + cross(dv, W[j]));
+            // N'_rot column j = spin(Fv) JD_j - Ỹ Br_j   (Ỹ column 3 (v1) is zero)
+            v3 yb = (Bu[j].x * Ye[0] + Bu[j].y * Ye[1]) + Bu[j].z * Ye[2];
+            yb = yb + Bv[j].y * Ye[4] + Bv[j].z * Ye[5];
+            Np[j] = cross(Fv, JD[j]) - yb;
+        }
+        for (int a = 0; a < 6; ++a) {
+            // D Br column a
+            v3 DBu = smul(Duu, Bu[a]) + mk(0., D15 * Bv[a].z, D24 * Bv[a].y);
+            v3 DBv = mk(D33 * Bv[a].x, D44 * Bv[a].y + D24 * Bu[a].z, D55 * Bv[a].z + D15 * Bu[a].y);
+            v3 hw = smul(Hpp, W[a]);
+            for (int b = a; b < 6; ++b) {
+                double t = dot(Bu[b], DBu) + dot(Bv[b], DBv) + dot(W[b], hw) + dot(W[a], Np[b]) + dot(W[b], Np[a]);
+                Sm[sym6(a, b)] += t;
+            }
+            // T column a: rₛₘ (D_uu Bu_a + D_uv Bv_a) + Q3ᵀ W_a
+            v3 C = mul(rsm, DBu);
+            // Q3ᵀ W_a = spin(Fu)ᵀ W_a - rₛₘ Ỹ_uᵀ W_a = W_a × Fu - rₛₘ (Ye_k · W_a)_k
+            v3 yw = mk(dot(Ye[0], W[a]), dot(Ye[1], W[a]), dot(Ye[2], W[a]));
+            v3 Tc = C + cross(W[a], Fu) - mul(rsm, yw);
+            out.Kc[6 + 0 * 6 + a] = Tc.x; out.Kc[6 + 1 * 6 + a] = Tc.y; out.Kc[6 + 2 * 6 + a] = Tc.z;
+        }
+    }
     int nan = 0;
     for (int i = 0; i < 12; ++i) nan |= (out.R[i] != out.R[i]);
-    for (int i = 0; i < 78; ++i) nan |= (out.K[i] != out.K[i]);
+    for (int i = 0; i < kCompact; ++i) nan |= (out.Kc[i] != out.Kc[i]);
     out.nan = nan;
+}
+
+// Full 12×12 entry from the compact form (host harnesses and checks).
+MH_HD double compact_entry(const double* Kc, int i, int j) {
+    double m;
+    int s = compact_slot(i, j, &m);
+    return m * Kc[s];
 }
 
 }  // namespace mh

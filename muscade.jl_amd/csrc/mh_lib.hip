@@ -4,12 +4,14 @@
 //   node coordinates   xyz[3*n + c]                      (node-major: one 24-byte gather per node)
 //   element dof maps   idx[k*nele + e]   int32, 0-based  (SoA: a wave loads 64 consecutive ints)
 //   beam data          o2[c*nele + e], sec[f*nele + e]   (SoA)
-//   element outputs    Kbuf[kbase + (i + nX*j)*nele + e] (SoA, one coalesced store per entry)
-//                      Rbuf[rbase + i*nele + e]
-//   assembly maps      ksrc[p] = Kbuf index of the p-th contribution in (column,row) order,
+//   element outputs    beams: Kbuf[kbase + 45*e + slot], the 45 numbers of the compact
+//                      static tangent (beam_math.hpp), element-major; other kinds:
+//                      Kbuf[kbase + nX²*e + i + nX*j]; Rbuf[rbase + i*nele + e] (SoA)
+//   assembly maps      ksrc[p] = (code << 29) | Kbuf index of the p-th contribution in
+//                      (column,row) order, code = exact multiplier ±1, ±½, ±¼;
 //                      kseg[k] = first contribution of nz k; rsrc/rseg likewise for the rhs.
 //
-// Assembly is a deterministic segmented gather-reduce: nzval[k] = Σ Kbuf[ksrc[p]],
+// Assembly is a deterministic segmented gather-reduce: nzval[k] = Σ mult·Kbuf[idx],
 // p = kseg[k]..kseg[k+1]-1, summed from 0 in increasing p. The radix sort that
 // builds ksrc is stable and its input is in Muscade's traversal order
 // (ieletyp, iele, jeledof, ieledof — src/Assemble.jl:384-395), so each nz is
@@ -42,6 +44,8 @@ struct Group {
     double* d_vals = nullptr;    // nele (DofLoad values)
     std::vector<int64_t> h_idx;  // nX × nele, 1-based (for mh_get_asm)
     int64_t kbase = 0, rbase = 0;
+    int64_t kper = 0;            // Kbuf entries per element (45 for beams, nX² otherwise)
+    bool compact = false;
 };
 
 struct Ctx {
@@ -56,7 +60,7 @@ struct Ctx {
     double* d_A = nullptr;
     // prepared
     bool prepared = false;
-    int64_t nK = 0, nR = 0, nnz = 0;
+    int64_t nK = 0, nP = 0, nR = 0, nnz = 0;
     double* d_Kbuf = nullptr;
     double* d_Rbuf = nullptr;
     double* d_dRbuf = nullptr;
@@ -65,9 +69,9 @@ struct Ctx {
     int32_t* d_colptr = nullptr;
     int32_t* d_rowval = nullptr;
     int32_t* d_kseg = nullptr;
-    int32_t* d_ksrc = nullptr;
+    uint32_t* d_ksrc = nullptr;
     int32_t* d_rseg = nullptr;
-    int32_t* d_rsrc = nullptr;
+    uint32_t* d_rsrc = nullptr;
     unsigned long long* d_flag = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timed = false;
@@ -101,8 +105,16 @@ void free_model(Ctx* c) {
 // ------------------------------------------------------------------ kernels
 constexpr int kBlock = 256;
 
+constexpr int kBeamBlock = 128;
+constexpr int kXCD = 8;   // XCDs of an MI355X; workgroups are dispatched round-robin over them
+
+// Workgroup id → logical tile such that consecutive tiles run on one XCD (its own L2).
+__device__ __forceinline__ int64_t xcd_tile(int64_t per_xcd) {
+    return (int64_t)(blockIdx.x % kXCD) * per_xcd + blockIdx.x / kXCD;
+}
+
 struct BeamArgs {
-    int64_t nele;
+    int64_t nele, per_xcd;
     const int32_t* idx;
     const int32_t* uidx;
     const int32_t* nodes;
@@ -111,48 +123,58 @@ struct BeamArgs {
     const double* sec;
     const double* X;
     const double* U;
-    double s[12];
-    double* K;
+    double sA[6], sT[18], sS[21], sR[12];   // dof-scale products per compact slot / residual entry
+    double* K;   // compact blocks, 45 per element
     double* R;
     unsigned long long* flag;
     unsigned long long gtag;
 };
 
-// One element per lane. Static EulerBeam3D residual + tangent (OX=0).
+// One element per lane. Static EulerBeam3D residual + compact tangent (OX=0).
+// The 45 tangent numbers of the block's elements are staged in LDS and written
+// as one contiguous, coalesced range.
 template <bool UDOF>
-__global__ __launch_bounds__(kBlock) void beam_static_kernel(BeamArgs a) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= a.nele) return;
+__global__ __launch_bounds__(kBeamBlock) void beam_static_kernel(BeamArgs a) {
+    __shared__ double tile[kBeamBlock * mh::kCompact];
+    const int64_t e0 = xcd_tile(a.per_xcd) * kBeamBlock;
+    if (e0 >= a.nele) return;   // whole workgroup leaves together
+    const int tid = threadIdx.x;
+    const int64_t e = e0 + tid;
     const int64_t n = a.nele;
-    double X[12];
+    if (e < n) {
+        double X[12];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) X[k] = a.X[a.idx[k * n + e]];
-    const int32_t n1 = a.nodes[e], n2 = a.nodes[n + e];
-    mh::v3 c1 = mh::mk(a.xyz[3 * (int64_t)n1], a.xyz[3 * (int64_t)n1 + 1], a.xyz[3 * (int64_t)n1 + 2]);
-    mh::v3 c2 = mh::mk(a.xyz[3 * (int64_t)n2], a.xyz[3 * (int64_t)n2 + 1], a.xyz[3 * (int64_t)n2 + 2]);
-    mh::v3 o2 = mh::mk(a.o2[e], a.o2[n + e], a.o2[2 * n + e]);
-    mh::BeamSection S;
-    S.EA = a.sec[e]; S.EI2 = a.sec[n + e]; S.EI3 = a.sec[2 * n + e]; S.GJ = a.sec[3 * n + e]; S.w = a.sec[6 * n + e];
-    mh::v3 fe = mh::mk(0., 0., S.w);
-    if (UDOF) {
-        fe.x = fe.x - a.U[a.uidx[e]];
-        fe.y = fe.y - a.U[a.uidx[n + e]];
-        fe.z = fe.z - a.U[a.uidx[2 * n + e]];
-    }
-    mh::BeamGeo G = mh::beam_geometry(c1, c2, o2);
-    mh::BeamOut out;
-    mh::beam_static(G, S, fe, X, out);
-#pragma unroll
-    for (int i = 0; i < 12; ++i) a.R[i * n + e] = out.R[i] * a.s[i];
-#pragma unroll
-    for (int i = 0; i < 12; ++i)
-#pragma unroll
-        for (int j = i; j < 12; ++j) {
-            double v = out.K[mh::tri(i, j)];
-            a.K[(i + 12 * j) * n + e] = v * a.s[i] * a.s[j];
-            if (i != j) a.K[(j + 12 * i) * n + e] = v * a.s[j] * a.s[i];
+        for (int k = 0; k < 12; ++k) X[k] = a.X[a.idx[k * n + e]];
+        const int32_t n1 = a.nodes[e], n2 = a.nodes[n + e];
+        mh::v3 c1 = mh::mk(a.xyz[3 * (int64_t)n1], a.xyz[3 * (int64_t)n1 + 1], a.xyz[3 * (int64_t)n1 + 2]);
+        mh::v3 c2 = mh::mk(a.xyz[3 * (int64_t)n2], a.xyz[3 * (int64_t)n2 + 1], a.xyz[3 * (int64_t)n2 + 2]);
+        mh::v3 o2 = mh::mk(a.o2[e], a.o2[n + e], a.o2[2 * n + e]);
+        mh::BeamSection S;
+        S.EA = a.sec[e]; S.EI2 = a.sec[n + e]; S.EI3 = a.sec[2 * n + e]; S.GJ = a.sec[3 * n + e]; S.w = a.sec[6 * n + e];
+        mh::v3 fe = mh::mk(0., 0., S.w);
+        if (UDOF) {
+            fe.x = fe.x - a.U[a.uidx[e]];
+            fe.y = fe.y - a.U[a.uidx[n + e]];
+            fe.z = fe.z - a.U[a.uidx[2 * n + e]];
         }
-    if (out.nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+        mh::BeamGeo G = mh::beam_geometry(c1, c2, o2);
+        mh::BeamOut out;
+        mh::beam_static(G, S, fe, X, out);
+#pragma unroll
+        for (int i = 0; i < 12; ++i) a.R[i * n + e] = out.R[i] * a.sR[i];
+        double* t = tile + tid * mh::kCompact;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) t[i] = out.Kc[i] * a.sA[i];
+#pragma unroll
+        for (int i = 0; i < 18; ++i) t[6 + i] = out.Kc[6 + i] * a.sT[i];
+#pragma unroll
+        for (int i = 0; i < 21; ++i) t[24 + i] = out.Kc[24 + i] * a.sS[i];
+        if (out.nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+    }
+    __syncthreads();
+    const int64_t nv = (n - e0 < kBeamBlock ? n - e0 : kBeamBlock) * mh::kCompact;
+    double* dst = a.K + e0 * mh::kCompact;
+    for (int64_t f = tid; f < nv; f += kBeamBlock) dst[f] = tile[f];
 }
 
 struct PointArgs {
@@ -182,54 +204,94 @@ __global__ __launch_bounds__(kBlock) void point_kernel(PointArgs a) {
         double R0 = -lam, R1 = -x;
         a.R[e] = R0 * a.s[0];
         a.R[n + e] = R1 * a.s[1];
-        a.K[0 * n + e] = 0.;
-        a.K[1 * n + e] = -1. * a.s[0] * a.s[1];   // (i=1,j=0): ∂R₁/∂x
-        a.K[2 * n + e] = -1. * a.s[1] * a.s[0];   // (i=0,j=1): ∂R₀/∂λ
-        a.K[3 * n + e] = 0.;
+        double* K = a.K + 4 * e;
+        K[0] = 0.;
+        K[1] = -1. * a.s[0] * a.s[1];   // (i=1,j=0): ∂R₁/∂x
+        K[2] = -1. * a.s[1] * a.s[0];   // (i=0,j=1): ∂R₀/∂λ
+        K[3] = 0.;
         if (x != x || lam != lam) atomicMin(a.flag, a.gtag | (unsigned long long)e);
     } else if (a.kind == MH_KIND_DOFLOAD) {
         double F = a.vals[e];
         a.R[e] = -F * a.s[0];
-        a.K[e] = 0.;
+        a.K[e] = 0.;   // 1×1 block
         if (F != F) atomicMin(a.flag, a.gtag | (unsigned long long)e);
     } else {
         for (int i = 0; i < nX; ++i) a.R[i * n + e] = 0.;
-        for (int i = 0; i < nX * nX; ++i) a.K[i * n + e] = 0.;
+        for (int i = 0; i < nX * nX; ++i) a.K[(int64_t)nX * nX * e + i] = 0.;
     }
     if (a.dR)
         for (int i = 0; i < nX; ++i) a.dR[i * n + e] = 0.;
 }
 
-// nzval[k] = Σ Kbuf[src[p]] over the segment of k, summed from zero in p order.
-__global__ __launch_bounds__(kBlock) void scatter_kernel(int64_t nseg, const int32_t* __restrict__ seg,
-                                                         const int32_t* __restrict__ src, const double* __restrict__ buf,
-                                                         double* __restrict__ out) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nseg) return;
-    const int32_t p0 = seg[k], p1 = seg[k + 1];
-    double acc = 0.;
-    for (int32_t p = p0; p < p1; ++p) acc += buf[src[p]];
-    out[k] = acc;
+// Contribution source: (code << 29) | index; the code selects an exact multiplier.
+constexpr uint32_t kIdxMask = (1u << 29) - 1;
+__device__ __forceinline__ double contrib(const double* __restrict__ buf, uint32_t s) {
+    const double v = buf[s & kIdxMask];
+    switch (s >> 29) {
+    case 0: return v;
+    case 1: return -v;
+    case 2: return 0.5 * v;
+    case 3: return -0.5 * v;
+    case 4: return 0.25 * v;
+    default: return -0.25 * v;
+    }
 }
-// rhs with the :step correction: for each contribution, acc += R then acc -= ∂R/∂δr
-// (add_value! then add_∂!{1,:minus}, src/SweepX.jl:56-57, per element in order).
-__global__ __launch_bounds__(kBlock) void scatter_rhs_step_kernel(int64_t nseg, const int32_t* __restrict__ seg,
-                                                                  const int32_t* __restrict__ src, const double* __restrict__ R,
-                                                                  const double* __restrict__ dR, double* __restrict__ out) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nseg) return;
-    const int32_t p0 = seg[k], p1 = seg[k + 1];
+// Segmented gather-reduce: out[k] = Σ contrib(src[p]), p = seg[k]..seg[k+1]-1, summed
+// from zero in increasing p — the order of Muscade's sequential add_∂!/add_value! loop.
+// A workgroup owns kTile consecutive segments, whose contributions are contiguous in p:
+// the whole workgroup loads them (coalesced indices, every lane gathering) into LDS in
+// chunks, then each lane sums its own segment sequentially from LDS. Lanes of long and
+// short segments therefore share the memory work evenly.
+// STEP (rhs of the :step mission): each contribution adds R then subtracts ∂R/∂δr
+// (add_value! then add_∂!{1,:minus}, src/SweepX.jl:56-57).
+constexpr int kTile = 256;
+constexpr int kChunk = 1024;
+template <bool STEP>
+__global__ __launch_bounds__(kTile) void scatter_kernel(int64_t nseg, int64_t per_xcd, const int32_t* __restrict__ seg,
+                                                        const uint32_t* __restrict__ src, const double* __restrict__ buf,
+                                                        const double* __restrict__ dbuf, double* __restrict__ out) {
+    __shared__ double vals[kChunk];
+    __shared__ double dvals[STEP ? kChunk : 1];
+    const int64_t k0 = xcd_tile(per_xcd) * kTile;
+    if (k0 >= nseg) return;   // the whole workgroup leaves together
+    const int t = threadIdx.x;
+    const int64_t k = k0 + t;
+    const int64_t klast = (k0 + kTile < nseg ? k0 + kTile : nseg);
+    const int32_t pbeg = seg[k0], pend = seg[klast];
+    int32_t p0 = 0, p1 = 0;
+    if (k < nseg) { p0 = seg[k]; p1 = seg[k + 1]; }
     double acc = 0.;
-    for (int32_t p = p0; p < p1; ++p) { acc += R[src[p]]; acc -= dR[src[p]]; }
-    out[k] = acc;
+    for (int32_t c = pbeg; c < pend; c += kChunk) {
+        const int32_t cend = (c + kChunk < pend ? c + kChunk : pend);
+        for (int32_t q = c + t; q < cend; q += kTile) {
+            const uint32_t sq = src[q];
+            vals[q - c] = contrib(buf, sq);
+            if (STEP) dvals[q - c] = dbuf[sq & kIdxMask];
+        }
+        __syncthreads();
+        const int32_t a = p0 > c ? p0 : c, b = p1 < cend ? p1 : cend;
+        for (int32_t q = a; q < b; ++q) {
+            acc += vals[q - c];
+            if (STEP) acc -= dvals[q - c];
+        }
+        __syncthreads();
+    }
+    if (k < nseg) out[k] = acc;
 }
 
 __global__ void fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
 
+// Plain streaming copy of n doubles: the reference kernel for HBM counter calibration.
+__global__ __launch_bounds__(kBlock) void copy_kernel(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = a[i];
+}
+
 // ---- pattern construction ----------------------------------------------------
-// Pairs of one group in traversal order (iele, jeledof, ieledof), key = J·nX + I.
+// Pairs of one group in traversal order (iele, jeledof, ieledof), key = J·nX + I,
+// value = coded source of the contribution.
 __global__ __launch_bounds__(kBlock) void gen_pairs_kernel(int64_t nele, int nX, const int32_t* idx, int64_t nXdof,
-                                                           int64_t kbase, unsigned long long* keys, int32_t* vals) {
+                                                           int64_t kbase, int compact, unsigned long long* keys,
+                                                           uint32_t* vals) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t per = (int64_t)nX * nX;
     if (q >= nele * per) return;
@@ -239,16 +301,23 @@ __global__ __launch_bounds__(kBlock) void gen_pairs_kernel(int64_t nele, int nX,
     const unsigned long long J = (unsigned long long)idx[(int64_t)j * nele + e];
     const unsigned long long I = (unsigned long long)idx[(int64_t)i * nele + e];
     keys[q] = J * (unsigned long long)nXdof + I;
-    vals[q] = (int32_t)(kbase + (int64_t)(i + nX * j) * nele + e);
+    if (compact) {
+        double m;
+        const int slot = mh::compact_slot(i, j, &m);
+        const uint32_t code = m == 1. ? 0u : m == -1. ? 1u : m == 0.5 ? 2u : m == -0.5 ? 3u : m == 0.25 ? 4u : 5u;
+        vals[q] = (code << 29) | (uint32_t)(kbase + e * mh::kCompact + slot);
+    } else {
+        vals[q] = (uint32_t)(kbase + e * per + i + (int64_t)nX * j);
+    }
 }
 __global__ __launch_bounds__(kBlock) void gen_rpairs_kernel(int64_t nele, int nX, const int32_t* idx, int64_t rbase,
-                                                            unsigned int* keys, int32_t* vals) {
+                                                            unsigned int* keys, uint32_t* vals) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nele * nX) return;
     const int64_t e = q / nX;
     const int i = (int)(q - e * nX);
     keys[q] = (unsigned int)idx[(int64_t)i * nele + e];
-    vals[q] = (int32_t)(rbase + (int64_t)i * nele + e);
+    vals[q] = (uint32_t)(rbase + (int64_t)i * nele + e);
 }
 template <class K>
 __global__ __launch_bounds__(kBlock) void head_kernel(int64_t n, const K* keys, int32_t* head) {
@@ -282,11 +351,19 @@ __global__ __launch_bounds__(kBlock) void rseg_kernel(int64_t n, const unsigned 
     const int64_t k = p == n ? ndof : (int64_t)keys[p];
     for (int64_t cc = kprev + 1; cc <= k; ++cc) seg[cc] = (int32_t)p;
 }
-// asm[2] dump: for every contribution p of nz k, out[ksrc[p]] = k+1.
-__global__ __launch_bounds__(kBlock) void asm_dump_kernel(int64_t nnz, const int32_t* seg, const int32_t* src, int32_t* out) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nnz) return;
-    for (int32_t p = seg[k]; p < seg[k + 1]; ++p) out[src[p]] = (int32_t)(k + 1);
+// asm[2] dump: nz number (1-based) of every element entry, found in the CSC pattern.
+__global__ __launch_bounds__(kBlock) void asm_dump_kernel(int64_t nele, int nX, const int32_t* idx, const int32_t* colptr,
+                                                          const int32_t* rowval, int32_t* out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)nX * nX;
+    if (q >= nele * per) return;
+    const int64_t e = q / per;
+    const int r = (int)(q - e * per);
+    const int j = r / nX, i = r - j * nX;
+    const int32_t J = idx[(int64_t)j * nele + e], I = idx[(int64_t)i * nele + e];
+    int32_t lo = colptr[J], hi = colptr[J + 1];
+    while (lo < hi) { int32_t mid = lo + (hi - lo) / 2; if (rowval[mid] < I) lo = mid + 1; else hi = mid; }
+    out[q] = lo + 1;
 }
 
 inline unsigned grid(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -298,6 +375,17 @@ int check_launch(Ctx* c, const char* what) {
 }
 
 }  // namespace
+
+// Products of dof scales per compact slot (scale.X of node 1 and node 2 are equal per field).
+void beam_scales(const double* sX, BeamArgs& a) {
+    for (int i = 0; i < 12; ++i) a.sR[i] = sX[i];
+    for (int aa = 0; aa < 3; ++aa)
+        for (int bb = aa; bb < 3; ++bb) a.sA[mh::sym3(aa, bb)] = sX[aa] * sX[bb];
+    for (int aa = 0; aa < 3; ++aa)
+        for (int r = 0; r < 6; ++r) a.sT[6 * aa + r] = sX[aa] * sX[r < 3 ? 3 + r : 6 + r];
+    for (int r = 0; r < 6; ++r)
+        for (int q = r; q < 6; ++q) a.sS[mh::sym6(r, q)] = sX[r < 3 ? 3 + r : 6 + r] * sX[q < 3 ? 3 + q : 6 + q];
+}
 
 // ============================================================================ C ABI
 struct mh_ctx : Ctx {};
@@ -405,6 +493,9 @@ int mh_group_add(mh_ctx* c, int kind, int64_t nele, int nX, const int64_t* idxX,
     Group g;
     g.kind = kind; g.nele = nele; g.nX = nX; g.nU = beam ? nU : 0;
     for (int k = 0; k < 12; ++k) g.scaleX[k] = (scaleX && k < nX) ? scaleX[k] : 1.;
+    if (beam)
+        for (int k = 0; k < 6; ++k)
+            if (g.scaleX[k] != g.scaleX[k + 6]) { c->err = "beam dof scales must be equal on both nodes"; return MH_ERR_ARG; }
     int r;
     if ((r = upload_idx(c, &g.d_idx, idxX, nX, nele, c->nX, false))) return r;
     g.h_idx.assign(idxX, idxX + (size_t)nX * nele);
@@ -447,14 +538,19 @@ int mh_prepare_sweepx(mh_ctx* c, int64_t* nnz_out) {
     HIPCHK(c, hipSetDevice(c->device));
     free_prepared(c);
     // contribution counts and buffer offsets, in eletype order
-    int64_t nK = 0, nR = 0;
+    int64_t nK = 0, nP = 0, nR = 0;
     for (auto& g : c->groups) {
+        g.compact = g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U;
+        g.kper = g.compact ? mh::kCompact : (int64_t)g.nX * g.nX;
         g.kbase = nK; g.rbase = nR;
-        nK += g.nele * g.nX * g.nX;
+        nK += g.nele * g.kper;
+        nP += g.nele * g.nX * g.nX;
         nR += g.nele * g.nX;
     }
-    if (nK >= INT32_MAX || nR >= INT32_MAX) { c->err = "more than 2^31 contributions"; return MH_ERR_SIZE; }
-    c->nK = nK; c->nR = nR;
+    if (nP >= INT32_MAX || nK >= (1ll << 29) || nR >= (1ll << 29)) {
+        c->err = "more than 2^31 contributions or 2^29 element outputs"; return MH_ERR_SIZE;
+    }
+    c->nK = nK; c->nP = nP; c->nR = nR;
     const int64_t nX = c->nX;
     hipStream_t s = c->stream;
     HIPCHK(c, hipMalloc(&c->d_Kbuf, sizeof(double) * std::max<int64_t>(nK, 1)));
@@ -465,44 +561,49 @@ int mh_prepare_sweepx(mh_ctx* c, int64_t* nnz_out) {
     HIPCHK(c, hipMalloc(&c->d_rseg, sizeof(int32_t) * (nX + 1)));
     // ---- matrix pairs
     unsigned long long *kin = nullptr, *kout = nullptr;
-    int32_t *vin = nullptr, *head = nullptr, *id = nullptr, *colk = nullptr;
-    HIPCHK(c, hipMalloc(&kin, sizeof(unsigned long long) * std::max<int64_t>(nK, 1)));
-    HIPCHK(c, hipMalloc(&kout, sizeof(unsigned long long) * std::max<int64_t>(nK, 1)));
-    HIPCHK(c, hipMalloc(&vin, sizeof(int32_t) * std::max<int64_t>(nK, 1)));
-    HIPCHK(c, hipMalloc(&c->d_ksrc, sizeof(int32_t) * std::max<int64_t>(nK, 1)));
-    for (auto& g : c->groups) {
-        if (g.nX == 0 || g.nele == 0) continue;
-        int64_t np = g.nele * g.nX * g.nX;
-        hipLaunchKernelGGL(gen_pairs_kernel, dim3(grid(np)), dim3(kBlock), 0, s, g.nele, g.nX, g.d_idx, nX, g.kbase,
-                           kin + g.kbase, vin + g.kbase);
+    uint32_t* vin = nullptr;
+    int32_t *head = nullptr, *id = nullptr, *colk = nullptr;
+    HIPCHK(c, hipMalloc(&kin, sizeof(unsigned long long) * std::max<int64_t>(nP, 1)));
+    HIPCHK(c, hipMalloc(&kout, sizeof(unsigned long long) * std::max<int64_t>(nP, 1)));
+    HIPCHK(c, hipMalloc(&vin, sizeof(uint32_t) * std::max<int64_t>(nP, 1)));
+    HIPCHK(c, hipMalloc(&c->d_ksrc, sizeof(uint32_t) * std::max<int64_t>(nP, 1)));
+    {
+        int64_t pbase = 0;
+        for (auto& g : c->groups) {
+            int64_t np = g.nele * g.nX * g.nX;
+            if (np == 0) continue;
+            hipLaunchKernelGGL(gen_pairs_kernel, dim3(grid(np)), dim3(kBlock), 0, s, g.nele, g.nX, g.d_idx, nX, g.kbase,
+                               (int)g.compact, kin + pbase, vin + pbase);
+            pbase += np;
+        }
     }
     if (int r = check_launch(c, "gen_pairs")) return r;
     int end_bit = 1;
     while (end_bit < 64 && ((unsigned long long)nX * (unsigned long long)nX) > (1ull << end_bit)) ++end_bit;
     size_t tmp_bytes = 0;
     void* tmp = nullptr;
-    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, kin, kout, vin, c->d_ksrc, (int)nK, 0, end_bit, s));
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, kin, kout, vin, c->d_ksrc, (int)nP, 0, end_bit, s));
     HIPCHK(c, hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
-    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kin, kout, vin, c->d_ksrc, (int)nK, 0, end_bit, s));
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kin, kout, vin, c->d_ksrc, (int)nP, 0, end_bit, s));
     dfree(tmp);
-    HIPCHK(c, hipMalloc(&head, sizeof(int32_t) * std::max<int64_t>(nK, 1)));
-    HIPCHK(c, hipMalloc(&id, sizeof(int32_t) * std::max<int64_t>(nK, 1)));
-    hipLaunchKernelGGL(head_kernel<unsigned long long>, dim3(grid(nK)), dim3(kBlock), 0, s, nK, kout, head);
+    HIPCHK(c, hipMalloc(&head, sizeof(int32_t) * std::max<int64_t>(nP, 1)));
+    HIPCHK(c, hipMalloc(&id, sizeof(int32_t) * std::max<int64_t>(nP, 1)));
+    hipLaunchKernelGGL(head_kernel<unsigned long long>, dim3(grid(nP)), dim3(kBlock), 0, s, nP, kout, head);
     tmp_bytes = 0;
-    HIPCHK(c, hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, head, id, (int)nK, s));
+    HIPCHK(c, hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, head, id, (int)nP, s));
     HIPCHK(c, hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
-    HIPCHK(c, hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, head, id, (int)nK, s));
+    HIPCHK(c, hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, head, id, (int)nP, s));
     dfree(tmp);
     int32_t nnz = 0;
-    if (nK > 0) HIPCHK(c, hipMemcpyAsync(&nnz, id + nK - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (nP > 0) HIPCHK(c, hipMemcpyAsync(&nnz, id + nP - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     c->nnz = nnz;
     HIPCHK(c, hipMalloc(&c->d_rowval, sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
     HIPCHK(c, hipMalloc(&c->d_kseg, sizeof(int32_t) * (nnz + 1)));
     HIPCHK(c, hipMalloc(&colk, sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
     HIPCHK(c, hipMalloc(&c->d_nzval, sizeof(double) * std::max<int64_t>(nnz, 1)));
-    hipLaunchKernelGGL(seg_kernel, dim3(grid(nK)), dim3(kBlock), 0, s, nK, kout, head, id, nX, c->d_kseg, c->d_rowval, colk);
-    int32_t nk32 = (int32_t)nK;
+    hipLaunchKernelGGL(seg_kernel, dim3(grid(nP)), dim3(kBlock), 0, s, nP, kout, head, id, nX, c->d_kseg, c->d_rowval, colk);
+    int32_t nk32 = (int32_t)nP;
     HIPCHK(c, hipMemcpyAsync(c->d_kseg + nnz, &nk32, sizeof(int32_t), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(gapfill_kernel, dim3(grid(nnz + 1)), dim3(kBlock), 0, s, (int64_t)nnz, colk, nX, c->d_colptr);
     if (int r = check_launch(c, "pattern")) return r;
@@ -510,11 +611,11 @@ int mh_prepare_sweepx(mh_ctx* c, int64_t* nnz_out) {
     dfree(kin); dfree(kout); dfree(vin); dfree(head); dfree(id); dfree(colk);
     // ---- vector pairs
     unsigned int *rkin = nullptr, *rkout = nullptr;
-    int32_t* rvin = nullptr;
+    uint32_t* rvin = nullptr;
     HIPCHK(c, hipMalloc(&rkin, sizeof(unsigned int) * std::max<int64_t>(nR, 1)));
     HIPCHK(c, hipMalloc(&rkout, sizeof(unsigned int) * std::max<int64_t>(nR, 1)));
-    HIPCHK(c, hipMalloc(&rvin, sizeof(int32_t) * std::max<int64_t>(nR, 1)));
-    HIPCHK(c, hipMalloc(&c->d_rsrc, sizeof(int32_t) * std::max<int64_t>(nR, 1)));
+    HIPCHK(c, hipMalloc(&rvin, sizeof(uint32_t) * std::max<int64_t>(nR, 1)));
+    HIPCHK(c, hipMalloc(&c->d_rsrc, sizeof(uint32_t) * std::max<int64_t>(nR, 1)));
     for (auto& g : c->groups) {
         if (g.nX == 0 || g.nele == 0) continue;
         int64_t np = g.nele * g.nX;
@@ -554,17 +655,16 @@ int mh_get_asm(mh_ctx* c, int igroup, int64_t* asm1, int64_t* asm2) {
     const Group& g = c->groups[igroup];
     if (asm1) std::copy(g.h_idx.begin(), g.h_idx.end(), asm1);
     if (asm2 && g.nX > 0 && g.nele > 0) {
+        const int64_t per = (int64_t)g.nX * g.nX, n = g.nele * per;
         int32_t* d = nullptr;
-        HIPCHK(c, hipMalloc(&d, sizeof(int32_t) * std::max<int64_t>(c->nK, 1)));
-        hipLaunchKernelGGL(asm_dump_kernel, dim3(grid(c->nnz)), dim3(kBlock), 0, c->stream, c->nnz, c->d_kseg, c->d_ksrc, d);
-        int64_t n = g.nele * g.nX * g.nX;
+        HIPCHK(c, hipMalloc(&d, sizeof(int32_t) * n));
+        hipLaunchKernelGGL(asm_dump_kernel, dim3(grid(n)), dim3(kBlock), 0, c->stream, g.nele, g.nX, g.d_idx, c->d_colptr,
+                           c->d_rowval, d);
         std::vector<int32_t> h(n);
-        HIPCHK(c, hipMemcpyAsync(h.data(), d + g.kbase, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(h.data(), d, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         dfree(d);
-        const int64_t per = (int64_t)g.nX * g.nX;
-        for (int64_t e = 0; e < g.nele; ++e)
-            for (int64_t r = 0; r < per; ++r) asm2[e * per + r] = h[r * g.nele + e];
+        for (int64_t q = 0; q < n; ++q) asm2[q] = h[q];   // q = e*nX² + i + nX*j
     }
     return MH_OK;
 }
@@ -621,10 +721,13 @@ int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err*
             BeamArgs a;
             a.nele = g.nele; a.idx = g.d_idx; a.uidx = g.d_uidx; a.nodes = g.d_nodes; a.xyz = c->d_xyz; a.o2 = g.d_o2;
             a.sec = g.d_sec; a.X = c->d_X[0]; a.U = c->d_U;
-            for (int k = 0; k < 12; ++k) a.s[k] = g.scaleX[k];
+            beam_scales(g.scaleX, a);
             a.K = c->d_Kbuf + g.kbase; a.R = c->d_Rbuf + g.rbase; a.flag = c->d_flag; a.gtag = gtag;
-            if (g.kind == MH_KIND_BEAM_U) hipLaunchKernelGGL(beam_static_kernel<true>, dim3(grid(g.nele)), dim3(kBlock), 0, s, a);
-            else hipLaunchKernelGGL(beam_static_kernel<false>, dim3(grid(g.nele)), dim3(kBlock), 0, s, a);
+            const int64_t ntile = (g.nele + kBeamBlock - 1) / kBeamBlock;
+            a.per_xcd = (ntile + kXCD - 1) / kXCD;
+            const dim3 gr((unsigned)(a.per_xcd * kXCD));
+            if (g.kind == MH_KIND_BEAM_U) hipLaunchKernelGGL(beam_static_kernel<true>, gr, dim3(kBeamBlock), 0, s, a);
+            else hipLaunchKernelGGL(beam_static_kernel<false>, gr, dim3(kBeamBlock), 0, s, a);
         } else {
             PointArgs a;
             a.kind = g.kind; a.nele = g.nele; a.nX = g.nX; a.idx = g.d_idx; a.X = c->d_X[0]; a.vals = g.d_vals;
@@ -636,15 +739,20 @@ int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err*
     }
     if (int r = check_launch(c, "element kernels")) return r;
     HIPCHK(c, hipEventRecord(c->ev[1], s));
-    if (c->nnz)
-        hipLaunchKernelGGL(scatter_kernel, dim3(grid(c->nnz)), dim3(kBlock), 0, s, c->nnz, c->d_kseg, c->d_ksrc, c->d_Kbuf, c->d_nzval);
+    if (c->nnz) {
+        const int64_t per = ((c->nnz + kTile - 1) / kTile + kXCD - 1) / kXCD;
+        hipLaunchKernelGGL(scatter_kernel<false>, dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nnz, per, c->d_kseg,
+                           c->d_ksrc, c->d_Kbuf, (const double*)nullptr, c->d_nzval);
+    }
     HIPCHK(c, hipEventRecord(c->ev[2], s));
     if (c->nX) {
+        const int64_t per = ((c->nX + kTile - 1) / kTile + kXCD - 1) / kXCD;
         if (step)
-            hipLaunchKernelGGL(scatter_rhs_step_kernel, dim3(grid(c->nX)), dim3(kBlock), 0, s, c->nX, c->d_rseg, c->d_rsrc,
-                               c->d_Rbuf, c->d_dRbuf, c->d_rhs);
+            hipLaunchKernelGGL(scatter_kernel<true>, dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nX, per, c->d_rseg,
+                               c->d_rsrc, c->d_Rbuf, (const double*)c->d_dRbuf, c->d_rhs);
         else
-            hipLaunchKernelGGL(scatter_kernel, dim3(grid(c->nX)), dim3(kBlock), 0, s, c->nX, c->d_rseg, c->d_rsrc, c->d_Rbuf, c->d_rhs);
+            hipLaunchKernelGGL(scatter_kernel<false>, dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nX, per, c->d_rseg,
+                               c->d_rsrc, c->d_Rbuf, (const double*)nullptr, c->d_rhs);
     }
     HIPCHK(c, hipEventRecord(c->ev[3], s));
     if (int r = check_launch(c, "scatter kernels")) return r;
@@ -688,13 +796,20 @@ int mh_get_element_blocks(mh_ctx* c, int igroup, double* R, double* K) {
     const Group& g = c->groups[igroup];
     const int64_t n = g.nele, nX = g.nX;
     if (n == 0 || nX == 0) return MH_OK;
-    std::vector<double> hk(n * nX * nX), hr(n * nX);
+    std::vector<double> hk(n * g.kper), hr(n * nX);
     HIPCHK(c, hipMemcpyAsync(hk.data(), c->d_Kbuf + g.kbase, sizeof(double) * hk.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(hr.data(), c->d_Rbuf + g.rbase, sizeof(double) * hr.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int64_t e = 0; e < n; ++e) {
         if (R) for (int64_t i = 0; i < nX; ++i) R[e * nX + i] = hr[i * n + e];
-        if (K) for (int64_t r = 0; r < nX * nX; ++r) K[e * nX * nX + r] = hk[r * n + e];
+        if (K)
+            for (int64_t j = 0; j < nX; ++j)
+                for (int64_t i = 0; i < nX; ++i) {
+                    double v;
+                    if (g.compact) { double m; int slot = mh::compact_slot((int)i, (int)j, &m); v = m * hk[e * g.kper + slot]; }
+                    else v = hk[e * g.kper + i + nX * j];
+                    K[e * nX * nX + i + nX * j] = v;
+                }
     }
     return MH_OK;
 }
@@ -707,6 +822,25 @@ int mh_last_timings(mh_ctx* c, double* ms) {
         HIPCHK(c, hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]));
         ms[i] = t;
     }
+    return MH_OK;
+}
+
+int mh_calib_copy(mh_ctx* c, int64_t n, double* ms) {
+    if (!c || n <= 0) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    double *a = nullptr, *b = nullptr;
+    HIPCHK(c, hipMalloc(&a, sizeof(double) * n));
+    HIPCHK(c, hipMalloc(&b, sizeof(double) * n));
+    HIPCHK(c, hipMemsetAsync(a, 0, sizeof(double) * n, c->stream));
+    hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(kBlock), 0, c->stream, n, a, b);   // warm-up
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(kBlock), 0, c->stream, n, a, b);
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float t = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
+    if (ms) *ms = t;
+    dfree(a); dfree(b);
     return MH_OK;
 }
 

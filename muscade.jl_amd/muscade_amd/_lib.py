@@ -46,6 +46,7 @@ SIGNATURES = {
     "mh_device_outputs": (_I, [_P, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "mh_get_element_blocks": (_I, [_P, _I, _D, _D]),
     "mh_last_timings": (_I, [_P, _D]),
+    "mh_calib_copy": (_I, [_P, _L, _D]),
     "mh_sync": (_I, [_P]),
 }
 
