@@ -28,7 +28,8 @@ FP64_PEAK = 78.6e12        # MI355X FP64 vector, FLOP/s
 # FP64 operations per beam of the static kernel (add/sub/mul/div/sqrt = 1, fma = 2),
 # from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters of one assembly of this
 # workload divided by the number of beams (profiles/beam_static_pmc.md).
-F_ELEM = None
+F_ELEM = 6299
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "space_frame_100x100x33_traffic.json")
 
 
 def log(*a):
@@ -91,7 +92,7 @@ def main():
     ap.add_argument("--nx", type=int, default=100)
     ap.add_argument("--ny", type=int, default=100)
     ap.add_argument("--nz", type=int, default=33)
-    ap.add_argument("--cpu-sample", type=int, default=16000)
+    ap.add_argument("--cpu-sample", type=int, default=40000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="run only the counter-calibration copy kernel")
     args = ap.parse_args()
@@ -154,9 +155,12 @@ def main():
                    "unit": "TFLOP/s"}
     else:
         rl_eval = rl_eval_hbm
-    for r_ in (rl_eval, rl_scatter, rl_eval_hbm):
+    traffic = {}
+    if (args.nx, args.ny, args.nz) == (100, 100, 33) and os.path.exists(TRAFFIC_FILE):
+        traffic = json.load(open(TRAFFIC_FILE))["bytes_per_launch"]
+    for r_, key in ((rl_eval, "eval"), (rl_scatter, "scatter"), (rl_eval_hbm, "eval")):
         r_["frac"] = r_["achieved"] / r_["peak"]
-        r_["traffic"] = None
+        r_["traffic"] = traffic.get(key)
     dominant = ("eval", rl_eval) if t_eval >= t_sk else ("scatter", rl_scatter)
     log(f"per step: eval {t_eval * 1e3:.3f} ms, nz scatter {t_sk * 1e3:.3f} ms, rhs scatter {t_sr * 1e3:.3f} ms, "
         f"wall {ms_step:.3f} ms")
