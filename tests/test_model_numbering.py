@@ -1,0 +1,179 @@
+"""Host model description: dof numbering and Disassembler, pinned to the reference's goldens
+(test/TestModelDescription.jl, test/TestAssemble.jl) and to the oracle's sequential restatement
+of addelement! (oracle.cpp orc_number_dofs, ModelDescription.jl:195-263)."""
+import numpy as np
+import pytest
+
+from tests import _oracle as orc
+from tests.helpers import model_api
+
+mm, el = model_api()
+
+
+class Turbine:
+    """Fixture element of test/SomeElements.jl:15-50 (doflist only)."""
+    kind = None
+
+    @classmethod
+    def construct(cls, model, nodes0, **kw):
+        return ("Turbine",), ((1, 1, 2, 2), ("X", "X", "A", "A"), ("tx1", "tx2", "Δseadrag", "Δskydrag")), kw
+
+
+class AnchorLine:
+    """Fixture element of test/SomeElements.jl:54-75 (doflist only)."""
+    kind = None
+
+    @classmethod
+    def construct(cls, model, nodes0, **kw):
+        return ("AnchorLine",), ((1, 1, 1, 2, 2), ("X", "X", "X", "A", "A"), ("tx1", "tx2", "rx3", "ΔL", "Δbuoyancy")), kw
+
+
+def test_model_description_golden():   # TestModelDescription.jl:9-96
+    model = mm.Model("TestModel")
+    n1 = mm.addnode(model, [0, 0, -10])
+    n2 = mm.addnode(model, [0., 0., 0.])
+    n3 = mm.addnode(model, [0., 0., 0.])
+    assert (n1, n2, n3) == (1, 2, 3)
+    e1 = mm.addelement(model, Turbine, [n1, n2])
+    e2 = mm.addelement(model, AnchorLine, [n1, n3])
+    assert e1 == (1, 1) and e2 == (2, 1)
+    E1, E2 = model.eletypes
+    # model.ele[1][1].dofID == [X1, X2, A1, A2]; model.ele[2][1].dofID == [X1, X2, X3, A3, A4]
+    assert E1.cat_dofid().tolist() == [[1, 2, 1, 2]]
+    assert E2.cat_dofid().tolist() == [[1, 2, 3, 3, 4]]
+    assert model.dofnode("X").tolist() == [0, 0, 0] and model.dofnode("A").tolist() == [1, 1, 2, 2]
+    # doftyp order and the dof types of X3 and A3 (idoftyp 5 and 6)
+    assert [tuple(d[:2]) for d in model.doftyp] == [("X", "tx1"), ("X", "tx2"), ("A", "Δseadrag"), ("A", "Δskydrag"),
+                                                    ("X", "rx3"), ("A", "ΔL"), ("A", "Δbuoyancy")]
+    assert model.doftypof("X")[2] == 4 and model.doftypof("A")[2] == 5
+    mm.setscale(model, scale={"X": {"tx1": 1., "tx2": 1., "rx3": 2.}, "A": {"Δseadrag": 3., "Δskydrag": 4., "ΔL": 5}},
+                Λscale=2)
+    assert model.doftyp[0][2] == 1. and model.doftyp[2][2] == 3. and model.doftyp[4][2] == 2.
+    assert model.scaleΛ == 2.
+    assert model.getndof("X") == 3 and model.getnele() == 2
+
+
+def test_disassembler_golden():   # TestAssemble.jl:43-67
+    model = mm.Model("TestModel")
+    n1 = mm.addnode(model, [0, 0, 100])
+    n2 = mm.addnode(model, [0., 0, 0])
+    n3 = mm.addnode(model, [0., 0, 0])
+    mm.addelement(model, Turbine, [n1, n2])
+    mm.addelement(model, AnchorLine, [n1, n3])
+    dis = mm.Disassembler(model)
+    assert dis.dis[0].X.tolist() == [[1, 2]] and dis.dis[0].U.shape == (1, 0) and dis.dis[0].A.tolist() == [[1, 2]]
+    assert dis.dis[1].X.tolist() == [[1, 2, 3]] and dis.dis[1].A.tolist() == [[3, 4]]
+    assert np.allclose(dis.scaleX, [1, 1, 1]) and np.allclose(dis.scaleΛ, [1, 1, 1]) and np.allclose(dis.scaleA, [1] * 4)
+    assert dis.fieldX == ["tx1", "tx2", "rx3"] and dis.fieldA == ["Δseadrag", "Δskydrag", "ΔL", "Δbuoyancy"]
+
+
+def oracle_numbering(calls, doflists, nnod):
+    """Run orc_number_dofs on a flat list of (type key, nodes) single-element calls."""
+    keys = sorted(doflists)
+    kidx = {k: i for i, k in enumerate(keys)}
+    fields = {}
+    dl_off, dl_inod, dl_class, dl_field = [0], [], [], []
+    for k in keys:
+        inod, cls, field = doflists[k]
+        for a, c, f in zip(inod, cls, field):
+            dl_inod.append(a)
+            dl_class.append("XUA".index(c))
+            dl_field.append(fields.setdefault(f, len(fields)))
+        dl_off.append(len(dl_inod))
+    tkey = np.array([kidx[k] for k, _ in calls], np.intc)
+    noff = np.cumsum([0] + [len(n) for _, n in calls]).astype(np.intc)
+    nodes = np.concatenate([np.asarray(n) for _, n in calls]).astype(np.intc)
+    ndoftot = sum(len(doflists[k][0]) for k, _ in calls)
+    ie, il, did = np.zeros(len(calls), np.intc), np.zeros(len(calls), np.intc), np.zeros(ndoftot, np.intc)
+    nd = np.zeros(3, np.int64)
+    cap = ndoftot + 1
+    dn = [np.zeros(cap, np.intc) for _ in range(3)]
+    df = [np.zeros(cap, np.intc) for _ in range(3)]
+    a = lambda x: np.ascontiguousarray(x, np.intc)   # noqa: E731
+    r = orc.lib.orc_number_dofs(len(calls), orc.ip(tkey), orc.ip(noff), orc.ip(nodes), len(keys), orc.ip(a(dl_off)),
+                                orc.ip(a(dl_inod)), orc.ip(a(dl_class)), orc.ip(a(dl_field)), orc.ip(ie), orc.ip(il),
+                                orc.ip(did), orc.ip64(nd), cap, orc.ip(dn[0]), orc.ip(df[0]), orc.ip(dn[1]),
+                                orc.ip(df[1]), orc.ip(dn[2]), orc.ip(df[2]))
+    assert r == 0
+    return ie, il, did, nd, [dn[c][:nd[c]] for c in range(3)]
+
+
+class Generic:
+    """An element whose doflist comes from a class attribute (for randomized numbering tests)."""
+    kind = None
+    doflists = {}
+
+    @classmethod
+    def construct(cls, model, nodes0, key):
+        return key, cls.doflists[key], {}
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_numbering_matches_oracle_random(seed):
+    """Vectorized first-encounter numbering == the oracle's sequential addelement! on random models
+    (mixed element types, shared nodes, repeated fields, X/U/A classes, batched and single calls)."""
+    rng = np.random.default_rng(seed)
+    fields = ["t1", "t2", "r1", "λ", "u", "a1"]
+    doflists = {}
+    for k in range(4):
+        nn = int(rng.integers(1, 4))
+        nd = int(rng.integers(1, 7))
+        inod = tuple(int(x) for x in rng.integers(1, nn + 1, nd))
+        inod = tuple(sorted(set(inod) | {nn})) if len(set(inod)) < nn else inod
+        cls = tuple(str(c) for c in rng.choice(list("XXXUA"), len(inod)))
+        fld = tuple(str(f) for f in rng.choice(fields, len(inod)))
+        # a (node, class, field) may not repeat inside one element
+        seen, keep = set(), []
+        for q, (a, c, f) in enumerate(zip(inod, cls, fld)):
+            if (a, c, f) not in seen:
+                seen.add((a, c, f)); keep.append(q)
+        inod = tuple(inod[q] for q in keep); cls = tuple(cls[q] for q in keep); fld = tuple(fld[q] for q in keep)
+        if max(inod) != max(inod):
+            continue
+        doflists[("G", k)] = (inod, cls, fld)
+    Generic.doflists = doflists
+    nnod = 30
+    model = mm.Model()
+    mm.addnode(model, rng.normal(size=(nnod, 3)))
+    calls = []
+    for _ in range(12):
+        key = list(doflists)[int(rng.integers(len(doflists)))]
+        nn = max(doflists[key][0])
+        nb = int(rng.integers(1, 6))
+        nodes = rng.integers(1, nnod + 1, size=(nb, nn))
+        mm.addelement(model, Generic, nodes, key=key)
+        calls += [(key, list(n)) for n in nodes]
+    ie, il, did, nd, dn = oracle_numbering(calls, doflists, nnod)
+    # element type numbering and dof ids, element by element in addition order
+    order = {E.key: i + 1 for i, E in enumerate(model.eletypes)}
+    pos = {k: 0 for k in order}
+    q = 0
+    for i, (key, nodes) in enumerate(calls):
+        E = model.eletypes[order[key] - 1]
+        assert ie[i] == order[key]
+        assert il[i] == pos[key] + 1
+        ids = E.cat_dofid()[pos[key]]
+        assert list(ids) == list(did[q:q + len(ids)])
+        q += len(ids)
+        pos[key] += 1
+    for c, cls in enumerate("XUA"):
+        assert model.ndof[cls] == nd[c]
+        assert np.array_equal(model.dofnode(cls) + 1, dn[c])   # oracle nodes are 1-based
+
+
+def test_space_frame_numbering_matches_oracle():
+    from muscade_amd.workloads import space_frame
+    model, state = space_frame(4, 3, 3, seed=3)
+    calls, doflists = [], {}
+    for E in model.eletypes:
+        doflists[E.key] = (E.inod, E.dofclass, E.field)
+    for E in model.eletypes:
+        pass
+    # reconstruct the addition order: beams (x, y, z members), then Holds field by field
+    for E in model.eletypes:
+        for nodes in E.nodes:
+            calls += [(E.key, list(n + 1)) for n in nodes]
+    ie, il, did, nd, dn = oracle_numbering(calls, doflists, model.nnod)
+    got = np.concatenate([np.concatenate(E.dofid).ravel() for E in model.eletypes])
+    assert np.array_equal(got, did)
+    assert model.ndof["X"] == nd[0] == 6 * 36 + 6 * 12
