@@ -77,11 +77,13 @@ int mh_model_begin(mh_ctx* ctx, int64_t nXdof, int64_t nUdof, int64_t nAdof, int
 int mh_group_add(mh_ctx* ctx, int kind, int64_t nele, int nX, const int64_t* idxX, const double* scaleX, int nU,
                  const int64_t* idxU, const int64_t* nodes, const double* orient2, const double* mat, int* igroup);
 
-/* ---- prepare(AssemblySweepX) (src/SweepX.jl:24-33) ----------------------
+/* ---- prepare(AssemblySweepX{OX}) (src/SweepX.jl:24-33) ------------------
  * Builds on the device the CSC pattern of Lλx and the sorted
  * contribution→nzval maps that replace asm[1,:] / asm[2,:]
- * (asmvec!/asmmat!, src/Assemble.jl:340-448). Returns nnz. */
-int mh_prepare_sweepx(mh_ctx* ctx, int64_t* nnz);
+ * (asmvec!/asmmat!, src/Assemble.jl:340-448). OX (0, 1, 2) is the order of
+ * the assembler, as in the type AssemblySweepX{OX}; mh_assemble_sweepx must be
+ * called with the same OX. Returns nnz. */
+int mh_prepare_sweepx(mh_ctx* ctx, int OX, int64_t* nnz);
 /* The pattern exactly as Muscade's SparseMatrixCSC: colptr (nX+1), rowval (nnz), 1-based. */
 int mh_get_pattern(mh_ctx* ctx, int64_t* colptr, int64_t* rowval);
 /* asm[1,ieletyp] (nX×nele) and asm[2,ieletyp] (nX²×nele, entry i+nX·(j-1)), 1-based. */

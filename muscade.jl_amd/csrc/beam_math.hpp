@@ -265,8 +265,15 @@ MH_HD int compact_slot(int i, int j, double* mult) {
 struct BeamSection { double EA, EI2, EI3, GJ, w; };
 struct BeamOut { double R[12]; double Kc[kCompact]; int nan; };
 
-// Static residual and compact tangent; fe = (0,0,w) - U is the external load per length.
-MH_HD void beam_static(const BeamGeo& G, const BeamSection& S, v3 fe, const double* X, BeamOut& out) {
+// Residual and compact tangent of Π = Π_int + Σ_gp e_gp·x_gp + m·vₛₘ with the loads
+// e_gp and the couple m held fixed.
+//   GEN = false: e_gp = dL_gp·fe with fe = (0,0,w) - U, m = 0 (the static element);
+//   GEN = true:  e_gp = eg[gp] (already × dL_gp) and m = mtot = Σ_gp dL_gp·mₑ, the
+//                frozen external loads of the dynamic element (inertia, added mass,
+//                damping, roll inertia). vₛₘ = Rodrigues⁻¹(rₛₘ) (BeamElement.jl:203).
+template <bool GEN = false>
+MH_HD void beam_static(const BeamGeo& G, const BeamSection& S, v3 fe, const double* X, BeamOut& out,
+                       const v3* eg = nullptr, v3 mtot = v3{0., 0., 0.}) {
     const double L = G.L;
     v3 u1 = mk(X[0], X[1], X[2]), v1 = mk(X[3], X[4], X[5]);
     v3 u2 = mk(X[6], X[7], X[8]), v2 = mk(X[9], X[10], X[11]);
@@ -340,7 +347,7 @@ MH_HD void beam_static(const BeamGeo& G, const BeamSection& S, v3 fe, const doub
             double ya = 2. * z, yu = -4. * (z * z * z) + 3. * z, yv = (z * z - 1. / 4.) * L;
             v3 y = mk(ya * ul.x, yu * ul.y + yv * vl.z, yu * ul.z - yv * vl.y);
             v3 s = mul(rsm, mk(L * z + y.x, y.y, y.z));
-            v3 e = G.dL[g] * fe;
+            v3 e = GEN ? eg[g] : G.dL[g] * fe;
             esum = esum + e;
             gphi_e = gphi_e + cross(s, e);
             double es = dot(e, s);
@@ -358,6 +365,26 @@ MH_HD void beam_static(const BeamGeo& G, const BeamSection& S, v3 fe, const doub
     v3 Fu = mul(rsm, mk(fq[0], fq[1], fq[2]));
     v3 Fv = mul(rsm, mk(fq[3], fq[4], fq[5]));
     v3 gphi = cross(Fu, d) + cross(Fv, dv) + gphi_e;
+    // couple term m·Log(rₛₘ): gradient T(ψ)⁻ᵀm w.r.t. the spatial spin of rₛₘ, and the
+    // second-order part -Jᵀ B J (J = T(ψ)⁻¹, B = dT_form(ψ, T⁻ᵀm)) of its Hessian
+    s3 Hm; Hm.xx = Hm.yy = Hm.zz = Hm.xy = Hm.xz = Hm.yz = 0.;
+    if (GEN) {
+        v3 psi = mk(0, 0, 0);
+        {
+            double sc = scac((at(rsm, 0, 0) + at(rsm, 1, 1) + at(rsm, 2, 2) - 1.) / 2.);
+            psi = mk((at(rsm, 2, 1) - at(rsm, 1, 2)) / 2. / sc, (at(rsm, 0, 2) - at(rsm, 2, 0)) / 2. / sc,
+                     (at(rsm, 1, 0) - at(rsm, 0, 1)) / 2. / sc);
+        }
+        ExpCoef kpsi = exp_coef(norm3(psi));
+        m3 Jp = tangent_inv(psi, kpsi);
+        v3 mu_m = mulT(Jp, mtot);
+        gphi = gphi + mu_m;
+        s3 B = dT_form(psi, mu_m, kpsi);
+        v3 J0 = col(Jp, 0), J1 = col(Jp, 1), J2 = col(Jp, 2);
+        v3 BJ0 = smul(B, J0), BJ1 = smul(B, J1), BJ2 = smul(B, J2);
+        Hm.xx = -dot(J0, BJ0); Hm.yy = -dot(J1, BJ1); Hm.zz = -dot(J2, BJ2);
+        Hm.xy = -dot(J0, BJ1); Hm.xz = -dot(J0, BJ2); Hm.yz = -dot(J1, BJ2);
+    }
     v3 gD = Fv;
     // translational residual R = Aᵀ g_z on u₁, u₂
     out.R[0] = -0.5 * Fu.x + 0.5 * esum.x; out.R[1] = -0.5 * Fu.y + 0.5 * esum.y; out.R[2] = -0.5 * Fu.z + 0.5 * esum.z;
@@ -432,6 +459,10 @@ MH_HD void beam_static(const BeamGeo& G, const BeamSection& S, v3 fe, const doub
         Hpp.xy = 0.5 * (Fu.x * d.y + d.x * Fu.y) + 0.5 * (Fv.x * dv.y + dv.x * Fv.y) + Hee.xy;
         Hpp.xz = 0.5 * (Fu.x * d.z + d.x * Fu.z) + 0.5 * (Fv.x * dv.z + dv.x * Fv.z) + Hee.xz;
         Hpp.yz = 0.5 * (Fu.y * d.z + d.y * Fu.z) + 0.5 * (Fv.y * dv.z + dv.y * Fv.z) + Hee.yz;
+        if (GEN) {
+            Hpp.xx += Hm.xx; Hpp.yy += Hm.yy; Hpp.zz += Hm.zz;
+            Hpp.xy += Hm.xy; Hpp.xz += Hm.xz; Hpp.yz += Hm.yz;
+        }
         double* Sm = out.Kc + 24;
         v3 Bu[6], Bv[6], Np[6];
         for (int j = 0; j < 6; ++j) {
@@ -460,6 +491,10 @@ MH_HD void beam_static(const BeamGeo& G, const BeamSection& S, v3 fe, const doub
         }
     }
     int nan = 0;
+    // Muscade's residual always contracts mₑ with ∂vₛₘ/∂X (BeamElement.jl:170), with
+    // vₛₘ = Rodrigues⁻¹(rₛₘ) = spin⁻¹(rₛₘ)/scac((tr rₛₘ - 1)/2): for a corotated frame at
+    // a half turn, scac and acos′ there make it NaN even where mₑ = 0.
+    if (!GEN) nan |= !((at(rsm, 0, 0) + at(rsm, 1, 1) + at(rsm, 2, 2) - 1.) / 2. > -1.);
     for (int i = 0; i < 12; ++i) nan |= (out.R[i] != out.R[i]);
     for (int i = 0; i < kCompact; ++i) nan |= (out.Kc[i] != out.Kc[i]);
     out.nan = nan;

@@ -4,11 +4,13 @@
 //   node coordinates   xyz[3*n + c]                      (node-major: one 24-byte gather per node)
 //   element dof maps   idx[k*nele + e]   int32, 0-based  (SoA: a wave loads 64 consecutive ints)
 //   beam data          o2[c*nele + e], sec[f*nele + e]   (SoA)
-//   element outputs    beams: Kbuf[kbase + 45*e + slot], the 45 numbers of the compact
-//                      static tangent (beam_math.hpp), element-major; other kinds:
-//                      Kbuf[kbase + nX²*e + i + nX*j]; Rbuf[rbase + i*nele + e] (SoA)
-//   assembly maps      ksrc[p] = (code << 29) | Kbuf index of the p-th contribution in
-//                      (column,row) order, code = exact multiplier ±1, ±½, ±¼;
+//   element outputs    static beams (OX = 0): Kbuf[kbase + 45*e + slot], the 45 numbers of
+//                      the compact static tangent (beam_math.hpp), element-major; dynamic
+//                      beams (OX > 0) and other kinds: Kbuf[kbase + nX²*e + i + nX*j];
+//                      Rbuf[rbase + i*nele + e] (SoA), dRbuf likewise (∂R/∂δr of :step)
+//   assembly maps      ksrc[p] = Kbuf index of the p-th contribution in (column,row)
+//                      order; with compact groups (OX = 0) it is (code << 29) | index,
+//                      code = exact multiplier ±1, ±½, ±¼;
 //                      kseg[k] = first contribution of nz k; rsrc/rseg likewise for the rhs.
 //
 // Assembly is a deterministic segmented gather-reduce: nzval[k] = Σ mult·Kbuf[idx],
@@ -28,6 +30,7 @@
 #include <vector>
 #include "../../include/muscade_hip.h"
 #include "beam_math.hpp"
+#include "beam_dyn.hpp"
 
 namespace {
 
@@ -44,7 +47,8 @@ struct Group {
     double* d_vals = nullptr;    // nele (DofLoad values)
     std::vector<int64_t> h_idx;  // nX × nele, 1-based (for mh_get_asm)
     int64_t kbase = 0, rbase = 0;
-    int64_t kper = 0;            // Kbuf entries per element (45 for beams, nX² otherwise)
+    int64_t kper = 0;            // Kbuf entries per element (45 for static beams, nX² otherwise)
+    int64_t kcbase = 0;          // dynamic beams: offset of the compact static part in Kc
     bool compact = false;
 };
 
@@ -60,7 +64,10 @@ struct Ctx {
     double* d_A = nullptr;
     // prepared
     bool prepared = false;
+    int OX = 0;                  // order of the AssemblySweepX{OX} the maps were prepared for
+    bool coded = false;          // ksrc carries multiplier codes (compact groups present)
     int64_t nK = 0, nP = 0, nR = 0, nnz = 0;
+    double* d_Kc = nullptr;      // dynamic beams: scaled compact static part, 45 per element
     double* d_Kbuf = nullptr;
     double* d_Rbuf = nullptr;
     double* d_dRbuf = nullptr;
@@ -89,7 +96,7 @@ struct Ctx {
 template <class T> void dfree(T*& p) { if (p) { (void)hipFree(p); p = nullptr; } }
 
 void free_prepared(Ctx* c) {
-    dfree(c->d_Kbuf); dfree(c->d_Rbuf); dfree(c->d_dRbuf); dfree(c->d_nzval); dfree(c->d_rhs);
+    dfree(c->d_Kbuf); dfree(c->d_Rbuf); dfree(c->d_dRbuf); dfree(c->d_nzval); dfree(c->d_rhs); dfree(c->d_Kc);
     dfree(c->d_colptr); dfree(c->d_rowval); dfree(c->d_kseg); dfree(c->d_ksrc); dfree(c->d_rseg); dfree(c->d_rsrc);
     c->prepared = false;
 }
@@ -223,6 +230,148 @@ __global__ __launch_bounds__(kBlock) void point_kernel(PointArgs a) {
         for (int i = 0; i < nX; ++i) a.dR[i * n + e] = 0.;
 }
 
+// ---- dynamic beams (OX = 1, 2) ------------------------------------------------
+struct BeamDynArgs {
+    int64_t nele, per_xcd;
+    const int32_t* idx;
+    const int32_t* uidx;
+    const int32_t* nodes;
+    const double* xyz;
+    const double* o2;
+    const double* sec;
+    const double* X[3];
+    const double* U;
+    double scale[12];
+    double sA[6], sT[18], sS[21], sR[12];
+    double c[6];        // Newmark (a₁,a₂,a₃,b₁,b₂,b₃)
+    double* Kc;         // value pass → direction pass: scaled compact static part, 45 per element
+    double* K;          // full 12×12 per element
+    double* R;
+    double* dR;         // ∂R/∂δr (:step)
+    unsigned long long* flag;
+    unsigned long long gtag;
+};
+
+template <int ND, bool UDOF>
+__device__ __forceinline__ void dyn_load(const BeamDynArgs& a, int64_t e, mh::BeamGeo& G, mh::BeamSection& S,
+                                         mh::BeamDynSection& D, double (*X)[12], double* U) {
+    const int64_t n = a.nele;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        const int32_t q = a.idx[k * n + e];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) X[d][k] = a.X[d][q];
+    }
+    const int32_t n1 = a.nodes[e], n2 = a.nodes[n + e];
+    mh::v3 c1 = mh::mk(a.xyz[3 * (int64_t)n1], a.xyz[3 * (int64_t)n1 + 1], a.xyz[3 * (int64_t)n1 + 2]);
+    mh::v3 c2 = mh::mk(a.xyz[3 * (int64_t)n2], a.xyz[3 * (int64_t)n2 + 1], a.xyz[3 * (int64_t)n2 + 2]);
+    G = mh::beam_geometry(c1, c2, mh::mk(a.o2[e], a.o2[n + e], a.o2[2 * n + e]));
+    S.EA = a.sec[e]; S.EI2 = a.sec[n + e]; S.EI3 = a.sec[2 * n + e]; S.GJ = a.sec[3 * n + e]; S.w = a.sec[6 * n + e];
+    D.mu = a.sec[4 * n + e]; D.iota1 = a.sec[5 * n + e]; D.w = S.w;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        D.Ca[q] = a.sec[(7 + 3 * q) * n + e]; D.Cl[q] = a.sec[(8 + 3 * q) * n + e]; D.Cq[q] = a.sec[(9 + 3 * q) * n + e];
+        U[q] = UDOF ? a.U[a.uidx[q * n + e]] : 0.;
+    }
+}
+
+// Value pass, one element per lane: the frozen loads of the time jets, then the
+// residual and the compact tangent of Π with those loads held fixed.
+template <int ND, bool UDOF>
+__global__ __launch_bounds__(kBeamBlock) void beam_dyn_value_kernel(BeamDynArgs a) {
+    __shared__ double tile[kBeamBlock * mh::kCompact];
+    const int64_t e0 = xcd_tile(a.per_xcd) * kBeamBlock;
+    if (e0 >= a.nele) return;   // whole workgroup leaves together
+    const int tid = threadIdx.x;
+    const int64_t e = e0 + tid;
+    const int64_t n = a.nele;
+    if (e < n) {
+        mh::BeamGeo G;
+        mh::BeamSection S;
+        mh::BeamDynSection D;
+        double X[ND][12], U[3];
+        dyn_load<ND, UDOF>(a, e, G, S, D, X, U);
+        mh::v3 eg[4], mtot;
+        mh::dyn_frozen_loads<ND>(G, D, X, U, eg, mtot);
+        mh::BeamOut out;
+        mh::beam_static<true>(G, S, mh::mk(0., 0., 0.), X[0], out, eg, mtot);
+#pragma unroll
+        for (int i = 0; i < 12; ++i) a.R[i * n + e] = out.R[i] * a.sR[i];
+        double* t = tile + tid * mh::kCompact;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) t[i] = out.Kc[i] * a.sA[i];
+#pragma unroll
+        for (int i = 0; i < 18; ++i) t[6 + i] = out.Kc[6 + i] * a.sT[i];
+#pragma unroll
+        for (int i = 0; i < 21; ++i) t[24 + i] = out.Kc[24 + i] * a.sS[i];
+        if (out.nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+    }
+    __syncthreads();
+    const int64_t nv = (n - e0 < kBeamBlock ? n - e0 : kBeamBlock) * mh::kCompact;
+    double* dst = a.Kc + e0 * mh::kCompact;
+    for (int64_t f = tid; f < nv; f += kBeamBlock) dst[f] = tile[f];
+}
+
+// Direction pass: 16 lanes per element, lane k = trial direction k (δX of dof k for
+// k < 12, δr for k = 12 on :step). Each lane propagates the time jets with its own ε
+// seed; the lanes of an element then exchange the Jacobian columns of x_gp and vₛₘ
+// through LDS, and lane k writes column k of K = (compact static part) +
+// Σ_gp dL·(J_xᵀ ∂fₑ/∂ε + J_vᵀ ∂mₑ/∂ε).
+constexpr int kDirLanes = 16;
+constexpr int kDirElems = 16;
+constexpr int kDirBlock = kDirLanes * kDirElems;
+template <int ND, bool UDOF, bool STEP>
+__global__ __launch_bounds__(kDirBlock) void beam_dyn_dir_kernel(BeamDynArgs a) {
+    __shared__ double Jsh[kDirElems][12][15];
+    const int64_t e0 = xcd_tile(a.per_xcd) * kDirElems;
+    if (e0 >= a.nele) return;   // whole workgroup leaves together
+    const int el = threadIdx.x / kDirLanes, k = threadIdx.x % kDirLanes;
+    const int64_t e = e0 + el;
+    const int64_t n = a.nele;
+    const int dir = k < 12 ? k : ((STEP && k == 12) ? 12 : -1);
+    const bool live = e < n && dir >= 0;
+    mh::BeamGeo G;
+    mh::DynDir d;
+    if (live) {
+        mh::BeamSection S;
+        mh::BeamDynSection D;
+        double X[ND][12], U[3];
+        dyn_load<ND, UDOF>(a, e, G, S, D, X, U);
+        mh::dyn_direction<ND>(G, D, X, U, a.scale, a.c, dir, d);
+        if (k < 12) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) Jsh[el][k][3 * g + q] = d.Jx[g][q];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) Jsh[el][k][12 + q] = d.Jv[q];
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    double col[12];
+    int nan = 0;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+        col[i] = mh::dyn_row(G, (const double(*)[3])(&Jsh[el][i][0]), &Jsh[el][i][12], d);
+        if (k < 12) {
+            double m;
+            const int slot = mh::compact_slot(i, k, &m);
+            col[i] = m * a.Kc[e * mh::kCompact + slot] + col[i];
+        }
+        nan |= col[i] != col[i];
+    }
+    if (k < 12) {
+        double* dst = a.K + e * 144 + 12 * k;
+#pragma unroll
+        for (int i = 0; i < 12; ++i) dst[i] = col[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 12; ++i) a.dR[i * n + e] = col[i];
+    }
+    if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+}
+
 // Contribution source: (code << 29) | index; the code selects an exact multiplier.
 constexpr uint32_t kIdxMask = (1u << 29) - 1;
 __device__ __forceinline__ double contrib(const double* __restrict__ buf, uint32_t s) {
@@ -246,7 +395,9 @@ __device__ __forceinline__ double contrib(const double* __restrict__ buf, uint32
 // (add_value! then add_∂!{1,:minus}, src/SweepX.jl:56-57).
 constexpr int kTile = 256;
 constexpr int kChunk = 1024;
-template <bool STEP>
+// CODED: src carries multiplier codes (compact static beams); otherwise src is the
+// plain 32-bit index.
+template <bool STEP, bool CODED>
 __global__ __launch_bounds__(kTile) void scatter_kernel(int64_t nseg, int64_t per_xcd, const int32_t* __restrict__ seg,
                                                         const uint32_t* __restrict__ src, const double* __restrict__ buf,
                                                         const double* __restrict__ dbuf, double* __restrict__ out) {
@@ -265,8 +416,8 @@ __global__ __launch_bounds__(kTile) void scatter_kernel(int64_t nseg, int64_t pe
         const int32_t cend = (c + kChunk < pend ? c + kChunk : pend);
         for (int32_t q = c + t; q < cend; q += kTile) {
             const uint32_t sq = src[q];
-            vals[q - c] = contrib(buf, sq);
-            if (STEP) dvals[q - c] = dbuf[sq & kIdxMask];
+            vals[q - c] = CODED ? contrib(buf, sq) : buf[sq];
+            if (STEP) dvals[q - c] = dbuf[CODED ? (sq & kIdxMask) : sq];
         }
         __syncthreads();
         const int32_t a = p0 > c ? p0 : c, b = p1 < cend ? p1 : cend;
@@ -385,6 +536,42 @@ void beam_scales(const double* sX, BeamArgs& a) {
         for (int r = 0; r < 6; ++r) a.sT[6 * aa + r] = sX[aa] * sX[r < 3 ? 3 + r : 6 + r];
     for (int r = 0; r < 6; ++r)
         for (int q = r; q < 6; ++q) a.sS[mh::sym6(r, q)] = sX[r < 3 ? 3 + r : 6 + r] * sX[q < 3 ? 3 + q : 6 + q];
+}
+
+// Value pass then direction pass of one dynamic beam group.
+template <int ND, bool UDOF>
+void launch_dyn(BeamDynArgs& a, bool step, hipStream_t s) {
+    {
+        const int64_t ntile = (a.nele + kBeamBlock - 1) / kBeamBlock;
+        a.per_xcd = (ntile + kXCD - 1) / kXCD;
+        hipLaunchKernelGGL((beam_dyn_value_kernel<ND, UDOF>), dim3((unsigned)(a.per_xcd * kXCD)), dim3(kBeamBlock), 0, s, a);
+    }
+    const int64_t ntile = (a.nele + kDirElems - 1) / kDirElems;
+    a.per_xcd = (ntile + kXCD - 1) / kXCD;
+    const dim3 gr((unsigned)(a.per_xcd * kXCD));
+    if (step) hipLaunchKernelGGL((beam_dyn_dir_kernel<ND, UDOF, true>), gr, dim3(kDirBlock), 0, s, a);
+    else hipLaunchKernelGGL((beam_dyn_dir_kernel<ND, UDOF, false>), gr, dim3(kDirBlock), 0, s, a);
+}
+int launch_dyn_beams(Ctx* c, const Group& g, int OX, bool step, const double* c6, unsigned long long gtag) {
+    BeamDynArgs a;
+    a.nele = g.nele; a.idx = g.d_idx; a.uidx = g.d_uidx; a.nodes = g.d_nodes; a.xyz = c->d_xyz; a.o2 = g.d_o2;
+    a.sec = g.d_sec;
+    for (int d = 0; d < 3; ++d) a.X[d] = c->d_X[d];
+    a.U = c->d_U;
+    BeamArgs b;
+    beam_scales(g.scaleX, b);
+    for (int i = 0; i < 6; ++i) a.sA[i] = b.sA[i];
+    for (int i = 0; i < 18; ++i) a.sT[i] = b.sT[i];
+    for (int i = 0; i < 21; ++i) a.sS[i] = b.sS[i];
+    for (int i = 0; i < 12; ++i) { a.sR[i] = b.sR[i]; a.scale[i] = g.scaleX[i]; }
+    for (int i = 0; i < 6; ++i) a.c[i] = c6[i];
+    a.Kc = c->d_Kc + g.kcbase; a.K = c->d_Kbuf + g.kbase; a.R = c->d_Rbuf + g.rbase;
+    a.dR = step ? c->d_dRbuf + g.rbase : nullptr;
+    a.flag = c->d_flag; a.gtag = gtag;
+    const bool u = g.kind == MH_KIND_BEAM_U;
+    if (OX == 1) { if (u) launch_dyn<2, true>(a, step, c->stream); else launch_dyn<2, false>(a, step, c->stream); }
+    else { if (u) launch_dyn<3, true>(a, step, c->stream); else launch_dyn<3, false>(a, step, c->stream); }
+    return check_launch(c, "dynamic beam kernels");
 }
 
 // ============================================================================ C ABI
@@ -533,29 +720,38 @@ int mh_group_add(mh_ctx* c, int kind, int64_t nele, int nX, const int64_t* idxX,
     return MH_OK;
 }
 
-int mh_prepare_sweepx(mh_ctx* c, int64_t* nnz_out) {
-    if (!c) return MH_ERR_ARG;
+int mh_prepare_sweepx(mh_ctx* c, int OX, int64_t* nnz_out) {
+    if (!c || OX < 0 || OX > 2) return MH_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     free_prepared(c);
-    // contribution counts and buffer offsets, in eletype order
-    int64_t nK = 0, nP = 0, nR = 0;
+    // contribution counts and buffer offsets, in eletype order. Static beams store the
+    // compact tangent; dynamic beams (K + a₁C + b₁M is not symmetric) the full block.
+    int64_t nK = 0, nP = 0, nR = 0, nKc = 0;
+    bool coded = false;
     for (auto& g : c->groups) {
-        g.compact = g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U;
+        const bool beam = g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U;
+        g.compact = beam && OX == 0;
+        coded = coded || g.compact;
         g.kper = g.compact ? mh::kCompact : (int64_t)g.nX * g.nX;
         g.kbase = nK; g.rbase = nR;
+        g.kcbase = nKc;
+        if (beam && OX > 0) nKc += g.nele * mh::kCompact;
         nK += g.nele * g.kper;
         nP += g.nele * g.nX * g.nX;
         nR += g.nele * g.nX;
     }
-    if (nP >= INT32_MAX || nK >= (1ll << 29) || nR >= (1ll << 29)) {
-        c->err = "more than 2^31 contributions or 2^29 element outputs"; return MH_ERR_SIZE;
+    const int64_t klim = coded ? (1ll << 29) : (int64_t)UINT32_MAX;
+    if (nP >= INT32_MAX || nK >= klim || nR >= (int64_t)UINT32_MAX) {
+        c->err = "more than 2^31 contributions or too many element outputs for 32-bit maps"; return MH_ERR_SIZE;
     }
+    c->OX = OX; c->coded = coded;
     c->nK = nK; c->nP = nP; c->nR = nR;
     const int64_t nX = c->nX;
     hipStream_t s = c->stream;
     HIPCHK(c, hipMalloc(&c->d_Kbuf, sizeof(double) * std::max<int64_t>(nK, 1)));
     HIPCHK(c, hipMalloc(&c->d_Rbuf, sizeof(double) * std::max<int64_t>(nR, 1)));
     HIPCHK(c, hipMalloc(&c->d_dRbuf, sizeof(double) * std::max<int64_t>(nR, 1)));
+    if (nKc) HIPCHK(c, hipMalloc(&c->d_Kc, sizeof(double) * nKc));
     HIPCHK(c, hipMalloc(&c->d_rhs, sizeof(double) * std::max<int64_t>(nX, 1)));
     HIPCHK(c, hipMalloc(&c->d_colptr, sizeof(int32_t) * (nX + 1)));
     HIPCHK(c, hipMalloc(&c->d_rseg, sizeof(int32_t) * (nX + 1)));
@@ -702,14 +898,13 @@ int mh_set_point_values(mh_ctx* c, int igroup, const double* v) {
 
 int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err* err) {
     if (err) { err->code = MH_OK; err->igroup = -1; err->iele = -1; }
-    if (!c || !c->prepared || OX < 0 || OX > 2 || mission < 0 || mission > 1) return MH_ERR_ARG;
-    (void)c6;
+    if (!c || !c->prepared || OX < 0 || OX > 2 || mission < 0 || mission > 1 || (OX > 0 && !c6)) return MH_ERR_ARG;
+    if (OX != c->OX) {
+        c->err = "assembler prepared for OX=" + std::to_string(c->OX) + ", called with OX=" + std::to_string(OX);
+        return MH_ERR_ARG;
+    }
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
-    for (auto& g : c->groups)
-        if ((g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U) && OX > 0) {
-            c->err = "dynamic (OX>0) beam assembly is not available in this build"; return MH_ERR_ARG;
-        }
     const bool step = (mission == 0) && OX > 0;
     hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, s, c->d_flag, ~0ull);
     HIPCHK(c, hipEventRecord(c->ev[0], s));
@@ -717,7 +912,9 @@ int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err*
         Group& g = c->groups[gi];
         if (g.nX == 0 || g.nele == 0) continue;
         const unsigned long long gtag = (unsigned long long)gi << 40;
-        if (g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U) {
+        if ((g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U) && OX > 0) {
+            if (int r = launch_dyn_beams(c, g, OX, step, c6, gtag)) return r;
+        } else if (g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U) {
             BeamArgs a;
             a.nele = g.nele; a.idx = g.d_idx; a.uidx = g.d_uidx; a.nodes = g.d_nodes; a.xyz = c->d_xyz; a.o2 = g.d_o2;
             a.sec = g.d_sec; a.X = c->d_X[0]; a.U = c->d_U;
@@ -741,18 +938,22 @@ int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err*
     HIPCHK(c, hipEventRecord(c->ev[1], s));
     if (c->nnz) {
         const int64_t per = ((c->nnz + kTile - 1) / kTile + kXCD - 1) / kXCD;
-        hipLaunchKernelGGL(scatter_kernel<false>, dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nnz, per, c->d_kseg,
-                           c->d_ksrc, c->d_Kbuf, (const double*)nullptr, c->d_nzval);
+        if (c->coded)
+            hipLaunchKernelGGL((scatter_kernel<false, true>), dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nnz, per,
+                               c->d_kseg, c->d_ksrc, c->d_Kbuf, (const double*)nullptr, c->d_nzval);
+        else
+            hipLaunchKernelGGL((scatter_kernel<false, false>), dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nnz, per,
+                               c->d_kseg, c->d_ksrc, c->d_Kbuf, (const double*)nullptr, c->d_nzval);
     }
     HIPCHK(c, hipEventRecord(c->ev[2], s));
     if (c->nX) {
         const int64_t per = ((c->nX + kTile - 1) / kTile + kXCD - 1) / kXCD;
         if (step)
-            hipLaunchKernelGGL(scatter_kernel<true>, dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nX, per, c->d_rseg,
-                               c->d_rsrc, c->d_Rbuf, (const double*)c->d_dRbuf, c->d_rhs);
+            hipLaunchKernelGGL((scatter_kernel<true, false>), dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nX, per,
+                               c->d_rseg, c->d_rsrc, c->d_Rbuf, (const double*)c->d_dRbuf, c->d_rhs);
         else
-            hipLaunchKernelGGL(scatter_kernel<false>, dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nX, per, c->d_rseg,
-                               c->d_rsrc, c->d_Rbuf, (const double*)nullptr, c->d_rhs);
+            hipLaunchKernelGGL((scatter_kernel<false, false>), dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nX, per,
+                               c->d_rseg, c->d_rsrc, c->d_Rbuf, (const double*)nullptr, c->d_rhs);
     }
     HIPCHK(c, hipEventRecord(c->ev[3], s));
     if (int r = check_launch(c, "scatter kernels")) return r;

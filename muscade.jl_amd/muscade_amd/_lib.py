@@ -33,7 +33,7 @@ SIGNATURES = {
     "mh_strerror": (ctypes.c_char_p, [_I]),
     "mh_model_begin": (_I, [_P, _L, _L, _L, _L, _D]),
     "mh_group_add": (_I, [_P, _I, _L, _I, _I64, _D, _I, _I64, _I64, _D, _D, ctypes.POINTER(_I)]),
-    "mh_prepare_sweepx": (_I, [_P, _I64]),
+    "mh_prepare_sweepx": (_I, [_P, _I, _I64]),
     "mh_get_pattern": (_I, [_P, _I64, _I64]),
     "mh_get_asm": (_I, [_P, _I, _I64, _I64]),
     "mh_set_X": (_I, [_P, _I, _D]),

@@ -89,7 +89,7 @@ class AssemblySweepX:
         self.ctx = DeviceContext(device)
         self.groups = upload_model(self.ctx, model, dis)
         nnz = ctypes.c_int64()
-        self.ctx.check(lib.mh_prepare_sweepx(self.ctx.h, ctypes.byref(nnz)), "prepare")
+        self.ctx.check(lib.mh_prepare_sweepx(self.ctx.h, OX, ctypes.byref(nnz)), "prepare")
         self.nX = model.ndof["X"]
         self.nnz = nnz.value
         self.colptr = np.zeros(self.nX + 1, np.int64)

@@ -29,10 +29,12 @@ class SweepX(metaclass=_SweepXMeta):
 
     @classmethod
     def run(cls, pstate, verbose, dbg, *, time, initialstate, β=0.25, γ=0.5, maxiter=50, maxΔx=1e-5,
-            maxLλ=math.inf, saveiter=False, device=0):
+            maxLλ=math.inf, saveiter=False, device=0, Assembly=AssemblySweepX):
+        """`Assembly` is the assembler type prepared for the model, as the type argument of
+        prepare(AssemblySweepX{OX},model,dis) (SweepX.jl:24,188)."""
         OX = cls.OX
         model, dis = initialstate.model, initialstate.dis
-        out = AssemblySweepX(OX, model, dis, device=device)
+        out = Assembly(OX, model, dis, device=device)
         nX = out.nX
         scaleX = dis.scaleX
         cΔx2, cLλ2 = maxΔx ** 2, maxLλ ** 2

@@ -24,7 +24,7 @@ def build_lattice(nx, ny, nz, seed=7, disp=1e-3, rot=1e-2, sigma=0.2):
     return space_frame(nx, ny, nz, seed=seed, disp=disp, rot=rot, sigma=sigma)
 
 
-KIND_TO_ORACLE = {1: orc.K_BEAM, 3: orc.K_HOLD, 4: orc.K_DOFLOAD, 5: orc.K_ZERO}
+KIND_TO_ORACLE = {1: orc.K_BEAM, 2: orc.K_BEAM, 3: orc.K_HOLD, 4: orc.K_DOFLOAD, 5: orc.K_ZERO}
 
 
 def oracle_model(model, dis, t=0.0):
@@ -62,3 +62,31 @@ def relerr(a, b):
     a, b = np.asarray(a), np.asarray(b)
     d = np.abs(b).max()
     return np.abs(a - b).max() / (d if d > 0 else 1.0)
+
+
+class OracleAssembly:
+    """The oracle's sequential assembler behind the interface of muscade_amd.AssemblySweepX,
+    so that the host solver loop can be run against it (test infrastructure only)."""
+
+    def __init__(self, OX, model, dis, device=0):
+        import scipy.sparse as sp
+        from muscade_amd.assemble import Newmarkβcoefficients
+        self._sp = sp
+        self.OX, self.model, self.dis = OX, model, dis
+        self.nX = model.ndof["X"]
+        self.c = Newmarkβcoefficients(OX)
+        self.om = None
+        self.Lλ = self.nzval = None
+
+    def assemble(self, state, mission, dbg=None, fetch=True):
+        self.om = oracle_model(self.model, self.dis, t=state.time)
+        r, rhs, nz, err = self.om.assemble(self.OX, 0 if mission == "step" else 1, self.c.array(),
+                                           state.X[:self.OX + 1], state.U[0] if self.model.ndof["U"] else None,
+                                           state.A if self.model.ndof["A"] else None)
+        assert r == 0, err
+        self.Lλ, self.nzval = rhs, nz
+        return rhs, nz
+
+    def Lλx(self):
+        om = self.om
+        return self._sp.csc_matrix((self.nzval.copy(), om.rowval - 1, om.colptr - 1), shape=(self.nX, self.nX))

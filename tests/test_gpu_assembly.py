@@ -179,3 +179,19 @@ def test_sweepx0_cantilever_equilibrium(mc):
     assert np.abs(ref["rhs"]).max() <= 1e-6 * F
     tip = mc.getdof(s, field="t3", nodID=[41])[0]
     assert 0.3 < tip < 10.   # large deflection under α = PL²/EI ≈ 2.38
+
+
+def test_corotated_half_turn_is_nan(mc):
+    """A beam along -x has a corotated frame at a half turn; Muscade's ∂vₛₘ/∂X is NaN there
+    (Rotations.jl:101), so the residual is NaN and assembly raises, as in the oracle."""
+    model = mc.Model("minus_x")
+    mc.addnode(model, [[1., 0., 0.], [0., 0., 0.]])
+    mat = mc.BeamCrossSection(1., 2., 3., 4., 0., 0.)
+    mc.addelement(model, mc.EulerBeam3D, [[1, 2]], mat=mat, orient2=[0., 1., 0.])
+    state = mc.initialize(model)
+    Ro, _, _ = orc.beam_diffed_residual([1., 0., 0.], [0., 0., 0.], [0., 1., 0.], mat.values, [np.zeros(12)])
+    assert np.isnan(Ro).all()
+    out = mc.AssemblySweepX(0, model, state.dis)
+    with pytest.raises(mc.MuscadeError) as ei:
+        out.assemble(state, "iter")
+    assert ei.value.dbg["iele"] == 1
