@@ -84,6 +84,52 @@ def cpu_baseline(model, state, nsample):
     return n / dt, n, dt
 
 
+def dynamic_bench(args):
+    """Secondary measurement: dynamic assembly (AssemblySweepX{2}, :iter) of the catenary
+    chains (inertia, added mass, drag, roll inertia; K + a₁C + b₁M), 256 chains by default."""
+    import muscade_amd as mc
+    from muscade_amd import _lib
+    from muscade_amd.workloads import catenary_chains
+    t0 = time.perf_counter()
+    model, state = catenary_chains(args.chains, args.chain_min, args.chain_max)
+    nbeam = model.eletypes[0].nele
+    log(f"dynamic model: {args.chains} chains, {nbeam} beams, nX={model.ndof['X']} ({time.perf_counter() - t0:.1f}s)")
+    t0 = time.perf_counter()
+    out = mc.AssemblySweepX(2, model, state.dis)
+    t_prep = time.perf_counter() - t0
+    log(f"dynamic prepare (nnz={out.nnz}): {t_prep:.2f}s")
+    out.c = mc.Newmarkβcoefficients(2, 0.05, 0.25, 0.5)
+    out.set_state(state, 0.0)
+    h = out.ctx.h
+    c6 = out.c.array()
+    err = _lib.mh_err()
+    res = {}
+    for mission, m in (("iter", 1), ("step", 0)):
+        def step():
+            r = _lib.lib.mh_assemble_sweepx(h, 2, m, _lib.dptr(c6), ctypes.byref(err))
+            if r != 0:
+                raise RuntimeError(_lib.lib.mh_last_error(h).decode())
+        for _ in range(args.warmup):
+            step()
+        _lib.lib.mh_sync(h)
+        ms = np.zeros(3)
+        acc = np.zeros(3)
+        nst = max(1, args.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(nst):
+            step()
+            _lib.lib.mh_last_timings(h, _lib.dptr(ms))
+            acc += ms
+        _lib.lib.mh_sync(h)
+        el = time.perf_counter() - t0
+        res[mission] = {"elements_per_s": nbeam * nst / el, "ms_per_step": el / nst * 1e3,
+                        "kernels_ms": dict(zip(("eval", "scatter_nz", "scatter_rhs"), (acc / nst).tolist()))}
+        log(f"dynamic {mission}: {el / nst * 1e3:.3f} ms/step, kernels {acc / nst}")
+    return {"workload": f"catenary chains x{args.chains}, {nbeam} EulerBeam3D with inertia/added mass/drag/roll "
+                        f"inertia, AssemblySweepX{{2}} (Newmark-beta, dt=0.05)",
+            "nbeam": nbeam, "nX": out.nX, "nnz": out.nnz, "prepare_s": t_prep, **res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +141,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=40000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="run only the counter-calibration copy kernel")
+    ap.add_argument("--dynamic", action="store_true", help="also measure the dynamic (SweepX{2}) assembly")
+    ap.add_argument("--only-dynamic", action="store_true", help="measure only the dynamic assembly")
+    ap.add_argument("--chains", type=int, default=256)
+    ap.add_argument("--chain-min", type=int, default=1000)
+    ap.add_argument("--chain-max", type=int, default=50000)
     args = ap.parse_args()
     if args.calib_copy:
         calib_copy()
@@ -104,6 +155,9 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.only_dynamic:
+        print(json.dumps({"dynamic": dynamic_bench(args)}))
+        return
     import muscade_amd as mc
     from muscade_amd import _lib
     from muscade_amd.workloads import space_frame
@@ -194,6 +248,9 @@ def main():
         "prepare_s": t_prep,
         "cpu_baseline": cpu,
     }
+    if args.dynamic:
+        del out
+        line["dynamic"] = dynamic_bench(args)
     print(json.dumps(line))
 
 

@@ -9,6 +9,14 @@ per node, random displacements N(0,1e-3 m) and rotations N(0,1e-2 rad).
 
 cantilever: the reference's static SweepX{0} case — 100 beams along x,
 L=10 m, clamped at node 1, tip load ramped to 5e4 N in 10 steps.
+
+catenary_chains: the dynamic SweepX{2} case — independent mooring-line-like
+chains of EulerBeam3D (n_e ~ U{nmin..nmax} elements, horizontal span and depth
+from a length ~ U(500,2000) m), EA=5e9 N, EI=2e7 Nm², w=1e3 N/m, with inertia,
+added mass, linear and quadratic drag and roll inertia; bottom node pinned by
+three Holds, a horizontal DofLoad at the top node; the state is the as-meshed
+catenary plus small random displacements, velocities and accelerations.
+256 chains with nmin=1000, nmax=50000 give about 6.5 million beams.
 """
 import numpy as np
 
@@ -88,3 +96,52 @@ def cantilever(nel=100, L=10., F=5e4, nstep=10):
         return F0 * t
     addelement(model, DofLoad, [nel + 1], field="t3", value=load)
     return model
+
+
+def chain_section():
+    """Section of the catenary chains. EA, EI and w are the workload's; μ, ι₁, GJ and the
+    hydrodynamic coefficients are chosen (a 0.3 m steel-like line in water)."""
+    return BeamCrossSection(EA=5e9, EI2=2e7, EI3=2e7, GJ=1.5e7, mu=1e3 / 9.81, iota1=0.5, w=1e3,
+                            Ca1=5., Cl1=1., Cq1=10., Ca2=72., Cl2=5., Cq2=300., Ca3=72., Cl3=5., Cq3=300.)
+
+
+def topload(t, F0=1e5, T=10.):
+    return F0 * np.sin(2. * np.pi * t / T)
+
+
+def catenary_chains(nchain=256, nmin=1000, nmax=50000, seed=1002, disp=1e-3, rot=1e-3, vel=0.1, acc=0.1):
+    rng = np.random.default_rng(seed)
+    model = Model("catenary_chains")
+    ne = rng.integers(nmin, nmax + 1, size=nchain)
+    length = rng.uniform(500., 2000., size=nchain)
+    coords, first = [], []
+    base = 0
+    for c in range(nchain):
+        n = int(ne[c])
+        s = np.linspace(0., 1., n + 1)
+        span, depth = 0.8 * length[c], 0.5 * length[c]
+        # a sagging curve from the anchor (0,y,0) to the top (span,y,depth), x increasing
+        x = span * s
+        z = depth * (np.cosh(1.5 * s) - 1.) / (np.cosh(1.5) - 1.)
+        y = np.full_like(s, 50. * c)
+        coords.append(np.stack([x, y, z], axis=1))
+        first.append(base)
+        base += n + 1
+    addnode(model, np.concatenate(coords))
+    sec = chain_section().values
+    nodes = np.concatenate([np.stack([f + 1 + np.arange(int(n)), f + 2 + np.arange(int(n))], axis=1)
+                            for f, n in zip(first, ne)])
+    addelement(model, EulerBeam3D, nodes, mat=sec, orient2=np.array([0., 1., 0.]))
+    anchors = np.array(first) + 1
+    tops = np.array(first) + ne + 1
+    for f in ("t1", "t2", "t3"):
+        addelement(model, Hold, anchors.reshape(-1, 1), field=f)
+    addelement(model, DofLoad, tops.reshape(-1, 1), field="t1", value=topload)
+    state = initialize(model, nXder=3)
+    nX = model.ndof["X"]
+    state.X[0][:] = rng.normal(0., disp, nX)
+    rdofs = np.concatenate([model.getdofID("X", f) for f in ("r1", "r2", "r3")]) - 1
+    state.X[0][rdofs] = rng.normal(0., rot, rdofs.size)
+    state.X[1][:] = rng.normal(0., vel, nX)
+    state.X[2][:] = rng.normal(0., acc, nX)
+    return model, state
