@@ -41,8 +41,10 @@ enum {
     MH_KIND_BEAM_U = 2,  /* EulerBeam3D{BeamCrossSection,true}: 3 U-dofs, load per length             */
     MH_KIND_HOLD = 3,    /* Hold = DofConstraint{:X} equal, gap=x (src/BasicElements.jl:426-438,484-488) */
     MH_KIND_DOFLOAD = 4, /* DofLoad, R = -value(t) evaluated by the host (BasicElements.jl:275-284)      */
-    MH_KIND_ZERO = 5     /* Lagrangian-only element (DofCost, BasicElements.jl:32-50): in SweepX its
+    MH_KIND_ZERO = 5,    /* Lagrangian-only element (DofCost, BasicElements.jl:32-50): in SweepX its
                             X-dof pairs enter the pattern and contribute structural zeros             */
+    MH_KIND_BAR = 6,     /* Bar3D{Mat,false} with AxisymmetricBarCrossSection (toolbox/BarElement.jl:89-202) */
+    MH_KIND_BAR_U = 7    /* Bar3D{Mat,true}: 3 U-dofs, load per length (BarElement.jl:115-118,192)     */
 };
 
 typedef struct mh_ctx mh_ctx;
@@ -68,11 +70,13 @@ int mh_model_begin(mh_ctx* ctx, int64_t nXdof, int64_t nUdof, int64_t nAdof, int
 /* Add one element group, in Muscade's eletype order.
  *   idxX   nX×nele  model X-dof of each element X-dof (dis.dis[ieletyp].index[iele].X)
  *   scaleX nX       dis.dis[ieletyp].scale.X
- *   idxU   nU×nele  model U-dof of each element U-dof (MH_KIND_BEAM_U only, else NULL)
- *   nodes  2×nele   1-based node numbers of each beam (beams only, else NULL)
+ *   idxU   nU×nele  model U-dof of each element U-dof (MH_KIND_BEAM_U, MH_KIND_BAR_U, else NULL)
+ *   nodes  2×nele   1-based node numbers of each beam or bar (else NULL)
  *   orient2 3×nele  first bending axis of each beam (EulerBeam3D's orient2 keyword,
- *                   or column 2 of its rₘ; beams only)
- *   mat    16×nele  BeamCrossSection fields in declaration order (beams only)
+ *                   or column 2 of its rₘ; beams only, NULL for bars)
+ *   mat    16×nele  beams: BeamCrossSection fields in declaration order
+ *          10×nele  bars: AxisymmetricBarCrossSection fields in declaration order, then ϵₛ
+ *                   (Bar3D's keyword, BarElement.jl:106; eps() by default)
  * Returns the 0-based group number in *igroup. */
 int mh_group_add(mh_ctx* ctx, int kind, int64_t nele, int nX, const int64_t* idxX, const double* scaleX, int nU,
                  const int64_t* idxU, const int64_t* nodes, const double* orient2, const double* mat, int* igroup);
@@ -96,6 +100,9 @@ int mh_set_U(mh_ctx* ctx, const double* u);
 int mh_set_A(mh_ctx* ctx, const double* a);
 /* Host-evaluated element values (DofLoad: value(t,args...) per element). */
 int mh_set_point_values(mh_ctx* ctx, int igroup, const double* values);
+/* state.time, passed to the residuals as t (Assemble.jl:484; Bar3D's weight ramp,
+ * BarElement.jl:144). */
+int mh_set_time(mh_ctx* ctx, double t);
 
 /* ---- assemble!{mission}(::AssemblySweepX{OX}) (src/Assemble.jl:470-505,
  *      src/SweepX.jl:39-96) ---------------------------------------------------

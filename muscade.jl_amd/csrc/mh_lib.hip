@@ -31,6 +31,7 @@
 #include "../../include/muscade_hip.h"
 #include "beam_math.hpp"
 #include "beam_dyn.hpp"
+#include "bar_math.hpp"
 
 namespace {
 
@@ -43,7 +44,7 @@ struct Group {
     int32_t* d_uidx = nullptr;   // nU × nele
     int32_t* d_nodes = nullptr;  // 2 × nele
     double* d_o2 = nullptr;      // 3 × nele
-    double* d_sec = nullptr;     // 16 × nele
+    double* d_sec = nullptr;     // 16 × nele (beams), 10 × nele (bars)
     double* d_vals = nullptr;    // nele (DofLoad values)
     std::vector<int64_t> h_idx;  // nX × nele, 1-based (for mh_get_asm)
     int64_t kbase = 0, rbase = 0;
@@ -62,6 +63,7 @@ struct Ctx {
     double* d_X[3] = {nullptr, nullptr, nullptr};
     double* d_U = nullptr;
     double* d_A = nullptr;
+    double t = 0.;               // state.time
     // prepared
     bool prepared = false;
     int OX = 0;                  // order of the AssemblySweepX{OX} the maps were prepared for
@@ -372,6 +374,71 @@ __global__ __launch_bounds__(kDirBlock) void beam_dyn_dir_kernel(BeamDynArgs a) 
     if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
 }
 
+// ---- Bar3D -----------------------------------------------------------------
+struct BarArgs {
+    int64_t nele, per_xcd;
+    const int32_t* idx;
+    const int32_t* uidx;
+    const int32_t* nodes;
+    const double* xyz;
+    const double* sec;   // 10 × nele: AxisymmetricBarCrossSection fields, then ϵₛ
+    const double* X[3];
+    const double* U;
+    double t;
+    double scale[6];
+    double c[6];
+    double* K;           // 36 per element, element-major, column-major inside
+    double* R;
+    double* dR;
+    unsigned long long* flag;
+    unsigned long long gtag;
+};
+constexpr int kBarBlock = 128;
+// One bar per lane (any OX): residual, K + a₁C + b₁M and, on :step, ∂R/∂δr. The 36
+// tangent numbers of the block are staged in LDS and written as one contiguous range.
+template <int ND, bool UDOF, bool STEP>
+__global__ __launch_bounds__(kBarBlock) void bar_kernel(BarArgs a) {
+    __shared__ double tile[kBarBlock * 36];
+    const int64_t e0 = xcd_tile(a.per_xcd) * kBarBlock;
+    if (e0 >= a.nele) return;   // whole workgroup leaves together
+    const int tid = threadIdx.x;
+    const int64_t e = e0 + tid;
+    const int64_t n = a.nele;
+    if (e < n) {
+        double X[ND][6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int32_t q = a.idx[k * n + e];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) X[d][k] = a.X[d][q];
+        }
+        const int32_t n1 = a.nodes[e], n2 = a.nodes[n + e];
+        mh::v3 c1 = mh::mk(a.xyz[3 * (int64_t)n1], a.xyz[3 * (int64_t)n1 + 1], a.xyz[3 * (int64_t)n1 + 2]);
+        mh::v3 c2 = mh::mk(a.xyz[3 * (int64_t)n2], a.xyz[3 * (int64_t)n2 + 1], a.xyz[3 * (int64_t)n2 + 2]);
+        const mh::BarGeo G = mh::bar_geometry(c1, c2, a.sec[9 * n + e]);
+        mh::BarSection S;
+        S.EA = a.sec[e]; S.mu = a.sec[n + e]; S.w = a.sec[2 * n + e]; S.Cat = a.sec[3 * n + e]; S.Clt = a.sec[4 * n + e];
+        S.Cqt = a.sec[5 * n + e]; S.Can = a.sec[6 * n + e]; S.Cln = a.sec[7 * n + e]; S.Cqn = a.sec[8 * n + e];
+        double U[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) U[q] = UDOF ? a.U[a.uidx[q * n + e]] : 0.;
+        double R[6], dR[6];
+        double* K = tile + tid * 36;
+        const int nan = mh::bar_element<ND>(G, S, X, U, a.t, a.scale, a.c, R, K, STEP ? dR : nullptr);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) a.R[i * n + e] = R[i];
+        if (STEP) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) a.dR[i * n + e] = dR[i];
+        }
+        if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+    }
+    __syncthreads();
+    const int64_t nv = (n - e0 < kBarBlock ? n - e0 : kBarBlock) * 36;
+    double* dst = a.K + e0 * 36;
+    for (int64_t f = tid; f < nv; f += kBarBlock) dst[f] = tile[f];
+}
+
 // Contribution source: (code << 29) | index; the code selects an exact multiplier.
 constexpr uint32_t kIdxMask = (1u << 29) - 1;
 __device__ __forceinline__ double contrib(const double* __restrict__ buf, uint32_t s) {
@@ -574,6 +641,30 @@ int launch_dyn_beams(Ctx* c, const Group& g, int OX, bool step, const double* c6
     return check_launch(c, "dynamic beam kernels");
 }
 
+template <int ND, bool UDOF>
+void launch_bar(BarArgs& a, bool step, hipStream_t s) {
+    const int64_t ntile = (a.nele + kBarBlock - 1) / kBarBlock;
+    a.per_xcd = (ntile + kXCD - 1) / kXCD;
+    const dim3 gr((unsigned)(a.per_xcd * kXCD));
+    if (step) hipLaunchKernelGGL((bar_kernel<ND, UDOF, true>), gr, dim3(kBarBlock), 0, s, a);
+    else hipLaunchKernelGGL((bar_kernel<ND, UDOF, false>), gr, dim3(kBarBlock), 0, s, a);
+}
+int launch_bars(Ctx* c, const Group& g, int OX, bool step, const double* c6, unsigned long long gtag) {
+    BarArgs a;
+    a.nele = g.nele; a.idx = g.d_idx; a.uidx = g.d_uidx; a.nodes = g.d_nodes; a.xyz = c->d_xyz; a.sec = g.d_sec;
+    for (int d = 0; d < 3; ++d) a.X[d] = c->d_X[d];
+    a.U = c->d_U;
+    a.t = c->t;
+    for (int i = 0; i < 6; ++i) { a.scale[i] = g.scaleX[i]; a.c[i] = (OX > 0 && c6) ? c6[i] : 0.; }
+    a.K = c->d_Kbuf + g.kbase; a.R = c->d_Rbuf + g.rbase; a.dR = step ? c->d_dRbuf + g.rbase : nullptr;
+    a.flag = c->d_flag; a.gtag = gtag;
+    const bool u = g.kind == MH_KIND_BAR_U;
+    if (OX == 0) { if (u) launch_bar<1, true>(a, false, c->stream); else launch_bar<1, false>(a, false, c->stream); }
+    else if (OX == 1) { if (u) launch_bar<2, true>(a, step, c->stream); else launch_bar<2, false>(a, step, c->stream); }
+    else { if (u) launch_bar<3, true>(a, step, c->stream); else launch_bar<3, false>(a, step, c->stream); }
+    return check_launch(c, "bar kernels");
+}
+
 // ============================================================================ C ABI
 struct mh_ctx : Ctx {};
 
@@ -672,13 +763,17 @@ int mh_group_add(mh_ctx* c, int kind, int64_t nele, int nX, const int64_t* idxX,
     HIPCHK(c, hipSetDevice(c->device));
     free_prepared(c);
     const bool beam = kind == MH_KIND_BEAM || kind == MH_KIND_BEAM_U;
+    const bool bar = kind == MH_KIND_BAR || kind == MH_KIND_BAR_U;
     if (beam && (nX != 12 || !nodes || !orient2 || !mat)) { c->err = "beam group needs 12 X dofs, nodes, orient2 and mat"; return MH_ERR_ARG; }
-    if (kind == MH_KIND_BEAM_U && (nU != 3 || !idxU)) { c->err = "Udof beam group needs 3 U dofs"; return MH_ERR_ARG; }
+    if (bar && (nX != 6 || !nodes || !mat)) { c->err = "bar group needs 6 X dofs, nodes and mat"; return MH_ERR_ARG; }
+    if ((kind == MH_KIND_BEAM_U || kind == MH_KIND_BAR_U) && (nU != 3 || !idxU)) {
+        c->err = "Udof beam or bar group needs 3 U dofs"; return MH_ERR_ARG;
+    }
     if (kind == MH_KIND_HOLD && nX != 2) { c->err = "Hold group needs 2 X dofs"; return MH_ERR_ARG; }
     if (kind == MH_KIND_DOFLOAD && nX != 1) { c->err = "DofLoad group needs 1 X dof"; return MH_ERR_ARG; }
-    if (kind < MH_KIND_BEAM || kind > MH_KIND_ZERO) { c->err = "unknown element kind"; return MH_ERR_ARG; }
+    if (kind < MH_KIND_BEAM || kind > MH_KIND_BAR_U) { c->err = "unknown element kind"; return MH_ERR_ARG; }
     Group g;
-    g.kind = kind; g.nele = nele; g.nX = nX; g.nU = beam ? nU : 0;
+    g.kind = kind; g.nele = nele; g.nX = nX; g.nU = (beam || bar) ? nU : 0;
     for (int k = 0; k < 12; ++k) g.scaleX[k] = (scaleX && k < nX) ? scaleX[k] : 1.;
     if (beam)
         for (int k = 0; k < 6; ++k)
@@ -686,7 +781,19 @@ int mh_group_add(mh_ctx* c, int kind, int64_t nele, int nX, const int64_t* idxX,
     int r;
     if ((r = upload_idx(c, &g.d_idx, idxX, nX, nele, c->nX, false))) return r;
     g.h_idx.assign(idxX, idxX + (size_t)nX * nele);
-    if (kind == MH_KIND_BEAM_U && (r = upload_idx(c, &g.d_uidx, idxU, 3, nele, c->nU, false))) return r;
+    if ((kind == MH_KIND_BEAM_U || kind == MH_KIND_BAR_U) && (r = upload_idx(c, &g.d_uidx, idxU, 3, nele, c->nU, false)))
+        return r;
+    if (bar) {
+        std::vector<int32_t> nd32((size_t)2 * nele);
+        for (int64_t e = 0; e < nele; ++e)
+            for (int k = 0; k < 2; ++k) {
+                int64_t v = nodes[e * 2 + k];
+                if (v < 1 || v > c->nnod) { c->err = "node index out of range"; return MH_ERR_ARG; }
+                nd32[e * 2 + k] = (int32_t)(v - 1);
+            }
+        if ((r = upload_soa<int32_t>(c, &g.d_nodes, nd32.data(), 2, nele))) return r;
+        if ((r = upload_soa<double>(c, &g.d_sec, mat, 10, nele))) return r;
+    }
     if (beam) {
         std::vector<int64_t> nd((size_t)2 * nele);
         for (int64_t e = 0; e < nele; ++e)
@@ -896,6 +1003,12 @@ int mh_set_point_values(mh_ctx* c, int igroup, const double* v) {
     return MH_OK;
 }
 
+int mh_set_time(mh_ctx* c, double t) {
+    if (!c) return MH_ERR_ARG;
+    c->t = t;
+    return MH_OK;
+}
+
 int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err* err) {
     if (err) { err->code = MH_OK; err->igroup = -1; err->iele = -1; }
     if (!c || !c->prepared || OX < 0 || OX > 2 || mission < 0 || mission > 1 || (OX > 0 && !c6)) return MH_ERR_ARG;
@@ -912,7 +1025,9 @@ int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err*
         Group& g = c->groups[gi];
         if (g.nX == 0 || g.nele == 0) continue;
         const unsigned long long gtag = (unsigned long long)gi << 40;
-        if ((g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U) && OX > 0) {
+        if (g.kind == MH_KIND_BAR || g.kind == MH_KIND_BAR_U) {
+            if (int r = launch_bars(c, g, OX, step, c6, gtag)) return r;
+        } else if ((g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U) && OX > 0) {
             if (int r = launch_dyn_beams(c, g, OX, step, c6, gtag)) return r;
         } else if (g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U) {
             BeamArgs a;

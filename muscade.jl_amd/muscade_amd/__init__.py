@@ -6,10 +6,12 @@ hand-written HIP kernels through the C ABI of include/muscade_hip.h.
 from ._lib import LIBPATH, lib
 from .model import (Disassembler, Model, MuscadeError, State, addelement, addnode, getdof, initialize, setdof,
                     setscale)
-from .elements import BeamCrossSection, DofCost, DofLoad, EulerBeam3D, EulerBeam3DU, Hold
+from .elements import (AxisymmetricBarCrossSection, Bar3D, Bar3DU, BeamCrossSection, DofCost, DofLoad, EulerBeam3D,
+                       EulerBeam3DU, Hold)
 from .assemble import AssemblySweepX, DeviceContext, Newmarkβcoefficients
 from .sweepx import SweepX, solve
 
 __all__ = ["LIBPATH", "lib", "Model", "MuscadeError", "State", "Disassembler", "addnode", "addelement", "initialize",
            "setscale", "getdof", "setdof", "BeamCrossSection", "EulerBeam3D", "EulerBeam3DU", "Hold", "DofLoad",
+           "AxisymmetricBarCrossSection", "Bar3D", "Bar3DU",
            "DofCost", "AssemblySweepX", "DeviceContext", "Newmarkβcoefficients", "SweepX", "solve"]

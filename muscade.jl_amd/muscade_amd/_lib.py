@@ -12,7 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libmuscade_hip.so")
 
 MH_OK, MH_ERR_ARG, MH_ERR_HIP, MH_ERR_NAN, MH_ERR_ORIENT, MH_ERR_SIZE = 0, 1, 2, 3, 4, 5
-MH_KIND_BEAM, MH_KIND_BEAM_U, MH_KIND_HOLD, MH_KIND_DOFLOAD, MH_KIND_ZERO = 1, 2, 3, 4, 5
+MH_KIND_BEAM, MH_KIND_BEAM_U, MH_KIND_HOLD, MH_KIND_DOFLOAD, MH_KIND_ZERO, MH_KIND_BAR, MH_KIND_BAR_U = 1, 2, 3, 4, 5, 6, 7
 
 # every function of include/muscade_hip.h, with its ctypes signature
 _P = ctypes.c_void_p
@@ -40,6 +40,7 @@ SIGNATURES = {
     "mh_set_U": (_I, [_P, _D]),
     "mh_set_A": (_I, [_P, _D]),
     "mh_set_point_values": (_I, [_P, _I, _D]),
+    "mh_set_time": (_I, [_P, ctypes.c_double]),
     "mh_assemble_sweepx": (_I, [_P, _I, _I, _D, ctypes.POINTER(mh_err)]),
     "mh_get_rhs": (_I, [_P, _D]),
     "mh_get_nzval": (_I, [_P, _D]),

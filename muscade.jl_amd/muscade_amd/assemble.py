@@ -70,9 +70,12 @@ def upload_model(ctx, model, dis):
             nodes = as_i64(E.cat_nodes()[:, :2] + 1)
             orient = as_f64(np.concatenate([p["orient2"] for p in E.params]))
             mat = as_f64(np.concatenate([p["mat"] for p in E.params]))
-            if kind == _lib.MH_KIND_BEAM_U:
-                idxU = as_i64(d.U)
-                nU = 3
+        elif kind in (_lib.MH_KIND_BAR, _lib.MH_KIND_BAR_U):
+            nodes = as_i64(E.cat_nodes()[:, :2] + 1)
+            mat = as_f64(np.concatenate([np.concatenate([p["mat"], p["eps_s"][:, None]], axis=1) for p in E.params]))
+        if kind in (_lib.MH_KIND_BEAM_U, _lib.MH_KIND_BAR_U):
+            idxU = as_i64(d.U)
+            nU = 3
         g = ctypes.c_int()
         ctx.check(lib.mh_group_add(ctx.h, kind, nele, nX, i64ptr(idx) if nX else None, dptr(sX), nU, i64ptr(idxU),
                                    i64ptr(nodes), dptr(orient), dptr(mat), ctypes.byref(g)), f"element type {it + 1}")
@@ -112,6 +115,7 @@ class AssemblySweepX:
 
     def set_state(self, state, t):
         h = self.ctx.h
+        self.ctx.check(lib.mh_set_time(h, float(t)), "set time")
         for ider in range(min(self.OX + 1, len(state.X))):
             self.ctx.check(lib.mh_set_X(h, ider, dptr(as_f64(state.X[ider]))), "set X")
         if self.model.ndof["U"]:

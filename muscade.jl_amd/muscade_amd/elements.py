@@ -86,6 +86,53 @@ class EulerBeam3DU(EulerBeam3D):
     kind = _lib.MH_KIND_BEAM_U
 
 
+BAR_FIELDS = ("EA", "μ", "w", "Caₜ", "Clₜ", "Cqₜ", "Caₙ", "Clₙ", "Cqₙ")
+
+
+class AxisymmetricBarCrossSection:
+    """AxisymmetricBarCrossSection(;EA,μ,w=0,Caₜ=0,Clₜ=0,Cqₜ=0,Caₙ=0,Clₙ=0,Cqₙ=0) — toolbox/BarElement.jl:36-48."""
+
+    def __init__(self, EA, mu, w=0., Cat=0., Clt=0., Cqt=0., Can=0., Cln=0., Cqn=0.):
+        self.values = np.array([EA, mu, w, Cat, Clt, Cqt, Can, Cln, Cqn], float)
+
+    def __repr__(self):
+        return "AxisymmetricBarCrossSection(" + ", ".join(f"{n}={v:g}" for n, v in zip(BAR_FIELDS, self.values)) + ")"
+
+
+class Bar3D:
+    """Bar3D{Udof}(nod; mat, ϵₛ=eps()) — toolbox/BarElement.jl:89-121.
+
+    `mat` is an AxisymmetricBarCrossSection or an (nele,9) array; `eps_s` (ϵₛ) makes the
+    stress-free length (1-ϵₛ)·L₀, a scalar or one value per element."""
+    Udof = False
+    kind = _lib.MH_KIND_BAR
+
+    @classmethod
+    def doflist(cls):   # BarElement.jl:106-113
+        inod = (1, 1, 1, 2, 2, 2)
+        field = ("t1", "t2", "t3") * 2
+        dclass = ("X",) * 6
+        if cls.Udof:
+            inod += (3, 3, 3)
+            field += ("t1", "t2", "t3")
+            dclass += ("U",) * 3
+        return inod, dclass, field
+
+    @classmethod
+    def construct(cls, model, nodes0, mat, eps_s=float(np.finfo(float).eps)):
+        nele = nodes0.shape[0]
+        m = mat.values if isinstance(mat, AxisymmetricBarCrossSection) else mat
+        m = _per_element(m, nele, 9, "mat")
+        e = np.broadcast_to(np.asarray(eps_s, float), (nele,)).copy()
+        return ("Bar3D", "AxisymmetricBarCrossSection", cls.Udof), cls.doflist(), {"mat": m, "eps_s": e}
+
+
+class Bar3DU(Bar3D):
+    """Bar3D{true}: with 3 U-dofs on a third node, a distributed load per length (BarElement.jl:110-113,192)."""
+    Udof = True
+    kind = _lib.MH_KIND_BAR_U
+
+
 class Hold:
     """Hold(nod; field, λfield=Symbol(:λ,field)) — src/BasicElements.jl:484-488.
     A DofConstraint{:X} with gap(x)=x[1] and mode `equal` (BasicElements.jl:426-438)."""

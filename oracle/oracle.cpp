@@ -6,7 +6,8 @@
 //   dof numbering          src/ModelDescription.jl:195-263   (orc_number_dofs)
 //   asmvec!/asmmat!        src/Assemble.jl:340-357,373-448   (orc_model_prepare_sweepx)
 //   assemble!/addin!       src/Assemble.jl:470-505,559-608, src/SweepX.jl:39-96 (orc_model_assemble_sweepx)
-//   element residuals      toolbox/BeamElement.jl, src/BasicElements.jl, test/SomeElements.jl
+//   element residuals      toolbox/BeamElement.jl, toolbox/BarElement.jl, src/BasicElements.jl,
+//                          test/SomeElements.jl
 //   rotations              toolbox/Rotations.jl (orc_sinc1, orc_scac_d, orc_rodrigues ...)
 // Parity pins: see the header of each test module in tests/ for the golden
 // literal (reference file:line) each function is checked against.
@@ -16,12 +17,13 @@
 #include <numeric>
 #include <tuple>
 #include <vector>
+#include "bar.hpp"
 #include "beam.hpp"
 #include "elements.hpp"
 
 using namespace orc;
 
-enum Kind { K_BEAM = 0, K_HOLD = 1, K_DOFLOAD = 2, K_ZERO = 3, K_TURBINE = 4, K_ANCHOR = 5 };
+enum Kind { K_BEAM = 0, K_HOLD = 1, K_DOFLOAD = 2, K_ZERO = 3, K_TURBINE = 4, K_ANCHOR = 5, K_BAR = 6 };
 
 struct EType {
     int kind, nele, nX, nU, nA;
@@ -29,6 +31,7 @@ struct EType {
     std::vector<int64_t> iX, iU, iA;   // 1-based model dof indices, [iele*n + k]
     std::vector<double> scaleX;        // per element dof (dis.dis[ieletyp].scale.X)
     std::vector<EulerBeam3D> beams;
+    std::vector<Bar3D> bars;
     std::vector<double> F;             // DofLoad values
     std::vector<Turbine> turbines;
     std::vector<AnchorLine> anchors;
@@ -36,6 +39,7 @@ struct EType {
 };
 struct OModel {
     int64_t nX, nU, nA;
+    double t = 0.;                     // state.time handed to time-dependent elements
     std::vector<EType> types;
     int64_t nnz = 0;
     std::vector<int64_t> colptr, rowval;
@@ -43,8 +47,14 @@ struct OModel {
 
 // --------------------------------------------------------------- elements
 template <class TX, int ND>
-static int elem_residual(const EType& ty, int e, const TX (*X)[12], const double* U, const double* A, TX* R) {
+static int elem_residual(const EType& ty, int e, const TX (*X)[12], const double* U, const double* A, double t, TX* R) {
     switch (ty.kind) {
+    case K_BAR: {
+        TX Uj[3];
+        for (int c = 0; c < 3; ++c) Uj[c] = TX(ty.udof ? U[c] : 0.);
+        bar_residual<TX, ND>(ty.bars[e], X, ty.udof ? Uj : nullptr, t, R);
+        return 0;
+    }
     case K_BEAM: {
         if (ty.udof) {
             TX Uj[1][3];
@@ -130,6 +140,45 @@ int orc_beam_construct(const double* c1, const double* c2, const double* orient2
     for (int s = 0; s < 6; ++s) for (int i = 0; i < 4; ++i) *p++ = sh[s][i];
     *p++ = o.L;
     for (int i = 0; i < 4; ++i) *p++ = o.dL[i];
+    return 0;
+}
+
+// Bar3D constructor (BarElement.jl:106-121). out: cm(3) tgm(3) L0 Ls wgp(4) zgp(4) znod(2) psi1(4) psi2(4) = 26.
+void orc_bar_construct(const double* c1, const double* c2, double eps_s, double* out) {
+    AxisymmetricBarCrossSection mat{}; Bar3D o;
+    make_bar(c1, c2, mat, eps_s, 0, &o);
+    double* p = out;
+    for (int i = 0; i < 3; ++i) *p++ = o.cm[i];
+    for (int i = 0; i < 3; ++i) *p++ = o.tgm[i];
+    *p++ = o.L0; *p++ = o.Ls;
+    for (int i = 0; i < 4; ++i) *p++ = o.wgp[i];
+    for (int i = 0; i < 4; ++i) *p++ = o.zgp[i];
+    for (int i = 0; i < 2; ++i) *p++ = o.znod[i];
+    for (int i = 0; i < 4; ++i) *p++ = o.psi1[i];
+    for (int i = 0; i < 4; ++i) *p++ = o.psi2[i];
+}
+// diffed_residual of one Bar3D (Diagnostic.jl:929-961): R (6), dR/dX[der] (ND × 6×6, column-major),
+// dR/dU (6×3) if udof. X: ND×6. mat9 = AxisymmetricBarCrossSection fields.
+int orc_bar_diffed_residual(const double* c1, const double* c2, const double* mat9, double eps_s, int udof, int ND,
+                            const double* X, const double* U, double t, double* R, double* dRdX, double* dRdU) {
+    AxisymmetricBarCrossSection mat; std::memcpy(&mat, mat9, sizeof(mat));
+    Bar3D o;
+    make_bar(c1, c2, mat, eps_s, udof, &o);
+    constexpr int NP = 21;   // revariate{1}((;X,U)) over ND*6 X entries then 3 U entries
+    using TX = Dual<NP, double>;
+    TX Xd[3][12];
+    for (int d = 0; d < ND; ++d) for (int i = 0; i < 6; ++i) Xd[d][i] = seed<NP, double>(X[d * 6 + i], d * 6 + i);
+    TX Ud[3];
+    for (int c = 0; c < 3; ++c) Ud[c] = udof ? seed<NP, double>(U[c], ND * 6 + c) : TX(0.);
+    TX Rr[6];
+    if (ND == 1) bar_residual<TX, 1>(o, Xd, udof ? Ud : nullptr, t, Rr);
+    else if (ND == 2) bar_residual<TX, 2>(o, Xd, udof ? Ud : nullptr, t, Rr);
+    else bar_residual<TX, 3>(o, Xd, udof ? Ud : nullptr, t, Rr);
+    for (int i = 0; i < 6; ++i) {
+        R[i] = Rr[i].x;
+        for (int d = 0; d < ND; ++d) for (int j = 0; j < 6; ++j) dRdX[d * 36 + i + 6 * j] = Rr[i].dx[d * 6 + j];
+        if (udof && dRdU) for (int c = 0; c < 3; ++c) dRdU[i + 6 * c] = Rr[i].dx[ND * 6 + c];
+    }
     return 0;
 }
 
@@ -264,6 +313,7 @@ void orc_model_free(void* h) { delete (OModel*)h; }
 // Append an element type (one Muscade eletype). idx*: 1-based model dofs, nele×n (element-major).
 // params per kind:
 //   K_BEAM    : nele × 25 = c1(3) c2(3) orient2(3) mat(16)
+//   K_BAR     : nele × 16 = c1(3) c2(3) mat(9: EA μ w Caₜ Clₜ Cqₜ Caₙ Clₙ Cqₙ) ϵₛ
 //   K_DOFLOAD : nele × 1  = F
 //   K_TURBINE : nele × 5  = xm(2) z seadrag skydrag
 //   K_ANCHOR  : nele × 10 = xmtop(3) dxmtop(3) xmbot(2) L buoyancy
@@ -283,6 +333,13 @@ int orc_model_add_type(void* h, int kind, int nele, int nX, int nU, int nA, cons
             BeamCrossSection mat; std::memcpy(&mat, p + 9, sizeof(mat));
             if (make_beam(p, p + 3, p + 6, mat, udof, &t.beams[e])) return 1;
         }
+    } else if (kind == K_BAR) {
+        t.bars.resize(nele);
+        for (int e = 0; e < nele; ++e) {
+            const double* p = params + 16 * (size_t)e;
+            AxisymmetricBarCrossSection mat; std::memcpy(&mat, p + 6, sizeof(mat));
+            make_bar(p, p + 3, mat, p[15], udof, &t.bars[e]);
+        }
     } else if (kind == K_DOFLOAD) {
         t.F.assign(params, params + nele);
     } else if (kind == K_TURBINE) {
@@ -298,6 +355,7 @@ int orc_model_add_type(void* h, int kind, int nele, int nX, int nU, int nA, cons
     m->types.push_back(std::move(t));
     return 0;
 }
+void orc_model_set_time(void* h, double t) { ((OModel*)h)->t = t; }
 int orc_model_set_dofload(void* h, int itype, const double* F) {
     OModel* m = (OModel*)h; EType& t = m->types[itype];
     if (t.kind != K_DOFLOAD) return 1;
@@ -412,7 +470,7 @@ static int assemble_sweepx(OModel* m, int mission, const double* c, const double
                     if (OX >= 2) { TX v = x[2][k] + c[3] * dX; v.dx[12] = b; Xd[2][k] = v; }
                 }
                 TX R[12];
-                elem_residual<TX, ND>(t, e, Xd, Ue, Ae, R);
+                elem_residual<TX, ND>(t, e, Xd, Ue, Ae, m->t, R);
                 for (int i = 0; i < nX; ++i) if (has_nan(R[i])) { *err_type = (int)it; *err_ele = e; return 3; }
                 for (int i = 0; i < nX; ++i) R[i] = R[i] * t.scaleX[i];
                 for (int i = 0; i < nX; ++i) if (a1[i]) rhs[a1[i] - 1] += R[i].x;            // add_value!
@@ -429,7 +487,7 @@ static int assemble_sweepx(OModel* m, int mission, const double* c, const double
                     if (OX >= 2) Xd[2][k] = x[2][k] + c[3] * dX;
                 }
                 TX R[12];
-                elem_residual<TX, ND>(t, e, Xd, Ue, Ae, R);
+                elem_residual<TX, ND>(t, e, Xd, Ue, Ae, m->t, R);
                 for (int i = 0; i < nX; ++i) if (has_nan(R[i])) { *err_type = (int)it; *err_ele = e; return 3; }
                 for (int i = 0; i < nX; ++i) R[i] = R[i] * t.scaleX[i];
                 for (int i = 0; i < nX; ++i) if (a1[i]) rhs[a1[i] - 1] += R[i].x;

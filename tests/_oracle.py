@@ -16,7 +16,7 @@ _I64 = ctypes.POINTER(ctypes.c_int64)
 _IP = ctypes.POINTER(ctypes.c_int)
 _V = ctypes.c_void_p
 
-K_BEAM, K_HOLD, K_DOFLOAD, K_ZERO, K_TURBINE, K_ANCHOR = 0, 1, 2, 3, 4, 5
+K_BEAM, K_HOLD, K_DOFLOAD, K_ZERO, K_TURBINE, K_ANCHOR, K_BAR = 0, 1, 2, 3, 4, 5, 6
 
 _SIG = {
     "orc_sinc1_k": (ctypes.c_double, [ctypes.c_int, ctypes.c_double]),
@@ -26,6 +26,10 @@ _SIG = {
     "orc_rodrigues_roundtrip": (None, [_D, _D, _D]),
     "orc_adjust": (None, [_D, _D, _D]),
     "orc_beam_construct": (ctypes.c_int, [_D, _D, _D, _D]),
+    "orc_bar_construct": (None, [_D, _D, ctypes.c_double, _D]),
+    "orc_bar_diffed_residual": (ctypes.c_int, [_D, _D, _D, ctypes.c_double, ctypes.c_int, ctypes.c_int, _D, _D,
+                                               ctypes.c_double, _D, _D, _D]),
+    "orc_model_set_time": (None, [_V, ctypes.c_double]),
     "orc_beam_diffed_residual": (ctypes.c_int, [_D, _D, _D, _D, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, _D]),
     "orc_beam_residual_plain": (ctypes.c_int, [_D, _D, _D, _D, ctypes.c_int, _D, _D, _D, _D]),
     "orc_beam_gauge": (ctypes.c_int, [_D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
@@ -97,6 +101,34 @@ def beam_diffed_residual(c1, c2, orient2, mat16, X, U=None):
     return R, Ks, dU.reshape(12, 3, order="F")
 
 
+EPS = float(np.finfo(float).eps)
+
+
+def bar_construct(c1, c2, eps_s=EPS):
+    out = np.zeros(26)
+    lib.orc_bar_construct(dp(f64(c1)), dp(f64(c2)), eps_s, dp(out))
+    names = [("cm", 3), ("tgm", 3), ("L0", 1), ("Ls", 1), ("wgp", 4), ("zgp", 4), ("znod", 2), ("psi1", 4), ("psi2", 4)]
+    r, q = {}, 0
+    for n, k in names:
+        r[n] = out[q:q + k] if k > 1 else out[q]
+        q += k
+    return r
+
+
+def bar_diffed_residual(c1, c2, mat9, X, U=None, t=0., eps_s=EPS):
+    """diffed_residual of one Bar3D: X is a list of ND 6-vectors. Returns R, [∂R/∂X_der], ∂R/∂U."""
+    ND = len(X)
+    Xa = f64(np.concatenate([np.asarray(x, float) for x in X]))
+    R = np.zeros(6)
+    dX = np.zeros(ND * 36)
+    dU = np.zeros(18)
+    udof = 0 if U is None else 1
+    lib.orc_bar_diffed_residual(dp(f64(c1)), dp(f64(c2)), dp(f64(mat9)), eps_s, udof, ND, dp(Xa),
+                                dp(f64(U) if U is not None else None), t, dp(R), dp(dX), dp(dU))
+    Ks = [dX[d * 36:(d + 1) * 36].reshape(6, 6, order="F") for d in range(ND)]
+    return R, Ks, dU.reshape(6, 3, order="F")
+
+
 def beam_residual_plain(c1, c2, orient2, mat16, X):
     ND = len(X)
     Xa = f64(np.concatenate([np.asarray(x, float) for x in X]))
@@ -142,6 +174,9 @@ class OracleModel:
             raise ValueError("orc_model_add_type failed")
         self.ntypes += 1
         return self.ntypes - 1
+
+    def set_time(self, t):
+        lib.orc_model_set_time(self.h, float(t))
 
     def set_dofload(self, itype, F):
         lib.orc_model_set_dofload(self.h, itype, dp(f64(F)))

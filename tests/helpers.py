@@ -24,7 +24,7 @@ def build_lattice(nx, ny, nz, seed=7, disp=1e-3, rot=1e-2, sigma=0.2):
     return space_frame(nx, ny, nz, seed=seed, disp=disp, rot=rot, sigma=sigma)
 
 
-KIND_TO_ORACLE = {1: orc.K_BEAM, 2: orc.K_BEAM, 3: orc.K_HOLD, 4: orc.K_DOFLOAD, 5: orc.K_ZERO}
+KIND_TO_ORACLE = {1: orc.K_BEAM, 2: orc.K_BEAM, 3: orc.K_HOLD, 4: orc.K_DOFLOAD, 5: orc.K_ZERO, 6: orc.K_BAR, 7: orc.K_BAR}
 
 
 def oracle_model(model, dis, t=0.0):
@@ -40,12 +40,18 @@ def oracle_model(model, dis, t=0.0):
             o2 = np.concatenate([p["orient2"] for p in E.params])
             mat = np.concatenate([p["mat"] for p in E.params])
             params = np.concatenate([xyz[nodes[:, 0], :3], xyz[nodes[:, 1], :3], o2, mat], axis=1)
+        elif kind == orc.K_BAR:
+            nodes = E.cat_nodes()
+            mat = np.concatenate([p["mat"] for p in E.params])
+            eps = np.concatenate([p["eps_s"] for p in E.params])[:, None]
+            params = np.concatenate([xyz[nodes[:, 0], :3], xyz[nodes[:, 1], :3], mat, eps], axis=1)
         elif kind == orc.K_DOFLOAD:
             params = np.concatenate([np.full(n.shape[0], float(p["value"](t, **p["args"])))
                                      for n, p in zip(E.nodes, E.params)])
         om.add_type(kind, d.X, d.U if d.U.shape[1] else None, d.A if d.A.shape[1] else None, d.scaleX, params,
                     udof=int(getattr(E.cls, "Udof", False)))
     om.prepare()
+    om.set_time(t)
     return om
 
 
