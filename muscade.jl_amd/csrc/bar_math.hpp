@@ -73,9 +73,10 @@ MH_HD BarLoad bar_load(const BarSection& S, v3 dl, v3 v, v3 a, double ramp) {
 // U-dofs); t: time; scale: scale.X (6); c: Newmark (a₁,a₂,a₃,b₁,b₂,b₃).
 // Outputs: R (6, scaled), K (36, column-major, = ∂R/∂δX scaled like Muscade's δX and R),
 // dRr (6, ∂R/∂δr of :step, scaled rows) when non-null. Returns 1 on NaN.
+// K = w0·∂R/∂X₀ + a₁·∂R/∂X₁ + b₁·∂R/∂X₂; w0 = 0 isolates the X′ and X″ partials.
 template <int ND>
 MH_HD int bar_element(const BarGeo& G, const BarSection& S, const double (*X)[6], const double* U, double t,
-                      const double* scale, const double* c, double* R, double* K, double* dRr) {
+                      const double* scale, const double* c, double* R, double* K, double* dRr, double w0 = 1.) {
     const v3 u1 = mk(X[0][0], X[0][1], X[0][2]), u2 = mk(X[0][3], X[0][4], X[0][5]);
     const v3 tg = mk((G.tgm.x + u2.x) - u1.x, (G.tgm.y + u2.y) - u1.y, (G.tgm.z + u2.z) - u1.z);
     const double L = sqrt((tg.x * tg.x + tg.y * tg.y) + tg.z * tg.z);
@@ -132,7 +133,7 @@ MH_HD int bar_element(const BarGeo& G, const BarSection& S, const double (*X)[6]
             const v3 dd = mk(sgn * P[0][q] / L, sgn * P[1][q] / L, sgn * P[2][q] / L);
             const double deps = sgn * comp(dl, q) / G.Ls;
             // ∂fₑ/∂(direction k) = Dδ ∂δ/∂X₀ₖ + a₁ Dv ψₖ e_q + b₁ Da ψₖ e_q
-            v3 dfe = mul(Ld.Dd, dd);
+            v3 dfe = w0 * mul(Ld.Dd, dd);
             if (ND >= 2) dfe = dfe + (c[0] * pk) * col(Ld.Dv, q);
             if (ND >= 3) dfe = dfe + (c[3] * pk) * col(Ld.Da, q);
             for (int j = 0; j < 6; ++j) {
@@ -140,7 +141,7 @@ MH_HD int bar_element(const BarGeo& G, const BarSection& S, const double (*X)[6]
                 const double sj = j < 3 ? -1. : 1.;
                 const double dedX = sj * sgn * P[qj][q] / (G.L0 * L);   // ∂(ε∂X₀)_j/∂X_k
                 const double xj = j < 3 ? p1 * comp(dfe, j) : p2 * comp(dfe, j - 3);
-                Kl[j][k] += ((S.EA * deps) * edX[j] + fi * dedX + xj) * w;
+                Kl[j][k] += (w0 * ((S.EA * deps) * edX[j] + fi * dedX) + xj) * w;
             }
         }
     }
@@ -149,6 +150,34 @@ MH_HD int bar_element(const BarGeo& G, const BarSection& S, const double (*X)[6]
     for (int k = 0; k < 6; ++k)
         for (int j = 0; j < 6; ++j) { K[j + 6 * k] = Kl[j][k] * (scale[j] * scale[k]); nan |= K[j + 6 * k] != K[j + 6 * k]; }
     if (dRr) for (int j = 0; j < 6; ++j) { dRr[j] = Rr[j] * scale[j]; nan |= dRr[j] != dRr[j]; }
+    return nan;
+}
+
+// Λ·∂R/∂X⁽ᵈᵉʳ⁾ (der < ND) and Λ·∂R/∂U for one bar (unscaled).
+template <int ND>
+MH_HD int bar_vjp_element(const BarGeo& G, const BarSection& S, const double (*X)[6], const double* U, double t,
+                          const double* lam, double (*gX)[6], double* gU) {
+    const double one[6] = {1., 1., 1., 1., 1., 1.};
+    double R[6], K[36];
+    int nan = 0;
+    for (int der = 0; der < ND; ++der) {
+        const double c[6] = {der == 1 ? 1. : 0., 0., 0., der == 2 ? 1. : 0., 0., 0.};
+        nan |= bar_element<ND>(G, S, X, U, t, one, c, R, K, nullptr, der == 0 ? 1. : 0.);
+        for (int k = 0; k < 6; ++k) {
+            double g = 0.;
+            for (int i = 0; i < 6; ++i) g = g + lam[i] * K[i + 6 * k];
+            gX[der][k] = g;
+        }
+    }
+    // ∂Rⱼ/∂U_q = -Σ_gp w ψⱼ δ(j mod 3, q)
+    for (int q = 0; q < 3; ++q) {
+        double s = 0.;
+        for (int g = 0; g < 4; ++g) {
+            const double z = G.zgp[g];
+            s = s + ((-z + 1. / 2.) * lam[q] + (z + 1. / 2.) * lam[3 + q]) * G.wgp[g];
+        }
+        gU[q] = -s;
+    }
     return nan;
 }
 

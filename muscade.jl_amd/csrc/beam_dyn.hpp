@@ -390,20 +390,24 @@ MH_HD v3 rotvec(const m3& r) {
 }
 
 // The element dofs of the solver's trial path as jets.
-//   dir 0..11: δX along dof `dir`, with weights (s, a₁s, b₁s), s = scale.X[dir];
+//   dir 0..11: δX along dof `dir`, with weights (w0·s, a₁s, b₁s), s = scale.X[dir]
+//              (w0 = 1 for the solver; w0 = 0 isolates the X′ or X″ partials);
 //   dir 12:    δr of :step, (0, a, b) with a = a₂X₁ + a₃X₂, b = b₂X₁ + b₃X₂ per dof;
+//   dir 13:    X₀ + ε·lam (a directional derivative along lam, 12 values);
 //   otherwise: no ε seed.
 // X: ND rows of 12; c = (a₁,a₂,a₃,b₁,b₂,b₃).
 template <int NT, bool E> MH_HD void dyn_seed(const double (*X)[12], const double* scale, const double* c, int dir,
-                                              Jet<NT, E>* Xj) {
+                                              Jet<NT, E>* Xj, double w0 = 1., const double* lam = nullptr) {
 #pragma unroll
     for (int i = 0; i < 12; ++i) {
         Jet<NT, E> q = jzero<NT, E>();
 #pragma unroll
         for (int n = 0; n < NT; ++n) q.v[n] = X[n][i];
         if constexpr (E) {
-            if (dir == i) {
-                q.e[0] = scale[i];
+            if (dir == 13) {
+                q.e[0] = lam[i];
+            } else if (dir == i) {
+                q.e[0] = w0 * scale[i];
                 if constexpr (NT > 1) q.e[1] = c[0] * scale[i];
                 if constexpr (NT > 2) q.e[2] = c[3] * scale[i];
             } else if (dir == 12) {
@@ -439,9 +443,10 @@ template <int ND> MH_HD void dyn_frozen_loads(const BeamGeo& G, const BeamDynSec
 // for a δX direction, the column of the Jacobians of x_gp and vₛₘ w.r.t. X₀ (Jx, Jv).
 struct DynDir { double DF[4][3]; double Dm[3]; double Jx[4][3]; double Jv[3]; };
 template <int ND> MH_HD void dyn_direction(const BeamGeo& G, const BeamDynSection& D, const double (*X)[12],
-                                           const double* U, const double* scale, const double* c, int dir, DynDir& o) {
+                                           const double* U, const double* scale, const double* c, int dir, DynDir& o,
+                                           double w0 = 1., const double* lam = nullptr) {
     Jet<ND, true> Xj[12];
-    dyn_seed<ND, true>(X, scale, c, dir, Xj);
+    dyn_seed<ND, true>(X, scale, c, dir, Xj, w0, lam);
     DynKin<true> k;
     dyn_kinematics<ND, true>(G, Xj, k);
     Jet<1, true> fe[4][3], me[3];
@@ -463,6 +468,65 @@ MH_HD double dyn_row(const BeamGeo& G, const double (*Jx_i)[3], const double* Jv
         s = s + t * G.dL[g];
     }
     return s;
+}
+
+// ---- adjoint (vector-Jacobian products) of the residual ------------------------
+// For L = Λ·R the load part of ∂L/∂X⁽ᵈ⁾ₖ is Σ_gp dL (y_gp·∂fₑ/∂X⁽ᵈ⁾ₖ + y_v·∂mₑ/∂X⁽ᵈ⁾ₖ) with
+// y_gp = J_x,gp Λ and y_v = J_v Λ, the derivatives of x_gp and vₛₘ along Λ.
+struct DynY { double y[4][3]; double yv[3]; };
+MH_HD void dyn_y(const BeamGeo& G, const double (*X)[12], const double* lam, DynY& o) {
+    Jet<1, true> Xj[12];
+    dyn_seed<1, true>(X, nullptr, nullptr, 13, Xj, 0., lam);
+    DynKin<true> k;
+    dyn_kinematics<1, true>(G, Xj, k);
+    for (int g = 0; g < 4; ++g)
+        for (int q = 0; q < 3; ++q) o.y[g][q] = k.xe[g][q];
+    v3 jv = dyn_vsm_e(k.r0);
+    o.yv[0] = jv.x; o.yv[1] = jv.y; o.yv[2] = jv.z;
+}
+MH_HD double dyn_ycontract(const BeamGeo& G, const DynY& y, const DynDir& d) {
+    double s = 0.;
+    for (int g = 0; g < 4; ++g) {
+        double t = (y.y[g][0] * d.DF[g][0] + y.y[g][1] * d.DF[g][1]) + y.y[g][2] * d.DF[g][2];
+        t = t + ((y.yv[0] * d.Dm[0] + y.yv[1] * d.Dm[1]) + y.yv[2] * d.Dm[2]);
+        s = s + t * G.dL[g];
+    }
+    return s;
+}
+// ∂L/∂U (U-dofs: fₑ - U, per length): -Σ_gp dL y_gp
+MH_HD v3 dyn_vjp_u(const BeamGeo& G, const DynY& y) {
+    v3 s = mk(0., 0., 0.);
+    for (int g = 0; g < 4; ++g) s = s + G.dL[g] * mk(y.y[g][0], y.y[g][1], y.y[g][2]);
+    return mk(-s.x, -s.y, -s.z);
+}
+// Whole dynamic element on one thread (host checks): gX[der][k] = Σ_i Λᵢ ∂Rᵢ/∂X⁽ᵈᵉʳ⁾ₖ (unscaled)
+// for der < ND, gU = Λ·∂R/∂U.
+template <int ND> MH_HD int dyn_vjp_element(const BeamGeo& G, const BeamSection& S, const BeamDynSection& D,
+                                            const double (*X)[12], const double* U, const double* lam, double (*gX)[12],
+                                            double* gU) {
+    v3 eg[4], mtot;
+    dyn_frozen_loads<ND>(G, D, X, U, eg, mtot);
+    BeamOut out;
+    beam_static<true>(G, S, mk(0., 0., 0.), X[0], out, eg, mtot);
+    DynY y;
+    dyn_y(G, X, lam, y);
+    const double one[12] = {1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1.};
+    int nan = out.nan;
+    for (int der = 0; der < ND; ++der) {
+        double c[6] = {der == 1 ? 1. : 0., 0., 0., der == 2 ? 1. : 0., 0., 0.};
+        for (int k = 0; k < 12; ++k) {
+            DynDir d;
+            dyn_direction<ND>(G, D, X, U, one, c, k, d, der == 0 ? 1. : 0.);
+            double g = dyn_ycontract(G, y, d);
+            if (der == 0)
+                for (int i = 0; i < 12; ++i) g = g + compact_entry(out.Kc, k, i) * lam[i];
+            gX[der][k] = g;
+            nan |= g != g;
+        }
+    }
+    v3 gu = dyn_vjp_u(G, y);
+    gU[0] = gu.x; gU[1] = gu.y; gU[2] = gu.z;
+    return nan;
 }
 
 // Whole element on one thread (host checks): scaled R, K = ∂R/∂δX (12×12,

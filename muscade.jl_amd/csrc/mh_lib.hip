@@ -50,6 +50,7 @@ struct Group {
     int64_t kbase = 0, rbase = 0;
     int64_t kper = 0;            // Kbuf entries per element (45 for static beams, nX² otherwise)
     int64_t kcbase = 0;          // dynamic beams: offset of the compact static part in Kc
+    int64_t ubase = 0;           // U-dof groups: offset of the per-element ∂L/∂U entries
     bool compact = false;
 };
 
@@ -82,6 +83,16 @@ struct Ctx {
     int32_t* d_rseg = nullptr;
     uint32_t* d_rsrc = nullptr;
     unsigned long long* d_flag = nullptr;
+    // adjoint (VJP) of the residual Lagrangian
+    int64_t nRU = 0;             // per-element ∂L/∂U entries
+    int32_t* d_useg = nullptr;
+    uint32_t* d_usrc = nullptr;
+    double* d_lam = nullptr;     // Λ (nX)
+    double* d_gbuf = nullptr;    // per-element ∂L/∂X⁽ᵈ⁾ entries, 3 × nR (Rbuf layout per order)
+    double* d_gX = nullptr;      // 3 × nX
+    double* d_gubuf = nullptr;   // nRU
+    double* d_gU = nullptr;      // nU
+    int vjp_nd = 0;              // derivative orders of the last VJP
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timed = false;
 };
@@ -100,6 +111,8 @@ template <class T> void dfree(T*& p) { if (p) { (void)hipFree(p); p = nullptr; }
 void free_prepared(Ctx* c) {
     dfree(c->d_Kbuf); dfree(c->d_Rbuf); dfree(c->d_dRbuf); dfree(c->d_nzval); dfree(c->d_rhs); dfree(c->d_Kc);
     dfree(c->d_colptr); dfree(c->d_rowval); dfree(c->d_kseg); dfree(c->d_ksrc); dfree(c->d_rseg); dfree(c->d_rsrc);
+    dfree(c->d_useg); dfree(c->d_usrc); dfree(c->d_gbuf); dfree(c->d_gX); dfree(c->d_gubuf); dfree(c->d_gU);
+    c->vjp_nd = 0;
     c->prepared = false;
 }
 void free_model(Ctx* c) {
@@ -108,7 +121,7 @@ void free_model(Ctx* c) {
     c->groups.clear();
     dfree(c->d_xyz);
     for (auto& x : c->d_X) dfree(x);
-    dfree(c->d_U); dfree(c->d_A);
+    dfree(c->d_U); dfree(c->d_A); dfree(c->d_lam);
 }
 
 // ------------------------------------------------------------------ kernels
@@ -297,8 +310,10 @@ __global__ __launch_bounds__(kBeamBlock) void beam_dyn_value_kernel(BeamDynArgs 
         mh::dyn_frozen_loads<ND>(G, D, X, U, eg, mtot);
         mh::BeamOut out;
         mh::beam_static<true>(G, S, mh::mk(0., 0., 0.), X[0], out, eg, mtot);
+        if (a.R) {
 #pragma unroll
-        for (int i = 0; i < 12; ++i) a.R[i * n + e] = out.R[i] * a.sR[i];
+            for (int i = 0; i < 12; ++i) a.R[i * n + e] = out.R[i] * a.sR[i];
+        }
         double* t = tile + tid * mh::kCompact;
 #pragma unroll
         for (int i = 0; i < 6; ++i) t[i] = out.Kc[i] * a.sA[i];
@@ -437,6 +452,171 @@ __global__ __launch_bounds__(kBarBlock) void bar_kernel(BarArgs a) {
     const int64_t nv = (n - e0 < kBarBlock ? n - e0 : kBarBlock) * 36;
     double* dst = a.K + e0 * 36;
     for (int64_t f = tid; f < nv; f += kBarBlock) dst[f] = tile[f];
+}
+
+// ---- adjoint: Λ·∂R/∂X⁽ᵈ⁾, Λ·∂R/∂U per element ---------------------------------
+struct VjpArgs {
+    int64_t nele, per_xcd;
+    int nX;
+    int kind;
+    const int32_t* idx;
+    const int32_t* uidx;
+    const int32_t* nodes;
+    const double* xyz;
+    const double* o2;
+    const double* sec;
+    const double* X[3];
+    const double* U;
+    const double* lam;
+    double t;
+    const double* Kc;    // dynamic beams: unscaled compact static part (value pass)
+    double* g;           // g[der*gstride + i*nele + e]
+    int64_t gstride;
+    double* gu;          // gu[q*nele + e] (U-dof groups)
+    unsigned long long* flag;
+    unsigned long long gtag;
+};
+
+// Static beams, one per lane: Λ·K = K·Λ with the compact symmetric tangent; U-dofs
+// through y = ∂x_gp/∂X·Λ (one ε-jet pass).
+template <bool UDOF>
+__global__ __launch_bounds__(kBeamBlock) void beam_vjp_static_kernel(VjpArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nele) return;
+    const int64_t n = a.nele;
+    double X[1][12], lam[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) { const int32_t q = a.idx[k * n + e]; X[0][k] = a.X[0][q]; lam[k] = a.lam[q]; }
+    const int32_t n1 = a.nodes[e], n2 = a.nodes[n + e];
+    mh::v3 c1 = mh::mk(a.xyz[3 * (int64_t)n1], a.xyz[3 * (int64_t)n1 + 1], a.xyz[3 * (int64_t)n1 + 2]);
+    mh::v3 c2 = mh::mk(a.xyz[3 * (int64_t)n2], a.xyz[3 * (int64_t)n2 + 1], a.xyz[3 * (int64_t)n2 + 2]);
+    const mh::BeamGeo G = mh::beam_geometry(c1, c2, mh::mk(a.o2[e], a.o2[n + e], a.o2[2 * n + e]));
+    mh::BeamSection S;
+    S.EA = a.sec[e]; S.EI2 = a.sec[n + e]; S.EI3 = a.sec[2 * n + e]; S.GJ = a.sec[3 * n + e]; S.w = a.sec[6 * n + e];
+    mh::v3 fe = mh::mk(0., 0., S.w);
+    if (UDOF) fe = fe - mh::mk(a.U[a.uidx[e]], a.U[a.uidx[n + e]], a.U[a.uidx[2 * n + e]]);
+    mh::BeamOut out;
+    mh::beam_static(G, S, fe, X[0], out);
+    int nan = out.nan;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+        double g = 0.;
+        for (int j = 0; j < 12; ++j) g = g + mh::compact_entry(out.Kc, i, j) * lam[j];
+        a.g[i * n + e] = g;
+        nan |= g != g;
+    }
+    if (UDOF) {
+        mh::DynY y;
+        mh::dyn_y(G, X, lam, y);
+        const mh::v3 gu = mh::dyn_vjp_u(G, y);
+        a.gu[e] = gu.x; a.gu[n + e] = gu.y; a.gu[2 * n + e] = gu.z;
+    }
+    if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+}
+
+// Dynamic beams, 16 lanes per element: lane 12 forms y = J_x Λ, J_v Λ (shared through LDS)
+// and ∂L/∂U; lane k < 12 differentiates the loads along X⁽ᵈᵉʳ⁾ₖ for every order and
+// contracts with y, adding the static part (Kc Λ)ₖ for der = 0.
+template <int ND, bool UDOF>
+__global__ __launch_bounds__(kDirBlock) void beam_dyn_vjp_kernel(VjpArgs a) {
+    __shared__ double Ysh[kDirElems][15];
+    const int64_t e0 = xcd_tile(a.per_xcd) * kDirElems;
+    if (e0 >= a.nele) return;   // whole workgroup leaves together
+    const int el = threadIdx.x / kDirLanes, k = threadIdx.x % kDirLanes;
+    const int64_t e = e0 + el;
+    const int64_t n = a.nele;
+    const bool live = e < n && k <= 12;
+    mh::BeamGeo G;
+    mh::BeamDynSection D;
+    double X[ND][12], U[3], lam[12];
+    if (live) {
+        mh::BeamSection S;
+        BeamDynArgs b;
+        b.nele = a.nele; b.idx = a.idx; b.uidx = a.uidx; b.nodes = a.nodes; b.xyz = a.xyz; b.o2 = a.o2; b.sec = a.sec;
+        for (int d = 0; d < 3; ++d) b.X[d] = a.X[d];
+        b.U = a.U;
+        dyn_load<ND, UDOF>(b, e, G, S, D, X, U);
+#pragma unroll
+        for (int i = 0; i < 12; ++i) lam[i] = a.lam[a.idx[i * n + e]];
+        if (k == 12) {
+            mh::DynY y;
+            mh::dyn_y(G, X, lam, y);
+            for (int g = 0; g < 4; ++g)
+                for (int q = 0; q < 3; ++q) Ysh[el][3 * g + q] = y.y[g][q];
+            for (int q = 0; q < 3; ++q) Ysh[el][12 + q] = y.yv[q];
+            if (UDOF) {
+                const mh::v3 gu = mh::dyn_vjp_u(G, y);
+                a.gu[e] = gu.x; a.gu[n + e] = gu.y; a.gu[2 * n + e] = gu.z;
+            }
+        }
+    }
+    __syncthreads();
+    if (!live || k == 12) return;
+    mh::DynY y;
+    for (int g = 0; g < 4; ++g)
+        for (int q = 0; q < 3; ++q) y.y[g][q] = Ysh[el][3 * g + q];
+    for (int q = 0; q < 3; ++q) y.yv[q] = Ysh[el][12 + q];
+    const double one[12] = {1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1.};
+    int nan = 0;
+    for (int der = 0; der < ND; ++der) {
+        const double c[6] = {der == 1 ? 1. : 0., 0., 0., der == 2 ? 1. : 0., 0., 0.};
+        mh::DynDir d;
+        mh::dyn_direction<ND>(G, D, X, U, one, c, k, d, der == 0 ? 1. : 0.);
+        double g = mh::dyn_ycontract(G, y, d);
+        if (der == 0)
+            for (int i = 0; i < 12; ++i) g = g + mh::compact_entry(a.Kc + e * mh::kCompact, k, i) * lam[i];
+        a.g[der * a.gstride + k * n + e] = g;
+        nan |= g != g;
+    }
+    if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+}
+
+template <int ND, bool UDOF>
+__global__ __launch_bounds__(kBarBlock) void bar_vjp_kernel(VjpArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nele) return;
+    const int64_t n = a.nele;
+    double X[ND][6], lam[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int32_t q = a.idx[k * n + e];
+        lam[k] = a.lam[q];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) X[d][k] = a.X[d][q];
+    }
+    const int32_t n1 = a.nodes[e], n2 = a.nodes[n + e];
+    mh::v3 c1 = mh::mk(a.xyz[3 * (int64_t)n1], a.xyz[3 * (int64_t)n1 + 1], a.xyz[3 * (int64_t)n1 + 2]);
+    mh::v3 c2 = mh::mk(a.xyz[3 * (int64_t)n2], a.xyz[3 * (int64_t)n2 + 1], a.xyz[3 * (int64_t)n2 + 2]);
+    const mh::BarGeo G = mh::bar_geometry(c1, c2, a.sec[9 * n + e]);
+    mh::BarSection S;
+    S.EA = a.sec[e]; S.mu = a.sec[n + e]; S.w = a.sec[2 * n + e]; S.Cat = a.sec[3 * n + e]; S.Clt = a.sec[4 * n + e];
+    S.Cqt = a.sec[5 * n + e]; S.Can = a.sec[6 * n + e]; S.Cln = a.sec[7 * n + e]; S.Cqn = a.sec[8 * n + e];
+    double U[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) U[q] = UDOF ? a.U[a.uidx[q * n + e]] : 0.;
+    double gX[ND][6], gu[3];
+    const int nan = mh::bar_vjp_element<ND>(G, S, X, U, a.t, lam, gX, gu);
+#pragma unroll
+    for (int der = 0; der < ND; ++der)
+#pragma unroll
+        for (int i = 0; i < 6; ++i) a.g[der * a.gstride + i * n + e] = gX[der][i];
+    if (UDOF)
+        for (int q = 0; q < 3; ++q) a.gu[q * n + e] = gu[q];
+    if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+}
+
+// Hold: L = Λₓ(-λ) + Λ_λ(-x); DofLoad and structural zeros do not depend on X.
+__global__ __launch_bounds__(kBlock) void point_vjp_kernel(VjpArgs a, int nd) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nele) return;
+    const int64_t n = a.nele;
+    for (int der = 0; der < nd; ++der)
+        for (int i = 0; i < a.nX; ++i) a.g[der * a.gstride + i * n + e] = 0.;
+    if (a.kind == MH_KIND_HOLD) {
+        const double lx = a.lam[a.idx[e]], ll = a.lam[a.idx[n + e]];
+        a.g[e] = -ll;
+        a.g[n + e] = -lx;
+    }
 }
 
 // Contribution source: (code << 29) | index; the code selects an exact multiplier.
@@ -665,6 +845,73 @@ int launch_bars(Ctx* c, const Group& g, int OX, bool step, const double* c6, uns
     return check_launch(c, "bar kernels");
 }
 
+// Adjoint of one group: Λ·∂R/∂X⁽ᵈ⁾ (d < nd) and Λ·∂R/∂U into the per-element buffers.
+int launch_vjp(Ctx* c, const Group& g, int nd, unsigned long long gtag) {
+    hipStream_t s = c->stream;
+    VjpArgs a;
+    a.nele = g.nele; a.nX = g.nX; a.kind = g.kind; a.idx = g.d_idx; a.uidx = g.d_uidx; a.nodes = g.d_nodes;
+    a.xyz = c->d_xyz; a.o2 = g.d_o2; a.sec = g.d_sec;
+    for (int d = 0; d < 3; ++d) a.X[d] = c->d_X[d];
+    a.U = c->d_U; a.lam = c->d_lam; a.t = c->t; a.Kc = nullptr;
+    a.g = c->d_gbuf + g.rbase; a.gstride = c->nR;
+    a.gu = g.nU ? c->d_gubuf + g.ubase : nullptr;
+    a.flag = c->d_flag; a.gtag = gtag;
+    const bool beam = g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U;
+    const bool bar = g.kind == MH_KIND_BAR || g.kind == MH_KIND_BAR_U;
+    const bool u = g.kind == MH_KIND_BEAM_U || g.kind == MH_KIND_BAR_U;
+    if (beam && nd == 1) {
+        const dim3 gr((unsigned)((g.nele + kBeamBlock - 1) / kBeamBlock));
+        if (u) hipLaunchKernelGGL((beam_vjp_static_kernel<true>), gr, dim3(kBeamBlock), 0, s, a);
+        else hipLaunchKernelGGL((beam_vjp_static_kernel<false>), gr, dim3(kBeamBlock), 0, s, a);
+    } else if (beam) {
+        // value pass with unit scales: the unscaled compact static part of the frozen-load Π
+        BeamDynArgs b;
+        b.nele = g.nele; b.idx = g.d_idx; b.uidx = g.d_uidx; b.nodes = g.d_nodes; b.xyz = c->d_xyz; b.o2 = g.d_o2;
+        b.sec = g.d_sec;
+        for (int d = 0; d < 3; ++d) b.X[d] = c->d_X[d];
+        b.U = c->d_U;
+        for (int i = 0; i < 6; ++i) b.sA[i] = 1.;
+        for (int i = 0; i < 18; ++i) b.sT[i] = 1.;
+        for (int i = 0; i < 21; ++i) b.sS[i] = 1.;
+        for (int i = 0; i < 12; ++i) { b.sR[i] = 1.; b.scale[i] = 1.; }
+        for (int i = 0; i < 6; ++i) b.c[i] = 0.;
+        b.Kc = c->d_Kc + g.kcbase; b.K = nullptr; b.R = nullptr; b.dR = nullptr;
+        b.flag = c->d_flag; b.gtag = gtag;
+        const int64_t ntile = (g.nele + kBeamBlock - 1) / kBeamBlock;
+        b.per_xcd = (ntile + kXCD - 1) / kXCD;
+        const dim3 grv((unsigned)(b.per_xcd * kXCD));
+        if (nd == 2) {
+            if (u) hipLaunchKernelGGL((beam_dyn_value_kernel<2, true>), grv, dim3(kBeamBlock), 0, s, b);
+            else hipLaunchKernelGGL((beam_dyn_value_kernel<2, false>), grv, dim3(kBeamBlock), 0, s, b);
+        } else {
+            if (u) hipLaunchKernelGGL((beam_dyn_value_kernel<3, true>), grv, dim3(kBeamBlock), 0, s, b);
+            else hipLaunchKernelGGL((beam_dyn_value_kernel<3, false>), grv, dim3(kBeamBlock), 0, s, b);
+        }
+        a.Kc = b.Kc;
+        const int64_t nt = (g.nele + kDirElems - 1) / kDirElems;
+        a.per_xcd = (nt + kXCD - 1) / kXCD;
+        const dim3 gr((unsigned)(a.per_xcd * kXCD));
+        if (nd == 2) {
+            if (u) hipLaunchKernelGGL((beam_dyn_vjp_kernel<2, true>), gr, dim3(kDirBlock), 0, s, a);
+            else hipLaunchKernelGGL((beam_dyn_vjp_kernel<2, false>), gr, dim3(kDirBlock), 0, s, a);
+        } else {
+            if (u) hipLaunchKernelGGL((beam_dyn_vjp_kernel<3, true>), gr, dim3(kDirBlock), 0, s, a);
+            else hipLaunchKernelGGL((beam_dyn_vjp_kernel<3, false>), gr, dim3(kDirBlock), 0, s, a);
+        }
+    } else if (bar) {
+        const dim3 gr((unsigned)((g.nele + kBarBlock - 1) / kBarBlock));
+        if (nd == 1) { if (u) hipLaunchKernelGGL((bar_vjp_kernel<1, true>), gr, dim3(kBarBlock), 0, s, a);
+                       else hipLaunchKernelGGL((bar_vjp_kernel<1, false>), gr, dim3(kBarBlock), 0, s, a); }
+        else if (nd == 2) { if (u) hipLaunchKernelGGL((bar_vjp_kernel<2, true>), gr, dim3(kBarBlock), 0, s, a);
+                            else hipLaunchKernelGGL((bar_vjp_kernel<2, false>), gr, dim3(kBarBlock), 0, s, a); }
+        else { if (u) hipLaunchKernelGGL((bar_vjp_kernel<3, true>), gr, dim3(kBarBlock), 0, s, a);
+               else hipLaunchKernelGGL((bar_vjp_kernel<3, false>), gr, dim3(kBarBlock), 0, s, a); }
+    } else {
+        hipLaunchKernelGGL(point_vjp_kernel, dim3(grid(g.nele)), dim3(kBlock), 0, s, a, nd);
+    }
+    return check_launch(c, "adjoint kernels");
+}
+
 // ============================================================================ C ABI
 struct mh_ctx : Ctx {};
 
@@ -833,9 +1080,11 @@ int mh_prepare_sweepx(mh_ctx* c, int OX, int64_t* nnz_out) {
     free_prepared(c);
     // contribution counts and buffer offsets, in eletype order. Static beams store the
     // compact tangent; dynamic beams (K + a₁C + b₁M is not symmetric) the full block.
-    int64_t nK = 0, nP = 0, nR = 0, nKc = 0;
+    int64_t nK = 0, nP = 0, nR = 0, nKc = 0, nRU = 0;
     bool coded = false;
     for (auto& g : c->groups) {
+        g.ubase = nRU;
+        nRU += g.nele * g.nU;
         const bool beam = g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U;
         g.compact = beam && OX == 0;
         coded = coded || g.compact;
@@ -851,7 +1100,7 @@ int mh_prepare_sweepx(mh_ctx* c, int OX, int64_t* nnz_out) {
     if (nP >= INT32_MAX || nK >= klim || nR >= (int64_t)UINT32_MAX) {
         c->err = "more than 2^31 contributions or too many element outputs for 32-bit maps"; return MH_ERR_SIZE;
     }
-    c->OX = OX; c->coded = coded;
+    c->OX = OX; c->coded = coded; c->nRU = nRU;
     c->nK = nK; c->nP = nP; c->nR = nR;
     const int64_t nX = c->nX;
     hipStream_t s = c->stream;
@@ -936,6 +1185,31 @@ int mh_prepare_sweepx(mh_ctx* c, int OX, int64_t* nnz_out) {
     if (int r = check_launch(c, "rhs pattern")) return r;
     HIPCHK(c, hipStreamSynchronize(s));
     dfree(rkin); dfree(rkout); dfree(rvin);
+    // ---- U-dof map (gradients ∂L/∂U of the adjoint), same construction over the U class
+    if (nRU > 0) {
+        HIPCHK(c, hipMalloc(&rkin, sizeof(unsigned int) * nRU));
+        HIPCHK(c, hipMalloc(&rkout, sizeof(unsigned int) * nRU));
+        HIPCHK(c, hipMalloc(&rvin, sizeof(uint32_t) * nRU));
+        HIPCHK(c, hipMalloc(&c->d_usrc, sizeof(uint32_t) * nRU));
+        HIPCHK(c, hipMalloc(&c->d_useg, sizeof(int32_t) * (c->nU + 1)));
+        for (auto& g : c->groups) {
+            if (g.nU == 0 || g.nele == 0) continue;
+            int64_t np = g.nele * g.nU;
+            hipLaunchKernelGGL(gen_rpairs_kernel, dim3(grid(np)), dim3(kBlock), 0, s, g.nele, g.nU, g.d_uidx, g.ubase,
+                               rkin + g.ubase, rvin + g.ubase);
+        }
+        int ubit = 1;
+        while (ubit < 32 && (unsigned long long)c->nU > (1ull << ubit)) ++ubit;
+        tmp_bytes = 0;
+        HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, rkin, rkout, rvin, c->d_usrc, (int)nRU, 0, ubit, s));
+        HIPCHK(c, hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
+        HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, rkin, rkout, rvin, c->d_usrc, (int)nRU, 0, ubit, s));
+        dfree(tmp);
+        hipLaunchKernelGGL(rseg_kernel, dim3(grid(nRU + 1)), dim3(kBlock), 0, s, nRU, rkout, c->nU, c->d_useg);
+        if (int r = check_launch(c, "U pattern")) return r;
+        HIPCHK(c, hipStreamSynchronize(s));
+        dfree(rkin); dfree(rkout); dfree(rvin);
+    }
     c->prepared = true;
     if (nnz_out) *nnz_out = nnz;
     return MH_OK;
@@ -1083,6 +1357,85 @@ int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err*
         if (err) { err->code = MH_ERR_NAN; err->igroup = gi; err->iele = e; }
         return MH_ERR_NAN;
     }
+    return MH_OK;
+}
+
+int mh_set_lambda(mh_ctx* c, const double* lam) {
+    if (!c || !lam || !c->prepared) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->d_lam) HIPCHK(c, hipMalloc(&c->d_lam, sizeof(double) * std::max<int64_t>(c->nX, 1)));
+    if (c->nX) HIPCHK(c, hipMemcpyAsync(c->d_lam, lam, sizeof(double) * c->nX, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MH_OK;
+}
+
+int mh_vjp_lagrangian(mh_ctx* c, int OX, mh_err* err) {
+    if (err) { err->code = MH_OK; err->igroup = -1; err->iele = -1; }
+    if (!c || !c->prepared || !c->d_lam || OX < 0 || OX > 2) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int nd = OX + 1;
+    if (!c->d_gbuf) {
+        HIPCHK(c, hipMalloc(&c->d_gbuf, sizeof(double) * std::max<int64_t>(3 * c->nR, 1)));
+        HIPCHK(c, hipMalloc(&c->d_gX, sizeof(double) * std::max<int64_t>(3 * c->nX, 1)));
+        HIPCHK(c, hipMalloc(&c->d_gubuf, sizeof(double) * std::max<int64_t>(c->nRU, 1)));
+        HIPCHK(c, hipMalloc(&c->d_gU, sizeof(double) * std::max<int64_t>(c->nU, 1)));
+    }
+    if (OX > 0 && !c->d_Kc) {
+        int64_t nKc = 0;
+        for (auto& g : c->groups)
+            if (g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U) { g.kcbase = nKc; nKc += g.nele * mh::kCompact; }
+        if (nKc) HIPCHK(c, hipMalloc(&c->d_Kc, sizeof(double) * nKc));
+    }
+    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, s, c->d_flag, ~0ull);
+    HIPCHK(c, hipEventRecord(c->ev[0], s));
+    for (size_t gi = 0; gi < c->groups.size(); ++gi) {
+        const Group& g = c->groups[gi];
+        if (g.nX == 0 || g.nele == 0) continue;
+        if (int r = launch_vjp(c, g, nd, (unsigned long long)gi << 40)) return r;
+    }
+    HIPCHK(c, hipEventRecord(c->ev[1], s));
+    if (c->nX) {
+        const int64_t per = ((c->nX + kTile - 1) / kTile + kXCD - 1) / kXCD;
+        for (int d = 0; d < nd; ++d)
+            hipLaunchKernelGGL((scatter_kernel<false, false>), dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nX, per,
+                               c->d_rseg, c->d_rsrc, c->d_gbuf + d * c->nR, (const double*)nullptr, c->d_gX + d * c->nX);
+    }
+    HIPCHK(c, hipEventRecord(c->ev[2], s));
+    if (c->nU) {
+        if (c->nRU) {
+            const int64_t per = ((c->nU + kTile - 1) / kTile + kXCD - 1) / kXCD;
+            hipLaunchKernelGGL((scatter_kernel<false, false>), dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nU, per,
+                               c->d_useg, c->d_usrc, c->d_gubuf, (const double*)nullptr, c->d_gU);
+        } else {
+            HIPCHK(c, hipMemsetAsync(c->d_gU, 0, sizeof(double) * c->nU, s));
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev[3], s));
+    if (int r = check_launch(c, "adjoint scatter")) return r;
+    unsigned long long flag = 0;
+    HIPCHK(c, hipMemcpyAsync(&flag, c->d_flag, sizeof(flag), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    c->timed = true;
+    c->vjp_nd = nd;
+    if (flag != ~0ull) {
+        int gi = (int)(flag >> 40);
+        int64_t e = (int64_t)(flag & ((1ull << 40) - 1));
+        c->err = "element " + std::to_string(e + 1) + " of element type " + std::to_string(gi + 1) + " returned NaN";
+        if (err) { err->code = MH_ERR_NAN; err->igroup = gi; err->iele = e; }
+        return MH_ERR_NAN;
+    }
+    return MH_OK;
+}
+
+int mh_get_vjp(mh_ctx* c, double* gX, double* gU, double* gA) {
+    if (!c || !c->prepared || c->vjp_nd == 0) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (gX && c->nX)
+        HIPCHK(c, hipMemcpyAsync(gX, c->d_gX, sizeof(double) * c->nX * c->vjp_nd, hipMemcpyDeviceToHost, c->stream));
+    if (gU && c->nU) HIPCHK(c, hipMemcpyAsync(gU, c->d_gU, sizeof(double) * c->nU, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (gA) for (int64_t i = 0; i < c->nA; ++i) gA[i] = 0.;   // no element kind here depends on A
     return MH_OK;
 }
 

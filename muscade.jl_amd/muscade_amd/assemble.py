@@ -144,6 +144,27 @@ class AssemblySweepX:
             self.ctx.check(lib.mh_get_nzval(self.ctx.h, dptr(self.nzval)), "nzval")
         return self.Lλ, self.nzval
 
+    def vjp(self, state, Λ, OX=None, dbg=None):
+        """Gradients of L = Σ_e Λ_eᵀ R_e(X,U,A,t) (getlagrangian, Assemble.jl:707-736) at `state`:
+        returns ([∂L/∂X, ∂L/∂X′, …] up to order OX, ∂L/∂U, ∂L/∂A), unscaled model coordinates.
+        Computed by the device adjoint kernels (include/muscade_hip.h, mh_vjp_lagrangian)."""
+        OX = self.OX if OX is None else OX
+        self.set_state(state, state.time)
+        self.ctx.check(lib.mh_set_lambda(self.ctx.h, dptr(as_f64(Λ))), "set Λ")
+        err = _lib.mh_err()
+        r = lib.mh_vjp_lagrangian(self.ctx.h, OX, ctypes.byref(err))
+        if r == _lib.MH_ERR_NAN:
+            E = self.model.eletypes[err.igroup]
+            raise MuscadeError(f"adjoint of {E.key[0]} returned NaN",
+                               dict(dbg or {}, ieletyp=err.igroup + 1, iele=err.iele + 1, t=state.time))
+        self.ctx.check(r, "adjoint")
+        gX = np.zeros((OX + 1) * self.nX)
+        gU = np.zeros(self.model.ndof["U"])
+        gA = np.zeros(self.model.ndof["A"])
+        self.ctx.check(lib.mh_get_vjp(self.ctx.h, dptr(gX), dptr(gU) if gU.size else None, dptr(gA) if gA.size else None),
+                       "adjoint results")
+        return [gX[d * self.nX:(d + 1) * self.nX] for d in range(OX + 1)], gU, gA
+
     def Lλx(self):
         """out.Lλx as a scipy CSC matrix sharing the pattern."""
         return sp.csc_matrix((self.nzval.copy(), self._indices0, self._indptr0), shape=(self.nX, self.nX))
