@@ -9,7 +9,10 @@ Lλ and Lλx.nzval, from a device-resident state to device-resident outputs.
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
 Prints one JSON line. Kernel times come from HIP events recorded by the
-library on the stream it launches on.
+library on the stream it launches on. Two secondary measurements ride along
+in the same line (skip them with --no-dynamic / --no-adjoint): the dynamic
+SweepX{2} assembly of 256 catenary chains ("dynamic") and batched vs plain
+adjoint gradients of 512 uneven cantilevers ("adjoint").
 """
 import argparse
 import ctypes
@@ -130,6 +133,75 @@ def dynamic_bench(args):
             "nbeam": nbeam, "nX": out.nX, "nnz": out.nnz, "prepare_s": t_prep, **res}
 
 
+def adjoint_bench(args):
+    """Secondary measurement: batched adjoint gradients vs the plain per-model path.
+    The same `--models` uneven cantilevers (EulerBeam3D{true} with 8 U-loads each) either
+    joined into one model (one adjoint launch per element kind for all of them) or one
+    context per model called in a loop. A gradient set is (∂L/∂X, ∂L/∂U) of
+    L = Σ_e Λ_eᵀ R_e for one model, computed on the device from resident X, U, Λ."""
+    import muscade_amd as mc
+    from muscade_amd import _lib
+    from muscade_amd.workloads import cantilever_batch, cantilever_batch_specs
+    specs = cantilever_batch_specs(args.models, args.model_min, args.model_max)
+    t0 = time.perf_counter()
+    [(mj, sj, lj)] = cantilever_batch(specs)
+    sep = cantilever_batch(specs, joined=False)
+    nbeam = mj.eletypes[0].nele
+    log(f"adjoint batch: {len(specs)} models, {nbeam} beams ({time.perf_counter() - t0:.1f}s)")
+    err = _lib.mh_err()
+    outj = mc.AssemblySweepX(0, mj, sj.dis)
+    outj.set_state(sj, 0.)
+    outj.ctx.check(_lib.lib.mh_set_lambda(outj.ctx.h, _lib.dptr(lj)), "Λ")
+    plain = []
+    for m, s_, l_ in sep:
+        o = mc.AssemblySweepX(0, m, s_.dis)
+        o.set_state(s_, 0.)
+        o.ctx.check(_lib.lib.mh_set_lambda(o.ctx.h, _lib.dptr(l_)), "Λ")
+        plain.append(o)
+
+    def batched():
+        if _lib.lib.mh_vjp_lagrangian(outj.ctx.h, 0, ctypes.byref(err)) != 0:
+            raise RuntimeError(_lib.lib.mh_last_error(outj.ctx.h).decode())
+
+    def per_model():
+        for o in plain:
+            if _lib.lib.mh_vjp_lagrangian(o.ctx.h, 0, ctypes.byref(err)) != 0:
+                raise RuntimeError(_lib.lib.mh_last_error(o.ctx.h).decode())
+
+    res = {}
+    for name, fn in (("batched", batched), ("plain", per_model)):
+        for _ in range(args.warmup):
+            fn()
+        n = max(2, args.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        el = (time.perf_counter() - t0) / n
+        res[name] = {"grads_per_s": len(specs) / el, "ms_per_batch": el * 1e3}
+        log(f"adjoint {name}: {el * 1e3:.3f} ms for {len(specs)} gradient sets")
+    ms = np.zeros(3)
+    batched()
+    _lib.lib.mh_last_timings(outj.ctx.h, _lib.dptr(ms))
+    # the batched gradients of a few models equal their plain ones
+    gXj = np.zeros(mj.ndof["X"])
+    _lib.lib.mh_get_vjp(outj.ctx.h, _lib.dptr(gXj), None, None)
+    xo, worst = 0, 0.
+    for k in range(min(4, len(specs))):
+        m, s_, l_ = sep[k]
+        g = np.zeros(m.ndof["X"])
+        plain[k].ctx.check(_lib.lib.mh_get_vjp(plain[k].ctx.h, _lib.dptr(g), None, None), "vjp")
+        nd = 6 * (specs[k]["ne"] + 1)
+        worst = max(worst, float(np.abs(gXj[xo:xo + nd] - g[:nd]).max()))
+        xo += nd
+    return {"workload": f"{len(specs)} cantilevers of U{{{args.model_min}..{args.model_max}}} EulerBeam3D{{true}} "
+                        f"({nbeam} beams), 8 U-loads each; gradient set = dL/dX, dL/dU of L = sum Lambda.R",
+            "metric": "batched adjoint grads/s vs plain",
+            "batched": res["batched"], "plain": res["plain"],
+            "speedup": res["batched"]["grads_per_s"] / res["plain"]["grads_per_s"],
+            "batched_kernels_ms": dict(zip(("elements", "scatter_X", "scatter_U"), ms.tolist())),
+            "batched_vs_plain_max_abs_diff": worst}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,8 +213,13 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=40000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="run only the counter-calibration copy kernel")
-    ap.add_argument("--dynamic", action="store_true", help="also measure the dynamic (SweepX{2}) assembly")
+    ap.add_argument("--no-dynamic", action="store_true", help="skip the dynamic (SweepX{2}) assembly measurement")
     ap.add_argument("--only-dynamic", action="store_true", help="measure only the dynamic assembly")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the batched vs plain adjoint measurement")
+    ap.add_argument("--only-adjoint", action="store_true", help="measure only the batched adjoint gradients")
+    ap.add_argument("--models", type=int, default=512)
+    ap.add_argument("--model-min", type=int, default=500)
+    ap.add_argument("--model-max", type=int, default=20000)
     ap.add_argument("--chains", type=int, default=256)
     ap.add_argument("--chain-min", type=int, default=1000)
     ap.add_argument("--chain-max", type=int, default=50000)
@@ -157,6 +234,9 @@ def main():
 
     if args.only_dynamic:
         print(json.dumps({"dynamic": dynamic_bench(args)}))
+        return
+    if args.only_adjoint:
+        print(json.dumps({"adjoint": adjoint_bench(args)}))
         return
     import muscade_amd as mc
     from muscade_amd import _lib
@@ -248,9 +328,11 @@ def main():
         "prepare_s": t_prep,
         "cpu_baseline": cpu,
     }
-    if args.dynamic:
-        del out
+    del out
+    if not args.no_dynamic:
         line["dynamic"] = dynamic_bench(args)
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_bench(args)
     print(json.dumps(line))
 
 

@@ -677,6 +677,31 @@ __global__ __launch_bounds__(kTile) void scatter_kernel(int64_t nseg, int64_t pe
     if (k < nseg) out[k] = acc;
 }
 
+// Long segments (many elements share a dof, e.g. a U-node loading a whole stretch of
+// beams): one workgroup per segment. Lane t sums a contiguous sub-range sequentially, then
+// lane 0 adds the kTile partials in lane order — a fixed, deterministic order.
+__global__ __launch_bounds__(kTile) void scatter_long_kernel(int64_t nseg, const int32_t* __restrict__ seg,
+                                                             const uint32_t* __restrict__ src,
+                                                             const double* __restrict__ buf, double* __restrict__ out) {
+    __shared__ double part[kTile];
+    const int64_t k = blockIdx.x;
+    if (k >= nseg) return;
+    const int t = threadIdx.x;
+    const int32_t p0 = seg[k], p1 = seg[k + 1];
+    const int32_t len = p1 - p0;
+    const int32_t per = (len + kTile - 1) / kTile;
+    const int32_t a = p0 + t * per, b = (a + per < p1 ? a + per : p1);
+    double acc = 0.;
+    for (int32_t q = a; q < b; ++q) acc += buf[src[q]];
+    part[t] = acc;
+    __syncthreads();
+    if (t == 0) {
+        double s = 0.;
+        for (int i = 0; i < kTile; ++i) s += part[i];
+        out[k] = s;
+    }
+}
+
 __global__ void fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
 
 // Plain streaming copy of n doubles: the reference kernel for HBM counter calibration.
@@ -1403,7 +1428,10 @@ int mh_vjp_lagrangian(mh_ctx* c, int OX, mh_err* err) {
     }
     HIPCHK(c, hipEventRecord(c->ev[2], s));
     if (c->nU) {
-        if (c->nRU) {
+        if (c->nRU > 16 * c->nU) {   // long segments: a workgroup per U-dof
+            hipLaunchKernelGGL(scatter_long_kernel, dim3((unsigned)c->nU), dim3(kTile), 0, s, c->nU, c->d_useg, c->d_usrc,
+                               c->d_gubuf, c->d_gU);
+        } else if (c->nRU) {
             const int64_t per = ((c->nU + kTile - 1) / kTile + kXCD - 1) / kXCD;
             hipLaunchKernelGGL((scatter_kernel<false, false>), dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nU, per,
                                c->d_useg, c->d_usrc, c->d_gubuf, (const double*)nullptr, c->d_gU);

@@ -73,8 +73,8 @@ class Model:
         self._doftypindex = {}
         self.scaleΛ = 1.0
         self.locked = False
-        self._regkeys = {c: np.zeros(0, np.int64) for c in CLASSES}
-        self._regids = {c: np.zeros(0, np.int64) for c in CLASSES}
+        # per class and dof type: node → dof number (0 = not numbered yet), grown on demand
+        self._lut = {c: {} for c in CLASSES}
         self._dofnode = {c: [] for c in CLASSES}
         self._doftypof = {c: [] for c in CLASSES}
         self.ndof = {c: 0 for c in CLASSES}
@@ -103,43 +103,50 @@ class Model:
             self.doftyp.append([cls, field, 1.0])
         return self._doftypindex[k]
 
+    def _lut_for(self, c, t, size):
+        a = self._lut[c].get(t)
+        if a is None or a.size < size:
+            n = max(size, 2 * (a.size if a is not None else 0), 1024)
+            b = np.zeros(n, np.int64)
+            if a is not None:
+                b[:a.size] = a
+            self._lut[c][t] = a = b
+        return a
+
     def _number(self, E, nodes):
         """First-encounter numbering of the dofs of a batch of elements (ModelDescription.jl:226-249)."""
         nele = nodes.shape[0]
         ndof = len(E.inod)
         idt = np.array([self._idoftyp(c, f) for c, f in zip(E.dofclass, E.field)], np.int64)
         dofid = np.zeros((nele, ndof), np.int64)
+        nmax = int(nodes.max()) + 1 if nodes.size else 1
         for c in CLASSES:
             ks = E.class_columns(c)
             if not ks or nele == 0:
                 continue
             cols = np.array([E.inod[k] - 1 for k in ks])
-            keys = (nodes[:, cols].astype(np.int64) << _SHIFT) | idt[ks][None, :]
-            flat = keys.ravel()   # element-major, doflist order within: traversal order
-            reg, rid = self._regkeys[c], self._regids[c]
-            pos = np.searchsorted(reg, flat)
-            found = np.zeros(flat.size, bool)
-            if reg.size:
-                posc = np.minimum(pos, reg.size - 1)
-                found = reg[posc] == flat
-            ids = np.empty(flat.size, np.int64)
-            ids[found] = rid[pos[found]]
-            new = flat[~found]
-            if new.size:
-                uk, first = np.unique(new, return_index=True)
+            nd = nodes[:, cols].astype(np.int64)             # nele × len(ks), element-major traversal
+            ids = np.empty(nd.shape, np.int64)
+            for q, k in enumerate(ks):
+                ids[:, q] = self._lut_for(c, int(idt[k]), nmax)[nd[:, q]]
+            flat_ids = ids.ravel()
+            new = flat_ids == 0
+            if new.any():
+                keys = ((nd << _SHIFT) | idt[ks][None, :]).ravel()[new]
+                uk, first = np.unique(keys, return_index=True)
                 order = np.argsort(first, kind="stable")
                 base = self.ndof[c]
                 newid = np.empty(uk.size, np.int64)
                 newid[order] = base + 1 + np.arange(uk.size)
-                ids[~found] = newid[np.searchsorted(uk, new)]
+                flat_ids[new] = newid[np.searchsorted(uk, keys)]
                 self.ndof[c] += uk.size
-                self._dofnode[c].append(uk[order] >> _SHIFT)
-                self._doftypof[c].append(uk[order] & ((1 << _SHIFT) - 1))
-                allk = np.concatenate([reg, uk])
-                alli = np.concatenate([rid, newid])
-                s = np.argsort(allk, kind="stable")
-                self._regkeys[c], self._regids[c] = allk[s], alli[s]
-            dofid[:, ks] = ids.reshape(nele, len(ks))
+                unode, utyp = uk >> _SHIFT, uk & ((1 << _SHIFT) - 1)
+                self._dofnode[c].append(unode[order])
+                self._doftypof[c].append(utyp[order])
+                for t in np.unique(utyp):
+                    m = utyp == t
+                    self._lut_for(c, int(t), nmax)[unode[m]] = newid[m]
+            dofid[:, ks] = flat_ids.reshape(nele, len(ks))
         return dofid
 
     def dofnode(self, c):

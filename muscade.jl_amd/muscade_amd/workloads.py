@@ -17,10 +17,18 @@ added mass, linear and quadratic drag and roll inertia; bottom node pinned by
 three Holds, a horizontal DofLoad at the top node; the state is the as-meshed
 catenary plus small random displacements, velocities and accelerations.
 256 chains with nmin=1000, nmax=50000 give about 6.5 million beams.
+
+cantilever_batch: the batched-adjoint case — many uneven cantilevers
+(n_e ~ U{nmin..nmax} EulerBeam3D{true} along x, h = 0.05 m, section stiffness
+× LogNormal(0, 0.1) per model), each clamped at its first node and loaded by
+8 distributed U-loads (one U-node per eighth of its length, 3 U-dofs each,
+N(0, 1) kN/m), with a state of small random displacements and rotations and a
+random adjoint vector Λ. The same models are built either one Model each (the
+plain path) or all into one Model as disconnected parts (the batch).
 """
 import numpy as np
 
-from .elements import BeamCrossSection, DofLoad, EulerBeam3D, Hold
+from .elements import BeamCrossSection, DofLoad, EulerBeam3D, EulerBeam3DU, Hold
 from .model import Model, addelement, addnode, initialize, setdof
 
 E_STEEL = 2.1e11
@@ -145,3 +153,74 @@ def catenary_chains(nchain=256, nmin=1000, nmax=50000, seed=1002, disp=1e-3, rot
     state.X[1][:] = rng.normal(0., vel, nX)
     state.X[2][:] = rng.normal(0., acc, nX)
     return model, state
+
+
+def cantilever_batch_specs(nmodel=512, nmin=500, nmax=20000, seed=1004):
+    """Per-model parameters of the batch: element count, stiffness factor, seeds."""
+    rng = np.random.default_rng(seed)
+    ne = rng.integers(nmin, nmax + 1, size=nmodel)
+    stiff = rng.lognormal(0., 0.1, size=nmodel)
+    return [dict(ne=int(ne[m]), stiff=float(stiff[m]), seed=int(seed * 1000 + m)) for m in range(nmodel)]
+
+
+def _add_cantilever_u(model, spec, h=0.05, yoff=0.):
+    """Add one cantilever of the batch to `model`; returns (first node, node count, nU-nodes)."""
+    n = spec["ne"]
+    x = np.arange(n + 1) * h
+    base = model.nnod
+    xyz = np.stack([x, np.full_like(x, yoff), np.zeros_like(x)], axis=1)
+    useg = np.minimum((np.arange(n) * 8) // n, 7)
+    uxyz = np.stack([(np.arange(8) + 0.5) * n * h / 8, np.full(8, yoff + 1.), np.zeros(8)], axis=1)
+    addnode(model, np.concatenate([xyz, uxyz]))
+    sec = base_section().values.copy()
+    sec[:4] *= spec["stiff"]
+    nodes = np.stack([base + 1 + np.arange(n), base + 2 + np.arange(n), base + n + 2 + useg], axis=1)
+    addelement(model, EulerBeam3DU, nodes, mat=sec, orient2=np.array([0., 1., 0.]))
+    return base + 1
+
+
+def _spec_vectors(spec):
+    """State and Λ of one cantilever in its own numbering: X = 6(n+1) node dofs (node-major,
+    t1..r3) then the 6 λ of its Holds; U = 8 U-nodes × 3."""
+    rng = np.random.default_rng(spec["seed"])
+    nd = 6 * (spec["ne"] + 1)
+    x = np.concatenate([rng.normal(0., 1e-3, nd), np.zeros(6)])
+    lam = rng.normal(0., 1., nd + 6)
+    u = rng.normal(0., 1e3, 24)
+    return x, lam, u
+
+
+def cantilever_batch(specs, joined=True):
+    """The batch as one Model (joined) or as one Model per spec. Returns a list of
+    (model, state, Λ); the Holds of a joined model come after all its beams, so its X-dofs
+    are the models' node dofs one model after the other, then the λ field by field."""
+    if joined:
+        model = Model("cantilever_batch")
+        firsts = [_add_cantilever_u(model, s, yoff=3. * m) for m, s in enumerate(specs)]
+        for fld in ("t1", "t2", "t3", "r1", "r2", "r3"):
+            addelement(model, Hold, np.array(firsts).reshape(-1, 1), field=fld)
+        state = initialize(model)
+        lam = np.zeros(model.ndof["X"])
+        nbeam = sum(6 * (s["ne"] + 1) for s in specs)
+        xo = 0
+        for m, s in enumerate(specs):
+            x, l, u = _spec_vectors(s)
+            nd = 6 * (s["ne"] + 1)
+            state.X[0][xo:xo + nd] = x[:nd]
+            lam[xo:xo + nd] = l[:nd]
+            lam[nbeam + m + len(specs) * np.arange(6)] = l[nd:]
+            state.U[0][24 * m:24 * m + 24] = u
+            xo += nd
+        return [(model, state, lam)]
+    out = []
+    for s in specs:
+        model = Model("cantilever")
+        f = _add_cantilever_u(model, s)
+        for fld in ("t1", "t2", "t3", "r1", "r2", "r3"):
+            addelement(model, Hold, np.array([[f]]), field=fld)
+        state = initialize(model)
+        x, lam, u = _spec_vectors(s)
+        state.X[0][:] = x
+        state.U[0][:] = u
+        out.append((model, state, lam))
+    return out

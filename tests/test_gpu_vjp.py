@@ -125,3 +125,22 @@ def test_vjp_is_deterministic(mc):
     a = [g.copy() for g in out.vjp(state, lam)[0]]
     b = out.vjp(state, lam)[0]
     assert all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def test_vjp_batched_equals_plain(mc):
+    """Uneven cantilevers with U-loads shared by whole stretches of beams (long U segments):
+    the gradients of the joined batch equal each model's own, bit for bit, and match the oracle."""
+    from muscade_amd.workloads import cantilever_batch, cantilever_batch_specs
+    specs = cantilever_batch_specs(6, 150, 700, seed=77)
+    [(mj, sj, lj)] = cantilever_batch(specs)
+    sep = cantilever_batch(specs, joined=False)
+    gXj, gUj, _ = mc.AssemblySweepX(0, mj, sj.dis).vjp(sj, lj)
+    xo = 0
+    for k, (m, s, l) in enumerate(sep):
+        gX, gU, _ = mc.AssemblySweepX(0, m, s.dis).vjp(s, l)
+        nd = 6 * (specs[k]["ne"] + 1)
+        assert np.array_equal(gXj[0][xo:xo + nd], gX[0][:nd])
+        assert np.array_equal(gUj[24 * k:24 * k + 24], gU)
+        xo += nd
+        if k < 2:
+            check(gX, gU, oracle_vjp(m, s, l, 0, None))
