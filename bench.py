@@ -251,6 +251,10 @@ def main():
     out = mc.AssemblySweepX(0, model, state.dis)
     t_prep = time.perf_counter() - t0
     log(f"prepare (device pattern, nnz={out.nnz}): {t_prep:.2f}s")
+    tv, tms = ctypes.c_int(), (ctypes.c_float * 8)()
+    _lib.lib.mh_get_tuning(out.ctx.h, ctypes.byref(tv), tms)
+    tuning = {"scatter_variant": tv.value, "scatter_variant_ms": [round(float(x), 4) for x in tms]}
+    log(f"scatter tuner: {tuning}")
     out.set_state(state, 0.0)
     h = out.ctx.h
     c6 = out.c.array()
@@ -326,6 +330,7 @@ def main():
         "roofline_eval_hbm": rl_eval_hbm,
         "roofline_scatter": rl_scatter,
         "prepare_s": t_prep,
+        "tuning": tuning,
         "cpu_baseline": cpu,
     }
     del out

@@ -93,6 +93,8 @@ struct Ctx {
     double* d_gubuf = nullptr;   // nRU
     double* d_gU = nullptr;      // nU
     int vjp_nd = 0;              // derivative orders of the last VJP
+    int scatter_variant = 0;     // nz scatter shape picked at prepare (launch_scatter)
+    float scatter_ms[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timed = false;
 };
@@ -643,38 +645,74 @@ __device__ __forceinline__ double contrib(const double* __restrict__ buf, uint32
 constexpr int kTile = 256;
 constexpr int kChunk = 1024;
 // CODED: src carries multiplier codes (compact static beams); otherwise src is the
-// plain 32-bit index.
-template <bool STEP, bool CODED>
+// plain 32-bit index. SPL segments per lane (lane t owns k0 + t + j·kTile, so the
+// writes stay coalesced) and CH contributions per LDS chunk are the shape the
+// prepare-time tuner picks (scatter_variant).
+template <bool STEP, bool CODED, int SPL = 1, int CH = kChunk>
 __global__ __launch_bounds__(kTile) void scatter_kernel(int64_t nseg, int64_t per_xcd, const int32_t* __restrict__ seg,
                                                         const uint32_t* __restrict__ src, const double* __restrict__ buf,
                                                         const double* __restrict__ dbuf, double* __restrict__ out) {
-    __shared__ double vals[kChunk];
-    __shared__ double dvals[STEP ? kChunk : 1];
-    const int64_t k0 = xcd_tile(per_xcd) * kTile;
+    __shared__ double vals[CH];
+    __shared__ double dvals[STEP ? CH : 1];
+    constexpr int kSeg = kTile * SPL;
+    const int64_t k0 = xcd_tile(per_xcd) * kSeg;
     if (k0 >= nseg) return;   // the whole workgroup leaves together
     const int t = threadIdx.x;
-    const int64_t k = k0 + t;
-    const int64_t klast = (k0 + kTile < nseg ? k0 + kTile : nseg);
+    const int64_t klast = (k0 + kSeg < nseg ? k0 + kSeg : nseg);
     const int32_t pbeg = seg[k0], pend = seg[klast];
-    int32_t p0 = 0, p1 = 0;
-    if (k < nseg) { p0 = seg[k]; p1 = seg[k + 1]; }
-    double acc = 0.;
-    for (int32_t c = pbeg; c < pend; c += kChunk) {
-        const int32_t cend = (c + kChunk < pend ? c + kChunk : pend);
+    int32_t p0[SPL], p1[SPL];
+    double acc[SPL];
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+        const int64_t k = k0 + t + j * kTile;
+        p0[j] = p1[j] = 0;
+        if (k < nseg) { p0[j] = seg[k]; p1[j] = seg[k + 1]; }
+        acc[j] = 0.;
+    }
+    for (int32_t c = pbeg; c < pend; c += CH) {
+        const int32_t cend = (c + CH < pend ? c + CH : pend);
         for (int32_t q = c + t; q < cend; q += kTile) {
             const uint32_t sq = src[q];
             vals[q - c] = CODED ? contrib(buf, sq) : buf[sq];
             if (STEP) dvals[q - c] = dbuf[CODED ? (sq & kIdxMask) : sq];
         }
         __syncthreads();
-        const int32_t a = p0 > c ? p0 : c, b = p1 < cend ? p1 : cend;
-        for (int32_t q = a; q < b; ++q) {
-            acc += vals[q - c];
-            if (STEP) acc -= dvals[q - c];
+#pragma unroll
+        for (int j = 0; j < SPL; ++j) {
+            const int32_t a = p0[j] > c ? p0[j] : c, b = p1[j] < cend ? p1[j] : cend;
+            for (int32_t q = a; q < b; ++q) {
+                acc[j] += vals[q - c];
+                if (STEP) acc[j] -= dvals[q - c];
+            }
         }
         __syncthreads();
     }
-    if (k < nseg) out[k] = acc;
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+        const int64_t k = k0 + t + j * kTile;
+        if (k < nseg) out[k] = acc[j];
+    }
+}
+// Launch one of the tuned shapes of the nz scatter: (segments per lane, LDS chunk).
+constexpr int kScatterVariants = 8;
+constexpr int kScatterSPL[kScatterVariants] = {1, 2, 2, 3, 4, 4, 6, 8};
+template <bool CODED>
+void launch_scatter(int variant, hipStream_t s, int64_t nseg, const int32_t* seg, const uint32_t* src, const double* buf,
+                    double* out) {
+    const int spl = kScatterSPL[variant];
+    const int64_t per = ((nseg + kTile * spl - 1) / (kTile * spl) + kXCD - 1) / kXCD;
+    const dim3 gr((unsigned)(per * kXCD));
+    const double* nul = nullptr;
+    switch (variant) {
+    case 0: hipLaunchKernelGGL((scatter_kernel<false, CODED, 1, 1024>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    case 1: hipLaunchKernelGGL((scatter_kernel<false, CODED, 2, 1024>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    case 2: hipLaunchKernelGGL((scatter_kernel<false, CODED, 2, 2048>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    case 3: hipLaunchKernelGGL((scatter_kernel<false, CODED, 3, 2048>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    case 4: hipLaunchKernelGGL((scatter_kernel<false, CODED, 4, 1024>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    case 5: hipLaunchKernelGGL((scatter_kernel<false, CODED, 4, 2048>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    case 6: hipLaunchKernelGGL((scatter_kernel<false, CODED, 6, 2048>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    default: hipLaunchKernelGGL((scatter_kernel<false, CODED, 8, 2048>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    }
 }
 
 // Long segments (many elements share a dof, e.g. a U-node loading a whole stretch of
@@ -1235,6 +1273,27 @@ int mh_prepare_sweepx(mh_ctx* c, int OX, int64_t* nnz_out) {
         HIPCHK(c, hipStreamSynchronize(s));
         dfree(rkin); dfree(rkout); dfree(rvin);
     }
+    // ---- tuner: time each shape of the nz scatter on these maps and keep the fastest.
+    // The sums are identical for every shape (each segment is summed in the same order).
+    if (nnz > 0) {
+        float best = 1e30f;
+        for (int v = 0; v < kScatterVariants; ++v) {
+            float tmin = 1e30f;
+            for (int rep = 0; rep < 3; ++rep) {
+                HIPCHK(c, hipEventRecord(c->ev[0], s));
+                if (coded) launch_scatter<true>(v, s, nnz, c->d_kseg, c->d_ksrc, c->d_Kbuf, c->d_nzval);
+                else launch_scatter<false>(v, s, nnz, c->d_kseg, c->d_ksrc, c->d_Kbuf, c->d_nzval);
+                HIPCHK(c, hipEventRecord(c->ev[1], s));
+                HIPCHK(c, hipEventSynchronize(c->ev[1]));
+                float ms = 0.f;
+                HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+                if (rep > 0 && ms < tmin) tmin = ms;
+            }
+            if (int r = check_launch(c, "scatter tuning")) return r;
+            c->scatter_ms[v] = tmin;
+            if (tmin < best) { best = tmin; c->scatter_variant = v; }
+        }
+    }
     c->prepared = true;
     if (nnz_out) *nnz_out = nnz;
     return MH_OK;
@@ -1352,12 +1411,9 @@ int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err*
     HIPCHK(c, hipEventRecord(c->ev[1], s));
     if (c->nnz) {
         const int64_t per = ((c->nnz + kTile - 1) / kTile + kXCD - 1) / kXCD;
-        if (c->coded)
-            hipLaunchKernelGGL((scatter_kernel<false, true>), dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nnz, per,
-                               c->d_kseg, c->d_ksrc, c->d_Kbuf, (const double*)nullptr, c->d_nzval);
-        else
-            hipLaunchKernelGGL((scatter_kernel<false, false>), dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nnz, per,
-                               c->d_kseg, c->d_ksrc, c->d_Kbuf, (const double*)nullptr, c->d_nzval);
+        (void)per;
+        if (c->coded) launch_scatter<true>(c->scatter_variant, s, c->nnz, c->d_kseg, c->d_ksrc, c->d_Kbuf, c->d_nzval);
+        else launch_scatter<false>(c->scatter_variant, s, c->nnz, c->d_kseg, c->d_ksrc, c->d_Kbuf, c->d_nzval);
     }
     HIPCHK(c, hipEventRecord(c->ev[2], s));
     if (c->nX) {
@@ -1538,6 +1594,13 @@ int mh_calib_copy(mh_ctx* c, int64_t n, double* ms) {
     HIPCHK(c, hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
     if (ms) *ms = t;
     dfree(a); dfree(b);
+    return MH_OK;
+}
+
+int mh_get_tuning(mh_ctx* c, int* variant, float* ms) {
+    if (!c || !c->prepared) return MH_ERR_ARG;
+    if (variant) *variant = c->scatter_variant;
+    if (ms) for (int v = 0; v < kScatterVariants; ++v) ms[v] = c->scatter_ms[v];
     return MH_OK;
 }
 

@@ -44,6 +44,7 @@ SIGNATURES = {
     "mh_set_lambda": (_I, [_P, _D]),
     "mh_vjp_lagrangian": (_I, [_P, _I, ctypes.POINTER(mh_err)]),
     "mh_get_vjp": (_I, [_P, _D, _D, _D]),
+    "mh_get_tuning": (_I, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)]),
     "mh_assemble_sweepx": (_I, [_P, _I, _I, _D, ctypes.POINTER(mh_err)]),
     "mh_get_rhs": (_I, [_P, _D]),
     "mh_get_nzval": (_I, [_P, _D]),
