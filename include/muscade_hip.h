@@ -133,6 +133,23 @@ int mh_device_outputs(mh_ctx* ctx, double** d_rhs, double** d_nzval);
  * checks, like diffed_residual): R (nX×nele) and ∂R/∂X·scale (nX²×nele). */
 int mh_get_element_blocks(mh_ctx* ctx, int igroup, double* R, double* K);
 
+/* ---- AssemblyDirect{OX,OU,IA}: one DirectXUA step (src/DirectXUA.jl:18-150) ----
+ * prepare(AssemblyDirect{OX,OU,IA},model,dis) (DirectXUA.jl:22-56) builds the class-pair
+ * patterns over the classes Λ, X, U (Λ and X share the X dofs): nnz4 = nnz of the patterns
+ * (Λ|X, Λ|X), (Λ|X, U), (U, Λ|X), (U, U), fetched 1-based with mh_get_direct_pattern
+ * (which = 0..3). Beams and bars contribute by the residual path (DirectXUA.jl:85-120):
+ * L1[Λ] += R·Δt, L2[Λ,β][1,j] += ∂R/∂β⁽ʲ⁾·scale.β·Δt, L2[β,Λ][j,1] += its transpose; Hold and
+ * DofLoad by the Lagrangian path of L = Λ∘₁R (:121-171), which needs Λ (mh_set_lambda).
+ * mission: 0 = :vectors (L1 only), 1 = :matrices. mh_get_direct returns L1 =
+ * [Λ (nX) | X (OX+1)·nX | U (OU+1)·nU] and L2 = the pairs (α,β) ∈ {Λ,X,U}² in row-major order
+ * without (Λ,Λ), each nder(α)×nder(β) blocks of nnz(α,β) values (nder = 1, OX+1, OU+1).
+ * Elements with user cost functions (DofCost) and A-dofs are not evaluated on the device. */
+int mh_set_group_scales(mh_ctx* ctx, int igroup, const double* scaleL, const double* scaleU);
+int mh_prepare_direct(mh_ctx* ctx, int OX, int OU, int IA, int64_t* nnz4);
+int mh_get_direct_pattern(mh_ctx* ctx, int which, int64_t* colptr, int64_t* rowval);
+int mh_assemble_direct(mh_ctx* ctx, int mission, double dt, mh_err* err);
+int mh_get_direct(mh_ctx* ctx, double* L1, double* L2);
+
 /* ---- measurement -----------------------------------------------------------
  * HIP-event times (ms) of the last assembly, measured on the context's stream:
  * ms[0] element evaluation (all groups), ms[1] nzval scatter, ms[2] rhs scatter. */

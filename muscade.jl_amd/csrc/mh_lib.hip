@@ -40,6 +40,8 @@ struct Group {
     int64_t nele = 0;
     int nX = 0, nU = 0;
     double scaleX[12];
+    double scaleL[12];           // scale.Λ (AssemblyDirect's Lagrangian path)
+    double scaleU[3];
     int32_t* d_idx = nullptr;    // nX × nele
     int32_t* d_uidx = nullptr;   // nU × nele
     int32_t* d_nodes = nullptr;  // 2 × nele
@@ -51,7 +53,19 @@ struct Group {
     int64_t kper = 0;            // Kbuf entries per element (45 for static beams, nX² otherwise)
     int64_t kcbase = 0;          // dynamic beams: offset of the compact static part in Kc
     int64_t ubase = 0;           // U-dof groups: offset of the per-element ∂L/∂U entries
+    int64_t kdbase = 0, kubase = 0;   // AssemblyDirect: offsets of the ∂R/∂X⁽ᵈ⁾ and ∂R/∂U blocks
     bool compact = false;
+};
+
+// One CSC pattern with its contribution maps (seg: first contribution of each nz; src:
+// element-block index of each contribution; srcT: the same for the transposed block).
+struct DMap {
+    int64_t nrow = 0, ncol = 0, nnz = 0, npair = 0;
+    int32_t* colptr = nullptr;
+    int32_t* rowval = nullptr;
+    int32_t* seg = nullptr;
+    uint32_t* src = nullptr;
+    uint32_t* srcT = nullptr;
 };
 
 struct Ctx {
@@ -93,6 +107,17 @@ struct Ctx {
     double* d_gubuf = nullptr;   // nRU
     double* d_gU = nullptr;      // nU
     int vjp_nd = 0;              // derivative orders of the last VJP
+    // AssemblyDirect{OX,OU,IA} (DirectXUA, one step)
+    bool direct = false;
+    int dOX = 0, dOU = 0, dIA = 0;
+    DMap dXX, dXU, dUX, dUU;     // patterns of (Λ|X, Λ|X), (Λ|X, U), (U, Λ|X), (U, U)
+    int64_t nKd = 0, nKU = 0;
+    double* d_Kd = nullptr;      // (OX+1) × nKd: per-element ∂R/∂X⁽ᵈ⁾ blocks (× scale of the column, × Δt)
+    double* d_KU = nullptr;      // nKU: per-element ∂R/∂U blocks
+    double* d_Rd = nullptr;      // nR: per-element L1[Λ] entries
+    double* d_L1x = nullptr;     // nR: per-element L1[X][1] entries (Lagrangian-path elements)
+    double* d_dL1 = nullptr;     // 2·nX: L1[Λ][1], L1[X][1]
+    double* d_dL2 = nullptr;     // 2(OX+1)·nnz(XX) + 2·nnz(XU): L2[Λ,X][1,d], L2[X,Λ][d,1], L2[Λ,U][1,1], L2[U,Λ][1,1]
     int scatter_variant = 0;     // nz scatter shape picked at prepare (launch_scatter)
     float scatter_ms[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -115,6 +140,12 @@ void free_prepared(Ctx* c) {
     dfree(c->d_colptr); dfree(c->d_rowval); dfree(c->d_kseg); dfree(c->d_ksrc); dfree(c->d_rseg); dfree(c->d_rsrc);
     dfree(c->d_useg); dfree(c->d_usrc); dfree(c->d_gbuf); dfree(c->d_gX); dfree(c->d_gubuf); dfree(c->d_gU);
     c->vjp_nd = 0;
+    for (DMap* M : {&c->dXX, &c->dXU, &c->dUX, &c->dUU}) {
+        dfree(M->colptr); dfree(M->rowval); dfree(M->seg); dfree(M->src); dfree(M->srcT);
+        *M = DMap();
+    }
+    dfree(c->d_Kd); dfree(c->d_KU); dfree(c->d_Rd); dfree(c->d_L1x); dfree(c->d_dL1); dfree(c->d_dL2);
+    c->direct = false;
     c->prepared = false;
 }
 void free_model(Ctx* c) {
@@ -621,6 +652,183 @@ __global__ __launch_bounds__(kBlock) void point_vjp_kernel(VjpArgs a, int nd) {
     }
 }
 
+// ---- AssemblyDirect: per-element blocks ∂R/∂X⁽ᵈ⁾, ∂R/∂U of one DirectXUA step ------
+// Residual path (beams, bars; DirectXUA.jl:85-120): R unscaled, partials × scale of the
+// variable (revariate{1}((;X,U),(;X=scale.X,U=scale.U))), all × Δt.
+struct DirectArgs {
+    int64_t nele, per_xcd;
+    int nX;
+    int kind;
+    const int32_t* idx;
+    const int32_t* uidx;
+    const int32_t* nodes;
+    const double* xyz;
+    const double* o2;
+    const double* sec;
+    const double* X[3];
+    const double* U;
+    const double* lam;
+    const double* vals;   // DofLoad values
+    double t, dt;
+    double sX[12], sU[3], sL[12];
+    const double* Kc;     // beams: unscaled compact static part (value pass)
+    double* Kd;           // Kd[d*nKd + kdbase + e*nX² + i + nX·k]
+    int64_t nKd;
+    double* KU;           // KU[kubase + e*nX*3 + i + nX·c]
+    double* R;            // per-element L1[Λ] (Rbuf layout)
+    double* L1x;          // per-element L1[X][1]
+    unsigned long long* flag;
+    unsigned long long gtag;
+};
+
+// Beams, 16 lanes per element: the X₀ pass of lane k gives column k of J_x, J_v (LDS) and
+// of ∂R/∂X₀; lanes then differentiate the loads along X′ₖ and X″ₖ for the other orders;
+// lane 12 forms ∂R/∂U = -Σ dL J_xᵀ.
+template <int ND, bool UDOF>
+__global__ __launch_bounds__(kDirBlock) void beam_direct_kernel(DirectArgs a) {
+    __shared__ double Jsh[kDirElems][12][15];
+    const int64_t e0 = xcd_tile(a.per_xcd) * kDirElems;
+    if (e0 >= a.nele) return;   // whole workgroup leaves together
+    const int el = threadIdx.x / kDirLanes, k = threadIdx.x % kDirLanes;
+    const int64_t e = e0 + el;
+    const int64_t n = a.nele;
+    const bool live = e < n && k <= 12;
+    mh::BeamGeo G;
+    mh::BeamDynSection D;
+    double X[ND][12], U[3];
+    const double one[12] = {1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1.};
+    mh::DynDir d0;
+    if (live) {
+        mh::BeamSection S;
+        BeamDynArgs b;
+        b.nele = a.nele; b.idx = a.idx; b.uidx = a.uidx; b.nodes = a.nodes; b.xyz = a.xyz; b.o2 = a.o2; b.sec = a.sec;
+        for (int d = 0; d < 3; ++d) b.X[d] = a.X[d];
+        b.U = a.U;
+        dyn_load<ND, UDOF>(b, e, G, S, D, X, U);
+        if (k < 12) {
+            const double c0[6] = {0., 0., 0., 0., 0., 0.};
+            mh::dyn_direction<ND>(G, D, X, U, one, c0, k, d0, 1.);
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) Jsh[el][k][3 * g + q] = d0.Jx[g][q];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) Jsh[el][k][12 + q] = d0.Jv[q];
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    int nan = 0;
+    if (k == 12) {
+        if (UDOF) {
+            double* dst = a.KU + e * 36;
+            for (int c = 0; c < 3; ++c)
+                for (int i = 0; i < 12; ++i) {
+                    double s = 0.;
+                    for (int g = 0; g < 4; ++g) s = s + Jsh[el][i][3 * g + c] * G.dL[g];
+                    dst[i + 12 * c] = ((-s) * a.sU[c]) * a.dt;
+                    nan |= dst[i + 12 * c] != dst[i + 12 * c];
+                }
+        }
+    } else {
+        for (int der = 0; der < ND; ++der) {
+            mh::DynDir d = d0;
+            if (der > 0) {
+                const double c[6] = {der == 1 ? 1. : 0., 0., 0., der == 2 ? 1. : 0., 0., 0.};
+                mh::dyn_direction<ND>(G, D, X, U, one, c, k, d, 0.);
+            }
+            double* dst = a.Kd + der * a.nKd + e * 144 + 12 * k;
+            for (int i = 0; i < 12; ++i) {
+                double v = mh::dyn_row(G, (const double(*)[3])(&Jsh[el][i][0]), &Jsh[el][i][12], d);
+                if (der == 0) {
+                    double m;
+                    const int slot = mh::compact_slot(i, k, &m);
+                    v = m * a.Kc[e * mh::kCompact + slot] + v;
+                }
+                dst[i] = (v * a.sX[k]) * a.dt;
+                nan |= dst[i] != dst[i];
+            }
+        }
+    }
+    if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+}
+
+template <int ND, bool UDOF>
+__global__ __launch_bounds__(kBarBlock) void bar_direct_kernel(DirectArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nele) return;
+    const int64_t n = a.nele;
+    double X[ND][6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int32_t q = a.idx[k * n + e];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) X[d][k] = a.X[d][q];
+    }
+    const int32_t n1 = a.nodes[e], n2 = a.nodes[n + e];
+    mh::v3 c1 = mh::mk(a.xyz[3 * (int64_t)n1], a.xyz[3 * (int64_t)n1 + 1], a.xyz[3 * (int64_t)n1 + 2]);
+    mh::v3 c2 = mh::mk(a.xyz[3 * (int64_t)n2], a.xyz[3 * (int64_t)n2 + 1], a.xyz[3 * (int64_t)n2 + 2]);
+    const mh::BarGeo G = mh::bar_geometry(c1, c2, a.sec[9 * n + e]);
+    mh::BarSection S;
+    S.EA = a.sec[e]; S.mu = a.sec[n + e]; S.w = a.sec[2 * n + e]; S.Cat = a.sec[3 * n + e]; S.Clt = a.sec[4 * n + e];
+    S.Cqt = a.sec[5 * n + e]; S.Can = a.sec[6 * n + e]; S.Cln = a.sec[7 * n + e]; S.Cqn = a.sec[8 * n + e];
+    double U[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) U[q] = UDOF ? a.U[a.uidx[q * n + e]] : 0.;
+    const double one[6] = {1., 1., 1., 1., 1., 1.};
+    double R[6], K[36];
+    int nan = 0;
+    for (int der = 0; der < ND; ++der) {
+        const double c[6] = {der == 1 ? 1. : 0., 0., 0., der == 2 ? 1. : 0., 0., 0.};
+        nan |= mh::bar_element<ND>(G, S, X, U, a.t, one, c, R, K, nullptr, der == 0 ? 1. : 0.);
+        double* dst = a.Kd + der * a.nKd + e * 36;
+        for (int k = 0; k < 6; ++k)
+            for (int i = 0; i < 6; ++i) dst[i + 6 * k] = (K[i + 6 * k] * a.sX[k]) * a.dt;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) a.R[i * n + e] = R[i] * a.dt;
+    if (UDOF) {
+        // ∂Rⱼ/∂U_q = -Σ_gp w ψⱼ [j mod 3 = q]
+        double* dst = a.KU + e * 18;
+        for (int q = 0; q < 3; ++q)
+            for (int j = 0; j < 6; ++j) {
+                double s = 0.;
+                if (j % 3 == q)
+                    for (int g = 0; g < 4; ++g) s = s + (j < 3 ? (-G.zgp[g] + 1. / 2.) : (G.zgp[g] + 1. / 2.)) * G.wgp[g];
+                dst[j + 6 * q] = ((-s) * a.sU[q]) * a.dt;
+            }
+    }
+    if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+}
+
+// Lagrangian path of Hold and DofLoad (L = Λ∘₁R, revariate{2} with scale.Λ, scale.X;
+// DirectXUA.jl:121-171): L1[Λ] = R·sΛ, L1[X] = (Λᵀ∂R/∂X)·sX, L2[Λ,X] = ∂R/∂X·sΛ·sX.
+__global__ __launch_bounds__(kBlock) void point_direct_kernel(DirectArgs a, int nd) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nele) return;
+    const int64_t n = a.nele;
+    const int nX = a.nX, per = nX * nX;
+    for (int der = 0; der < nd; ++der)
+        for (int q = 0; q < per; ++q) a.Kd[der * a.nKd + e * per + q] = 0.;
+    for (int i = 0; i < nX; ++i) { a.R[i * n + e] = 0.; a.L1x[i * n + e] = 0.; }
+    int nan = 0;
+    if (a.kind == MH_KIND_HOLD) {
+        const double x = a.X[0][a.idx[e]], lam = a.X[0][a.idx[n + e]];
+        const double Lx = a.lam[a.idx[e]], Ll = a.lam[a.idx[n + e]];
+        a.R[e] = (-lam * a.sL[0]) * a.dt;
+        a.R[n + e] = (-x * a.sL[1]) * a.dt;
+        a.L1x[e] = (-Ll * a.sX[0]) * a.dt;
+        a.L1x[n + e] = (-Lx * a.sX[1]) * a.dt;
+        a.Kd[e * 4 + 1] = ((-1. * a.sL[1]) * a.sX[0]) * a.dt;   // (i=1, k=0): ∂R₁/∂x
+        a.Kd[e * 4 + 2] = ((-1. * a.sL[0]) * a.sX[1]) * a.dt;   // (i=0, k=1): ∂R₀/∂λ
+        nan = (x != x) || (lam != lam);
+    } else if (a.kind == MH_KIND_DOFLOAD) {
+        a.R[e] = (-a.vals[e] * a.sL[0]) * a.dt;
+        nan = a.vals[e] != a.vals[e];
+    }
+    if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+}
+
 // Contribution source: (code << 29) | index; the code selects an exact multiplier.
 constexpr uint32_t kIdxMask = (1u << 29) - 1;
 __device__ __forceinline__ double contrib(const double* __restrict__ buf, uint32_t s) {
@@ -812,6 +1020,23 @@ __global__ __launch_bounds__(kBlock) void rseg_kernel(int64_t n, const unsigned 
     const int64_t k = p == n ? ndof : (int64_t)keys[p];
     for (int64_t cc = kprev + 1; cc <= k; ++cc) seg[cc] = (int32_t)p;
 }
+// Pairs of a rectangular class block (rows: nI element dofs through idxI, columns: nJ
+// through idxJ) in traversal order (iele, jeledof, ieledof); key = J·nrowdof + I, value =
+// the element-block entry (i + nI·j), or of the transposed block (j + nJ·i).
+__global__ __launch_bounds__(kBlock) void gen_pairs_rect_kernel(int64_t nele, int nI, const int32_t* idxI, int nJ,
+                                                                const int32_t* idxJ, int64_t nrowdof, int64_t kbase,
+                                                                int transpose, unsigned long long* keys, uint32_t* vals) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)nI * nJ;
+    if (q >= nele * per) return;
+    const int64_t e = q / per;
+    const int r = (int)(q - e * per);
+    const int j = r / nI, i = r - j * nI;
+    const unsigned long long J = (unsigned long long)idxJ[(int64_t)j * nele + e];
+    const unsigned long long I = (unsigned long long)idxI[(int64_t)i * nele + e];
+    keys[q] = J * (unsigned long long)nrowdof + I;
+    if (vals) vals[q] = (uint32_t)(kbase + e * per + (transpose ? (j + (int64_t)nJ * i) : (i + (int64_t)nI * j)));
+}
 // asm[2] dump: nz number (1-based) of every element entry, found in the CSC pattern.
 __global__ __launch_bounds__(kBlock) void asm_dump_kernel(int64_t nele, int nX, const int32_t* idx, const int32_t* colptr,
                                                           const int32_t* rowval, int32_t* out) {
@@ -1001,6 +1226,187 @@ static int upload_soa(Ctx* c, T** dst, const T* src, int n, int64_t nele) {
 }
 
 
+// The rhs map (asm[1] for the X dofs): contributions of every element dof into Lλ, in
+// traversal order, stably sorted by dof (Assemble.jl:340-357 with add_value!'s order).
+static int build_rmap(Ctx* c) {
+    hipStream_t s = c->stream;
+    const int64_t nR = c->nR, nX = c->nX;
+    size_t tmp_bytes = 0;
+    void* tmp = nullptr;
+    HIPCHK(c, hipMalloc(&c->d_rseg, sizeof(int32_t) * (nX + 1)));
+    unsigned int *rkin = nullptr, *rkout = nullptr;
+    uint32_t* rvin = nullptr;
+    HIPCHK(c, hipMalloc(&rkin, sizeof(unsigned int) * std::max<int64_t>(nR, 1)));
+    HIPCHK(c, hipMalloc(&rkout, sizeof(unsigned int) * std::max<int64_t>(nR, 1)));
+    HIPCHK(c, hipMalloc(&rvin, sizeof(uint32_t) * std::max<int64_t>(nR, 1)));
+    HIPCHK(c, hipMalloc(&c->d_rsrc, sizeof(uint32_t) * std::max<int64_t>(nR, 1)));
+    for (auto& g : c->groups) {
+        if (g.nX == 0 || g.nele == 0) continue;
+        int64_t np = g.nele * g.nX;
+        hipLaunchKernelGGL(gen_rpairs_kernel, dim3(grid(np)), dim3(kBlock), 0, s, g.nele, g.nX, g.d_idx, g.rbase,
+                           rkin + g.rbase, rvin + g.rbase);
+    }
+    int rbit = 1;
+    while (rbit < 32 && (unsigned long long)nX > (1ull << rbit)) ++rbit;
+    tmp_bytes = 0;
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, rkin, rkout, rvin, c->d_rsrc, (int)nR, 0, rbit, s));
+    HIPCHK(c, hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, rkin, rkout, rvin, c->d_rsrc, (int)nR, 0, rbit, s));
+    dfree(tmp);
+    hipLaunchKernelGGL(rseg_kernel, dim3(grid(nR + 1)), dim3(kBlock), 0, s, nR, rkout, nX, c->d_rseg);
+    if (int r = check_launch(c, "rhs pattern")) return r;
+    HIPCHK(c, hipStreamSynchronize(s));
+    dfree(rkin); dfree(rkout); dfree(rvin);
+    return MH_OK;
+}
+
+// One class-pair block of AssemblyDirect: asmmat!(asm[α,β], asm[α], asm[β]) on the device
+// (Assemble.jl:373-448) — pairs in traversal order, stable radix sort on (column, row),
+// run-length numbering; src maps every pair to its element-block entry (srcT: to the
+// transposed entry, same order).
+struct PairSpec { int64_t nele; int nI; const int32_t* idxI; int nJ; const int32_t* idxJ; int64_t kbase; };
+static int build_dmap(Ctx* c, const std::vector<PairSpec>& ps, int64_t nrow, int64_t ncol, bool want_src, bool want_srcT,
+                      DMap& M) {
+    hipStream_t s = c->stream;
+    int64_t np = 0;
+    for (auto& p : ps) np += p.nele * p.nI * p.nJ;
+    M.nrow = nrow; M.ncol = ncol; M.npair = np; M.nnz = 0;
+    HIPCHK(c, hipMalloc(&M.colptr, sizeof(int32_t) * (ncol + 1)));
+    if (np == 0) {
+        HIPCHK(c, hipMemsetAsync(M.colptr, 0, sizeof(int32_t) * (ncol + 1), s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        return MH_OK;
+    }
+    if (np >= INT32_MAX || np >= (int64_t)UINT32_MAX) { c->err = "too many contributions for 32-bit maps"; return MH_ERR_SIZE; }
+    unsigned long long *kin = nullptr, *kout = nullptr;
+    uint32_t* vin = nullptr;
+    int32_t *head = nullptr, *id = nullptr, *colk = nullptr;
+    HIPCHK(c, hipMalloc(&kin, sizeof(unsigned long long) * np));
+    HIPCHK(c, hipMalloc(&kout, sizeof(unsigned long long) * np));
+    HIPCHK(c, hipMalloc(&vin, sizeof(uint32_t) * np));
+    int end_bit = 1;
+    while (end_bit < 64 && ((unsigned long long)nrow * (unsigned long long)ncol) > (1ull << end_bit)) ++end_bit;
+    size_t tmp_bytes = 0;
+    void* tmp = nullptr;
+    // one sort per wanted source map (stable, identical keys: identical order); at least one
+    // sort always runs, for the sorted keys
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool tr = pass == 1;
+        if (tr ? !want_srcT : (!want_src && want_srcT)) continue;
+        uint32_t* dst = nullptr;
+        HIPCHK(c, hipMalloc(&dst, sizeof(uint32_t) * np));
+        int64_t off = 0;
+        for (auto& p : ps) {
+            const int64_t n = p.nele * p.nI * p.nJ;
+            if (n == 0) continue;
+            hipLaunchKernelGGL(gen_pairs_rect_kernel, dim3(grid(n)), dim3(kBlock), 0, s, p.nele, p.nI, p.idxI, p.nJ, p.idxJ,
+                               nrow, p.kbase, tr ? 1 : 0, kin + off, vin + off);
+            off += n;
+        }
+        if (int r = check_launch(c, "direct pairs")) return r;
+        tmp_bytes = 0;
+        HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, kin, kout, vin, dst, (int)np, 0, end_bit, s));
+        HIPCHK(c, hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
+        HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kin, kout, vin, dst, (int)np, 0, end_bit, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        dfree(tmp);
+        if (tr) M.srcT = dst;
+        else if (want_src) M.src = dst;
+        else dfree(dst);
+    }
+    HIPCHK(c, hipMalloc(&head, sizeof(int32_t) * np));
+    HIPCHK(c, hipMalloc(&id, sizeof(int32_t) * np));
+    hipLaunchKernelGGL(head_kernel<unsigned long long>, dim3(grid(np)), dim3(kBlock), 0, s, np, kout, head);
+    tmp_bytes = 0;
+    HIPCHK(c, hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, head, id, (int)np, s));
+    HIPCHK(c, hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
+    HIPCHK(c, hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, head, id, (int)np, s));
+    dfree(tmp);
+    int32_t nnz = 0;
+    HIPCHK(c, hipMemcpyAsync(&nnz, id + np - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    M.nnz = nnz;
+    HIPCHK(c, hipMalloc(&M.rowval, sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
+    HIPCHK(c, hipMalloc(&M.seg, sizeof(int32_t) * (nnz + 1)));
+    HIPCHK(c, hipMalloc(&colk, sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
+    hipLaunchKernelGGL(seg_kernel, dim3(grid(np)), dim3(kBlock), 0, s, np, kout, head, id, nrow, M.seg, M.rowval, colk);
+    int32_t np32 = (int32_t)np;
+    HIPCHK(c, hipMemcpyAsync(M.seg + nnz, &np32, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(gapfill_kernel, dim3(grid(nnz + 1)), dim3(kBlock), 0, s, (int64_t)nnz, colk, ncol, M.colptr);
+    if (int r = check_launch(c, "direct pattern")) return r;
+    HIPCHK(c, hipStreamSynchronize(s));
+    dfree(kin); dfree(kout); dfree(vin); dfree(head); dfree(id); dfree(colk);
+    return MH_OK;
+}
+
+// Value pass then the direct-blocks pass of one beam group.
+template <int ND>
+static void launch_direct_beam(bool u, dim3 grv, dim3 gr, hipStream_t s, const BeamDynArgs& b, const DirectArgs& a) {
+    if (u) {
+        hipLaunchKernelGGL((beam_dyn_value_kernel<ND, true>), grv, dim3(kBeamBlock), 0, s, b);
+        hipLaunchKernelGGL((beam_direct_kernel<ND, true>), gr, dim3(kDirBlock), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((beam_dyn_value_kernel<ND, false>), grv, dim3(kBeamBlock), 0, s, b);
+        hipLaunchKernelGGL((beam_direct_kernel<ND, false>), gr, dim3(kDirBlock), 0, s, a);
+    }
+}
+
+// Element blocks of one group for a DirectXUA step.
+static int launch_direct(Ctx* c, const Group& g, int nd, double dt, unsigned long long gtag) {
+    hipStream_t s = c->stream;
+    DirectArgs a;
+    a.nele = g.nele; a.nX = g.nX; a.kind = g.kind; a.idx = g.d_idx; a.uidx = g.d_uidx; a.nodes = g.d_nodes;
+    a.xyz = c->d_xyz; a.o2 = g.d_o2; a.sec = g.d_sec;
+    for (int d = 0; d < 3; ++d) a.X[d] = c->d_X[d];
+    a.U = c->d_U; a.lam = c->d_lam; a.vals = g.d_vals; a.t = c->t; a.dt = dt;
+    for (int i = 0; i < 12; ++i) { a.sX[i] = g.scaleX[i]; a.sL[i] = g.scaleL[i]; }
+    for (int i = 0; i < 3; ++i) a.sU[i] = g.scaleU[i];
+    a.Kc = nullptr;
+    a.Kd = c->d_Kd + g.kdbase; a.nKd = c->nKd;
+    a.KU = g.nU ? c->d_KU + g.kubase : nullptr;
+    a.R = c->d_Rd + g.rbase; a.L1x = c->d_L1x + g.rbase;
+    a.flag = c->d_flag; a.gtag = gtag;
+    const bool beam = g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U;
+    const bool bar = g.kind == MH_KIND_BAR || g.kind == MH_KIND_BAR_U;
+    const bool u = g.kind == MH_KIND_BEAM_U || g.kind == MH_KIND_BAR_U;
+    if (beam) {
+        // value pass with unit scales (unscaled compact static part), R × Δt
+        BeamDynArgs b;
+        b.nele = g.nele; b.idx = g.d_idx; b.uidx = g.d_uidx; b.nodes = g.d_nodes; b.xyz = c->d_xyz; b.o2 = g.d_o2;
+        b.sec = g.d_sec;
+        for (int d = 0; d < 3; ++d) b.X[d] = c->d_X[d];
+        b.U = c->d_U;
+        for (int i = 0; i < 6; ++i) b.sA[i] = 1.;
+        for (int i = 0; i < 18; ++i) b.sT[i] = 1.;
+        for (int i = 0; i < 21; ++i) b.sS[i] = 1.;
+        for (int i = 0; i < 12; ++i) { b.sR[i] = dt; b.scale[i] = 1.; }
+        for (int i = 0; i < 6; ++i) b.c[i] = 0.;
+        b.Kc = c->d_Kc + g.kcbase; b.K = nullptr; b.R = c->d_Rd + g.rbase; b.dR = nullptr;
+        b.flag = c->d_flag; b.gtag = gtag;
+        const int64_t ntile = (g.nele + kBeamBlock - 1) / kBeamBlock;
+        b.per_xcd = (ntile + kXCD - 1) / kXCD;
+        const dim3 grv((unsigned)(b.per_xcd * kXCD));
+        a.Kc = b.Kc;
+        const int64_t nt = (g.nele + kDirElems - 1) / kDirElems;
+        a.per_xcd = (nt + kXCD - 1) / kXCD;
+        const dim3 gr((unsigned)(a.per_xcd * kXCD));
+        if (nd == 1) launch_direct_beam<1>(u, grv, gr, s, b, a);
+        else if (nd == 2) launch_direct_beam<2>(u, grv, gr, s, b, a);
+        else launch_direct_beam<3>(u, grv, gr, s, b, a);
+    } else if (bar) {
+        const dim3 gr((unsigned)((g.nele + kBarBlock - 1) / kBarBlock));
+        if (nd == 1) { if (u) hipLaunchKernelGGL((bar_direct_kernel<1, true>), gr, dim3(kBarBlock), 0, s, a);
+                       else hipLaunchKernelGGL((bar_direct_kernel<1, false>), gr, dim3(kBarBlock), 0, s, a); }
+        else if (nd == 2) { if (u) hipLaunchKernelGGL((bar_direct_kernel<2, true>), gr, dim3(kBarBlock), 0, s, a);
+                            else hipLaunchKernelGGL((bar_direct_kernel<2, false>), gr, dim3(kBarBlock), 0, s, a); }
+        else { if (u) hipLaunchKernelGGL((bar_direct_kernel<3, true>), gr, dim3(kBarBlock), 0, s, a);
+               else hipLaunchKernelGGL((bar_direct_kernel<3, false>), gr, dim3(kBarBlock), 0, s, a); }
+    } else {
+        hipLaunchKernelGGL(point_direct_kernel, dim3(grid(g.nele)), dim3(kBlock), 0, s, a, nd);
+    }
+    return check_launch(c, "direct element kernels");
+}
+
 extern "C" {
 
 const char* mh_strerror(int code) {
@@ -1085,6 +1491,8 @@ int mh_group_add(mh_ctx* c, int kind, int64_t nele, int nX, const int64_t* idxX,
     Group g;
     g.kind = kind; g.nele = nele; g.nX = nX; g.nU = (beam || bar) ? nU : 0;
     for (int k = 0; k < 12; ++k) g.scaleX[k] = (scaleX && k < nX) ? scaleX[k] : 1.;
+    for (int k = 0; k < 12; ++k) g.scaleL[k] = g.scaleX[k];
+    for (int k = 0; k < 3; ++k) g.scaleU[k] = 1.;
     if (beam)
         for (int k = 0; k < 6; ++k)
             if (g.scaleX[k] != g.scaleX[k + 6]) { c->err = "beam dof scales must be equal on both nodes"; return MH_ERR_ARG; }
@@ -1173,7 +1581,6 @@ int mh_prepare_sweepx(mh_ctx* c, int OX, int64_t* nnz_out) {
     if (nKc) HIPCHK(c, hipMalloc(&c->d_Kc, sizeof(double) * nKc));
     HIPCHK(c, hipMalloc(&c->d_rhs, sizeof(double) * std::max<int64_t>(nX, 1)));
     HIPCHK(c, hipMalloc(&c->d_colptr, sizeof(int32_t) * (nX + 1)));
-    HIPCHK(c, hipMalloc(&c->d_rseg, sizeof(int32_t) * (nX + 1)));
     // ---- matrix pairs
     unsigned long long *kin = nullptr, *kout = nullptr;
     uint32_t* vin = nullptr;
@@ -1224,30 +1631,9 @@ int mh_prepare_sweepx(mh_ctx* c, int OX, int64_t* nnz_out) {
     if (int r = check_launch(c, "pattern")) return r;
     HIPCHK(c, hipStreamSynchronize(s));
     dfree(kin); dfree(kout); dfree(vin); dfree(head); dfree(id); dfree(colk);
-    // ---- vector pairs
+    if (int r = build_rmap(c)) return r;
     unsigned int *rkin = nullptr, *rkout = nullptr;
     uint32_t* rvin = nullptr;
-    HIPCHK(c, hipMalloc(&rkin, sizeof(unsigned int) * std::max<int64_t>(nR, 1)));
-    HIPCHK(c, hipMalloc(&rkout, sizeof(unsigned int) * std::max<int64_t>(nR, 1)));
-    HIPCHK(c, hipMalloc(&rvin, sizeof(uint32_t) * std::max<int64_t>(nR, 1)));
-    HIPCHK(c, hipMalloc(&c->d_rsrc, sizeof(uint32_t) * std::max<int64_t>(nR, 1)));
-    for (auto& g : c->groups) {
-        if (g.nX == 0 || g.nele == 0) continue;
-        int64_t np = g.nele * g.nX;
-        hipLaunchKernelGGL(gen_rpairs_kernel, dim3(grid(np)), dim3(kBlock), 0, s, g.nele, g.nX, g.d_idx, g.rbase,
-                           rkin + g.rbase, rvin + g.rbase);
-    }
-    int rbit = 1;
-    while (rbit < 32 && (unsigned long long)nX > (1ull << rbit)) ++rbit;
-    tmp_bytes = 0;
-    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, rkin, rkout, rvin, c->d_rsrc, (int)nR, 0, rbit, s));
-    HIPCHK(c, hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
-    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, rkin, rkout, rvin, c->d_rsrc, (int)nR, 0, rbit, s));
-    dfree(tmp);
-    hipLaunchKernelGGL(rseg_kernel, dim3(grid(nR + 1)), dim3(kBlock), 0, s, nR, rkout, nX, c->d_rseg);
-    if (int r = check_launch(c, "rhs pattern")) return r;
-    HIPCHK(c, hipStreamSynchronize(s));
-    dfree(rkin); dfree(rkout); dfree(rvin);
     // ---- U-dof map (gradients ∂L/∂U of the adjoint), same construction over the U class
     if (nRU > 0) {
         HIPCHK(c, hipMalloc(&rkin, sizeof(unsigned int) * nRU));
@@ -1442,7 +1828,7 @@ int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err*
 }
 
 int mh_set_lambda(mh_ctx* c, const double* lam) {
-    if (!c || !lam || !c->prepared) return MH_ERR_ARG;
+    if (!c || !lam || !(c->prepared || c->direct)) return MH_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     if (!c->d_lam) HIPCHK(c, hipMalloc(&c->d_lam, sizeof(double) * std::max<int64_t>(c->nX, 1)));
     if (c->nX) HIPCHK(c, hipMemcpyAsync(c->d_lam, lam, sizeof(double) * c->nX, hipMemcpyHostToDevice, c->stream));
@@ -1611,4 +1997,168 @@ int mh_sync(mh_ctx* c) {
     return MH_OK;
 }
 
+
+// ============================================================================ AssemblyDirect
+int mh_set_group_scales(mh_ctx* c, int igroup, const double* scaleL, const double* scaleU) {
+    if (!c || igroup < 0 || igroup >= (int)c->groups.size()) return MH_ERR_ARG;
+    Group& g = c->groups[igroup];
+    if (scaleL) for (int k = 0; k < g.nX && k < 12; ++k) g.scaleL[k] = scaleL[k];
+    if (scaleU) for (int k = 0; k < g.nU && k < 3; ++k) g.scaleU[k] = scaleU[k];
+    return MH_OK;
+}
+
+int mh_prepare_direct(mh_ctx* c, int OX, int OU, int IA, int64_t* nnz4) {
+    if (!c || OX < 0 || OX > 2 || OU < 0 || OU > 2 || IA < 0 || IA > 1) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    free_prepared(c);
+    if (c->nA) { c->err = "A-dofs are not evaluated on the device in AssemblyDirect"; return MH_ERR_ARG; }
+    int64_t nR = 0, nKd = 0, nKU = 0, nKc = 0;
+    for (auto& g : c->groups) {
+        if (g.kind == MH_KIND_ZERO) {
+            c->err = "Lagrangian elements with user cost functions (DofCost) are not evaluated on the device in AssemblyDirect";
+            return MH_ERR_ARG;
+        }
+        g.rbase = nR; g.kdbase = nKd; g.kubase = nKU; g.kcbase = nKc;
+        nR += g.nele * g.nX;
+        nKd += g.nele * g.nX * g.nX;
+        nKU += g.nele * g.nX * g.nU;
+        if (g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U) nKc += g.nele * mh::kCompact;
+    }
+    if (nR >= (int64_t)UINT32_MAX || nKd >= (int64_t)UINT32_MAX) { c->err = "too many element outputs for 32-bit maps"; return MH_ERR_SIZE; }
+    c->nR = nR; c->nKd = nKd; c->nKU = nKU;
+    c->dOX = OX; c->dOU = OU; c->dIA = IA;
+    if (int r = build_rmap(c)) return r;
+    std::vector<PairSpec> xx, xu, ux, uu;
+    for (auto& g : c->groups) {
+        if (g.nX == 0 || g.nele == 0) continue;
+        xx.push_back({g.nele, g.nX, g.d_idx, g.nX, g.d_idx, g.kdbase});
+        if (g.nU) {
+            xu.push_back({g.nele, g.nX, g.d_idx, g.nU, g.d_uidx, g.kubase});
+            ux.push_back({g.nele, g.nU, g.d_uidx, g.nX, g.d_idx, g.kubase});
+            uu.push_back({g.nele, g.nU, g.d_uidx, g.nU, g.d_uidx, 0});
+        }
+    }
+    if (int r = build_dmap(c, xx, c->nX, c->nX, true, true, c->dXX)) return r;
+    if (int r = build_dmap(c, xu, c->nX, c->nU, true, false, c->dXU)) return r;
+    if (int r = build_dmap(c, ux, c->nU, c->nX, false, true, c->dUX)) return r;
+    if (int r = build_dmap(c, uu, c->nU, c->nU, false, false, c->dUU)) return r;
+    HIPCHK(c, hipMalloc(&c->d_Kd, sizeof(double) * std::max<int64_t>((OX + 1) * nKd, 1)));
+    HIPCHK(c, hipMalloc(&c->d_KU, sizeof(double) * std::max<int64_t>(nKU, 1)));
+    HIPCHK(c, hipMalloc(&c->d_Kc, sizeof(double) * std::max<int64_t>(nKc, 1)));
+    HIPCHK(c, hipMalloc(&c->d_Rd, sizeof(double) * std::max<int64_t>(nR, 1)));
+    HIPCHK(c, hipMalloc(&c->d_L1x, sizeof(double) * std::max<int64_t>(nR, 1)));
+    HIPCHK(c, hipMalloc(&c->d_dL1, sizeof(double) * std::max<int64_t>(2 * c->nX, 1)));
+    HIPCHK(c, hipMalloc(&c->d_dL2, sizeof(double) * std::max<int64_t>(2 * (OX + 1) * c->dXX.nnz + 2 * c->dXU.nnz, 1)));
+    c->direct = true;
+    if (nnz4) { nnz4[0] = c->dXX.nnz; nnz4[1] = c->dXU.nnz; nnz4[2] = c->dUX.nnz; nnz4[3] = c->dUU.nnz; }
+    return MH_OK;
+}
+
+int mh_get_direct_pattern(mh_ctx* c, int which, int64_t* colptr, int64_t* rowval) {
+    if (!c || !c->direct || which < 0 || which > 3) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    const DMap& M = which == 0 ? c->dXX : which == 1 ? c->dXU : which == 2 ? c->dUX : c->dUU;
+    std::vector<int32_t> cp(M.ncol + 1, 0), rv(M.nnz);
+    if (M.colptr) HIPCHK(c, hipMemcpy(cp.data(), M.colptr, sizeof(int32_t) * cp.size(), hipMemcpyDeviceToHost));
+    if (M.nnz) HIPCHK(c, hipMemcpy(rv.data(), M.rowval, sizeof(int32_t) * rv.size(), hipMemcpyDeviceToHost));
+    if (colptr) for (size_t i = 0; i < cp.size(); ++i) colptr[i] = (int64_t)cp[i] + 1;
+    if (rowval) for (size_t i = 0; i < rv.size(); ++i) rowval[i] = (int64_t)rv[i] + 1;
+    return MH_OK;
+}
+
+int mh_assemble_direct(mh_ctx* c, int mission, double dt, mh_err* err) {
+    if (err) { err->code = MH_OK; err->igroup = -1; err->iele = -1; }
+    if (!c || !c->direct || mission < 0 || mission > 1) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    for (auto& g : c->groups)
+        if (g.kind == MH_KIND_HOLD && g.nele && !c->d_lam) { c->err = "AssemblyDirect needs Λ (mh_set_lambda)"; return MH_ERR_ARG; }
+    hipStream_t s = c->stream;
+    const int nd = c->dOX + 1;
+    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, s, c->d_flag, ~0ull);
+    HIPCHK(c, hipEventRecord(c->ev[0], s));
+    for (size_t gi = 0; gi < c->groups.size(); ++gi) {
+        const Group& g = c->groups[gi];
+        if (g.nX == 0 || g.nele == 0) continue;
+        if (int r = launch_direct(c, g, nd, dt, (unsigned long long)gi << 40)) return r;
+    }
+    HIPCHK(c, hipEventRecord(c->ev[1], s));
+    const double* nul = nullptr;
+    if (c->nX) {
+        const int64_t per = ((c->nX + kTile - 1) / kTile + kXCD - 1) / kXCD;
+        hipLaunchKernelGGL((scatter_kernel<false, false>), dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nX, per,
+                           c->d_rseg, c->d_rsrc, c->d_Rd, nul, c->d_dL1);
+        hipLaunchKernelGGL((scatter_kernel<false, false>), dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nX, per,
+                           c->d_rseg, c->d_rsrc, c->d_L1x, nul, c->d_dL1 + c->nX);
+    }
+    HIPCHK(c, hipEventRecord(c->ev[2], s));
+    if (mission == 1) {
+        const int64_t nxx = c->dXX.nnz, nxu = c->dXU.nnz;
+        for (int d = 0; d < nd; ++d)
+            if (nxx) {
+                launch_scatter<false>(c->scatter_variant, s, nxx, c->dXX.seg, c->dXX.src, c->d_Kd + d * c->nKd,
+                                      c->d_dL2 + d * nxx);
+                launch_scatter<false>(c->scatter_variant, s, nxx, c->dXX.seg, c->dXX.srcT, c->d_Kd + d * c->nKd,
+                                      c->d_dL2 + (nd + d) * nxx);
+            }
+        if (nxu) {
+            launch_scatter<false>(c->scatter_variant, s, nxu, c->dXU.seg, c->dXU.src, c->d_KU, c->d_dL2 + 2 * nd * nxx);
+            launch_scatter<false>(c->scatter_variant, s, c->dUX.nnz, c->dUX.seg, c->dUX.srcT, c->d_KU,
+                                  c->d_dL2 + 2 * nd * nxx + nxu);
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev[3], s));
+    if (int r = check_launch(c, "direct scatter")) return r;
+    unsigned long long flag = 0;
+    HIPCHK(c, hipMemcpyAsync(&flag, c->d_flag, sizeof(flag), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    c->timed = true;
+    if (flag != ~0ull) {
+        int gi = (int)(flag >> 40);
+        int64_t e = (int64_t)(flag & ((1ull << 40) - 1));
+        c->err = "element " + std::to_string(e + 1) + " of element type " + std::to_string(gi + 1) + " returned NaN";
+        if (err) { err->code = MH_ERR_NAN; err->igroup = gi; err->iele = e; }
+        return MH_ERR_NAN;
+    }
+    return MH_OK;
+}
+
+int mh_get_direct(mh_ctx* c, double* L1, double* L2) {
+    if (!c || !c->direct) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t nX = c->nX, nU = c->nU;
+    const int ndX = c->dOX + 1, ndU = c->dOU + 1;
+    const int64_t nxx = c->dXX.nnz, nxu = c->dXU.nnz, nux = c->dUX.nnz, nuu = c->dUU.nnz;
+    if (L1) {
+        const int64_t n1 = nX + (int64_t)ndX * nX + (int64_t)ndU * nU;
+        std::fill(L1, L1 + n1, 0.);
+        if (nX) {
+            HIPCHK(c, hipMemcpy(L1, c->d_dL1, sizeof(double) * nX, hipMemcpyDeviceToHost));
+            HIPCHK(c, hipMemcpy(L1 + nX, c->d_dL1 + nX, sizeof(double) * nX, hipMemcpyDeviceToHost));
+        }
+    }
+    if (L2) {
+        // pairs (α,β) ∈ {Λ,X,U}² row-major, (Λ,Λ) has no matrices; nder(α)×nder(β) blocks each
+        const int nder[3] = {1, ndX, ndU};
+        const int64_t nnzp[3][3] = {{0, nxx, nxu}, {nxx, nxx, nxu}, {nux, nux, nuu}};
+        int64_t off = 0;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                if (a == 0 && b == 0) continue;
+                const int64_t blk = nnzp[a][b];
+                for (int i = 0; i < nder[a]; ++i)
+                    for (int j = 0; j < nder[b]; ++j) {
+                        double* dst = L2 + off + ((int64_t)i * nder[b] + j) * blk;
+                        const double* src = nullptr;
+                        if (a == 0 && b == 1 && i == 0) src = c->d_dL2 + j * nxx;
+                        if (a == 1 && b == 0 && j == 0) src = c->d_dL2 + (ndX + i) * nxx;
+                        if (a == 0 && b == 2 && i == 0 && j == 0) src = c->d_dL2 + 2 * ndX * nxx;
+                        if (a == 2 && b == 0 && i == 0 && j == 0) src = c->d_dL2 + 2 * ndX * nxx + nxu;
+                        if (src && blk) HIPCHK(c, hipMemcpy(dst, src, sizeof(double) * blk, hipMemcpyDeviceToHost));
+                        else std::fill(dst, dst + blk, 0.);
+                    }
+                off += (int64_t)nder[a] * nder[b] * blk;
+            }
+    }
+    return MH_OK;
+}
 }  // extern "C"

@@ -8,10 +8,10 @@ from .model import (Disassembler, Model, MuscadeError, State, addelement, addnod
                     setscale)
 from .elements import (AxisymmetricBarCrossSection, Bar3D, Bar3DU, BeamCrossSection, DofCost, DofLoad, EulerBeam3D,
                        EulerBeam3DU, Hold)
-from .assemble import AssemblySweepX, DeviceContext, Newmarkβcoefficients
+from .assemble import AssemblyDirect, AssemblySweepX, DeviceContext, Newmarkβcoefficients
 from .sweepx import SweepX, solve
 
 __all__ = ["LIBPATH", "lib", "Model", "MuscadeError", "State", "Disassembler", "addnode", "addelement", "initialize",
            "setscale", "getdof", "setdof", "BeamCrossSection", "EulerBeam3D", "EulerBeam3DU", "Hold", "DofLoad",
            "AxisymmetricBarCrossSection", "Bar3D", "Bar3DU",
-           "DofCost", "AssemblySweepX", "DeviceContext", "Newmarkβcoefficients", "SweepX", "solve"]
+           "DofCost", "AssemblyDirect", "AssemblySweepX", "DeviceContext", "Newmarkβcoefficients", "SweepX", "solve"]

@@ -45,6 +45,11 @@ SIGNATURES = {
     "mh_vjp_lagrangian": (_I, [_P, _I, ctypes.POINTER(mh_err)]),
     "mh_get_vjp": (_I, [_P, _D, _D, _D]),
     "mh_get_tuning": (_I, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)]),
+    "mh_set_group_scales": (_I, [_P, _I, _D, _D]),
+    "mh_prepare_direct": (_I, [_P, _I, _I, _I, _I64]),
+    "mh_get_direct_pattern": (_I, [_P, _I, _I64, _I64]),
+    "mh_assemble_direct": (_I, [_P, _I, ctypes.c_double, ctypes.POINTER(mh_err)]),
+    "mh_get_direct": (_I, [_P, _D, _D]),
     "mh_assemble_sweepx": (_I, [_P, _I, _I, _D, ctypes.POINTER(mh_err)]),
     "mh_get_rhs": (_I, [_P, _D]),
     "mh_get_nzval": (_I, [_P, _D]),

@@ -181,3 +181,91 @@ class AssemblySweepX:
         ms = np.zeros(3)
         self.ctx.check(lib.mh_last_timings(self.ctx.h, dptr(ms)), "timings")
         return ms
+
+
+CLASSES_DIRECT = ("Λ", "X", "U")
+
+
+class AssemblyDirect:
+    """AssemblyDirect{OX,OU,IA} for one DirectXUA step on the GPU: prepare (DirectXUA.jl:22-56)
+    and assemble!{:vectors|:matrices} (DirectXUA.jl:85-171) over the classes Λ, X, U.
+
+    After `assemble`, `L1[α][αder]` are vectors and `L2[(α,β)][αder][βder]` are scipy CSC
+    matrices with the class-pair patterns, as out.L1/out.L2 of the reference (without the
+    empty L2[Λ,Λ])."""
+
+    def __init__(self, OX, OU, IA, model, dis, device=0):
+        self.OX, self.OU, self.IA = OX, OU, IA
+        self.model, self.dis = model, dis
+        self.ctx = DeviceContext(device)
+        self.groups = upload_model(self.ctx, model, dis)
+        for it, g in enumerate(self.groups):
+            d = dis.dis[it]
+            if d.X.shape[1]:
+                sU = as_f64(d.scaleU) if d.U.shape[1] else None
+                self.ctx.check(lib.mh_set_group_scales(self.ctx.h, g, dptr(as_f64(d.scaleΛ)), dptr(sU)), "scales")
+        nnz = np.zeros(4, np.int64)
+        self.ctx.check(lib.mh_prepare_direct(self.ctx.h, OX, OU, IA, i64ptr(nnz)), "prepare direct")
+        self.nX, self.nU = model.ndof["X"], model.ndof["U"]
+        shapes = [(self.nX, self.nX), (self.nX, self.nU), (self.nU, self.nX), (self.nU, self.nU)]
+        self.patterns = []
+        for w, (nr, nc) in enumerate(shapes):
+            cp = np.zeros(nc + 1, np.int64)
+            rv = np.zeros(nnz[w], np.int64)
+            self.ctx.check(lib.mh_get_direct_pattern(self.ctx.h, w, i64ptr(cp), i64ptr(rv)), "pattern")
+            self.patterns.append((cp, rv, (nr, nc)))
+        self.nder = {"Λ": 1, "X": OX + 1, "U": OU + 1}
+
+    def _which(self, a, b):
+        return (0 if a != "U" else 2) if b != "U" else (1 if a != "U" else 3)
+
+    def assemble(self, state, Λ, mission="matrices", Δt=1.0, dbg=None):
+        h = self.ctx.h
+        self.ctx.check(lib.mh_set_time(h, float(state.time)), "time")
+        for ider in range(self.OX + 1):
+            self.ctx.check(lib.mh_set_X(h, ider, dptr(as_f64(state.X[ider]))), "X")
+        if self.nU:
+            self.ctx.check(lib.mh_set_U(h, dptr(as_f64(state.U[0]))), "U")
+        self.ctx.check(lib.mh_set_lambda(h, dptr(as_f64(Λ))), "Λ")
+        for it, E in enumerate(self.model.eletypes):
+            if E.cls.kind == _lib.MH_KIND_DOFLOAD and E.nele:
+                vals = np.concatenate([np.full(n.shape[0], float(p["value"](state.time, **p["args"])))
+                                       for n, p in zip(E.nodes, E.params)])
+                self.ctx.check(lib.mh_set_point_values(h, self.groups[it], dptr(as_f64(vals))), "DofLoad values")
+        err = _lib.mh_err()
+        r = lib.mh_assemble_direct(h, 1 if mission == "matrices" else 0, float(Δt), ctypes.byref(err))
+        if r == _lib.MH_ERR_NAN:
+            E = self.model.eletypes[err.igroup]
+            raise MuscadeError(f"residual({E.key[0]},...) returned NaN",
+                               dict(dbg or {}, ieletyp=err.igroup + 1, iele=err.iele + 1, t=state.time))
+        self.ctx.check(r, "assemble direct")
+        nnz = [p[1].size for p in self.patterns]
+        nL1 = self.nX + (self.OX + 1) * self.nX + (self.OU + 1) * self.nU
+        sizes = {}
+        nL2 = 0
+        for a in CLASSES_DIRECT:
+            for b in CLASSES_DIRECT:
+                if a == "Λ" and b == "Λ":
+                    continue
+                sizes[(a, b)] = nL2
+                nL2 += self.nder[a] * self.nder[b] * nnz[self._which(a, b)]
+        L1f, L2f = np.zeros(nL1), np.zeros(nL2)
+        self.ctx.check(lib.mh_get_direct(h, dptr(L1f), dptr(L2f)), "direct results")
+        self.flat = (L1f, L2f)
+        L1 = {"Λ": [L1f[:self.nX]],
+              "X": [L1f[self.nX * (1 + d):self.nX * (2 + d)] for d in range(self.OX + 1)],
+              "U": [L1f[self.nX * (self.OX + 2) + self.nU * d:self.nX * (self.OX + 2) + self.nU * (d + 1)]
+                    for d in range(self.OU + 1)]}
+        L2 = {}
+        for (a, b), off in sizes.items():
+            cp, rv, shape = self.patterns[self._which(a, b)]
+            n = rv.size
+            blocks = []
+            for i in range(self.nder[a]):
+                row = []
+                for j in range(self.nder[b]):
+                    v = L2f[off + (i * self.nder[b] + j) * n: off + (i * self.nder[b] + j + 1) * n]
+                    row.append(sp.csc_matrix((v.copy(), rv - 1, cp - 1), shape=shape))
+                blocks.append(row)
+            L2[(a, b)] = blocks
+        return L1, L2

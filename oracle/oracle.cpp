@@ -30,6 +30,7 @@ struct EType {
     int udof;
     std::vector<int64_t> iX, iU, iA;   // 1-based model dof indices, [iele*n + k]
     std::vector<double> scaleX;        // per element dof (dis.dis[ieletyp].scale.X)
+    std::vector<double> scaleL, scaleU;   // scale.Λ, scale.U (AssemblyDirect's revariate)
     std::vector<EulerBeam3D> beams;
     std::vector<Bar3D> bars;
     std::vector<double> F;             // DofLoad values
@@ -37,8 +38,16 @@ struct EType {
     std::vector<AnchorLine> anchors;
     std::vector<int64_t> asm1, asm2;   // asm[1,ieletyp] (nX×nele), asm[2,ieletyp] (nX²×nele), 1-based, 0 = skip
 };
+// One class-pair pattern of AssemblyDirect (asmmat!(asm[α,β], asm[α], asm[β]), Assemble.jl:373-448).
+struct DPattern {
+    int64_t nr = 0, nc = 0, nnz = 0;
+    std::vector<int64_t> colptr, rowval;
+    std::vector<std::vector<int64_t>> asm2;   // per type: (nα·nβ)×nele, entry i + nα·j, 1-based
+};
 struct OModel {
     int64_t nX, nU, nA;
+    int dOX = -1, dOU = 0, dIA = 0;    // AssemblyDirect{OX,OU,IA} prepared
+    DPattern dpat[3][3];               // classes Λ, X, U
     double t = 0.;                     // state.time handed to time-dependent elements
     std::vector<EType> types;
     int64_t nnz = 0;
@@ -326,6 +335,8 @@ int orc_model_add_type(void* h, int kind, int nele, int nX, int nU, int nA, cons
     if (nA) t.iA.assign(iA, iA + (size_t)nele * nA);
     t.scaleX.assign(nX, 1.);
     if (scaleX) t.scaleX.assign(scaleX, scaleX + nX);
+    t.scaleL = t.scaleX;
+    t.scaleU.assign(nU, 1.);
     if (kind == K_BEAM) {
         t.beams.resize(nele);
         for (int e = 0; e < nele; ++e) {
@@ -510,4 +521,221 @@ int orc_model_assemble_sweepx(void* h, int OX, int mission, const double* c, con
     return 1;
 }
 
+}  // extern "C"
+
+// ================================================================ AssemblyDirect (DirectXUA)
+// The element-level maps of one class (asmvec!, Assemble.jl:340-357): with the all-dofs
+// groups of DirectXUA.jl:23, Λ and X use index.X, U uses index.U.
+static const std::vector<int64_t>& class_index(const EType& t, int cls) { return cls == 2 ? t.iU : t.iX; }
+static int class_ndof(const EType& t, int cls) { return cls == 2 ? t.nU : t.nX; }
+
+// asmmat! for (α,β): pairs (jmod, imod) in traversal order, stable sort, run-length
+// numbering, asm2 per type (Assemble.jl:373-448).
+static void build_dpattern(OModel* m, int a, int b, DPattern& P) {
+    P.nr = a == 2 ? m->nU : m->nX;
+    P.nc = b == 2 ? m->nU : m->nX;
+    std::vector<std::pair<int64_t, int64_t>> A;
+    for (auto& t : m->types) {
+        const int na = class_ndof(t, a), nb = class_ndof(t, b);
+        const auto& ia = class_index(t, a); const auto& ib = class_index(t, b);
+        for (int e = 0; e < t.nele; ++e)
+            for (int j = 0; j < nb; ++j) for (int i = 0; i < na; ++i)
+                A.push_back({ib[(size_t)e * nb + j], ia[(size_t)e * na + i]});
+    }
+    const int64_t npair = (int64_t)A.size();
+    std::vector<int64_t> I(npair);
+    std::iota(I.begin(), I.end(), 0);
+    std::stable_sort(I.begin(), I.end(), [&](int64_t x, int64_t y) { return A[x] < A[y]; });
+    std::vector<int64_t> K(npair);
+    P.colptr.assign(P.nc + 1, 0);
+    P.rowval.clear();
+    P.colptr[0] = 1;
+    int64_t inz = 0, icol = 1;
+    for (int64_t q = 0; q < npair; ++q) {
+        if (q == 0 || A[I[q]] != A[I[q - 1]]) {
+            ++inz;
+            P.rowval.push_back(A[I[q]].second);
+            while (A[I[q]].first > icol) { ++icol; P.colptr[icol - 1] = inz; }
+        }
+        K[I[q]] = inz;
+    }
+    for (int64_t i = icol + 1; i <= P.nc + 1; ++i) P.colptr[i - 1] = inz + 1;
+    P.nnz = inz;
+    P.asm2.assign(m->types.size(), {});
+    int64_t ip = 0;
+    for (size_t it = 0; it < m->types.size(); ++it) {
+        auto& t = m->types[it];
+        const int na = class_ndof(t, a), nb = class_ndof(t, b);
+        P.asm2[it].assign((size_t)t.nele * na * nb, 0);
+        for (int e = 0; e < t.nele; ++e)
+            for (int j = 0; j < nb; ++j) for (int i = 0; i < na; ++i)
+                P.asm2[it][(size_t)e * na * nb + i + (size_t)na * j] = K[ip++];
+    }
+}
+
+// Derivative orders of a class in AssemblyDirect{OX,OU,IA} (nder, DirectXUA.jl:25)
+static int dnder(const OModel* m, int cls) { return cls == 0 ? 1 : (cls == 1 ? m->dOX + 1 : m->dOU + 1); }
+// Does L2[α,β] hold matrices (DirectXUA.jl:37-48; no independence flags)
+static bool dhas(int a, int b) { return !(a == 0 && b == 0); }
+
+// Offsets of the flat outputs: L1 = [Λ | X·(OX+1) | U·(OU+1)], L2 = pairs (α,β) in
+// row-major class order, each nder(α)×nder(β) blocks (i-major) of nnz(α,β) values.
+static int64_t l1_offset(const OModel* m, int cls) {
+    return cls == 0 ? 0 : (cls == 1 ? m->nX : m->nX + (int64_t)(m->dOX + 1) * m->nX);
+}
+static int64_t l2_offset(const OModel* m, int a, int b) {
+    int64_t o = 0;
+    for (int x = 0; x < 3; ++x) for (int y = 0; y < 3; ++y) {
+        if (x == a && y == b) return o;
+        if (dhas(x, y)) o += (int64_t)dnder(m, x) * dnder(m, y) * m->dpat[x][y].nnz;
+    }
+    return o;
+}
+
+template <class TX, int ND>
+static void direct_residual(const EType& t, int e, const TX (*X)[12], const TX* U, double time, TX* R) {
+    if (t.kind == K_BEAM) {
+        if (t.udof) { TX Uj[1][3]; for (int c = 0; c < 3; ++c) Uj[0][c] = U[c]; beam_residual<TX, ND, 1>(t.beams[e], X, Uj, R); }
+        else beam_residual<TX, ND, 1>(t.beams[e], X, nullptr, R);
+    } else if (t.kind == K_BAR) {
+        bar_residual<TX, ND>(t.bars[e], X, t.udof ? U : nullptr, time, R);
+    } else if (t.kind == K_HOLD) {
+        hold_residual<TX>(X[0], R);
+    } else if (t.kind == K_DOFLOAD) {
+        dofload_residual<TX>(t.F[e], R);
+    }
+}
+
+// assemble!{mission}(::AssemblyDirect{OX,OU,IA}) for one step (DirectXUA.jl:85-150):
+// beams and bars by the residual path (no_second_order, :85-120), Hold and DofLoad by
+// the Lagrangian path L = Λ∘₁R with second-order partials (:121-171).
+template <int OX>
+static int assemble_direct(OModel* m, int mission, double dt, const double* Lam, const double* const* Xs,
+                           const double* const* Us, double time, double* L1, double* L2) {
+    constexpr int ND = OX + 1;
+    const int OU = m->dOU;
+    const int64_t nL1 = l1_offset(m, 2) + (int64_t)(OU + 1) * m->nU;
+    std::fill(L1, L1 + nL1, 0.);
+    const int64_t nL2 = l2_offset(m, 3, 0);
+    std::fill(L2, L2 + nL2, 0.);
+    const bool mats = mission == 1;
+    for (size_t it = 0; it < m->types.size(); ++it) {
+        const EType& t = m->types[it];
+        const int Nx = t.nX, Nu = t.nU;
+        if (t.kind == K_ZERO || t.kind == K_TURBINE || t.kind == K_ANCHOR) return 4;   // not restated here
+        for (int e = 0; e < t.nele; ++e) {
+            double x[3][12] = {}, u[3][3] = {}, lam[12] = {};
+            for (int d = 0; d < ND; ++d) for (int k = 0; k < Nx; ++k) x[d][k] = Xs[d][t.iX[(size_t)e * Nx + k] - 1];
+            for (int d = 0; d <= OU; ++d) for (int k = 0; k < Nu; ++k) u[d][k] = Us[d][t.iU[(size_t)e * Nu + k] - 1];
+            for (int k = 0; k < Nx; ++k) lam[k] = Lam[t.iX[(size_t)e * Nx + k] - 1];
+            const int64_t* aL = &t.iX[(size_t)e * Nx];
+            const int64_t* aU = Nu ? &t.iU[(size_t)e * Nu] : nullptr;
+            if (t.kind == K_BEAM || t.kind == K_BAR) {
+                // residual path: revariate{1}((;X,U),(;X=scale.X,U=scale.U)) — partials X₀..X_OX, U₀..U_OU
+                using TX = Dual<45, double>;
+                TX Xd[3][12], Ud[3];
+                for (int d = 0; d < ND; ++d) for (int k = 0; k < Nx; ++k) {
+                    TX v(x[d][k]); v.dx[d * Nx + k] = t.scaleX[k]; Xd[d][k] = v; }
+                for (int k = 0; k < Nu; ++k) { TX v(u[0][k]); v.dx[ND * Nx + k] = t.scaleU[k]; Ud[k] = v; }
+                TX R[12];
+                direct_residual<TX, ND>(t, e, Xd, Ud, time, R);
+                for (int i = 0; i < Nx; ++i) if (has_nan(R[i])) return 3;
+                for (int i = 0; i < Nx; ++i) L1[aL[i] - 1] += R[i].x * dt;
+                if (!mats) continue;
+                for (int b = 1; b <= 2; ++b) {
+                    const int Nb = b == 1 ? Nx : Nu;
+                    for (int j = 0; j < dnder(m, b); ++j) {
+                        const int pbase = b == 1 ? j * Nx : ND * Nx + j * Nu;
+                        double* Lab = L2 + l2_offset(m, 0, b) + (int64_t)j * m->dpat[0][b].nnz;
+                        double* Lba = L2 + l2_offset(m, b, 0) + (int64_t)j * dnder(m, 0) * m->dpat[b][0].nnz;
+                        const auto& A0b = m->dpat[0][b].asm2[it];
+                        const auto& Ab0 = m->dpat[b][0].asm2[it];
+                        for (int i = 0; i < Nx; ++i) for (int k = 0; k < Nb; ++k) {
+                            const bool ok = b == 1 || j <= OU;
+                            const double v = (ok && (b == 1 ? j < ND : j == 0)) ? R[i].dx[pbase + k] : 0.;
+                            Lab[A0b[(size_t)e * Nx * Nb + i + (size_t)Nx * k] - 1] += v * dt;     // add_∂!{1}
+                            Lba[Ab0[(size_t)e * Nb * Nx + k + (size_t)Nb * i] - 1] += v * dt;     // add_∂!{1,:plus,:transpose}
+                        }
+                    }
+                }
+            } else {
+                // Lagrangian path: revariate{2}((;Λ,X,U),(;Λ=scale.Λ,X=scale.X,U=scale.U)), L = Λ∘₁R
+                constexpr int NP = 12;
+                using D1 = Dual<NP, double>;
+                using TT = Dual<NP, D1>;
+                auto var = [](double v, int p, double sc) { TT r; r.x = D1(v); r.x.dx[p] = sc; r.dx[p] = D1(sc); return r; };
+                int p = 0;
+                TT Lv[12], Xd[3][12], Ud[3];
+                int pL = p; for (int k = 0; k < Nx; ++k) Lv[k] = var(lam[k], p++, t.scaleL[k]);
+                int pX = p; for (int d = 0; d < ND; ++d) for (int k = 0; k < Nx; ++k) Xd[d][k] = var(x[d][k], p++, t.scaleX[k]);
+                int pU = p; for (int d = 0; d <= OU; ++d) for (int k = 0; k < Nu; ++k) Ud[k] = var(u[d][k], p++, t.scaleU[k]);
+                if (p > NP) return 5;
+                TT R[12];
+                direct_residual<TT, ND>(t, e, Xd, Ud, time, R);
+                TT L = Lv[0] * R[0];
+                for (int i = 1; i < Nx; ++i) L = L + Lv[i] * R[i];
+                auto pstart = [&](int c, int d) { return c == 0 ? pL : (c == 1 ? pX + d * Nx : pU + d * Nu); };
+                auto nloc = [&](int c) { return c == 2 ? Nu : Nx; };
+                auto amap = [&](int c) { return c == 2 ? aU : aL; };
+                for (int a = 0; a < 3; ++a)
+                    for (int i = 0; i < dnder(m, a); ++i)
+                        for (int k = 0; k < nloc(a); ++k)
+                            L1[l1_offset(m, a) + (int64_t)i * (a == 2 ? m->nU : m->nX) + amap(a)[k] - 1] +=
+                                L.dx[pstart(a, i) + k].x * dt;
+                if (!mats) continue;
+                for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) {
+                    if (!dhas(a, b)) continue;
+                    const auto& Aab = m->dpat[a][b].asm2[it];
+                    const int na = nloc(a), nb = nloc(b);
+                    for (int i = 0; i < dnder(m, a); ++i) for (int j = 0; j < dnder(m, b); ++j) {
+                        double* Lab = L2 + l2_offset(m, a, b) + ((int64_t)i * dnder(m, b) + j) * m->dpat[a][b].nnz;
+                        for (int k = 0; k < na; ++k) for (int l = 0; l < nb; ++l)
+                            Lab[Aab[(size_t)e * na * nb + k + (size_t)na * l] - 1] += L.dx[pstart(a, i) + k].dx[pstart(b, j) + l] * dt;
+                    }
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+extern "C" {
+int orc_model_set_scales(void* h, int itype, const double* scaleL, const double* scaleU) {
+    OModel* m = (OModel*)h; EType& t = m->types[itype];
+    if (scaleL) t.scaleL.assign(scaleL, scaleL + t.nX);
+    if (scaleU) t.scaleU.assign(scaleU, scaleU + t.nU);
+    return 0;
+}
+// prepare(AssemblyDirect{OX,OU,IA}) (DirectXUA.jl:22-56); nnz of the 9 class pairs (Λ,X,U)².
+int orc_model_prepare_direct(void* h, int OX, int OU, int IA, int64_t* nnz9) {
+    OModel* m = (OModel*)h;
+    if (m->nA) return 1;
+    m->dOX = OX; m->dOU = OU; m->dIA = IA;
+    for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) {
+        build_dpattern(m, a, b, m->dpat[a][b]);
+        if (nnz9) nnz9[3 * a + b] = m->dpat[a][b].nnz;
+    }
+    return 0;
+}
+void orc_model_direct_pattern(void* h, int a, int b, int64_t* colptr, int64_t* rowval) {
+    OModel* m = (OModel*)h; const DPattern& P = m->dpat[a][b];
+    std::copy(P.colptr.begin(), P.colptr.end(), colptr);
+    std::copy(P.rowval.begin(), P.rowval.end(), rowval);
+}
+int64_t orc_model_direct_sizes(void* h, int64_t* nL1) {
+    OModel* m = (OModel*)h;
+    if (nL1) *nL1 = l1_offset(m, 2) + (int64_t)(m->dOU + 1) * m->nU;
+    return l2_offset(m, 3, 0);
+}
+int orc_model_assemble_direct(void* h, int mission, double dt, const double* Lam, const double* X0, const double* X1,
+                              const double* X2, const double* U0, const double* U1, const double* U2, double t,
+                              double* L1, double* L2) {
+    OModel* m = (OModel*)h;
+    const double* Xs[3] = {X0, X1, X2};
+    const double* Us[3] = {U0, U1, U2};
+    if (m->dOX == 0) return assemble_direct<0>(m, mission, dt, Lam, Xs, Us, t, L1, L2);
+    if (m->dOX == 1) return assemble_direct<1>(m, mission, dt, Lam, Xs, Us, t, L1, L2);
+    if (m->dOX == 2) return assemble_direct<2>(m, mission, dt, Lam, Xs, Us, t, L1, L2);
+    return 1;
+}
 }  // extern "C"

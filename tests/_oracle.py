@@ -30,6 +30,12 @@ _SIG = {
     "orc_bar_diffed_residual": (ctypes.c_int, [_D, _D, _D, ctypes.c_double, ctypes.c_int, ctypes.c_int, _D, _D,
                                                ctypes.c_double, _D, _D, _D]),
     "orc_model_set_time": (None, [_V, ctypes.c_double]),
+    "orc_model_set_scales": (ctypes.c_int, [_V, ctypes.c_int, _D, _D]),
+    "orc_model_prepare_direct": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I64]),
+    "orc_model_direct_pattern": (None, [_V, ctypes.c_int, ctypes.c_int, _I64, _I64]),
+    "orc_model_direct_sizes": (ctypes.c_int64, [_V, _I64]),
+    "orc_model_assemble_direct": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_double, _D, _D, _D, _D, _D, _D, _D,
+                                                 ctypes.c_double, _D, _D]),
     "orc_beam_diffed_residual": (ctypes.c_int, [_D, _D, _D, _D, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, _D]),
     "orc_beam_residual_plain": (ctypes.c_int, [_D, _D, _D, _D, ctypes.c_int, _D, _D, _D, _D]),
     "orc_beam_gauge": (ctypes.c_int, [_D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
@@ -158,6 +164,7 @@ class OracleModel:
     def __init__(self, nX, nU=0, nA=0):
         self.h = lib.orc_model_new(nX, nU, nA)
         self.nX = nX
+        self.nU = nU
         self.ntypes = 0
 
     def add_type(self, kind, iX, iU=None, iA=None, scaleX=None, params=None, udof=0):
@@ -174,6 +181,40 @@ class OracleModel:
             raise ValueError("orc_model_add_type failed")
         self.ntypes += 1
         return self.ntypes - 1
+
+    def set_scales(self, itype, scaleL=None, scaleU=None):
+        lib.orc_model_set_scales(self.h, itype, dp(f64(scaleL) if scaleL is not None else None),
+                                 dp(f64(scaleU) if scaleU is not None else None))
+
+    def prepare_direct(self, OX, OU, IA):
+        """prepare(AssemblyDirect{OX,OU,IA}): patterns of the class pairs (Λ,X,U)²."""
+        nnz = np.zeros(9, np.int64)
+        r = lib.orc_model_prepare_direct(self.h, OX, OU, IA, ip64(nnz))
+        if r:
+            raise ValueError("orc_model_prepare_direct failed")
+        self.dOX, self.dOU = OX, OU
+        self.dnnz = nnz.reshape(3, 3)
+        self.dpattern = {}
+        nU = self.nU
+        for a in range(3):
+            for b in range(3):
+                ncol = nU if b == 2 else self.nX
+                cp = np.zeros(ncol + 1, np.int64)
+                rv = np.zeros(self.dnnz[a, b], np.int64)
+                lib.orc_model_direct_pattern(self.h, a, b, ip64(cp), ip64(rv))
+                self.dpattern[(a, b)] = (cp, rv)
+        return self.dnnz
+
+    def assemble_direct(self, mission, dt, Lam, X, U=None, t=0.):
+        nL1 = np.zeros(1, np.int64)
+        nL2 = lib.orc_model_direct_sizes(self.h, ip64(nL1))
+        L1 = np.zeros(nL1[0])
+        L2 = np.zeros(nL2)
+        Xs = [f64(x) for x in X] + [None] * (3 - len(X))
+        Us = ([f64(u) for u in U] if U is not None else []) + [None] * (3 - (len(U) if U is not None else 0))
+        r = lib.orc_model_assemble_direct(self.h, mission, dt, dp(f64(Lam)), dp(Xs[0]), dp(Xs[1]), dp(Xs[2]),
+                                          dp(Us[0]), dp(Us[1]), dp(Us[2]), t, dp(L1), dp(L2))
+        return r, L1, L2
 
     def set_time(self, t):
         lib.orc_model_set_time(self.h, float(t))

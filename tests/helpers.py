@@ -50,6 +50,7 @@ def oracle_model(model, dis, t=0.0):
                                      for n, p in zip(E.nodes, E.params)])
         om.add_type(kind, d.X, d.U if d.U.shape[1] else None, d.A if d.A.shape[1] else None, d.scaleX, params,
                     udof=int(getattr(E.cls, "Udof", False)))
+        om.set_scales(it, d.scaleΛ, d.scaleU if d.U.shape[1] else None)
     om.prepare()
     om.set_time(t)
     return om
