@@ -202,6 +202,42 @@ def adjoint_bench(args):
             "batched_vs_plain_max_abs_diff": worst}
 
 
+def graph_bench(args):
+    """Per-assembly time of small models (launch-bound), eager launches vs one captured HIP
+    graph replayed (the Newton iterations of a time step): the 100-beam cantilever of the
+    reference's CPU case and a 2000-beam one. Device-resident state, the NaN-flag readback
+    included, outputs left on the device."""
+    import muscade_amd as mc
+    from muscade_amd import _lib
+    from muscade_amd.workloads import cantilever
+    res = {}
+    for nel in (100, 2000):
+        model = cantilever(nel=nel)
+        state = mc.initialize(model)
+        state.X[0][:] = np.random.default_rng(3).normal(size=model.ndof["X"]) * 1e-3
+        state.time = 1.0
+        row = {}
+        for mode in ("eager", "graph"):
+            out = mc.AssemblySweepX(0, model, state.dis, graph=(mode == "graph"))
+            out.set_state(state, 1.0)
+            fn = _lib.lib.mh_assemble_sweepx_graph if mode == "graph" else _lib.lib.mh_assemble_sweepx
+            err = _lib.mh_err()
+            c6 = out.c.array()
+            for _ in range(20):
+                fn(out.ctx.h, 0, 1, _lib.dptr(c6), ctypes.byref(err))
+            n = 2000
+            t0 = time.perf_counter()
+            for _ in range(n):
+                if fn(out.ctx.h, 0, 1, _lib.dptr(c6), ctypes.byref(err)) != 0:
+                    raise RuntimeError(_lib.lib.mh_last_error(out.ctx.h).decode())
+            row[mode + "_us"] = (time.perf_counter() - t0) / n * 1e6
+            del out
+        row["speedup"] = row["eager_us"] / row["graph_us"]
+        res[f"cantilever_{nel}"] = row
+        log(f"graph: {nel} beams: eager {row['eager_us']:.1f} us, graph {row['graph_us']:.1f} us per assembly")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +252,8 @@ def main():
     ap.add_argument("--no-dynamic", action="store_true", help="skip the dynamic (SweepX{2}) assembly measurement")
     ap.add_argument("--only-dynamic", action="store_true", help="measure only the dynamic assembly")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the batched vs plain adjoint measurement")
+    ap.add_argument("--no-graph", action="store_true", help="skip the eager vs HIP-graph small-model measurement")
+    ap.add_argument("--only-graph", action="store_true", help="measure only eager vs HIP-graph assembly")
     ap.add_argument("--only-adjoint", action="store_true", help="measure only the batched adjoint gradients")
     ap.add_argument("--models", type=int, default=512)
     ap.add_argument("--model-min", type=int, default=500)
@@ -237,6 +275,9 @@ def main():
         return
     if args.only_adjoint:
         print(json.dumps({"adjoint": adjoint_bench(args)}))
+        return
+    if args.only_graph:
+        print(json.dumps({"graph": graph_bench(args)}))
         return
     import muscade_amd as mc
     from muscade_amd import _lib
@@ -338,6 +379,8 @@ def main():
         line["dynamic"] = dynamic_bench(args)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_bench(args)
+    if not args.no_graph:
+        line["graph"] = graph_bench(args)
     print(json.dumps(line))
 
 

@@ -125,6 +125,16 @@ int mh_get_vjp(mh_ctx* ctx, double* gX, double* gU, double* gA);
  * Newmarkβcoefficients{OX} (src/SweepX.jl:3-15). Results stay on the device
  * (out.Lλ and out.Lλx.nzval); fetch them with mh_get_rhs / mh_get_nzval. */
 int mh_assemble_sweepx(mh_ctx* ctx, int OX, int mission, const double* c6, mh_err* err);
+/* The same assembly launched as one captured HIP graph (no counterpart in the reference;
+ * a launch-overhead option for the Newton iterations of SweepX.jl:186-221). The graph is
+ * captured on the first call and replayed while OX, c6, the time (mh_set_time) and the
+ * dof scales are unchanged — the iterations of one time step (one graph is kept per
+ * mission, :step and :iter); any change recaptures it. State setters (mh_set_X/U/A, mh_set_point_values) act through device
+ * buffers and need no recapture. Results are bitwise those of mh_assemble_sweepx;
+ * mh_last_timings then gives the whole graph's time in ms[0] (ms[1] = ms[2] = 0). */
+int mh_assemble_sweepx_graph(mh_ctx* ctx, int OX, int mission, const double* c6, mh_err* err);
+/* Number of graph captures so far (a measurement utility). */
+int mh_graph_captures(mh_ctx* ctx, int64_t* n);
 int mh_get_rhs(mh_ctx* ctx, double* rhs);
 int mh_get_nzval(mh_ctx* ctx, double* nzval);
 /* Device pointers of the assembled arrays (valid until the next prepare). */

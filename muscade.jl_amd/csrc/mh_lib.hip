@@ -97,6 +97,7 @@ struct Ctx {
     int32_t* d_rseg = nullptr;
     uint32_t* d_rsrc = nullptr;
     unsigned long long* d_flag = nullptr;
+    unsigned long long* h_flag = nullptr;   // pinned host copy of d_flag (copied on the stream)
     // adjoint (VJP) of the residual Lagrangian
     int64_t nRU = 0;             // per-element ∂L/∂U entries
     int32_t* d_useg = nullptr;
@@ -122,6 +123,16 @@ struct Ctx {
     float scatter_ms[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timed = false;
+    bool graph_timed = false;    // the last timings are of a graph launch (total only)
+    // captured SweepX assembly (mh_assemble_sweepx_graph): replayed while its launch
+    // arguments (OX, mission, Newmark coefficients, time, dof scales) are unchanged
+    // (one slot per mission: :step and :iter alternate within a time step)
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    double gkey[2][9] = {};
+    int gOX[2] = {-1, -1};
+    uint64_t gen = 0, ggen[2] = {~0ull, ~0ull};   // gen: bumped by setters that change launch arguments
+    int64_t graph_captures = 0;
 };
 
 #define HIPCHK(ctx, call)                                                                       \
@@ -135,7 +146,14 @@ struct Ctx {
 
 template <class T> void dfree(T*& p) { if (p) { (void)hipFree(p); p = nullptr; } }
 
+void free_graph(Ctx* c, int slot) {
+    if (c->gexec[slot]) { (void)hipGraphExecDestroy(c->gexec[slot]); c->gexec[slot] = nullptr; }
+    if (c->graph[slot]) { (void)hipGraphDestroy(c->graph[slot]); c->graph[slot] = nullptr; }
+    c->gOX[slot] = -1;
+}
+void free_graph(Ctx* c) { free_graph(c, 0); free_graph(c, 1); }
 void free_prepared(Ctx* c) {
+    free_graph(c);
     dfree(c->d_Kbuf); dfree(c->d_Rbuf); dfree(c->d_dRbuf); dfree(c->d_nzval); dfree(c->d_rhs); dfree(c->d_Kc);
     dfree(c->d_colptr); dfree(c->d_rowval); dfree(c->d_kseg); dfree(c->d_ksrc); dfree(c->d_rseg); dfree(c->d_rsrc);
     dfree(c->d_useg); dfree(c->d_usrc); dfree(c->d_gbuf); dfree(c->d_gX); dfree(c->d_gubuf); dfree(c->d_gU);
@@ -247,10 +265,8 @@ struct PointArgs {
     unsigned long long gtag;
 };
 
-// Hold, DofLoad and structural-zero elements.
-__global__ __launch_bounds__(kBlock) void point_kernel(PointArgs a) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= a.nele) return;
+// Hold, DofLoad and structural-zero elements: element e of one group.
+__device__ __forceinline__ void point_element(const PointArgs& a, int64_t e) {
     const int64_t n = a.nele;
     const int nX = a.nX;
     if (a.kind == MH_KIND_HOLD) {
@@ -276,6 +292,21 @@ __global__ __launch_bounds__(kBlock) void point_kernel(PointArgs a) {
     }
     if (a.dR)
         for (int i = 0; i < nX; ++i) a.dR[i * n + e] = 0.;
+}
+// Up to kPointBatch point groups in one launch (they are small; one launch each would
+// cost more than their work). Thread q runs element q - start[g] of group g.
+constexpr int kPointBatch = 12;
+struct PointBatch {
+    int ngroup;
+    int64_t start[kPointBatch + 1];
+    PointArgs g[kPointBatch];
+};
+__global__ __launch_bounds__(kBlock) void point_kernel(PointBatch b) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= b.start[b.ngroup]) return;
+    int gi = 0;
+    while (q >= b.start[gi + 1]) ++gi;
+    point_element(b.g[gi], q - b.start[gi]);
 }
 
 // ---- dynamic beams (OX = 1, 2) ------------------------------------------------
@@ -1435,6 +1466,9 @@ int mh_ctx_create(int device, mh_ctx** out) {
     }
     for (auto& e : c->ev) if (hipEventCreate(&e) != hipSuccess) { delete c; return MH_ERR_HIP; }
     if (hipMalloc(&c->d_flag, sizeof(unsigned long long)) != hipSuccess) { delete c; return MH_ERR_HIP; }
+    if (hipHostMalloc((void**)&c->h_flag, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+        delete c; return MH_ERR_HIP;
+    }
     *out = c;
     return MH_OK;
 }
@@ -1445,6 +1479,7 @@ int mh_ctx_destroy(mh_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     free_model(c);
     dfree(c->d_flag);
+    if (c->h_flag) (void)hipHostFree(c->h_flag);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1753,18 +1788,23 @@ int mh_set_time(mh_ctx* c, double t) {
     return MH_OK;
 }
 
-int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err* err) {
-    if (err) { err->code = MH_OK; err->igroup = -1; err->iele = -1; }
-    if (!c || !c->prepared || OX < 0 || OX > 2 || mission < 0 || mission > 1 || (OX > 0 && !c6)) return MH_ERR_ARG;
-    if (OX != c->OX) {
-        c->err = "assembler prepared for OX=" + std::to_string(c->OX) + ", called with OX=" + std::to_string(OX);
-        return MH_ERR_ARG;
-    }
-    HIPCHK(c, hipSetDevice(c->device));
+}  // extern "C"
+
+// Every launch of one assemble!{mission} of AssemblySweepX{OX}: the NaN-flag reset, the
+// element kernels of all groups, the nz scatter and the rhs scatter, on the context's
+// stream. With timed, HIP events bracket the three phases (not inside a graph capture).
+static int enqueue_sweepx(Ctx* c, int OX, bool step, const double* c6, bool timed) {
     hipStream_t s = c->stream;
-    const bool step = (mission == 0) && OX > 0;
     hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, s, c->d_flag, ~0ull);
-    HIPCHK(c, hipEventRecord(c->ev[0], s));
+    if (timed) HIPCHK(c, hipEventRecord(c->ev[0], s));
+    PointBatch pb;
+    pb.ngroup = 0;
+    pb.start[0] = 0;
+    auto flush_points = [&]() {
+        if (pb.ngroup == 0) return;
+        hipLaunchKernelGGL(point_kernel, dim3(grid(pb.start[pb.ngroup])), dim3(kBlock), 0, s, pb);
+        pb.ngroup = 0;
+    };
     for (size_t gi = 0; gi < c->groups.size(); ++gi) {
         Group& g = c->groups[gi];
         if (g.nX == 0 || g.nele == 0) continue;
@@ -1785,23 +1825,23 @@ int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err*
             if (g.kind == MH_KIND_BEAM_U) hipLaunchKernelGGL(beam_static_kernel<true>, gr, dim3(kBeamBlock), 0, s, a);
             else hipLaunchKernelGGL(beam_static_kernel<false>, gr, dim3(kBeamBlock), 0, s, a);
         } else {
-            PointArgs a;
+            PointArgs& a = pb.g[pb.ngroup];
             a.kind = g.kind; a.nele = g.nele; a.nX = g.nX; a.idx = g.d_idx; a.X = c->d_X[0]; a.vals = g.d_vals;
             for (int k = 0; k < 12; ++k) a.s[k] = g.scaleX[k];
             a.K = c->d_Kbuf + g.kbase; a.R = c->d_Rbuf + g.rbase; a.dR = step ? c->d_dRbuf + g.rbase : nullptr;
             a.flag = c->d_flag; a.gtag = gtag;
-            hipLaunchKernelGGL(point_kernel, dim3(grid(g.nele)), dim3(kBlock), 0, s, a);
+            pb.start[pb.ngroup + 1] = pb.start[pb.ngroup] + g.nele;
+            if (++pb.ngroup == kPointBatch) flush_points();
         }
     }
+    flush_points();
     if (int r = check_launch(c, "element kernels")) return r;
-    HIPCHK(c, hipEventRecord(c->ev[1], s));
+    if (timed) HIPCHK(c, hipEventRecord(c->ev[1], s));
     if (c->nnz) {
-        const int64_t per = ((c->nnz + kTile - 1) / kTile + kXCD - 1) / kXCD;
-        (void)per;
         if (c->coded) launch_scatter<true>(c->scatter_variant, s, c->nnz, c->d_kseg, c->d_ksrc, c->d_Kbuf, c->d_nzval);
         else launch_scatter<false>(c->scatter_variant, s, c->nnz, c->d_kseg, c->d_ksrc, c->d_Kbuf, c->d_nzval);
     }
-    HIPCHK(c, hipEventRecord(c->ev[2], s));
+    if (timed) HIPCHK(c, hipEventRecord(c->ev[2], s));
     if (c->nX) {
         const int64_t per = ((c->nX + kTile - 1) / kTile + kXCD - 1) / kXCD;
         if (step)
@@ -1811,12 +1851,16 @@ int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err*
             hipLaunchKernelGGL((scatter_kernel<false, false>), dim3((unsigned)(per * kXCD)), dim3(kTile), 0, s, c->nX, per,
                                c->d_rseg, c->d_rsrc, c->d_Rbuf, (const double*)nullptr, c->d_rhs);
     }
-    HIPCHK(c, hipEventRecord(c->ev[3], s));
-    if (int r = check_launch(c, "scatter kernels")) return r;
-    unsigned long long flag = 0;
-    HIPCHK(c, hipMemcpyAsync(&flag, c->d_flag, sizeof(flag), hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipStreamSynchronize(s));
-    c->timed = true;
+    if (timed) HIPCHK(c, hipEventRecord(c->ev[3], s));
+    // the NaN flag goes to pinned host memory on the stream (a node of a captured graph)
+    HIPCHK(c, hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    return check_launch(c, "scatter kernels");
+}
+
+// Wait for the assembly, then report the first element that returned NaN.
+static int finish_sweepx(Ctx* c, mh_err* err) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const unsigned long long flag = *c->h_flag;
     if (flag != ~0ull) {
         int gi = (int)(flag >> 40);
         int64_t e = (int64_t)(flag & ((1ull << 40) - 1));
@@ -1824,6 +1868,69 @@ int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err*
         if (err) { err->code = MH_ERR_NAN; err->igroup = gi; err->iele = e; }
         return MH_ERR_NAN;
     }
+    return MH_OK;
+}
+
+static int check_sweepx_args(Ctx* c, int OX, int mission, const double* c6, mh_err* err) {
+    if (err) { err->code = MH_OK; err->igroup = -1; err->iele = -1; }
+    if (!c || !c->prepared || OX < 0 || OX > 2 || mission < 0 || mission > 1 || (OX > 0 && !c6)) return MH_ERR_ARG;
+    if (OX != c->OX) {
+        c->err = "assembler prepared for OX=" + std::to_string(c->OX) + ", called with OX=" + std::to_string(OX);
+        return MH_ERR_ARG;
+    }
+    return MH_OK;
+}
+
+extern "C" {
+
+int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err* err) {
+    if (int r = check_sweepx_args(c, OX, mission, c6, err)) return r;
+    HIPCHK(c, hipSetDevice(c->device));
+    const bool step = (mission == 0) && OX > 0;
+    if (int r = enqueue_sweepx(c, OX, step, c6, true)) return r;
+    c->timed = true;
+    c->graph_timed = false;
+    return finish_sweepx(c, err);
+}
+
+int mh_assemble_sweepx_graph(mh_ctx* c, int OX, int mission, const double* c6, mh_err* err) {
+    if (int r = check_sweepx_args(c, OX, mission, c6, err)) return r;
+    HIPCHK(c, hipSetDevice(c->device));
+    const bool step = (mission == 0) && OX > 0;
+    double key[9] = {0, 0, 0, 0, 0, 0, 0, 0, c->t};
+    for (int i = 0; i < 6 && OX > 0; ++i) key[i] = c6[i];
+    const int sl = step ? 1 : 0;
+    bool same = c->gexec[sl] && c->gOX[sl] == OX && c->ggen[sl] == c->gen;
+    for (int i = 0; i < 9 && same; ++i) same = std::memcmp(&key[i], &c->gkey[sl][i], sizeof(double)) == 0;
+    hipStream_t s = c->stream;
+    if (!same) {
+        free_graph(c, sl);
+        HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int r = enqueue_sweepx(c, OX, step, c6, false);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(s, &g);
+        if (r != MH_OK || ec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (r == MH_OK) c->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(ec);
+            return r != MH_OK ? r : MH_ERR_HIP;
+        }
+        c->graph[sl] = g;
+        HIPCHK(c, hipGraphInstantiateWithFlags(&c->gexec[sl], c->graph[sl], 0));
+        std::memcpy(c->gkey[sl], key, sizeof(key));
+        c->gOX[sl] = OX; c->ggen[sl] = c->gen;
+        ++c->graph_captures;
+    }
+    HIPCHK(c, hipEventRecord(c->ev[0], s));
+    HIPCHK(c, hipGraphLaunch(c->gexec[sl], s));
+    HIPCHK(c, hipEventRecord(c->ev[3], s));
+    c->timed = true;
+    c->graph_timed = true;
+    return finish_sweepx(c, err);
+}
+
+int mh_graph_captures(mh_ctx* c, int64_t* n) {
+    if (!c || !n) return MH_ERR_ARG;
+    *n = c->graph_captures;
     return MH_OK;
 }
 
@@ -1957,6 +2064,11 @@ int mh_last_timings(mh_ctx* c, double* ms) {
     if (!c || !ms || !c->timed) return MH_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     float t = 0.f;
+    if (c->graph_timed) {   // one graph launch: its whole duration, phases not separated
+        HIPCHK(c, hipEventElapsedTime(&t, c->ev[0], c->ev[3]));
+        ms[0] = t; ms[1] = 0.; ms[2] = 0.;
+        return MH_OK;
+    }
     for (int i = 0; i < 3; ++i) {
         HIPCHK(c, hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]));
         ms[i] = t;
@@ -2004,6 +2116,7 @@ int mh_set_group_scales(mh_ctx* c, int igroup, const double* scaleL, const doubl
     Group& g = c->groups[igroup];
     if (scaleL) for (int k = 0; k < g.nX && k < 12; ++k) g.scaleL[k] = scaleL[k];
     if (scaleU) for (int k = 0; k < g.nU && k < 3; ++k) g.scaleU[k] = scaleU[k];
+    ++c->gen;
     return MH_OK;
 }
 

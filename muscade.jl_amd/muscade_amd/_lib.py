@@ -51,6 +51,8 @@ SIGNATURES = {
     "mh_assemble_direct": (_I, [_P, _I, ctypes.c_double, ctypes.POINTER(mh_err)]),
     "mh_get_direct": (_I, [_P, _D, _D]),
     "mh_assemble_sweepx": (_I, [_P, _I, _I, _D, ctypes.POINTER(mh_err)]),
+    "mh_assemble_sweepx_graph": (_I, [_P, _I, _I, _D, ctypes.POINTER(mh_err)]),
+    "mh_graph_captures": (_I, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "mh_get_rhs": (_I, [_P, _D]),
     "mh_get_nzval": (_I, [_P, _D]),
     "mh_device_outputs": (_I, [_P, ctypes.POINTER(_D), ctypes.POINTER(_D)]),

@@ -86,8 +86,12 @@ def upload_model(ctx, model, dis):
 class AssemblySweepX:
     """AssemblySweepX{OX}: prepare(AssemblySweepX{OX},model,dis) + assemble!{mission} (SweepX.jl:17-96)."""
 
-    def __init__(self, OX, model, dis, device=0):
+    def __init__(self, OX, model, dis, device=0, graph=False):
+        """graph=True: each assemble! replays one captured HIP graph of the element kernels and
+        scatters (recaptured when the Newmark coefficients or the time change, i.e. once per
+        time step of the Newton loop); results are bitwise those of the eager launches."""
         self.OX = OX
+        self.graph = graph
         self.model, self.dis = model, dis
         self.ctx = DeviceContext(device)
         self.groups = upload_model(self.ctx, model, dis)
@@ -133,7 +137,8 @@ class AssemblySweepX:
         self.set_state(state, state.time)
         err = _lib.mh_err()
         m = 0 if mission == "step" else 1
-        r = lib.mh_assemble_sweepx(self.ctx.h, self.OX, m, dptr(self.c.array()), ctypes.byref(err))
+        fn = lib.mh_assemble_sweepx_graph if self.graph else lib.mh_assemble_sweepx
+        r = fn(self.ctx.h, self.OX, m, dptr(self.c.array()), ctypes.byref(err))
         if r == _lib.MH_ERR_NAN:
             E = self.model.eletypes[err.igroup]
             raise MuscadeError(f"residual({E.key[0]},...) returned NaN in R, FB or derivatives",
@@ -143,6 +148,11 @@ class AssemblySweepX:
             self.ctx.check(lib.mh_get_rhs(self.ctx.h, dptr(self.Lλ)), "rhs")
             self.ctx.check(lib.mh_get_nzval(self.ctx.h, dptr(self.nzval)), "nzval")
         return self.Lλ, self.nzval
+
+    def graph_captures(self):
+        n = ctypes.c_int64()
+        self.ctx.check(lib.mh_graph_captures(self.ctx.h, ctypes.byref(n)), "graph captures")
+        return n.value
 
     def vjp(self, state, Λ, OX=None, dbg=None):
         """Gradients of L = Σ_e Λ_eᵀ R_e(X,U,A,t) (getlagrangian, Assemble.jl:707-736) at `state`:

@@ -29,12 +29,14 @@ class SweepX(metaclass=_SweepXMeta):
 
     @classmethod
     def run(cls, pstate, verbose, dbg, *, time, initialstate, β=0.25, γ=0.5, maxiter=50, maxΔx=1e-5,
-            maxLλ=math.inf, saveiter=False, device=0, Assembly=AssemblySweepX):
+            maxLλ=math.inf, saveiter=False, device=0, Assembly=AssemblySweepX, graph=True):
         """`Assembly` is the assembler type prepared for the model, as the type argument of
-        prepare(AssemblySweepX{OX},model,dis) (SweepX.jl:24,188)."""
+        prepare(AssemblySweepX{OX},model,dis) (SweepX.jl:24,188). graph: the device assembler
+        replays captured HIP graphs for the Newton iterations of each time step."""
         OX = cls.OX
         model, dis = initialstate.model, initialstate.dis
-        out = Assembly(OX, model, dis, device=device)
+        kw = {"graph": graph} if Assembly is AssemblySweepX else {}
+        out = Assembly(OX, model, dis, device=device, **kw)
         nX = out.nX
         scaleX = dis.scaleX
         cΔx2, cLλ2 = maxΔx ** 2, maxLλ ** 2
