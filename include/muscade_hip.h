@@ -108,11 +108,13 @@ int mh_set_time(mh_ctx* ctx, double t);
  * L(Λ,X,U,A) = Σ_e Λ_eᵀ R_e(X,U,A,t): the Lagrangian Muscade forms from residual
  * elements (getlagrangian: L = Λ∘₁R, src/Assemble.jl:707-736; DirectXUA's residual
  * path, src/DirectXUA.jl:96-120). After mh_set_lambda (Λ, nX values) and the state
- * (mh_set_X/U/A, mh_set_time), mh_vjp_lagrangian forms ∂L/∂X⁽ᵈᵉʳ⁾ (der = 0..OX), ∂L/∂U
- * and ∂L/∂A on the device, in unscaled model coordinates, recomputing each element's
+ * (mh_set_X/U/A, mh_set_time), mh_vjp_lagrangian forms ∂L/∂X⁽ᵈᵉʳ⁾ (der = 0..OX) and
+ * ∂L/∂U on the device, in unscaled model coordinates, recomputing each element's
  * internal state (tangent, time jets) instead of storing it; the per-element products
  * are reduced into the model vectors in Muscade's element order (no atomics).
- * mh_get_vjp copies them out: gX is (OX+1)×nX (der-major), gU nU, gA nA. There is no
+ * mh_get_vjp copies them out: gX is (OX+1)×nX (der-major), gU nU, gA nA; gA is zero,
+ * since no device element kind has A-dofs (BeamElement.jl:109-118, BarElement.jl:107-114,
+ * BasicElements.jl:275-284,406-438: A-dofs belong to host-evaluated cost elements). There is no
  * counterpart in the reference (which forms such gradients with forward-mode duals,
  * src/Diagnostic.jl:851-911); it serves the gradient (rrule) of a solve's Lagrangian. */
 int mh_set_lambda(mh_ctx* ctx, const double* lam);
