@@ -64,9 +64,11 @@ def calib_copy(n=1 << 27):
                       "GB_s": 16 * n / (ms[0] * 1e-3) / 1e9}))
 
 
-def cpu_baseline(model, state, nsample):
-    """The oracle (Muscade's nested-dual algorithm restated, sequential add_value!/add_∂!),
-    timed on the first `nsample` beams of the same model, 1 thread."""
+def cpu_baseline(model, state, nsample, nthreads=1):
+    """The oracle (Muscade's nested-dual algorithm restated, add_value!/add_∂! in element
+    order), timed on the first `nsample` beams of the same model: 1 thread (Muscade's
+    assemble! is single-threaded), or the element evaluations on `nthreads` OpenMP threads
+    followed by the same ordered scatter."""
     sys.path.insert(0, ROOT)
     from tests import _oracle as orc
     E = model.eletypes[0]
@@ -81,10 +83,20 @@ def cpu_baseline(model, state, nsample):
     om.add_type(orc.K_BEAM, d.X[:n], None, None, d.scaleX, params)
     om.prepare()
     t0 = time.perf_counter()
-    r, rhs, nz, _ = om.assemble(0, 1, np.zeros(6), [state.X[0]])
+    if nthreads == 1:
+        r, rhs, nz, _ = om.assemble(0, 1, np.zeros(6), [state.X[0]])
+    else:
+        r, rhs, nz, _ = om.assemble0_parallel(nthreads, state.X[0])
     dt = time.perf_counter() - t0
     assert r == 0
     return n / dt, n, dt
+
+
+def host_threads():
+    """Threads of the all-cores CPU baseline: OMP_NUM_THREADS if set (the host share of one
+    GPU), else the visible cores, at most 16."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    return max(1, int(env)) if env and env.isdigit() else max(1, min(16, os.cpu_count() or 1))
 
 
 def dynamic_bench(args):
@@ -238,6 +250,39 @@ def graph_bench(args):
     return res
 
 
+def newton_bench(args):
+    """One Newton step of SweepX{0} (SweepX.jl:204-210) on a reduced lattice: device
+    assembly, download of Lλ and Lλx, host sparse LU (SciPy SuperLU, as the reference's
+    host `lu`) and back-substitution, each timed. Host factorization of the full
+    100x100x33 frame is out of reach for SuperLU (its fill grows from 14M entries at
+    20x20x6 to 115M at 30x30x10); the factorization stays on the host and is reported
+    apart from the assembly."""
+    import scipy.sparse.linalg as spla
+    import muscade_amd as mc
+    from muscade_amd.workloads import space_frame
+    nx, ny, nz = 20, 20, 6
+    model, state = space_frame(nx, ny, nz, seed=1001)
+    out = mc.AssemblySweepX(0, model, state.dis)
+    out.assemble(state, "iter")   # warm-up
+    t0 = time.perf_counter()
+    out.assemble(state, "iter", fetch=False)
+    t1 = time.perf_counter()
+    out.assemble(state, "iter")
+    t2 = time.perf_counter()
+    K = out.Lλx()
+    t3 = time.perf_counter()
+    lu = spla.splu(K)
+    t4 = time.perf_counter()
+    lu.solve(out.Lλ)
+    t5 = time.perf_counter()
+    res = {"workload": f"static space frame {nx}x{ny}x{nz} nodes, {model.eletypes[0].nele} EulerBeam3D, nX={out.nX}, "
+                       f"nnz={out.nnz}", "assembly_ms": (t1 - t0) * 1e3, "assembly_with_download_ms": (t2 - t1) * 1e3,
+           "host_lu_ms": (t4 - t3) * 1e3, "host_solve_ms": (t5 - t4) * 1e3, "lu_fill": int(lu.L.nnz + lu.U.nnz)}
+    log(f"newton step ({nx}x{ny}x{nz}): assembly {res['assembly_ms']:.2f} ms, with download "
+        f"{res['assembly_with_download_ms']:.2f} ms, host LU {res['host_lu_ms']:.0f} ms, solve {res['host_solve_ms']:.1f} ms")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,12 +292,14 @@ def main():
     ap.add_argument("--ny", type=int, default=100)
     ap.add_argument("--nz", type=int, default=33)
     ap.add_argument("--cpu-sample", type=int, default=40000)
+    ap.add_argument("--cpu-sample-parallel", type=int, default=200000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="run only the counter-calibration copy kernel")
     ap.add_argument("--no-dynamic", action="store_true", help="skip the dynamic (SweepX{2}) assembly measurement")
     ap.add_argument("--only-dynamic", action="store_true", help="measure only the dynamic assembly")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the batched vs plain adjoint measurement")
     ap.add_argument("--no-graph", action="store_true", help="skip the eager vs HIP-graph small-model measurement")
+    ap.add_argument("--no-newton", action="store_true", help="skip the reduced-lattice Newton step (host LU) timing")
     ap.add_argument("--only-graph", action="store_true", help="measure only eager vs HIP-graph assembly")
     ap.add_argument("--only-adjoint", action="store_true", help="measure only the batched adjoint gradients")
     ap.add_argument("--models", type=int, default=512)
@@ -343,13 +390,36 @@ def main():
     dominant = ("eval", rl_eval) if t_eval >= t_sk else ("scatter", rl_scatter)
     log(f"per step: eval {t_eval * 1e3:.3f} ms, nz scatter {t_sk * 1e3:.3f} ms, rhs scatter {t_sr * 1e3:.3f} ms, "
         f"wall {ms_step:.3f} ms")
-    cpu = None
+    # PCIe-inclusive rate (not `value`): state uploaded from host memory, rhs and nzval
+    # downloaded to host memory, every step
+    nz_host, rhs_host = np.zeros(out.nnz), np.zeros(out.nX)
+    x_host = np.ascontiguousarray(state.X[0], dtype=np.float64)
+    npc = max(2, min(5, args.steps))
+    t0 = time.perf_counter()
+    for _ in range(npc):
+        _lib.lib.mh_set_X(h, 0, _lib.dptr(x_host))
+        step()
+        _lib.lib.mh_get_rhs(h, _lib.dptr(rhs_host))
+        _lib.lib.mh_get_nzval(h, _lib.dptr(nz_host))
+    t_pc = (time.perf_counter() - t0) / npc
+    pcie = {"ms_per_step": t_pc * 1e3, "elements_per_s": nbeam / t_pc,
+            "note": "H2D of X (16 MB) + assembly + D2H of rhs and nzval (673 MB) into pageable host memory"}
+    log(f"PCIe-inclusive: {t_pc * 1e3:.2f} ms/step")
+    cpu = cpu1 = None
     if not args.no_cpu_baseline:
+        nth = host_threads()
+        log(f"cpu baseline (oracle, {nth} threads)...")
+        eps, n, dt = cpu_baseline(model, state, args.cpu_sample_parallel, nth)
+        cpu = {"value": eps, "unit": "elements/s", "cores": nth, "kind": "port",
+               "sample": f"first {n} beams of the same 100x100x33 lattice, one assemble!{{:iter}} ({dt:.1f} s): "
+                         f"nested forward-mode duals per element on {nth} OpenMP threads, then add_value!/add_∂! "
+                         f"in element order"}
         log("cpu baseline (oracle, 1 thread)...")
-        eps, n, dt = cpu_baseline(model, state, args.cpu_sample)
-        cpu = {"value": eps, "unit": "elements/s", "cores": 1, "kind": "port",
-               "sample": f"first {n} beams of the same 100x100x33 lattice, one sequential assemble!{{:iter}} "
-                         f"({dt:.1f} s): nested forward-mode duals + sequential add_value!/add_∂!"}
+        eps1, n1, dt1 = cpu_baseline(model, state, args.cpu_sample, 1)
+        cpu1 = {"value": eps1, "unit": "elements/s", "cores": 1, "kind": "port",
+                "sample": f"first {n1} beams of the same 100x100x33 lattice, one sequential assemble!{{:iter}} "
+                          f"({dt1:.1f} s): nested forward-mode duals + sequential add_value!/add_∂! "
+                          f"(Muscade's assemble! is single-threaded)"}
     line = {
         "metric": "EulerBeam3D elements assembled/s (FP64 R+K)",
         "value": value,
@@ -373,6 +443,8 @@ def main():
         "prepare_s": t_prep,
         "tuning": tuning,
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
+        "pcie_inclusive": pcie,
     }
     del out
     if not args.no_dynamic:
@@ -381,6 +453,8 @@ def main():
         line["adjoint"] = adjoint_bench(args)
     if not args.no_graph:
         line["graph"] = graph_bench(args)
+    if not args.no_newton:
+        line["newton_step"] = newton_bench(args)
     print(json.dumps(line))
 
 

@@ -509,7 +509,57 @@ static int assemble_sweepx(OModel* m, int mission, const double* c, const double
     }
     return 0;
 }
+// The same static assemble!{:iter} (OX = 0) with the element evaluations spread over
+// nthreads OpenMP threads (the all-cores CPU baseline): each element's scaled R and
+// ∂R/∂δX go to its own slot, then add_value!/add_∂! run sequentially in (ieletyp, iele)
+// order, so rhs and nzval equal the sequential assembly bit for bit.
+static int assemble_sweepx0_omp(OModel* m, int nthreads, const double* X0, const double* U, const double* A,
+                                double* rhs, double* nzval, int* err_type, int* err_ele) {
+    std::fill(rhs, rhs + m->nX, 0.);
+    std::fill(nzval, nzval + m->nnz, 0.);
+    using TX = Dual<12, double>;
+    for (size_t it = 0; it < m->types.size(); ++it) {
+        const EType& t = m->types[it];
+        const int nX = t.nX;
+        if (nX == 0) continue;
+        std::vector<double> Rv((size_t)t.nele * nX), Kv((size_t)t.nele * nX * nX);
+        std::vector<int> bad(t.nele, 0);
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
+        for (int e = 0; e < t.nele; ++e) {
+            double Ue[3] = {0, 0, 0}, Ae[4] = {0, 0, 0, 0};
+            for (int k = 0; k < t.nU && k < 3; ++k) Ue[k] = U[t.iU[(size_t)e * t.nU + k] - 1];
+            for (int k = 0; k < t.nA && k < 4; ++k) Ae[k] = A[t.iA[(size_t)e * t.nA + k] - 1];
+            TX Xd[3][12];
+            for (int k = 0; k < nX; ++k) {
+                TX dX; dX.dx[k] = t.scaleX[k];
+                Xd[0][k] = X0[t.iX[(size_t)e * nX + k] - 1] + dX;
+            }
+            TX R[12];
+            elem_residual<TX, 1>(t, e, Xd, Ue, Ae, m->t, R);
+            for (int i = 0; i < nX; ++i) bad[e] |= has_nan(R[i]);
+            for (int i = 0; i < nX; ++i) {
+                const TX r = R[i] * t.scaleX[i];
+                Rv[(size_t)e * nX + i] = r.x;
+                for (int j = 0; j < nX; ++j) Kv[(size_t)e * nX * nX + i + (size_t)nX * j] = r.dx[j];
+            }
+        }
+        for (int e = 0; e < t.nele; ++e) {
+            if (bad[e]) { *err_type = (int)it; *err_ele = e; return 3; }
+            const int64_t* a1 = &t.asm1[(size_t)e * nX];
+            const int64_t* a2 = &t.asm2[(size_t)e * nX * nX];
+            for (int i = 0; i < nX; ++i) if (a1[i]) rhs[a1[i] - 1] += Rv[(size_t)e * nX + i];
+            for (int i = 0; i < nX; ++i) for (int j = 0; j < nX; ++j) {
+                int64_t k = a2[i + nX * j]; if (k) nzval[k - 1] += Kv[(size_t)e * nX * nX + i + (size_t)nX * j]; }
+        }
+    }
+    return 0;
+}
+
 extern "C" {
+int orc_model_assemble_sweepx0_omp(void* h, int nthreads, const double* X0, const double* U, const double* A,
+                                   double* rhs, double* nzval, int* err_type, int* err_ele) {
+    return assemble_sweepx0_omp((OModel*)h, nthreads, X0, U, A, rhs, nzval, err_type, err_ele);
+}
 int orc_model_assemble_sweepx(void* h, int OX, int mission, const double* c, const double* X0, const double* X1,
                               const double* X2, const double* U, const double* A, double* rhs, double* nzval,
                               int* err_type, int* err_ele) {

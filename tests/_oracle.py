@@ -51,6 +51,7 @@ _SIG = {
     "orc_model_get_asm": (None, [_V, ctypes.c_int, _I64, _I64]),
     "orc_model_assemble_sweepx": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, _D, _D, _D, _D, _IP,
                                                  _IP]),
+    "orc_model_assemble_sweepx0_omp": (ctypes.c_int, [_V, ctypes.c_int, _D, _D, _D, _D, _D, _IP, _IP]),
 }
 
 lib = ctypes.CDLL(LIBPATH)
@@ -244,6 +245,17 @@ class OracleModel:
         r = lib.orc_model_assemble_sweepx(self.h, OX, mission, dp(f64(c6)), dp(Xs[0]), dp(Xs[1]), dp(Xs[2]),
                                           dp(f64(U) if U is not None else None), dp(f64(A) if A is not None else None),
                                           dp(rhs), dp(nz), ip(et), ip(ee))
+        return r, rhs, nz, (int(et[0]), int(ee[0]))
+
+    def assemble0_parallel(self, nthreads, X0, U=None, A=None):
+        """Static assemble!{:iter} with element evaluations on nthreads OpenMP threads
+        (same results as assemble(0, 1, ...), bit for bit)."""
+        rhs = np.zeros(self.nX)
+        nz = np.zeros(self.nnz)
+        et = np.zeros(1, np.intc)
+        ee = np.zeros(1, np.intc)
+        r = lib.orc_model_assemble_sweepx0_omp(self.h, int(nthreads), dp(f64(X0)), dp(f64(U) if U is not None else None),
+                                               dp(f64(A) if A is not None else None), dp(rhs), dp(nz), ip(et), ip(ee))
         return r, rhs, nz, (int(et[0]), int(ee[0]))
 
     def __del__(self):

@@ -87,3 +87,16 @@ def test_pattern_bruteforce_random():
                         k = a2[e, i + nd * j] - 1
                         assert om.rowval[k] == iX[e, i]
                         assert om.colptr[iX[e, j] - 1] - 1 <= k < om.colptr[iX[e, j]] - 1
+
+
+def test_parallel_assembly_equals_sequential():
+    """The all-cores CPU baseline (element evaluations on OpenMP threads, then the ordered
+    add_value!/add_∂!) gives the sequential assembly bit for bit, Holds included."""
+    from tests.helpers import build_lattice, oracle_model
+    model, state = build_lattice(4, 3, 3, seed=2)
+    om = oracle_model(model, state.dis)
+    r1, rhs1, nz1, _ = om.assemble(0, 1, np.zeros(6), [state.X[0]])
+    r2, rhs2, nz2, _ = om.assemble0_parallel(4, state.X[0])
+    assert r1 == 0 and r2 == 0
+    assert np.array_equal(rhs1.view(np.uint64), rhs2.view(np.uint64))
+    assert np.array_equal(nz1.view(np.uint64), nz2.view(np.uint64))
