@@ -2160,6 +2160,8 @@ int mh_prepare_direct(mh_ctx* c, int OX, int OU, int IA, int64_t* nnz4) {
     HIPCHK(c, hipMalloc(&c->d_Kc, sizeof(double) * std::max<int64_t>(nKc, 1)));
     HIPCHK(c, hipMalloc(&c->d_Rd, sizeof(double) * std::max<int64_t>(nR, 1)));
     HIPCHK(c, hipMalloc(&c->d_L1x, sizeof(double) * std::max<int64_t>(nR, 1)));
+    // residual-path elements (beams, bars) add nothing to L1[X] and never write their slots
+    HIPCHK(c, hipMemsetAsync(c->d_L1x, 0, sizeof(double) * std::max<int64_t>(nR, 1), c->stream));
     HIPCHK(c, hipMalloc(&c->d_dL1, sizeof(double) * std::max<int64_t>(2 * c->nX, 1)));
     HIPCHK(c, hipMalloc(&c->d_dL2, sizeof(double) * std::max<int64_t>(2 * (OX + 1) * c->dXX.nnz + 2 * c->dXU.nnz, 1)));
     c->direct = true;

@@ -6,12 +6,15 @@ hand-written HIP kernels through the C ABI of include/muscade_hip.h.
 from ._lib import LIBPATH, lib
 from .model import (Disassembler, Model, MuscadeError, State, addelement, addnode, getdof, initialize, setdof,
                     setscale)
-from .elements import (AxisymmetricBarCrossSection, Bar3D, Bar3DU, BeamCrossSection, DofCost, DofLoad, EulerBeam3D,
-                       EulerBeam3DU, Hold)
+from .elements import (Acost, AxisymmetricBarCrossSection, Bar3D, Bar3DU, BeamCrossSection, DofCost, DofLoad,
+                       EulerBeam3D, EulerBeam3DU, Hold, SingleAcost, SingleDofCost, SingleUdof)
 from .assemble import AssemblyDirect, AssemblySweepX, DeviceContext, Newmarkβcoefficients
 from .sweepx import SweepX, solve
+from .directxua import DirectXUA
+from .hostcost import Dual2, exp10
 
 __all__ = ["LIBPATH", "lib", "Model", "MuscadeError", "State", "Disassembler", "addnode", "addelement", "initialize",
            "setscale", "getdof", "setdof", "BeamCrossSection", "EulerBeam3D", "EulerBeam3DU", "Hold", "DofLoad",
            "AxisymmetricBarCrossSection", "Bar3D", "Bar3DU",
-           "DofCost", "AssemblyDirect", "AssemblySweepX", "DeviceContext", "Newmarkβcoefficients", "SweepX", "solve"]
+           "DofCost", "SingleDofCost", "SingleUdof", "Acost", "SingleAcost", "DirectXUA", "Dual2", "exp10",
+           "AssemblyDirect", "AssemblySweepX", "DeviceContext", "Newmarkβcoefficients", "SweepX", "solve"]

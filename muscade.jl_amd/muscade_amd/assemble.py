@@ -51,13 +51,19 @@ class DeviceContext:
             self.h = None
 
 
-def upload_model(ctx, model, dis):
-    """Describe the model to the device: one group per element type, in eletype order."""
+def upload_model(ctx, model, dis, skip=(), nA=None):
+    """Describe the model to the device: one group per element type, in eletype order.
+    Element types in `skip` (evaluated on the host) get no group (None); nA overrides the
+    model's A-dof count (0 when no device kind uses A-dofs)."""
     xyz = as_f64(model.coords()[:, :3]) if model.nnod else np.zeros((0, 3))
-    ctx.check(lib.mh_model_begin(ctx.h, model.ndof["X"], model.ndof["U"], model.ndof["A"], model.nnod,
+    nA = model.ndof["A"] if nA is None else nA
+    ctx.check(lib.mh_model_begin(ctx.h, model.ndof["X"], model.ndof["U"], nA, model.nnod,
                                  dptr(xyz) if model.nnod else None), "mh_model_begin")
     groups = []
     for it, E in enumerate(model.eletypes):
+        if it in skip:
+            groups.append(None)
+            continue
         d = dis.dis[it]
         nele = d.X.shape[0]
         nX = d.X.shape[1]
@@ -204,14 +210,16 @@ class AssemblyDirect:
     matrices with the class-pair patterns, as out.L1/out.L2 of the reference (without the
     empty L2[Λ,Λ])."""
 
-    def __init__(self, OX, OU, IA, model, dis, device=0):
+    def __init__(self, OX, OU, IA, model, dis, device=0, skip=()):
+        """skip: element types left to the host (user-cost elements, whose Lagrangians the
+        DirectXUA solver evaluates itself); the device then assembles the others only."""
         self.OX, self.OU, self.IA = OX, OU, IA
         self.model, self.dis = model, dis
         self.ctx = DeviceContext(device)
-        self.groups = upload_model(self.ctx, model, dis)
+        self.groups = upload_model(self.ctx, model, dis, skip=skip, nA=0 if skip else None)
         for it, g in enumerate(self.groups):
             d = dis.dis[it]
-            if d.X.shape[1]:
+            if g is not None and d.X.shape[1]:
                 sU = as_f64(d.scaleU) if d.U.shape[1] else None
                 self.ctx.check(lib.mh_set_group_scales(self.ctx.h, g, dptr(as_f64(d.scaleΛ)), dptr(sU)), "scales")
         nnz = np.zeros(4, np.int64)
@@ -238,7 +246,7 @@ class AssemblyDirect:
             self.ctx.check(lib.mh_set_U(h, dptr(as_f64(state.U[0]))), "U")
         self.ctx.check(lib.mh_set_lambda(h, dptr(as_f64(Λ))), "Λ")
         for it, E in enumerate(self.model.eletypes):
-            if E.cls.kind == _lib.MH_KIND_DOFLOAD and E.nele:
+            if E.cls.kind == _lib.MH_KIND_DOFLOAD and E.nele and self.groups[it] is not None:
                 vals = np.concatenate([np.full(n.shape[0], float(p["value"](state.time, **p["args"])))
                                        for n, p in zip(E.nodes, E.params)])
                 self.ctx.check(lib.mh_set_point_values(h, self.groups[it], dptr(as_f64(vals))), "DofLoad values")

@@ -1,0 +1,311 @@
+"""solve(DirectXUA{OX,OU,IA}; initialstate, time, maxiter, maxΔλ, maxΔx, maxΔu, maxΔa, saveiter) —
+the all-steps Newton solver of optimisation FEM, src/DirectXUA.jl:245-513.
+
+Each Newton iteration assembles, for every step of every experiment, the step's gradient
+L1[α][αder] and Hessian blocks L2[α,β][αder,βder] over Λ, X, U (and A):
+
+- the residual elements (beams, bars) on the GPU, through AssemblyDirect (residual path,
+  src/DirectXUA.jl:85-120);
+- the Lagrangian elements on the host with second-order duals (hostcost.Dual2): the
+  user-cost elements (DofCost, SingleDofCost, SingleUdof; Acost once), which carry user
+  Functors, and Hold and DofLoad, whose Lagrangians are a few products
+  (src/DirectXUA.jl:121-171).
+
+These are combined into one sparse system over all steps with the finite-difference kernels
+of src/FiniteDifferences.jl for the time derivatives (assemblebig!, :322-357). The sparse LU
+of that system stays on the host (UMFPACK `lu` in the reference, SuperLU here). The
+increments are applied back through the same kernels (decrementbig!, :358-383).
+
+The unknowns are laid out step after step, [Λ (nX) | X (nX) | U (nU)] per step, then A. This
+is the block order of makepattern (:245-296); the sparsity pattern of the big matrix itself
+is built by scipy, not bit for bit as SparseTools.prepare does.
+
+One deliberate difference. For residual elements (beams, bars: no_second_order), the
+reference's step assembly adds L1[Λ] = R and the Λ-row/column Jacobian blocks, but not the
+gradient Λᵀ∂R/∂(X,U) (src/DirectXUA.jl:104-119). Its Newton step then returns the whole
+multiplier in ΔΛ instead of an increment, and the iterates' Λ keeps drifting. Here that
+gradient is added from the same blocks, (L2[Λ,X])ᵀΛ and (L2[Λ,U])ᵀΛ, so Λ converges to the
+multiplier. X and U converge to the same KKT point either way. The Hessian keeps the
+no_second_order approximation (no Λᵀ∂²R). Hold and DofLoad are evaluated here as
+Lagrangians on the host, so the device assembles residual elements only.
+"""
+import copy
+import math
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from .assemble import AssemblyDirect
+from .hostcost import Dual2
+from .model import MuscadeError
+
+CLASSES = ("Λ", "X", "U", "A")
+
+# [order][left | right | centre][points] — src/FiniteDifferences.jl:2-4
+_FD = [[[(0, 1.)]],
+       [[(0, -1.), (1, 1.)], [(-1, -1.), (0, 1.)], [(-1, -.5), (1, .5)]],
+       [[(0, 1.), (1, -2.), (2, 1.)], [(-2, 1.), (-1, -2.), (0, 1.)], [(-1, 1.), (0, -2.), (1, 1.)]]]
+
+
+def finitediff(order, n, s):
+    """(Δs, w) of the finite-difference kernel of `order` at step s of n (1-based) —
+    src/FiniteDifferences.jl:8-34."""
+    if order > 0 and n < 6:
+        raise MuscadeError("Number of steps must be ≥6")
+    if order == 0:
+        return _FD[0][0]
+    return _FD[order][0] if s == 1 else (_FD[order][1] if s == n else _FD[order][2])
+
+
+class HostLagrangians:
+    """Gradient and Hessian of the host-evaluated elements' Lagrangians, in the layout of
+    DirectXUA_lagrangian_addition! (src/DirectXUA.jl:121-150): partials Λ | X⁽⁰..OX⁾ | U⁽⁰..OU⁾
+    | A (IA = 1), each scaled like revariate with the element type's dof scales."""
+
+    def __init__(self, model, dis, OX, OU, IA):
+        self.model, self.dis = model, dis
+        self.OX, self.OU, self.IA = OX, OU, IA
+        self.nder = {"Λ": 1, "X": OX + 1, "U": OU + 1, "A": IA}
+        self.ndof = {"Λ": model.ndof["X"], "X": model.ndof["X"], "U": model.ndof["U"], "A": model.ndof["A"]}
+        self.types = [it for it, E in enumerate(model.eletypes)
+                      if getattr(E.cls, "host", False) or getattr(E.cls, "direct_host", False)]
+
+    def _chunks(self, it):
+        E = self.model.eletypes[it]
+        d = self.dis.dis[it]
+        e0 = 0
+        for nodes, p in zip(E.nodes, E.params):
+            m = nodes.shape[0]
+            yield p, d, slice(e0, e0 + m)
+            e0 += m
+
+    def step(self, state):
+        """Per-step contributions: L1[α][der] dense vectors, and a list of COO pieces
+        (α, i, β, j, rows, cols, vals) of L2 (0-based model dofs)."""
+        L1 = {c: [np.zeros(self.ndof[c]) for _ in range(self.nder[c])] for c in CLASSES}
+        L2 = []
+        for it in self.types:
+            E = self.model.eletypes[it]
+            if getattr(E.cls, "once_off", False):
+                continue
+            for p, d, sl in self._chunks(it):
+                iX, iU, iA = d.X[sl] - 1, d.U[sl] - 1, d.A[sl] - 1
+                m, Nx, Nu, Na = iX.shape[0], iX.shape[1], iU.shape[1], iA.shape[1]
+                # partial layout and global dofs of each (class, derivative)
+                layout, slot = [], 0
+                for c, idx, n in (("Λ", iX, Nx), ("X", iX, Nx), ("U", iU, Nu), ("A", iA, Na)):
+                    for der in range(self.nder[c] if c != "A" else self.IA):
+                        layout.append((c, der, slot, idx))
+                        slot += n
+                npar = slot
+
+                def seeded(c, der, idx, vals, scale):
+                    for (cc, dd, s0, _) in layout:
+                        if cc == c and dd == der:
+                            return [Dual2.seed(vals[:, k], s0 + k, npar, scale[k]) for k in range(idx.shape[1])]
+                    return [vals[:, k] for k in range(idx.shape[1])]
+
+                Λ = seeded("Λ", 0, iX, state.Λ[0][iX], d.scaleΛ)
+                X = [seeded("X", der, iX, state.X[der][iX], d.scaleX) for der in range(self.OX + 1)]
+                U = [seeded("U", der, iU, state.U[der][iU], d.scaleU) for der in range(self.OU + 1)]
+                A = seeded("A", 0, iA, state.A[iA], d.scaleA)
+                L = E.cls.lagrangian(p, Λ, X, U, A, state.time)
+                if not isinstance(L, Dual2):
+                    continue
+                if not np.all(np.isfinite(L.g)) or not np.all(np.isfinite(L.h)):
+                    raise MuscadeError(f"lagrangian({E.key[0]},...) returned NaN",
+                                       dict(ieletyp=it + 1, t=state.time))
+                for (a, i, sa, ia) in layout:
+                    na = ia.shape[1]
+                    np.add.at(L1[a][i], ia, L.g[:, sa:sa + na])
+                    for (b, j, sb, ib) in layout:
+                        if a == "Λ" and b == "Λ":
+                            continue
+                        nb = ib.shape[1]
+                        H = L.h[:, sa:sa + na, sb:sb + nb]
+                        rows = np.broadcast_to(ia[:, :, None], (m, na, nb)).ravel()
+                        cols = np.broadcast_to(ib[:, None, :], (m, na, nb)).ravel()
+                        L2.append((a, i, b, j, rows, cols, H.ravel()))
+        return L1, L2
+
+    def once(self, state):
+        """Once-off A costs (assembleA!, src/Assemble.jl:507-519; Acost addin!, src/DirectXUA.jl:66-78):
+        gradient and Hessian over the unscaled A-dofs."""
+        gA = np.zeros(self.ndof["A"])
+        pieces = []
+        for it in self.types:
+            E = self.model.eletypes[it]
+            if not getattr(E.cls, "once_off", False):
+                continue
+            for p, d, sl in self._chunks(it):
+                iA = d.A[sl] - 1
+                m, Na = iA.shape
+                A = [Dual2.seed(state.A[iA[:, k]], k, Na) for k in range(Na)]
+                C = E.cls.acost(p, A)
+                if not isinstance(C, Dual2):
+                    continue
+                np.add.at(gA, iA, C.g)
+                rows = np.broadcast_to(iA[:, :, None], (m, Na, Na)).ravel()
+                cols = np.broadcast_to(iA[:, None, :], (m, Na, Na)).ravel()
+                pieces.append((rows, cols, C.h.ravel()))
+        return gA, pieces
+
+
+class _DirectXUAMeta(type):
+    def __getitem__(cls, key):
+        OX, OU, IA = key
+        if OX not in (0, 1, 2) or OU not in (0, 1, 2) or IA not in (0, 1):
+            raise MuscadeError("DirectXUA{OX,OU,IA} needs OX, OU in 0..2 and IA in 0..1")
+        return type(f"DirectXUA{{{OX},{OU},{IA}}}", (DirectXUA,), {"OX": OX, "OU": OU, "IA": IA})
+
+
+class DirectXUA(metaclass=_DirectXUAMeta):
+    """DirectXUA{OX,OU,IA}: DirectXUA[OX, OU, IA] (src/DirectXUA.jl:385-437)."""
+    OX, OU, IA = 0, 0, 0
+
+    @classmethod
+    def run(cls, pstate, verbose, dbg, *, time, initialstate, maxiter=50, maxΔλ=1e-5, maxΔx=1e-5, maxΔu=1e-5,
+            maxΔa=1e-5, saveiter=False, device=0, Assembly=AssemblyDirect):
+        """`Assembly` is the per-step assembler of the device element kinds (prepare(AssemblyDirect{OX,OU,IA}),
+        src/DirectXUA.jl:22-56)."""
+        OX, OU, IA = cls.OX, cls.OU, cls.IA
+        nexp = len(time)
+        if len(initialstate) != nexp:
+            raise MuscadeError("initialstate and time must be of the same length")
+        nstep = [len(t) for t in time]
+        Δt = [float(t[1] - t[0]) if len(t) > 1 else 1.0 for t in time]
+        model, dis = initialstate[0].model, initialstate[0].dis
+        nX, nU, nA = model.ndof["X"], model.ndof["U"], model.ndof["A"]
+        if nA and not IA:
+            nA_used = 0
+        else:
+            nA_used = nA
+        nder = {"Λ": 1, "X": OX + 1, "U": OU + 1, "A": IA}
+        classes = CLASSES if IA else CLASSES[:3]
+        maxΔ2 = {"Λ": maxΔλ ** 2, "X": maxΔx ** 2, "U": maxΔu ** 2, "A": maxΔa ** 2}
+        scale = {"Λ": dis.scaleΛ, "X": dis.scaleX, "U": dis.scaleU, "A": dis.scaleA}
+        # state storage: state[iexp][istep], all sharing one A
+        A = initialstate[0].A.copy()
+        state = []
+        for iexp, s0 in enumerate(initialstate):
+            steps = []
+            for t in time[iexp]:
+                s = s0.copy(time=float(t)).with_ders(1, OX + 1, OU + 1)
+                s.A = A
+                steps.append(s)
+            state.append(steps)
+        stateiter = [] if saveiter else None
+        pstate[0] = stateiter if saveiter else state
+
+        host = HostLagrangians(model, dis, OX, OU, IA)
+        out = Assembly(OX, OU, IA, model, dis, device=device, skip=tuple(host.types))
+        per_step = 2 * nX + nU
+        exp0 = np.cumsum([0] + [n * per_step for n in nstep])
+        nv = int(exp0[-1]) + (nA_used if IA else 0)
+        Aoff = int(exp0[-1])
+        coff = {"Λ": 0, "X": nX, "U": 2 * nX}
+
+        def block(iexp, istep, c):   # istep 1-based
+            return int(exp0[iexp]) + (istep - 1) * per_step + coff[c]
+
+        for it in range(1, maxiter + 1):
+            if verbose:
+                print(f"\n    Iteration {it:3d}\n        Assembling", end="")
+            rows, cols, vals = [], [], []
+            Lv = np.zeros(nv)
+            if IA:
+                gA, pieces = host.once(state[0][0])
+                Lv[Aoff:Aoff + nA] += gA
+                for r, c, v in pieces:
+                    rows.append(r + Aoff); cols.append(c + Aoff); vals.append(v)
+
+            def off(iexp, istep, c):
+                return Aoff if c == "A" else block(iexp, istep, c)
+
+            for iexp in range(nexp):
+                dt = Δt[iexp]
+                for istep in range(1, nstep[iexp] + 1):
+                    st = state[iexp][istep - 1]
+                    st.SP = dict(γ=0., iter=it)
+                    L1d, L2d = out.assemble(st, st.Λ[0], "matrices", 1.0, dbg=dict(dbg, iter=it, step=istep))
+                    L1h, L2h = host.step(st)
+                    # the device kinds take the residual path, which leaves Λᵀ∂R/∂(X,U) out of L1
+                    # (src/DirectXUA.jl:104-119); add it from the same Jacobian blocks
+                    for b, pair in (("X", ("Λ", "X")), ("U", ("Λ", "U"))):
+                        if pair in L2d and L1h[b]:
+                            for bder, M in enumerate(L2d[pair][0][:nder[b]]):
+                                L1h[b][bder] = L1h[b][bder] + M.T @ st.Λ[0]
+                    for b in classes:
+                        for bder in range(nder[b]):
+                            v = L1h[b][bder]
+                            if b in L1d and bder < len(L1d[b]):
+                                v = v + L1d[b][bder]
+                            s = dt ** (-bder)
+                            for Δs, w in finitediff(bder, nstep[iexp], istep):
+                                o = off(iexp, istep + Δs, b)
+                                Lv[o:o + v.size] += w * s * v
+                    for (a, b), blocks in L2d.items():
+                        for i, row in enumerate(blocks):
+                            for j, M in enumerate(row):
+                                C = M.tocoo()
+                                if C.nnz == 0:
+                                    continue
+                                s = dt ** (-i - j)
+                                for Δa, wa in finitediff(i, nstep[iexp], istep):
+                                    for Δb, wb in finitediff(j, nstep[iexp], istep):
+                                        rows.append(C.row + off(iexp, istep + Δa, a))
+                                        cols.append(C.col + off(iexp, istep + Δb, b))
+                                        vals.append(C.data * (wa * wb * s))
+                    for (a, i, b, j, r, c, v) in L2h:
+                        s = dt ** (-i - j)
+                        for Δa, wa in finitediff(i, nstep[iexp], istep):
+                            for Δb, wb in finitediff(j, nstep[iexp], istep):
+                                rows.append(r + off(iexp, istep + Δa, a))
+                                cols.append(c + off(iexp, istep + Δb, b))
+                                vals.append(v * (wa * wb * s))
+            Lvv = sp.coo_matrix((np.concatenate(vals) if vals else np.zeros(0),
+                                 (np.concatenate(rows) if rows else np.zeros(0, int),
+                                  np.concatenate(cols) if cols else np.zeros(0, int))), shape=(nv, nv)).tocsc()
+            Lvv.eliminate_zeros()
+            if verbose:
+                print(", solving", end="")
+            try:
+                Δv = spla.splu(Lvv).solve(Lv)
+            except RuntimeError as e:
+                raise MuscadeError(f"Lvv matrix factorization failed: {e}", dict(dbg, iter=it))
+            if verbose:
+                print(", decrementing.")
+            Δ2 = {c: 0. for c in classes}
+            for iexp in range(nexp):
+                dt = Δt[iexp]
+                for istep in range(1, nstep[iexp] + 1):
+                    st = state[iexp][istep - 1]
+                    for b in ("Λ", "X", "U"):
+                        n = nX if b != "U" else nU
+                        if n == 0:
+                            continue
+                        target = st.Λ if b == "Λ" else (st.X if b == "X" else st.U)
+                        for bder in range(nder[b]):
+                            s = dt ** (-bder)
+                            for Δs, w in finitediff(bder, nstep[iexp], istep):
+                                o = block(iexp, istep + Δs, b)
+                                Δb = Δv[o:o + n]
+                                target[bder] -= Δb * (w * s) * scale[b]
+                                if bder == 0:
+                                    Δ2[b] = max(Δ2[b], float(Δb @ Δb))
+            if IA and nA:
+                Δa = Δv[Aoff:Aoff + nA]
+                Δ2["A"] = float(Δa @ Δa)
+                A -= Δa * scale["A"]
+            if saveiter:
+                stateiter.append(copy.deepcopy(state))
+            if verbose:
+                for c in classes:
+                    print(f"        maxₜ(|Δ{c}|)={math.sqrt(Δ2[c]):7.1e} ≤ {math.sqrt(maxΔ2[c]):7.1e}")
+            if all(Δ2[c] <= maxΔ2[c] for c in classes):
+                if verbose:
+                    print(f"\n    Converged in {it:3d} iterations.\n    nvar={nv}, nstep={sum(nstep)}")
+                return
+            if it == maxiter:
+                raise MuscadeError(f"no convergence after {it:3d} iterations.", dbg)

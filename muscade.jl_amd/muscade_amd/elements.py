@@ -8,6 +8,7 @@ residuals themselves are evaluated by the HIP kernels; `kind` names the kernel.
 import numpy as np
 
 from . import _lib
+from .hostcost import call_cost
 from .model import MuscadeError
 
 BEAM_FIELDS = ("EA", "EI₂", "EI₃", "GJ", "μ", "ι₁", "w", "Ca₁", "Cl₁", "Cq₁", "Ca₂", "Cl₂", "Cq₂", "Ca₃", "Cl₃", "Cq₃")
@@ -137,6 +138,11 @@ class Hold:
     """Hold(nod; field, λfield=Symbol(:λ,field)) — src/BasicElements.jl:484-488.
     A DofConstraint{:X} with gap(x)=x[1] and mode `equal` (BasicElements.jl:426-438)."""
     kind = _lib.MH_KIND_HOLD
+    direct_host = True   # in DirectXUA: L = Λ∘₁R on the host (getlagrangian, src/Assemble.jl:707-736)
+
+    @staticmethod
+    def lagrangian(p, Λ, X, U, A, t):
+        return Λ[0] * (-X[0][1]) + Λ[1] * (-X[0][0])
 
     @classmethod
     def construct(cls, model, nodes0, field, λfield=None):
@@ -151,6 +157,11 @@ class DofLoad:
     `value` is a host callable; it is evaluated on the host at each step and
     the values are handed to the device (no device kernel for user functions)."""
     kind = _lib.MH_KIND_DOFLOAD
+    direct_host = True   # in DirectXUA: L = Λ·(−value(t)) on the host
+
+    @staticmethod
+    def lagrangian(p, Λ, X, U, A, t):
+        return Λ[0] * (-float(p["value"](t, **p["args"])))
 
     @classmethod
     def construct(cls, model, nodes0, field, value, **args):
@@ -162,8 +173,16 @@ class DofCost:
     """DofCost(nod; xinod, xfield, uinod, ufield, ainod, afield, cost, costargs) — src/BasicElements.jl:32-50.
 
     A Lagrangian-only element: in SweepX its residual ∂L/∂Λ is identically
-    zero, but its X-dof pairs are part of the pattern (structural zeros)."""
+    zero, but its X-dof pairs are part of the pattern (structural zeros). In DirectXUA
+    its Lagrangian L = cost(X, U, A, t, costargs...) is evaluated on the host
+    (`lagrangian`): X[d][k] is time derivative d of the element's k-th X-dof, likewise U;
+    A[k] the k-th A-dof."""
     kind = _lib.MH_KIND_ZERO
+    host = True
+
+    @staticmethod
+    def lagrangian(p, Λ, X, U, A, t):
+        return call_cost(p["cost"], (X, U, A, t), p["costargs"])
 
     @classmethod
     def construct(cls, model, nodes0, cost, xinod=(), xfield=(), uinod=(), ufield=(), ainod=(), afield=(), costargs=()):
@@ -173,3 +192,77 @@ class DofCost:
         name = getattr(cost, "__name__", type(cost).__name__)
         key = ("DofCost", tuple(xinod), tuple(xfield), tuple(uinod), tuple(ufield), tuple(ainod), tuple(afield), name)
         return key, (inod, dclass, field), {"cost": cost, "costargs": costargs}
+
+
+class SingleDofCost:
+    """SingleDofCost(nod; class, field, cost, derivative=0, costargs) — src/BasicElements.jl:198-208:
+    a DofCost on one X- or U-dof, L = cost(∂ⁿx, t, costargs...) with n = derivative.
+    `class` is a Python keyword: pass it as dofclass="X" (or **{"class": "X"})."""
+    kind = _lib.MH_KIND_ZERO
+    host = True
+
+    @classmethod
+    def construct(cls, model, nodes0, cost, field, derivative=0, costargs=(), **kw):
+        dclass = kw.get("class", kw.get("dofclass", "X"))
+        if dclass not in ("X", "U"):
+            raise MuscadeError("'class' must be :X or :U")
+        name = getattr(cost, "__name__", type(cost).__name__)
+        key = ("DofCost", dclass, field, derivative, name)
+        return key, ((1,), (dclass,), (field,)), {"cost": cost, "costargs": costargs, "derivative": int(derivative),
+                                                   "class": dclass}
+
+    @staticmethod
+    def lagrangian(p, Λ, X, U, A, t):
+        d = p["derivative"]
+        v = X[d][0] if p["class"] == "X" else U[d][0]
+        return call_cost(p["cost"], (v, t), p["costargs"])
+
+
+class SingleUdof:
+    """SingleUdof(nod; Xfield, Ufield, cost, costargs) — src/BasicElements.jl:239-248: a U-dof applied
+    as a load on an X-dof of the same node, with L = cost(u, t, costargs...) − λ·u."""
+    kind = _lib.MH_KIND_ZERO
+    host = True
+
+    @classmethod
+    def construct(cls, model, nodes0, Xfield, Ufield, cost, costargs=()):
+        name = getattr(cost, "__name__", type(cost).__name__)
+        return ("SingleUdof", Xfield, Ufield, name), ((1, 1), ("X", "U"), (Xfield, Ufield)), \
+            {"cost": cost, "costargs": costargs}
+
+    @staticmethod
+    def lagrangian(p, Λ, X, U, A, t):
+        u = U[0][0]
+        return call_cost(p["cost"], (u, t), p["costargs"]) - Λ[0] * u
+
+
+class Acost:
+    """Acost(nod; inod, field, cost, costargs) — src/BasicElements.jl:77-86: a once-off cost
+    cost(A, costargs...) on A-dofs, assembled once (assembleA!, src/Assemble.jl:507-519), not per step."""
+    kind = _lib.MH_KIND_ZERO
+    host = True
+    once_off = True
+
+    @classmethod
+    def construct(cls, model, nodes0, cost, inod=(), field=(), costargs=()):
+        name = getattr(cost, "__name__", type(cost).__name__)
+        return ("Acost", tuple(inod), tuple(field), name), (tuple(inod), ("A",) * len(inod), tuple(field)), \
+            {"cost": cost, "costargs": costargs}
+
+    @staticmethod
+    def acost(p, A):
+        return call_cost(p["cost"], (A,), p["costargs"])
+
+
+class SingleAcost(Acost):
+    """SingleAcost(nod; field, cost, costargs) — src/BasicElements.jl:163-167: Acost on one A-dof,
+    cost(a, costargs...)."""
+
+    @classmethod
+    def construct(cls, model, nodes0, field, cost, costargs=()):
+        name = getattr(cost, "__name__", type(cost).__name__)
+        return ("Acost", (1,), (field,), "single", name), ((1,), ("A",), (field,)), {"cost": cost, "costargs": costargs}
+
+    @staticmethod
+    def acost(p, A):
+        return call_cost(p["cost"], (A[0],), p["costargs"])

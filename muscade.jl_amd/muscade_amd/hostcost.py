@@ -1,0 +1,133 @@
+"""User-cost elements evaluated on the host, and the second-order duals that differentiate them.
+
+Muscade evaluates the Lagrangian of DofCost, Acost, SingleDofCost, SingleUdof and ElementCost
+elements with nested forward-mode duals (src/BasicElements.jl:32-248; src/DirectXUA.jl:152-171:
+`revariate{2}`, then `∂{2,Np}` gives the gradient and the Hessian over all partials). Their
+cost functions are user callables, so they stay on the host, like any user `Functor`; only
+their gradient and Hessian enter the assembled system.
+
+`Dual2` carries, for a batch of m elements at once, a value (m,), a gradient (m, n) and a
+Hessian (m, n, n) over n partials, and propagates them through +, −, ×, ÷, powers and the
+numpy functions sin, cos, tan, exp, log, sqrt, tanh, arctan (chain rule to second order).
+"""
+import numpy as np
+
+
+class Dual2:
+    __array_priority__ = 100
+
+    def __init__(self, v, g, h):
+        self.v, self.g, self.h = v, g, h
+
+    @staticmethod
+    def seed(value, slot, n, scale=1.0):
+        """The variable `value` (m,) as partial number `slot` of n, scaled: x = value + scale·δ."""
+        v = np.asarray(value, dtype=float)
+        g = np.zeros(v.shape + (n,))
+        g[..., slot] = scale
+        return Dual2(v, g, np.zeros(v.shape + (n, n)))
+
+    # ---- helpers
+    def _const(self, c):
+        c = np.asarray(c, dtype=float)
+        return Dual2(np.broadcast_to(c, self.v.shape).astype(float), np.zeros_like(self.g), np.zeros_like(self.h))
+
+    def _lift(self, o):
+        return o if isinstance(o, Dual2) else self._const(o)
+
+    def chain(self, f, f1, f2):
+        """f(x) with f, f′, f″ evaluated at x.v."""
+        g1 = f1[..., None] * self.g
+        h = f1[..., None, None] * self.h + f2[..., None, None] * (self.g[..., :, None] * self.g[..., None, :])
+        return Dual2(f, g1, h)
+
+    # ---- arithmetic
+    def __add__(self, o):
+        if not isinstance(o, Dual2):
+            return Dual2(self.v + o, self.g, self.h)
+        return Dual2(self.v + o.v, self.g + o.g, self.h + o.h)
+
+    __radd__ = __add__
+
+    def __neg__(self):
+        return Dual2(-self.v, -self.g, -self.h)
+
+    def __sub__(self, o):
+        return self + (-o)
+
+    def __rsub__(self, o):
+        return (-self) + o
+
+    def __mul__(self, o):
+        if not isinstance(o, Dual2):
+            o = np.asarray(o, dtype=float)
+            return Dual2(self.v * o, self.g * o[..., None], self.h * o[..., None, None])
+        gg = self.g[..., :, None] * o.g[..., None, :]
+        return Dual2(self.v * o.v, self.g * o.v[..., None] + o.g * self.v[..., None],
+                     self.h * o.v[..., None, None] + o.h * self.v[..., None, None] + gg + np.swapaxes(gg, -1, -2))
+
+    __rmul__ = __mul__
+
+    def reciprocal(self):
+        x = self.v
+        return self.chain(1. / x, -1. / x ** 2, 2. / x ** 3)
+
+    def __truediv__(self, o):
+        if not isinstance(o, Dual2):
+            return self * (1. / np.asarray(o, dtype=float))
+        return self * o.reciprocal()
+
+    def __rtruediv__(self, o):
+        return self.reciprocal() * o
+
+    def __pow__(self, p):
+        if isinstance(p, Dual2):
+            return np.exp(p * np.log(self))
+        x = self.v
+        return self.chain(x ** p, p * x ** (p - 1), p * (p - 1) * x ** (p - 2) if p != 1 else np.zeros_like(x))
+
+    def __rpow__(self, b):
+        return np.exp(self * np.log(b))
+
+    # ---- numpy functions
+    _FUN = {
+        np.sin: lambda x: (np.sin(x), np.cos(x), -np.sin(x)),
+        np.cos: lambda x: (np.cos(x), -np.sin(x), -np.cos(x)),
+        np.tan: lambda x: (np.tan(x), 1. / np.cos(x) ** 2, 2. * np.tan(x) / np.cos(x) ** 2),
+        np.exp: lambda x: (np.exp(x), np.exp(x), np.exp(x)),
+        np.log: lambda x: (np.log(x), 1. / x, -1. / x ** 2),
+        np.sqrt: lambda x: (np.sqrt(x), 0.5 / np.sqrt(x), -0.25 / x ** 1.5),
+        np.tanh: lambda x: (np.tanh(x), 1. - np.tanh(x) ** 2, -2. * np.tanh(x) * (1. - np.tanh(x) ** 2)),
+        np.arctan: lambda x: (np.arctan(x), 1. / (1. + x * x), -2. * x / (1. + x * x) ** 2),
+    }
+
+    def __array_ufunc__(self, ufunc, method, *inputs, **kwargs):
+        if method != "__call__":
+            return NotImplemented
+        if ufunc in self._FUN and len(inputs) == 1:
+            return self.chain(*self._FUN[ufunc](self.v))
+        if ufunc is np.add:
+            return inputs[0] + inputs[1] if isinstance(inputs[0], Dual2) else inputs[1] + inputs[0]
+        if ufunc is np.multiply:
+            return inputs[0] * inputs[1] if isinstance(inputs[0], Dual2) else inputs[1] * inputs[0]
+        if ufunc is np.subtract:
+            return inputs[0] - inputs[1] if isinstance(inputs[0], Dual2) else -(inputs[1] - inputs[0])
+        if ufunc is np.true_divide:
+            return inputs[0] / inputs[1] if isinstance(inputs[0], Dual2) else inputs[1].__rtruediv__(inputs[0])
+        if ufunc is np.power and isinstance(inputs[0], Dual2):
+            return inputs[0] ** inputs[1]
+        if ufunc is np.negative:
+            return -inputs[0]
+        return NotImplemented
+
+
+def exp10(x):
+    """10^x for floats and Dual2 (Muscade's exp10 in test cost functions)."""
+    return np.exp(x * np.log(10.)) if isinstance(x, Dual2) else 10. ** x
+
+
+def call_cost(cost, args, costargs):
+    """cost(args..., costargs...): a tuple splats positionally, a dict (NamedTuple) by name."""
+    if isinstance(costargs, dict):
+        return cost(*args, **costargs)
+    return cost(*args, *tuple(costargs))

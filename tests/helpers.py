@@ -27,11 +27,15 @@ def build_lattice(nx, ny, nz, seed=7, disp=1e-3, rot=1e-2, sigma=0.2):
 KIND_TO_ORACLE = {1: orc.K_BEAM, 2: orc.K_BEAM, 3: orc.K_HOLD, 4: orc.K_DOFLOAD, 5: orc.K_ZERO, 6: orc.K_BAR, 7: orc.K_BAR}
 
 
-def oracle_model(model, dis, t=0.0):
-    """The same model for the oracle's sequential assembler."""
-    om = orc.OracleModel(model.ndof["X"], model.ndof["U"], model.ndof["A"])
+def oracle_model(model, dis, t=0.0, skip=(), nA=None):
+    """The same model for the oracle's sequential assembler (element types in `skip` left out)."""
+    om = orc.OracleModel(model.ndof["X"], model.ndof["U"], model.ndof["A"] if nA is None else nA)
     xyz = model.coords()
+    io = -1
     for it, E in enumerate(model.eletypes):
+        if it in skip:
+            continue
+        io += 1
         d = dis.dis[it]
         kind = KIND_TO_ORACLE[E.cls.kind]
         params = None
@@ -50,7 +54,7 @@ def oracle_model(model, dis, t=0.0):
                                      for n, p in zip(E.nodes, E.params)])
         om.add_type(kind, d.X, d.U if d.U.shape[1] else None, d.A if d.A.shape[1] else None, d.scaleX, params,
                     udof=int(getattr(E.cls, "Udof", False)))
-        om.set_scales(it, d.scaleΛ, d.scaleU if d.U.shape[1] else None)
+        om.set_scales(io, d.scaleΛ, d.scaleU if d.U.shape[1] else None)
     om.prepare()
     om.set_time(t)
     return om
@@ -97,3 +101,44 @@ class OracleAssembly:
     def Lλx(self):
         om = self.om
         return self._sp.csc_matrix((self.nzval.copy(), om.rowval - 1, om.colptr - 1), shape=(self.nX, self.nX))
+
+
+class OracleAssemblyDirect:
+    """The oracle's AssemblyDirect behind the interface of muscade_amd.AssemblyDirect, so that the
+    DirectXUA host solver can be run against it (test infrastructure only)."""
+
+    def __init__(self, OX, OU, IA, model, dis, device=0, skip=()):
+        self.OX, self.OU, self.IA, self.model, self.dis, self.skip = OX, OU, IA, model, dis, tuple(skip)
+        self.nX, self.nU = model.ndof["X"], model.ndof["U"]
+        self.nder = {"Λ": 1, "X": OX + 1, "U": OU + 1}
+
+    def assemble(self, state, Λ, mission="matrices", Δt=1.0, dbg=None):
+        import scipy.sparse as sp
+        om = oracle_model(self.model, self.dis, t=state.time, skip=self.skip, nA=0)
+        om.prepare_direct(self.OX, self.OU, 0)
+        U = [state.U[d] for d in range(self.OU + 1)] if self.nU else None
+        r, L1f, L2f = om.assemble_direct(1 if mission == "matrices" else 0, Δt, Λ, state.X[:self.OX + 1], U,
+                                         t=state.time)
+        assert r == 0, r
+        nX, nU, OX, OU = self.nX, self.nU, self.OX, self.OU
+        L1 = {"Λ": [L1f[:nX]], "X": [L1f[nX * (1 + d):nX * (2 + d)] for d in range(OX + 1)],
+              "U": [L1f[nX * (OX + 2) + nU * d:nX * (OX + 2) + nU * (d + 1)] for d in range(OU + 1)]}
+        cls = ("Λ", "X", "U")
+        L2, off = {}, 0
+        for a in range(3):
+            for b in range(3):
+                if a == 0 and b == 0:
+                    continue
+                cp, rv = om.dpattern[(a, b)]
+                n = rv.size
+                shape = (nU if a == 2 else nX, nU if b == 2 else nX)
+                blocks = []
+                for i in range(self.nder[cls[a]]):
+                    row = []
+                    for j in range(self.nder[cls[b]]):
+                        v = L2f[off:off + n]
+                        off += n
+                        row.append(sp.csc_matrix((v.copy(), rv - 1, cp - 1), shape=shape))
+                    blocks.append(row)
+                L2[(cls[a], cls[b])] = blocks
+        return L1, L2

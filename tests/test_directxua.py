@@ -1,0 +1,152 @@
+"""solve(DirectXUA{OX,OU,IA}) host logic (muscade_amd/directxua.py, src/DirectXUA.jl:245-513), run
+against the oracle's per-step AssemblyDirect (tests/helpers.py OracleAssemblyDirect) so it needs
+no GPU. Pins:
+- the finite-difference kernels (src/FiniteDifferences.jl:2-34) are exact on polynomials of
+  their order;
+- the host second-order duals agree with central differences of the same functions;
+- DirectXUA{0,0,0} on a clamped cantilever equals SweepX{0}, step by step;
+- an inverse problem (a U-load on the tip, a cost on its mismatch to a target displacement)
+  reaches the optimum that a reduced-space 1-D search over forward SweepX{0} solves finds;
+- DirectXUA{2,0,0}: the converged history satisfies R(X, X′, X″) = 0 at every step, with the
+  time derivatives that the finite differences of X give.
+"""
+import numpy as np
+import pytest
+
+import muscade_amd as mc
+from muscade_amd.directxua import finitediff
+from tests.helpers import OracleAssembly, OracleAssemblyDirect
+
+
+def test_finitediff_kernels_exact_on_polynomials():
+    n = 9
+    s = np.arange(1, n + 1, dtype=float)
+    for f, d1, d2 in ((lambda x: 3 * x + 1, lambda x: 3 + 0 * x, None),
+                      (lambda x: x * x - 2 * x, None, lambda x: 2 + 0 * x)):
+        y = f(s)
+        for k in range(1, n + 1):
+            if d1 is not None:
+                assert sum(w * y[k - 1 + Δ] for Δ, w in finitediff(1, n, k)) == pytest.approx(d1(s[k - 1]))
+            if d2 is not None:
+                assert sum(w * y[k - 1 + Δ] for Δ, w in finitediff(2, n, k)) == pytest.approx(d2(s[k - 1]))
+    assert finitediff(0, 3, 2) == [(0, 1.)]
+    with pytest.raises(mc.MuscadeError):
+        finitediff(1, 5, 2)
+
+
+def test_dual2_against_central_differences():
+    rng = np.random.default_rng(0)
+    x0 = rng.uniform(0.2, 0.9, size=(5, 3))
+
+    def f(a, b, c):
+        return np.sin(a) * b ** 2 / (1. + c * c) + np.exp(a * c) - np.sqrt(b + 3.) + mc.exp10(0.3 * c) * np.log(b + 1.)
+
+    D = [mc.Dual2.seed(x0[:, k], k, 3, 1.) for k in range(3)]
+    y = f(*D)
+    h = 1e-4
+    for k in range(3):
+        e = np.zeros(3); e[k] = h
+        gp = (f(*(x0 + e).T) - f(*(x0 - e).T)) / (2 * h)
+        assert np.abs(y.g[:, k] - gp).max() <= 1e-7
+        for j in range(3):
+            ej = np.zeros(3); ej[j] = h
+            hp = (f(*(x0 + e + ej).T) - f(*(x0 + e - ej).T) - f(*(x0 - e + ej).T) + f(*(x0 - e - ej).T)) / (4 * h * h)
+            assert np.abs(y.h[:, k, j] - hp).max() <= 1e-5
+    assert np.allclose(y.v, f(*x0.T))
+
+
+def cantilever(nel=6, F=2e3):
+    from muscade_amd.workloads import cantilever as cl
+    return cl(nel=nel, F=F, nstep=6)
+
+
+def test_static_directxua_equals_sweepx():
+    model = cantilever()
+    st0 = mc.initialize(model, time=0.)
+    time = np.arange(1., 7.)
+    ref = mc.solve(mc.SweepX[0], initialstate=st0, time=time, maxΔx=1e-12, verbose=False, catcherror=False,
+                   Assembly=OracleAssembly)
+    # X holds the Holds' reactions (λ-fields, ~1e3 here): 1e-9 is near the round-off of the bordered system
+    out = mc.solve(mc.DirectXUA[0, 0, 0], initialstate=[st0], time=[time], maxΔx=1e-9, maxΔλ=1e-9,
+                   verbose=False, catcherror=False, Assembly=OracleAssemblyDirect)
+    disp = model.getdofID("X", "t3", np.arange(1, 8)) - 1
+    for sr, sd in zip(ref, out[0]):
+        assert np.abs(sd.X[0] - sr.X[0]).max() <= 1e-10 * np.abs(sr.X[0]).max()
+        assert np.abs(sd.X[0][disp] - sr.X[0][disp]).max() <= 1e-9 * np.abs(sr.X[0][disp]).max()
+        assert np.abs(sd.Λ[0]).max() == 0.   # no cost: the adjoint stays at zero
+
+
+def test_inverse_tip_load_matches_reduced_search():
+    """min over u of cu·u² + cx·(x_tip(u) − x_m(t))² per step, x_tip from the static beam: the
+    full-space DirectXUA Newton and a 1-D search over forward solves agree."""
+    from scipy.optimize import minimize_scalar
+    nel, cu, cx = 4, 1e-8, 1e2
+
+    def xm(t):
+        return 0.02 * t
+
+    def build(with_u):
+        from muscade_amd.workloads import base_section
+        model = mc.Model("inverse")
+        x = np.linspace(0., 2., nel + 1)
+        mc.addnode(model, np.stack([x, 0 * x, 0 * x], axis=1))
+        mc.addelement(model, mc.EulerBeam3D, np.stack([np.arange(1, nel + 1), np.arange(2, nel + 2)], axis=1),
+                      mat=base_section(), orient2=np.array([0., 1., 0.]))
+        for f in ("t1", "t2", "t3", "r1", "r2", "r3"):
+            mc.addelement(model, mc.Hold, [1], field=f)
+        if with_u:
+            mc.addelement(model, mc.SingleUdof, [nel + 1], Xfield="t3", Ufield="ut3", cost=lambda u, t: cu * u * u)
+            mc.addelement(model, mc.SingleDofCost, [nel + 1], dofclass="X", field="t3",
+                          cost=lambda x, t: cx * (x - xm(t)) ** 2)
+        return model
+
+    model = build(True)
+    st0 = mc.initialize(model, time=0.)
+    time = np.arange(1., 4.)
+    out = mc.solve(mc.DirectXUA[0, 0, 0], initialstate=[st0], time=[time], maxΔx=1e-10, maxΔλ=1e-6, maxΔu=1e-6,
+                   maxiter=40, verbose=False, catcherror=False, Assembly=OracleAssemblyDirect)
+    tip = model.getdofID("X", "t3", [nel + 1])[0] - 1
+
+    def xtip(u):
+        m = build(False)
+        mc.addelement(m, mc.DofLoad, [nel + 1], field="t3", value=lambda t, F=u: F)
+        s = mc.solve(mc.SweepX[0], initialstate=mc.initialize(m, time=0.), time=[1.], maxΔx=1e-13, verbose=False,
+                     catcherror=False, Assembly=OracleAssembly)
+        return s[-1].X[0][tip]
+
+    for k, t in enumerate(time):
+        sd = out[0][k]
+        u = sd.U[0][0]
+        J = lambda v: cu * v * v + cx * (xtip(v) - xm(t)) ** 2   # noqa: E731
+        r = minimize_scalar(J, bracket=(0.5 * u, 1.5 * u), tol=1e-10)
+        assert abs(u - r.x) <= 1e-5 * abs(r.x), (t, u, r.x)
+        assert abs(sd.X[0][tip] - xtip(u)) <= 1e-8
+
+
+def test_dynamic_directxua_residual_vanishes():
+    from muscade_amd.workloads import base_section
+    nel = 4
+    model = mc.Model("dyn")
+    x = np.linspace(0., 2., nel + 1)
+    mc.addnode(model, np.stack([x, 0 * x, 0 * x], axis=1))
+    sec = base_section()
+    mc.addelement(model, mc.EulerBeam3D, np.stack([np.arange(1, nel + 1), np.arange(2, nel + 2)], axis=1), mat=sec,
+                  orient2=np.array([0., 1., 0.]))
+    for f in ("t1", "t2", "t3", "r1", "r2", "r3"):
+        mc.addelement(model, mc.Hold, [1], field=f)
+    mc.addelement(model, mc.DofLoad, [nel + 1], field="t3", value=lambda t: 50. * np.sin(t))
+    st0 = mc.initialize(model, time=0.)
+    time = np.arange(1., 9.) * 0.1
+    out = mc.solve(mc.DirectXUA[2, 0, 0], initialstate=[st0], time=[time], maxΔx=1e-10, maxΔλ=1e-6,
+                   verbose=False, catcherror=False, Assembly=OracleAssemblyDirect)
+    asm = OracleAssemblyDirect(2, 0, 0, model, st0.dis)
+    X = np.array([s.X[0] for s in out[0]])
+    dt = time[1] - time[0]
+    for k, s in enumerate(out[0]):
+        # the state's derivatives are the finite differences of its X history
+        for der in (1, 2):
+            fd = sum(w * X[k + Δ] for Δ, w in finitediff(der, len(time), k + 1)) / dt ** der
+            assert np.abs(s.X[der] - fd).max() <= 1e-9 * (1. + np.abs(fd).max())
+        L1, _ = asm.assemble(s, s.Λ[0], "vectors", 1.0)
+        assert np.abs(L1["Λ"][0]).max() <= 1e-7 * 50.
+    assert np.abs(X[:, model.getdofID("X", "t3", [nel + 1])[0] - 1]).max() > 1e-6

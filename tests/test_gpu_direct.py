@@ -29,9 +29,26 @@ def add_supports(mc, model, nodes, rng):
     mc.addelement(model, mc.DofLoad, np.array([[nodes[-1]]]), field="t2", value=load)
 
 
+def pollute_device_memory(mc):
+    """Fill and free device memory first, so a buffer the library fails to initialise shows up
+    (freshly mapped memory is zero; reused memory is not)."""
+    from tests.helpers import build_lattice
+    m, st = build_lattice(12, 10, 6, seed=3, rot=5e-2, disp=1e-2)
+    a = mc.AssemblySweepX(0, m, st.dis)
+    a.assemble(st, "iter", fetch=False)
+    del a
+
+
 def compare(mc, model, state, lam, OX, OU, dt=0.05):
+    pollute_device_memory(mc)
     out = mc.AssemblyDirect(OX, OU, 0, model, state.dis)
     L1, L2 = out.assemble(state, lam, "matrices", dt)
+    # L1[X][1] comes only from Lagrangian-path elements (Hold, DofLoad): zero elsewhere, exactly
+    touched = np.zeros(model.ndof["X"], bool)
+    for it, E in enumerate(model.eletypes):
+        if E.cls.kind in (3, 4):
+            touched[state.dis.dis[it].X.ravel() - 1] = True
+    assert np.all(L1["X"][0][~touched] == 0.)
     om = oracle_model(model, state.dis, t=state.time)
     om.prepare_direct(OX, OU, 0)
     Us = [state.U[0]] + [np.zeros(model.ndof["U"])] * OU if model.ndof["U"] else None
