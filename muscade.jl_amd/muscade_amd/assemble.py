@@ -210,13 +210,13 @@ class AssemblyDirect:
     matrices with the class-pair patterns, as out.L1/out.L2 of the reference (without the
     empty L2[Λ,Λ])."""
 
-    def __init__(self, OX, OU, IA, model, dis, device=0, skip=()):
+    def __init__(self, OX, OU, IA, model, dis, device=0, skip=(), nA=None):
         """skip: element types left to the host (user-cost elements, whose Lagrangians the
         DirectXUA solver evaluates itself); the device then assembles the others only."""
         self.OX, self.OU, self.IA = OX, OU, IA
         self.model, self.dis = model, dis
         self.ctx = DeviceContext(device)
-        self.groups = upload_model(self.ctx, model, dis, skip=skip, nA=0 if skip else None)
+        self.groups = upload_model(self.ctx, model, dis, skip=skip, nA=0 if (skip or nA == 0) else None)
         for it, g in enumerate(self.groups):
             d = dis.dis[it]
             if g is not None and d.X.shape[1]:

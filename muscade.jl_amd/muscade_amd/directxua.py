@@ -152,6 +152,82 @@ class HostLagrangians:
         return gA, pieces
 
 
+class _Replica:
+    """n disjoint copies of a model's device element types, numbered copy after copy: copy s
+    owns nodes s·nnod.., X-dofs s·nX.. and U-dofs s·nU.. (just what upload_model reads)."""
+
+    class _Type:
+        def __init__(self, E, n, nnod):
+            self.cls, self.key = E.cls, E.key
+            self._nodes = np.concatenate([E.cat_nodes() + s * nnod for s in range(n)])
+            self.params = [p for s in range(n) for p in E.params]
+            self.nodes = [q for s in range(n) for q in E.nodes]
+
+        def cat_nodes(self):
+            return self._nodes
+
+    class _Dis:
+        def __init__(self, d, n, nX, nU):
+            self.X = np.concatenate([d.X + s * nX for s in range(n)])
+            self.U = np.concatenate([d.U + s * nU for s in range(n)])
+            self.A = np.zeros((self.X.shape[0], 0), np.int64)
+            self.scaleX, self.scaleU, self.scaleA, self.scaleΛ = d.scaleX, d.scaleU, d.scaleA, d.scaleΛ
+
+    def __init__(self, model, dis, types, n):
+        nX, nU = model.ndof["X"], model.ndof["U"]
+        self.nnod = model.nnod * n
+        self._xyz = np.tile(model.coords(), (n, 1))
+        self.ndof = {"X": nX * n, "U": nU * n, "A": 0}
+        self.eletypes = [self._Type(model.eletypes[it], n, model.nnod) for it in types]
+        self.dis = [self._Dis(dis.dis[it], n, nX, nU) for it in types]
+
+    def coords(self):
+        return self._xyz
+
+
+class StepBatchAssembly:
+    """The residual elements of all steps of one experiment in one device launch: AssemblyDirect
+    on `nstep` disjoint copies of them (copy s carries step s's state), whose block-diagonal
+    step blocks are cut back out. Each step block is bitwise what a launch of that step alone
+    gives: the same element values, summed in the same element order."""
+
+    def __init__(self, OX, OU, IA, model, dis, nstep, skip=(), device=0):
+        types = [it for it in range(len(model.eletypes)) if it not in skip]
+        for it in types:
+            if model.eletypes[it].cls.kind not in (1, 2):   # beams only: bars read the time
+                raise MuscadeError("step batching needs time-independent residual elements (beams)")
+        self.rep = _Replica(model, dis, types, nstep)
+
+        class _RDis:
+            pass
+        rdis = _RDis()
+        rdis.dis = self.rep.dis
+        self.nstep, self.nX, self.nU = nstep, model.ndof["X"], model.ndof["U"]
+        self.OX, self.OU = OX, OU
+        self.out = AssemblyDirect(OX, OU, IA, self.rep, rdis, device=device, skip=(), nA=0)
+
+    def assemble_steps(self, states, dbg=None):
+        """L1, L2 of every step (lists over steps, as AssemblyDirect.assemble returns them)."""
+        class _S:
+            pass
+        st = _S()
+        st.time = states[0].time
+        st.X = [np.concatenate([s.X[d] for s in states]) for d in range(self.OX + 1)]
+        st.U = [np.concatenate([s.U[d] for s in states]) for d in range(self.OU + 1)]
+        lam = np.concatenate([s.Λ[0] for s in states])
+        L1, L2 = self.out.assemble(st, lam, "matrices", 1.0, dbg=dbg)
+        n = {"Λ": self.nX, "X": self.nX, "U": self.nU}
+        res = []
+        for k in range(self.nstep):
+            l1 = {c: [v[k * n[c]:(k + 1) * n[c]] for v in vs] for c, vs in L1.items()}
+            l2 = {}
+            for (a, b), blocks in L2.items():
+                ra, rb = slice(k * n[a], (k + 1) * n[a]), slice(k * n[b], (k + 1) * n[b])
+                l2[(a, b)] = [[M[ra, rb] for M in row] for row in blocks]
+            res.append((l1, l2))
+        return res
+
+
 class _DirectXUAMeta(type):
     def __getitem__(cls, key):
         OX, OU, IA = key
@@ -166,9 +242,10 @@ class DirectXUA(metaclass=_DirectXUAMeta):
 
     @classmethod
     def run(cls, pstate, verbose, dbg, *, time, initialstate, maxiter=50, maxΔλ=1e-5, maxΔx=1e-5, maxΔu=1e-5,
-            maxΔa=1e-5, saveiter=False, device=0, Assembly=AssemblyDirect):
+            maxΔa=1e-5, saveiter=False, device=0, Assembly=AssemblyDirect, batch_steps=False):
         """`Assembly` is the per-step assembler of the device element kinds (prepare(AssemblyDirect{OX,OU,IA}),
-        src/DirectXUA.jl:22-56)."""
+        src/DirectXUA.jl:22-56). batch_steps: all steps of an experiment in one device launch per
+        Newton iteration (StepBatchAssembly; beams only)."""
         OX, OU, IA = cls.OX, cls.OU, cls.IA
         nexp = len(time)
         if len(initialstate) != nexp:
@@ -199,7 +276,11 @@ class DirectXUA(metaclass=_DirectXUAMeta):
         pstate[0] = stateiter if saveiter else state
 
         host = HostLagrangians(model, dis, OX, OU, IA)
-        out = Assembly(OX, OU, IA, model, dis, device=device, skip=tuple(host.types))
+        if batch_steps:
+            batched = [StepBatchAssembly(OX, OU, IA, model, dis, n, skip=tuple(host.types), device=device)
+                       for n in nstep]
+        else:
+            out = Assembly(OX, OU, IA, model, dis, device=device, skip=tuple(host.types))
         per_step = 2 * nX + nU
         exp0 = np.cumsum([0] + [n * per_step for n in nstep])
         nv = int(exp0[-1]) + (nA_used if IA else 0)
@@ -225,10 +306,15 @@ class DirectXUA(metaclass=_DirectXUAMeta):
 
             for iexp in range(nexp):
                 dt = Δt[iexp]
+                if batch_steps:
+                    stepblocks = batched[iexp].assemble_steps(state[iexp], dbg=dict(dbg, iter=it))
                 for istep in range(1, nstep[iexp] + 1):
                     st = state[iexp][istep - 1]
                     st.SP = dict(γ=0., iter=it)
-                    L1d, L2d = out.assemble(st, st.Λ[0], "matrices", 1.0, dbg=dict(dbg, iter=it, step=istep))
+                    if batch_steps:
+                        L1d, L2d = stepblocks[istep - 1]
+                    else:
+                        L1d, L2d = out.assemble(st, st.Λ[0], "matrices", 1.0, dbg=dict(dbg, iter=it, step=istep))
                     L1h, L2h = host.step(st)
                     # the device kinds take the residual path, which leaves Λᵀ∂R/∂(X,U) out of L1
                     # (src/DirectXUA.jl:104-119); add it from the same Jacobian blocks

@@ -49,3 +49,21 @@ def test_directxua_gpu_matches_oracle_assembly(mc, OX, inverse):
         assert d <= 1e-7 * np.abs(so.X[0][disp]).max() + 1e-12, d
         if inverse:
             assert abs(sg.U[0][0] - so.U[0][0]) <= 1e-7 * abs(so.U[0][0]) + 1e-9
+
+
+@pytest.mark.parametrize("OX,inverse", [(2, False), (2, True)])
+def test_directxua_step_batching_is_bitwise(mc, OX, inverse):
+    """All steps in one device launch (StepBatchAssembly) give the per-step launches' history
+    bit for bit."""
+    model = beam_model(mc, dyn_load=not inverse, inverse=inverse)
+    st0 = mc.initialize(model, time=0.)
+    time = np.arange(1., 9.) * 0.1
+    kw = dict(initialstate=[st0], time=[time], maxΔx=1e-9, maxΔλ=1e-6, maxΔu=1e-6, maxiter=30, verbose=False,
+              catcherror=False)
+    a = mc.solve(mc.DirectXUA[OX, 0, 0], batch_steps=False, **kw)
+    b = mc.solve(mc.DirectXUA[OX, 0, 0], batch_steps=True, **kw)
+    for sa, sb in zip(a[0], b[0]):
+        for k in range(OX + 1):
+            assert np.array_equal(sa.X[k].view(np.uint64), sb.X[k].view(np.uint64))
+        assert np.array_equal(sa.Λ[0].view(np.uint64), sb.Λ[0].view(np.uint64))
+        assert np.array_equal(sa.U[0].view(np.uint64), sb.U[0].view(np.uint64))
