@@ -283,6 +283,39 @@ def newton_bench(args):
     return res
 
 
+def directxua_bench(args):
+    """solve(DirectXUA{2,0,0}) of a load-identification problem: a 100-beam curved cantilever over
+    40 steps, a U-load at the tip and a cost on the tip's deviation from a target motion.
+    Residual elements per step on the GPU, one launch per step or all steps in one launch;
+    the all-steps system and its LU on the host."""
+    import muscade_amd as mc
+    from muscade_amd.workloads import chain_section
+    nel, nstep = 100, 40
+    model = mc.Model("directxua")
+    x = np.linspace(0., 20., nel + 1)
+    mc.addnode(model, np.stack([x, 0 * x, 0.02 * x * x], axis=1))
+    mc.addelement(model, mc.EulerBeam3D, np.stack([np.arange(1, nel + 1), np.arange(2, nel + 2)], axis=1),
+                  mat=chain_section(), orient2=np.array([0., 1., 0.]))
+    for f in ("t1", "t2", "t3", "r1", "r2", "r3"):
+        mc.addelement(model, mc.Hold, [1], field=f)
+    mc.addelement(model, mc.SingleUdof, [nel + 1], Xfield="t2", Ufield="ut2", cost=lambda u, t: 1e-10 * u * u)
+    mc.addelement(model, mc.SingleDofCost, [nel + 1], dofclass="X", field="t2",
+                  cost=lambda x, t: 1e3 * (x - 1e-2 * np.sin(t)) ** 2)
+    st0 = mc.initialize(model, time=0.)
+    time_ = np.arange(1, nstep + 1) * 0.05
+    res = {"workload": f"load identification, {nel} EulerBeam3D x {nstep} steps, DirectXUA{{2,0,0}}",
+           "unknowns": nstep * (2 * model.ndof["X"] + model.ndof["U"])}
+    for mode in (False, True):
+        st = {}
+        t0 = time.perf_counter()
+        mc.solve(mc.DirectXUA[2, 0, 0], initialstate=[st0], time=[time_], maxiter=30, verbose=False,
+                 catcherror=False, batch_steps=mode, stats=st)   # the reference's default tolerances (1e-5)
+        st["total_s"] = time.perf_counter() - t0
+        res["steps_in_one_launch" if mode else "launch_per_step"] = st
+        log(f"directxua ({'one launch' if mode else 'per step'}): {st}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,6 +333,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the batched vs plain adjoint measurement")
     ap.add_argument("--no-graph", action="store_true", help="skip the eager vs HIP-graph small-model measurement")
     ap.add_argument("--no-newton", action="store_true", help="skip the reduced-lattice Newton step (host LU) timing")
+    ap.add_argument("--no-directxua", action="store_true", help="skip the DirectXUA load-identification solve")
+    ap.add_argument("--only-directxua", action="store_true", help="measure only the DirectXUA solve")
     ap.add_argument("--only-graph", action="store_true", help="measure only eager vs HIP-graph assembly")
     ap.add_argument("--only-adjoint", action="store_true", help="measure only the batched adjoint gradients")
     ap.add_argument("--models", type=int, default=512)
@@ -325,6 +360,9 @@ def main():
         return
     if args.only_graph:
         print(json.dumps({"graph": graph_bench(args)}))
+        return
+    if args.only_directxua:
+        print(json.dumps({"directxua": directxua_bench(args)}))
         return
     import muscade_amd as mc
     from muscade_amd import _lib
@@ -455,6 +493,8 @@ def main():
         line["graph"] = graph_bench(args)
     if not args.no_newton:
         line["newton_step"] = newton_bench(args)
+    if not args.no_directxua:
+        line["directxua"] = directxua_bench(args)
     print(json.dumps(line))
 
 

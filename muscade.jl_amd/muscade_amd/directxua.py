@@ -31,6 +31,7 @@ Lagrangians on the host, so the device assembles residual elements only.
 """
 import copy
 import math
+import time as _time
 
 import numpy as np
 import scipy.sparse as sp
@@ -242,10 +243,14 @@ class DirectXUA(metaclass=_DirectXUAMeta):
 
     @classmethod
     def run(cls, pstate, verbose, dbg, *, time, initialstate, maxiter=50, maxΔλ=1e-5, maxΔx=1e-5, maxΔu=1e-5,
-            maxΔa=1e-5, saveiter=False, device=0, Assembly=AssemblyDirect, batch_steps=False):
+            maxΔa=1e-5, saveiter=False, device=0, Assembly=AssemblyDirect, batch_steps=False, stats=None):
         """`Assembly` is the per-step assembler of the device element kinds (prepare(AssemblyDirect{OX,OU,IA}),
         src/DirectXUA.jl:22-56). batch_steps: all steps of an experiment in one device launch per
-        Newton iteration (StepBatchAssembly; beams only)."""
+        Newton iteration (StepBatchAssembly; beams only). stats (a dict) receives the iteration count
+        and the seconds spent in device assembly, host assembly and the sparse LU."""
+        clock = _time.perf_counter
+        st_ = stats if stats is not None else {}
+        st_.update(iterations=0, device_assembly_s=0., host_assembly_s=0., lu_s=0.)
         OX, OU, IA = cls.OX, cls.OU, cls.IA
         nexp = len(time)
         if len(initialstate) != nexp:
@@ -304,17 +309,23 @@ class DirectXUA(metaclass=_DirectXUAMeta):
             def off(iexp, istep, c):
                 return Aoff if c == "A" else block(iexp, istep, c)
 
+            t_iter = clock()
+            t_dev = 0.
             for iexp in range(nexp):
                 dt = Δt[iexp]
                 if batch_steps:
+                    t0 = clock()
                     stepblocks = batched[iexp].assemble_steps(state[iexp], dbg=dict(dbg, iter=it))
+                    t_dev += clock() - t0
                 for istep in range(1, nstep[iexp] + 1):
                     st = state[iexp][istep - 1]
                     st.SP = dict(γ=0., iter=it)
                     if batch_steps:
                         L1d, L2d = stepblocks[istep - 1]
                     else:
+                        t0 = clock()
                         L1d, L2d = out.assemble(st, st.Λ[0], "matrices", 1.0, dbg=dict(dbg, iter=it, step=istep))
+                        t_dev += clock() - t0
                     L1h, L2h = host.step(st)
                     # the device kinds take the residual path, which leaves Λᵀ∂R/∂(X,U) out of L1
                     # (src/DirectXUA.jl:104-119); add it from the same Jacobian blocks
@@ -354,12 +365,17 @@ class DirectXUA(metaclass=_DirectXUAMeta):
                                  (np.concatenate(rows) if rows else np.zeros(0, int),
                                   np.concatenate(cols) if cols else np.zeros(0, int))), shape=(nv, nv)).tocsc()
             Lvv.eliminate_zeros()
+            st_["device_assembly_s"] += t_dev
+            st_["host_assembly_s"] += clock() - t_iter - t_dev
+            st_["iterations"] = it
             if verbose:
                 print(", solving", end="")
+            t0 = clock()
             try:
                 Δv = spla.splu(Lvv).solve(Lv)
             except RuntimeError as e:
                 raise MuscadeError(f"Lvv matrix factorization failed: {e}", dict(dbg, iter=it))
+            st_["lu_s"] += clock() - t0
             if verbose:
                 print(", decrementing.")
             Δ2 = {c: 0. for c in classes}
