@@ -259,10 +259,6 @@ class DirectXUA(metaclass=_DirectXUAMeta):
         Δt = [float(t[1] - t[0]) if len(t) > 1 else 1.0 for t in time]
         model, dis = initialstate[0].model, initialstate[0].dis
         nX, nU, nA = model.ndof["X"], model.ndof["U"], model.ndof["A"]
-        if nA and not IA:
-            nA_used = 0
-        else:
-            nA_used = nA
         nder = {"Λ": 1, "X": OX + 1, "U": OU + 1, "A": IA}
         classes = CLASSES if IA else CLASSES[:3]
         maxΔ2 = {"Λ": maxΔλ ** 2, "X": maxΔx ** 2, "U": maxΔu ** 2, "A": maxΔa ** 2}
@@ -288,7 +284,7 @@ class DirectXUA(metaclass=_DirectXUAMeta):
             out = Assembly(OX, OU, IA, model, dis, device=device, skip=tuple(host.types))
         per_step = 2 * nX + nU
         exp0 = np.cumsum([0] + [n * per_step for n in nstep])
-        nv = int(exp0[-1]) + (nA_used if IA else 0)
+        nv = int(exp0[-1]) + (nA if IA else 0)   # A-dofs are unknowns only when IA = 1
         Aoff = int(exp0[-1])
         coff = {"Λ": 0, "X": nX, "U": 2 * nX}
 
