@@ -279,22 +279,56 @@ template <int NT, bool E> MH_HD Jet<1, E> jslice(const Jet<NT, E>& a, int n) {
     return r;
 }
 
+// The rotation chain (v₁,v₂) ↦ (rₛₘ, Δv) of BeamElement.jl:192-208 on time jets.
+template <int NT, bool E> MH_HD void dyn_rotation_chain(const BeamGeo& G, const JV3<NT, E>& v1, const JV3<NT, E>& v2,
+                                                        JM3<NT, E>& rsm, JV3<NT, E>& dv) {
+    JM3<NT, E> R1 = jrodrigues(v1);
+    JM3<NT, E> M = jmulABt(jrodrigues(v2), R1);
+    JV3<NT, E> phi = jrodrigues_inv(M);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dv.a[c] = 0.5 * phi.a[c];
+    rsm = jmul_const(jmul(jrodrigues(dv), R1), G.rm);
+}
+// A time jet with no ε part as one whose ε part is zero.
+template <int NT, bool E> MH_HD Jet<NT, E> jlift(const Jet<NT, false>& a) {
+    Jet<NT, E> r = jzero<NT, E>();
+#pragma unroll
+    for (int n = 0; n < NT; ++n) r.v[n] = a.v[n];
+    return r;
+}
+
 // Kinematics (BeamElement.jl:176-208) on time jets. Xj: the 12 element dofs as jets.
-template <int NT, bool E> MH_HD void dyn_kinematics(const BeamGeo& G, const Jet<NT, E>* Xj, DynKin<E>& k) {
+// ROTE = false: the ε seed is on a translation dof, so the rotation chain carries no ε part
+// and is evaluated on plain time jets (about half the work of a direction).
+template <int NT, bool E, bool ROTE = true>
+MH_HD void dyn_kinematics(const BeamGeo& G, const Jet<NT, E>* Xj, DynKin<E>& k) {
     using J = Jet<NT, E>;
     const double L = G.L;
-    JV3<NT, E> v1, v2, u1, u2;
+    JV3<NT, E> u1, u2;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) { u1.a[c] = Xj[c]; v1.a[c] = Xj[3 + c]; u2.a[c] = Xj[6 + c]; v2.a[c] = Xj[9 + c]; }
-    JM3<NT, E> R1 = jrodrigues(v1);
+    for (int c = 0; c < 3; ++c) { u1.a[c] = Xj[c]; u2.a[c] = Xj[6 + c]; }
     JM3<NT, E> rsm;
     JV3<NT, E> dv;
-    {
-        JM3<NT, E> M = jmulABt(jrodrigues(v2), R1);
-        JV3<NT, E> phi = jrodrigues_inv(M);
+    if constexpr (ROTE || !E) {
+        JV3<NT, E> v1, v2;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) dv.a[c] = 0.5 * phi.a[c];
-        rsm = jmul_const(jmul(jrodrigues(dv), R1), G.rm);
+        for (int c = 0; c < 3; ++c) { v1.a[c] = Xj[3 + c]; v2.a[c] = Xj[9 + c]; }
+        dyn_rotation_chain(G, v1, v2, rsm, dv);
+    } else {
+        JV3<NT, false> w1, w2;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int n = 0; n < NT; ++n) { w1.a[c].v[n] = Xj[3 + c].v[n]; w2.a[c].v[n] = Xj[9 + c].v[n]; }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { w1.a[c].e[0] = 0.; w2.a[c].e[0] = 0.; }
+        JM3<NT, false> rf;
+        JV3<NT, false> df;
+        dyn_rotation_chain(G, w1, w2, rf, df);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) rsm.a[i] = jlift<NT, E>(rf.a[i]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dv.a[c] = jlift<NT, E>(df.a[c]);
     }
     // uₗ₂ = rₛₘᵀ(u₂ + tgₘ/2 - cₛ) - tgₑ/2,  vₗ₂ = rₛₘᵀ Δv
     JV3<NT, E> dd, cs;
@@ -442,20 +476,26 @@ template <int ND> MH_HD void dyn_frozen_loads(const BeamGeo& G, const BeamDynSec
 // One trial direction (see dyn_seed): the ε-derivatives of the loads (DF, Dm) and,
 // for a δX direction, the column of the Jacobians of x_gp and vₛₘ w.r.t. X₀ (Jx, Jv).
 struct DynDir { double DF[4][3]; double Dm[3]; double Jx[4][3]; double Jv[3]; };
-template <int ND> MH_HD void dyn_direction(const BeamGeo& G, const BeamDynSection& D, const double (*X)[12],
-                                           const double* U, const double* scale, const double* c, int dir, DynDir& o,
-                                           double w0 = 1., const double* lam = nullptr) {
+// ROTE = false only for δX along a translation dof (0..2, 6..8): rₛₘ and vₛₘ then carry no ε.
+template <int ND, bool ROTE = true>
+MH_HD void dyn_direction(const BeamGeo& G, const BeamDynSection& D, const double (*X)[12], const double* U,
+                         const double* scale, const double* c, int dir, DynDir& o, double w0 = 1.,
+                         const double* lam = nullptr) {
     Jet<ND, true> Xj[12];
     dyn_seed<ND, true>(X, scale, c, dir, Xj, w0, lam);
     DynKin<true> k;
-    dyn_kinematics<ND, true>(G, Xj, k);
+    dyn_kinematics<ND, true, ROTE>(G, Xj, k);
     Jet<1, true> fe[4][3], me[3];
     dyn_loads<true>(k, D, U, fe, me);
     for (int g = 0; g < 4; ++g)
         for (int q = 0; q < 3; ++q) { o.DF[g][q] = fe[g][q].e[0]; o.Jx[g][q] = k.xe[g][q]; }
     for (int q = 0; q < 3; ++q) o.Dm[q] = me[q].e[0];
-    v3 jv = dyn_vsm_e(k.r0);
-    o.Jv[0] = jv.x; o.Jv[1] = jv.y; o.Jv[2] = jv.z;
+    if constexpr (ROTE) {
+        v3 jv = dyn_vsm_e(k.r0);
+        o.Jv[0] = jv.x; o.Jv[1] = jv.y; o.Jv[2] = jv.z;
+    } else {
+        o.Jv[0] = 0.; o.Jv[1] = 0.; o.Jv[2] = 0.;
+    }
 }
 // Row i of the load-variation part of the tangent along direction d:
 // Σ_gp dL_gp·(∂x_gp/∂X₀ᵢ · DF_gp + ∂vₛₘ/∂X₀ᵢ · Dm)   (Jx_i, Jv_i: column i of the Jacobians,

@@ -393,24 +393,34 @@ __global__ __launch_bounds__(kBeamBlock) void beam_dyn_value_kernel(BeamDynArgs 
     for (int64_t f = tid; f < nv; f += kBeamBlock) dst[f] = tile[f];
 }
 
-// Direction pass: 16 lanes per element, lane k = trial direction k (δX of dof k for
-// k < 12, δr for k = 12 on :step). Each lane propagates the time jets with its own ε
-// seed; the lanes of an element then exchange the Jacobian columns of x_gp and vₛₘ
-// through LDS, and lane k writes column k of K = (compact static part) +
-// Σ_gp dL·(J_xᵀ ∂fₑ/∂ε + J_vᵀ ∂mₑ/∂ε).
-constexpr int kDirLanes = 16;
+// Direction pass: one lane per trial direction of an element (δX of dof k, k < 12; δr on
+// :step). Each lane propagates the time jets with its own ε seed; the lanes of an element
+// then exchange the Jacobian columns of x_gp and vₛₘ through LDS, and the lane of direction k
+// writes column k of K = (compact static part) + Σ_gp dL·(J_xᵀ ∂fₑ/∂ε + J_vᵀ ∂mₑ/∂ε).
+// A translation direction leaves the rotation chain free of ε (about half the work), so a
+// workgroup of kDirWElems elements runs them in a wave of their own: wave 0 = the 6
+// translation directions of each element, wave 1 = the 6 rotation directions and δr. Each
+// wave then follows one code path.
+constexpr int kDirLanes = 16;              // the adjoint and DirectXUA kernels: 16 lanes per element
 constexpr int kDirElems = 16;
 constexpr int kDirBlock = kDirLanes * kDirElems;
+constexpr int kDirWElems = 9;              // SweepX direction pass: elements per 2-wave workgroup
+constexpr int kDirWBlock = 128;
 template <int ND, bool UDOF, bool STEP>
-__global__ __launch_bounds__(kDirBlock) void beam_dyn_dir_kernel(BeamDynArgs a) {
-    __shared__ double Jsh[kDirElems][12][15];
-    const int64_t e0 = xcd_tile(a.per_xcd) * kDirElems;
+__global__ __launch_bounds__(kDirWBlock) void beam_dyn_dir_kernel(BeamDynArgs a) {
+    __shared__ double Jsh[kDirWElems][12][15];
+    const int64_t e0 = xcd_tile(a.per_xcd) * kDirWElems;
     if (e0 >= a.nele) return;   // whole workgroup leaves together
-    const int el = threadIdx.x / kDirLanes, k = threadIdx.x % kDirLanes;
+    const bool rot = threadIdx.x >= 64;   // uniform within each wave
+    const int l = rot ? threadIdx.x - 64 : threadIdx.x;
+    const int per = rot ? 7 : 6;
+    const int el = l / per, j = l - el * per;
+    int dir;
+    if (!rot) dir = j < 3 ? j : j + 3;                          // t of node 1, t of node 2
+    else dir = j < 3 ? 3 + j : (j < 6 ? 6 + j : (STEP ? 12 : -1));   // r of node 1, r of node 2, δr
     const int64_t e = e0 + el;
     const int64_t n = a.nele;
-    const int dir = k < 12 ? k : ((STEP && k == 12) ? 12 : -1);
-    const bool live = e < n && dir >= 0;
+    const bool live = el < kDirWElems && e < n && dir >= 0;
     mh::BeamGeo G;
     mh::DynDir d;
     if (live) {
@@ -418,14 +428,15 @@ __global__ __launch_bounds__(kDirBlock) void beam_dyn_dir_kernel(BeamDynArgs a) 
         mh::BeamDynSection D;
         double X[ND][12], U[3];
         dyn_load<ND, UDOF>(a, e, G, S, D, X, U);
-        mh::dyn_direction<ND>(G, D, X, U, a.scale, a.c, dir, d);
-        if (k < 12) {
+        if (rot) mh::dyn_direction<ND, true>(G, D, X, U, a.scale, a.c, dir, d);
+        else mh::dyn_direction<ND, false>(G, D, X, U, a.scale, a.c, dir, d);
+        if (dir < 12) {
 #pragma unroll
             for (int g = 0; g < 4; ++g)
 #pragma unroll
-                for (int q = 0; q < 3; ++q) Jsh[el][k][3 * g + q] = d.Jx[g][q];
+                for (int q = 0; q < 3; ++q) Jsh[el][dir][3 * g + q] = d.Jx[g][q];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) Jsh[el][k][12 + q] = d.Jv[q];
+            for (int q = 0; q < 3; ++q) Jsh[el][dir][12 + q] = d.Jv[q];
         }
     }
     __syncthreads();
@@ -435,15 +446,15 @@ __global__ __launch_bounds__(kDirBlock) void beam_dyn_dir_kernel(BeamDynArgs a) 
 #pragma unroll
     for (int i = 0; i < 12; ++i) {
         col[i] = mh::dyn_row(G, (const double(*)[3])(&Jsh[el][i][0]), &Jsh[el][i][12], d);
-        if (k < 12) {
+        if (dir < 12) {
             double m;
-            const int slot = mh::compact_slot(i, k, &m);
+            const int slot = mh::compact_slot(i, dir, &m);
             col[i] = m * a.Kc[e * mh::kCompact + slot] + col[i];
         }
         nan |= col[i] != col[i];
     }
-    if (k < 12) {
-        double* dst = a.K + e * 144 + 12 * k;
+    if (dir < 12) {
+        double* dst = a.K + e * 144 + 12 * dir;
 #pragma unroll
         for (int i = 0; i < 12; ++i) dst[i] = col[i];
     } else {
@@ -1112,11 +1123,11 @@ void launch_dyn(BeamDynArgs& a, bool step, hipStream_t s) {
         a.per_xcd = (ntile + kXCD - 1) / kXCD;
         hipLaunchKernelGGL((beam_dyn_value_kernel<ND, UDOF>), dim3((unsigned)(a.per_xcd * kXCD)), dim3(kBeamBlock), 0, s, a);
     }
-    const int64_t ntile = (a.nele + kDirElems - 1) / kDirElems;
+    const int64_t ntile = (a.nele + kDirWElems - 1) / kDirWElems;
     a.per_xcd = (ntile + kXCD - 1) / kXCD;
     const dim3 gr((unsigned)(a.per_xcd * kXCD));
-    if (step) hipLaunchKernelGGL((beam_dyn_dir_kernel<ND, UDOF, true>), gr, dim3(kDirBlock), 0, s, a);
-    else hipLaunchKernelGGL((beam_dyn_dir_kernel<ND, UDOF, false>), gr, dim3(kDirBlock), 0, s, a);
+    if (step) hipLaunchKernelGGL((beam_dyn_dir_kernel<ND, UDOF, true>), gr, dim3(kDirWBlock), 0, s, a);
+    else hipLaunchKernelGGL((beam_dyn_dir_kernel<ND, UDOF, false>), gr, dim3(kDirWBlock), 0, s, a);
 }
 int launch_dyn_beams(Ctx* c, const Group& g, int OX, bool step, const double* c6, unsigned long long gtag) {
     BeamDynArgs a;
