@@ -377,7 +377,7 @@ def main():
     out = mc.AssemblySweepX(0, model, state.dis)
     t_prep = time.perf_counter() - t0
     log(f"prepare (device pattern, nnz={out.nnz}): {t_prep:.2f}s")
-    tv, tms = ctypes.c_int(), (ctypes.c_float * 8)()
+    tv, tms = ctypes.c_int(), (ctypes.c_float * 16)()
     _lib.lib.mh_get_tuning(out.ctx.h, ctypes.byref(tv), tms)
     tuning = {"scatter_variant": tv.value, "scatter_variant_ms": [round(float(x), 4) for x in tms]}
     log(f"scatter tuner: {tuning}")

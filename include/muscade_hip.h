@@ -170,8 +170,9 @@ int mh_last_timings(mh_ctx* ctx, double* ms);
  * n doubles on the context's stream, time in *ms. It is the known-size kernel
  * that HBM byte counters (FETCH_SIZE, WRITE_SIZE) are calibrated against. */
 int mh_calib_copy(mh_ctx* ctx, int64_t n, double* ms);
-/* Shape of the nz scatter picked by the prepare-time tuner (0..7: segments per lane ×
- * LDS chunk; see mh_lib.hip launch_scatter) and each shape's measured time in ms (8).
+/* Shape of the nz scatter picked by the prepare-time tuner (0..15: segments per lane ×
+ * LDS chunk, with or without nontemporal index loads and stores; see mh_lib.hip
+ * launch_scatter) and each shape's measured time in ms (16).
  * A measurement utility; no counterpart in the reference. */
 int mh_get_tuning(mh_ctx* ctx, int* variant, float* ms);
 int mh_sync(mh_ctx* ctx);

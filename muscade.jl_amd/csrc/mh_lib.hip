@@ -120,7 +120,7 @@ struct Ctx {
     double* d_dL1 = nullptr;     // 2·nX: L1[Λ][1], L1[X][1]
     double* d_dL2 = nullptr;     // 2(OX+1)·nnz(XX) + 2·nnz(XU): L2[Λ,X][1,d], L2[X,Λ][d,1], L2[Λ,U][1,1], L2[U,Λ][1,1]
     int scatter_variant = 0;     // nz scatter shape picked at prepare (launch_scatter)
-    float scatter_ms[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float scatter_ms[16] = {};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timed = false;
     bool graph_timed = false;    // the last timings are of a graph launch (total only)
@@ -898,7 +898,7 @@ constexpr int kChunk = 1024;
 // plain 32-bit index. SPL segments per lane (lane t owns k0 + t + j·kTile, so the
 // writes stay coalesced) and CH contributions per LDS chunk are the shape the
 // prepare-time tuner picks (scatter_variant).
-template <bool STEP, bool CODED, int SPL = 1, int CH = kChunk>
+template <bool STEP, bool CODED, int SPL = 1, int CH = kChunk, bool NTM = false>
 __global__ __launch_bounds__(kTile) void scatter_kernel(int64_t nseg, int64_t per_xcd, const int32_t* __restrict__ seg,
                                                         const uint32_t* __restrict__ src, const double* __restrict__ buf,
                                                         const double* __restrict__ dbuf, double* __restrict__ out) {
@@ -916,13 +916,16 @@ __global__ __launch_bounds__(kTile) void scatter_kernel(int64_t nseg, int64_t pe
     for (int j = 0; j < SPL; ++j) {
         const int64_t k = k0 + t + j * kTile;
         p0[j] = p1[j] = 0;
-        if (k < nseg) { p0[j] = seg[k]; p1[j] = seg[k + 1]; }
+        if (k < nseg) {
+            if (NTM) { p0[j] = __builtin_nontemporal_load(seg + k); p1[j] = __builtin_nontemporal_load(seg + k + 1); }
+            else { p0[j] = seg[k]; p1[j] = seg[k + 1]; }
+        }
         acc[j] = 0.;
     }
     for (int32_t c = pbeg; c < pend; c += CH) {
         const int32_t cend = (c + CH < pend ? c + CH : pend);
         for (int32_t q = c + t; q < cend; q += kTile) {
-            const uint32_t sq = src[q];
+            const uint32_t sq = NTM ? __builtin_nontemporal_load(src + q) : src[q];
             vals[q - c] = CODED ? contrib(buf, sq) : buf[sq];
             if (STEP) dvals[q - c] = dbuf[CODED ? (sq & kIdxMask) : sq];
         }
@@ -940,12 +943,17 @@ __global__ __launch_bounds__(kTile) void scatter_kernel(int64_t nseg, int64_t pe
 #pragma unroll
     for (int j = 0; j < SPL; ++j) {
         const int64_t k = k0 + t + j * kTile;
-        if (k < nseg) out[k] = acc[j];
+        if (k < nseg) {
+            if (NTM) __builtin_nontemporal_store(acc[j], out + k);
+            else out[k] = acc[j];
+        }
     }
 }
 // Launch one of the tuned shapes of the nz scatter: (segments per lane, LDS chunk).
-constexpr int kScatterVariants = 8;
-constexpr int kScatterSPL[kScatterVariants] = {1, 2, 2, 3, 4, 4, 6, 8};
+// Variants 8..15 are 0..7 with nontemporal index loads and nzval stores (streamed data kept
+// out of the caches, which then hold more of the gathered element outputs).
+constexpr int kScatterVariants = 16;
+constexpr int kScatterSPL[kScatterVariants] = {1, 2, 2, 3, 4, 4, 6, 8, 1, 2, 2, 3, 4, 4, 6, 8};
 template <bool CODED>
 void launch_scatter(int variant, hipStream_t s, int64_t nseg, const int32_t* seg, const uint32_t* src, const double* buf,
                     double* out) {
@@ -961,7 +969,15 @@ void launch_scatter(int variant, hipStream_t s, int64_t nseg, const int32_t* seg
     case 4: hipLaunchKernelGGL((scatter_kernel<false, CODED, 4, 1024>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
     case 5: hipLaunchKernelGGL((scatter_kernel<false, CODED, 4, 2048>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
     case 6: hipLaunchKernelGGL((scatter_kernel<false, CODED, 6, 2048>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
-    default: hipLaunchKernelGGL((scatter_kernel<false, CODED, 8, 2048>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    case 7: hipLaunchKernelGGL((scatter_kernel<false, CODED, 8, 2048>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    case 8: hipLaunchKernelGGL((scatter_kernel<false, CODED, 1, 1024, true>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    case 9: hipLaunchKernelGGL((scatter_kernel<false, CODED, 2, 1024, true>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    case 10: hipLaunchKernelGGL((scatter_kernel<false, CODED, 2, 2048, true>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    case 11: hipLaunchKernelGGL((scatter_kernel<false, CODED, 3, 2048, true>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    case 12: hipLaunchKernelGGL((scatter_kernel<false, CODED, 4, 1024, true>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    case 13: hipLaunchKernelGGL((scatter_kernel<false, CODED, 4, 2048, true>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    case 14: hipLaunchKernelGGL((scatter_kernel<false, CODED, 6, 2048, true>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    default: hipLaunchKernelGGL((scatter_kernel<false, CODED, 8, 2048, true>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
     }
 }
 
