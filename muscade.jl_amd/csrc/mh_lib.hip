@@ -215,15 +215,21 @@ __global__ __launch_bounds__(kBeamBlock) void beam_static_kernel(BeamArgs a) {
     const int64_t e = e0 + tid;
     const int64_t n = a.nele;
     if (e < n) {
+        // per-element streams are read once (nontemporal: they leave the caches to the node
+        // data, gathered again by neighbouring elements, and to the compact tangents written
+        // here for the scatter); X and coordinates are gathers and stay cached
         double X[12];
 #pragma unroll
-        for (int k = 0; k < 12; ++k) X[k] = a.X[a.idx[k * n + e]];
-        const int32_t n1 = a.nodes[e], n2 = a.nodes[n + e];
+        for (int k = 0; k < 12; ++k) X[k] = a.X[__builtin_nontemporal_load(a.idx + k * n + e)];
+        const int32_t n1 = __builtin_nontemporal_load(a.nodes + e), n2 = __builtin_nontemporal_load(a.nodes + n + e);
         mh::v3 c1 = mh::mk(a.xyz[3 * (int64_t)n1], a.xyz[3 * (int64_t)n1 + 1], a.xyz[3 * (int64_t)n1 + 2]);
         mh::v3 c2 = mh::mk(a.xyz[3 * (int64_t)n2], a.xyz[3 * (int64_t)n2 + 1], a.xyz[3 * (int64_t)n2 + 2]);
-        mh::v3 o2 = mh::mk(a.o2[e], a.o2[n + e], a.o2[2 * n + e]);
+        mh::v3 o2 = mh::mk(__builtin_nontemporal_load(a.o2 + e), __builtin_nontemporal_load(a.o2 + n + e),
+                           __builtin_nontemporal_load(a.o2 + 2 * n + e));
         mh::BeamSection S;
-        S.EA = a.sec[e]; S.EI2 = a.sec[n + e]; S.EI3 = a.sec[2 * n + e]; S.GJ = a.sec[3 * n + e]; S.w = a.sec[6 * n + e];
+        S.EA = __builtin_nontemporal_load(a.sec + e); S.EI2 = __builtin_nontemporal_load(a.sec + n + e);
+        S.EI3 = __builtin_nontemporal_load(a.sec + 2 * n + e); S.GJ = __builtin_nontemporal_load(a.sec + 3 * n + e);
+        S.w = __builtin_nontemporal_load(a.sec + 6 * n + e);
         mh::v3 fe = mh::mk(0., 0., S.w);
         if (UDOF) {
             fe.x = fe.x - a.U[a.uidx[e]];
