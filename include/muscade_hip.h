@@ -31,7 +31,8 @@ enum {
     MH_ERR_HIP = 2,     /* HIP runtime error                                   */
     MH_ERR_NAN = 3,     /* an element returned NaN (Assemble.jl:625-631)       */
     MH_ERR_ORIENT = 4,  /* orient2 nearly parallel to a beam (BeamElement.jl:135) */
-    MH_ERR_SIZE = 5     /* model exceeds the 32-bit index range of the device maps */
+    MH_ERR_SIZE = 5,    /* model exceeds the 32-bit index range of the device maps */
+    MH_ERR_SOLVE = 6    /* linear solve failed: zero or non-finite pivot (SweepX.jl:205-208) */
 };
 
 /* Element kinds with a device kernel. Each Muscade element type (one
@@ -161,6 +162,38 @@ int mh_prepare_direct(mh_ctx* ctx, int OX, int OU, int IA, int64_t* nnz4);
 int mh_get_direct_pattern(mh_ctx* ctx, int which, int64_t* colptr, int64_t* rowval);
 int mh_assemble_direct(mh_ctx* ctx, int mission, double dt, mh_err* err);
 int mh_get_direct(mh_ctx* ctx, double* L1, double* L2);
+
+/* ---- linear solve and Newton decrement on the device, chain topologies ------
+ * Replaces SweepX's host factorization and solve, `lu(out.Lλx)` / `lu!` and
+ * `Δx = Lλx\out.Lλ` (src/SweepX.jl:205-209), for models whose multi-node elements join
+ * their nodes into paths (beam and bar chains, many chains batched in one model), and
+ * Newmarkβdecrement!{OX} with the convergence norms (src/SweepX.jl:98-132,210-211), so
+ * that a Newton iteration leaves X on the device and returns two scalars. The host
+ * (muscade_amd/chainsolve.py) finds the layout once, after prepare:
+ *   B           3 or 6: positions per node block (a node's X dofs, padded)
+ *   nblock      number of node blocks, in path order (paths one after another)
+ *   pos         nX: for X dof d+1, its 1-based position (block·B + slot + 1) in the block
+ *               system, or −p for the λ dof of λ pair p (a Hold's Lagrange-multiplier dof,
+ *               coupled to its held dof only: src/BasicElements.jl:426-438)
+ *   pair_hpos   npair: 1-based position of each pair's held dof
+ *   dummy_pos   ndummy: 1-based padding positions (identity rows)
+ * Every position must be taken exactly once. mh_chain_solve solves Lλx·Δx = Lλ from the
+ * last assembly by block cyclic reduction (partial pivoting inside the B×B blocks);
+ * Δx stays on the device (mh_get_dx copies it). MH_ERR_SOLVE: zero or non-finite pivot,
+ * as SweepX's "matrix factorization failed". mh_chain_timings: ms of [pack, elimination,
+ * back substitution] of the last solve.
+ * mh_newton_prepare uploads the dof scales (dis.scaleX); mh_newton_update applies
+ * Newmarkβdecrement!{OX} (first = the :step iteration; c6 as for the assembly) to the
+ * device X, X′, X″ with the last Δx, in the host's operation order, and returns
+ * norms = [Σ Δx², Σ Lλ²]; mh_get_X copies a device X⁽ⁱᵈᵉʳ⁾ out. */
+int mh_chain_prepare(mh_ctx* ctx, int B, int64_t nblock, const int64_t* pos, int64_t npair, const int64_t* pair_hpos,
+                     int64_t ndummy, const int64_t* dummy_pos);
+int mh_chain_solve(mh_ctx* ctx);
+int mh_chain_timings(mh_ctx* ctx, double* ms);
+int mh_get_dx(mh_ctx* ctx, double* dx);
+int mh_newton_prepare(mh_ctx* ctx, const double* scaleX);
+int mh_newton_update(mh_ctx* ctx, int OX, int first, const double* c6, double* norms);
+int mh_get_X(mh_ctx* ctx, int ider, double* x);
 
 /* ---- measurement -----------------------------------------------------------
  * HIP-event times (ms) of the last assembly, measured on the context's stream:

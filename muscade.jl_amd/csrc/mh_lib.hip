@@ -32,6 +32,7 @@
 #include "beam_math.hpp"
 #include "beam_dyn.hpp"
 #include "bar_math.hpp"
+#include "chain_solve.hpp"
 
 namespace {
 
@@ -133,6 +134,23 @@ struct Ctx {
     int gOX[2] = {-1, -1};
     uint64_t gen = 0, ggen[2] = {~0ull, ~0ull};   // gen: bumped by setters that change launch arguments
     int64_t graph_captures = 0;
+    // on-device solve of Lλx·Δx = Lλ for chain topologies (mh_chain_prepare), and the
+    // Newton-Newmark decrement (mh_newton_update)
+    int cB = 0;                  // block size (3 or 6); 0 = not prepared
+    int64_t cnb = 0, cnp = 0, cnd = 0;   // blocks, λ pairs, padding positions
+    int32_t* d_cpos = nullptr;   // nX: block position of each dof, or −(p+1) for the λ of pair p
+    int32_t* d_chpos = nullptr;  // cnp: block position of each pair's held dof
+    int32_t* d_cdum = nullptr;   // cnd: padding positions
+    double* d_cbuf = nullptr;    // D | L | U (3·cnb·B²) | b (cnb·B) | pair couplings (4·cnp)
+    double* d_csaved = nullptr;  // cnp × (3B+1): original rows of the held dofs
+    double* d_dx = nullptr;      // nX: Δx of the last solve
+    unsigned* d_cflag = nullptr;
+    unsigned* h_cflag = nullptr; // pinned
+    double* d_scaleX = nullptr;  // nX: dof scales (mh_newton_prepare)
+    double* d_part = nullptr;    // 2 × workgroups of newmark_kernel
+    double* d_norm = nullptr;
+    double* h_norm = nullptr;    // pinned: Σ Δx², Σ Lλ²
+    hipEvent_t cev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 #define HIPCHK(ctx, call)                                                                       \
@@ -152,8 +170,16 @@ void free_graph(Ctx* c, int slot) {
     c->gOX[slot] = -1;
 }
 void free_graph(Ctx* c) { free_graph(c, 0); free_graph(c, 1); }
+void free_chain(Ctx* c) {
+    dfree(c->d_cpos); dfree(c->d_chpos); dfree(c->d_cdum); dfree(c->d_cbuf); dfree(c->d_csaved); dfree(c->d_dx);
+    dfree(c->d_cflag);
+    if (c->h_cflag) { (void)hipHostFree(c->h_cflag); c->h_cflag = nullptr; }
+    for (auto& e : c->cev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
+    c->cB = 0; c->cnb = c->cnp = c->cnd = 0;
+}
 void free_prepared(Ctx* c) {
     free_graph(c);
+    free_chain(c);
     dfree(c->d_Kbuf); dfree(c->d_Rbuf); dfree(c->d_dRbuf); dfree(c->d_nzval); dfree(c->d_rhs); dfree(c->d_Kc);
     dfree(c->d_colptr); dfree(c->d_rowval); dfree(c->d_kseg); dfree(c->d_ksrc); dfree(c->d_rseg); dfree(c->d_rsrc);
     dfree(c->d_useg); dfree(c->d_usrc); dfree(c->d_gbuf); dfree(c->d_gX); dfree(c->d_gubuf); dfree(c->d_gU);
@@ -173,6 +199,8 @@ void free_model(Ctx* c) {
     dfree(c->d_xyz);
     for (auto& x : c->d_X) dfree(x);
     dfree(c->d_U); dfree(c->d_A); dfree(c->d_lam);
+    dfree(c->d_scaleX); dfree(c->d_part); dfree(c->d_norm);
+    if (c->h_norm) { (void)hipHostFree(c->h_norm); c->h_norm = nullptr; }
 }
 
 // ------------------------------------------------------------------ kernels
@@ -1481,6 +1509,7 @@ const char* mh_strerror(int code) {
     case MH_ERR_NAN: return "element returned NaN in R or its derivatives";
     case MH_ERR_ORIENT: return "orient2 is nearly parallel to a beam element";
     case MH_ERR_SIZE: return "model exceeds the 32-bit index range of the device maps";
+    case MH_ERR_SOLVE: return "linear solve failed";
     }
     return "unknown error";
 }
@@ -2309,4 +2338,175 @@ int mh_get_direct(mh_ctx* c, double* L1, double* L2) {
     }
     return MH_OK;
 }
+}  // extern "C"
+
+// ------------------------------------------------------------------ chain solve
+namespace {
+
+int64_t cgrid(int64_t n) { return (n + chain::kCB - 1) / chain::kCB; }
+
+template <int B>
+int enqueue_chain_solve(Ctx* c) {
+    using namespace chain;
+    hipStream_t s = c->stream;
+    const int64_t nb = c->cnb, BB = (int64_t)B * B;
+    double* D = c->d_cbuf;
+    double* L = D + nb * BB;
+    double* U = L + nb * BB;
+    double* b = U + nb * BB;
+    double* pc = b + nb * B;
+    const int64_t ntot = 3 * nb * BB + nb * B + 4 * c->cnp;
+    HIPCHK(c, hipEventRecord(c->cev[0], s));
+    hipLaunchKernelGGL(fill_f64, dim3(cgrid(ntot)), dim3(kCB), 0, s, ntot, D, 0.0);
+    hipLaunchKernelGGL(clear_flag, dim3(1), dim3(1), 0, s, c->d_cflag);
+    if (c->cnd) hipLaunchKernelGGL(dummy_kernel<B>, dim3(cgrid(c->cnd)), dim3(kCB), 0, s, c->cnd, c->d_cdum, D);
+    hipLaunchKernelGGL(pack_kernel<B>, dim3(cgrid(c->nX)), dim3(kCB), 0, s, c->nX, c->d_colptr, c->d_rowval,
+                       c->d_nzval, c->d_rhs, c->d_cpos, c->d_chpos, D, L, U, b, pc, c->d_cflag);
+    if (c->cnp) hipLaunchKernelGGL(pair_kernel<B>, dim3(cgrid(c->cnp)), dim3(kCB), 0, s, c->cnp, c->d_chpos, pc, D, L,
+                                   U, b, c->d_csaved, c->d_cflag);
+    HIPCHK(c, hipEventRecord(c->cev[1], s));
+    int64_t top = 1;
+    for (int64_t st = 1; st < nb; st *= 2) {
+        const int64_t nel = (nb - st + 2 * st - 1) / (2 * st), nkeep = (nb + 2 * st - 1) / (2 * st);
+        hipLaunchKernelGGL(factor_kernel<B>, dim3(cgrid(nel)), dim3(kCB), 0, s, nb, st, D, L, U, b, c->d_cflag);
+        hipLaunchKernelGGL(update_kernel<B>, dim3(cgrid(nkeep * B)), dim3(kCB), 0, s, nb, st, D, L, U, b);
+        top = 2 * st;
+    }
+    hipLaunchKernelGGL(top_kernel<B>, dim3(1), dim3(64), 0, s, D, b, c->d_cflag);
+    HIPCHK(c, hipEventRecord(c->cev[2], s));
+    for (int64_t st = top / 2; st >= 1; st /= 2) {
+        const int64_t nel = (nb - st + 2 * st - 1) / (2 * st);
+        hipLaunchKernelGGL(back_kernel<B>, dim3(cgrid(nel * B)), dim3(kCB), 0, s, nb, st, L, U, b);
+    }
+    hipLaunchKernelGGL(unpack_kernel<B>, dim3(cgrid(c->nX)), dim3(kCB), 0, s, c->nX, nb, c->d_cpos, c->d_chpos, pc,
+                       c->d_csaved, b, c->d_dx);
+    HIPCHK(c, hipEventRecord(c->cev[3], s));
+    HIPCHK(c, hipGetLastError());
+    return MH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mh_chain_prepare(mh_ctx* c, int B, int64_t nblock, const int64_t* pos, int64_t npair, const int64_t* pair_hpos,
+                     int64_t ndummy, const int64_t* dummy_pos) {
+    if (!c || !c->prepared || (B != 3 && B != 6) || nblock < 1 || npair < 0 || ndummy < 0 || !pos ||
+        (npair && !pair_hpos) || (ndummy && !dummy_pos))
+        return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_chain(c);
+    const int64_t npos = nblock * B;
+    if (npos >= INT32_MAX || npair >= INT32_MAX) { c->err = "chain system too large for 32-bit positions"; return MH_ERR_SIZE; }
+    // every position is taken by exactly one dof or padding entry, every pair by one λ dof
+    std::vector<int32_t> p32(c->nX), h32(npair), d32(ndummy);
+    std::vector<char> used(npos, 0), lam(npair, 0);
+    for (int64_t d = 0; d < c->nX; ++d) {
+        const int64_t q = pos[d];
+        if (q > 0 && q <= npos && !used[q - 1]) { used[q - 1] = 1; p32[d] = (int32_t)(q - 1); }
+        else if (q < 0 && -q <= npair && !lam[-q - 1]) { lam[-q - 1] = 1; p32[d] = (int32_t)q; }
+        else { c->err = "chain layout: bad or repeated position of X dof " + std::to_string(d + 1); return MH_ERR_ARG; }
+    }
+    for (int64_t q = 0; q < ndummy; ++q) {
+        const int64_t v = dummy_pos[q];
+        if (v < 1 || v > npos || used[v - 1]) { c->err = "chain layout: bad padding position"; return MH_ERR_ARG; }
+        used[v - 1] = 1; d32[q] = (int32_t)(v - 1);
+    }
+    for (int64_t q = 0; q < npos; ++q)
+        if (!used[q]) { c->err = "chain layout: position " + std::to_string(q + 1) + " is not taken"; return MH_ERR_ARG; }
+    std::vector<char> held(npos, 0);
+    for (int64_t p = 0; p < npair; ++p) {
+        const int64_t v = pair_hpos[p];
+        if (!lam[p] || v < 1 || v > npos || held[v - 1]) { c->err = "chain layout: bad λ pair " + std::to_string(p + 1); return MH_ERR_ARG; }
+        held[v - 1] = 1; h32[p] = (int32_t)(v - 1);
+    }
+    const int64_t ntot = 3 * nblock * B * B + npos + 4 * npair;
+    HIPCHK(c, hipMalloc(&c->d_cpos, sizeof(int32_t) * std::max<int64_t>(c->nX, 1)));
+    HIPCHK(c, hipMalloc(&c->d_chpos, sizeof(int32_t) * std::max<int64_t>(npair, 1)));
+    HIPCHK(c, hipMalloc(&c->d_cdum, sizeof(int32_t) * std::max<int64_t>(ndummy, 1)));
+    HIPCHK(c, hipMalloc(&c->d_cbuf, sizeof(double) * ntot));
+    HIPCHK(c, hipMalloc(&c->d_csaved, sizeof(double) * std::max<int64_t>(npair * (3 * B + 1), 1)));
+    HIPCHK(c, hipMalloc(&c->d_dx, sizeof(double) * std::max<int64_t>(c->nX, 1)));
+    HIPCHK(c, hipMalloc(&c->d_cflag, sizeof(unsigned)));
+    HIPCHK(c, hipHostMalloc((void**)&c->h_cflag, sizeof(unsigned), hipHostMallocDefault));
+    for (auto& e : c->cev) HIPCHK(c, hipEventCreate(&e));
+    if (c->nX) HIPCHK(c, hipMemcpy(c->d_cpos, p32.data(), sizeof(int32_t) * c->nX, hipMemcpyHostToDevice));
+    if (npair) HIPCHK(c, hipMemcpy(c->d_chpos, h32.data(), sizeof(int32_t) * npair, hipMemcpyHostToDevice));
+    if (ndummy) HIPCHK(c, hipMemcpy(c->d_cdum, d32.data(), sizeof(int32_t) * ndummy, hipMemcpyHostToDevice));
+    c->cB = B; c->cnb = nblock; c->cnp = npair; c->cnd = ndummy;
+    return MH_OK;
+}
+
+int mh_chain_solve(mh_ctx* c) {
+    if (!c || !c->prepared || !c->cB) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (int r = (c->cB == 6 ? enqueue_chain_solve<6>(c) : enqueue_chain_solve<3>(c))) return r;
+    HIPCHK(c, hipMemcpyAsync(c->h_cflag, c->d_cflag, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const unsigned f = *c->h_cflag;
+    if (f & chain::kBadPattern) { c->err = "matrix entry outside the chain's block-tridiagonal structure"; return MH_ERR_ARG; }
+    if (f & chain::kBadPair) { c->err = "λ row with a nonzero diagonal or a zero coupling"; return MH_ERR_SOLVE; }
+    if (f & chain::kSingular) { c->err = "zero or non-finite pivot in a diagonal block"; return MH_ERR_SOLVE; }
+    return MH_OK;
+}
+
+int mh_chain_timings(mh_ctx* c, double* ms) {
+    if (!c || !c->cB || !ms) return MH_ERR_ARG;
+    float a = 0.f, b = 0.f, d = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&a, c->cev[0], c->cev[1]));
+    HIPCHK(c, hipEventElapsedTime(&b, c->cev[1], c->cev[2]));
+    HIPCHK(c, hipEventElapsedTime(&d, c->cev[2], c->cev[3]));
+    ms[0] = a; ms[1] = b; ms[2] = d;
+    return MH_OK;
+}
+
+int mh_get_dx(mh_ctx* c, double* dx) {
+    if (!c || !c->cB || !dx) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->nX) HIPCHK(c, hipMemcpy(dx, c->d_dx, sizeof(double) * c->nX, hipMemcpyDeviceToHost));
+    return MH_OK;
+}
+
+int mh_newton_prepare(mh_ctx* c, const double* scaleX) {
+    if (!c || (c->nX && !scaleX)) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c->d_scaleX); dfree(c->d_part); dfree(c->d_norm);
+    if (c->h_norm) { (void)hipHostFree(c->h_norm); c->h_norm = nullptr; }
+    const int64_t n = std::max<int64_t>(c->nX, 1);
+    HIPCHK(c, hipMalloc(&c->d_scaleX, sizeof(double) * n));
+    HIPCHK(c, hipMalloc(&c->d_part, sizeof(double) * 2 * cgrid(n)));
+    HIPCHK(c, hipMalloc(&c->d_norm, sizeof(double) * 2));
+    HIPCHK(c, hipHostMalloc((void**)&c->h_norm, sizeof(double) * 2, hipHostMallocDefault));
+    if (c->nX) HIPCHK(c, hipMemcpy(c->d_scaleX, scaleX, sizeof(double) * c->nX, hipMemcpyHostToDevice));
+    return MH_OK;
+}
+
+int mh_newton_update(mh_ctx* c, int OX, int first, const double* c6, double* norms) {
+    if (!c || !c->cB || !c->d_scaleX || OX < 0 || OX > 2 || (OX > 0 && !c6) || !norms) return MH_ERR_ARG;
+    if (OX > 0 && !c->d_X[OX]) { c->err = "X derivatives of order " + std::to_string(OX) + " are not on the device"; return MH_ERR_ARG; }
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const double z[6] = {0, 0, 0, 0, 0, 0};
+    const double* k = OX > 0 ? c6 : z;
+    const int64_t ng = cgrid(std::max<int64_t>(c->nX, 1));
+    hipLaunchKernelGGL(chain::newmark_kernel, dim3(ng), dim3(chain::kCB), 0, s, c->nX, OX, first, k[0], k[1], k[2], k[3],
+                       k[4], k[5], c->d_dx, c->d_rhs, c->d_scaleX, c->d_X[0], c->d_X[1], c->d_X[2], c->d_part);
+    hipLaunchKernelGGL(chain::norm_kernel, dim3(1), dim3(chain::kCB), 0, s, ng, c->d_part, c->d_norm);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->h_norm, c->d_norm, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    norms[0] = c->h_norm[0];
+    norms[1] = c->h_norm[1];
+    return MH_OK;
+}
+
+int mh_get_X(mh_ctx* c, int ider, double* x) {
+    if (!c || ider < 0 || ider > 2 || !x || (c->nX && !c->d_X[ider])) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->nX) HIPCHK(c, hipMemcpy(x, c->d_X[ider], sizeof(double) * c->nX, hipMemcpyDeviceToHost));
+    return MH_OK;
+}
+
 }  // extern "C"

@@ -11,7 +11,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libmuscade_hip.so")
 
-MH_OK, MH_ERR_ARG, MH_ERR_HIP, MH_ERR_NAN, MH_ERR_ORIENT, MH_ERR_SIZE = 0, 1, 2, 3, 4, 5
+MH_OK, MH_ERR_ARG, MH_ERR_HIP, MH_ERR_NAN, MH_ERR_ORIENT, MH_ERR_SIZE, MH_ERR_SOLVE = 0, 1, 2, 3, 4, 5, 6
 MH_KIND_BEAM, MH_KIND_BEAM_U, MH_KIND_HOLD, MH_KIND_DOFLOAD, MH_KIND_ZERO, MH_KIND_BAR, MH_KIND_BAR_U = 1, 2, 3, 4, 5, 6, 7
 
 # every function of include/muscade_hip.h, with its ctypes signature
@@ -57,6 +57,13 @@ SIGNATURES = {
     "mh_get_nzval": (_I, [_P, _D]),
     "mh_device_outputs": (_I, [_P, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "mh_get_element_blocks": (_I, [_P, _I, _D, _D]),
+    "mh_chain_prepare": (_I, [_P, _I, _L, _I64, _L, _I64, _L, _I64]),
+    "mh_chain_solve": (_I, [_P]),
+    "mh_chain_timings": (_I, [_P, _D]),
+    "mh_get_dx": (_I, [_P, _D]),
+    "mh_newton_prepare": (_I, [_P, _D]),
+    "mh_newton_update": (_I, [_P, _I, _I, _D, _D]),
+    "mh_get_X": (_I, [_P, _I, _D]),
     "mh_last_timings": (_I, [_P, _D]),
     "mh_calib_copy": (_I, [_P, _L, _D]),
     "mh_sync": (_I, [_P]),

@@ -123,10 +123,11 @@ class AssemblySweepX:
         self.ctx.check(lib.mh_get_asm(self.ctx.h, self.groups[ieletyp], i64ptr(a1), i64ptr(a2)), "asm")
         return a1, a2
 
-    def set_state(self, state, t):
+    def set_state(self, state, t, X=True):
+        """Upload the state (X=False: all but X, X′, X″, which stay as they are on the device)."""
         h = self.ctx.h
         self.ctx.check(lib.mh_set_time(h, float(t)), "set time")
-        for ider in range(min(self.OX + 1, len(state.X))):
+        for ider in range(min(self.OX + 1, len(state.X)) if X else 0):
             self.ctx.check(lib.mh_set_X(h, ider, dptr(as_f64(state.X[ider]))), "set X")
         if self.model.ndof["U"]:
             self.ctx.check(lib.mh_set_U(h, dptr(as_f64(state.U[0]))), "set U")
@@ -138,9 +139,12 @@ class AssemblySweepX:
                                        for n, p in zip(E.nodes, E.params)])
                 self.ctx.check(lib.mh_set_point_values(h, self.groups[it], dptr(as_f64(vals))), "DofLoad values")
 
-    def assemble(self, state, mission, dbg=None, fetch=True):
-        """assemble!{:step|:iter}(out,asm,dis,model,state,Δt,dbg). mission: 'step' or 'iter'."""
-        self.set_state(state, state.time)
+    def assemble(self, state, mission, dbg=None, fetch=True, upload=True):
+        """assemble!{:step|:iter}(out,asm,dis,model,state,Δt,dbg). mission: 'step' or 'iter'.
+        upload=False assembles from the state already on the device (a device-resident
+        Newton loop, muscade_amd.chainsolve)."""
+        if upload:
+            self.set_state(state, state.time)
         err = _lib.mh_err()
         m = 0 if mission == "step" else 1
         fn = lib.mh_assemble_sweepx_graph if self.graph else lib.mh_assemble_sweepx

@@ -2,8 +2,11 @@
 and the solve wrapper of src/Solve.jl:16-35.
 
 The Newton/Newmark loop stays on the host like in the reference; assembly of
-Lλ and Lλx runs on the GPU; the sparse factorization (UMFPACK `lu` in the
-reference, SuperLU here) stays on the host.
+Lλ and Lλx runs on the GPU. The sparse factorization (UMFPACK `lu` in the reference)
+runs on the host with SuperLU (linsolve="host"), or, for chain topologies, on the
+device (linsolve="device", muscade_amd.chainsolve): then the Newmark decrement runs on
+the device too, X stays there through the iterations of a step and each iteration
+returns only |Δx|² and |Lλ|².
 """
 import math
 import sys
@@ -13,6 +16,7 @@ import numpy as np
 import scipy.sparse.linalg as spla
 
 from .assemble import AssemblySweepX, Newmarkβcoefficients
+from .chainsolve import ChainSolver
 from .model import MuscadeError
 
 
@@ -29,14 +33,22 @@ class SweepX(metaclass=_SweepXMeta):
 
     @classmethod
     def run(cls, pstate, verbose, dbg, *, time, initialstate, β=0.25, γ=0.5, maxiter=50, maxΔx=1e-5,
-            maxLλ=math.inf, saveiter=False, device=0, Assembly=AssemblySweepX, graph=True):
+            maxLλ=math.inf, saveiter=False, device=0, Assembly=AssemblySweepX, graph=True, linsolve="host"):
         """`Assembly` is the assembler type prepared for the model, as the type argument of
         prepare(AssemblySweepX{OX},model,dis) (SweepX.jl:24,188). graph: the device assembler
-        replays captured HIP graphs for the Newton iterations of each time step."""
+        replays captured HIP graphs for the Newton iterations of each time step.
+        linsolve: "host" (SuperLU) or "device" (chain topologies only; raises
+        chainsolve.NotAChain otherwise)."""
         OX = cls.OX
         model, dis = initialstate.model, initialstate.dis
+        if linsolve not in ("host", "device"):
+            raise MuscadeError(f"linsolve must be 'host' or 'device', not {linsolve!r}")
+        dev = linsolve == "device"
+        if dev and Assembly is not AssemblySweepX:
+            raise MuscadeError("linsolve='device' solves the device assembler's system (Assembly=AssemblySweepX)")
         kw = {"graph": graph} if Assembly is AssemblySweepX else {}
         out = Assembly(OX, model, dis, device=device, **kw)
+        solver = ChainSolver(out) if dev else None
         nX = out.nX
         scaleX = dis.scaleX
         cΔx2, cLλ2 = maxΔx ** 2, maxLλ ** 2
@@ -53,10 +65,34 @@ class SweepX(metaclass=_SweepXMeta):
                 raise MuscadeError("Time step length not strictly positive at step=%3d" % step)
             out.c = Newmarkβcoefficients(OX, Δt, β, γ)
             c = out.c
+            if dev:   # X, X′, X″ are on the device from the first step on
+                out.set_state(state, state.time, X=(step == 1))
             for iiter in range(1, maxiter + 1):
                 citer += 1
                 first = iiter == 1
                 d = dict(dbg, solver="SweepX", step=step, iiter=iiter)
+                if dev:
+                    out.assemble(state, "step" if first else "iter", dbg=d, fetch=False, upload=False)
+                    solver.solve(d)
+                    Δx2, Lλ2 = solver.newton_update(OX, first, c.array())
+                    if not math.isfinite(Δx2):
+                        raise MuscadeError("matrix factorization failed at step=%i, iiter=%i" % (step, iiter))
+                    conv = Δx2 <= cΔx2 and Lλ2 <= cLλ2
+                    if saveiter or conv:
+                        solver.fetch_X(state, OX)
+                    if saveiter:
+                        states[iiter - 1] = state.copy()
+                    if conv:
+                        if verbose:
+                            print("    step %3d converged in %3d iterations. |Δx|=%7.1e |Lλ|=%7.1e"
+                                  % (step, iiter, math.sqrt(Δx2), math.sqrt(Lλ2)))
+                        if not saveiter:
+                            states[step - 1] = state.copy()
+                        break
+                    if iiter == maxiter:
+                        raise MuscadeError("no convergence of step %3d after %3d iterations |Δx|=%g / %g, |Lλ|=%g / %g"
+                                           % (step, iiter, math.sqrt(Δx2), maxΔx, math.sqrt(Lλ2), maxLλ))
+                    continue
                 Lλ, _ = out.assemble(state, "step" if first else "iter", dbg=d)
                 try:
                     lu = spla.splu(out.Lλx())

@@ -1,0 +1,161 @@
+"""Device linear solve of SweepX's system for chain topologies (mh_chain_solve) and the device
+Newton-Newmark decrement (mh_newton_update), against the host path of src/SweepX.jl:98-132,
+205-211 (SciPy's sparse LU standing for `lu`/`ldiv!`):
+
+- Δx of the device solve is a backward-stable solution of Lλx·Δx = Lλ (normwise backward
+  error ≤ 1e-14) and matches sparse LU to 1e-10 where Lλx is well conditioned (the static
+  cantilevers and the bar chain; the dynamic catenary chains have condition numbers ~1e19,
+  see tests/test_chain_layout.py);
+- the device decrement equals the host formulas bitwise for the same Δx, and its norms
+  agree with numpy's to 1e-14;
+- solve(SweepX{0|2}; linsolve="device") reaches the states of the host-LU solve;
+- non-chain models are refused.
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from tests.helpers import relerr
+from tests.test_chain_layout import backward_error, bar_chain, cantilever_model, random_state
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mc():
+    import muscade_amd
+    return muscade_amd
+
+
+def two_chains(mc):
+    from muscade_amd.workloads import base_section
+    m = mc.Model("two")
+    x = np.linspace(0., 2., 5)
+    mc.addnode(m, np.stack([x, 0 * x, 0 * x], axis=1))
+    mc.addnode(m, np.stack([x[::-1], 1 + 0 * x, 0 * x], axis=1))
+    mc.addnode(m, np.array([[2.5, 0., -0.2]]))
+    mc.addelement(m, mc.EulerBeam3D, np.array([[6, 7], [3, 4], [1, 2], [8, 9], [2, 3], [10, 9], [4, 5], [7, 8]]),
+                  mat=base_section(), orient2=np.array([0., 0., 1.]))
+    for f in ("t1", "t2", "t3", "r1", "r2", "r3"):
+        mc.addelement(m, mc.Hold, [1], field=f)
+        mc.addelement(m, mc.Hold, [10], field=f)
+    mc.addelement(m, mc.Bar3D, [5, 11], mat=mc.AxisymmetricBarCrossSection(EA=1e6, mu=5., w=10.))
+    return m
+
+
+def device_solve(mc, model, state, OX, mission="iter", Δt=0.05):
+    from muscade_amd.chainsolve import ChainSolver
+    out = mc.AssemblySweepX(OX, model, state.dis)
+    if OX:
+        out.c = mc.Newmarkβcoefficients(OX, Δt)
+    r, nz = out.assemble(state, mission)
+    K, r = out.Lλx(), r.copy()
+    cs = ChainSolver(out)
+    cs.solve()
+    return out, cs, K, r, cs.dx()
+
+
+CASES = ["cantilever_static", "cantilever_dynamic", "bar_chain", "two_chains", "catenary"]
+
+
+def make_case(mc, name):
+    from muscade_amd.workloads import catenary_chains
+    if name == "cantilever_static":
+        m = cantilever_model(nel=9)
+        return m, random_state(m, 0), 0, "iter", True
+    if name == "cantilever_dynamic":
+        m = cantilever_model(nel=9)
+        return m, random_state(m, 2, seed=1), 2, "step", True
+    if name == "bar_chain":
+        m = bar_chain(n=11)
+        return m, random_state(m, 2, scale=1e-2), 2, "step", True
+    if name == "two_chains":
+        m = two_chains(mc)
+        return m, random_state(m, 2, seed=4), 2, "step", False
+    m, st = catenary_chains(nchain=5, nmin=20, nmax=300, seed=21)
+    st.time = 1.0
+    return m, st, 2, "step", False
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_device_solve(mc, name):
+    print("case", name, flush=True)
+    model, st, OX, mission, well = make_case(mc, name)
+    out, cs, K, r, dx = device_solve(mc, model, st, OX, mission)
+    assert backward_error(K, dx, r) <= 1e-14, name
+    if well:
+        assert relerr(dx, spla.spsolve(K.tocsc(), r)) <= 1e-10, name
+    t = cs.timings()
+    assert np.all(t >= 0)
+
+
+@pytest.mark.parametrize("OX", [0, 1, 2])
+def test_newmark_update_bitwise(mc, OX):
+    print("newmark OX", OX, flush=True)
+    model = cantilever_model(nel=9)
+    st = random_state(model, 2, seed=7)
+    out, cs, K, r, dx = device_solve(mc, model, st, OX, "step" if OX else "iter")
+    c = mc.Newmarkβcoefficients(OX, 0.05)
+    out.c = c
+    for first in (True, False):
+        X = [x.copy() for x in st.X]
+        s = st.dis.scaleX
+        n2 = cs.newton_update(OX, first, c.array())
+        # the host loop's formulas (SweepX.run, linsolve="host")
+        if OX == 2:
+            if first:
+                x1, x2 = X[1] / s, X[2] / s
+                a, b = c.a2 * x1 + c.a3 * x2, c.b2 * x1 + c.b3 * x2
+                d1, d2 = c.a1 * dx + a, c.b1 * dx + b
+            else:
+                d1, d2 = c.a1 * dx, c.b1 * dx
+            X[0] -= dx * s; X[1] -= d1 * s; X[2] -= d2 * s
+        elif OX == 1:
+            d1 = c.a1 * dx + c.a2 * (X[1] / s) if first else c.a1 * dx
+            X[0] -= dx * s; X[1] -= d1 * s
+        else:
+            X[0] -= dx * s
+        got = mc.State(st.time, st.Λ, [x.copy() for x in st.X], st.U, st.A, st.SP, model, st.dis)
+        cs.fetch_X(got, OX)
+        for k in range(OX + 1):
+            assert np.array_equal(got.X[k].view(np.uint64), X[k].view(np.uint64)), (OX, first, k)
+        assert abs(n2[0] - np.sum(dx ** 2)) <= 1e-14 * np.sum(dx ** 2)
+        assert abs(n2[1] - np.sum(r ** 2)) <= 1e-14 * np.sum(r ** 2)
+        st.X[:OX + 1] = [x.copy() for x in X[:OX + 1]]
+
+
+def test_sweepx_static_device_linsolve(mc):
+    from muscade_amd.workloads import cantilever
+    model = cantilever(nel=20, nstep=4)
+    st0 = mc.initialize(model, time=0.)
+    time = np.arange(1., 5.)
+    host = mc.solve(mc.SweepX[0], initialstate=st0, time=time, maxΔx=1e-11, verbose=False, catcherror=False)
+    devs = mc.solve(mc.SweepX[0], initialstate=st0, time=time, maxΔx=1e-11, verbose=False, catcherror=False,
+                    linsolve="device")
+    for sh, sd in zip(host, devs):
+        assert relerr(sd.X[0], sh.X[0]) <= 1e-9
+
+
+def test_sweepx_dynamic_device_linsolve(mc):
+    from muscade_amd.workloads import catenary_chains
+    model, st0 = catenary_chains(nchain=4, nmin=30, nmax=120, seed=5, disp=0., rot=0., vel=0., acc=0.)
+    st0.time = 0.
+    time = np.arange(1, 6) * 0.05
+    # the reference's tolerance scale: |Δx| stagnates near 2e-7 (the round-off of the
+    # Hold-bordered system), 4 iterations per step
+    kw = dict(initialstate=st0, time=time, maxΔx=1e-6, verbose=False, catcherror=False)
+    host = mc.solve(mc.SweepX[2], **kw)
+    devs = mc.solve(mc.SweepX[2], linsolve="device", **kw)
+    disp = model.getdofID("X", "t1") - 1
+    for sh, sd in zip(host, devs):
+        for k in range(3):
+            assert relerr(sd.X[k][disp], sh.X[k][disp]) <= 1e-5, k
+    assert np.abs(devs[-1].X[0][disp]).max() > 1e-2   # the chains moved
+
+
+def test_refuses_lattice(mc):
+    from muscade_amd.chainsolve import NotAChain
+    from tests.helpers import build_lattice
+    model, st = build_lattice(3, 3, 2, seed=1)
+    with pytest.raises(NotAChain):
+        mc.solve(mc.SweepX[0], initialstate=st, time=[1.], verbose=False, catcherror=False, linsolve="device")
