@@ -140,9 +140,96 @@ def dynamic_bench(args):
         res[mission] = {"elements_per_s": nbeam * nst / el, "ms_per_step": el / nst * 1e3,
                         "kernels_ms": dict(zip(("eval", "scatter_nz", "scatter_rhs"), (acc / nst).tolist()))}
         log(f"dynamic {mission}: {el / nst * 1e3:.3f} ms/step, kernels {acc / nst}")
+    if not args.no_chain_solve:
+        res["newton_device"] = device_newton_iteration(args, out, state, c6)
     return {"workload": f"catenary chains x{args.chains}, {nbeam} EulerBeam3D with inertia/added mass/drag/roll "
                         f"inertia, AssemblySweepX{{2}} (Newmark-beta, dt=0.05)",
             "nbeam": nbeam, "nX": out.nX, "nnz": out.nnz, "prepare_s": t_prep, **res}
+
+
+def device_newton_iteration(args, out, state, c6):
+    """One Newton iteration of SweepX{2} (the :step one) on the device for the catenary chains:
+    assembly, block-cyclic-reduction solve of Lλx·Δx = Lλ and the Newmark decrement, X
+    resident, two scalars back (SweepX.jl:202-211). Each timed iteration starts from the
+    same uploaded state (upload untimed)."""
+    from muscade_amd.chainsolve import ChainSolver
+    t0 = time.perf_counter()
+    cs = ChainSolver(out)
+    t_lay = time.perf_counter() - t0
+    L = cs.layout
+    log(f"chain layout: {L.nblock} blocks of {L.B}, {L.pair_hpos.size} λ pairs ({t_lay:.1f}s)")
+
+    def iteration():
+        out.assemble(state, "step", fetch=False, upload=False)
+        cs.solve()
+        return cs.newton_update(2, True, c6)
+
+    out.set_state(state, 0.0)
+    iteration()
+    n = max(2, args.steps // 4)
+    wall, asm, sol = [], np.zeros(3), np.zeros(3)
+    for _ in range(n):
+        out.set_state(state, 0.0)
+        out.ctx.check(_lib_sync(out), "sync")
+        t0 = time.perf_counter()
+        n2 = iteration()
+        wall.append(time.perf_counter() - t0)
+        asm += out.timings()
+        sol += cs.timings()
+    w = float(np.mean(wall)) * 1e3
+    res = {"layout_s": t_lay, "nblock": L.nblock, "B": L.B, "npair": int(L.pair_hpos.size),
+           "iteration_ms": w, "assembly_kernels_ms": (asm / n).tolist(),
+           "solve_ms": {"pack": sol[0] / n, "eliminate": sol[1] / n, "back_substitute": sol[2] / n},
+           "norms": [float(np.sqrt(n2[0])), float(np.sqrt(n2[1]))]}
+    log(f"device Newton iteration: {w:.2f} ms (solve: pack {sol[0] / n:.2f}, eliminate {sol[1] / n:.2f}, back "
+        f"{sol[2] / n:.2f} ms), |Δx|={res['norms'][0]:.3e}")
+    return res
+
+
+def _lib_sync(out):
+    from muscade_amd import _lib
+    return _lib.lib.mh_sync(out.ctx.h)
+
+
+def chain_solve_bench(args):
+    """Device block-cyclic-reduction solve against the host sparse LU (SciPy SuperLU, as the
+    reference's `lu` + `ldiv!`, SweepX.jl:205-209) on the same dynamic Lλx of a reduced
+    catenary model (16 chains of 1000-4000 beams), both from the assembled matrix."""
+    import scipy.sparse.linalg as spla
+    import muscade_amd as mc
+    from muscade_amd.chainsolve import ChainSolver
+    from muscade_amd.workloads import catenary_chains
+    model, state = catenary_chains(16, 1000, 4000, seed=1005)
+    state.time = 1.0
+    out = mc.AssemblySweepX(2, model, state.dis)
+    out.c = mc.Newmarkβcoefficients(2, 0.05, 0.25, 0.5)
+    out.assemble(state, "step")
+    K, r = out.Lλx(), out.Lλ.copy()
+    cs = ChainSolver(out)
+    cs.solve()
+    n = 10
+    t0 = time.perf_counter()
+    for _ in range(n):
+        cs.solve()
+    t_dev = (time.perf_counter() - t0) / n
+    dx = cs.dx()
+    t0 = time.perf_counter()
+    lu = spla.splu(K.tocsc())
+    t1 = time.perf_counter()
+    ref = lu.solve(r)
+    t2 = time.perf_counter()
+    Ka = abs(K)
+    denom = Ka.sum(axis=1).max() * np.abs(dx).max() + np.abs(r).max()
+    res = {"workload": f"catenary chains x16, {model.eletypes[0].nele} EulerBeam3D, nX={out.nX}, dynamic Lλx (:step)",
+           "device_solve_ms": t_dev * 1e3, "host_lu_ms": (t1 - t0) * 1e3, "host_solve_ms": (t2 - t1) * 1e3,
+           "speedup_vs_host_lu_and_solve": (t2 - t0) / t_dev,
+           "device_backward_error": float(np.abs(K @ dx - r).max() / denom),
+           "host_backward_error": float(np.abs(K @ ref - r).max() /
+                                        (Ka.sum(axis=1).max() * np.abs(ref).max() + np.abs(r).max()))}
+    log(f"chain solve ({out.nX} dofs): device {t_dev * 1e3:.2f} ms, host LU {res['host_lu_ms']:.0f} ms + solve "
+        f"{res['host_solve_ms']:.1f} ms; backward errors {res['device_backward_error']:.1e} / "
+        f"{res['host_backward_error']:.1e}")
+    return res
 
 
 def adjoint_bench(args):
@@ -336,6 +423,9 @@ def main():
     ap.add_argument("--no-directxua", action="store_true", help="skip the DirectXUA load-identification solve")
     ap.add_argument("--only-directxua", action="store_true", help="measure only the DirectXUA solve")
     ap.add_argument("--only-graph", action="store_true", help="measure only eager vs HIP-graph assembly")
+    ap.add_argument("--no-chain-solve", action="store_true",
+                    help="skip the device linear solve measurements (chain topologies)")
+    ap.add_argument("--only-chain-solve", action="store_true", help="measure only the device vs host chain solve")
     ap.add_argument("--only-adjoint", action="store_true", help="measure only the batched adjoint gradients")
     ap.add_argument("--models", type=int, default=512)
     ap.add_argument("--model-min", type=int, default=500)
@@ -360,6 +450,9 @@ def main():
         return
     if args.only_graph:
         print(json.dumps({"graph": graph_bench(args)}))
+        return
+    if args.only_chain_solve:
+        print(json.dumps({"chain_solve": chain_solve_bench(args)}))
         return
     if args.only_directxua:
         print(json.dumps({"directxua": directxua_bench(args)}))
@@ -493,6 +586,8 @@ def main():
         line["graph"] = graph_bench(args)
     if not args.no_newton:
         line["newton_step"] = newton_bench(args)
+    if not args.no_chain_solve:
+        line["chain_solve"] = chain_solve_bench(args)
     if not args.no_directxua:
         line["directxua"] = directxua_bench(args)
     print(json.dumps(line))

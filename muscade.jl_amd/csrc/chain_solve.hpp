@@ -5,11 +5,14 @@
 // Block layout
 //   The X dofs of one node form one block of B (3 or 6) positions; blocks follow the
 //   nodes along each path, paths one after another. Lλx is then block tridiagonal:
-//     D[i]  rows of block i, columns of block i        (B×B, row-major)
-//     L[i]  rows of block i, columns of block i-1
-//     U[i]  rows of block i, columns of block i+1
+//     D[i]  rows of block i, columns of block i        (B×B, column-major: entry (r, c)
+//     L[i]  rows of block i, columns of block i-1       at i·B² + c·B + r, so that the rows
+//     U[i]  rows of block i, columns of block i+1       of one CSC column land contiguously)
 //   Two paths meet with zero L/U blocks, so many chains form one system. A node with
 //   fewer than B dofs is padded with identity rows (zero solution).
+//   At prepare, slot_kernel gives each nz of Lλx its destination (slot) in D|L|U, and the
+//   slots no nz covers are listed; a solve zeroes those, then scatters nzval by slot with
+//   coalesced reads.
 //   The λ dof of a Hold (src/BasicElements.jl:426-438) is coupled to its held dof h only:
 //   row λ reads cL·x_h = r_λ (its diagonal is zero), row h holds cU·λ. The λ row fixes x_h;
 //   the pack replaces row h of the block system by that equation, saving the original row
@@ -111,57 +114,100 @@ __global__ __launch_bounds__(kCB) void fill_f64(int64_t n, double* __restrict__ 
     if (i < n) p[i] = v;
 }
 
-// Identity rows of padding positions.
+__global__ __launch_bounds__(kCB) void fill_i32(int64_t n, int32_t* __restrict__ p, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * kCB + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Identity rows of padding positions (their slots are uncovered, so zeroed before).
 template <int B>
-__global__ __launch_bounds__(kCB) void dummy_kernel(int64_t n, const int32_t* __restrict__ dpos, double* __restrict__ D) {
+__global__ __launch_bounds__(kCB) void dummy_kernel(int64_t n, const int32_t* __restrict__ dpos, double* __restrict__ D,
+                                                    double* __restrict__ b) {
     const int64_t q = (int64_t)blockIdx.x * kCB + threadIdx.x;
     if (q >= n) return;
     const int64_t p = dpos[q], blk = p / B, i = p - blk * B;
-    D[(blk * B + i) * B + i] = 1.0;
+    D[blk * B * B + i * B + i] = 1.0;
+    b[p] = 0.0;
 }
 
-// One thread per column c of the CSC matrix: its entries into the blocks, the λ-pair
-// couplings into pc[4p..4p+3] = (cL, cU, diagonal, r_λ), and rhs[c] into b.
-// pos[c] ≥ 0: block position of dof c; pos[c] = −(p+1): the λ dof of pair p.
+// Prepare: one thread per column c of the CSC pattern. slot[k] of each nz k: ≥ 0, its index
+// in D|L|U; −(4p + w + 1), coupling w (0: cL, row λ col h; 1: cU, row h col λ; 2: the λ
+// diagonal) of λ pair p. unc[q] is cleared for every slot q that an nz covers.
+// pos[d] ≥ 0: block position of dof d; pos[d] = −(p+1): the λ dof of pair p.
 template <int B>
-__global__ __launch_bounds__(kCB) void pack_kernel(int64_t nX, const int32_t* __restrict__ colptr,
-                                                   const int32_t* __restrict__ rowval, const double* __restrict__ nzval,
-                                                   const double* __restrict__ rhs, const int32_t* __restrict__ pos,
-                                                   const int32_t* __restrict__ hpos, double* __restrict__ D,
-                                                   double* __restrict__ L, double* __restrict__ U,
-                                                   double* __restrict__ b, double* __restrict__ pc,
-                                                   unsigned* __restrict__ flag) {
+__global__ __launch_bounds__(kCB) void slot_kernel(int64_t nX, int64_t nb, const int32_t* __restrict__ colptr,
+                                                   const int32_t* __restrict__ rowval, const int32_t* __restrict__ pos,
+                                                   const int32_t* __restrict__ hpos, int32_t* __restrict__ slot,
+                                                   int32_t* __restrict__ unc, unsigned* __restrict__ flag) {
     const int64_t c = (int64_t)blockIdx.x * kCB + threadIdx.x;
     if (c >= nX) return;
+    const int64_t BB = (int64_t)B * B;
     const int32_t pcol = pos[c];
-    if (pcol >= 0) b[pcol] = rhs[c];
-    else pc[4 * (int64_t)(-pcol - 1) + 3] = rhs[c];
     bool bad = false;
     const int32_t k1 = colptr[c + 1];
     for (int32_t k = colptr[c]; k < k1; ++k) {
         const int32_t prow = pos[rowval[k]];
-        const double v = nzval[k];
+        int64_t q = -1;
         if (pcol >= 0 && prow >= 0) {
             const int64_t br = prow / B, bc = pcol / B;
-            const int64_t o = (br * B + (prow - br * B)) * B + (pcol - bc * B);
-            if (br == bc) D[o] = v;
-            else if (br == bc + 1) L[o] = v;
-            else if (br + 1 == bc) U[o] = v;
+            const int64_t o = br * BB + (pcol - bc * B) * B + (prow - br * B);
+            if (br == bc) q = o;
+            else if (br == bc + 1) q = nb * BB + o;
+            else if (br + 1 == bc) q = 2 * nb * BB + o;
             else bad = true;
+            if (q >= 0) { slot[k] = (int32_t)q; unc[q] = 0; }
         } else if (pcol < 0 && prow < 0) {
-            if (pcol == prow) pc[4 * (int64_t)(-pcol - 1) + 2] = v;
+            if (pcol == prow) slot[k] = -(4 * (-pcol - 1) + 2 + 1);
             else bad = true;
-        } else if (pcol < 0) {            // row h, column λ: cU
-            const int64_t p = -pcol - 1;
-            if (prow == hpos[p]) pc[4 * p + 1] = v;
+        } else if (pcol < 0) {
+            const int32_t p = -pcol - 1;
+            if (prow == hpos[p]) slot[k] = -(4 * p + 1 + 1);
             else bad = true;
-        } else {                           // row λ, column h: cL
-            const int64_t p = -prow - 1;
-            if (pcol == hpos[p]) pc[4 * p + 0] = v;
+        } else {
+            const int32_t p = -prow - 1;
+            if (pcol == hpos[p]) slot[k] = -(4 * p + 0 + 1);
             else bad = true;
         }
+        if (bad) slot[k] = INT32_MIN;
     }
     if (bad) atomicOr(flag, kBadPattern);
+}
+
+// Prepare: the list of uncovered slots (positions from an exclusive scan of unc).
+__global__ __launch_bounds__(kCB) void compact_kernel(int64_t n, const int32_t* __restrict__ unc,
+                                                      const int32_t* __restrict__ at, int32_t* __restrict__ list) {
+    const int64_t q = (int64_t)blockIdx.x * kCB + threadIdx.x;
+    if (q < n && unc[q]) list[at[q]] = (int32_t)q;
+}
+
+__global__ __launch_bounds__(kCB) void zero_slots_kernel(int64_t n, const int32_t* __restrict__ list,
+                                                         double* __restrict__ DLU) {
+    const int64_t q = (int64_t)blockIdx.x * kCB + threadIdx.x;
+    if (q < n) DLU[list[q]] = 0.0;
+}
+
+// One thread per nz: nzval into D|L|U by slot (coalesced reads; the rows of one column
+// are contiguous slots), λ couplings into pc[4p + w].
+__global__ __launch_bounds__(kCB) void pack_kernel(int64_t nnz, const int32_t* __restrict__ slot,
+                                                   const double* __restrict__ nzval, double* __restrict__ DLU,
+                                                   double* __restrict__ pc) {
+    const int64_t k = (int64_t)blockIdx.x * kCB + threadIdx.x;
+    if (k >= nnz) return;
+    const int32_t q = __builtin_nontemporal_load(slot + k);
+    const double v = __builtin_nontemporal_load(nzval + k);
+    if (q >= 0) DLU[q] = v;
+    else pc[-(int64_t)q - 1] = v;
+}
+
+// One thread per dof: Lλ into b, or into the λ pair's r_λ.
+__global__ __launch_bounds__(kCB) void pack_rhs_kernel(int64_t nX, const int32_t* __restrict__ pos,
+                                                       const double* __restrict__ rhs, double* __restrict__ b,
+                                                       double* __restrict__ pc) {
+    const int64_t d = (int64_t)blockIdx.x * kCB + threadIdx.x;
+    if (d >= nX) return;
+    const int32_t p = pos[d];
+    if (p >= 0) b[p] = rhs[d];
+    else pc[4 * (int64_t)(-p - 1) + 3] = rhs[d];
 }
 
 // One thread per λ pair: save row h (L | D | U parts and r_h), replace it by x_h = r_λ/cL.
@@ -177,15 +223,15 @@ __global__ __launch_bounds__(kCB) void pair_kernel(int64_t np, const int32_t* __
     if (dg != 0.0 || cL == 0.0 || cU == 0.0) atomicOr(flag, kBadPair);
     const int64_t h = hpos[p], blk = h / B, i = h - blk * B;
     double* sv = saved + p * (3 * B + 1);
-    const int64_t o = (blk * B + i) * B;
+    const int64_t o = blk * B * B + i;    // row i of the blocks: entries o + j·B
 #pragma unroll
     for (int j = 0; j < B; ++j) {
-        sv[j] = L[o + j];
-        sv[B + j] = D[o + j];
-        sv[2 * B + j] = U[o + j];
-        L[o + j] = 0.0;
-        U[o + j] = 0.0;
-        D[o + j] = (j == i) ? 1.0 : 0.0;
+        sv[j] = L[o + j * B];
+        sv[B + j] = D[o + j * B];
+        sv[2 * B + j] = U[o + j * B];
+        L[o + j * B] = 0.0;
+        U[o + j * B] = 0.0;
+        D[o + j * B] = (j == i) ? 1.0 : 0.0;
     }
     sv[3 * B] = b[h];
     b[h] = rl / cL;
@@ -203,9 +249,9 @@ __global__ __launch_bounds__(kCB) void factor_kernel(int64_t nb, int64_t s, doub
     int piv[B];
     const double* Di = D + i * B * B;
 #pragma unroll
-    for (int r = 0; r < B; ++r)
+    for (int c = 0; c < B; ++c)
 #pragma unroll
-        for (int c = 0; c < B; ++c) a[r][c] = Di[r * B + c];
+        for (int r = 0; r < B; ++r) a[r][c] = Di[c * B + r];
     if (!lu_factor<B>(a, piv)) atomicOr(flag, kSingular);
     double v[B];
     double* bi = b + i * B;
@@ -220,10 +266,10 @@ __global__ __launch_bounds__(kCB) void factor_kernel(int64_t nb, int64_t s, doub
 #pragma unroll 1
         for (int c = 0; c < B; ++c) {
 #pragma unroll
-            for (int r = 0; r < B; ++r) v[r] = M[r * B + c];
+            for (int r = 0; r < B; ++r) v[r] = M[c * B + r];
             lu_solve<B>(a, piv, v);
 #pragma unroll
-            for (int r = 0; r < B; ++r) M[r * B + c] = v[r];
+            for (int r = 0; r < B; ++r) M[c * B + r] = v[r];
         }
     }
 }
@@ -238,10 +284,12 @@ __global__ __launch_bounds__(kCB) void update_kernel(int64_t nb, int64_t s, doub
     const int r = (int)(g - t * B);
     const int64_t j = 2 * s * t;
     if (j >= nb) return;
-    const int64_t o = (j * B + r) * B;
+    const int64_t o = j * B * B + r;       // row r of block j: entries o + c·B
     double Lr[B], Ur[B], Dr[B], nL[B], nU[B];
 #pragma unroll
-    for (int c = 0; c < B; ++c) { Lr[c] = L[o + c]; Ur[c] = U[o + c]; Dr[c] = D[o + c]; nL[c] = 0.0; nU[c] = 0.0; }
+    for (int c = 0; c < B; ++c) {
+        Lr[c] = L[o + c * B]; Ur[c] = U[o + c * B]; Dr[c] = D[o + c * B]; nL[c] = 0.0; nU[c] = 0.0;
+    }
     double br = b[j * B + r];
     if (j >= s) {                          // left neighbour l = j − s (eliminated)
         const int64_t l = j - s;
@@ -254,8 +302,8 @@ __global__ __launch_bounds__(kCB) void update_kernel(int64_t nb, int64_t s, doub
             br -= f * bl[k];
 #pragma unroll
             for (int c = 0; c < B; ++c) {
-                Dr[c] -= f * Ul[k * B + c];
-                nL[c] -= f * Ll[k * B + c];
+                Dr[c] -= f * Ul[c * B + k];
+                nL[c] -= f * Ll[c * B + k];
             }
         }
     }
@@ -270,13 +318,13 @@ __global__ __launch_bounds__(kCB) void update_kernel(int64_t nb, int64_t s, doub
             br -= f * bq[k];
 #pragma unroll
             for (int c = 0; c < B; ++c) {
-                Dr[c] -= f * Lq[k * B + c];
-                nU[c] -= f * Uq[k * B + c];
+                Dr[c] -= f * Lq[c * B + k];
+                nU[c] -= f * Uq[c * B + k];
             }
         }
     }
 #pragma unroll
-    for (int c = 0; c < B; ++c) { D[o + c] = Dr[c]; L[o + c] = nL[c]; U[o + c] = nU[c]; }
+    for (int c = 0; c < B; ++c) { D[o + c * B] = Dr[c]; L[o + c * B] = nL[c]; U[o + c * B] = nU[c]; }
     b[j * B + r] = br;
 }
 
@@ -290,7 +338,7 @@ __global__ void top_kernel(double* __restrict__ D, double* __restrict__ b, unsig
     for (int r = 0; r < B; ++r) {
         v[r] = b[r];
 #pragma unroll
-        for (int c = 0; c < B; ++c) a[r][c] = D[r * B + c];
+        for (int c = 0; c < B; ++c) a[r][c] = D[c * B + r];
     }
     if (!lu_factor<B>(a, piv)) atomicOr(flag, kSingular);
     lu_solve<B>(a, piv, v);
@@ -307,15 +355,15 @@ __global__ __launch_bounds__(kCB) void back_kernel(int64_t nb, int64_t s, const 
     const int r = (int)(g - t * B);
     const int64_t i = s + 2 * s * t;
     if (i >= nb) return;
-    const int64_t o = (i * B + r) * B;
+    const int64_t o = i * B * B + r;
     double x = b[i * B + r];
     const double* xl = b + (i - s) * B;
 #pragma unroll
-    for (int k = 0; k < B; ++k) x -= L[o + k] * xl[k];
+    for (int k = 0; k < B; ++k) x -= L[o + k * B] * xl[k];
     if (i + s < nb) {
         const double* xr = b + (i + s) * B;
 #pragma unroll
-        for (int k = 0; k < B; ++k) x -= U[o + k] * xr[k];
+        for (int k = 0; k < B; ++k) x -= U[o + k * B] * xr[k];
     }
     b[i * B + r] = x;
 }

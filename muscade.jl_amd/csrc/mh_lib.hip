@@ -141,6 +141,9 @@ struct Ctx {
     int32_t* d_cpos = nullptr;   // nX: block position of each dof, or −(p+1) for the λ of pair p
     int32_t* d_chpos = nullptr;  // cnp: block position of each pair's held dof
     int32_t* d_cdum = nullptr;   // cnd: padding positions
+    int32_t* d_cslot = nullptr;  // nnz: slot of each nz in D|L|U, or −(4p+w+1) for a λ coupling
+    int32_t* d_cunc = nullptr;   // cnu: the slots no nz covers (zeroed by each solve)
+    int64_t cnu = 0;
     double* d_cbuf = nullptr;    // D | L | U (3·cnb·B²) | b (cnb·B) | pair couplings (4·cnp)
     double* d_csaved = nullptr;  // cnp × (3B+1): original rows of the held dofs
     double* d_dx = nullptr;      // nX: Δx of the last solve
@@ -172,6 +175,7 @@ void free_graph(Ctx* c, int slot) {
 void free_graph(Ctx* c) { free_graph(c, 0); free_graph(c, 1); }
 void free_chain(Ctx* c) {
     dfree(c->d_cpos); dfree(c->d_chpos); dfree(c->d_cdum); dfree(c->d_cbuf); dfree(c->d_csaved); dfree(c->d_dx);
+    dfree(c->d_cslot); dfree(c->d_cunc); c->cnu = 0;
     dfree(c->d_cflag);
     if (c->h_cflag) { (void)hipHostFree(c->h_cflag); c->h_cflag = nullptr; }
     for (auto& e : c->cev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
@@ -2355,13 +2359,13 @@ int enqueue_chain_solve(Ctx* c) {
     double* U = L + nb * BB;
     double* b = U + nb * BB;
     double* pc = b + nb * B;
-    const int64_t ntot = 3 * nb * BB + nb * B + 4 * c->cnp;
     HIPCHK(c, hipEventRecord(c->cev[0], s));
-    hipLaunchKernelGGL(fill_f64, dim3(cgrid(ntot)), dim3(kCB), 0, s, ntot, D, 0.0);
     hipLaunchKernelGGL(clear_flag, dim3(1), dim3(1), 0, s, c->d_cflag);
-    if (c->cnd) hipLaunchKernelGGL(dummy_kernel<B>, dim3(cgrid(c->cnd)), dim3(kCB), 0, s, c->cnd, c->d_cdum, D);
-    hipLaunchKernelGGL(pack_kernel<B>, dim3(cgrid(c->nX)), dim3(kCB), 0, s, c->nX, c->d_colptr, c->d_rowval,
-                       c->d_nzval, c->d_rhs, c->d_cpos, c->d_chpos, D, L, U, b, pc, c->d_cflag);
+    if (c->cnp) hipLaunchKernelGGL(fill_f64, dim3(cgrid(4 * c->cnp)), dim3(kCB), 0, s, 4 * c->cnp, pc, 0.0);
+    if (c->cnu) hipLaunchKernelGGL(zero_slots_kernel, dim3(cgrid(c->cnu)), dim3(kCB), 0, s, c->cnu, c->d_cunc, D);
+    if (c->cnd) hipLaunchKernelGGL(dummy_kernel<B>, dim3(cgrid(c->cnd)), dim3(kCB), 0, s, c->cnd, c->d_cdum, D, b);
+    if (c->nnz) hipLaunchKernelGGL(pack_kernel, dim3(cgrid(c->nnz)), dim3(kCB), 0, s, c->nnz, c->d_cslot, c->d_nzval, D, pc);
+    hipLaunchKernelGGL(pack_rhs_kernel, dim3(cgrid(c->nX)), dim3(kCB), 0, s, c->nX, c->d_cpos, c->d_rhs, b, pc);
     if (c->cnp) hipLaunchKernelGGL(pair_kernel<B>, dim3(cgrid(c->cnp)), dim3(kCB), 0, s, c->cnp, c->d_chpos, pc, D, L,
                                    U, b, c->d_csaved, c->d_cflag);
     HIPCHK(c, hipEventRecord(c->cev[1], s));
@@ -2385,6 +2389,40 @@ int enqueue_chain_solve(Ctx* c) {
     return MH_OK;
 }
 
+// Prepare-time maps: the slot of every nz, and the list of uncovered slots.
+template <int B>
+int build_chain_slots(Ctx* c) {
+    using namespace chain;
+    hipStream_t s = c->stream;
+    const int64_t nslot = 3 * c->cnb * B * B;
+    int32_t *unc = nullptr, *at = nullptr;
+    HIPCHK(c, hipMalloc(&c->d_cslot, sizeof(int32_t) * std::max<int64_t>(c->nnz, 1)));
+    HIPCHK(c, hipMalloc(&unc, sizeof(int32_t) * (nslot + 1)));
+    HIPCHK(c, hipMalloc(&at, sizeof(int32_t) * (nslot + 1)));
+    hipLaunchKernelGGL(clear_flag, dim3(1), dim3(1), 0, s, c->d_cflag);
+    hipLaunchKernelGGL(fill_i32, dim3(cgrid(nslot + 1)), dim3(kCB), 0, s, nslot + 1, unc, 1);
+    hipLaunchKernelGGL(slot_kernel<B>, dim3(cgrid(c->nX)), dim3(kCB), 0, s, c->nX, c->cnb, c->d_colptr, c->d_rowval,
+                       c->d_cpos, c->d_chpos, c->d_cslot, unc, c->d_cflag);
+    size_t tmp_bytes = 0;
+    void* tmp = nullptr;
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, unc, at, (int)(nslot + 1), s));
+    HIPCHK(c, hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, unc, at, (int)(nslot + 1), s));
+    int32_t nu = 0;
+    HIPCHK(c, hipMemcpyAsync(&nu, at + nslot, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(c->h_cflag, c->d_cflag, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    dfree(tmp);
+    c->cnu = nu;
+    HIPCHK(c, hipMalloc(&c->d_cunc, sizeof(int32_t) * std::max<int64_t>(nu, 1)));
+    hipLaunchKernelGGL(compact_kernel, dim3(cgrid(nslot)), dim3(kCB), 0, s, nslot, unc, at, c->d_cunc);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(s));
+    dfree(unc); dfree(at);
+    if (*c->h_cflag & kBadPattern) { c->err = "matrix entry outside the chain's block-tridiagonal structure"; return MH_ERR_ARG; }
+    return MH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2398,7 +2436,9 @@ int mh_chain_prepare(mh_ctx* c, int B, int64_t nblock, const int64_t* pos, int64
     HIPCHK(c, hipStreamSynchronize(c->stream));
     free_chain(c);
     const int64_t npos = nblock * B;
-    if (npos >= INT32_MAX || npair >= INT32_MAX) { c->err = "chain system too large for 32-bit positions"; return MH_ERR_SIZE; }
+    if (3 * npos * B >= INT32_MAX || 4 * npair >= INT32_MAX) {
+        c->err = "chain system too large for 32-bit slots"; return MH_ERR_SIZE;
+    }
     // every position is taken by exactly one dof or padding entry, every pair by one λ dof
     std::vector<int32_t> p32(c->nX), h32(npair), d32(ndummy);
     std::vector<char> used(npos, 0), lam(npair, 0);
@@ -2421,7 +2461,7 @@ int mh_chain_prepare(mh_ctx* c, int B, int64_t nblock, const int64_t* pos, int64
         if (!lam[p] || v < 1 || v > npos || held[v - 1]) { c->err = "chain layout: bad λ pair " + std::to_string(p + 1); return MH_ERR_ARG; }
         held[v - 1] = 1; h32[p] = (int32_t)(v - 1);
     }
-    const int64_t ntot = 3 * nblock * B * B + npos + 4 * npair;
+    const int64_t ntot = 3 * nblock * B * B + npos + 4 * npair;   // D | L | U | b | pair couplings
     HIPCHK(c, hipMalloc(&c->d_cpos, sizeof(int32_t) * std::max<int64_t>(c->nX, 1)));
     HIPCHK(c, hipMalloc(&c->d_chpos, sizeof(int32_t) * std::max<int64_t>(npair, 1)));
     HIPCHK(c, hipMalloc(&c->d_cdum, sizeof(int32_t) * std::max<int64_t>(ndummy, 1)));
@@ -2435,7 +2475,13 @@ int mh_chain_prepare(mh_ctx* c, int B, int64_t nblock, const int64_t* pos, int64
     if (npair) HIPCHK(c, hipMemcpy(c->d_chpos, h32.data(), sizeof(int32_t) * npair, hipMemcpyHostToDevice));
     if (ndummy) HIPCHK(c, hipMemcpy(c->d_cdum, d32.data(), sizeof(int32_t) * ndummy, hipMemcpyHostToDevice));
     c->cB = B; c->cnb = nblock; c->cnp = npair; c->cnd = ndummy;
-    return MH_OK;
+    const int r = B == 6 ? build_chain_slots<6>(c) : build_chain_slots<3>(c);
+    if (r != MH_OK) {
+        const std::string e = c->err;
+        free_chain(c);
+        c->err = e;
+    }
+    return r;
 }
 
 int mh_chain_solve(mh_ctx* c) {
@@ -2445,7 +2491,6 @@ int mh_chain_solve(mh_ctx* c) {
     HIPCHK(c, hipMemcpyAsync(c->h_cflag, c->d_cflag, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const unsigned f = *c->h_cflag;
-    if (f & chain::kBadPattern) { c->err = "matrix entry outside the chain's block-tridiagonal structure"; return MH_ERR_ARG; }
     if (f & chain::kBadPair) { c->err = "λ row with a nonzero diagonal or a zero coupling"; return MH_ERR_SOLVE; }
     if (f & chain::kSingular) { c->err = "zero or non-finite pivot in a diagonal block"; return MH_ERR_SOLVE; }
     return MH_OK;

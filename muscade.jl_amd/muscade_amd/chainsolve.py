@@ -19,7 +19,6 @@ A model whose elements join more than two nodes, or whose node graph has a branc
 closed loop, is not a chain: `chain_layout` raises NotAChain and the host factorization
 remains the way to solve it.
 """
-import ctypes
 
 import numpy as np
 import scipy.sparse as sp
@@ -100,7 +99,11 @@ def chain_layout(model, dis, colptr, rowval):
         sel = valid.any(axis=1) & (mn != mx)
         if sel.any():
             edges.append(np.stack([mn[sel], mx[sel]], axis=1))
-    Ed = np.unique(np.concatenate(edges), axis=0) if edges else np.zeros((0, 2), np.int64)
+    if edges:
+        key = np.unique(np.concatenate([e[:, 0] * nnod + e[:, 1] for e in edges]))
+        Ed = np.stack([key // nnod, key % nnod], axis=1)
+    else:
+        Ed = np.zeros((0, 2), np.int64)
     deg = np.bincount(Ed.ravel(), minlength=nnod)
     if deg.size and deg.max() > 2:
         raise NotAChain("a node joins more than two elements' nodes (a branch)")
