@@ -177,7 +177,21 @@ def device_newton_iteration(args, out, state, c6):
         asm += out.timings()
         sol += cs.timings()
     w = float(np.mean(wall)) * 1e3
-    res = {"layout_s": t_lay, "nblock": L.nblock, "B": L.B, "npair": int(L.pair_hpos.size),
+    # full-size property: the device Δx solves the assembled system to a backward error
+    # ‖Lλx·Δx − Lλ‖∞ / (‖Lλx‖∞‖Δx‖∞ + ‖Lλ‖∞) of a backward-stable solver (host check)
+    out.set_state(state, 0.0)
+    out.assemble(state, "step", upload=False)
+    cs.solve()
+    dx = cs.dx()
+    K = out.Lλx()
+    rn = np.bincount(out._indices0, weights=np.abs(out.nzval), minlength=out.nX).max()
+    res_ = np.abs(K @ dx - out.Lλ).max()
+    berr = float(res_ / (rn * np.abs(dx).max() + np.abs(out.Lλ).max()))
+    rres = float(res_ / np.abs(out.Lλ).max())
+    del K
+    log(f"device Δx at full size: backward error {berr:.2e}, residual/|Lλ| {rres:.2e}")
+    res = {"layout_s": t_lay, "nblock": L.nblock, "B": L.B, "npair": int(L.pair_hpos.size), "backward_error": berr,
+           "relative_residual": rres,
            "iteration_ms": w, "assembly_kernels_ms": (asm / n).tolist(),
            "solve_ms": {"pack": sol[0] / n, "eliminate": sol[1] / n, "back_substitute": sol[2] / n},
            "norms": [float(np.sqrt(n2[0])), float(np.sqrt(n2[1]))]}

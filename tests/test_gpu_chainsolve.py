@@ -159,3 +159,35 @@ def test_refuses_lattice(mc):
     model, st = build_lattice(3, 3, 2, seed=1)
     with pytest.raises(NotAChain):
         mc.solve(mc.SweepX[0], initialstate=st, time=[1.], verbose=False, catcherror=False, linsolve="device")
+
+
+def test_many_chains_and_repeat_bitwise(mc):
+    """64 chains in one system; a second solve of the same assembly (blocks re-packed after
+    the elimination overwrote them) gives the same Δx bit for bit."""
+    from muscade_amd.workloads import catenary_chains
+    model, st = catenary_chains(nchain=64, nmin=500, nmax=3000, seed=33)
+    st.time = 1.0
+    print("many chains nX", model.ndof["X"], flush=True)
+    out, cs, K, r, dx = device_solve(mc, model, st, 2, "step")
+    assert backward_error(K, dx, r) <= 1e-14
+    cs.solve()
+    assert np.array_equal(cs.dx().view(np.uint64), dx.view(np.uint64))
+
+
+def test_singular_block_raises(mc):
+    """A straight, unstrained bar chain has no transverse stiffness: exact zero pivots, reported
+    as SweepX's "matrix factorization failed" (src/SweepX.jl:207-208)."""
+    from muscade_amd.chainsolve import ChainSolver
+    m = mc.Model("straight")
+    x = np.linspace(0., 4., 5)
+    mc.addnode(m, np.stack([x, 0 * x, 0 * x], axis=1))
+    mc.addelement(m, mc.Bar3D, np.stack([np.arange(1, 5), np.arange(2, 6)], axis=1),
+                  mat=mc.AxisymmetricBarCrossSection(EA=1e6, mu=1.), eps_s=0.)
+    for f in ("t1", "t2", "t3"):
+        mc.addelement(m, mc.Hold, [1], field=f)
+    st = mc.initialize(m, time=1.0)
+    out = mc.AssemblySweepX(0, m, st.dis)
+    out.assemble(st, "iter", fetch=False)
+    cs = ChainSolver(out)
+    with pytest.raises(mc.MuscadeError, match="matrix factorization failed"):
+        cs.solve({"step": 1, "iiter": 1})
