@@ -146,6 +146,21 @@ int mh_device_outputs(mh_ctx* ctx, double** d_rhs, double** d_nzval);
  * checks, like diffed_residual): R (nX×nele) and ∂R/∂X·scale (nX²×nele). */
 int mh_get_element_blocks(mh_ctx* ctx, int igroup, double* R, double* K);
 
+/* ---- element results: getresult / @espy of EulerBeam3D (src/Output.jl:131-181,
+ *      toolbox/BeamElement.jl:28-64,151-174) ---------------------------------------
+ * The requestables of every beam of group igroup at the device state (mh_set_X/U, X up to
+ * ND time levels, ND = 1..3), with no host re-evaluation: per element
+ *   ε[ND], κ[ND][3]                     (☼ε, ♢κ = (vₗ₂[1], vₗ₂[3], −vₗ₂[2])·2/L)
+ *   per Gauss point (4): x[ND][3], κgp[ND][3], fᵢ, mᵢ[3], fₑ[3], mₑ[3]
+ *                                      (☼x, ☼κgp; the resultants ☼fᵢ, ☼mᵢ, ☼fₑ as returned by
+ *                                       `resultants`, before the U-load, and ☼mₑ)
+ * that is nres = 4·ND + 4·(6·ND + 10) numbers, followed by ngauge strain-gauge strains
+ * (StrainGaugeOnEulerBeam3D's ☼ε, toolbox/StrainGaugeOnBeamElement.jl:62-76) for the gauge
+ * tables `gauges` (4×ngauge×nele: E, K1, K2, K3 of each gauge of each element, as the
+ * wrapper's constructor computes them from P and D, StrainGaugeOnBeamElement.jl:57-66).
+ * out: (nres+ngauge)×nele, one element's numbers contiguous. */
+int mh_beam_results(mh_ctx* ctx, int igroup, int ND, int ngauge, const double* gauges, double* out);
+
 /* ---- AssemblyDirect{OX,OU,IA}: one DirectXUA step (src/DirectXUA.jl:18-150) ----
  * prepare(AssemblyDirect{OX,OU,IA},model,dis) (DirectXUA.jl:22-56) builds the class-pair
  * patterns over the classes Λ, X, U (Λ and X share the X dofs): nnz4 = nnz of the patterns

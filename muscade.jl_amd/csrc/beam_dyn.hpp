@@ -186,8 +186,10 @@ template <int NT, bool E> MH_HD Jet<NT, E> jscac(const Jet<NT, E>& a) {
 template <int NT, bool E> struct JM3 { Jet<NT, E> a[9]; };   // column-major 3×3
 template <int NT, bool E> struct JV3 { Jet<NT, E> a[3]; };
 
-// Rodrigues(v) = I + sinc1(θ) S + (sinc1(θ/2)²/2) spin²(S), θ = norm3(v) (Rotations.jl:106-135)
-template <int NT, bool E> MH_HD JM3<NT, E> jrodrigues(const JV3<NT, E>& v) {
+// Rodrigues(v) = I + sinc1(θ) S + (sinc1(θ/2)²/2) spin²(S), θ = norm3(v) (Rotations.jl:106-135).
+// XSQ = false: the square by Muscade's power rule on jets over solver partials (the
+// assembly); XSQ = true: the exact square (jets over plain values, as getresult evaluates).
+template <int NT, bool E, bool XSQ = false> MH_HD JM3<NT, E> jrodrigues(const JV3<NT, E>& v) {
     using J = Jet<NT, E>;
     const J& x = v.a[0]; const J& y = v.a[1]; const J& z = v.a[2];
     J ss = (x * x + y * y) + z * z;
@@ -196,7 +198,9 @@ template <int NT, bool E> MH_HD JM3<NT, E> jrodrigues(const JV3<NT, E>& v) {
     else th = jsqrt(ss);
     J a = jsinc1(th);
     J s = jsinc1(0.5 * th);
-    J b = 0.5 * jsq_muscade(s);
+    J b;
+    if constexpr (XSQ) b = 0.5 * (s * s);
+    else b = 0.5 * jsq_muscade(s);
     // spin²(S) with S = spin(v) (Rotations.jl:114-122)
     J ab = x * y, ca = z * x, bc = y * z;
     J ma2 = -(x * x), mb2 = -(y * y), mc2 = -(z * z);
@@ -280,14 +284,15 @@ template <int NT, bool E> MH_HD Jet<1, E> jslice(const Jet<NT, E>& a, int n) {
 }
 
 // The rotation chain (v₁,v₂) ↦ (rₛₘ, Δv) of BeamElement.jl:192-208 on time jets.
-template <int NT, bool E> MH_HD void dyn_rotation_chain(const BeamGeo& G, const JV3<NT, E>& v1, const JV3<NT, E>& v2,
-                                                        JM3<NT, E>& rsm, JV3<NT, E>& dv) {
-    JM3<NT, E> R1 = jrodrigues(v1);
-    JM3<NT, E> M = jmulABt(jrodrigues(v2), R1);
+template <int NT, bool E, bool XSQ = false>
+MH_HD void dyn_rotation_chain(const BeamGeo& G, const JV3<NT, E>& v1, const JV3<NT, E>& v2, JM3<NT, E>& rsm,
+                              JV3<NT, E>& dv) {
+    JM3<NT, E> R1 = jrodrigues<NT, E, XSQ>(v1);
+    JM3<NT, E> M = jmulABt(jrodrigues<NT, E, XSQ>(v2), R1);
     JV3<NT, E> phi = jrodrigues_inv(M);
 #pragma unroll
     for (int c = 0; c < 3; ++c) dv.a[c] = 0.5 * phi.a[c];
-    rsm = jmul_const(jmul(jrodrigues(dv), R1), G.rm);
+    rsm = jmul_const(jmul(jrodrigues<NT, E, XSQ>(dv), R1), G.rm);
 }
 // A time jet with no ε part as one whose ε part is zero.
 template <int NT, bool E> MH_HD Jet<NT, E> jlift(const Jet<NT, false>& a) {
@@ -300,7 +305,7 @@ template <int NT, bool E> MH_HD Jet<NT, E> jlift(const Jet<NT, false>& a) {
 // Kinematics (BeamElement.jl:176-208) on time jets. Xj: the 12 element dofs as jets.
 // ROTE = false: the ε seed is on a translation dof, so the rotation chain carries no ε part
 // and is evaluated on plain time jets (about half the work of a direction).
-template <int NT, bool E, bool ROTE = true>
+template <int NT, bool E, bool ROTE = true, bool XSQ = false>
 MH_HD void dyn_kinematics(const BeamGeo& G, const Jet<NT, E>* Xj, DynKin<E>& k) {
     using J = Jet<NT, E>;
     const double L = G.L;
@@ -313,7 +318,7 @@ MH_HD void dyn_kinematics(const BeamGeo& G, const Jet<NT, E>* Xj, DynKin<E>& k) 
         JV3<NT, E> v1, v2;
 #pragma unroll
         for (int c = 0; c < 3; ++c) { v1.a[c] = Xj[3 + c]; v2.a[c] = Xj[9 + c]; }
-        dyn_rotation_chain(G, v1, v2, rsm, dv);
+        dyn_rotation_chain<NT, E, XSQ>(G, v1, v2, rsm, dv);
     } else {
         JV3<NT, false> w1, w2;
 #pragma unroll
@@ -324,7 +329,7 @@ MH_HD void dyn_kinematics(const BeamGeo& G, const Jet<NT, E>* Xj, DynKin<E>& k) 
         for (int c = 0; c < 3; ++c) { w1.a[c].e[0] = 0.; w2.a[c].e[0] = 0.; }
         JM3<NT, false> rf;
         JV3<NT, false> df;
-        dyn_rotation_chain(G, w1, w2, rf, df);
+        dyn_rotation_chain<NT, false, XSQ>(G, w1, w2, rf, df);
 #pragma unroll
         for (int i = 0; i < 9; ++i) rsm.a[i] = jlift<NT, E>(rf.a[i]);
 #pragma unroll
@@ -470,6 +475,84 @@ template <int ND> MH_HD void dyn_frozen_loads(const BeamGeo& G, const BeamDynSec
     for (int g = 0; g < 4; ++g) {
         eg[g] = G.dL[g] * mk(fe[g][0].v[0], fe[g][1].v[0], fe[g][2].v[0]);
         mtot = mtot + G.dL[g] * mk(me[0].v[0], me[1].v[0], me[2].v[0]);
+    }
+}
+
+// Requestables of the element (BeamElement.jl:28-64,151-174) at X (ND time levels), what
+// getresult extracts (src/Output.jl:131-181). getresult evaluates the element on plain
+// values, so the time jets carry no solver partials and every square is exact (no
+// power-rule artefact). ☼ε and ♢κ = (vₗ₂[1], vₗ₂[3], −vₗ₂[2])·2/L
+// with their ND time levels, then per Gauss point ☼x and ☼κgp (ND levels each) and the
+// resultants' ☼fᵢ, ☼mᵢ, ☼fₑ (as returned by `resultants`, before the U-load is subtracted)
+// and ☼mₑ. Layout: ε[ND], κ[ND][3], then per Gauss point x[ND][3], κgp[ND][3], fᵢ, mᵢ[3],
+// fₑ[3], mₑ[3] (beam_nres(ND) numbers).
+constexpr int beam_nres(int ND) { return 4 * ND + 4 * (6 * ND + 10); }
+template <int ND> MH_HD void dyn_requestables(const BeamGeo& G, const BeamSection& S, const BeamDynSection& D,
+                                              const double (*X)[12], double* __restrict__ out) {
+    using J = Jet<ND, false>;
+    J Xj[12];
+    dyn_seed<ND, false>(X, nullptr, nullptr, -1, Xj);
+    const double L = G.L;
+    JV3<ND, false> u1, u2, v1, v2;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { u1.a[c] = Xj[c]; u2.a[c] = Xj[6 + c]; v1.a[c] = Xj[3 + c]; v2.a[c] = Xj[9 + c]; }
+    JM3<ND, false> rsm;
+    JV3<ND, false> dv;
+    dyn_rotation_chain<ND, false, true>(G, v1, v2, rsm, dv);
+    JV3<ND, false> dd, cs;
+    const double tg[3] = {G.tgm.x, G.tgm.y, G.tgm.z};
+    const double cm[3] = {G.cm.x, G.cm.y, G.cm.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        cs.a[c] = 0.5 * (u1.a[c] + u2.a[c]);
+        dd.a[c] = (u2.a[c] + tg[c] * 0.5) - cs.a[c];
+    }
+    JV3<ND, false> ul = jmulT(rsm, dd), vl = jmulT(rsm, dv);
+    ul.a[0] = ul.a[0] + (-(L * 0.5));
+    // ε = √((uₗ₂[1]+L/2)² + uₗ₂[2]² + uₗ₂[3]²)·2/L − 1
+    const J a0 = ul.a[0] + 0.5 * L;
+    const J eps = (2. / L) * jsqrt((a0 * a0 + ul.a[1] * ul.a[1]) + ul.a[2] * ul.a[2]) + (-1.);
+    int q = 0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) out[q++] = eps.v[d];
+    const J km[3] = {(2. / L) * vl.a[0], (2. / L) * vl.a[2], (2. / L) * (-vl.a[1])};
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[q++] = km[c].v[d];
+    // loads (values) from the time jets of the Gauss points
+    DynKin<false> k;
+    dyn_kinematics<ND, false, true, true>(G, Xj, k);
+    Jet<1, false> fe[4][3], me[3];
+    const double U0[3] = {0., 0., 0.};
+    dyn_loads<false>(k, D, U0, fe, me);
+    const double ka = 2. / L, kv = 2. / L;
+    for (int g = 0; g < 4; ++g) {
+        const double z = G.zgp[g];
+        const double ya = 2. * z, yu = -4. * (z * z * z) + 3. * z, yv = (z * z - 1. / 4.) * L;
+        const double ku = (-24. * z) / (L * L);
+        JV3<ND, false> p;
+        p.a[0] = ya * ul.a[0] + L * z;
+        p.a[1] = yu * ul.a[1] + yv * vl.a[2];
+        p.a[2] = yu * ul.a[2] - yv * vl.a[1];
+        JV3<ND, false> x = jmulv(rsm, p);
+        J kg[3] = {ka * vl.a[0], ku * ul.a[1] + kv * vl.a[2], ku * ul.a[2] - kv * vl.a[1]};
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) out[q++] = d == 0 ? (x.a[c].v[0] + cs.a[c].v[0]) + cm[c] : x.a[c].v[d] + cs.a[c].v[d];
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) out[q++] = kg[c].v[d];
+        out[q++] = S.EA * eps.v[0];
+        out[q++] = S.GJ * kg[0].v[0];
+        out[q++] = S.EI3 * kg[1].v[0];
+        out[q++] = S.EI2 * kg[2].v[0];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[q++] = fe[g][c].v[0];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[q++] = me[c].v[0];
     }
 }
 

@@ -431,6 +431,29 @@ __global__ __launch_bounds__(kBeamBlock) void beam_dyn_value_kernel(BeamDynArgs 
     for (int64_t f = tid; f < nv; f += kBeamBlock) dst[f] = tile[f];
 }
 
+// Requestables of the beams of one group at the device state (mh_beam_results): one
+// element per lane, written element-major (beam_nres(ND) + ngauge numbers per element);
+// with strain gauges (E, K1, K2, K3 each, per element) their strains E·ε + K1·κ₁ + K2·κ₂ + K3·κ₃ follow
+// (toolbox/StrainGaugeOnBeamElement.jl:70-76).
+template <int ND>
+__global__ __launch_bounds__(kBlock) void beam_results_kernel(BeamDynArgs a, int ngauge, const double* __restrict__ gauges,
+                                                              double* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= a.nele) return;
+    mh::BeamGeo G;
+    mh::BeamSection S;
+    mh::BeamDynSection D;
+    double X[ND][12], U[3];
+    dyn_load<ND, false>(a, e, G, S, D, X, U);
+    constexpr int NR = mh::beam_nres(ND);
+    double* o = out + e * (NR + ngauge);
+    mh::dyn_requestables<ND>(G, S, D, X, o);
+    const double eps = o[0], k1 = o[ND], k2 = o[ND + 1], k3 = o[ND + 2];
+    const double* ge = gauges + e * 4 * ngauge;
+    for (int k = 0; k < ngauge; ++k)
+        o[NR + k] = ((ge[4 * k] * eps + ge[4 * k + 1] * k1) + ge[4 * k + 2] * k2) + ge[4 * k + 3] * k3;
+}
+
 // Direction pass: one lane per trial direction of an element (δX of dof k, k < 12; δr on
 // :step). Each lane propagates the time jets with its own ε seed; the lanes of an element
 // then exchange the Jacobian columns of x_gp and vₛₘ through LDS, and the lane of direction k
@@ -2551,6 +2574,49 @@ int mh_get_X(mh_ctx* c, int ider, double* x) {
     if (!c || ider < 0 || ider > 2 || !x || (c->nX && !c->d_X[ider])) return MH_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     if (c->nX) HIPCHK(c, hipMemcpy(x, c->d_X[ider], sizeof(double) * c->nX, hipMemcpyDeviceToHost));
+    return MH_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int mh_beam_results(mh_ctx* c, int igroup, int ND, int ngauge, const double* gauges, double* out) {
+    if (!c || igroup < 0 || igroup >= (int)c->groups.size() || ND < 1 || ND > 3 || ngauge < 0 || !out ||
+        (ngauge > 0 && !gauges))
+        return MH_ERR_ARG;
+    const Group& g = c->groups[igroup];
+    if (g.kind != MH_KIND_BEAM && g.kind != MH_KIND_BEAM_U) { c->err = "group is not an EulerBeam3D group"; return MH_ERR_ARG; }
+    HIPCHK(c, hipSetDevice(c->device));
+    if (g.nele == 0) return MH_OK;
+    const int NR = ND == 1 ? mh::beam_nres(1) : ND == 2 ? mh::beam_nres(2) : mh::beam_nres(3);
+    const int64_t n = g.nele * (NR + ngauge);
+    double *d_out = nullptr, *d_g = nullptr;
+    HIPCHK(c, hipMalloc(&d_out, sizeof(double) * n));
+    if (ngauge) {
+        if (hipMalloc(&d_g, sizeof(double) * 4 * ngauge * g.nele) != hipSuccess) {
+            dfree(d_out); c->err = "hipMalloc"; return MH_ERR_HIP;
+        }
+    }
+    BeamDynArgs b;
+    b.nele = g.nele; b.idx = g.d_idx; b.uidx = g.d_uidx; b.nodes = g.d_nodes; b.xyz = c->d_xyz; b.o2 = g.d_o2;
+    b.sec = g.d_sec;
+    for (int d = 0; d < 3; ++d) b.X[d] = c->d_X[d];
+    b.U = c->d_U;
+    hipStream_t s = c->stream;
+    hipError_t e = hipSuccess;
+    if (ngauge) e = hipMemcpyAsync(d_g, gauges, sizeof(double) * 4 * ngauge * g.nele, hipMemcpyHostToDevice, s);
+    const dim3 gr((unsigned)((g.nele + kBlock - 1) / kBlock));
+    if (e == hipSuccess) {
+        if (ND == 1) hipLaunchKernelGGL(beam_results_kernel<1>, gr, dim3(kBlock), 0, s, b, ngauge, d_g, d_out);
+        else if (ND == 2) hipLaunchKernelGGL(beam_results_kernel<2>, gr, dim3(kBlock), 0, s, b, ngauge, d_g, d_out);
+        else hipLaunchKernelGGL(beam_results_kernel<3>, gr, dim3(kBlock), 0, s, b, ngauge, d_g, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dfree(d_out); dfree(d_g);
+    if (e != hipSuccess) { c->err = std::string("beam results: ") + hipGetErrorString(e); return MH_ERR_HIP; }
     return MH_OK;
 }
 

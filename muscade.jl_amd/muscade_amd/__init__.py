@@ -7,14 +7,17 @@ from ._lib import LIBPATH, lib
 from .model import (Disassembler, Model, MuscadeError, State, addelement, addnode, getdof, initialize, setdof,
                     setscale)
 from .elements import (Acost, AxisymmetricBarCrossSection, Bar3D, Bar3DU, BeamCrossSection, DofCost, DofLoad,
-                       EulerBeam3D, EulerBeam3DU, Hold, SingleAcost, SingleDofCost, SingleUdof)
+                       EulerBeam3D, EulerBeam3DU, Hold, SingleAcost, SingleDofCost, SingleUdof,
+                       StrainGaugeOnEulerBeam3D)
 from .assemble import AssemblyDirect, AssemblySweepX, DeviceContext, Newmarkβcoefficients
 from .sweepx import SweepX, solve
 from .directxua import DirectXUA
 from .hostcost import Dual2, exp10
+from .results import getresult, request
 
 __all__ = ["LIBPATH", "lib", "Model", "MuscadeError", "State", "Disassembler", "addnode", "addelement", "initialize",
            "setscale", "getdof", "setdof", "BeamCrossSection", "EulerBeam3D", "EulerBeam3DU", "Hold", "DofLoad",
            "AxisymmetricBarCrossSection", "Bar3D", "Bar3DU",
            "DofCost", "SingleDofCost", "SingleUdof", "Acost", "SingleAcost", "DirectXUA", "Dual2", "exp10",
+           "StrainGaugeOnEulerBeam3D", "getresult", "request",
            "AssemblyDirect", "AssemblySweepX", "DeviceContext", "Newmarkβcoefficients", "SweepX", "solve"]

@@ -87,6 +87,34 @@ class EulerBeam3DU(EulerBeam3D):
     kind = _lib.MH_KIND_BEAM_U
 
 
+class StrainGaugeOnEulerBeam3D(EulerBeam3D):
+    """StrainGaugeOnEulerBeam3D(nod; P, D, elementkwargs=(mat, orient2)) —
+    toolbox/StrainGaugeOnBeamElement.jl:55-76. An EulerBeam3D (same dofs, same residual: the
+    device assembles it as a beam) with strain gauges halfway along it. P (3×Ngauge) are the
+    gauges' offsets from the axis (P[0,:] = 0), D (3×Ngauge) their directions. Its
+    requestables (muscade_amd.results.getresult): εₐₓ, κ and the gauge strains
+    ε = E·εₐₓ + K1·κ₁ + K2·κ₂ + K3·κ₃, with E, K1, K2, K3 from P and D as the reference's
+    constructor computes them."""
+
+    @classmethod
+    def construct(cls, model, nodes0, P, D, elementkwargs):
+        P = np.asarray(P, dtype=np.float64)
+        D = np.asarray(D, dtype=np.float64)
+        if P.ndim != 2 or P.shape[0] != 3 or D.shape != P.shape:
+            raise MuscadeError("P and D must be 3×Ngauge")
+        if not np.all(P[0] == 0.):
+            raise MuscadeError("In arguments of StrainGaugeOnEulerBeam3D, P[1,:] must all be zero")
+        key, doflist, params = EulerBeam3D.construct.__func__(cls, model, nodes0, **elementkwargs)
+        E = D[0] ** 2
+        K1 = D[0] * (D[2] * P[1] - D[1] * P[2])
+        K2 = -D[0] ** 2 * P[1]
+        K3 = -D[0] ** 2 * P[2]
+        nele = nodes0.shape[0]
+        params["gauges"] = np.broadcast_to(np.stack([E, K1, K2, K3], axis=1), (nele, P.shape[1], 4)).copy()
+        params["P"], params["D"] = P, D
+        return ("StrainGaugeOnEulerBeam3D", P.shape[1]) + key, doflist, params
+
+
 BAR_FIELDS = ("EA", "μ", "w", "Caₜ", "Clₜ", "Cqₜ", "Caₙ", "Clₙ", "Cqₙ")
 
 

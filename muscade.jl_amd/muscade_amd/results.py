@@ -1,0 +1,104 @@
+"""Element results on the device: getresult of EulerBeam3D's requestables (src/Output.jl:131-181,
+toolbox/BeamElement.jl:28-64,151-174) and of StrainGaugeOnEulerBeam3D
+(toolbox/StrainGaugeOnBeamElement.jl:62-76), without host re-evaluation of the elements.
+
+A request mirrors `@request`: `request("ε", gp=request("κgp", resultants=request("fᵢ", "mᵢ")))`
+stands for `@request ε, gp(κgp, resultants(fᵢ, mᵢ))`. getresult returns nested dicts of numpy
+arrays whose first axis is the element (eleres[iele] in the reference) and, under "gp", whose
+second axis is the Gauss point (eleres[iele].gp[igp]). Quantities with time levels (ε, κ, x,
+κgp) carry an axis of the state's X derivatives (ND) before their components.
+"""
+import numpy as np
+
+from . import _lib
+from ._lib import as_f64, dptr, lib
+from .assemble import DeviceContext, upload_model
+from .model import MuscadeError
+
+BEAM_GP = ("x", "κgp", "resultants")
+BEAM_RESULTANTS = ("fᵢ", "mᵢ", "fₑ", "mₑ")
+
+
+def request(*names, **sub):
+    """@request: leaf names, and sub-requests for nested requestables."""
+    r = {n: None for n in names}
+    r.update(sub)
+    return r
+
+
+def _nres(ND):
+    return 4 * ND + 4 * (6 * ND + 10)
+
+
+def _split(raw, ND, ngauge):
+    """mh_beam_results' layout as named arrays (leading axis: element)."""
+    n = raw.shape[0]
+    q = 0
+    out = {"ε": raw[:, q:q + ND]}; q += ND
+    out["κ"] = raw[:, q:q + 3 * ND].reshape(n, ND, 3); q += 3 * ND
+    per = 6 * ND + 10
+    g = raw[:, q:q + 4 * per].reshape(n, 4, per); q += 4 * per
+    gp = {"x": g[:, :, :3 * ND].reshape(n, 4, ND, 3), "κgp": g[:, :, 3 * ND:6 * ND].reshape(n, 4, ND, 3)}
+    o = 6 * ND
+    gp["resultants"] = {"fᵢ": g[:, :, o], "mᵢ": g[:, :, o + 1:o + 4], "fₑ": g[:, :, o + 4:o + 7],
+                        "mₑ": g[:, :, o + 7:o + 10]}
+    out["gp"] = gp
+    out["gauges"] = raw[:, q:q + ngauge]
+    return out
+
+
+def _select(res, req, where):
+    sel = {}
+    for k, v in req.items():
+        if k not in res:
+            raise MuscadeError(f"{k} is not a requestable of {where} (requestables: {', '.join(res)})")
+        sel[k] = res[k] if v is None else _select(res[k], v, f"{where}.{k}")
+    return sel
+
+
+class ElementResults:
+    """The model's element groups on a device context, for result extraction."""
+
+    def __init__(self, model, dis, device=0):
+        self.model, self.dis = model, dis
+        self.ctx = DeviceContext(device)
+        self.groups = upload_model(self.ctx, model, dis)
+
+    def beam(self, state, ieletyp, ND=None):
+        """All requestables of beam type ieletyp (0-based) at `state` (X up to ND levels)."""
+        E = self.model.eletypes[ieletyp]
+        if E.cls.kind not in (_lib.MH_KIND_BEAM, _lib.MH_KIND_BEAM_U):
+            raise MuscadeError(f"element type {ieletyp + 1} ({E.key[0]}) has no device requestables")
+        ND = min(len(state.X), 3) if ND is None else ND
+        h = self.ctx.h
+        for d in range(ND):
+            self.ctx.check(lib.mh_set_X(h, d, dptr(as_f64(state.X[d]))), "set X")
+        if self.model.ndof["U"]:
+            self.ctx.check(lib.mh_set_U(h, dptr(as_f64(state.U[0]))), "set U")
+        gauges = None
+        ng = 0
+        if "gauges" in E.params[0]:
+            gauges = as_f64(np.concatenate([p["gauges"] for p in E.params]))
+            ng = gauges.shape[1]
+        raw = np.zeros((E.nele, _nres(ND) + ng))
+        self.ctx.check(lib.mh_beam_results(h, self.groups[ieletyp], ND, ng, dptr(gauges), dptr(raw)), "beam results")
+        return _split(raw, ND, ng)
+
+
+def getresult(state, req, ieletyp, device=0, _cache={}):
+    """eleres = getresult(state, req, eletype) for all elements of type ieletyp (0-based)
+    (src/Output.jl:164-175). EulerBeam3D requestables: ε, κ, gp(x, κgp, resultants(fᵢ, mᵢ,
+    fₑ, mₑ)); StrainGaugeOnEulerBeam3D: εₐₓ, κ, ε (the gauge strains) and eleres(...) with
+    the beam's requestables."""
+    model = state.model
+    key = (id(model), device)
+    er = _cache.get(key)
+    if er is None or er.model is not model:
+        er = _cache[key] = ElementResults(model, state.dis, device)
+    r = er.beam(state, ieletyp)
+    E = model.eletypes[ieletyp]
+    if "gauges" in E.params[0]:
+        res = {"εₐₓ": r["ε"][:, 0], "κ": r["κ"][:, 0], "ε": r["gauges"],
+               "eleres": {k: r[k] for k in ("ε", "κ", "gp")}}
+        return _select(res, req, E.key[0])
+    return _select({k: r[k] for k in ("ε", "κ", "gp")}, req, E.key[0])

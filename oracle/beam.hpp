@@ -13,6 +13,7 @@
 // result equals direct evaluation through second order (Taylor.jl:271-295).
 #pragma once
 #include <cmath>
+#include <type_traits>
 #include "dual.hpp"
 #include "rotations.hpp"
 
@@ -177,11 +178,20 @@ template <class TX, int ND> inline void intrinsic_rates(const M3<TX>* r, V3<TX>&
 
 // Optional requestables of the residual (used by the strain-gauge wrapper).
 template <class TX> struct BeamReq { TX eps; V3<TX> kappa_mid; };
+// All requestables of the residual (BeamElement.jl:28-64,151-174), plain values: ☼ε, ♢κ,
+// and per Gauss point ☼x, ☼κgp with their ND time levels, and the resultants' ☼fᵢ, ☼mᵢ,
+// ☼fₑ (before the U-load is subtracted, BeamElement.jl:161), ☼mₑ.
+struct BeamRes {
+    double eps[3], kmid[3][3];
+    double x[4][3][3], kgp[4][3][3];   // [gp][der][comp]
+    double fi[4], mi[4][3], fe[4][3], me[4][3];
+};
 
 // residual(::EulerBeam3D, X, U, A, t) — BeamElement.jl:151-174.
 // X: ND vectors of 12; U: NDU vectors of 3 (only if o.Udof).
 template <class TX, int ND, int NDU>
-inline void beam_residual(const EulerBeam3D& o, const TX (*X)[12], const TX (*U)[3], TX* R, BeamReq<TX>* req = nullptr) {
+inline void beam_residual(const EulerBeam3D& o, const TX (*X)[12], const TX (*U)[3], TX* R, BeamReq<TX>* req = nullptr,
+                          BeamRes* res = nullptr) {
     using J = Jet<TX, ND>;
     using JT = typename J::type;
     // direct kinematics over time-jets
@@ -235,6 +245,14 @@ inline void beam_residual(const EulerBeam3D& o, const TX (*X)[12], const TX (*U)
         TX fint = m.EA * eps[0];
         V3<TX> mint;
         mint[0] = m.GJ * kap[g][0][0]; mint[1] = m.EI3 * kap[g][1][0]; mint[2] = m.EI2 * kap[g][2][0];
+        if constexpr (std::is_same_v<TX, double>) {
+            if (res) {
+                res->fi[g] = fint;
+                for (int c = 0; c < 3; ++c) { res->mi[g][c] = mint[c]; res->fe[g][c] = fe[c]; res->me[g][c] = me[c]; }
+                for (int d = 0; d < ND; ++d)
+                    for (int c = 0; c < 3; ++c) { res->x[g][d][c] = xg[g][c][d]; res->kgp[g][d][c] = kap[g][c][d]; }
+            }
+        }
         if (o.Udof && U) for (int c = 0; c < 3; ++c) fe[c] = fe[c] - U[0][c];
         for (int j = 0; j < 12; ++j) {
             TX a = fint * epsJ[j];
@@ -246,6 +264,17 @@ inline void beam_residual(const EulerBeam3D& o, const TX (*X)[12], const TX (*U)
         }
     }
     for (int j = 0; j < 12; ++j) R[j] = Rsum[j];
+    if constexpr (std::is_same_v<TX, double>) {
+        if (res) {
+            TX tmp[3];
+            V3<JT> kk; kk[0] = kj.vl2[0]; kk[1] = kj.vl2[2]; kk[2] = -kj.vl2[1];
+            for (int d = 0; d < ND; ++d) res->eps[d] = eps[d];
+            for (int c = 0; c < 3; ++c) {
+                J::unpack(kk[c], tmp);
+                for (int d = 0; d < ND; ++d) res->kmid[d][c] = tmp[d] * (2. / o.L);
+            }
+        }
+    }
     if (req) {
         req->eps = eps[0];
         TX tmp[3];

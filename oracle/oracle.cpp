@@ -241,6 +241,35 @@ int orc_beam_residual_plain(const double* c1, const double* c2, const double* or
     return 0;
 }
 
+// All requestables of EulerBeam3D at X (ND time levels) and U (3 or none), in the device's
+// layout (include/muscade_hip.h, mh_beam_results): ε[ND], κ[ND][3], then per Gauss point
+// x[ND][3], κgp[ND][3], fᵢ, mᵢ[3], fₑ[3], mₑ[3].
+int orc_beam_results(const double* c1, const double* c2, const double* orient2, const double* mat16, int ND,
+                     const double* X, const double* U, double* out) {
+    BeamCrossSection mat; std::memcpy(&mat, mat16, sizeof(mat));
+    EulerBeam3D o;
+    if (make_beam(c1, c2, orient2, mat, U ? 1 : 0, &o)) return 1;
+    double Xd[3][12], Ud[1][3] = {{0., 0., 0.}}, R[12];
+    for (int d = 0; d < ND; ++d) for (int i = 0; i < 12; ++i) Xd[d][i] = X[d * 12 + i];
+    if (U) for (int c = 0; c < 3; ++c) Ud[0][c] = U[c];
+    BeamRes rs;
+    if (ND == 1) beam_residual<double, 1, 1>(o, Xd, Ud, R, nullptr, &rs);
+    else if (ND == 2) beam_residual<double, 2, 1>(o, Xd, Ud, R, nullptr, &rs);
+    else beam_residual<double, 3, 1>(o, Xd, Ud, R, nullptr, &rs);
+    int q = 0;
+    for (int d = 0; d < ND; ++d) out[q++] = rs.eps[d];
+    for (int d = 0; d < ND; ++d) for (int c = 0; c < 3; ++c) out[q++] = rs.kmid[d][c];
+    for (int g = 0; g < 4; ++g) {
+        for (int d = 0; d < ND; ++d) for (int c = 0; c < 3; ++c) out[q++] = rs.x[g][d][c];
+        for (int d = 0; d < ND; ++d) for (int c = 0; c < 3; ++c) out[q++] = rs.kgp[g][d][c];
+        out[q++] = rs.fi[g];
+        for (int c = 0; c < 3; ++c) out[q++] = rs.mi[g][c];
+        for (int c = 0; c < 3; ++c) out[q++] = rs.fe[g][c];
+        for (int c = 0; c < 3; ++c) out[q++] = rs.me[g][c];
+    }
+    return 0;
+}
+
 // ε and ♢κ with their gradients w.r.t. X₀ (12): the strain-gauge requestables
 // (toolbox/StrainGaugeOnBeamElement.jl:70-76) differentiated as by
 // diffed_lagrangian{1} (Diagnostic.jl:851-911). Also R and the first-order

@@ -39,6 +39,7 @@ _SIG = {
     "orc_beam_diffed_residual": (ctypes.c_int, [_D, _D, _D, _D, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, _D]),
     "orc_beam_residual_plain": (ctypes.c_int, [_D, _D, _D, _D, ctypes.c_int, _D, _D, _D, _D]),
     "orc_beam_gauge": (ctypes.c_int, [_D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
+    "orc_beam_results": (ctypes.c_int, [_D, _D, _D, _D, ctypes.c_int, _D, _D, _D]),
     "orc_number_dofs": (ctypes.c_int, [ctypes.c_int, _IP, _IP, _IP, ctypes.c_int, _IP, _IP, _IP, _IP, _IP, _IP, _IP,
                                        _I64, ctypes.c_int, _IP, _IP, _IP, _IP, _IP, _IP]),
     "orc_model_new": (_V, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
@@ -145,6 +146,21 @@ def beam_residual_plain(c1, c2, orient2, mat16, X):
     lib.orc_beam_residual_plain(dp(f64(c1)), dp(f64(c2)), dp(f64(orient2)), dp(f64(mat16)), ND, dp(Xa), dp(R), dp(eps),
                                 dp(kap))
     return R, eps[0], kap
+
+
+def beam_nres(ND):
+    return 4 * ND + 4 * (6 * ND + 10)
+
+
+def beam_results(c1, c2, orient2, mat16, X, U=None):
+    """The requestables of one beam (layout of mh_beam_results) at X (list of ND vectors)."""
+    ND = len(X)
+    Xa = f64(np.concatenate([np.asarray(x, float) for x in X]))
+    out = np.zeros(beam_nres(ND))
+    r = lib.orc_beam_results(dp(f64(c1)), dp(f64(c2)), dp(f64(orient2)), dp(f64(mat16)), ND, dp(Xa),
+                             dp(f64(U)) if U is not None else None, dp(out))
+    assert r == 0
+    return out
 
 
 def beam_gauge(c1, c2, orient2, mat16, X):
