@@ -161,6 +161,15 @@ int mh_get_element_blocks(mh_ctx* ctx, int igroup, double* R, double* K);
  * out: (nres+ngauge)×nele, one element's numbers contiguous. */
 int mh_beam_results(mh_ctx* ctx, int igroup, int ND, int ngauge, const double* gauges, double* out);
 
+/* Derivatives of the strain-gauge requestables for costs on them (ElementCost wrapping a
+ * StrainGaugeOnEulerBeam3D, src/BasicElements.jl:117-132, in DirectXUA's Lagrangian path,
+ * src/DirectXUA.jl:121-171): for the elements sel (1-based, nsel of them; NULL = all) of
+ * beam group igroup at the device X, the values of εₐₓ, κ₁, κ₂, κ₃, their gradients (4×12)
+ * and their Hessians (4×78, upper triangle row-major) w.r.t. the 12 element X-dofs in
+ * unscaled model coordinates, exactly as revariate{2} (src/Taylor.jl:143-151) forms them.
+ * out: 364×n. The gauge strains are linear in these (E, K1, K2, K3). */
+int mh_beam_gauge_derivs(mh_ctx* ctx, int igroup, int64_t nsel, const int64_t* sel, double* out);
+
 /* ---- AssemblyDirect{OX,OU,IA}: one DirectXUA step (src/DirectXUA.jl:18-150) ----
  * prepare(AssemblyDirect{OX,OU,IA},model,dis) (DirectXUA.jl:22-56) builds the class-pair
  * patterns over the classes Λ, X, U (Λ and X share the X dofs): nnz4 = nnz of the patterns

@@ -556,6 +556,42 @@ template <int ND> MH_HD void dyn_requestables(const BeamGeo& G, const BeamSectio
     }
 }
 
+// The strain-gauge requestables εₐₓ = ε and κ = ♢κ (BeamElement.jl:155,172;
+// StrainGaugeOnBeamElement.jl:73) on jets whose "time" level carries ∂/∂X_i and whose ε
+// level carries ∂/∂X_j: with the seeds X_k = X₀_k + t·δ_ki + ε·δ_kj, v[0] is the value,
+// v[1] = ∂/∂X_i, e[0] = ∂/∂X_j and e[1] = ∂²/∂X_i∂X_j (exact squares: second derivatives
+// at the value level, as revariate{2} forms them, have no power-rule artefact).
+MH_HD void gauge_jets(const BeamGeo& G, const double* X0, int i, int j, Jet<2, true>* q) {
+    using J = Jet<2, true>;
+    J Xj[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        Xj[k].v[0] = X0[k]; Xj[k].v[1] = k == i ? 1. : 0.;
+        Xj[k].e[0] = k == j ? 1. : 0.; Xj[k].e[1] = 0.;
+    }
+    const double L = G.L;
+    JV3<2, true> u1, u2, v1, v2;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { u1.a[c] = Xj[c]; u2.a[c] = Xj[6 + c]; v1.a[c] = Xj[3 + c]; v2.a[c] = Xj[9 + c]; }
+    JM3<2, true> rsm;
+    JV3<2, true> dv;
+    dyn_rotation_chain<2, true, true>(G, v1, v2, rsm, dv);
+    JV3<2, true> dd;
+    const double tg[3] = {G.tgm.x, G.tgm.y, G.tgm.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const J cs = 0.5 * (u1.a[c] + u2.a[c]);
+        dd.a[c] = (u2.a[c] + tg[c] * 0.5) - cs;
+    }
+    JV3<2, true> ul = jmulT(rsm, dd), vl = jmulT(rsm, dv);
+    ul.a[0] = ul.a[0] + (-(L * 0.5));
+    const J a0 = ul.a[0] + 0.5 * L;
+    q[0] = (2. / L) * jsqrt((a0 * a0 + ul.a[1] * ul.a[1]) + ul.a[2] * ul.a[2]) + (-1.);
+    q[1] = (2. / L) * vl.a[0];
+    q[2] = (2. / L) * vl.a[2];
+    q[3] = (2. / L) * (-vl.a[1]);
+}
+
 // One trial direction (see dyn_seed): the ε-derivatives of the loads (DF, Dm) and,
 // for a δX direction, the column of the Jacobians of x_gp and vₛₘ w.r.t. X₀ (Jx, Jv).
 struct DynDir { double DF[4][3]; double Dm[3]; double Jx[4][3]; double Jv[3]; };

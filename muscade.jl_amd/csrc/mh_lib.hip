@@ -454,6 +454,37 @@ __global__ __launch_bounds__(kBlock) void beam_results_kernel(BeamDynArgs a, int
         o[NR + k] = ((ge[4 * k] * eps + ge[4 * k + 1] * k1) + ge[4 * k + 2] * k2) + ge[4 * k + 3] * k3;
 }
 
+// First and second derivatives of the strain-gauge requestables (εₐₓ, κ₁, κ₂, κ₃) w.r.t.
+// the 12 element dofs at X₀ (mh_beam_gauge_derivs): one lane per dof pair i ≤ j (78 per
+// element). Per element: values (4), gradients (4×12), Hessians (4×78, upper triangle).
+constexpr int kPairs = 78;
+constexpr int kGaugeOut = 4 + 4 * 12 + 4 * kPairs;
+__global__ __launch_bounds__(kBlock) void beam_gauge_derivs_kernel(BeamDynArgs a, const int32_t* __restrict__ sel,
+                                                                   int64_t nsel, double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t s = t / kPairs;
+    if (s >= nsel) return;
+    const int p = (int)(t - s * kPairs);
+    int i = 0, rem = p;
+    while (rem >= 12 - i) { rem -= 12 - i; ++i; }
+    const int j = i + rem;
+    const int64_t e = sel ? sel[s] : s;
+    mh::BeamGeo G;
+    mh::BeamSection S;
+    mh::BeamDynSection D;
+    double X[1][12], U[3];
+    dyn_load<1, false>(a, e, G, S, D, X, U);
+    mh::Jet<2, true> q[4];
+    mh::gauge_jets(G, X[0], i, j, q);
+    double* o = out + s * kGaugeOut;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (p == 0) o[k] = q[k].v[0];
+        if (i == j) o[4 + 12 * k + i] = q[k].v[1];
+        o[52 + kPairs * k + p] = q[k].e[1];
+    }
+}
+
 // Direction pass: one lane per trial direction of an element (δX of dof k, k < 12; δr on
 // :step). Each lane propagates the time jets with its own ε seed; the lanes of an element
 // then exchange the Jacobian columns of x_gp and vₛₘ through LDS, and the lane of direction k
@@ -2617,6 +2648,46 @@ int mh_beam_results(mh_ctx* c, int igroup, int ND, int ngauge, const double* gau
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     dfree(d_out); dfree(d_g);
     if (e != hipSuccess) { c->err = std::string("beam results: ") + hipGetErrorString(e); return MH_ERR_HIP; }
+    return MH_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int mh_beam_gauge_derivs(mh_ctx* c, int igroup, int64_t nsel, const int64_t* sel, double* out) {
+    if (!c || igroup < 0 || igroup >= (int)c->groups.size() || nsel < 0 || !out) return MH_ERR_ARG;
+    const Group& g = c->groups[igroup];
+    if (g.kind != MH_KIND_BEAM && g.kind != MH_KIND_BEAM_U) { c->err = "group is not an EulerBeam3D group"; return MH_ERR_ARG; }
+    const int64_t n = sel ? nsel : g.nele;
+    if (sel)
+        for (int64_t k = 0; k < nsel; ++k)
+            if (sel[k] < 1 || sel[k] > g.nele) { c->err = "element number out of range"; return MH_ERR_ARG; }
+    HIPCHK(c, hipSetDevice(c->device));
+    if (n == 0) return MH_OK;
+    std::vector<int32_t> s32;
+    if (sel) { s32.resize(n); for (int64_t k = 0; k < n; ++k) s32[k] = (int32_t)(sel[k] - 1); }
+    double* d_out = nullptr;
+    int32_t* d_sel = nullptr;
+    HIPCHK(c, hipMalloc(&d_out, sizeof(double) * n * kGaugeOut));
+    if (sel && hipMalloc(&d_sel, sizeof(int32_t) * n) != hipSuccess) { dfree(d_out); c->err = "hipMalloc"; return MH_ERR_HIP; }
+    BeamDynArgs b;
+    b.nele = g.nele; b.idx = g.d_idx; b.uidx = g.d_uidx; b.nodes = g.d_nodes; b.xyz = c->d_xyz; b.o2 = g.d_o2;
+    b.sec = g.d_sec;
+    for (int d = 0; d < 3; ++d) b.X[d] = c->d_X[d];
+    b.U = c->d_U;
+    hipStream_t s = c->stream;
+    hipError_t e = hipSuccess;
+    if (sel) e = hipMemcpyAsync(d_sel, s32.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(beam_gauge_derivs_kernel, dim3((unsigned)((n * kPairs + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                           b, d_sel, n, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, sizeof(double) * n * kGaugeOut, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dfree(d_out); dfree(d_sel);
+    if (e != hipSuccess) { c->err = std::string("gauge derivatives: ") + hipGetErrorString(e); return MH_ERR_HIP; }
     return MH_OK;
 }
 

@@ -65,6 +65,7 @@ SIGNATURES = {
     "mh_newton_update": (_I, [_P, _I, _I, _D, _D]),
     "mh_get_X": (_I, [_P, _I, _D]),
     "mh_beam_results": (_I, [_P, _I, _I, _I, _D, _D]),
+    "mh_beam_gauge_derivs": (_I, [_P, _I, _L, _I64, _D]),
     "mh_last_timings": (_I, [_P, _D]),
     "mh_calib_copy": (_I, [_P, _L, _D]),
     "mh_sync": (_I, [_P]),

@@ -238,6 +238,12 @@ class AssemblyDirect:
             self.patterns.append((cp, rv, (nr, nc)))
         self.nder = {"Λ": 1, "X": OX + 1, "U": OU + 1}
 
+    def gauge_derivatives(self, it, state):
+        """Values, gradients and Hessians of the gauge requestables of beam type it at
+        state.X[0] (for ElementCost types; mh_beam_gauge_derivs on this context)."""
+        from .results import gauge_derivatives_on
+        return gauge_derivatives_on(self.ctx, self.groups[it], state, self.model.eletypes[it].nele)
+
     def _which(self, a, b):
         return (0 if a != "U" else 2) if b != "U" else (1 if a != "U" else 3)
 

@@ -64,13 +64,19 @@ class HostLagrangians:
     DirectXUA_lagrangian_addition! (src/DirectXUA.jl:121-150): partials Λ | X⁽⁰..OX⁾ | U⁽⁰..OU⁾
     | A (IA = 1), each scaled like revariate with the element type's dof scales."""
 
-    def __init__(self, model, dis, OX, OU, IA):
+    def __init__(self, model, dis, OX, OU, IA, gauge_provider=None):
+        """gauge_provider(it, state) → values (nele,4), gradients (nele,4,12), Hessians
+        (nele,4,12,12) of the gauge requestables of element type it, for the ElementCost
+        types (their wrapped beams stay on the device: `skip` lists only the host types)."""
         self.model, self.dis = model, dis
         self.OX, self.OU, self.IA = OX, OU, IA
         self.nder = {"Λ": 1, "X": OX + 1, "U": OU + 1, "A": IA}
         self.ndof = {"Λ": model.ndof["X"], "X": model.ndof["X"], "U": model.ndof["U"], "A": model.ndof["A"]}
-        self.types = [it for it, E in enumerate(model.eletypes)
-                      if getattr(E.cls, "host", False) or getattr(E.cls, "direct_host", False)]
+        self.skip = [it for it, E in enumerate(model.eletypes)
+                     if getattr(E.cls, "host", False) or getattr(E.cls, "direct_host", False)]
+        self.costed = [it for it, E in enumerate(model.eletypes) if getattr(E.cls, "host_cost", False)]
+        self.types = sorted(self.skip + self.costed)
+        self.gauge_provider = gauge_provider
 
     def _chunks(self, it):
         E = self.model.eletypes[it]
@@ -90,7 +96,14 @@ class HostLagrangians:
             E = self.model.eletypes[it]
             if getattr(E.cls, "once_off", False):
                 continue
+            gd = None
+            if it in self.costed:
+                if self.gauge_provider is None:
+                    raise MuscadeError(f"element type {it + 1} ({E.key[0]}) needs the device requestables")
+                gd = self.gauge_provider(it, state)
             for p, d, sl in self._chunks(it):
+                if gd is not None:
+                    p = dict(p, _gd=tuple(a[sl] for a in gd))
                 iX, iU, iA = d.X[sl] - 1, d.U[sl] - 1, d.A[sl] - 1
                 m, Nx, Nu, Na = iX.shape[0], iX.shape[1], iU.shape[1], iA.shape[1]
                 # partial layout and global dofs of each (class, derivative)
@@ -278,10 +291,17 @@ class DirectXUA(metaclass=_DirectXUAMeta):
 
         host = HostLagrangians(model, dis, OX, OU, IA)
         if batch_steps:
-            batched = [StepBatchAssembly(OX, OU, IA, model, dis, n, skip=tuple(host.types), device=device)
+            batched = [StepBatchAssembly(OX, OU, IA, model, dis, n, skip=tuple(host.skip), device=device)
                        for n in nstep]
+            provider = None
         else:
-            out = Assembly(OX, OU, IA, model, dis, device=device, skip=tuple(host.types))
+            out = Assembly(OX, OU, IA, model, dis, device=device, skip=tuple(host.skip))
+            provider = getattr(out, "gauge_derivatives", None)
+        if host.costed:
+            if provider is None:
+                from .results import ElementResults
+                provider = ElementResults(model, dis, device).gauge_derivatives
+            host.gauge_provider = provider
         per_step = 2 * nX + nU
         exp0 = np.cumsum([0] + [n * per_step for n in nstep])
         nv = int(exp0[-1]) + (nA if IA else 0)   # A-dofs are unknowns only when IA = 1

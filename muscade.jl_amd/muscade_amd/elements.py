@@ -8,7 +8,7 @@ residuals themselves are evaluated by the HIP kernels; `kind` names the kernel.
 import numpy as np
 
 from . import _lib
-from .hostcost import call_cost
+from .hostcost import call_cost, compose
 from .model import MuscadeError
 
 BEAM_FIELDS = ("EA", "EI₂", "EI₃", "GJ", "μ", "ι₁", "w", "Ca₁", "Cl₁", "Cq₁", "Ca₂", "Cl₂", "Cq₂", "Ca₃", "Cl₃", "Cq₃")
@@ -113,6 +113,43 @@ class StrainGaugeOnEulerBeam3D(EulerBeam3D):
         params["gauges"] = np.broadcast_to(np.stack([E, K1, K2, K3], axis=1), (nele, P.shape[1], 4)).copy()
         params["P"], params["D"] = P, D
         return ("StrainGaugeOnEulerBeam3D", P.shape[1]) + key, doflist, params
+
+
+class ElementCost(EulerBeam3D):
+    """ElementCost(nod; req, cost, costargs=(), ElementType, elementkwargs) — src/BasicElements.jl:88-132,
+    for ElementType = StrainGaugeOnEulerBeam3D: L = Λ·R(beam) + cost(eleres, t, costargs...),
+    with eleres the requested gauge requestables (εₐₓ, κ, ε). The wrapped beam's residual is
+    assembled on the device like any beam; the cost is evaluated on the host from the device's
+    values, gradients and Hessians of εₐₓ and κ (mh_beam_gauge_derivs), composed with the
+    second-order duals of the solver's partials (DirectXUA's Lagrangian path,
+    src/DirectXUA.jl:121-171). In SweepX the cost contributes nothing (it has no Λ)."""
+    host_cost = True
+
+    @classmethod
+    def construct(cls, model, nodes0, req, cost, ElementType, elementkwargs, costargs=()):
+        if ElementType is not StrainGaugeOnEulerBeam3D:
+            raise MuscadeError("ElementCost is built for StrainGaugeOnEulerBeam3D elements (the device requestables)")
+        key, doflist, params = ElementType.construct(model, nodes0, **elementkwargs)
+        name = getattr(cost, "__name__", type(cost).__name__)
+        params.update(req=dict(req), cost=cost, costargs=costargs)
+        return ("ElementCost", name) + key, doflist, params
+
+    @staticmethod
+    def lagrangian(p, Λ, X, U, A, t):
+        """The cost part; p["_gd"] = (values (m,4), gradients (m,4,12), Hessians (m,4,12,12)) of
+        (εₐₓ, κ₁, κ₂, κ₃) w.r.t. the element's X-dofs at this state."""
+        v, g, H = p["_gd"]
+        q = [compose(v[:, k], g[:, k], H[:, k], X[0]) for k in range(4)]
+        G = p["gauges"]                                   # m, ngauge, 4
+        eps = [((G[:, k, 0] * q[0] + G[:, k, 1] * q[1]) + G[:, k, 2] * q[2]) + G[:, k, 3] * q[3]
+               for k in range(G.shape[1])]
+        full = {"εₐₓ": q[0], "κ": q[1:], "ε": eps}
+        eleres = {}
+        for name in p["req"]:
+            if name not in full:
+                raise MuscadeError(f"{name} is not a requestable of StrainGaugeOnEulerBeam3D")
+            eleres[name] = full[name]
+        return call_cost(p["cost"], (eleres, t), p["costargs"])
 
 
 BAR_FIELDS = ("EA", "μ", "w", "Caₜ", "Clₜ", "Cqₜ", "Caₙ", "Clₙ", "Cqₙ")

@@ -121,6 +121,22 @@ class Dual2:
         return NotImplemented
 
 
+def compose(v, g, H, x):
+    """y = f(x) as a Dual2 from f's value v (m,), gradient g (m, k) and Hessian H (m, k, k)
+    over k inputs x (a list of Dual2 or arrays): the chain rule to second order,
+    ∇y = gᵀ∇x and ∇²y = ∇xᵀ H ∇x + Σ_k g_k ∇²x_k."""
+    dx = [xi for xi in x if isinstance(xi, Dual2)]
+    if not dx:
+        return np.asarray(v, dtype=float)
+    n = dx[0].g.shape[-1]
+    m = np.shape(v)[0]
+    G = np.stack([xi.g if isinstance(xi, Dual2) else np.zeros((m, n)) for xi in x], axis=1)          # m,k,n
+    Hx = np.stack([xi.h if isinstance(xi, Dual2) else np.zeros((m, n, n)) for xi in x], axis=1)      # m,k,n,n
+    yg = np.einsum("mk,mkn->mn", g, G)
+    yh = np.einsum("mkl,mkn,mlp->mnp", H, G, G) + np.einsum("mk,mknp->mnp", g, Hx)
+    return Dual2(np.asarray(v, dtype=float), yg, yh)
+
+
 def exp10(x):
     """10^x for floats and Dual2 (Muscade's exp10 in test cost functions)."""
     return np.exp(x * np.log(10.)) if isinstance(x, Dual2) else 10. ** x

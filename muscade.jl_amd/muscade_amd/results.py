@@ -84,6 +84,23 @@ class ElementResults:
         self.ctx.check(lib.mh_beam_results(h, self.groups[ieletyp], ND, ng, dptr(gauges), dptr(raw)), "beam results")
         return _split(raw, ND, ng)
 
+    def gauge_derivatives(self, ieletyp, state):
+        return gauge_derivatives_on(self.ctx, self.groups[ieletyp], state, self.model.eletypes[ieletyp].nele)
+
+
+def gauge_derivatives_on(ctx, group, state, nele):
+    """(values (nele,4), gradients (nele,4,12), Hessians (nele,4,12,12)) of εₐₓ, κ₁, κ₂, κ₃
+    w.r.t. the element X-dofs of a beam group at state.X[0] (mh_beam_gauge_derivs)."""
+    ctx.check(lib.mh_set_X(ctx.h, 0, dptr(as_f64(state.X[0]))), "set X")
+    raw = np.zeros((nele, 364))
+    ctx.check(lib.mh_beam_gauge_derivs(ctx.h, group, 0, None, dptr(raw)), "gauge derivatives")
+    H = np.zeros((nele, 4, 12, 12))
+    iu = np.triu_indices(12)
+    h78 = raw[:, 52:].reshape(nele, 4, 78)
+    H[:, :, iu[0], iu[1]] = h78
+    H[:, :, iu[1], iu[0]] = h78
+    return raw[:, :4], raw[:, 4:52].reshape(nele, 4, 12), H
+
 
 def getresult(state, req, ieletyp, device=0, _cache={}):
     """eleres = getresult(state, req, eletype) for all elements of type ieletyp (0-based)

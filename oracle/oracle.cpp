@@ -270,6 +270,31 @@ int orc_beam_results(const double* c1, const double* c2, const double* orient2, 
     return 0;
 }
 
+// εₐₓ = ε and κ = ♢κ with their first and second derivatives w.r.t. X₀, over the
+// restated kinematics on second-order nested duals (revariate{2}, Taylor.jl:143-151).
+// out (364): values (4), gradients (4×12), Hessians (4×78: upper triangle, row-major,
+// (0,0), (0,1) … (0,11), (1,1) …).
+int orc_beam_gauge2(const double* c1, const double* c2, const double* orient2, const double* mat16, const double* X,
+                    double* out) {
+    BeamCrossSection mat; std::memcpy(&mat, mat16, sizeof(mat));
+    EulerBeam3D o;
+    if (make_beam(c1, c2, orient2, mat, 0, &o)) return 1;
+    using T = Dual<12, Dual<12, double>>;
+    T X0[12];
+    for (int i = 0; i < 12; ++i) { X0[i].x = seed<12, double>(X[i], i); X0[i].dx[i] = Dual<12, double>(1.); }
+    Kin<T> k = kinematics<T>(o, X0);
+    const double f = 2. / o.L;
+    T q[4] = {k.eps, k.vl2[0] * f, k.vl2[2] * f, k.vl2[1] * (-f)};
+    for (int a = 0; a < 4; ++a) {
+        out[a] = q[a].x.x;
+        for (int i = 0; i < 12; ++i) out[4 + 12 * a + i] = q[a].dx[i].x;
+        int pr = 0;
+        for (int i = 0; i < 12; ++i)
+            for (int j = i; j < 12; ++j) out[52 + 78 * a + pr++] = q[a].dx[i].dx[j];
+    }
+    return 0;
+}
+
 // ε and ♢κ with their gradients w.r.t. X₀ (12): the strain-gauge requestables
 // (toolbox/StrainGaugeOnBeamElement.jl:70-76) differentiated as by
 // diffed_lagrangian{1} (Diagnostic.jl:851-911). Also R and the first-order

@@ -40,6 +40,7 @@ _SIG = {
     "orc_beam_residual_plain": (ctypes.c_int, [_D, _D, _D, _D, ctypes.c_int, _D, _D, _D, _D]),
     "orc_beam_gauge": (ctypes.c_int, [_D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
     "orc_beam_results": (ctypes.c_int, [_D, _D, _D, _D, ctypes.c_int, _D, _D, _D]),
+    "orc_beam_gauge2": (ctypes.c_int, [_D, _D, _D, _D, _D, _D]),
     "orc_number_dofs": (ctypes.c_int, [ctypes.c_int, _IP, _IP, _IP, ctypes.c_int, _IP, _IP, _IP, _IP, _IP, _IP, _IP,
                                        _I64, ctypes.c_int, _IP, _IP, _IP, _IP, _IP, _IP]),
     "orc_model_new": (_V, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
@@ -161,6 +162,23 @@ def beam_results(c1, c2, orient2, mat16, X, U=None):
                              dp(f64(U)) if U is not None else None, dp(out))
     assert r == 0
     return out
+
+
+def upper_to_full(h78):
+    """4×78 upper triangles (row-major) → 4×12×12 symmetric."""
+    H = np.zeros(h78.shape[:-1] + (12, 12))
+    iu = np.triu_indices(12)
+    H[..., iu[0], iu[1]] = h78
+    H[..., iu[1], iu[0]] = h78
+    return H
+
+
+def beam_gauge2(c1, c2, orient2, mat16, X):
+    """(εₐₓ, κ₁, κ₂, κ₃) with gradients (4×12) and Hessians (4×12×12) w.r.t. X₀."""
+    out = np.zeros(364)
+    r = lib.orc_beam_gauge2(dp(f64(c1)), dp(f64(c2)), dp(f64(orient2)), dp(f64(mat16)), dp(f64(X)), dp(out))
+    assert r == 0
+    return out[:4], out[4:52].reshape(4, 12), upper_to_full(out[52:].reshape(4, 78))
 
 
 def beam_gauge(c1, c2, orient2, mat16, X):

@@ -112,6 +112,20 @@ class OracleAssemblyDirect:
         self.nX, self.nU = model.ndof["X"], model.ndof["U"]
         self.nder = {"Λ": 1, "X": OX + 1, "U": OU + 1}
 
+    def gauge_derivatives(self, it, state):
+        """The oracle's (εₐₓ, κ) values, gradients and Hessians (orc_beam_gauge2) per element."""
+        E = self.model.eletypes[it]
+        xyz = self.model.coords()
+        nodes = E.cat_nodes()
+        o2 = np.concatenate([p["orient2"] for p in E.params])
+        mat = np.concatenate([p["mat"] for p in E.params])
+        idx = self.dis.dis[it].X - 1
+        v, g, H = np.zeros((E.nele, 4)), np.zeros((E.nele, 4, 12)), np.zeros((E.nele, 4, 12, 12))
+        for e in range(E.nele):
+            v[e], g[e], H[e] = orc.beam_gauge2(xyz[nodes[e, 0], :3], xyz[nodes[e, 1], :3], o2[e], mat[e],
+                                               state.X[0][idx[e]])
+        return v, g, H
+
     def assemble(self, state, Λ, mission="matrices", Δt=1.0, dbg=None):
         import scipy.sparse as sp
         om = oracle_model(self.model, self.dis, t=state.time, skip=self.skip, nA=0)

@@ -1,0 +1,139 @@
+"""ElementCost over StrainGaugeOnEulerBeam3D in DirectXUA's Lagrangian path (src/BasicElements.jl:
+117-132, src/DirectXUA.jl:121-171), host side against the oracle (OracleAssemblyDirect and
+orc_beam_gauge2 as the gauge-derivative provider, so no GPU):
+- the Lagrangian gradient Λᵀ∂R/∂X + ∂cost/∂X of the costed beam equals the golden ∇L of
+  TestBeamElementStrainGauge.jl:90-111;
+- its Hessian equals central differences of that gradient;
+- solve(DirectXUA{0,0,0}) of a load identification from gauge strains recovers the load
+  that produced them (SweepX{0} forward solve)."""
+import math
+
+import numpy as np
+import pytest
+
+import muscade_amd as mc
+from muscade_amd.directxua import HostLagrangians
+from tests import _oracle as orc
+from tests.helpers import OracleAssembly, OracleAssemblyDirect
+
+P = np.array([[0., .5, 0.], [0., 0, .5], [0., -.5, 0.], [0., 0, -.5], [0., .5, 0.]]).T
+D = np.array([[1., 0., 0.], [1., 0., 0.], [1., 0., 0.], [1., 0., 0.], [1 / math.sqrt(2), 0, 1 / math.sqrt(2)]]).T
+SIGMA = 15e-6
+
+
+def straincost(eleres, t):
+    em = np.array([math.cos(t), 0., -math.cos(t), 0., math.cos(t) / 2]) * 0.001
+    return sum((e - m) * (e - m) for e, m in zip(eleres["ε"], em)) / (2 * SIGMA ** 2)
+
+
+def costed_beam_model():
+    model = mc.Model("costed")
+    n1 = mc.addnode(model, [0., 0., 0.])
+    n2 = mc.addnode(model, [4., 0., 0.])
+    mat = mc.BeamCrossSection(EA=10., EI2=3., EI3=3., GJ=4., mu=1., iota1=1.0)
+    mc.addelement(model, mc.ElementCost, [n1, n2], req=mc.request("ε"), cost=straincost,
+                  ElementType=mc.StrainGaugeOnEulerBeam3D,
+                  elementkwargs=dict(P=P, D=D, elementkwargs=dict(mat=mat, orient2=np.array([0., 1., 0.]))))
+    return model
+
+
+def lagrangian_gradient(model, st, Assembly, provider):
+    out = Assembly(0, 0, 0, model, st.dis, skip=())
+    host = HostLagrangians(model, st.dis, 0, 0, 0, gauge_provider=provider(out))
+    L1d, L2d = out.assemble(st, st.Λ[0], "matrices", 1.0)
+    L1h, L2h = host.step(st)
+    gX = L1h["X"][0] + L2d[("Λ", "X")][0][0].T @ st.Λ[0]
+    HXX = np.zeros((12, 12))
+    for (a, i, b, j, r, c, v) in L2h:
+        if (a, i, b, j) == ("X", 0, "X", 0):
+            np.add.at(HXX, (r, c), v)
+    return gX, HXX
+
+
+GOLD = [277777.7777783019, 0.0, 1.1102230246251565e-16, 201441.02435832855, 2.7777777927777793e7,
+        -1.2430497627256343e6, -277777.7777783019, 0.0, -1.1102230246251565e-16, -76336.75341947998,
+        -2.7777777927777793e7, 1.2569502372743965e6]
+
+
+def state_of(model):
+    st = mc.initialize(model, time=0.)
+    idx = st.dis.dis[0].X[0] - 1
+    st.X[0][idx] = [0, 0, 0, 0, .1, 0, 0, 0, 0, 0, -.1, 0]
+    st.Λ[0][idx] = [0, 0, 0, 0, .1, 0, 0, 0, 0, 0, -.1, 0]
+    return st, idx
+
+
+def test_costed_beam_gradient_golden():   # TestBeamElementStrainGauge.jl:90-111
+    model = costed_beam_model()
+    st, idx = state_of(model)
+    gX, _ = lagrangian_gradient(model, st, OracleAssemblyDirect, lambda out: out.gauge_derivatives)
+    g = gX[idx]
+    assert np.linalg.norm(g - GOLD) <= math.sqrt(np.finfo(float).eps) * np.linalg.norm(GOLD)
+
+
+def test_costed_beam_hessian_is_gradient_jacobian():
+    model = costed_beam_model()
+    st, idx = state_of(model)
+    rng = np.random.default_rng(2)
+    st.X[0][idx] += rng.normal(size=12) * 0.01
+    _, H = lagrangian_gradient(model, st, OracleAssemblyDirect, lambda out: out.gauge_derivatives)
+    h = 1e-7
+    for k in range(12):
+        sp_, sm = st.copy(), st.copy()
+        sp_.X[0][idx[k]] += h
+        sm.X[0][idx[k]] -= h
+        # the cost's gradient only (Λ = 0): ∂cost/∂X
+        for s in (sp_, sm):
+            s.Λ[0][:] = 0.
+        gp, _ = lagrangian_gradient(model, sp_, OracleAssemblyDirect, lambda out: out.gauge_derivatives)
+        gm, _ = lagrangian_gradient(model, sm, OracleAssemblyDirect, lambda out: out.gauge_derivatives)
+        fd = (gp - gm)[idx] / (2 * h)
+        assert np.allclose(H[idx, idx[k]], fd, rtol=2e-5, atol=1e-3 * np.abs(fd).max()), k
+
+
+def gauge_cantilever(F=None, target=None, nel=4, cu=1e-12):
+    from muscade_amd.workloads import base_section
+    model = mc.Model("gauged")
+    x = np.linspace(0., 4., nel + 1)
+    mc.addnode(model, np.stack([x, 0 * x, 0 * x], axis=1))
+    sec = base_section()
+    o2 = np.array([0., 1., 0.])
+    Pg = np.array([[0., 0., .05], [0., 0., -.05], [0., .05, 0.]]).T
+    Dg = np.array([[1., 0., 0.], [1., 0., 0.], [1., 0., 0.]]).T
+    kw = dict(P=Pg, D=Dg, elementkwargs=dict(mat=sec, orient2=o2))
+    if target is None:
+        mc.addelement(model, mc.StrainGaugeOnEulerBeam3D, [1, 2], **kw)
+    else:
+        def cost(eleres, t, target=target):
+            return sum((e - m) * (e - m) for e, m in zip(eleres["ε"], target)) * 1e12
+        mc.addelement(model, mc.ElementCost, [1, 2], req=mc.request("ε"), cost=cost,
+                      ElementType=mc.StrainGaugeOnEulerBeam3D, elementkwargs=kw)
+    mc.addelement(model, mc.EulerBeam3D, np.stack([np.arange(2, nel + 1), np.arange(3, nel + 2)], axis=1), mat=sec,
+                  orient2=o2)
+    for f in ("t1", "t2", "t3", "r1", "r2", "r3"):
+        mc.addelement(model, mc.Hold, [1], field=f)
+    if F is not None:
+        mc.addelement(model, mc.DofLoad, [nel + 1], field="t3", value=lambda t, F=F: F)
+    else:
+        mc.addelement(model, mc.SingleUdof, [nel + 1], Xfield="t3", Ufield="ut3", cost=lambda u, t: cu * u * u)
+    return model
+
+
+def test_load_identification_from_gauges():
+    F = 3e4
+    fwd = gauge_cantilever(F=F)
+    s = mc.solve(mc.SweepX[0], initialstate=mc.initialize(fwd, time=0.), time=[1.], maxΔx=1e-9, verbose=False,
+                 catcherror=False, Assembly=OracleAssembly)[-1]
+    E = fwd.eletypes[0]
+    xyz = fwd.coords()
+    idx = s.dis.dis[0].X[0] - 1
+    v, _, _ = orc.beam_gauge2(xyz[0], xyz[1], [0., 1., 0.], E.params[0]["mat"][0], s.X[0][idx])
+    G = E.params[0]["gauges"][0]
+    target = G[:, 0] * v[0] + G[:, 1] * v[1] + G[:, 2] * v[2] + G[:, 3] * v[3]
+    assert np.abs(target).max() > 1e-5
+    inv = gauge_cantilever(target=target)
+    out = mc.solve(mc.DirectXUA[0, 0, 0], initialstate=[mc.initialize(inv, time=0.)], time=[np.array([1.])],
+                   maxΔx=1e-9, maxΔλ=1e-3, maxΔu=1e-6, maxiter=30, verbose=False, catcherror=False,
+                   Assembly=OracleAssemblyDirect)
+    u = out[0][0].U[0][0]
+    assert abs(u - F) <= 1e-6 * F, (u, F)

@@ -132,3 +132,26 @@ def test_roll_acceleration_couple():
     r = split(orc.beam_results(C1, C2, O2, M, [rot, np.zeros(12), acc]), 3)
     for d in r["gp"]:
         assert np.allclose(d["mₑ"], M[5] * α * t, rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_gauge_second_derivatives(seed):
+    """orc_beam_gauge2: values and gradients equal the pinned first-order requestables
+    (orc_beam_gauge, checked against the costedbeam ∇L golden), Hessians equal central
+    differences of those gradients and are symmetric."""
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=12) * 0.2
+    if seed == 0:
+        X[[3, 4, 5, 9, 10, 11]] = 0.   # zero rotations (the series branches)
+    v, g, H = orc.beam_gauge2(C1, C2, O2, M, X)
+    _, _, eps, deps, kap, dkap = orc.beam_gauge(C1, C2, O2, M, X)
+    assert np.allclose(v, [eps, *kap], rtol=1e-14, atol=1e-15)
+    assert np.allclose(g, np.vstack([deps, dkap]), rtol=1e-13, atol=1e-14)
+    assert np.abs(H - np.swapaxes(H, 1, 2)).max() == 0.
+    h = 1e-6
+    for i in range(12):
+        e = np.zeros(12); e[i] = h
+        _, _, _, dp, _, kp = orc.beam_gauge(C1, C2, O2, M, X + e)
+        _, _, _, dm, _, km = orc.beam_gauge(C1, C2, O2, M, X - e)
+        fd = (np.vstack([dp, kp]) - np.vstack([dm, km])) / (2 * h)
+        assert np.allclose(H[:, i, :], fd, rtol=1e-6, atol=1e-8), i
