@@ -45,7 +45,10 @@ enum {
     MH_KIND_ZERO = 5,    /* Lagrangian-only element (DofCost, BasicElements.jl:32-50): in SweepX its
                             X-dof pairs enter the pattern and contribute structural zeros             */
     MH_KIND_BAR = 6,     /* Bar3D{Mat,false} with AxisymmetricBarCrossSection (toolbox/BarElement.jl:89-202) */
-    MH_KIND_BAR_U = 7    /* Bar3D{Mat,true}: 3 U-dofs, load per length (BarElement.jl:115-118,192)     */
+    MH_KIND_BAR_U = 7,   /* Bar3D{Mat,true}: 3 U-dofs, load per length (BarElement.jl:115-118,192)     */
+    MH_KIND_HOST = 8     /* residual evaluated by the host with user functions (DofConstraint{:X} with a
+                            gap Functor, BasicElements.jl:426-438): the host hands R and ∂R/∂X to
+                            mh_set_host_blocks before each assembly; the device scatters them        */
 };
 
 typedef struct mh_ctx mh_ctx;
@@ -145,6 +148,12 @@ int mh_device_outputs(mh_ctx* ctx, double** d_rhs, double** d_nzval);
 /* Per-element results of the last assembly for one group (for element-level
  * checks, like diffed_residual): R (nX×nele) and ∂R/∂X·scale (nX²×nele). */
 int mh_get_element_blocks(mh_ctx* ctx, int igroup, double* R, double* K);
+
+/* Element results of a MH_KIND_HOST group for the next SweepX assemblies: R (nX×nele) and
+ * ∂R/∂X (nX²×nele, entry i+nX·j of an element's block), unscaled; the library applies the
+ * group's dof scales (R_i·s_i, K_ij·s_i·s_j, the convention of the device kinds). The
+ * constrained element is static: its C and M blocks and its ∂R/∂δr (:step) are zero. */
+int mh_set_host_blocks(mh_ctx* ctx, int igroup, const double* R, const double* K);
 
 /* ---- element results: getresult / @espy of EulerBeam3D (src/Output.jl:131-181,
  *      toolbox/BeamElement.jl:28-64,151-174) ---------------------------------------

@@ -324,7 +324,7 @@ __device__ __forceinline__ void point_element(const PointArgs& a, int64_t e) {
         a.R[e] = -F * a.s[0];
         a.K[e] = 0.;   // 1×1 block
         if (F != F) atomicMin(a.flag, a.gtag | (unsigned long long)e);
-    } else {
+    } else if (a.kind != MH_KIND_HOST) {   // host groups: R and K come from mh_set_host_blocks
         for (int i = 0; i < nX; ++i) a.R[i * n + e] = 0.;
         for (int i = 0; i < nX * nX; ++i) a.K[(int64_t)nX * nX * e + i] = 0.;
     }
@@ -1642,7 +1642,8 @@ int mh_group_add(mh_ctx* c, int kind, int64_t nele, int nX, const int64_t* idxX,
     }
     if (kind == MH_KIND_HOLD && nX != 2) { c->err = "Hold group needs 2 X dofs"; return MH_ERR_ARG; }
     if (kind == MH_KIND_DOFLOAD && nX != 1) { c->err = "DofLoad group needs 1 X dof"; return MH_ERR_ARG; }
-    if (kind < MH_KIND_BEAM || kind > MH_KIND_BAR_U) { c->err = "unknown element kind"; return MH_ERR_ARG; }
+    if (kind < MH_KIND_BEAM || kind > MH_KIND_HOST) { c->err = "unknown element kind"; return MH_ERR_ARG; }
+    if (kind == MH_KIND_HOST && (nX < 1 || nX > 12)) { c->err = "host-evaluated group needs 1..12 X dofs"; return MH_ERR_ARG; }
     Group g;
     g.kind = kind; g.nele = nele; g.nX = nX; g.nU = (beam || bar) ? nU : 0;
     for (int k = 0; k < 12; ++k) g.scaleX[k] = (scaleX && k < nX) ? scaleX[k] : 1.;
@@ -2066,6 +2067,8 @@ int mh_set_lambda(mh_ctx* c, const double* lam) {
 int mh_vjp_lagrangian(mh_ctx* c, int OX, mh_err* err) {
     if (err) { err->code = MH_OK; err->igroup = -1; err->iele = -1; }
     if (!c || !c->prepared || !c->d_lam || OX < 0 || OX > 2) return MH_ERR_ARG;
+    for (auto& g : c->groups)
+        if (g.kind == MH_KIND_HOST && g.nele) { c->err = "the adjoint does not cover host-evaluated elements"; return MH_ERR_ARG; }
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const int nd = OX + 1;
@@ -2247,8 +2250,8 @@ int mh_prepare_direct(mh_ctx* c, int OX, int OU, int IA, int64_t* nnz4) {
     if (c->nA) { c->err = "A-dofs are not evaluated on the device in AssemblyDirect"; return MH_ERR_ARG; }
     int64_t nR = 0, nKd = 0, nKU = 0, nKc = 0;
     for (auto& g : c->groups) {
-        if (g.kind == MH_KIND_ZERO) {
-            c->err = "Lagrangian elements with user cost functions (DofCost) are not evaluated on the device in AssemblyDirect";
+        if (g.kind == MH_KIND_ZERO || g.kind == MH_KIND_HOST) {
+            c->err = "elements with user functions (DofCost, DofConstraint) are not evaluated on the device in AssemblyDirect";
             return MH_ERR_ARG;
         }
         g.rbase = nR; g.kdbase = nKd; g.kubase = nKU; g.kcbase = nKc;
@@ -2688,6 +2691,33 @@ int mh_beam_gauge_derivs(mh_ctx* c, int igroup, int64_t nsel, const int64_t* sel
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     dfree(d_out); dfree(d_sel);
     if (e != hipSuccess) { c->err = std::string("gauge derivatives: ") + hipGetErrorString(e); return MH_ERR_HIP; }
+    return MH_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int mh_set_host_blocks(mh_ctx* c, int igroup, const double* R, const double* K) {
+    if (!c || !c->prepared || igroup < 0 || igroup >= (int)c->groups.size() || !R || !K) return MH_ERR_ARG;
+    const Group& g = c->groups[igroup];
+    if (g.kind != MH_KIND_HOST) { c->err = "group is not host-evaluated"; return MH_ERR_ARG; }
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = g.nele;
+    const int nX = g.nX;
+    if (n == 0) return MH_OK;
+    std::vector<double> r((size_t)nX * n), k((size_t)nX * nX * n);
+    for (int64_t e = 0; e < n; ++e)
+        for (int i = 0; i < nX; ++i) {
+            r[(size_t)i * n + e] = R[(size_t)nX * e + i] * g.scaleX[i];
+            for (int j = 0; j < nX; ++j) {
+                const size_t q = (size_t)nX * nX * e + i + (size_t)nX * j;
+                k[q] = K[q] * g.scaleX[i] * g.scaleX[j];
+            }
+        }
+    HIPCHK(c, hipMemcpyAsync(c->d_Rbuf + g.rbase, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_Kbuf + g.kbase, k.data(), sizeof(double) * k.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return MH_OK;
 }
 

@@ -13,6 +13,7 @@ LIBPATH = os.path.join(_HERE, "libmuscade_hip.so")
 
 MH_OK, MH_ERR_ARG, MH_ERR_HIP, MH_ERR_NAN, MH_ERR_ORIENT, MH_ERR_SIZE, MH_ERR_SOLVE = 0, 1, 2, 3, 4, 5, 6
 MH_KIND_BEAM, MH_KIND_BEAM_U, MH_KIND_HOLD, MH_KIND_DOFLOAD, MH_KIND_ZERO, MH_KIND_BAR, MH_KIND_BAR_U = 1, 2, 3, 4, 5, 6, 7
+MH_KIND_HOST = 8
 
 # every function of include/muscade_hip.h, with its ctypes signature
 _P = ctypes.c_void_p
@@ -65,6 +66,7 @@ SIGNATURES = {
     "mh_newton_update": (_I, [_P, _I, _I, _D, _D]),
     "mh_get_X": (_I, [_P, _I, _D]),
     "mh_beam_results": (_I, [_P, _I, _I, _I, _D, _D]),
+    "mh_set_host_blocks": (_I, [_P, _I, _D, _D]),
     "mh_beam_gauge_derivs": (_I, [_P, _I, _L, _I64, _D]),
     "mh_last_timings": (_I, [_P, _D]),
     "mh_calib_copy": (_I, [_P, _L, _D]),

@@ -111,6 +111,10 @@ class AssemblySweepX:
         self.c = Newmarkβcoefficients(OX)
         self.Lλ = np.zeros(self.nX)
         self.nzval = np.zeros(self.nnz)
+        # element types evaluated on the host (user functions): their blocks are handed over
+        # before each assembly (mh_set_host_blocks)
+        self.host_types = [it for it, E in enumerate(model.eletypes)
+                           if E.cls.kind == _lib.MH_KIND_HOST and E.nele and dis.dis[it].X.shape[1]]
         self._indptr0 = self.colptr - 1
         self._indices0 = self.rowval - 1
 
@@ -145,6 +149,8 @@ class AssemblySweepX:
         Newton loop, muscade_amd.chainsolve)."""
         if upload:
             self.set_state(state, state.time)
+        if self.host_types:
+            self._host_blocks(state, upload, dbg)
         err = _lib.mh_err()
         m = 0 if mission == "step" else 1
         fn = lib.mh_assemble_sweepx_graph if self.graph else lib.mh_assemble_sweepx
@@ -158,6 +164,32 @@ class AssemblySweepX:
             self.ctx.check(lib.mh_get_rhs(self.ctx.h, dptr(self.Lλ)), "rhs")
             self.ctx.check(lib.mh_get_nzval(self.ctx.h, dptr(self.nzval)), "nzval")
         return self.Lλ, self.nzval
+
+    def _host_blocks(self, state, upload, dbg):
+        """R and ∂R/∂X of the host-evaluated element types at the current X (SweepX.jl:45-96)."""
+        if upload:
+            X0 = state.X[0]
+        else:
+            X0 = np.zeros(self.nX)
+            self.ctx.check(lib.mh_get_X(self.ctx.h, 0, dptr(X0)), "X")
+        SP = getattr(state, "SP", None)
+        γ = float(SP.get("γ", 0.)) if isinstance(SP, dict) else 0.   # default{:γ}(SP, 0.)
+        for it in self.host_types:
+            E = self.model.eletypes[it]
+            d = self.dis.dis[it]
+            Rs, Ks, e0 = [], [], 0
+            for nodes, p in zip(E.nodes, E.params):
+                m = nodes.shape[0]
+                R, K = E.cls.residual_blocks(p, X0[d.X[e0:e0 + m] - 1], state.time, γ)
+                Rs.append(R); Ks.append(K)
+                e0 += m
+            R, K = np.concatenate(Rs), np.concatenate(Ks)
+            bad = ~(np.isfinite(R).all(axis=1) & np.isfinite(K).reshape(K.shape[0], -1).all(axis=1))
+            if bad.any():
+                raise MuscadeError(f"residual({E.key[0]},...) returned NaN in R, FB or derivatives",
+                                   dict(dbg or {}, ieletyp=it + 1, iele=int(np.nonzero(bad)[0][0]) + 1, t=state.time))
+            self.ctx.check(lib.mh_set_host_blocks(self.ctx.h, self.groups[it], dptr(as_f64(R)),
+                                                  dptr(as_f64(K.transpose(0, 2, 1)))), "host blocks")
 
     def graph_captures(self):
         n = ctypes.c_int64()

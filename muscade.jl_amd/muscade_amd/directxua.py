@@ -78,6 +78,10 @@ class HostLagrangians:
         self.types = sorted(self.skip + self.costed)
         self.gauge_provider = gauge_provider
 
+    def _once_off(self, E):
+        f = getattr(E.cls, "once_off_of", None)
+        return bool(getattr(E.cls, "once_off", False) or (f is not None and E.nele and f(E.params[0])))
+
     def _chunks(self, it):
         E = self.model.eletypes[it]
         d = self.dis.dis[it]
@@ -94,7 +98,7 @@ class HostLagrangians:
         L2 = []
         for it in self.types:
             E = self.model.eletypes[it]
-            if getattr(E.cls, "once_off", False):
+            if self._once_off(E):
                 continue
             gd = None
             if it in self.costed:
@@ -124,7 +128,10 @@ class HostLagrangians:
                 X = [seeded("X", der, iX, state.X[der][iX], d.scaleX) for der in range(self.OX + 1)]
                 U = [seeded("U", der, iU, state.U[der][iU], d.scaleU) for der in range(self.OU + 1)]
                 A = seeded("A", 0, iA, state.A[iA], d.scaleA)
-                L = E.cls.lagrangian(p, Λ, X, U, A, state.time)
+                if getattr(E.cls, "uses_SP", False):
+                    L = E.cls.lagrangian(p, Λ, X, U, A, state.time, SP=state.SP)
+                else:
+                    L = E.cls.lagrangian(p, Λ, X, U, A, state.time)
                 if not isinstance(L, Dual2):
                     continue
                 if not np.all(np.isfinite(L.g)) or not np.all(np.isfinite(L.h)):
@@ -150,13 +157,13 @@ class HostLagrangians:
         pieces = []
         for it in self.types:
             E = self.model.eletypes[it]
-            if not getattr(E.cls, "once_off", False):
+            if not self._once_off(E):
                 continue
             for p, d, sl in self._chunks(it):
                 iA = d.A[sl] - 1
                 m, Na = iA.shape
                 A = [Dual2.seed(state.A[iA[:, k]], k, Na) for k in range(Na)]
-                C = E.cls.acost(p, A)
+                C = E.cls.acost(p, A, SP=state.SP) if getattr(E.cls, "uses_SP", False) else E.cls.acost(p, A)
                 if not isinstance(C, Dual2):
                     continue
                 np.add.at(gA, iA, C.g)

@@ -8,7 +8,7 @@ residuals themselves are evaluated by the HIP kernels; `kind` names the kernel.
 import numpy as np
 
 from . import _lib
-from .hostcost import call_cost, compose
+from .hostcost import KKT, S, Dual2, call_cost, compose
 from .model import MuscadeError
 
 BEAM_FIELDS = ("EA", "EI₂", "EI₃", "GJ", "μ", "ι₁", "w", "Ca₁", "Cl₁", "Cq₁", "Ca₂", "Cl₂", "Cq₂", "Ca₃", "Cl₃", "Cq₃")
@@ -214,6 +214,136 @@ class Hold:
         λf = λfield if λfield is not None else "λ" + field
         doflist = ((1, 1), ("X", "X"), (field, λf))
         return ("DofConstraint", "X", (field,), λf, "equal"), doflist, {}
+
+
+def equal(t):
+    """mode=equal (src/BasicElements.jl:322)."""
+    return "equal"
+
+
+def positive(t):
+    """mode=positive (src/BasicElements.jl:333)."""
+    return "positive"
+
+
+def off(t):
+    """mode=off (src/BasicElements.jl:311)."""
+    return "off"
+
+
+class DofConstraint:
+    """DofConstraint(nod; xinod, xfield, uinod, ufield, ainod, afield, λinod, λclass, λfield, gap, gargs, mode)
+    — src/BasicElements.jl:349-460. Gap g ≥ 0, multiplier λ ≥ 0; mode(t) ∈ equal, positive, off.
+      λclass "X": residual R = (−∂g/∂x·λ, −g | −S(λ,g,γ) | −λ for off), gap(x, t, gargs...);
+      λclass "U": L = −g·λ | −KKT(λ,g,γ) | −½λ², gap(x, u, a, t, gargs...), λ a U-dof;
+      λclass "A": the same over A-dofs only, gap(a, gargs...), once-off.
+    The gap is a user function, so the element is evaluated on the host: in SweepX as a
+    host-evaluated group (R and ∂R/∂X handed to the device scatter, MH_KIND_HOST), in DirectXUA
+    as a host Lagrangian (the X class by the residual path: R, ∂R/∂X, no X-X block, like the
+    device kinds). γ comes from state.SP (default 0)."""
+    kind = _lib.MH_KIND_HOST
+    host = True
+    uses_SP = True
+
+    @classmethod
+    def construct(cls, model, nodes0, λinod, λclass, λfield, gap, mode, xinod=(), xfield=(), uinod=(), ufield=(),
+                  ainod=(), afield=(), gargs=()):
+        xinod, xfield, uinod, ufield = tuple(xinod), tuple(xfield), tuple(uinod), tuple(ufield)
+        ainod, afield = tuple(ainod), tuple(afield)
+        Nx, Nu, Na = len(xinod), len(uinod), len(ainod)
+        if λclass not in ("X", "U", "A"):
+            raise MuscadeError("λclass must be :X, :U or :A")
+        if λclass == "X" and (Nu > 0 or Na > 0):
+            raise MuscadeError("Constraints with λclass=:X must have zero U-dofs and zero A-dofs")
+        if λclass == "A" and (Nx > 0 or Nu > 0):
+            raise MuscadeError("Constraints with λclass=:A must have zero X-dofs and zero U-dofs")
+        inod = xinod + uinod + ainod + (λinod,)
+        dclass = ("X",) * Nx + ("U",) * Nu + ("A",) * Na + (λclass,)
+        field = xfield + ufield + afield + (λfield,)
+        name = getattr(gap, "__name__", type(gap).__name__)
+        key = ("DofConstraint", λclass, xinod, xfield, uinod, ufield, ainod, afield, λinod, λfield, name,
+               getattr(mode, "__name__", str(mode)))
+        return key, (inod, dclass, field), {"gap": gap, "gargs": tuple(gargs), "mode": mode, "λclass": λclass,
+                                            "N": (Nx, Nu, Na)}
+
+    @staticmethod
+    def once_off_of(p):
+        return p["λclass"] == "A"
+
+    @staticmethod
+    def residual_blocks(p, Xv, t, γ=0.):
+        """SweepX: R (m, nX) and ∂R/∂X (m, nX, nX) over the element's X-dofs, from their values
+        Xv (m, nX). X class: x then λ; U class: structural zeros (its X-dofs carry no residual)."""
+        mE, nXe = Xv.shape
+        if p["λclass"] != "X":
+            return np.zeros((mE, nXe)), np.zeros((mE, nXe, nXe))
+        Nx = p["N"][0]
+        seeds = [Dual2.seed(Xv[:, k], k, Nx) for k in range(Nx)]
+        mode = p["mode"](t)
+        g = p["gap"](seeds, t, *p["gargs"])
+        if not isinstance(g, Dual2):
+            g = Dual2(np.broadcast_to(np.asarray(g, float), (mE,)).copy(), np.zeros((mE, Nx)), np.zeros((mE, Nx, Nx)))
+        lam = Xv[:, Nx]
+        R = np.zeros((mE, Nx + 1))
+        K = np.zeros((mE, Nx + 1, Nx + 1))
+        if mode == "off":
+            R[:, Nx] = -lam
+            K[:, Nx, Nx] = -1.
+            return R, K
+        R[:, :Nx] = -g.g * lam[:, None]
+        K[:, :Nx, :Nx] = -g.h * lam[:, None, None]
+        K[:, :Nx, Nx] = -g.g
+        if mode == "equal":
+            R[:, Nx] = -g.v
+            K[:, Nx, :Nx] = -g.g
+        elif mode == "positive":
+            R[:, Nx] = -S(lam, g.v, γ)
+            K[:, Nx, :Nx] = -lam[:, None] * g.g
+            K[:, Nx, Nx] = -g.v
+        else:
+            raise MuscadeError(f"unknown constraint mode {mode!r}")
+        return R, K
+
+    @staticmethod
+    def lagrangian(p, Λ, X, U, A, t, SP=None):
+        γ = float((SP or {}).get("γ", 0.))
+        Nx, Nu, Na = p["N"]
+        if p["λclass"] == "X":
+            # residual path (src/DirectXUA.jl:85-120): L = Λ·R with R carrying ∂R/∂X but no
+            # second derivatives, so L2 has the (Λ, X) blocks and no (X, X) block
+            xs = X[0]
+            vals = np.stack([v.v if isinstance(v, Dual2) else np.asarray(v, float) for v in xs], axis=1)
+            R, K = DofConstraint.residual_blocks(p, vals, t, γ)
+            L = 0.
+            for k in range(Nx + 1):
+                Rk = compose(R[:, k], K[:, k, :], np.zeros((R.shape[0], Nx + 1, Nx + 1)), xs)
+                L = L + Λ[k] * Rk
+            return L
+        mode = p["mode"](t)
+        x = X[0][:Nx]
+        u = U[0][:Nu]
+        a = A[:Na] if Na else []
+        lam = U[0][Nu]
+        g = call_cost(p["gap"], (x, u, a, t), p["gargs"])
+        return DofConstraint._mode_lagrangian(mode, lam, g, γ)
+
+    @staticmethod
+    def _mode_lagrangian(mode, lam, g, γ):
+        if mode == "equal":
+            return -(g * lam)
+        if mode == "positive":
+            return -KKT(lam, g, γ)
+        if mode == "off":
+            return -0.5 * (lam * lam)
+        raise MuscadeError(f"unknown constraint mode {mode!r}")
+
+    @staticmethod
+    def acost(p, A, SP=None):
+        """λclass A, once-off: L = −g·λ | −KKT(λ,g,γ) | −½λ² with gap(a, gargs...)."""
+        γ = float((SP or {}).get("γ", 0.))
+        Na = p["N"][2]
+        g = call_cost(p["gap"], (A[:Na],), p["gargs"])
+        return DofConstraint._mode_lagrangian(p["mode"](0.), A[Na], g, γ)
 
 
 class DofLoad:

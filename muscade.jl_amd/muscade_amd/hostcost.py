@@ -137,6 +137,34 @@ def compose(v, g, H, x):
     return Dual2(np.asarray(v, dtype=float), yg, yh)
 
 
+def S(λ, g, γ):
+    """Complementary slackness g·λ − γ (src/BasicElements.jl:289)."""
+    return g * λ - γ
+
+
+def KKT(λ, g, γ):
+    """The pseudo-potential of src/BasicElements.jl:291-307: value 0, "gradient"
+    λ·∇g + S(λ, g, γ)·∇λ, and as second derivatives the Jacobian of that gradient,
+    H_ij = ∂_i(λ∇g + S∇λ)/∂p_j over the solver's nested partials (not symmetric)."""
+    lg, gg = isinstance(λ, Dual2), isinstance(g, Dual2)
+    if not lg and not gg:
+        return 0.
+    ref = λ if lg else g
+    m, n = ref.g.shape
+    lv = λ.v if lg else np.broadcast_to(np.asarray(λ, float), (m,))
+    gv = g.v if gg else np.broadcast_to(np.asarray(g, float), (m,))
+    lgr = λ.g if lg else np.zeros((m, n))
+    ggr = g.g if gg else np.zeros((m, n))
+    lh = λ.h if lg else np.zeros((m, n, n))
+    gh = g.h if gg else np.zeros((m, n, n))
+    s = S(lv, gv, γ)
+    grad = lv[:, None] * ggr + s[:, None] * lgr
+    dS = gv[:, None] * lgr + lv[:, None] * ggr
+    H = (ggr[:, :, None] * lgr[:, None, :] + lv[:, None, None] * gh + lgr[:, :, None] * dS[:, None, :]
+         + s[:, None, None] * lh)
+    return Dual2(np.zeros(m), grad, H)
+
+
 def exp10(x):
     """10^x for floats and Dual2 (Muscade's exp10 in test cost functions)."""
     return np.exp(x * np.log(10.)) if isinstance(x, Dual2) else 10. ** x
