@@ -121,24 +121,32 @@ class ElementCost(EulerBeam3D):
     with eleres the requested gauge requestables (εₐₓ, κ, ε). The wrapped beam's residual is
     assembled on the device like any beam; the cost is evaluated on the host from the device's
     values, gradients and Hessians of εₐₓ and κ (mh_beam_gauge_derivs), composed with the
-    second-order duals of the solver's partials (DirectXUA's Lagrangian path,
-    src/DirectXUA.jl:121-171). In SweepX the cost contributes nothing (it has no Λ)."""
+    second-order duals of the solver's partials. In SweepX the cost contributes nothing (it
+    has no Λ).
+
+    DirectXUA evaluates ElementCost with its accelerator (src/DirectXUA.jl:180-199): the
+    wrapped element's requestables are revariated to first order only, so the cost's Hessian
+    is Jᵀ·∂²cost·J (Gauss-Newton, without Σ ∂cost/∂ε·∂²ε/∂X²). That is the default here, so
+    that the Newton iterates are the reference's; exact=True adds the second derivatives of
+    the requestables (exact Newton; the same optimum, and the gradient is the same)."""
     host_cost = True
 
     @classmethod
-    def construct(cls, model, nodes0, req, cost, ElementType, elementkwargs, costargs=()):
+    def construct(cls, model, nodes0, req, cost, ElementType, elementkwargs, costargs=(), exact=False):
         if ElementType is not StrainGaugeOnEulerBeam3D:
             raise MuscadeError("ElementCost is built for StrainGaugeOnEulerBeam3D elements (the device requestables)")
         key, doflist, params = ElementType.construct(model, nodes0, **elementkwargs)
         name = getattr(cost, "__name__", type(cost).__name__)
-        params.update(req=dict(req), cost=cost, costargs=costargs)
-        return ("ElementCost", name) + key, doflist, params
+        params.update(req=dict(req), cost=cost, costargs=costargs, exact=bool(exact))
+        return ("ElementCost", name, bool(exact)) + key, doflist, params
 
     @staticmethod
     def lagrangian(p, Λ, X, U, A, t):
         """The cost part; p["_gd"] = (values (m,4), gradients (m,4,12), Hessians (m,4,12,12)) of
         (εₐₓ, κ₁, κ₂, κ₃) w.r.t. the element's X-dofs at this state."""
         v, g, H = p["_gd"]
+        if not p.get("exact", False):
+            H = np.zeros_like(H)
         q = [compose(v[:, k], g[:, k], H[:, k], X[0]) for k in range(4)]
         G = p["gauges"]                                   # m, ngauge, 4
         eps = [((G[:, k, 0] * q[0] + G[:, k, 1] * q[1]) + G[:, k, 2] * q[2]) + G[:, k, 3] * q[3]

@@ -3,7 +3,8 @@
 orc_beam_gauge2 as the gauge-derivative provider, so no GPU):
 - the Lagrangian gradient Λᵀ∂R/∂X + ∂cost/∂X of the costed beam equals the golden ∇L of
   TestBeamElementStrainGauge.jl:90-111;
-- its Hessian equals central differences of that gradient;
+- with exact=True its Hessian equals central differences of that gradient; by default it is
+  the accelerator's Gauss-Newton form (src/DirectXUA.jl:180-199), Jᵀ·∂²cost·J;
 - solve(DirectXUA{0,0,0}) of a load identification from gauge strains recovers the load
   that produced them (SweepX{0} forward solve)."""
 import math
@@ -26,13 +27,13 @@ def straincost(eleres, t):
     return sum((e - m) * (e - m) for e, m in zip(eleres["ε"], em)) / (2 * SIGMA ** 2)
 
 
-def costed_beam_model():
+def costed_beam_model(exact=False):
     model = mc.Model("costed")
     n1 = mc.addnode(model, [0., 0., 0.])
     n2 = mc.addnode(model, [4., 0., 0.])
     mat = mc.BeamCrossSection(EA=10., EI2=3., EI3=3., GJ=4., mu=1., iota1=1.0)
     mc.addelement(model, mc.ElementCost, [n1, n2], req=mc.request("ε"), cost=straincost,
-                  ElementType=mc.StrainGaugeOnEulerBeam3D,
+                  ElementType=mc.StrainGaugeOnEulerBeam3D, exact=exact,
                   elementkwargs=dict(P=P, D=D, elementkwargs=dict(mat=mat, orient2=np.array([0., 1., 0.]))))
     return model
 
@@ -72,11 +73,20 @@ def test_costed_beam_gradient_golden():   # TestBeamElementStrainGauge.jl:90-111
 
 
 def test_costed_beam_hessian_is_gradient_jacobian():
-    model = costed_beam_model()
+    model = costed_beam_model(exact=True)
     st, idx = state_of(model)
     rng = np.random.default_rng(2)
     st.X[0][idx] += rng.normal(size=12) * 0.01
     _, H = lagrangian_gradient(model, st, OracleAssemblyDirect, lambda out: out.gauge_derivatives)
+    # the accelerator's (default) Hessian: Jᵀ ∂²cost J with J the gauge strains' Jacobian
+    mg = costed_beam_model()
+    _, Hg = lagrangian_gradient(mg, st, OracleAssemblyDirect, lambda out: out.gauge_derivatives)
+    E = mg.eletypes[0]
+    xyz = mg.coords()
+    v, gr, _ = orc.beam_gauge2(xyz[0], xyz[1], [0., 1., 0.], E.params[0]["mat"][0], st.X[0][idx])
+    Gt = E.params[0]["gauges"][0]
+    J = Gt[:, 0:1] * gr[0] + Gt[:, 1:2] * gr[1] + Gt[:, 2:3] * gr[2] + Gt[:, 3:4] * gr[3]
+    assert np.allclose(Hg[np.ix_(idx, idx)], J.T @ J / SIGMA ** 2, rtol=1e-12, atol=1e-9 * np.abs(Hg).max())
     h = 1e-7
     for k in range(12):
         sp_, sm = st.copy(), st.copy()
