@@ -550,6 +550,21 @@ def main():
     pcie = {"ms_per_step": t_pc * 1e3, "elements_per_s": nbeam / t_pc,
             "note": "H2D of X (16 MB) + assembly + D2H of rhs and nzval (673 MB) into pageable host memory"}
     log(f"PCIe-inclusive: {t_pc * 1e3:.2f} ms/step")
+    # the same assemble!{:iter} replayed from a captured HIP graph (the Newton iterations of a
+    # time step): bitwise the eager result; reported beside `value`, which is the eager path
+    def gstep():
+        r = _lib.lib.mh_assemble_sweepx_graph(h, 0, 1, _lib.dptr(c6), ctypes.byref(err))
+        if r != 0:
+            raise RuntimeError(_lib.lib.mh_last_error(h).decode())
+    for _ in range(max(2, args.warmup)):
+        gstep()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gstep()
+    _lib.lib.mh_sync(h)
+    t_g = (time.perf_counter() - t0) / args.steps
+    graph_replay = {"ms_per_step": t_g * 1e3, "elements_per_s": nbeam / t_g}
+    log(f"graph replay: {t_g * 1e3:.3f} ms/step")
     cpu = cpu1 = None
     if not args.no_cpu_baseline:
         nth = host_threads()
@@ -590,6 +605,7 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baseline_1thread": cpu1,
         "pcie_inclusive": pcie,
+        "graph_replay": graph_replay,
     }
     del out
     if not args.no_dynamic:
