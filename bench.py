@@ -387,8 +387,8 @@ def newton_bench(args):
 def directxua_bench(args):
     """solve(DirectXUA{2,0,0}) of a load-identification problem: a 100-beam curved cantilever over
     40 steps, a U-load at the tip and a cost on the tip's deviation from a target motion.
-    Residual elements per step on the GPU, one launch per step or all steps in one launch;
-    the all-steps system and its LU on the host."""
+    Residual elements per step on the GPU, one launch per step or all steps in one launch; the
+    all-steps system assembled on the host or on the device; its LU on the host."""
     import muscade_amd as mc
     from muscade_amd.workloads import chain_section
     nel, nstep = 100, 40
@@ -406,14 +406,15 @@ def directxua_bench(args):
     time_ = np.arange(1, nstep + 1) * 0.05
     res = {"workload": f"load identification, {nel} EulerBeam3D x {nstep} steps, DirectXUA{{2,0,0}}",
            "unknowns": nstep * (2 * model.ndof["X"] + model.ndof["U"])}
-    for mode in (False, True):
+    for name, mode, big in (("launch_per_step", False, "host"), ("steps_in_one_launch", True, "host"),
+                            ("steps_in_one_launch_big_system_on_device", True, "device")):
         st = {}
         t0 = time.perf_counter()
         mc.solve(mc.DirectXUA[2, 0, 0], initialstate=[st0], time=[time_], maxiter=30, verbose=False,
-                 catcherror=False, batch_steps=mode, stats=st)   # the reference's default tolerances (1e-5)
+                 catcherror=False, batch_steps=mode, big=big, stats=st)   # the reference's default tolerances (1e-5)
         st["total_s"] = time.perf_counter() - t0
-        res["steps_in_one_launch" if mode else "launch_per_step"] = st
-        log(f"directxua ({'one launch' if mode else 'per step'}): {st}")
+        res[name] = st
+        log(f"directxua ({name}): {st}")
     return res
 
 

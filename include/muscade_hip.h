@@ -228,6 +228,25 @@ int mh_newton_prepare(mh_ctx* ctx, const double* scaleX);
 int mh_newton_update(mh_ctx* ctx, int OX, int first, const double* c6, double* norms);
 int mh_get_X(mh_ctx* ctx, int ider, double* x);
 
+/* ---- DirectXUA's all-steps system on the device (src/DirectXUA.jl:245-383:
+ *      makepattern / preparebig / assemblebig!, with SparseTools.prepare / addin!,
+ *      src/SparseTools.jl:32-137, and the finite-difference weights of
+ *      src/FiniteDifferences.jl:2-34) --------------------------------------------------
+ * After mh_prepare_direct on a context whose model holds every step (copies of the model,
+ * one per step), mh_big_prepare takes the np contributions to the nv×nv system once: for
+ * contribution p, its 0-based row and column, its source src (an index into the context's
+ * per-step block values — L2[Λ,X][1,d] (d = 0..OX), L2[X,Λ][d,1], L2[Λ,U][1,1], L2[U,Λ][1,1]
+ * in the layout of mh_get_direct's blocks, each nnz of its pattern — or, from
+ * 2(OX+1)·nnz(XX) + 2·nnz(XU) on, an index into the nhost host-provided values) and its weight
+ * w (the stencil weights and Δt powers). It sorts them into a CSC pattern (nnz returned,
+ * mh_big_get_pattern: 1-based) and the contribution maps. mh_big_assemble, after each
+ * mh_assemble_direct, sums each nz's contributions on the device in contribution order and
+ * copies the values out. */
+int mh_big_prepare(mh_ctx* ctx, int64_t nv, int64_t np, const int64_t* rows, const int64_t* cols, const int64_t* src,
+                   const double* w, int64_t nhost, int64_t* nnz);
+int mh_big_get_pattern(mh_ctx* ctx, int64_t* colptr, int64_t* rowval);
+int mh_big_assemble(mh_ctx* ctx, const double* hostvals, double* nzval);
+
 /* ---- measurement -----------------------------------------------------------
  * HIP-event times (ms) of the last assembly, measured on the context's stream:
  * ms[0] element evaluation (all groups), ms[1] nzval scatter, ms[2] rhs scatter. */

@@ -154,6 +154,15 @@ struct Ctx {
     double* d_norm = nullptr;
     double* h_norm = nullptr;    // pinned: Σ Δx², Σ Lλ²
     hipEvent_t cev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // DirectXUA's all-steps system assembled on the device (mh_big_prepare / mh_big_assemble)
+    int64_t bnv = 0, bnp = 0, bnnz = 0, bnhost = 0, bndev = 0;
+    int32_t* d_bcolptr = nullptr;
+    int32_t* d_browval = nullptr;
+    int32_t* d_bseg = nullptr;
+    int64_t* d_bsrc = nullptr;   // per contribution, sorted: index into d_dL2 (< bndev) or the host values
+    double* d_bw = nullptr;      // per contribution, sorted: its weight (finite-difference factors)
+    double* d_bhost = nullptr;   // bnhost host-provided values
+    double* d_bval = nullptr;    // bnnz assembled values
 };
 
 #define HIPCHK(ctx, call)                                                                       \
@@ -181,9 +190,15 @@ void free_chain(Ctx* c) {
     for (auto& e : c->cev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
     c->cB = 0; c->cnb = c->cnp = c->cnd = 0;
 }
+void free_big(Ctx* c) {
+    dfree(c->d_bcolptr); dfree(c->d_browval); dfree(c->d_bseg); dfree(c->d_bsrc); dfree(c->d_bw); dfree(c->d_bhost);
+    dfree(c->d_bval);
+    c->bnv = c->bnp = c->bnnz = c->bnhost = c->bndev = 0;
+}
 void free_prepared(Ctx* c) {
     free_graph(c);
     free_chain(c);
+    free_big(c);
     dfree(c->d_Kbuf); dfree(c->d_Rbuf); dfree(c->d_dRbuf); dfree(c->d_nzval); dfree(c->d_rhs); dfree(c->d_Kc);
     dfree(c->d_colptr); dfree(c->d_rowval); dfree(c->d_kseg); dfree(c->d_ksrc); dfree(c->d_rseg); dfree(c->d_rsrc);
     dfree(c->d_useg); dfree(c->d_usrc); dfree(c->d_gbuf); dfree(c->d_gX); dfree(c->d_gubuf); dfree(c->d_gU);
@@ -2718,6 +2733,147 @@ int mh_set_host_blocks(mh_ctx* c, int igroup, const double* R, const double* K) 
     HIPCHK(c, hipMemcpyAsync(c->d_Rbuf + g.rbase, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_Kbuf + g.kbase, k.data(), sizeof(double) * k.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MH_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ DirectXUA big system
+namespace {
+
+__global__ __launch_bounds__(kBlock) void iota_u32(int64_t n, uint32_t* v) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) v[p] = (uint32_t)p;
+}
+__global__ __launch_bounds__(kBlock) void big_permute(int64_t n, const uint32_t* perm, const int64_t* src_in,
+                                                      const double* w_in, int64_t* src, double* w) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t q = perm[p];
+    src[p] = src_in[q];
+    w[p] = w_in[q];
+}
+// val[k] = Σ w·source over the contributions of nz k, in contribution order (deterministic).
+__global__ __launch_bounds__(kBlock) void big_scatter(int64_t nnz, const int32_t* __restrict__ seg,
+                                                      const int64_t* __restrict__ src, const double* __restrict__ w,
+                                                      const double* __restrict__ dev, int64_t ndev,
+                                                      const double* __restrict__ host, double* __restrict__ val) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    double acc = 0.;
+    for (int32_t p = seg[k]; p < seg[k + 1]; ++p) {
+        const int64_t q = src[p];
+        acc += w[p] * (q < ndev ? dev[q] : host[q - ndev]);
+    }
+    val[k] = acc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mh_big_prepare(mh_ctx* c, int64_t nv, int64_t np, const int64_t* rows, const int64_t* cols, const int64_t* src,
+                   const double* w, int64_t nhost, int64_t* nnz_out) {
+    if (!c || !c->direct || nv < 1 || np < 0 || nhost < 0 || (np && (!rows || !cols || !src || !w))) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_big(c);
+    if (nv >= INT32_MAX || np >= INT32_MAX) { c->err = "big system too large for 32-bit maps"; return MH_ERR_SIZE; }
+    const int64_t ndev = 2 * (int64_t)(c->dOX + 1) * c->dXX.nnz + 2 * c->dXU.nnz;
+    std::vector<unsigned long long> keys(np);
+    for (int64_t p = 0; p < np; ++p) {
+        if (rows[p] < 0 || rows[p] >= nv || cols[p] < 0 || cols[p] >= nv || src[p] < 0 || src[p] >= ndev + nhost) {
+            c->err = "big contribution " + std::to_string(p + 1) + " out of range"; return MH_ERR_ARG;
+        }
+        keys[p] = (unsigned long long)cols[p] * (unsigned long long)nv + (unsigned long long)rows[p];
+    }
+    hipStream_t s = c->stream;
+    c->bnv = nv; c->bnp = np; c->bnhost = nhost; c->bndev = ndev;
+    HIPCHK(c, hipMalloc(&c->d_bcolptr, sizeof(int32_t) * (nv + 1)));
+    HIPCHK(c, hipMalloc(&c->d_bhost, sizeof(double) * std::max<int64_t>(nhost, 1)));
+    if (np == 0) {
+        HIPCHK(c, hipMemsetAsync(c->d_bcolptr, 0, sizeof(int32_t) * (nv + 1), s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        if (nnz_out) *nnz_out = 0;
+        return MH_OK;
+    }
+    unsigned long long *kin = nullptr, *kout = nullptr;
+    uint32_t *vin = nullptr, *perm = nullptr;
+    int32_t *head = nullptr, *id = nullptr, *colk = nullptr;
+    int64_t* src_in = nullptr;
+    double* w_in = nullptr;
+    HIPCHK(c, hipMalloc(&kin, sizeof(unsigned long long) * np));
+    HIPCHK(c, hipMalloc(&kout, sizeof(unsigned long long) * np));
+    HIPCHK(c, hipMalloc(&vin, sizeof(uint32_t) * np));
+    HIPCHK(c, hipMalloc(&perm, sizeof(uint32_t) * np));
+    HIPCHK(c, hipMalloc(&src_in, sizeof(int64_t) * np));
+    HIPCHK(c, hipMalloc(&w_in, sizeof(double) * np));
+    HIPCHK(c, hipMemcpyAsync(kin, keys.data(), sizeof(unsigned long long) * np, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(src_in, src, sizeof(int64_t) * np, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(w_in, w, sizeof(double) * np, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(iota_u32, dim3(grid(np)), dim3(kBlock), 0, s, np, vin);
+    int end_bit = 1;
+    while (end_bit < 64 && ((unsigned long long)nv * (unsigned long long)nv) > (1ull << end_bit)) ++end_bit;
+    size_t tmp_bytes = 0;
+    void* tmp = nullptr;
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, kin, kout, vin, perm, (int)np, 0, end_bit, s));
+    HIPCHK(c, hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kin, kout, vin, perm, (int)np, 0, end_bit, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    dfree(tmp);
+    HIPCHK(c, hipMalloc(&head, sizeof(int32_t) * np));
+    HIPCHK(c, hipMalloc(&id, sizeof(int32_t) * np));
+    hipLaunchKernelGGL(head_kernel<unsigned long long>, dim3(grid(np)), dim3(kBlock), 0, s, np, kout, head);
+    tmp_bytes = 0;
+    HIPCHK(c, hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, head, id, (int)np, s));
+    HIPCHK(c, hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
+    HIPCHK(c, hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, head, id, (int)np, s));
+    int32_t nnz = 0;
+    HIPCHK(c, hipMemcpyAsync(&nnz, id + np - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    dfree(tmp);
+    c->bnnz = nnz;
+    HIPCHK(c, hipMalloc(&c->d_browval, sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
+    HIPCHK(c, hipMalloc(&c->d_bseg, sizeof(int32_t) * (nnz + 1)));
+    HIPCHK(c, hipMalloc(&c->d_bsrc, sizeof(int64_t) * np));
+    HIPCHK(c, hipMalloc(&c->d_bw, sizeof(double) * np));
+    HIPCHK(c, hipMalloc(&c->d_bval, sizeof(double) * std::max<int64_t>(nnz, 1)));
+    HIPCHK(c, hipMalloc(&colk, sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
+    hipLaunchKernelGGL(seg_kernel, dim3(grid(np)), dim3(kBlock), 0, s, np, kout, head, id, nv, c->d_bseg, c->d_browval, colk);
+    const int32_t np32 = (int32_t)np;
+    HIPCHK(c, hipMemcpyAsync(c->d_bseg + nnz, &np32, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(gapfill_kernel, dim3(grid(nnz + 1)), dim3(kBlock), 0, s, (int64_t)nnz, colk, nv, c->d_bcolptr);
+    hipLaunchKernelGGL(big_permute, dim3(grid(np)), dim3(kBlock), 0, s, np, perm, src_in, w_in, c->d_bsrc, c->d_bw);
+    if (int r = check_launch(c, "big pattern")) return r;
+    HIPCHK(c, hipStreamSynchronize(s));
+    dfree(kin); dfree(kout); dfree(vin); dfree(perm); dfree(head); dfree(id); dfree(colk); dfree(src_in); dfree(w_in);
+    if (nnz_out) *nnz_out = nnz;
+    return MH_OK;
+}
+
+int mh_big_get_pattern(mh_ctx* c, int64_t* colptr, int64_t* rowval) {
+    if (!c || !c->d_bcolptr || !colptr || (c->bnnz && !rowval)) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<int32_t> cp(c->bnv + 1), rv(c->bnnz);
+    HIPCHK(c, hipMemcpy(cp.data(), c->d_bcolptr, sizeof(int32_t) * cp.size(), hipMemcpyDeviceToHost));
+    if (c->bnnz) HIPCHK(c, hipMemcpy(rv.data(), c->d_browval, sizeof(int32_t) * rv.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < cp.size(); ++i) colptr[i] = (int64_t)cp[i] + 1;
+    for (size_t i = 0; i < rv.size(); ++i) rowval[i] = (int64_t)rv[i] + 1;
+    return MH_OK;
+}
+
+int mh_big_assemble(mh_ctx* c, const double* hostvals, double* nzval) {
+    if (!c || !c->direct || !c->d_bcolptr || (c->bnhost && !hostvals) || (c->bnnz && !nzval)) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    if (c->bnhost) HIPCHK(c, hipMemcpyAsync(c->d_bhost, hostvals, sizeof(double) * c->bnhost, hipMemcpyHostToDevice, s));
+    if (c->bnnz) {
+        hipLaunchKernelGGL(big_scatter, dim3(grid(c->bnnz)), dim3(kBlock), 0, s, c->bnnz, c->d_bseg, c->d_bsrc, c->d_bw,
+                           c->d_dL2, c->bndev, c->d_bhost, c->d_bval);
+        if (int r = check_launch(c, "big scatter")) return r;
+        HIPCHK(c, hipMemcpyAsync(nzval, c->d_bval, sizeof(double) * c->bnnz, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(c, hipStreamSynchronize(s));
     return MH_OK;
 }
 

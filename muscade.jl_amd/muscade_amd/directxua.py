@@ -30,6 +30,7 @@ no_second_order approximation (no Λᵀ∂²R). Hold and DofLoad are evaluated h
 Lagrangians on the host, so the device assembles residual elements only.
 """
 import copy
+import ctypes
 import math
 import time as _time
 
@@ -37,6 +38,7 @@ import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
+from ._lib import dptr, i64ptr, lib
 from .assemble import AssemblyDirect
 from .hostcost import Dual2
 from .model import MuscadeError
@@ -249,6 +251,88 @@ class StepBatchAssembly:
         return res
 
 
+class DeviceBigSystem:
+    """DirectXUA's all-steps system assembled on the device (makepattern / preparebig /
+    assemblebig!, src/DirectXUA.jl:245-383; SparseTools.jl:32-137) from the per-step blocks that
+    a StepBatchAssembly leaves in its context, plus the host elements' pieces. The contribution
+    list (row, column, source, finite-difference weight) is built once here and sorted on the
+    device into the CSC pattern (mh_big_prepare); each iteration sums the contributions of every
+    nz on the device (mh_big_assemble) and the host receives the finished values."""
+
+    def __init__(self, batched, nv, block, dt, hcontrib):
+        out = batched.out
+        self.ctx, self.nv = out.ctx, nv
+        OX, nX, nU, n = batched.OX, batched.nX, batched.nU, batched.nstep
+        cpXX, rvXX, _ = out.patterns[0]
+        cpXU, rvXU, _ = out.patterns[1]
+        cpUX, rvUX, _ = out.patterns[2]
+
+        def coo(cp, rv):
+            col = np.repeat(np.arange(cp.size - 1), np.diff(cp))
+            return rv - 1, col
+        R0, C0 = coo(cpXX, rvXX)
+        nxx, nxu = R0.size, rvXU.size
+        ndX = OX + 1
+        rows, cols, src, w = [], [], [], []
+        step = R0 // max(nX, 1)
+        r0, c0 = R0 - step * nX, C0 - step * nX
+        for s_ in range(n):
+            k = np.nonzero(step == s_)[0]
+            if k.size == 0:
+                continue
+            for d in range(ndX):
+                sc = dt ** (-d)
+                for Δ, wt in finitediff(d, n, s_ + 1):
+                    # L2[Λ,X][1,d]: rows Λ of step s, columns X of step s+Δ
+                    rows.append(block(s_ + 1, "Λ") + r0[k]); cols.append(block(s_ + 1 + Δ, "X") + c0[k])
+                    src.append(d * nxx + k); w.append(np.full(k.size, wt * sc))
+                    # L2[X,Λ][d,1]
+                    rows.append(block(s_ + 1 + Δ, "X") + r0[k]); cols.append(block(s_ + 1, "Λ") + c0[k])
+                    src.append((ndX + d) * nxx + k); w.append(np.full(k.size, wt * sc))
+        if nU and nxu:
+            R1, C1 = coo(cpXU, rvXU)
+            st1 = R1 // nX
+            rows.append(np.array([block(q + 1, "Λ") for q in st1], np.int64) + R1 - st1 * nX)
+            cols.append(np.array([block(q + 1, "U") for q in st1], np.int64) + C1 - st1 * nU)
+            src.append(2 * ndX * nxx + np.arange(nxu)); w.append(np.ones(nxu))
+            R2, C2 = coo(cpUX, rvUX)
+            st2 = R2 // nU
+            rows.append(np.array([block(q + 1, "U") for q in st2], np.int64) + R2 - st2 * nU)
+            cols.append(np.array([block(q + 1, "Λ") for q in st2], np.int64) + C2 - st2 * nX)
+            src.append(2 * ndX * nxx + nxu + np.arange(R2.size)); w.append(np.ones(R2.size))
+        ndev = 2 * ndX * nxx + 2 * nxu
+        self.hsizes = [v.size for (_, _, _, v) in hcontrib]
+        o = ndev
+        for (r, c, wh, v) in hcontrib:
+            rows.append(np.asarray(r, np.int64)); cols.append(np.asarray(c, np.int64))
+            src.append(o + np.arange(v.size)); w.append(np.asarray(wh, float))
+            o += v.size
+        cat = lambda a, t: np.ascontiguousarray(np.concatenate(a) if a else np.zeros(0), dtype=t)   # noqa: E731
+        rows, cols, src, w = cat(rows, np.int64), cat(cols, np.int64), cat(src, np.int64), cat(w, np.float64)
+        self.nhost = o - ndev
+        nnz = ctypes.c_int64()
+        self.ctx.check(lib.mh_big_prepare(self.ctx.h, nv, rows.size, i64ptr(rows), i64ptr(cols), i64ptr(src), dptr(w),
+                                          self.nhost, ctypes.byref(nnz)), "big prepare")
+        self.colptr = np.zeros(nv + 1, np.int64)
+        self.rowval = np.zeros(nnz.value, np.int64)
+        self.ctx.check(lib.mh_big_get_pattern(self.ctx.h, i64ptr(self.colptr), i64ptr(self.rowval)), "big pattern")
+        self.colptr -= 1
+        self.rowval -= 1
+        self.ncontrib = rows.size
+
+    def matches(self, hcontrib):
+        return [v.size for (_, _, _, v) in hcontrib] == self.hsizes
+
+    def assemble(self, hcontrib):
+        hv = np.ascontiguousarray(np.concatenate([v for (_, _, _, v) in hcontrib]) if hcontrib else np.zeros(0),
+                                  dtype=np.float64)
+        nz = np.zeros(self.rowval.size)
+        self.ctx.check(lib.mh_big_assemble(self.ctx.h, dptr(hv) if hv.size else None, dptr(nz)), "big assemble")
+        M = sp.csc_matrix((nz, self.rowval.copy(), self.colptr.copy()), shape=(self.nv, self.nv))
+        M.eliminate_zeros()   # as the host path: structural entries that are zero leave the LU's pattern
+        return M
+
+
 class _DirectXUAMeta(type):
     def __getitem__(cls, key):
         OX, OU, IA = key
@@ -263,11 +347,14 @@ class DirectXUA(metaclass=_DirectXUAMeta):
 
     @classmethod
     def run(cls, pstate, verbose, dbg, *, time, initialstate, maxiter=50, maxΔλ=1e-5, maxΔx=1e-5, maxΔu=1e-5,
-            maxΔa=1e-5, saveiter=False, device=0, Assembly=AssemblyDirect, batch_steps=False, stats=None):
+            maxΔa=1e-5, saveiter=False, device=0, Assembly=AssemblyDirect, batch_steps=False, big="host", stats=None):
         """`Assembly` is the per-step assembler of the device element kinds (prepare(AssemblyDirect{OX,OU,IA}),
         src/DirectXUA.jl:22-56). batch_steps: all steps of an experiment in one device launch per
-        Newton iteration (StepBatchAssembly; beams only). stats (a dict) receives the iteration count
-        and the seconds spent in device assembly, host assembly and the sparse LU."""
+        Newton iteration (StepBatchAssembly; beams only). big: "host" assembles the all-steps
+        system on the host (assemblebig!, DirectXUA.jl:336-383), "device" on the device from the
+        step-batched blocks (DeviceBigSystem; batch_steps and one experiment). stats (a dict)
+        receives the iteration count and the seconds spent in device assembly, host assembly and
+        the sparse LU."""
         clock = _time.perf_counter
         st_ = stats if stats is not None else {}
         st_.update(iterations=0, device_assembly_s=0., host_assembly_s=0., lu_s=0.)
@@ -296,6 +383,10 @@ class DirectXUA(metaclass=_DirectXUAMeta):
         stateiter = [] if saveiter else None
         pstate[0] = stateiter if saveiter else state
 
+        if big not in ("host", "device"):
+            raise MuscadeError(f"big must be 'host' or 'device', not {big!r}")
+        if big == "device" and (not batch_steps or nexp != 1):
+            raise MuscadeError("big='device' assembles the step-batched blocks of one experiment (batch_steps=True)")
         host = HostLagrangians(model, dis, OX, OU, IA)
         if batch_steps:
             batched = [StepBatchAssembly(OX, OU, IA, model, dis, n, skip=tuple(host.skip), device=device)
@@ -318,16 +409,19 @@ class DirectXUA(metaclass=_DirectXUAMeta):
         def block(iexp, istep, c):   # istep 1-based
             return int(exp0[iexp]) + (istep - 1) * per_step + coff[c]
 
+        dbig = None
         for it in range(1, maxiter + 1):
             if verbose:
                 print(f"\n    Iteration {it:3d}\n        Assembling", end="")
             rows, cols, vals = [], [], []
+            hcontrib = []   # device big system: (rows, cols, w, values) of the host pieces, in order
             Lv = np.zeros(nv)
             if IA:
                 gA, pieces = host.once(state[0][0])
                 Lv[Aoff:Aoff + nA] += gA
                 for r, c, v in pieces:
                     rows.append(r + Aoff); cols.append(c + Aoff); vals.append(v)
+                    hcontrib.append((r + Aoff, c + Aoff, np.ones(v.size), v))
 
             def off(iexp, istep, c):
                 return Aoff if c == "A" else block(iexp, istep, c)
@@ -365,6 +459,14 @@ class DirectXUA(metaclass=_DirectXUAMeta):
                             for Δs, w in finitediff(bder, nstep[iexp], istep):
                                 o = off(iexp, istep + Δs, b)
                                 Lv[o:o + v.size] += w * s * v
+                    if big == "device":
+                        for (a, i, b, j, r, c, v) in L2h:
+                            sc = dt ** (-i - j)
+                            for Δa, wa in finitediff(i, nstep[iexp], istep):
+                                for Δb, wb in finitediff(j, nstep[iexp], istep):
+                                    hcontrib.append((r + off(iexp, istep + Δa, a), c + off(iexp, istep + Δb, b),
+                                                     np.full(v.size, wa * wb * sc), v))
+                        continue
                     for (a, b), blocks in L2d.items():
                         for i, row in enumerate(blocks):
                             for j, M in enumerate(row):
@@ -384,10 +486,17 @@ class DirectXUA(metaclass=_DirectXUAMeta):
                                 rows.append(r + off(iexp, istep + Δa, a))
                                 cols.append(c + off(iexp, istep + Δb, b))
                                 vals.append(v * (wa * wb * s))
-            Lvv = sp.coo_matrix((np.concatenate(vals) if vals else np.zeros(0),
-                                 (np.concatenate(rows) if rows else np.zeros(0, int),
-                                  np.concatenate(cols) if cols else np.zeros(0, int))), shape=(nv, nv)).tocsc()
-            Lvv.eliminate_zeros()
+            if big == "device":
+                if dbig is None or not dbig.matches(hcontrib):
+                    dbig = DeviceBigSystem(batched[0], nv, lambda s_, c_: block(0, s_, c_), Δt[0], hcontrib)
+                t0 = clock()
+                Lvv = dbig.assemble(hcontrib)
+                t_dev += clock() - t0
+            else:
+                Lvv = sp.coo_matrix((np.concatenate(vals) if vals else np.zeros(0),
+                                     (np.concatenate(rows) if rows else np.zeros(0, int),
+                                      np.concatenate(cols) if cols else np.zeros(0, int))), shape=(nv, nv)).tocsc()
+                Lvv.eliminate_zeros()
             st_["device_assembly_s"] += t_dev
             st_["host_assembly_s"] += clock() - t_iter - t_dev
             st_["iterations"] = it
