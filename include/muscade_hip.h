@@ -260,6 +260,9 @@ int mh_calib_copy(mh_ctx* ctx, int64_t n, double* ms);
  * launch_scatter) and each shape's measured time in ms (16).
  * A measurement utility; no counterpart in the reference. */
 int mh_get_tuning(mh_ctx* ctx, int* variant, float* ms);
+/* Override the tuner's pick (a testing and measurement utility: every shape gives bitwise the
+ * same sums). */
+int mh_set_scatter_variant(mh_ctx* ctx, int variant);
 int mh_sync(mh_ctx* ctx);
 
 #ifdef __cplusplus

@@ -2233,6 +2233,13 @@ int mh_calib_copy(mh_ctx* c, int64_t n, double* ms) {
     return MH_OK;
 }
 
+int mh_set_scatter_variant(mh_ctx* c, int variant) {
+    if (!c || !c->prepared || variant < 0 || variant >= kScatterVariants) return MH_ERR_ARG;
+    c->scatter_variant = variant;
+    ++c->gen;   // a captured graph holds the old launch
+    return MH_OK;
+}
+
 int mh_get_tuning(mh_ctx* c, int* variant, float* ms) {
     if (!c || !c->prepared) return MH_ERR_ARG;
     if (variant) *variant = c->scatter_variant;

@@ -46,6 +46,7 @@ SIGNATURES = {
     "mh_vjp_lagrangian": (_I, [_P, _I, ctypes.POINTER(mh_err)]),
     "mh_get_vjp": (_I, [_P, _D, _D, _D]),
     "mh_get_tuning": (_I, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)]),
+    "mh_set_scatter_variant": (_I, [_P, _I]),
     "mh_set_group_scales": (_I, [_P, _I, _D, _D]),
     "mh_prepare_direct": (_I, [_P, _I, _I, _I, _I64]),
     "mh_get_direct_pattern": (_I, [_P, _I, _I64, _I64]),
