@@ -8,7 +8,8 @@ from .model import (Disassembler, Model, MuscadeError, State, addelement, addnod
                     setscale)
 from .elements import (Acost, AxisymmetricBarCrossSection, Bar3D, Bar3DU, BeamCrossSection, DofCost, DofLoad,
                        EulerBeam3D, EulerBeam3DU, Hold, SingleAcost, SingleDofCost, SingleUdof,
-                       StrainGaugeOnEulerBeam3D, ElementCost, DofConstraint, equal, positive, off)
+                       StrainGaugeOnEulerBeam3D, ElementCost, ElementConstraint, DofConstraint, equal, positive,
+                       off)
 from .assemble import AssemblyDirect, AssemblySweepX, DeviceContext, Newmarkβcoefficients
 from .sweepx import SweepX, solve
 from .directxua import DirectXUA
@@ -19,6 +20,6 @@ __all__ = ["LIBPATH", "lib", "Model", "MuscadeError", "State", "Disassembler", "
            "setscale", "getdof", "setdof", "BeamCrossSection", "EulerBeam3D", "EulerBeam3DU", "Hold", "DofLoad",
            "AxisymmetricBarCrossSection", "Bar3D", "Bar3DU",
            "DofCost", "SingleDofCost", "SingleUdof", "Acost", "SingleAcost", "DirectXUA", "Dual2", "exp10",
-           "StrainGaugeOnEulerBeam3D", "ElementCost", "DofConstraint", "equal", "positive", "off",
+           "StrainGaugeOnEulerBeam3D", "ElementCost", "ElementConstraint", "DofConstraint", "equal", "positive", "off",
            "getresult", "request",
            "AssemblyDirect", "AssemblySweepX", "DeviceContext", "Newmarkβcoefficients", "SweepX", "solve"]

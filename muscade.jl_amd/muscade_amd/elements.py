@@ -141,23 +141,84 @@ class ElementCost(EulerBeam3D):
         return ("ElementCost", name, bool(exact)) + key, doflist, params
 
     @staticmethod
-    def lagrangian(p, Λ, X, U, A, t):
-        """The cost part; p["_gd"] = (values (m,4), gradients (m,4,12), Hessians (m,4,12,12)) of
-        (εₐₓ, κ₁, κ₂, κ₃) w.r.t. the element's X-dofs at this state."""
+    def eleres(p, X):
+        """The requested gauge requestables as second-order duals over the solver's partials;
+        p["_gd"] = (values (m,4), gradients (m,4,12), Hessians (m,4,12,12)) of (εₐₓ, κ₁, κ₂, κ₃)
+        w.r.t. the element's X-dofs at this state (the Hessians dropped unless p["exact"])."""
         v, g, H = p["_gd"]
         if not p.get("exact", False):
             H = np.zeros_like(H)
-        q = [compose(v[:, k], g[:, k], H[:, k], X[0]) for k in range(4)]
+        q = [compose(v[:, k], g[:, k], H[:, k], X[0][:12]) for k in range(4)]
         G = p["gauges"]                                   # m, ngauge, 4
         eps = [((G[:, k, 0] * q[0] + G[:, k, 1] * q[1]) + G[:, k, 2] * q[2]) + G[:, k, 3] * q[3]
                for k in range(G.shape[1])]
         full = {"εₐₓ": q[0], "κ": q[1:], "ε": eps}
-        eleres = {}
+        out = {}
         for name in p["req"]:
             if name not in full:
                 raise MuscadeError(f"{name} is not a requestable of StrainGaugeOnEulerBeam3D")
-            eleres[name] = full[name]
+            out[name] = full[name]
+        return out
+
+    @staticmethod
+    def term(p, eleres, t):
+        """cost(eleres, t, costargs...) (BasicElements.jl:128-129)."""
         return call_cost(p["cost"], (eleres, t), p["costargs"])
+
+    @staticmethod
+    def lagrangian(p, Λ, X, U, A, t):
+        """The cost part (the wrapped beam's Λ·R is assembled on the device)."""
+        return ElementCost.term(p, ElementCost.eleres(p, X), t)
+
+
+class ElementConstraint(EulerBeam3D):
+    """ElementConstraint(nod; λinod, λfield, req, gap, gargs=(), mode, ElementType, elementkwargs)
+    — src/BasicElements.jl:490-576, for ElementType = StrainGaugeOnEulerBeam3D. A constraint
+    gap(eleres, t, gargs...) ≥ 0 (or = 0) on the requested gauge requestables, with a Lagrange
+    multiplier λ of class U at element node λinod, field λfield, appended to the beam's dofs:
+    L = Λ·R(beam) − gap·λ (mode equal) | −KKT(λ, gap, γ) (positive) | −½λ² (off), γ from SP.
+    As for ElementCost, the beam is assembled on the device and the constraint term on the host
+    from the device's gauge values, gradients and Hessians. In SweepX the term contributes
+    nothing (it has no Λ; λ is a U-dof).
+
+    DirectXUA's accelerator for this element (src/DirectXUA.jl:204-241) revariates the
+    requestables to first order, like ElementCost's, but also takes λ = ∂0(U)[Nu] undifferentiated,
+    so that ∂L/∂λ = −gap, the constraint equation itself, is lost and the λ rows of the all-steps
+    Hessian are zero. Here λ is differentiated as in the element's own lagrangian; the
+    requestables are first order by default (the Hessian misses −λ·∂²gap/∂X², the same
+    solution), and exact=True adds their second derivatives."""
+    host_cost = True
+    uses_SP = True
+
+    @classmethod
+    def construct(cls, model, nodes0, λinod, λfield, req, gap, mode, ElementType, elementkwargs, gargs=(),
+                  exact=False):
+        if ElementType is not StrainGaugeOnEulerBeam3D:
+            raise MuscadeError("ElementConstraint is built for StrainGaugeOnEulerBeam3D elements (the device "
+                               "requestables)")
+        λinod = int(λinod)
+        if not 1 <= λinod <= max(2, nodes0.shape[1]):
+            raise MuscadeError("λinod must be an element node number")
+        key, (inod, dclass, field), params = ElementType.construct(model, nodes0, **elementkwargs)
+        name = getattr(gap, "__name__", type(gap).__name__)
+        params.update(req=dict(req), gap=gap, gargs=tuple(gargs), mode=mode, exact=bool(exact))
+        doflist = (inod + (λinod,), dclass + ("U",), field + (λfield,))
+        key = ("ElementConstraint", name, getattr(mode, "__name__", str(mode)), λinod, λfield, bool(exact)) + key
+        return key, doflist, params
+
+    @staticmethod
+    def term(p, eleres, lam, t, γ=0.):
+        """−gap·λ | −KKT(λ, gap, γ) | −½λ² with gap = gap(eleres, t, gargs...)
+        (BasicElements.jl:569-575)."""
+        mode = p["mode"](t)
+        if mode == "off":
+            return -0.5 * (lam * lam)
+        return DofConstraint._mode_lagrangian(mode, lam, call_cost(p["gap"], (eleres, t), p["gargs"]), γ)
+
+    @staticmethod
+    def lagrangian(p, Λ, X, U, A, t, SP=None):
+        γ = float((SP or {}).get("γ", 0.))
+        return ElementConstraint.term(p, ElementCost.eleres(p, X), U[0][-1], t, γ)
 
 
 BAR_FIELDS = ("EA", "μ", "w", "Caₜ", "Clₜ", "Cqₜ", "Caₙ", "Clₙ", "Cqₙ")

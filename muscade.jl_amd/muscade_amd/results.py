@@ -106,7 +106,8 @@ def getresult(state, req, ieletyp, device=0, _cache={}):
     """eleres = getresult(state, req, eletype) for all elements of type ieletyp (0-based)
     (src/Output.jl:164-175). EulerBeam3D requestables: ε, κ, gp(x, κgp, resultants(fᵢ, mᵢ,
     fₑ, mₑ)); StrainGaugeOnEulerBeam3D: εₐₓ, κ, ε (the gauge strains) and eleres(...) with
-    the beam's requestables."""
+    the beam's requestables; ElementConstraint over StrainGaugeOnEulerBeam3D: λ, gap and
+    eleres(...) with the strain gauge's requestables."""
     model = state.model
     key = (id(model), device)
     er = _cache.get(key)
@@ -117,5 +118,31 @@ def getresult(state, req, ieletyp, device=0, _cache={}):
     if "gauges" in E.params[0]:
         res = {"εₐₓ": r["ε"][:, 0], "κ": r["κ"][:, 0], "ε": r["gauges"],
                "eleres": {k: r[k] for k in ("ε", "κ", "gp")}}
+        if E.key[0] == "ElementConstraint":
+            res = _constraint_results(state, ieletyp, res)
         return _select(res, req, E.key[0])
     return _select({k: r[k] for k in ("ε", "κ", "gp")}, req, E.key[0])
+
+
+def _constraint_results(state, ieletyp, gauge):
+    """ElementConstraint's requestables (src/BasicElements.jl:564-576): λ, gap (the gap function
+    of the requested gauge requestables, evaluated on the host) and eleres(...), the target
+    StrainGaugeOnEulerBeam3D's requestables."""
+    from .hostcost import call_cost
+    E = state.model.eletypes[ieletyp]
+    lam = state.U[0][state.dis.dis[ieletyp].U[:, -1] - 1]
+    gap = np.zeros(E.nele)
+    e0 = 0
+    for p in E.params:
+        m = p["gauges"].shape[0]
+        sl = slice(e0, e0 + m)
+        full = {"εₐₓ": gauge["εₐₓ"][sl], "κ": [gauge["κ"][sl, k] for k in range(3)],
+                "ε": [gauge["ε"][sl, k] for k in range(gauge["ε"].shape[1])]}
+        eleres = {}
+        for name in p["req"]:
+            if name not in full:
+                raise MuscadeError(f"{name} is not a requestable of StrainGaugeOnEulerBeam3D")
+            eleres[name] = full[name]
+        gap[sl] = call_cost(p["gap"], (eleres, state.time), p["gargs"])
+        e0 += m
+    return {"λ": lam, "gap": gap, "eleres": gauge}

@@ -39,8 +39,11 @@ inline const double* anchor_poly() {
                                  687.83550335374};
     return p;
 }
-template <class TX> inline void anchorline_residual(const AnchorLine& o, const TX* x, const double* A, TX* R) {
-    double L = o.L * (1. + A[0]), buoy = o.buoyancy * (1. + A[1]);
+// AnchorLine's residual (Fd, m3) and its requestables Fh and cr (SomeElements.jl:77-93), with
+// the A-dofs (ΔL, Δbuoyancy) of the same scalar type so that they can carry partials too.
+template <class TX> inline void anchorline_eval(const AnchorLine& o, const TX* x, const TX* A, TX* R, TX* Fh_out,
+                                                TX* cr_out) {
+    TX L = o.L * (1. + A[0]), buoy = o.buoyancy * (1. + A[1]);
     TX Xtop[3] = {x[0] + o.xmtop[0], x[1] + o.xmtop[1], TX(0. + o.xmtop[2])};
     TX alpha = x[2];
     TX c = dcos(alpha), s = dsin(alpha);
@@ -59,6 +62,12 @@ template <class TX> inline void anchorline_residual(const AnchorLine& o, const T
     TX Fd[2] = {ch[0] / xaf * Fh, ch[1] / xaf * Fh};
     TX m3 = dXtop[0] * Fd[1] - dXtop[1] * Fd[0];
     R[0] = Fd[0]; R[1] = Fd[1]; R[2] = m3;
+    if (Fh_out) *Fh_out = Fh;
+    if (cr_out) *cr_out = cr;
+}
+template <class TX> inline void anchorline_residual(const AnchorLine& o, const TX* x, const double* A, TX* R) {
+    const TX Ad[2] = {TX(A[0]), TX(A[1])};
+    anchorline_eval<TX>(o, x, Ad, R, nullptr, nullptr);
 }
 
 }  // namespace orc

@@ -295,6 +295,35 @@ int orc_beam_gauge2(const double* c1, const double* c2, const double* orient2, c
     return 0;
 }
 
+// AnchorLine (test/SomeElements.jl:54-94) at x (tx1, tx2, rx3) and A (ΔL, Δbuoyancy): R (3),
+// its gradient (3×5, row-major, partials x then A), the requestables Fh and cr, Fh's gradient
+// (5) and Hessian (5×5). p10 = xmtop(3), Δxmtop(3), xmbot(2), L, buoyancy.
+int orc_anchor_eval(const double* p10, const double* x, const double* A, double* R, double* dR, double* Fh,
+                    double* dFh, double* HFh, double* cr) {
+    AnchorLine o;
+    for (int i = 0; i < 3; ++i) { o.xmtop[i] = p10[i]; o.dxmtop[i] = p10[3 + i]; }
+    o.xmbot[0] = p10[6]; o.xmbot[1] = p10[7]; o.L = p10[8]; o.buoyancy = p10[9];
+    using T = Dual<5, Dual<5, double>>;
+    T v[5];
+    for (int i = 0; i < 5; ++i) {
+        v[i].x = seed<5, double>(i < 3 ? x[i] : A[i - 3], i);
+        v[i].dx[i] = Dual<5, double>(1.);
+    }
+    T Rr[3], fh, c;
+    anchorline_eval<T>(o, v, v + 3, Rr, &fh, &c);
+    for (int k = 0; k < 3; ++k) {
+        R[k] = Rr[k].x.x;
+        for (int i = 0; i < 5; ++i) dR[5 * k + i] = Rr[k].dx[i].x;
+    }
+    *Fh = fh.x.x;
+    *cr = c.x.x;
+    for (int i = 0; i < 5; ++i) {
+        dFh[i] = fh.dx[i].x;
+        for (int j = 0; j < 5; ++j) HFh[5 * i + j] = fh.dx[i].dx[j];
+    }
+    return 0;
+}
+
 // ε and ♢κ with their gradients w.r.t. X₀ (12): the strain-gauge requestables
 // (toolbox/StrainGaugeOnBeamElement.jl:70-76) differentiated as by
 // diffed_lagrangian{1} (Diagnostic.jl:851-911). Also R and the first-order

@@ -41,6 +41,7 @@ _SIG = {
     "orc_beam_gauge": (ctypes.c_int, [_D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
     "orc_beam_results": (ctypes.c_int, [_D, _D, _D, _D, ctypes.c_int, _D, _D, _D]),
     "orc_beam_gauge2": (ctypes.c_int, [_D, _D, _D, _D, _D, _D]),
+    "orc_anchor_eval": (ctypes.c_int, [_D, _D, _D, _D, _D, _D, _D, _D, _D]),
     "orc_number_dofs": (ctypes.c_int, [ctypes.c_int, _IP, _IP, _IP, ctypes.c_int, _IP, _IP, _IP, _IP, _IP, _IP, _IP,
                                        _I64, ctypes.c_int, _IP, _IP, _IP, _IP, _IP, _IP]),
     "orc_model_new": (_V, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
@@ -179,6 +180,14 @@ def beam_gauge2(c1, c2, orient2, mat16, X):
     r = lib.orc_beam_gauge2(dp(f64(c1)), dp(f64(c2)), dp(f64(orient2)), dp(f64(mat16)), dp(f64(X)), dp(out))
     assert r == 0
     return out[:4], out[4:52].reshape(4, 12), upper_to_full(out[52:].reshape(4, 78))
+
+
+def anchor_eval(p10, x, A):
+    """AnchorLine: R (3), ∂R/∂(x, A) (3×5), Fh, ∂Fh (5), ∂²Fh (5×5), cr."""
+    R, dR, Fh, dFh, HFh, cr = np.zeros(3), np.zeros(15), np.zeros(1), np.zeros(5), np.zeros(25), np.zeros(1)
+    r = lib.orc_anchor_eval(dp(f64(p10)), dp(f64(x)), dp(f64(A)), dp(R), dp(dR), dp(Fh), dp(dFh), dp(HFh), dp(cr))
+    assert r == 0
+    return R, dR.reshape(3, 5), Fh[0], dFh, HFh.reshape(5, 5), cr[0]
 
 
 def beam_gauge(c1, c2, orient2, mat16, X):

@@ -1,0 +1,260 @@
+"""ElementConstraint (src/BasicElements.jl:490-576) and the cost/constraint terms of
+ElementCost (src/BasicElements.jl:117-132), host side, against the oracle (no GPU):
+- the TestElementCost.jl goldens on AnchorLine (test/SomeElements.jl:54-94): the requestables
+  Fh and cr (:90-95, pinning the oracle's AnchorLine), ElementCost's L and ∇L (:28-34) and
+  ElementConstraint's L and ∇L (:81-87), with the product's cost and constraint terms applied
+  to the oracle's Fh as second-order duals over Λ, X, U (λ) and A;
+- over StrainGaugeOnEulerBeam3D (OracleAssemblyDirect's gauge provider): ∂L/∂λ is −gap,
+  −S(λ, gap, γ) or −λ for the modes equal, positive, off; ∂L/∂X is −λ∂gap/∂X (central
+  differences of the oracle's gauge strains); the X-X Hessian block is the Gauss-Newton form
+  by default and the Jacobian of the gradient with exact=True;
+- solve(DirectXUA{0,0,0}) with an equality constraint on a gauge strain recovers the load that
+  produced it (SweepX{0} forward solve), and with mode off leaves λ = 0."""
+import math
+
+import numpy as np
+import pytest
+
+import muscade_amd as mc
+from muscade_amd.directxua import HostLagrangians
+from muscade_amd.hostcost import Dual2, S, compose
+from tests import _oracle as orc
+from tests.helpers import OracleAssembly, OracleAssemblyDirect
+
+ANCHOR = [0., 0., 100., 5., 0., 0., 250., 0., 290., -5e3]   # xₘtop, Δxₘtop, xₘbot, L, buoyancy
+
+
+def anchor_terms(Nu):
+    """Λ (3), X (3), U (Nu), A (2) seeded as Dual2 over Nz = 8 + Nu partials at Λ = 0, X = 1,
+    U = 1, A = 0; R and Fh of the oracle's AnchorLine composed over (X, A)."""
+    Nz = 8 + Nu
+    Λ = [Dual2.seed(np.zeros(1), k, Nz) for k in range(3)]
+    X = [Dual2.seed(np.ones(1), 3 + k, Nz) for k in range(3)]
+    U = [Dual2.seed(np.ones(1), 6 + k, Nz) for k in range(Nu)]
+    A = [Dual2.seed(np.zeros(1), 6 + Nu + k, Nz) for k in range(2)]
+    R, dR, Fh, dFh, HFh, cr = orc.anchor_eval(ANCHOR, [1., 1., 1.], [0., 0.])
+    xa = X + A
+    # the accelerator's revariate{P-1}: R and the requestables to first order
+    Rd = [compose(np.array([R[k]]), dR[k][None], np.zeros((1, 5, 5)), xa) for k in range(3)]
+    Fhd = compose(np.array([Fh]), dFh[None], HFh[None], xa)
+    L = Λ[0] * Rd[0] + Λ[1] * Rd[1] + Λ[2] * Rd[2]
+    return L, {"Fh": Fhd}, U, (Fh, cr)
+
+
+def test_anchorline_requestables_golden():   # TestElementCost.jl:90-95
+    _, _, _, (Fh, cr) = anchor_terms(0)
+    assert Fh == pytest.approx(438959.2060034336, rel=1e-14)
+    assert cr == pytest.approx(87.79184120068672, rel=1e-14)
+
+
+def test_elementcost_golden():   # TestElementCost.jl:28-34
+    L0, eleres, _, _ = anchor_terms(0)
+    p = {"cost": lambda eleres, t: eleres["Fh"] ** 2, "costargs": ()}
+    L = L0 + mc.ElementCost.term(p, eleres, 0.)
+    gold = [-438861.1307445675, 9278.602091074139, 1.8715107899328927e6, -2.322235123921358e10,
+            4.9097753633879846e8, 9.903105530914653e10, -6.735986859485705e12, 3.853703690703298e11]
+    assert L.v[0] == pytest.approx(1.926851845351649e11, rel=1e-13)
+    assert np.allclose(L.g[0], gold, rtol=1e-12, atol=0)
+
+
+def test_elementconstraint_golden():   # TestElementCost.jl:81-87
+    L0, eleres, U, _ = anchor_terms(1)
+    p = {"gap": lambda eleres, t: eleres["Fh"] ** 2, "gargs": (), "mode": mc.equal}
+    L = L0 + mc.ElementConstraint.term(p, eleres, U[0], 0.)
+    gold = [-438861.1307445675, 9278.602091074139, 1.8715107899328927e6, 2.322235123921358e10,
+            -4.9097753633879846e8, -9.903105530914653e10, -1.926851845351649e11, 6.735986859485705e12,
+            -3.853703690703298e11]
+    assert L.v[0] == pytest.approx(-1.926851845351649e11, rel=1e-13)
+    assert np.allclose(L.g[0], gold, rtol=1e-12, atol=0)
+
+
+def test_elementconstraint_doflist():   # TestElementCost.jl:83: the target's dofs, then λ (class U)
+    model = constrained_beam_model(mc.equal)
+    E = model.eletypes[0]
+    inod, dclass, field = E.inod, E.dofclass, E.field
+    assert inod == (1,) * 6 + (2,) * 6 + (1,)
+    assert dclass == ("X",) * 12 + ("U",)
+    assert field == ("t1", "t2", "t3", "r1", "r2", "r3") * 2 + ("λ",)
+
+
+P1 = np.array([[0., 0., .05], [0., .05, 0.]]).T
+D1 = np.array([[1., 0., 0.], [.8, .6, 0.]]).T
+
+
+def gap_fn(eleres, t, c=2e-4):
+    return eleres["ε"][0] * 1e3 + (eleres["εₐₓ"] * 1e3) ** 2 - c
+
+
+def constrained_beam_model(mode, exact=False):
+    from muscade_amd.workloads import base_section
+    model = mc.Model("constrained")
+    mc.addnode(model, [[0., 0., 0.], [4., 0., 0.]])
+    mc.addelement(model, mc.ElementConstraint, [1, 2], λinod=1, λfield="λ", req=mc.request("ε", "εₐₓ"), gap=gap_fn,
+                  mode=mode, exact=exact, ElementType=mc.StrainGaugeOnEulerBeam3D,
+                  elementkwargs=dict(P=P1, D=D1, elementkwargs=dict(mat=base_section(), orient2=np.array([0., 1., 0.]))))
+    return model
+
+
+def host_blocks(model, st):
+    out = OracleAssemblyDirect(0, 0, 0, model, st.dis)
+    host = HostLagrangians(model, st.dis, 0, 0, 0, gauge_provider=out.gauge_derivatives)
+    L1, L2 = host.step(st)
+    nX, nU = model.ndof["X"], model.ndof["U"]
+    H = {k: np.zeros(s) for k, s in ((("X", "X"), (nX, nX)), (("U", "X"), (nU, nX)), (("U", "U"), (nU, nU)))}
+    for (a, i, b, j, r, c, v) in L2:
+        if (a, b) in H and i == 0 and j == 0:
+            np.add.at(H[(a, b)], (r, c), v)
+    return L1, H
+
+
+def gap_at(model, st):
+    """gap from the oracle's gauge values at state.X[0]."""
+    E = model.eletypes[0]
+    xyz = model.coords()
+    idx = st.dis.dis[0].X[0] - 1
+    v, g, _ = orc.beam_gauge2(xyz[0], xyz[1], [0., 1., 0.], E.params[0]["mat"][0], st.X[0][idx])
+    G = E.params[0]["gauges"][0]
+    eps = [G[k, 0] * v[0] + G[k, 1] * v[1] + G[k, 2] * v[2] + G[k, 3] * v[3] for k in range(G.shape[0])]
+    return gap_fn({"ε": eps, "εₐₓ": v[0]}, 0.)
+
+
+def state_for(model, lam, seed=3):
+    st = mc.initialize(model, time=0.)
+    rng = np.random.default_rng(seed)
+    st.X[0][:] = rng.normal(size=model.ndof["X"]) * 1e-3
+    st.U[0][:] = lam
+    return st
+
+
+@pytest.mark.parametrize("mode,γ", [(mc.equal, 0.), (mc.positive, 0.), (mc.positive, 0.3), (mc.off, 0.)])
+def test_beam_constraint_gradient(mode, γ):
+    model = constrained_beam_model(mode)
+    lam = 1.7
+    st = state_for(model, lam)
+    st.SP = {"γ": γ}
+    L1, H = host_blocks(model, st)
+    g0 = gap_at(model, st)
+    gU = L1["U"][0][0]
+    if mode is mc.off:
+        assert gU == pytest.approx(-lam, rel=1e-15)
+        assert np.all(L1["X"][0] == 0.)
+        assert H[("U", "U")][0, 0] == -1.
+        return
+    want = -g0 if mode is mc.equal else -S(lam, g0, γ)
+    assert gU == pytest.approx(want, rel=1e-12, abs=1e-12 * abs(g0))
+    h = 1e-6
+    fd = np.zeros(12)
+    for k in range(12):
+        sp_, sm = st.copy(), st.copy()
+        sp_.X[0][k] += h
+        sm.X[0][k] -= h
+        fd[k] = -lam * (gap_at(model, sp_) - gap_at(model, sm)) / (2 * h)
+    gX = L1["X"][0]
+    assert np.allclose(gX, fd, rtol=1e-6, atol=1e-7 * np.abs(fd).max())
+    # the λ row: −∂gap/∂X (equal) or −λ∂gap/∂X (positive: the Jacobian of S(λ, gap, γ))
+    w = 1. if mode is mc.equal else lam
+    assert np.allclose(H[("U", "X")][0], w * gX / lam, rtol=1e-12, atol=1e-12 * np.abs(gX).max())
+
+
+def test_beam_constraint_hessian():
+    lam = 0.9
+    m_exact = constrained_beam_model(mc.equal, exact=True)
+    st = state_for(m_exact, lam, seed=5)
+    _, He = host_blocks(m_exact, st)
+    h = 1e-5
+    scale = np.abs(He[("X", "X")]).max()
+    for k in range(12):
+        sp_, sm = st.copy(), st.copy()
+        sp_.X[0][k] += h
+        sm.X[0][k] -= h
+        fd = (host_blocks(m_exact, sp_)[0]["X"][0] - host_blocks(m_exact, sm)[0]["X"][0]) / (2 * h)
+        assert np.allclose(He[("X", "X")][:, k], fd, rtol=1e-5, atol=1e-9 * scale), k
+    # Gauss-Newton (default): −λ·Jᵀ ∂²gap J, with ∂²gap/∂εₐₓ² = 2e6 the only second derivative
+    mg = constrained_beam_model(mc.equal)
+    _, Hg = host_blocks(mg, st)
+    E = mg.eletypes[0]
+    xyz = mg.coords()
+    _, g, _ = orc.beam_gauge2(xyz[0], xyz[1], [0., 1., 0.], E.params[0]["mat"][0], st.X[0][:12])
+    want = -lam * 2e6 * np.outer(g[0], g[0])
+    assert np.allclose(Hg[("X", "X")], want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+
+
+def gauge_cantilever_constrained(target=None, F=None, mode=mc.equal, nel=4, cu=1e-12):
+    from muscade_amd.workloads import base_section
+    model = mc.Model("gauged")
+    x = np.linspace(0., 4., nel + 1)
+    mc.addnode(model, np.stack([x, 0 * x, 0 * x], axis=1))
+    sec = base_section()
+    o2 = np.array([0., 1., 0.])
+    kw = dict(P=P1, D=D1, elementkwargs=dict(mat=sec, orient2=o2))
+    if target is None:
+        mc.addelement(model, mc.StrainGaugeOnEulerBeam3D, [1, 2], **kw)
+    else:
+        def gap(eleres, t, target=target):
+            return (eleres["ε"][0] - target) * 1e4
+        mc.addelement(model, mc.ElementConstraint, [1, 2], λinod=1, λfield="λ", req=mc.request("ε"), gap=gap,
+                      mode=mode, ElementType=mc.StrainGaugeOnEulerBeam3D, elementkwargs=kw)
+    mc.addelement(model, mc.EulerBeam3D, np.stack([np.arange(2, nel + 1), np.arange(3, nel + 2)], axis=1), mat=sec,
+                  orient2=o2)
+    for f in ("t1", "t2", "t3", "r1", "r2", "r3"):
+        mc.addelement(model, mc.Hold, [1], field=f)
+    if F is not None:
+        mc.addelement(model, mc.DofLoad, [nel + 1], field="t3", value=lambda t, F=F: F)
+    else:
+        mc.addelement(model, mc.SingleUdof, [nel + 1], Xfield="t3", Ufield="ut3", cost=lambda u, t: cu * u * u)
+    return model
+
+
+def gauge_target(F, Assembly=OracleAssembly):
+    fwd = gauge_cantilever_constrained(F=F)
+    s = mc.solve(mc.SweepX[0], initialstate=mc.initialize(fwd, time=0.), time=[1.], maxΔx=1e-9, verbose=False,
+                 catcherror=False, Assembly=Assembly)[-1]
+    E = fwd.eletypes[0]
+    xyz = fwd.coords()
+    idx = s.dis.dis[0].X[0] - 1
+    v, _, _ = orc.beam_gauge2(xyz[0], xyz[1], [0., 1., 0.], E.params[0]["mat"][0], s.X[0][idx])
+    G = E.params[0]["gauges"][0]
+    return G[0, 0] * v[0] + G[0, 1] * v[1] + G[0, 2] * v[2] + G[0, 3] * v[3]
+
+
+SOLVE_KW = dict(time=[np.array([1.])], maxΔx=1e-9, maxΔλ=1e-3, maxΔu=1e-6, maxiter=30, verbose=False,
+                catcherror=False)
+
+
+def test_constrained_load_identification():
+    F = 3e4
+    target = gauge_target(F)
+    assert abs(target) > 1e-5
+    inv = gauge_cantilever_constrained(target=target)
+    out = mc.solve(mc.DirectXUA[0, 0, 0], initialstate=[mc.initialize(inv, time=0.)], Assembly=OracleAssemblyDirect,
+                   **SOLVE_KW)
+    st = out[0][0]
+    iu = inv.getdofID("U", "ut3")[0] - 1
+    il = inv.getdofID("U", "λ")[0] - 1
+    assert abs(st.U[0][iu] - F) <= 1e-6 * F, (st.U[0][iu], F)
+    # stationarity in u: 2·cu·u = λ·∂gap/∂u with the load's lever on the gauge: λ ≠ 0
+    assert abs(st.U[0][il]) > 0.
+    assert math.isfinite(st.U[0][il])
+
+
+def test_off_constraint_leaves_lambda_zero():
+    target = gauge_target(3e4)
+    inv = gauge_cantilever_constrained(target=target, mode=mc.off)
+    out = mc.solve(mc.DirectXUA[0, 0, 0], initialstate=[mc.initialize(inv, time=0.)], Assembly=OracleAssemblyDirect,
+                   **SOLVE_KW)
+    st = out[0][0]
+    assert st.U[0][inv.getdofID("U", "λ")[0] - 1] == 0.
+    assert abs(st.U[0][inv.getdofID("U", "ut3")[0] - 1]) <= 1e-6
+
+
+def test_sweepx_ignores_the_constraint_term():
+    """In SweepX the constraint has no Λ, so the constrained model's static solution is the
+    plain strain-gauged model's (λ is a U-dof, an input)."""
+    F = 3e4
+    plain = gauge_cantilever_constrained(F=F)
+    cons = gauge_cantilever_constrained(F=F, target=1e-4)
+    assert cons.ndof["X"] == plain.ndof["X"]
+    kw = dict(time=[1.], maxΔx=1e-9, verbose=False, catcherror=False, Assembly=OracleAssembly)
+    sp_ = mc.solve(mc.SweepX[0], initialstate=mc.initialize(plain, time=0.), **kw)[-1]
+    sc = mc.solve(mc.SweepX[0], initialstate=mc.initialize(cons, time=0.), **kw)[-1]
+    assert np.array_equal(sc.X[0], sp_.X[0])
