@@ -388,7 +388,8 @@ def directxua_bench(args):
     """solve(DirectXUA{2,0,0}) of a load-identification problem: a 100-beam curved cantilever over
     40 steps, a U-load at the tip and a cost on the tip's deviation from a target motion.
     Residual elements per step on the GPU, one launch per step or all steps in one launch; the
-    all-steps system assembled on the host or on the device; its LU on the host."""
+    all-steps system assembled on the host or on the device; its LU on the host (SciPy) or on the
+    device (dense block LU of the steps)."""
     import muscade_amd as mc
     from muscade_amd.workloads import chain_section
     nel, nstep = 100, 40
@@ -406,15 +407,22 @@ def directxua_bench(args):
     time_ = np.arange(1, nstep + 1) * 0.05
     res = {"workload": f"load identification, {nel} EulerBeam3D x {nstep} steps, DirectXUA{{2,0,0}}",
            "unknowns": nstep * (2 * model.ndof["X"] + model.ndof["U"])}
-    for name, mode, big in (("launch_per_step", False, "host"), ("steps_in_one_launch", True, "host"),
-                            ("steps_in_one_launch_big_system_on_device", True, "device")):
+    finals = {}
+    for name, mode, big, lin in (("launch_per_step", False, "host", "host"),
+                                 ("steps_in_one_launch", True, "host", "host"),
+                                 ("steps_in_one_launch_big_system_on_device", True, "device", "host"),
+                                 ("steps_in_one_launch_big_system_and_solve_on_device", True, "device", "device")):
         st = {}
         t0 = time.perf_counter()
-        mc.solve(mc.DirectXUA[2, 0, 0], initialstate=[st0], time=[time_], maxiter=30, verbose=False,
-                 catcherror=False, batch_steps=mode, big=big, stats=st)   # the reference's default tolerances (1e-5)
+        out = mc.solve(mc.DirectXUA[2, 0, 0], initialstate=[st0], time=[time_], maxiter=30, verbose=False,
+                       catcherror=False, batch_steps=mode, big=big, linsolve=lin, stats=st)   # tolerances 1e-5
         st["total_s"] = time.perf_counter() - t0
+        finals[name] = np.array([s_.U[0][0] for s_ in out[0]])
         res[name] = st
         log(f"directxua ({name}): {st}")
+    ref = finals["launch_per_step"]
+    res["max_rel_diff_of_identified_loads"] = {k: float(np.abs(v - ref).max() / np.abs(ref).max())
+                                               for k, v in finals.items() if k != "launch_per_step"}
     return res
 
 

@@ -241,11 +241,25 @@ int mh_get_X(mh_ctx* ctx, int ider, double* x);
  * w (the stencil weights and Δt powers). It sorts them into a CSC pattern (nnz returned,
  * mh_big_get_pattern: 1-based) and the contribution maps. mh_big_assemble, after each
  * mh_assemble_direct, sums each nz's contributions on the device in contribution order and
- * copies the values out. */
+ * copies the values out (nzval = NULL: the values stay on the device). */
 int mh_big_prepare(mh_ctx* ctx, int64_t nv, int64_t np, const int64_t* rows, const int64_t* cols, const int64_t* src,
                    const double* w, int64_t nhost, int64_t* nnz);
 int mh_big_get_pattern(mh_ctx* ctx, int64_t* colptr, int64_t* rowval);
 int mh_big_assemble(mh_ctx* ctx, const double* hostvals, double* nzval);
+
+/* The LU and back substitution of the all-steps system on the device (in place of
+ * `lu(Lvv)`, `Δv = Lvv\Lv` on the host, src/DirectXUA.jl:489-490). The unknowns are
+ * nstep steps of per_step unknowns each (Λ, X, U of one step), then na border unknowns (the
+ * A-dofs), so that nv = per_step·nstep + na. Finite differences couple steps at most a few
+ * apart: mh_big_solve_prepare reads that width g off the pattern and lays the matrix out as
+ * dense blocks of g steps (order m = g·per_step, returned with their count nb), block
+ * tridiagonal and bordered by the A rows and columns. MH_ERR_SIZE when the blocks do not fit
+ * in the device memory. mh_big_solve factors the values of the last mh_big_assemble (which
+ * may then pass nzval = NULL: nothing leaves the device) by block LU, partial pivoting
+ * inside the diagonal blocks, and returns x (nv) for the right-hand side rhs (nv).
+ * MH_ERR_SOLVE: a zero pivot or a non-finite solution. */
+int mh_big_solve_prepare(mh_ctx* ctx, int64_t per_step, int64_t nstep, int64_t na, int64_t* m, int64_t* nb);
+int mh_big_solve(mh_ctx* ctx, const double* rhs, double* x);
 
 /* ---- measurement -----------------------------------------------------------
  * HIP-event times (ms) of the last assembly, measured on the context's stream:

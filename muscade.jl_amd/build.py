@@ -12,8 +12,12 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-I", os.path.join(HERE, "..", "include")]
 
 
+# rocSOLVER and rocBLAS: the dense block LU of DirectXUA's all-steps system (mh_big_solve)
+LIBS = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
+
+
 def build(verbose=False):
-    cmd = [HIPCC] + FLAGS + ["-o", OUT + ".tmp"] + SRC
+    cmd = [HIPCC] + FLAGS + ["-o", OUT + ".tmp"] + SRC + LIBS
     if verbose:
         print(" ".join(cmd), file=sys.stderr)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -21,6 +21,8 @@
 // (src/Assemble.jl:470-505,572-586). No atomics touch any assembled value.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -163,6 +165,26 @@ struct Ctx {
     double* d_bw = nullptr;      // per contribution, sorted: its weight (finite-difference factors)
     double* d_bhost = nullptr;   // bnhost host-provided values
     double* d_bval = nullptr;    // bnnz assembled values
+    // its dense block solve (mh_big_solve_prepare / mh_big_solve): g steps per diagonal block
+    // of order m, nb blocks, na border unknowns (the A-dofs)
+    int64_t sps = 0, snst = 0, sna = 0, sg = 0, sm = 0, snb = 0, stot = 0, snpad = 0;
+    int64_t so[4] = {0, 0, 0, 0};   // offsets of the band, W, C and E in d_sbuf
+    int64_t* d_sslot = nullptr;  // per nz of the big pattern: its position in d_sbuf
+    int32_t* d_srowptr = nullptr;  // the pattern by rows (residual of the refinement steps)
+    int32_t* d_scolidx = nullptr;
+    int32_t* d_sperm = nullptr;    // row-ordered nz → its index in the column-ordered values
+    double* d_sb = nullptr;        // the right-hand side, kept
+    double* d_sx = nullptr;        // the solution being refined
+    int32_t* d_scol = nullptr;     // column of each nz (column-ordered)
+    double* d_ssr = nullptr;       // row and column equilibration scales
+    double* d_ssc = nullptr;
+    int64_t* d_spad = nullptr;   // padding diagonal positions of the last block
+    double* d_sbuf = nullptr;
+    double* d_srhs = nullptr;
+    int32_t* d_sipiv = nullptr;
+    int32_t* d_sinfo = nullptr;
+    int32_t* d_sprm = nullptr;     // per block column: the row permutation of its interchanges
+    rocblas_handle blas = nullptr;
 };
 
 #define HIPCHK(ctx, call)                                                                       \
@@ -190,7 +212,14 @@ void free_chain(Ctx* c) {
     for (auto& e : c->cev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
     c->cB = 0; c->cnb = c->cnp = c->cnd = 0;
 }
+void free_bsolve(Ctx* c) {
+    dfree(c->d_sslot); dfree(c->d_spad); dfree(c->d_sbuf); dfree(c->d_srhs); dfree(c->d_sipiv); dfree(c->d_sinfo);
+    dfree(c->d_srowptr); dfree(c->d_scolidx); dfree(c->d_sperm); dfree(c->d_sb); dfree(c->d_sx);
+    dfree(c->d_scol); dfree(c->d_ssr); dfree(c->d_ssc); dfree(c->d_sprm);
+    c->sps = c->snst = c->sna = c->sg = c->sm = c->snb = c->stot = c->snpad = 0;
+}
 void free_big(Ctx* c) {
+    free_bsolve(c);
     dfree(c->d_bcolptr); dfree(c->d_browval); dfree(c->d_bseg); dfree(c->d_bsrc); dfree(c->d_bw); dfree(c->d_bhost);
     dfree(c->d_bval);
     c->bnv = c->bnp = c->bnnz = c->bnhost = c->bndev = 0;
@@ -1616,6 +1645,7 @@ int mh_ctx_destroy(mh_ctx* c) {
     dfree(c->d_flag);
     if (c->h_flag) (void)hipHostFree(c->h_flag);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+    if (c->blas) (void)rocblas_destroy_handle(c->blas);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MH_OK;
@@ -2870,7 +2900,7 @@ int mh_big_get_pattern(mh_ctx* c, int64_t* colptr, int64_t* rowval) {
 }
 
 int mh_big_assemble(mh_ctx* c, const double* hostvals, double* nzval) {
-    if (!c || !c->direct || !c->d_bcolptr || (c->bnhost && !hostvals) || (c->bnnz && !nzval)) return MH_ERR_ARG;
+    if (!c || !c->direct || !c->d_bcolptr || (c->bnhost && !hostvals)) return MH_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
     if (c->bnhost) HIPCHK(c, hipMemcpyAsync(c->d_bhost, hostvals, sizeof(double) * c->bnhost, hipMemcpyHostToDevice, s));
@@ -2878,9 +2908,333 @@ int mh_big_assemble(mh_ctx* c, const double* hostvals, double* nzval) {
         hipLaunchKernelGGL(big_scatter, dim3(grid(c->bnnz)), dim3(kBlock), 0, s, c->bnnz, c->d_bseg, c->d_bsrc, c->d_bw,
                            c->d_dL2, c->bndev, c->d_bhost, c->d_bval);
         if (int r = check_launch(c, "big scatter")) return r;
-        HIPCHK(c, hipMemcpyAsync(nzval, c->d_bval, sizeof(double) * c->bnnz, hipMemcpyDeviceToHost, s));
+        if (nzval) HIPCHK(c, hipMemcpyAsync(nzval, c->d_bval, sizeof(double) * c->bnnz, hipMemcpyDeviceToHost, s));
     }
     HIPCHK(c, hipStreamSynchronize(s));
+    return MH_OK;
+}
+
+#define BLASCHK(ctx, call)                                                                      \
+    do {                                                                                        \
+        rocblas_status st_ = (call);                                                            \
+        if (st_ != rocblas_status_success) {                                                    \
+            (ctx)->err = std::string(#call) + ": " + rocblas_status_to_string(st_);              \
+            return MH_ERR_HIP;                                                                  \
+        }                                                                                       \
+    } while (0)
+
+// Dense block storage of the all-steps system, LAPACK band storage at block level: block column j
+// (m columns) holds block rows j−2 (fill from the row interchanges), j−1, j, j+1 as one 4m × m
+// column-major panel, so that the band is one 4m × (nb·m) matrix with leading dimension 4m and
+// block (i, j) starts at row (i − j + 2)·m. W (nb·m × (na+1), leading dimension nb·m) holds the
+// border columns of the step rows and the right-hand side (later the solution); C (na × nb·m) the
+// border rows; E (na × (na+1)) the corner and the border's right-hand side.
+__device__ __forceinline__ int64_t bsolve_vpos(int64_t i, int64_t nsp, int64_t nbm, int64_t na, int64_t oW,
+                                               int64_t oE) {
+    return i < nsp ? oW + nbm * na + i : oE + na * na + (i - nsp);
+}
+// The row permutation of the interchanges piv[0..n) (1-based) over `rows` rows: row i of the
+// result is row perm[i] of the original. One workgroup, in LDS.
+__global__ __launch_bounds__(256) void piv_to_perm(int n, int rows, const int32_t* __restrict__ piv,
+                                                   int32_t* __restrict__ perm) {
+    extern __shared__ int32_t shp[];
+    int32_t* pr = shp;
+    int32_t* pv = shp + rows;
+    for (int i = threadIdx.x; i < rows; i += 256) pr[i] = i;
+    for (int i = threadIdx.x; i < n; i += 256) pv[i] = piv[i] - 1;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 0; i < n; ++i) {
+            const int p = pv[i];
+            if (p != i) { const int32_t x = pr[i]; pr[i] = pr[p]; pr[p] = x; }
+        }
+    __syncthreads();
+    for (int i = threadIdx.x; i < rows; i += 256) perm[i] = pr[i];
+}
+// Rows of each column permuted: a[i] ← a[perm[i]], one workgroup per column through LDS.
+__global__ __launch_bounds__(256) void perm_rows(int rows, double* __restrict__ A, int lda, const int32_t* __restrict__ perm) {
+    extern __shared__ double shc[];
+    double* a = A + (int64_t)blockIdx.x * lda;
+    for (int i = threadIdx.x; i < rows; i += 256) shc[i] = a[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < rows; i += 256) a[i] = shc[perm[i]];
+}
+
+// Equilibration (as SuperLU's default): row scales sr = 1/max|row|, then column scales
+// sc = 1/max|sr·column|; the dense factors hold diag(sr)·M·diag(sc).
+__global__ __launch_bounds__(kBlock) void bsolve_rowscale(int64_t nv, const int32_t* __restrict__ rowptr,
+                                                          const int32_t* __restrict__ perm, const double* __restrict__ val,
+                                                          double* __restrict__ sr) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nv) return;
+    double mx = 0.;
+    for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) mx = fmax(mx, fabs(val[perm[p]]));
+    sr[i] = mx > 0. ? 1. / mx : 1.;
+}
+__global__ __launch_bounds__(kBlock) void bsolve_colscale(int64_t nv, const int32_t* __restrict__ colptr,
+                                                          const int32_t* __restrict__ rowval, const double* __restrict__ val,
+                                                          const double* __restrict__ sr, double* __restrict__ sc) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j >= nv) return;
+    double mx = 0.;
+    for (int32_t p = colptr[j]; p < colptr[j + 1]; ++p) mx = fmax(mx, fabs(val[p] * sr[rowval[p]]));
+    sc[j] = mx > 0. ? 1. / mx : 1.;
+}
+__global__ __launch_bounds__(kBlock) void bsolve_fill(int64_t nnz, const int64_t* __restrict__ slot,
+                                                      const double* __restrict__ val, const int32_t* __restrict__ rowval,
+                                                      const int32_t* __restrict__ colv, const double* __restrict__ sr,
+                                                      const double* __restrict__ sc, double* __restrict__ buf) {
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k < nnz) buf[slot[k]] = (val[k] * sr[rowval[k]]) * sc[colv[k]];
+}
+__global__ __launch_bounds__(kBlock) void bsolve_ones(int64_t n, const int64_t* __restrict__ pos, double* __restrict__ buf) {
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k < n) buf[pos[k]] = 1.;
+}
+__global__ __launch_bounds__(kBlock) void bsolve_rhs(int64_t nv, int64_t nsp, int64_t nbm, int64_t na, int64_t oW,
+                                                     int64_t oE, const double* __restrict__ rhs,
+                                                     const double* __restrict__ sr, double* __restrict__ buf) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < nv) buf[bsolve_vpos(i, nsp, nbm, na, oW, oE)] = rhs[i] * sr[i];
+}
+// r = b − M x over the pattern by rows, each row summed in column order (deterministic)
+__global__ __launch_bounds__(kBlock) void bsolve_resid(int64_t nv, const int32_t* __restrict__ rowptr,
+                                                       const int32_t* __restrict__ colidx, const int32_t* __restrict__ perm,
+                                                       const double* __restrict__ val, const double* __restrict__ x,
+                                                       const double* __restrict__ b, double* __restrict__ r) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nv) return;
+    double acc = 0.;
+    for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) acc += val[perm[p]] * x[colidx[p]];
+    r[i] = b[i] - acc;
+}
+__global__ __launch_bounds__(kBlock) void bsolve_add(int64_t nv, const double* __restrict__ d, double* __restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < nv) x[i] += d[i];
+}
+__global__ __launch_bounds__(kBlock) void bsolve_sol(int64_t nv, int64_t nsp, int64_t nbm, int64_t na, int64_t oW,
+                                                     int64_t oE, const double* __restrict__ buf,
+                                                     const double* __restrict__ sc, double* __restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < nv) x[i] = buf[bsolve_vpos(i, nsp, nbm, na, oW, oE)] * sc[i];
+}
+
+int mh_big_solve_prepare(mh_ctx* c, int64_t per_step, int64_t nstep, int64_t na, int64_t* m_out, int64_t* nb_out) {
+    if (!c || !c->d_bcolptr || per_step < 1 || nstep < 1 || na < 0 || per_step * nstep + na != c->bnv) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipStreamSynchronize(s));
+    free_bsolve(c);
+    const int64_t nv = c->bnv, nnz = c->bnnz, nsp = per_step * nstep;
+    std::vector<int32_t> cp(nv + 1), rv(std::max<int64_t>(nnz, 1));
+    HIPCHK(c, hipMemcpy(cp.data(), c->d_bcolptr, sizeof(int32_t) * (nv + 1), hipMemcpyDeviceToHost));
+    if (nnz) HIPCHK(c, hipMemcpy(rv.data(), c->d_browval, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+    // the widest coupling between steps fixes how many steps one diagonal block holds, so that the
+    // blocks are tridiagonal (finite differences of order OX couple steps up to two apart)
+    int64_t bw = 0;
+    for (int64_t j = 0; j < nsp; ++j)
+        for (int32_t p = cp[j]; p < cp[j + 1]; ++p) {
+            const int64_t r = rv[p];
+            if (r < nsp) bw = std::max<int64_t>(bw, std::llabs(r / per_step - j / per_step));
+        }
+    const int64_t g = std::max<int64_t>(bw, 1), m = g * per_step, nb = (nstep + g - 1) / g, nbm = nb * m;
+    // the row permutations go through LDS: 2m doubles per column, 3m indices
+    if (4 * m >= INT32_MAX / 2 || nbm >= INT32_MAX / 2 || na >= 65536 || 2 * m * 8 > 64000) {
+        c->err = "diagonal blocks too large for a dense solve"; return MH_ERR_SIZE;
+    }
+    int64_t* o = c->so;
+    o[0] = 0; o[1] = 4 * m * nbm; o[2] = o[1] + nbm * (na + 1); o[3] = o[2] + na * nbm;
+    const int64_t tot = o[3] + na * (na + 1);
+    size_t fr = 0, tt = 0;
+    HIPCHK(c, hipMemGetInfo(&fr, &tt));
+    if ((double)tot * 8. > 0.7 * (double)fr) {
+        c->err = "the dense block storage (" + std::to_string(tot * 8 / (1 << 20)) + " MiB) exceeds the free device memory";
+        return MH_ERR_SIZE;
+    }
+    std::vector<int64_t> slot(std::max<int64_t>(nnz, 1));
+    for (int64_t j = 0; j < nv; ++j)
+        for (int32_t p = cp[j]; p < cp[j + 1]; ++p) {
+            const int64_t r = rv[p];
+            int64_t q;
+            if (r < nsp && j < nsp) {
+                const int64_t kr = (r / per_step) / g, kc = (j / per_step) / g;
+                q = o[0] + j * 4 * m + (kr - kc + 2) * m + (r - kr * m);
+            } else if (r < nsp) {
+                q = o[1] + r + nbm * (j - nsp);
+            } else if (j < nsp) {
+                q = o[2] + (r - nsp) + na * j;
+            } else {
+                q = o[3] + (r - nsp) + na * (j - nsp);
+            }
+            slot[p] = q;
+        }
+    std::vector<int64_t> pad;   // identity rows for the unused rows of the last block
+    for (int64_t i = nsp; i < nbm; ++i) pad.push_back(o[0] + i * 4 * m + 2 * m + (i - (nb - 1) * m));
+    HIPCHK(c, hipMalloc(&c->d_sslot, sizeof(int64_t) * slot.size()));
+    HIPCHK(c, hipMalloc(&c->d_spad, sizeof(int64_t) * std::max<size_t>(pad.size(), 1)));
+    HIPCHK(c, hipMalloc(&c->d_sbuf, sizeof(double) * tot));
+    HIPCHK(c, hipMalloc(&c->d_srhs, sizeof(double) * nv));
+    HIPCHK(c, hipMalloc(&c->d_sipiv, sizeof(int32_t) * (nbm + std::max<int64_t>(na, 1))));
+    HIPCHK(c, hipMalloc(&c->d_sinfo, sizeof(int32_t) * (nb + 1)));
+    HIPCHK(c, hipMalloc(&c->d_sprm, sizeof(int32_t) * nb * 2 * m));
+    HIPCHK(c, hipMemcpy(c->d_sslot, slot.data(), sizeof(int64_t) * slot.size(), hipMemcpyHostToDevice));
+    {   // the pattern by rows, for the residuals of the refinement steps
+        std::vector<int32_t> rp(nv + 1, 0), ci(std::max<int64_t>(nnz, 1)), pm(std::max<int64_t>(nnz, 1));
+        for (int64_t p = 0; p < nnz; ++p) ++rp[rv[p] + 1];
+        for (int64_t i = 0; i < nv; ++i) rp[i + 1] += rp[i];
+        std::vector<int32_t> next(rp.begin(), rp.end() - 1);
+        for (int64_t j = 0; j < nv; ++j)
+            for (int32_t p = cp[j]; p < cp[j + 1]; ++p) { const int32_t q = next[rv[p]]++; ci[q] = (int32_t)j; pm[q] = p; }
+        HIPCHK(c, hipMalloc(&c->d_srowptr, sizeof(int32_t) * (nv + 1)));
+        HIPCHK(c, hipMalloc(&c->d_scolidx, sizeof(int32_t) * ci.size()));
+        HIPCHK(c, hipMalloc(&c->d_sperm, sizeof(int32_t) * pm.size()));
+        HIPCHK(c, hipMalloc(&c->d_sb, sizeof(double) * nv));
+        HIPCHK(c, hipMalloc(&c->d_sx, sizeof(double) * nv));
+        HIPCHK(c, hipMalloc(&c->d_ssr, sizeof(double) * nv));
+        HIPCHK(c, hipMalloc(&c->d_ssc, sizeof(double) * nv));
+        std::vector<int32_t> cv(std::max<int64_t>(nnz, 1));
+        for (int64_t j = 0; j < nv; ++j)
+            for (int32_t p = cp[j]; p < cp[j + 1]; ++p) cv[p] = (int32_t)j;
+        HIPCHK(c, hipMalloc(&c->d_scol, sizeof(int32_t) * cv.size()));
+        HIPCHK(c, hipMemcpy(c->d_scol, cv.data(), sizeof(int32_t) * cv.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_srowptr, rp.data(), sizeof(int32_t) * (nv + 1), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_scolidx, ci.data(), sizeof(int32_t) * ci.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_sperm, pm.data(), sizeof(int32_t) * pm.size(), hipMemcpyHostToDevice));
+    }
+    if (!pad.empty()) HIPCHK(c, hipMemcpy(c->d_spad, pad.data(), sizeof(int64_t) * pad.size(), hipMemcpyHostToDevice));
+    if (!c->blas) BLASCHK(c, rocblas_create_handle(&c->blas));
+    BLASCHK(c, rocblas_set_stream(c->blas, s));
+    c->sps = per_step; c->snst = nstep; c->sna = na; c->sg = g; c->sm = m; c->snb = nb; c->stot = tot;
+    c->snpad = (int64_t)pad.size();
+    if (m_out) *m_out = m;
+    if (nb_out) *nb_out = nb;
+    return MH_OK;
+}
+
+// Banded block LU with partial pivoting (LAPACK gbtrf at block level) of the step part T: block
+// column k (rocSOLVER's getrf) pivots over block rows k and k+1, whose row interchanges reach
+// block columns k+1 and k+2 (the fill) and the columns of W as one composed permutation. The border is taken by bordering: W ← T⁻¹[B | b], then
+// the Schur complement S = E − C T⁻¹B (dense, na × na) gives a = S⁻¹(b_A − C T⁻¹b) and the step
+// unknowns are T⁻¹b − T⁻¹B a. The system is equilibrated first (its unknowns Λ, X, U, A differ
+// in scale by many orders), and one step of iterative refinement on the residual of the sparse
+// system follows.
+int mh_big_solve(mh_ctx* c, const double* rhs, double* x) {
+    if (!c || !c->d_sbuf || !rhs || !x) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    rocblas_handle h = c->blas;
+    BLASCHK(c, rocblas_set_stream(h, s));
+    const int64_t nv = c->bnv, nsp = c->sps * c->snst, m = c->sm, nb = c->snb, na = c->sna, nbm = nb * m;
+    const int64_t* o = c->so;
+    double* B = c->d_sbuf;
+    const int im = (int)m, ina = (int)na, iw = (int)(na + 1), l4 = (int)(4 * m), lw = (int)nbm, inbm = (int)nbm;
+    const double one = 1., mone = -1.;
+    const rocblas_operation N = rocblas_operation_none;
+    hipLaunchKernelGGL(chain::fill_f64, dim3((unsigned)((c->stot + 255) / 256)), dim3(256), 0, s, c->stot, B, 0.);
+    hipLaunchKernelGGL(bsolve_rowscale, dim3(grid(nv)), dim3(kBlock), 0, s, nv, c->d_srowptr, c->d_sperm, c->d_bval,
+                       c->d_ssr);
+    hipLaunchKernelGGL(bsolve_colscale, dim3(grid(nv)), dim3(kBlock), 0, s, nv, c->d_bcolptr, c->d_browval, c->d_bval,
+                       c->d_ssr, c->d_ssc);
+    if (c->bnnz) hipLaunchKernelGGL(bsolve_fill, dim3(grid(c->bnnz)), dim3(kBlock), 0, s, c->bnnz, c->d_sslot, c->d_bval,
+                                    c->d_browval, c->d_scol, c->d_ssr, c->d_ssc, B);
+    if (c->snpad) hipLaunchKernelGGL(bsolve_ones, dim3(grid(c->snpad)), dim3(kBlock), 0, s, c->snpad, c->d_spad, B);
+    HIPCHK(c, hipMemcpyAsync(c->d_sb, rhs, sizeof(double) * nv, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(bsolve_rhs, dim3(grid(nv)), dim3(kBlock), 0, s, nv, nsp, nbm, na, o[1], o[3], c->d_sb, c->d_ssr,
+                       B);
+    hipLaunchKernelGGL(chain::fill_i32, dim3((unsigned)((nb + 1 + 255) / 256)), dim3(256), 0, s, nb + 1, c->d_sinfo, 0);
+    if (int r = check_launch(c, "big solve fill")) return r;
+    auto col = [&](int64_t j) { return B + o[0] + j * 4 * m * m; };   // block column j, row 0 (block row j−2)
+    double* W = B + o[1];
+    double* C = B + o[2];
+    double* E = B + o[3];
+    double* z = W + nbm * na;   // the step part of the right-hand side / solution
+    double* a = E + na * na;    // the border part
+    // forward elimination of `ncol` columns of W from its column `c0` (the factor sweep does the
+    // factorization at the same time, the refinement steps only the last column)
+    auto permute = [&](double* A, int lda, int ncol, int rows, const int32_t* prm) {
+        hipLaunchKernelGGL(perm_rows, dim3(ncol), dim3(256), sizeof(double) * rows, s, rows, A, lda, prm);
+    };
+    auto forward = [&](bool factor, int c0, int ncol) -> int {
+        for (int64_t k = 0; k < nb; ++k) {
+            double* P = col(k) + 2 * m;   // D_k, then L_{k+1} below it
+            int32_t* piv = c->d_sipiv + k * m;
+            int32_t* prm = c->d_sprm + k * 2 * m;
+            const bool two = k + 1 < nb;
+            const int Mr = two ? 2 * im : im;
+            if (factor) {
+                BLASCHK(c, rocsolver_dgetrf(h, Mr, im, P, l4, piv, c->d_sinfo + k));
+                hipLaunchKernelGGL(piv_to_perm, dim3(1), dim3(256), sizeof(int32_t) * (Mr + im), s, im, Mr, piv, prm);
+                for (int jj = 1; jj <= 2; ++jj) {
+                    if (k + jj >= nb) break;
+                    double* T = col(k + jj) + (2 - jj) * m;   // block rows k, k+1 of column k+jj
+                    permute(T, l4, im, Mr, prm);
+                    BLASCHK(c, rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, N, rocblas_diagonal_unit, im, im,
+                                             &one, P, l4, T, l4));
+                    BLASCHK(c, rocblas_dgemm(h, N, N, im, im, im, &mone, P + m, l4, T, l4, &one, T + m, l4));
+                }
+            }
+            double* Wk = W + (int64_t)c0 * nbm + k * m;
+            permute(Wk, lw, ncol, Mr, prm);
+            BLASCHK(c, rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, N, rocblas_diagonal_unit, im, ncol, &one, P,
+                                     l4, Wk, lw));
+            if (two) BLASCHK(c, rocblas_dgemm(h, N, N, im, ncol, im, &mone, P + m, l4, Wk, lw, &one, Wk + m, lw));
+        }
+        return MH_OK;
+    };
+    // X_k = U_kk⁻¹ (W_k − U_{k,k+1} X_{k+1} − U_{k,k+2} X_{k+2}) over the same columns
+    auto backward = [&](int c0, int ncol) -> int {
+        for (int64_t k = nb - 1; k >= 0; --k) {
+            double* Xk = W + (int64_t)c0 * nbm + k * m;
+            for (int jj = 1; jj <= 2; ++jj) {
+                if (k + jj >= nb) break;
+                BLASCHK(c, rocblas_dgemm(h, N, N, im, ncol, im, &mone, col(k + jj) + (2 - jj) * m, l4, Xk + jj * m, lw,
+                                         &one, Xk, lw));
+            }
+            BLASCHK(c, rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, N, rocblas_diagonal_non_unit, im, ncol,
+                                     &one, col(k) + 2 * m, l4, Xk, lw));
+        }
+        return MH_OK;
+    };
+    if (int r = forward(true, 0, iw)) return r;
+    if (int r = backward(0, iw)) return r;
+    if (na) {
+        BLASCHK(c, rocblas_dgemm(h, N, N, ina, iw, inbm, &mone, C, ina, W, lw, &one, E, ina));   // [S | b_A − C z]
+        BLASCHK(c, rocsolver_dgetrf(h, ina, ina, E, ina, c->d_sipiv + nbm, c->d_sinfo + nb));
+        BLASCHK(c, rocsolver_dgetrs(h, N, ina, 1, E, ina, c->d_sipiv + nbm, a, ina));
+        BLASCHK(c, rocblas_dgemv(h, N, inbm, ina, &mone, W, lw, a, 1, &one, z, 1));
+    } else {
+        HIPCHK(c, hipMemsetAsync(c->d_sinfo + nb, 0, sizeof(int32_t), s));
+    }
+    hipLaunchKernelGGL(bsolve_sol, dim3(grid(nv)), dim3(kBlock), 0, s, nv, nsp, nbm, na, o[1], o[3], B, c->d_ssc, c->d_sx);
+    for (int it = 0; it < 1; ++it) {
+        hipLaunchKernelGGL(bsolve_resid, dim3(grid(nv)), dim3(kBlock), 0, s, nv, c->d_srowptr, c->d_scolidx,
+                           c->d_sperm, c->d_bval, c->d_sx, c->d_sb, c->d_srhs);
+        hipLaunchKernelGGL(bsolve_rhs, dim3(grid(nv)), dim3(kBlock), 0, s, nv, nsp, nbm, na, o[1], o[3], c->d_srhs,
+                           c->d_ssr, B);
+        if (int r = forward(false, ina, 1)) return r;
+        if (int r = backward(ina, 1)) return r;
+        if (na) {
+            BLASCHK(c, rocblas_dgemv(h, N, ina, inbm, &mone, C, ina, z, 1, &one, a, 1));
+            BLASCHK(c, rocsolver_dgetrs(h, N, ina, 1, E, ina, c->d_sipiv + nbm, a, ina));
+            BLASCHK(c, rocblas_dgemv(h, N, inbm, ina, &mone, W, lw, a, 1, &one, z, 1));
+        }
+        hipLaunchKernelGGL(bsolve_sol, dim3(grid(nv)), dim3(kBlock), 0, s, nv, nsp, nbm, na, o[1], o[3], B, c->d_ssc,
+                           c->d_srhs);
+        hipLaunchKernelGGL(bsolve_add, dim3(grid(nv)), dim3(kBlock), 0, s, nv, c->d_srhs, c->d_sx);
+    }
+    if (int r = check_launch(c, "big solve gather")) return r;
+    std::vector<int32_t> info(nb + 1);
+    HIPCHK(c, hipMemcpyAsync(x, c->d_sx, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(info.data(), c->d_sinfo, sizeof(int32_t) * (nb + 1), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    for (int64_t k = 0; k <= nb; ++k)
+        if (info[k] > 0) {
+            c->err = "zero pivot " + std::to_string(info[k]) + (k < nb ? " in block column " + std::to_string(k + 1)
+                                                                        : std::string(" in the border block"));
+            return MH_ERR_SOLVE;
+        }
+    for (int64_t i = 0; i < nv; ++i)
+        if (!std::isfinite(x[i])) { c->err = "non-finite solution"; return MH_ERR_SOLVE; }
     return MH_OK;
 }
 

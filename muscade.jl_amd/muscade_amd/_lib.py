@@ -71,6 +71,8 @@ SIGNATURES = {
     "mh_big_prepare": (_I, [_P, _L, _L, _I64, _I64, _I64, _D, _L, _I64]),
     "mh_big_get_pattern": (_I, [_P, _I64, _I64]),
     "mh_big_assemble": (_I, [_P, _D, _D]),
+    "mh_big_solve_prepare": (_I, [_P, _L, _L, _L, _I64, _I64]),
+    "mh_big_solve": (_I, [_P, _D, _D]),
     "mh_beam_gauge_derivs": (_I, [_P, _I, _L, _I64, _D]),
     "mh_last_timings": (_I, [_P, _D]),
     "mh_calib_copy": (_I, [_P, _L, _D]),

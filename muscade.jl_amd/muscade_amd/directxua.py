@@ -38,7 +38,8 @@ import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
-from ._lib import dptr, i64ptr, lib
+from . import _lib
+from ._lib import as_f64, dptr, i64ptr, lib
 from .assemble import AssemblyDirect
 from .hostcost import Dual2
 from .model import MuscadeError
@@ -323,9 +324,34 @@ class DeviceBigSystem:
     def matches(self, hcontrib):
         return [v.size for (_, _, _, v) in hcontrib] == self.hsizes
 
+    @staticmethod
+    def _host_values(hcontrib):
+        return np.ascontiguousarray(np.concatenate([v for (_, _, _, v) in hcontrib]) if hcontrib else np.zeros(0),
+                                    dtype=np.float64)
+
+    def prepare_solve(self, per_step, nstep, na):
+        """Lay the system out for the device solve (mh_big_solve_prepare): dense diagonal blocks
+        of the steps that finite differences couple, block tridiagonal, bordered by the A-dofs.
+        Returns (block order, block count); MuscadeError when they do not fit in device memory."""
+        m, nb = ctypes.c_int64(), ctypes.c_int64()
+        self.ctx.check(lib.mh_big_solve_prepare(self.ctx.h, per_step, nstep, na, ctypes.byref(m), ctypes.byref(nb)),
+                       "big solve prepare")
+        return m.value, nb.value
+
+    def solve(self, hcontrib, rhs, dbg=None):
+        """Assemble the system on the device and solve Lvv·Δv = Lv there (mh_big_solve): only the
+        right-hand side goes up and only Δv comes back."""
+        hv = self._host_values(hcontrib)
+        self.ctx.check(lib.mh_big_assemble(self.ctx.h, dptr(hv) if hv.size else None, None), "big assemble")
+        x = np.zeros(self.nv)
+        r = lib.mh_big_solve(self.ctx.h, dptr(as_f64(rhs)), dptr(x))
+        if r == _lib.MH_ERR_SOLVE:
+            raise MuscadeError(f"Lvv matrix factorization failed: {lib.mh_last_error(self.ctx.h).decode()}", dbg)
+        self.ctx.check(r, "big solve")
+        return x
+
     def assemble(self, hcontrib):
-        hv = np.ascontiguousarray(np.concatenate([v for (_, _, _, v) in hcontrib]) if hcontrib else np.zeros(0),
-                                  dtype=np.float64)
+        hv = self._host_values(hcontrib)
         nz = np.zeros(self.rowval.size)
         self.ctx.check(lib.mh_big_assemble(self.ctx.h, dptr(hv) if hv.size else None, dptr(nz)), "big assemble")
         M = sp.csc_matrix((nz, self.rowval.copy(), self.colptr.copy()), shape=(self.nv, self.nv))
@@ -347,14 +373,17 @@ class DirectXUA(metaclass=_DirectXUAMeta):
 
     @classmethod
     def run(cls, pstate, verbose, dbg, *, time, initialstate, maxiter=50, maxΔλ=1e-5, maxΔx=1e-5, maxΔu=1e-5,
-            maxΔa=1e-5, saveiter=False, device=0, Assembly=AssemblyDirect, batch_steps=False, big="host", stats=None):
+            maxΔa=1e-5, saveiter=False, device=0, Assembly=AssemblyDirect, batch_steps=False, big="host", linsolve="host",
+            stats=None):
         """`Assembly` is the per-step assembler of the device element kinds (prepare(AssemblyDirect{OX,OU,IA}),
         src/DirectXUA.jl:22-56). batch_steps: all steps of an experiment in one device launch per
         Newton iteration (StepBatchAssembly; beams only). big: "host" assembles the all-steps
         system on the host (assemblebig!, DirectXUA.jl:336-383), "device" on the device from the
-        step-batched blocks (DeviceBigSystem; batch_steps and one experiment). stats (a dict)
-        receives the iteration count and the seconds spent in device assembly, host assembly and
-        the sparse LU."""
+        step-batched blocks (DeviceBigSystem; batch_steps and one experiment). linsolve: "host"
+        factorizes the system with SciPy's sparse LU (lu(Lvv), DirectXUA.jl:489-490), "device"
+        factorizes it where it was assembled (big="device"; DeviceBigSystem.solve, a dense block
+        LU of the steps). stats (a dict) receives the iteration count and the seconds spent in
+        device assembly, host assembly and the linear solve (lu_s)."""
         clock = _time.perf_counter
         st_ = stats if stats is not None else {}
         st_.update(iterations=0, device_assembly_s=0., host_assembly_s=0., lu_s=0.)
@@ -387,6 +416,10 @@ class DirectXUA(metaclass=_DirectXUAMeta):
             raise MuscadeError(f"big must be 'host' or 'device', not {big!r}")
         if big == "device" and (not batch_steps or nexp != 1):
             raise MuscadeError("big='device' assembles the step-batched blocks of one experiment (batch_steps=True)")
+        if linsolve not in ("host", "device"):
+            raise MuscadeError(f"linsolve must be 'host' or 'device', not {linsolve!r}")
+        if linsolve == "device" and big != "device":
+            raise MuscadeError("linsolve='device' solves the system where it is assembled (big='device')")
         host = HostLagrangians(model, dis, OX, OU, IA)
         if batch_steps:
             batched = [StepBatchAssembly(OX, OU, IA, model, dis, n, skip=tuple(host.skip), device=device)
@@ -486,28 +519,38 @@ class DirectXUA(metaclass=_DirectXUAMeta):
                                 rows.append(r + off(iexp, istep + Δa, a))
                                 cols.append(c + off(iexp, istep + Δb, b))
                                 vals.append(v * (wa * wb * s))
+            Δv = None
+            t_lin = 0.
             if big == "device":
                 if dbig is None or not dbig.matches(hcontrib):
                     dbig = DeviceBigSystem(batched[0], nv, lambda s_, c_: block(0, s_, c_), Δt[0], hcontrib)
+                    if linsolve == "device":
+                        st_["block_order"], st_["blocks"] = dbig.prepare_solve(per_step, nstep[0], nv - Aoff)
                 t0 = clock()
-                Lvv = dbig.assemble(hcontrib)
-                t_dev += clock() - t0
+                if linsolve == "device":
+                    Δv = dbig.solve(hcontrib, Lv, dict(dbg, iter=it))
+                    t_lin = clock() - t0
+                    st_["lu_s"] += t_lin
+                else:
+                    Lvv = dbig.assemble(hcontrib)
+                    t_dev += clock() - t0
             else:
                 Lvv = sp.coo_matrix((np.concatenate(vals) if vals else np.zeros(0),
                                      (np.concatenate(rows) if rows else np.zeros(0, int),
                                       np.concatenate(cols) if cols else np.zeros(0, int))), shape=(nv, nv)).tocsc()
                 Lvv.eliminate_zeros()
             st_["device_assembly_s"] += t_dev
-            st_["host_assembly_s"] += clock() - t_iter - t_dev
+            st_["host_assembly_s"] += clock() - t_iter - t_dev - t_lin
             st_["iterations"] = it
             if verbose:
                 print(", solving", end="")
-            t0 = clock()
-            try:
-                Δv = spla.splu(Lvv).solve(Lv)
-            except RuntimeError as e:
-                raise MuscadeError(f"Lvv matrix factorization failed: {e}", dict(dbg, iter=it))
-            st_["lu_s"] += clock() - t0
+            if Δv is None:
+                t0 = clock()
+                try:
+                    Δv = spla.splu(Lvv).solve(Lv)
+                except RuntimeError as e:
+                    raise MuscadeError(f"Lvv matrix factorization failed: {e}", dict(dbg, iter=it))
+                st_["lu_s"] += clock() - t0
             if verbose:
                 print(", decrementing.")
             Δ2 = {c: 0. for c in classes}
