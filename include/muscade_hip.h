@@ -253,8 +253,8 @@ int mh_big_assemble(mh_ctx* ctx, const double* hostvals, double* nzval);
  * A-dofs), so that nv = per_step·nstep + na. Finite differences couple steps at most a few
  * apart: mh_big_solve_prepare reads that width g off the pattern and lays the matrix out as
  * dense blocks of g steps (order m = g·per_step, returned with their count nb), block
- * tridiagonal and bordered by the A rows and columns. MH_ERR_SIZE when the blocks do not fit
- * in the device memory. mh_big_solve factors the values of the last mh_big_assemble (which
+ * tridiagonal and bordered by the A rows and columns. MH_ERR_SIZE when m > 4000 (the row
+ * permutations pass through LDS) or the blocks do not fit in the device memory. mh_big_solve factors the values of the last mh_big_assemble (which
  * may then pass nzval = NULL: nothing leaves the device) by block LU, partial pivoting
  * inside the diagonal blocks, and returns x (nv) for the right-hand side rhs (nv).
  * MH_ERR_SOLVE: a zero pivot or a non-finite solution. */

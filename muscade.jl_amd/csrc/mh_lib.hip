@@ -3119,7 +3119,7 @@ int mh_big_solve_prepare(mh_ctx* c, int64_t per_step, int64_t nstep, int64_t na,
 // in scale by many orders), and one step of iterative refinement on the residual of the sparse
 // system follows.
 int mh_big_solve(mh_ctx* c, const double* rhs, double* x) {
-    if (!c || !c->d_sbuf || !rhs || !x) return MH_ERR_ARG;
+    if (!c || !c->d_sbuf || c->sm <= 0 || !c->d_bval || !rhs || !x) return MH_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
     rocblas_handle h = c->blas;

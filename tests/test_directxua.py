@@ -150,3 +150,18 @@ def test_dynamic_directxua_residual_vanishes():
         L1, _ = asm.assemble(s, s.Λ[0], "vectors", 1.0)
         assert np.abs(L1["Λ"][0]).max() <= 1e-7 * 50.
     assert np.abs(X[:, model.getdofID("X", "t3", [nel + 1])[0] - 1]).max() > 1e-6
+
+
+@pytest.mark.parametrize("kw,msg", [(dict(linsolve="device"), "where it is assembled"),
+                                    (dict(linsolve="gpu"), "linsolve must be"),
+                                    (dict(big="device"), "batch_steps=True")])
+def test_device_options_are_checked(kw, msg):
+    """linsolve="device" factorizes the all-steps system where it is assembled (big="device",
+    which needs the step-batched blocks); bad option values are refused before any work."""
+    model = mc.Model("opts")
+    mc.addnode(model, [[0., 0., 0.], [1., 0., 0.]])
+    mc.addelement(model, mc.EulerBeam3D, [1, 2], mat=mc.BeamCrossSection(EA=1., EI2=1., EI3=1., GJ=1., mu=1., iota1=1.))
+    st0 = mc.initialize(model, time=0.)
+    with pytest.raises(mc.MuscadeError, match=msg):
+        mc.solve(mc.DirectXUA[0, 0, 0], initialstate=[st0], time=[np.array([1.])], verbose=False, catcherror=False,
+                 Assembly=OracleAssemblyDirect, **kw)
