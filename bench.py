@@ -461,7 +461,7 @@ def main():
         calib_copy()
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "muscade-hip measures one GPU; runs on several GPUs are not implemented",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
