@@ -2933,15 +2933,189 @@ __device__ __forceinline__ int64_t bsolve_vpos(int64_t i, int64_t nsp, int64_t n
                                                int64_t oE) {
     return i < nsp ? oW + nbm * na + i : oE + na * na + (i - nsp);
 }
-// The row permutation of the interchanges piv[0..n) (1-based) over `rows` rows: row i of the
-// result is row perm[i] of the original. One workgroup, in LDS.
-__global__ __launch_bounds__(256) void piv_to_perm(int n, int rows, const int32_t* __restrict__ piv,
+// LU with partial pivoting of a leaf panel (M ≤ kPT·kPR rows × w ≤ kPW columns, leading
+// dimension lda) in one workgroup, the steps of LAPACK's dgetf2: the first largest |entry| as
+// pivot, the row interchange, the column scaled by the pivot's reciprocal, the update. The panel
+// goes in sub-panels of kSW columns held in registers (row i in thread i % kPT, slot i / kPT);
+// after each sub-panel its rows of the columns to the right are solved with its unit lower
+// triangle and the rows below take the rank-kSW update, in global memory. piv[j] = roff + pivot
+// row + 1; *info = roff + j + 1 at the first exactly zero pivot (the column is then left as is).
+constexpr int kPW = 32;
+constexpr int kPT = 1024;
+constexpr int kPR = 5;
+constexpr int kSW = 4;
+__global__ __launch_bounds__(kPT) void panel_lu_kernel(int M, int w, double* __restrict__ P, int lda,
+                                                       int32_t* __restrict__ piv, int32_t* __restrict__ info, int roff) {
+    __shared__ double sv[kPT / 64];
+    __shared__ int si[kPT / 64];
+    __shared__ int sp;
+    __shared__ double prow[kSW], jrow[kSW];
+    __shared__ double L11[kSW][kSW];
+    __shared__ double U12[kSW][kPW];
+    const int t = threadIdx.x;
+    for (int sb = 0; sb < w; sb += kSW) {
+        const int sw = min(kSW, w - sb);
+        double v[kPR][kSW];
+#pragma unroll
+        for (int r = 0; r < kPR; ++r) {
+            const int i = t + r * kPT;
+#pragma unroll
+            for (int c = 0; c < kSW; ++c) v[r][c] = (i < M && c < sw) ? P[(int64_t)(sb + c) * lda + i] : 0.;
+        }
+#pragma unroll
+        for (int jj = 0; jj < kSW; ++jj) {
+            if (jj >= sw) break;
+            const int j = sb + jj;
+            double best = -1.;
+            int bi = M;
+#pragma unroll
+            for (int r = 0; r < kPR; ++r) {   // a thread's rows increase with r: > keeps the first
+                const int i = t + r * kPT;
+                const double a = fabs(v[r][jj]);
+                if (i >= j && i < M && a > best) { best = a; bi = i; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(best, off, 64);
+                const int oi = __shfl_xor(bi, off, 64);
+                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            }
+            if ((t & 63) == 0) { sv[t >> 6] = best; si[t >> 6] = bi; }
+            __syncthreads();
+            if (t == 0) {
+                double b = sv[0];
+                int i0 = si[0];
+                for (int q = 1; q < kPT / 64; ++q)
+                    if (sv[q] > b || (sv[q] == b && si[q] < i0)) { b = sv[q]; i0 = si[q]; }
+                if (i0 >= M) i0 = j;   // nothing comparable in the column (NaN): keep the row
+                sp = i0;
+                piv[j] = roff + i0 + 1;
+            }
+            __syncthreads();
+            const int p = sp;
+            const int tp = p % kPT, rp = p / kPT;
+            if (t == tp) {
+#pragma unroll
+                for (int r = 0; r < kPR; ++r)
+                    if (r == rp) {
+#pragma unroll
+                        for (int c = 0; c < kSW; ++c) prow[c] = v[r][c];
+                    }
+            }
+            if (t == j) {   // row j < kPW lives in thread j, slot 0
+#pragma unroll
+                for (int c = 0; c < kSW; ++c) jrow[c] = v[0][c];
+            }
+            // the same interchange in the panel's other columns (one thread per column, in order)
+            if (p != j && t < w && (t < sb || t >= sb + sw)) {
+                double* cc = P + (int64_t)t * lda;
+                const double x = cc[j];
+                cc[j] = cc[p];
+                cc[p] = x;
+            }
+            __syncthreads();
+            if (p != j) {
+                if (t == j) {
+#pragma unroll
+                    for (int c = 0; c < kSW; ++c) v[0][c] = prow[c];
+                }
+                if (t == tp) {
+#pragma unroll
+                    for (int r = 0; r < kPR; ++r)
+                        if (r == rp) {
+#pragma unroll
+                            for (int c = 0; c < kSW; ++c) v[r][c] = jrow[c];
+                        }
+                }
+            }
+            const double d = prow[jj];
+            if (d == 0.) {
+                if (t == 0 && *info == 0) *info = roff + j + 1;
+            } else {
+                const double inv = 1. / d;
+#pragma unroll
+                for (int r = 0; r < kPR; ++r) {
+                    const int i = t + r * kPT;
+                    if (i > j && i < M) {
+                        const double l = v[r][jj] * inv;
+                        v[r][jj] = l;
+#pragma unroll
+                        for (int c = jj + 1; c < kSW; ++c) v[r][c] -= l * prow[c];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int r = 0; r < kPR; ++r) {
+            const int i = t + r * kPT;
+            if (i < M) {
+#pragma unroll
+                for (int c = 0; c < kSW; ++c)
+                    if (c < sw) P[(int64_t)(sb + c) * lda + i] = v[r][c];
+            }
+        }
+        if (t >= sb && t < sb + sw) {   // rows sb.. of the sub-panel: thread = row, slot 0
+#pragma unroll
+            for (int c = 0; c < kSW; ++c) L11[t - sb][c] = v[0][c];
+        }
+        __syncthreads();
+        const int nr = w - sb - sw;
+        if (nr > 0) {
+            if (t < nr) {   // U12 = L11⁻¹ A12 for right column sb + sw + t
+                double* cc = P + (int64_t)(sb + sw + t) * lda + sb;
+                double u[kSW];
+#pragma unroll
+                for (int q = 0; q < kSW; ++q) u[q] = q < sw ? cc[q] : 0.;
+#pragma unroll
+                for (int q = 1; q < kSW; ++q)
+#pragma unroll
+                    for (int k = 0; k < q; ++k) u[q] -= L11[q][k] * u[k];
+#pragma unroll
+                for (int q = 0; q < kSW; ++q) {
+                    if (q < sw) cc[q] = u[q];
+                    U12[q][t] = q < sw ? u[q] : 0.;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < kPR; ++r) {
+                const int i = t + r * kPT;
+                if (i >= sb + sw && i < M) {
+                    double* row = P + (int64_t)(sb + sw) * lda + i;
+#pragma unroll 4
+                    for (int c = 0; c < nr; ++c) {
+                        double a = row[(int64_t)c * lda];
+#pragma unroll
+                        for (int q = 0; q < kSW; ++q) a -= v[r][q] * U12[q][c];
+                        row[(int64_t)c * lda] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+// Row interchanges piv[0..n) (1-based, offset roff) applied in order to ncol columns of A, one
+// thread per column (for few interchanges).
+__global__ __launch_bounds__(kBlock) void laswp_cols(int ncol, double* __restrict__ A, int lda, int n,
+                                                     const int32_t* __restrict__ piv, int roff) {
+    const int q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= ncol) return;
+    double* a = A + (int64_t)q * lda;
+    for (int i = 0; i < n; ++i) {
+        const int p = piv[i] - 1 - roff;
+        if (p != i) { const double x = a[i]; a[i] = a[p]; a[p] = x; }
+    }
+}
+// The row permutation of the interchanges piv[0..n) (1-based, offset roff) over `rows` rows: row i
+// of the result is row perm[i] of the original. One workgroup, in LDS.
+__global__ __launch_bounds__(256) void piv_to_perm(int n, int rows, const int32_t* __restrict__ piv, int roff,
                                                    int32_t* __restrict__ perm) {
     extern __shared__ int32_t shp[];
     int32_t* pr = shp;
     int32_t* pv = shp + rows;
     for (int i = threadIdx.x; i < rows; i += 256) pr[i] = i;
-    for (int i = threadIdx.x; i < n; i += 256) pv[i] = piv[i] - 1;
+    for (int i = threadIdx.x; i < n; i += 256) pv[i] = piv[i] - 1 - roff;
     __syncthreads();
     if (threadIdx.x == 0)
         for (int i = 0; i < n; ++i) {
@@ -3019,6 +3193,47 @@ __global__ __launch_bounds__(kBlock) void bsolve_sol(int64_t nv, int64_t nsp, in
     if (i < nv) x[i] = buf[bsolve_vpos(i, nsp, nbm, na, oW, oE)] * sc[i];
 }
 
+// Recursive LU with partial pivoting (LAPACK's getrf2 splitting) of an M × N block with leading
+// dimension lda: halves down to leaf panels of kPW columns (panel_lu_kernel), the row
+// interchanges composed into permutations where they are many, rocBLAS trsm and gemm for the
+// off-diagonal parts. piv holds global 1-based rows (offset roff = the block's first row).
+struct RecLU {
+    Ctx* c;
+    hipStream_t s;
+    rocblas_handle h;
+    int lda;
+    int32_t* info;
+    int32_t* tmp;   // a row permutation (M entries)
+    int swap(double* A, int M, int ncol, int n, const int32_t* piv, int roff) {
+        if (ncol <= 0 || n <= 0) return MH_OK;
+        if (n <= 64) {
+            hipLaunchKernelGGL(laswp_cols, dim3((ncol + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ncol, A, lda, n, piv, roff);
+        } else {
+            hipLaunchKernelGGL(piv_to_perm, dim3(1), dim3(256), sizeof(int32_t) * (M + n), s, n, M, piv, roff, tmp);
+            hipLaunchKernelGGL(perm_rows, dim3(ncol), dim3(256), sizeof(double) * M, s, M, A, lda, tmp);
+        }
+        return MH_OK;
+    }
+    int run(double* A, int M, int N, int32_t* piv, int roff) {
+        if (N <= kPW) {
+            hipLaunchKernelGGL(panel_lu_kernel, dim3(1), dim3(kPT), 0, s, M, N, A, lda, piv, info, roff);
+            return MH_OK;
+        }
+        const int n1 = ((N / 2) + kPW - 1) / kPW * kPW;
+        const double one = 1., mone = -1.;
+        const rocblas_operation Nop = rocblas_operation_none;
+        if (int r = run(A, M, n1, piv, roff)) return r;
+        swap(A + (int64_t)n1 * lda, M, N - n1, n1, piv, roff);
+        BLASCHK(c, rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, Nop, rocblas_diagonal_unit, n1, N - n1, &one, A,
+                                 lda, A + (int64_t)n1 * lda, lda));
+        BLASCHK(c, rocblas_dgemm(h, Nop, Nop, M - n1, N - n1, n1, &mone, A + n1, lda, A + (int64_t)n1 * lda, lda, &one,
+                                 A + n1 + (int64_t)n1 * lda, lda));
+        if (int r = run(A + n1 + (int64_t)n1 * lda, M - n1, N - n1, piv + n1, roff + n1)) return r;
+        swap(A + n1, M - n1, n1, N - n1, piv + n1, roff + n1);
+        return MH_OK;
+    }
+};
+
 int mh_big_solve_prepare(mh_ctx* c, int64_t per_step, int64_t nstep, int64_t na, int64_t* m_out, int64_t* nb_out) {
     if (!c || !c->d_bcolptr || per_step < 1 || nstep < 1 || na < 0 || per_step * nstep + na != c->bnv) return MH_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
@@ -3076,7 +3291,7 @@ int mh_big_solve_prepare(mh_ctx* c, int64_t per_step, int64_t nstep, int64_t na,
     HIPCHK(c, hipMalloc(&c->d_srhs, sizeof(double) * nv));
     HIPCHK(c, hipMalloc(&c->d_sipiv, sizeof(int32_t) * (nbm + std::max<int64_t>(na, 1))));
     HIPCHK(c, hipMalloc(&c->d_sinfo, sizeof(int32_t) * (nb + 1)));
-    HIPCHK(c, hipMalloc(&c->d_sprm, sizeof(int32_t) * nb * 2 * m));
+    HIPCHK(c, hipMalloc(&c->d_sprm, sizeof(int32_t) * (nb + 1) * 2 * m));   // + one for the recursive LU
     HIPCHK(c, hipMemcpy(c->d_sslot, slot.data(), sizeof(int64_t) * slot.size(), hipMemcpyHostToDevice));
     {   // the pattern by rows, for the residuals of the refinement steps
         std::vector<int32_t> rp(nv + 1, 0), ci(std::max<int64_t>(nnz, 1)), pm(std::max<int64_t>(nnz, 1));
@@ -3112,8 +3327,9 @@ int mh_big_solve_prepare(mh_ctx* c, int64_t per_step, int64_t nstep, int64_t na,
 }
 
 // Banded block LU with partial pivoting (LAPACK gbtrf at block level) of the step part T: block
-// column k (rocSOLVER's getrf) pivots over block rows k and k+1, whose row interchanges reach
-// block columns k+1 and k+2 (the fill) and the columns of W as one composed permutation. The border is taken by bordering: W ← T⁻¹[B | b], then
+// column k (RecLU; rocSOLVER's getrf beyond kPT·kPR rows) pivots over block rows k and k+1,
+// whose row interchanges reach block columns k+1 and k+2 (the fill) and the columns of W as one
+// composed permutation. The border is taken by bordering: W ← T⁻¹[B | b], then
 // the Schur complement S = E − C T⁻¹B (dense, na × na) gives a = S⁻¹(b_A − C T⁻¹b) and the step
 // unknowns are T⁻¹b − T⁻¹B a. The system is equilibrated first (its unknowns Λ, X, U, A differ
 // in scale by many orders), and one step of iterative refinement on the residual of the sparse
@@ -3162,8 +3378,14 @@ int mh_big_solve(mh_ctx* c, const double* rhs, double* x) {
             const bool two = k + 1 < nb;
             const int Mr = two ? 2 * im : im;
             if (factor) {
-                BLASCHK(c, rocsolver_dgetrf(h, Mr, im, P, l4, piv, c->d_sinfo + k));
-                hipLaunchKernelGGL(piv_to_perm, dim3(1), dim3(256), sizeof(int32_t) * (Mr + im), s, im, Mr, piv, prm);
+                if (Mr <= kPT * kPR) {
+                    RecLU lu{c, s, h, l4, c->d_sinfo + k, c->d_sprm + nb * 2 * m};
+                    if (int r = lu.run(P, Mr, im, piv, 0)) return r;
+                    if (int r = check_launch(c, "block LU")) return r;
+                } else {
+                    BLASCHK(c, rocsolver_dgetrf(h, Mr, im, P, l4, piv, c->d_sinfo + k));
+                }
+                hipLaunchKernelGGL(piv_to_perm, dim3(1), dim3(256), sizeof(int32_t) * (Mr + im), s, im, Mr, piv, 0, prm);
                 for (int jj = 1; jj <= 2; ++jj) {
                     if (k + jj >= nb) break;
                     double* T = col(k + jj) + (2 - jj) * m;   // block rows k, k+1 of column k+jj
