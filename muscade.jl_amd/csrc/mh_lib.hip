@@ -25,6 +25,7 @@
 #include <rocsolver/rocsolver.h>
 #include <algorithm>
 #include <climits>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -3171,16 +3172,40 @@ __global__ __launch_bounds__(kBlock) void bsolve_rhs(int64_t nv, int64_t nsp, in
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < nv) buf[bsolve_vpos(i, nsp, nbm, na, oW, oE)] = rhs[i] * sr[i];
 }
-// r = b − M x over the pattern by rows, each row summed in column order (deterministic)
+// r = b − M x over the pattern by rows, each row summed in column order (deterministic), and
+// each row's componentwise backward error |r_i| / (Σ_j |M_ij x_j| + |b_i|) (∞ for a nonzero
+// residual over a zero denominator)
 __global__ __launch_bounds__(kBlock) void bsolve_resid(int64_t nv, const int32_t* __restrict__ rowptr,
                                                        const int32_t* __restrict__ colidx, const int32_t* __restrict__ perm,
                                                        const double* __restrict__ val, const double* __restrict__ x,
-                                                       const double* __restrict__ b, double* __restrict__ r) {
+                                                       const double* __restrict__ b, double* __restrict__ r,
+                                                       double* __restrict__ wrow) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= nv) return;
-    double acc = 0.;
-    for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) acc += val[perm[p]] * x[colidx[p]];
-    r[i] = b[i] - acc;
+    double acc = 0., den = fabs(b[i]);
+    for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+        const double t = val[perm[p]] * x[colidx[p]];
+        acc += t;
+        den += fabs(t);
+    }
+    const double ri = b[i] - acc;
+    r[i] = ri;
+    const double q = fabs(ri);
+    wrow[i] = den > 0. ? q / den : (q > 0. ? INFINITY : 0.);
+}
+// ω = max_i wrow_i, the componentwise backward error of x. One workgroup.
+__global__ __launch_bounds__(kBlock) void bsolve_omega(int64_t nv, const double* __restrict__ wrow,
+                                                       double* __restrict__ omega) {
+    __shared__ double red[kBlock];
+    double w = 0.;
+    for (int64_t i = threadIdx.x; i < nv; i += kBlock) w = fmax(w, wrow[i]);
+    red[threadIdx.x] = w;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *omega = red[0];
 }
 __global__ __launch_bounds__(kBlock) void bsolve_add(int64_t nv, const double* __restrict__ d, double* __restrict__ x) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -3304,7 +3329,7 @@ int mh_big_solve_prepare(mh_ctx* c, int64_t per_step, int64_t nstep, int64_t na,
         HIPCHK(c, hipMalloc(&c->d_scolidx, sizeof(int32_t) * ci.size()));
         HIPCHK(c, hipMalloc(&c->d_sperm, sizeof(int32_t) * pm.size()));
         HIPCHK(c, hipMalloc(&c->d_sb, sizeof(double) * nv));
-        HIPCHK(c, hipMalloc(&c->d_sx, sizeof(double) * nv));
+        HIPCHK(c, hipMalloc(&c->d_sx, sizeof(double) * (2 * nv + 1)));   // x | ω | per-row backward errors
         HIPCHK(c, hipMalloc(&c->d_ssr, sizeof(double) * nv));
         HIPCHK(c, hipMalloc(&c->d_ssc, sizeof(double) * nv));
         std::vector<int32_t> cv(std::max<int64_t>(nnz, 1));
@@ -3333,7 +3358,7 @@ int mh_big_solve_prepare(mh_ctx* c, int64_t per_step, int64_t nstep, int64_t na,
 // the Schur complement S = E − C T⁻¹B (dense, na × na) gives a = S⁻¹(b_A − C T⁻¹b) and the step
 // unknowns are T⁻¹b − T⁻¹B a. The system is equilibrated first (its unknowns Λ, X, U, A differ
 // in scale by many orders), and one step of iterative refinement on the residual of the sparse
-// system follows.
+// system follows unless the componentwise backward error is already within 8 ε.
 int mh_big_solve(mh_ctx* c, const double* rhs, double* x) {
     if (!c || !c->d_sbuf || c->sm <= 0 || !c->d_bval || !rhs || !x) return MH_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
@@ -3430,7 +3455,13 @@ int mh_big_solve(mh_ctx* c, const double* rhs, double* x) {
     hipLaunchKernelGGL(bsolve_sol, dim3(grid(nv)), dim3(kBlock), 0, s, nv, nsp, nbm, na, o[1], o[3], B, c->d_ssc, c->d_sx);
     for (int it = 0; it < 1; ++it) {
         hipLaunchKernelGGL(bsolve_resid, dim3(grid(nv)), dim3(kBlock), 0, s, nv, c->d_srowptr, c->d_scolidx,
-                           c->d_sperm, c->d_bval, c->d_sx, c->d_sb, c->d_srhs);
+                           c->d_sperm, c->d_bval, c->d_sx, c->d_sb, c->d_srhs, c->d_sx + nv + 1);
+        // refine only if x is not already a componentwise backward-stable solution
+        hipLaunchKernelGGL(bsolve_omega, dim3(1), dim3(kBlock), 0, s, nv, c->d_sx + nv + 1, c->d_sx + nv);
+        double omega = 0.;
+        HIPCHK(c, hipMemcpyAsync(&omega, c->d_sx + nv, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        if (omega <= 8. * DBL_EPSILON) break;
         hipLaunchKernelGGL(bsolve_rhs, dim3(grid(nv)), dim3(kBlock), 0, s, nv, nsp, nbm, na, o[1], o[3], c->d_srhs,
                            c->d_ssr, B);
         if (int r = forward(false, ina, 1)) return r;
