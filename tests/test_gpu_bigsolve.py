@@ -4,8 +4,8 @@ against SciPy's sparse LU of the same matrix (downloaded from the same device as
 - each iteration's Δv: normwise backward error ≤ 1e-13 and at most SuperLU's (×10; the
   systems are too ill-conditioned for a forward-error comparison);
 - solve(DirectXUA{OX,0,IA}; big="device", linsolve="device") reaches the linsolve="host"
-  history (within one iteration, states to 1e-8), for OX = 0 (block diagonal), OX = 2 (two steps
-  per diagonal block), and IA = 1 (the A-dof borders every step's block);
+  history (within one iteration, states to 1e-8), for OX = 0 (block diagonal), OX = 1 and 2 (two
+  steps per diagonal block), and IA = 1 (the A-dof borders every step's block);
 - a singular system is reported as DirectXUA's factorization failure."""
 import numpy as np
 import pytest
@@ -40,7 +40,8 @@ def run_pair(mc, model, OX, IA, time, spy=None):
     return host, dev, sh, sd
 
 
-@pytest.mark.parametrize("OX,inverse,IA", [(0, True, 0), (2, False, 0), (2, True, 0), (2, True, 1)])
+@pytest.mark.parametrize("OX,inverse,IA", [(0, True, 0), (0, True, 1), (1, False, 0), (2, False, 0), (2, True, 0),
+                                           (2, True, 1)])
 def test_device_solve_reaches_host_history(mc, OX, inverse, IA):
     print("case", OX, inverse, IA, flush=True)
     model = beam_model(mc, dyn_load=not inverse, inverse=inverse)
