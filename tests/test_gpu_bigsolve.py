@@ -6,7 +6,8 @@ against SciPy's sparse LU of the same matrix (downloaded from the same device as
 - solve(DirectXUA{OX,0,IA}; big="device", linsolve="device") reaches the linsolve="host"
   history (within one iteration, states to 1e-8), for OX = 0 (block diagonal), OX = 1 and 2 (two
   steps per diagonal block), and IA = 1 (the A-dof borders every step's block);
-- a singular system is reported as DirectXUA's factorization failure."""
+- a singular system is reported as DirectXUA's factorization failure, and blocks too large for
+  the dense solve are refused at prepare."""
 import numpy as np
 import pytest
 import scipy.sparse.linalg as spla
@@ -105,3 +106,13 @@ def test_singular_system_raises(mc):
     with pytest.raises(mc.MuscadeError, match="factorization failed"):
         mc.solve(mc.DirectXUA[0, 0, 0], initialstate=[st0], time=[np.arange(1., 4.)], maxiter=3, verbose=False,
                  catcherror=False, batch_steps=True, big="device", linsolve="device")
+
+
+def test_too_large_blocks_are_refused(mc):
+    """Diagonal blocks beyond the dense solve's limit (m > 4000) are refused at prepare with
+    MH_ERR_SIZE, before any factorization."""
+    model = beam_model(mc, nel=400, dyn_load=True)
+    st0 = mc.initialize(model, time=0.)
+    with pytest.raises(mc.MuscadeError, match="too large"):
+        mc.solve(mc.DirectXUA[2, 0, 0], initialstate=[st0], time=[np.arange(1., 8.) * 0.1], maxiter=2,
+                 verbose=False, catcherror=False, batch_steps=True, big="device", linsolve="device")
