@@ -9,6 +9,9 @@
 //   element residuals      toolbox/BeamElement.jl, toolbox/BarElement.jl, src/BasicElements.jl,
 //                          test/SomeElements.jl
 //   rotations              toolbox/Rotations.jl (orc_sinc1, orc_scac_d, orc_rodrigues ...)
+//   requestables           toolbox/BeamElement.jl:28-64,151-174 (orc_beam_results), the strain
+//                          gauges' εₐₓ, κ with second derivatives (orc_beam_gauge2), AnchorLine's
+//                          Fh, cr (orc_anchor_eval, pinned by test/TestElementCost.jl:28-34,81-95)
 // Parity pins: see the header of each test module in tests/ for the golden
 // literal (reference file:line) each function is checked against.
 #include <algorithm>
