@@ -47,8 +47,10 @@ enum {
     MH_KIND_BAR = 6,     /* Bar3D{Mat,false} with AxisymmetricBarCrossSection (toolbox/BarElement.jl:89-202) */
     MH_KIND_BAR_U = 7,   /* Bar3D{Mat,true}: 3 U-dofs, load per length (BarElement.jl:115-118,192)     */
     MH_KIND_HOST = 8     /* residual evaluated by the host with user functions (DofConstraint{:X} with a
-                            gap Functor, BasicElements.jl:426-438): the host hands R and ∂R/∂X to
-                            mh_set_host_blocks before each assembly; the device scatters them        */
+                            gap Functor, BasicElements.jl:426-438; user elements that define
+                            `residual`, ElementAPI.jl:144-173): the host hands R and ∂R/∂δX to
+                            mh_set_host_blocks (and ∂R/∂δr to mh_set_host_dr) before each assembly;
+                            the device scatters them                                                  */
 };
 
 typedef struct mh_ctx mh_ctx;
@@ -150,10 +152,16 @@ int mh_device_outputs(mh_ctx* ctx, double** d_rhs, double** d_nzval);
 int mh_get_element_blocks(mh_ctx* ctx, int igroup, double* R, double* K);
 
 /* Element results of a MH_KIND_HOST group for the next SweepX assemblies: R (nX×nele) and
- * ∂R/∂X (nX²×nele, entry i+nX·j of an element's block), unscaled; the library applies the
- * group's dof scales (R_i·s_i, K_ij·s_i·s_j, the convention of the device kinds). The
- * constrained element is static: its C and M blocks and its ∂R/∂δr (:step) are zero. */
+ * ∂R/∂δX (nX²×nele, entry i+nX·j of an element's block), unscaled; the library applies the
+ * group's dof scales (R_i·s_i, K_ij·s_i·s_j, the convention of the device kinds). For a static
+ * element (a DofConstraint) ∂R/∂δX = ∂R/∂X and its ∂R/∂δr (:step) is zero. */
 int mh_set_host_blocks(mh_ctx* ctx, int igroup, const double* R, const double* K);
+/* ∂R/∂δr of a MH_KIND_HOST group's elements for the next :step assembly (nX×nele, unscaled; the
+ * library applies scale.X): the rhs term −C·a − M·b that `add_∂!{1,:minus}(out.Lλ,asm[1],iele,
+ * Lλ,1:Nx,(Nx+1,))` subtracts in SweepX{1,2}'s :step (src/SweepX.jl:45-58,70-80), for host
+ * elements with inertia or damping. K of mh_set_host_blocks is then ∂R/∂δX along the trial path
+ * (K + a₁C + b₁M, src/SweepX.jl:53-66). mh_set_host_blocks resets ∂R/∂δr to zero. */
+int mh_set_host_dr(mh_ctx* ctx, int igroup, const double* dR);
 
 /* ---- element results: getresult / @espy of EulerBeam3D (src/Output.jl:131-181,
  *      toolbox/BeamElement.jl:28-64,151-174) ---------------------------------------

@@ -373,7 +373,7 @@ __device__ __forceinline__ void point_element(const PointArgs& a, int64_t e) {
         for (int i = 0; i < nX; ++i) a.R[i * n + e] = 0.;
         for (int i = 0; i < nX * nX; ++i) a.K[(int64_t)nX * nX * e + i] = 0.;
     }
-    if (a.dR)
+    if (a.dR && a.kind != MH_KIND_HOST)   // host groups: ∂R/∂δr comes from mh_set_host_blocks / mh_set_host_dr
         for (int i = 0; i < nX; ++i) a.dR[i * n + e] = 0.;
 }
 // Up to kPointBatch point groups in one launch (they are small; one launch each would
@@ -504,7 +504,9 @@ __global__ __launch_bounds__(kBlock) void beam_results_kernel(BeamDynArgs a, int
 // element). Per element: values (4), gradients (4×12), Hessians (4×78, upper triangle).
 constexpr int kPairs = 78;
 constexpr int kGaugeOut = 4 + 4 * 12 + 4 * kPairs;
-__global__ __launch_bounds__(kBlock) void beam_gauge_derivs_kernel(BeamDynArgs a, const int32_t* __restrict__ sel,
+// sel: element numbers (SelT, counted from kSelBase), or null for all elements in order.
+template <class SelT, int kSelBase>
+__global__ __launch_bounds__(kBlock) void beam_gauge_derivs_kernel(BeamDynArgs a, const SelT* __restrict__ sel,
                                                                    int64_t nsel, double* __restrict__ out) {
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t s = t / kPairs;
@@ -513,7 +515,7 @@ __global__ __launch_bounds__(kBlock) void beam_gauge_derivs_kernel(BeamDynArgs a
     int i = 0, rem = p;
     while (rem >= 12 - i) { rem -= 12 - i; ++i; }
     const int j = i + rem;
-    const int64_t e = sel ? sel[s] : s;
+    const int64_t e = sel ? (int64_t)sel[s] - kSelBase : s;
     mh::BeamGeo G;
     mh::BeamSection S;
     mh::BeamDynSection D;
@@ -2736,7 +2738,7 @@ int mh_beam_gauge_derivs(mh_ctx* c, int igroup, int64_t nsel, const int64_t* sel
     hipError_t e = hipSuccess;
     if (sel) e = hipMemcpyAsync(d_sel, s32.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(beam_gauge_derivs_kernel, dim3((unsigned)((n * kPairs + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL((beam_gauge_derivs_kernel<int32_t, 0>), dim3((unsigned)((n * kPairs + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
                            b, d_sel, n, d_out);
         e = hipGetLastError();
     }
@@ -2770,6 +2772,25 @@ int mh_set_host_blocks(mh_ctx* c, int igroup, const double* R, const double* K) 
         }
     HIPCHK(c, hipMemcpyAsync(c->d_Rbuf + g.rbase, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_Kbuf + g.kbase, k.data(), sizeof(double) * k.size(), hipMemcpyHostToDevice, c->stream));
+    // a static element has no ∂R/∂δr: the :step rhs term is zero until mh_set_host_dr gives one
+    std::fill(r.begin(), r.end(), 0.);
+    HIPCHK(c, hipMemcpyAsync(c->d_dRbuf + g.rbase, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MH_OK;
+}
+
+int mh_set_host_dr(mh_ctx* c, int igroup, const double* dR) {
+    if (!c || !c->prepared || igroup < 0 || igroup >= (int)c->groups.size() || !dR) return MH_ERR_ARG;
+    const Group& g = c->groups[igroup];
+    if (g.kind != MH_KIND_HOST) { c->err = "group is not host-evaluated"; return MH_ERR_ARG; }
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = g.nele;
+    const int nX = g.nX;
+    if (n == 0) return MH_OK;
+    std::vector<double> r((size_t)nX * n);
+    for (int64_t e = 0; e < n; ++e)
+        for (int i = 0; i < nX; ++i) r[(size_t)i * n + e] = dR[(size_t)nX * e + i] * g.scaleX[i];
+    HIPCHK(c, hipMemcpyAsync(c->d_dRbuf + g.rbase, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MH_OK;
 }

@@ -68,6 +68,7 @@ SIGNATURES = {
     "mh_get_X": (_I, [_P, _I, _D]),
     "mh_beam_results": (_I, [_P, _I, _I, _I, _D, _D]),
     "mh_set_host_blocks": (_I, [_P, _I, _D, _D]),
+    "mh_set_host_dr": (_I, [_P, _I, _D]),
     "mh_big_prepare": (_I, [_P, _L, _L, _I64, _I64, _I64, _D, _L, _I64]),
     "mh_big_get_pattern": (_I, [_P, _I64, _I64]),
     "mh_big_assemble": (_I, [_P, _D, _D]),

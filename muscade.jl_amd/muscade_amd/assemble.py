@@ -150,7 +150,7 @@ class AssemblySweepX:
         if upload:
             self.set_state(state, state.time)
         if self.host_types:
-            self._host_blocks(state, upload, dbg)
+            self._host_blocks(state, upload, dbg, mission)
         err = _lib.mh_err()
         m = 0 if mission == "step" else 1
         fn = lib.mh_assemble_sweepx_graph if self.graph else lib.mh_assemble_sweepx
@@ -165,23 +165,35 @@ class AssemblySweepX:
             self.ctx.check(lib.mh_get_nzval(self.ctx.h, dptr(self.nzval)), "nzval")
         return self.Lλ, self.nzval
 
-    def _host_blocks(self, state, upload, dbg):
-        """R and ∂R/∂X of the host-evaluated element types at the current X (SweepX.jl:45-96)."""
+    def _host_blocks(self, state, upload, dbg, mission="iter"):
+        """R and ∂R/∂δX (and ∂R/∂δr on :step) of the host-evaluated element types at the current
+        state, along SweepX{OX}'s trial path (SweepX.jl:45-96)."""
+        nd = self.OX + 1 if any(getattr(self.model.eletypes[it].cls, "residual_host", False)
+                                for it in self.host_types) else 1
         if upload:
-            X0 = state.X[0]
+            Xs = [state.X[d] if d < len(state.X) else np.zeros(self.nX) for d in range(nd)]
         else:
-            X0 = np.zeros(self.nX)
-            self.ctx.check(lib.mh_get_X(self.ctx.h, 0, dptr(X0)), "X")
+            Xs = [np.zeros(self.nX) for _ in range(nd)]
+            for d in range(nd):
+                self.ctx.check(lib.mh_get_X(self.ctx.h, d, dptr(Xs[d])), "X")
         SP = getattr(state, "SP", None)
         γ = float(SP.get("γ", 0.)) if isinstance(SP, dict) else 0.   # default{:γ}(SP, 0.)
         for it in self.host_types:
             E = self.model.eletypes[it]
             d = self.dis.dis[it]
-            Rs, Ks, e0 = [], [], 0
+            Rs, Ks, dRs, e0 = [], [], [], 0
             for nodes, p in zip(E.nodes, E.params):
                 m = nodes.shape[0]
-                R, K = E.cls.residual_blocks(p, X0[d.X[e0:e0 + m] - 1], state.time, γ)
-                Rs.append(R); Ks.append(K)
+                iX = d.X[e0:e0 + m] - 1
+                if getattr(E.cls, "residual_host", False):
+                    Uv = state.U[0][d.U[e0:e0 + m] - 1] if d.U.shape[1] else np.zeros((m, 0))
+                    Av = state.A[d.A[e0:e0 + m] - 1] if d.A.shape[1] else np.zeros((m, 0))
+                    R, K, dR = E.cls.sweepx_blocks(p, [x[iX] for x in Xs], Uv, Av, state.time, self.c, mission,
+                                                   self.OX)
+                else:
+                    R, K = E.cls.residual_blocks(p, Xs[0][iX], state.time, γ)
+                    dR = None
+                Rs.append(R); Ks.append(K); dRs.append(dR)
                 e0 += m
             R, K = np.concatenate(Rs), np.concatenate(Ks)
             bad = ~(np.isfinite(R).all(axis=1) & np.isfinite(K).reshape(K.shape[0], -1).all(axis=1))
@@ -190,6 +202,12 @@ class AssemblySweepX:
                                    dict(dbg or {}, ieletyp=it + 1, iele=int(np.nonzero(bad)[0][0]) + 1, t=state.time))
             self.ctx.check(lib.mh_set_host_blocks(self.ctx.h, self.groups[it], dptr(as_f64(R)),
                                                   dptr(as_f64(K.transpose(0, 2, 1)))), "host blocks")
+            if any(x is not None for x in dRs):
+                dR = np.concatenate([x if x is not None else np.zeros_like(r) for x, r in zip(dRs, Rs)])
+                if not np.isfinite(dR).all():
+                    raise MuscadeError(f"residual({E.key[0]},...) returned NaN in R, FB or derivatives",
+                                       dict(dbg or {}, ieletyp=it + 1, t=state.time))
+                self.ctx.check(lib.mh_set_host_dr(self.ctx.h, self.groups[it], dptr(as_f64(dR))), "host ∂R/∂δr")
 
     def graph_captures(self):
         n = ctypes.c_int64()

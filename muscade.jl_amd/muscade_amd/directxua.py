@@ -62,6 +62,136 @@ def finitediff(order, n, s):
     return _FD[order][0] if s == 1 else (_FD[order][1] if s == n else _FD[order][2])
 
 
+def class_patterns(model, dis, OX, OU, IA):
+    """The class-pair patterns of prepare(AssemblyDirect{OX,OU,IA}) (DirectXUA.jl:22-56): asmmat!
+    (Assemble.jl:373-448) over the dof groups allΛdofs, allXdofs, allUdofs, allAdofs and every
+    element type. Returns {(α,β): (colptr, rowval (0-based), shape, nder α, nder β)}, with nder
+    0×0 for (Λ,Λ), which is always zero."""
+    idx = {"Λ": [d.X for d in dis.dis], "X": [d.X for d in dis.dis], "U": [d.U for d in dis.dis],
+           "A": [d.A for d in dis.dis]}
+    n = {"Λ": model.ndof["X"], "X": model.ndof["X"], "U": model.ndof["U"], "A": model.ndof["A"]}
+    nder = {"Λ": 1, "X": OX + 1, "U": OU + 1, "A": IA}
+    pats = {}
+    for a in CLASSES:
+        for b in CLASSES:
+            keys = []
+            if n[a] and n[b]:
+                for ia, ib in zip(idx[a], idx[b]):
+                    if ia.shape[1] and ib.shape[1] and ia.shape[0]:
+                        keys.append(((ib[:, None, :] - 1) * n[a] + (ia[:, :, None] - 1)).ravel())
+            u = np.unique(np.concatenate(keys)) if keys else np.zeros(0, np.int64)
+            col, row = (u // n[a], u % n[a]) if n[a] else (u, u)
+            colptr = np.searchsorted(col, np.arange(n[b] + 1)).astype(np.int64)
+            na, nb = (0, 0) if a == b == "Λ" else (nder[a], nder[b])
+            pats[(a, b)] = (colptr, row.astype(np.int64), (n[a], n[b]), na, nb)
+    return pats
+
+
+class BigPattern:
+    """The sparsity of DirectXUA's all-steps matrix as Muscade lays it out: makepattern
+    (DirectXUA.jl:245-296) decides which blocks (step, class) × (step, class) are structurally
+    nonzero; SparseTools.prepare (SparseTools.jl:32-94) expands each block by its class-pair
+    pattern. Block rows are the steps' Λ, X, U blocks of every experiment in turn, then the A
+    block; `pgr` holds their first rows (0-based, the reference's Lvasm/Lvdis minus one).
+    colptr/rowval are 0-based."""
+
+    def __init__(self, IA, nstep, pats):
+        cls3 = CLASSES[:3]
+        blk = {}                                   # (block row, block column) → class pair, first appearance
+        cum = 0
+        for ns in nstep:
+            for istep in range(1, ns + 1):
+                for ia, a in enumerate(cls3):
+                    for ib, b in enumerate(cls3):
+                        _, _, _, na, nb = pats[(a, b)]
+                        for ad in range(na):
+                            for bd in range(nb):
+                                for Δa, _ in finitediff(ad, ns, istep):
+                                    for Δb, _ in finitediff(bd, ns, istep):
+                                        blk.setdefault((cum + 3 * (istep + Δa - 1) + ia, cum + 3 * (istep + Δb - 1) + ib),
+                                                       (a, b))
+            cum += 3 * ns
+        nblock = 3 * sum(nstep) + (1 if IA == 1 else 0)
+        if IA == 1:
+            Ab = nblock - 1
+            blk.setdefault((Ab, Ab), ("A", "A"))
+            cum = 0
+            for ns in nstep:
+                for istep in range(1, ns + 1):
+                    for ia, a in enumerate(cls3):
+                        if pats[("A", a)][3] > 0:
+                            blk.setdefault((Ab, cum + 3 * (istep - 1) + ia), ("A", a))
+                            blk.setdefault((cum + 3 * (istep - 1) + ia, Ab), (a, "A"))
+                cum += 3 * ns
+        size = {c: pats[(c, c)][2][0] for c in CLASSES}
+        nl = np.full(nblock, -1, np.int64)
+        for (r, c), (a, b) in blk.items():
+            nl[r] = size[a]
+            nl[c] = size[b]
+        if (nl < 0).any():
+            raise MuscadeError("invalid sparse-of-sparse pattern: a block row or column has only empty blocks")
+        self.pgr = np.concatenate([[0], np.cumsum(nl)]).astype(np.int64)
+        self.nv = int(self.pgr[-1])
+        # SparseTools.prepare: global column by global column, the blocks of the block column in
+        # increasing block row, each block's entries of that local column in its rowval order
+        bycol = {}
+        for (r, c), pair in blk.items():
+            bycol.setdefault(c, []).append((r, pair))
+        rows, cols = [], []
+        for c in range(nblock):
+            entries = sorted(bycol.get(c, []))
+            lr, lc, rk = [], [], []
+            for k, (r, pair) in enumerate(entries):
+                cp, rv, _, _, _ = pats[pair]
+                lc.append(np.repeat(np.arange(cp.size - 1), np.diff(cp)))
+                lr.append(rv + self.pgr[r])
+                rk.append(np.full(rv.size, k))
+            if not lr:
+                continue
+            lr, lc, rk = np.concatenate(lr), np.concatenate(lc), np.concatenate(rk)
+            o = np.lexsort((rk, lc))
+            rows.append(lr[o])
+            cols.append(lc[o] + self.pgr[c])
+        self.rowval = np.concatenate(rows).astype(np.int64) if rows else np.zeros(0, np.int64)
+        col = np.concatenate(cols).astype(np.int64) if cols else np.zeros(0, np.int64)
+        self.colptr = np.searchsorted(col, np.arange(self.nv + 1)).astype(np.int64)
+        self._keys = col * self.nv + self.rowval
+
+    @property
+    def nnz(self):
+        return int(self.rowval.size)
+
+    def locate(self, rows, cols):
+        """nz numbers of the entries (rows, cols); every entry must be in the pattern."""
+        k = np.asarray(cols, np.int64) * self.nv + np.asarray(rows, np.int64)
+        pos = np.searchsorted(self._keys, k)
+        if k.size and (pos.max() >= self._keys.size or not np.array_equal(self._keys[pos], k)):
+            raise MuscadeError("a contribution falls outside the all-steps pattern")
+        return pos
+
+    def coords(self):
+        """(rows, cols) of every nz, in nz order."""
+        return self.rowval, np.repeat(np.arange(self.nv), np.diff(self.colptr))
+
+    def matrix(self, rows, cols, vals):
+        """The matrix with this pattern holding the sum of the contributions (rows, cols, vals),
+        each nz summed in contribution order."""
+        pos = self.locate(rows, cols)
+        nz = np.bincount(pos, weights=vals, minlength=self.nnz) if pos.size else np.zeros(self.nnz)
+        return sp.csc_matrix((nz, self.rowval.copy(), self.colptr.copy()), shape=(self.nv, self.nv))
+
+
+def sparser(M, rtol=1e-20):
+    """sparser!(cLvv, Lvv, rtol) (SparseTools.jl:192-196, called with 1e-20 by DirectXUA.jl:486):
+    drop the stored entries below rtol·max|M|, before the LU."""
+    M = M.copy()
+    atol = rtol * np.abs(M.data).max() if M.nnz else 0.
+    if atol > 0.:
+        M.data[np.abs(M.data) < atol] = 0.
+        M.eliminate_zeros()
+    return M
+
+
 class HostLagrangians:
     """Gradient and Hessian of the host-evaluated elements' Lagrangians, in the layout of
     DirectXUA_lagrangian_addition! (src/DirectXUA.jl:121-150): partials Λ | X⁽⁰..OX⁾ | U⁽⁰..OU⁾
@@ -82,8 +212,8 @@ class HostLagrangians:
         self.gauge_provider = gauge_provider
 
     def _once_off(self, E):
-        f = getattr(E.cls, "once_off_of", None)
-        return bool(getattr(E.cls, "once_off", False) or (f is not None and E.nele and f(E.params[0])))
+        """Acost types: assembled once by assembleA! (Assemble.jl:507-519)."""
+        return bool(getattr(E.cls, "once_off", False))
 
     def _chunks(self, it):
         E = self.model.eletypes[it]
@@ -94,15 +224,16 @@ class HostLagrangians:
             yield p, d, slice(e0, e0 + m)
             e0 += m
 
-    def step(self, state):
+    def step(self, state, Λgradient=True):
         """Per-step contributions: L1[α][der] dense vectors, and a list of COO pieces
-        (α, i, β, j, rows, cols, vals) of L2 (0-based model dofs)."""
+        (α, i, β, j, rows, cols, vals) of L2 (0-based model dofs). Λgradient=False: host residual
+        elements with no_second_order add only L1[Λ] = R to the gradient, as the reference's
+        residual path does (DirectXUA.jl:104-119)."""
         L1 = {c: [np.zeros(self.ndof[c]) for _ in range(self.nder[c])] for c in CLASSES}
         L2 = []
         for it in self.types:
             E = self.model.eletypes[it]
-            if self._once_off(E):
-                continue
+            once = self._once_off(E)
             gd = None
             if it in self.costed:
                 if self.gauge_provider is None:
@@ -131,7 +262,13 @@ class HostLagrangians:
                 X = [seeded("X", der, iX, state.X[der][iX], d.scaleX) for der in range(self.OX + 1)]
                 U = [seeded("U", der, iU, state.U[der][iU], d.scaleU) for der in range(self.OU + 1)]
                 A = seeded("A", 0, iA, state.A[iA], d.scaleA)
-                if getattr(E.cls, "uses_SP", False):
+                if once:
+                    # The per-step element loop reaches Acost elements too: its Acost method
+                    # (Assemble.jl:487) is written for one element, but assemble! passes it the vector of
+                    # a type's elements, so they take the generic Lagrangian path (DirectXUA.jl:152-171)
+                    # at every step, with scaled partials, besides assembleA!'s once-off addition.
+                    L = E.cls.acost(p, A)
+                elif getattr(E.cls, "uses_SP", False):
                     L = E.cls.lagrangian(p, Λ, X, U, A, state.time, SP=state.SP)
                 else:
                     L = E.cls.lagrangian(p, Λ, X, U, A, state.time)
@@ -140,9 +277,11 @@ class HostLagrangians:
                 if not np.all(np.isfinite(L.g)) or not np.all(np.isfinite(L.h)):
                     raise MuscadeError(f"lagrangian({E.key[0]},...) returned NaN",
                                        dict(ieletyp=it + 1, t=state.time))
+                rpath = (not Λgradient and getattr(E.cls, "residual_host", False) and E.cls.no_second_order)
                 for (a, i, sa, ia) in layout:
                     na = ia.shape[1]
-                    np.add.at(L1[a][i], ia, L.g[:, sa:sa + na])
+                    if not rpath or a == "Λ":
+                        np.add.at(L1[a][i], ia, L.g[:, sa:sa + na])
                     for (b, j, sb, ib) in layout:
                         if a == "Λ" and b == "Λ":
                             continue
@@ -166,7 +305,7 @@ class HostLagrangians:
                 iA = d.A[sl] - 1
                 m, Na = iA.shape
                 A = [Dual2.seed(state.A[iA[:, k]], k, Na) for k in range(Na)]
-                C = E.cls.acost(p, A, SP=state.SP) if getattr(E.cls, "uses_SP", False) else E.cls.acost(p, A)
+                C = E.cls.acost(p, A)
                 if not isinstance(C, Dual2):
                     continue
                 np.add.at(gA, iA, C.g)
@@ -260,7 +399,10 @@ class DeviceBigSystem:
     device into the CSC pattern (mh_big_prepare); each iteration sums the contributions of every
     nz on the device (mh_big_assemble) and the host receives the finished values."""
 
-    def __init__(self, batched, nv, block, dt, hcontrib):
+    def __init__(self, batched, nv, block, dt, hcontrib, pattern=None):
+        """pattern (a BigPattern): every one of its entries is added as a structural contribution
+        of weight 0 after the real ones, so that the device pattern is Muscade's
+        (makepattern/preparebig) and each nz's sum is unchanged."""
         out = batched.out
         self.ctx, self.nv = out.ctx, nv
         OX, nX, nU, n = batched.OX, batched.nX, batched.nU, batched.nstep
@@ -308,6 +450,12 @@ class DeviceBigSystem:
             rows.append(np.asarray(r, np.int64)); cols.append(np.asarray(c, np.int64))
             src.append(o + np.arange(v.size)); w.append(np.asarray(wh, float))
             o += v.size
+        if pattern is not None:
+            pr, pc = pattern.coords()
+            rows.append(pr); cols.append(pc)
+            src.append(np.full(pr.size, o, np.int64)); w.append(np.zeros(pr.size))
+            o += 1                                   # one host value, always 0, for them all
+        self.structural = pattern is not None
         cat = lambda a, t: np.ascontiguousarray(np.concatenate(a) if a else np.zeros(0), dtype=t)   # noqa: E731
         rows, cols, src, w = cat(rows, np.int64), cat(cols, np.int64), cat(src, np.int64), cat(w, np.float64)
         self.nhost = o - ndev
@@ -324,10 +472,9 @@ class DeviceBigSystem:
     def matches(self, hcontrib):
         return [v.size for (_, _, _, v) in hcontrib] == self.hsizes
 
-    @staticmethod
-    def _host_values(hcontrib):
-        return np.ascontiguousarray(np.concatenate([v for (_, _, _, v) in hcontrib]) if hcontrib else np.zeros(0),
-                                    dtype=np.float64)
+    def _host_values(self, hcontrib):
+        v = [v for (_, _, _, v) in hcontrib] + ([np.zeros(1)] if self.structural else [])
+        return np.ascontiguousarray(np.concatenate(v) if v else np.zeros(0), dtype=np.float64)
 
     def prepare_solve(self, per_step, nstep, na):
         """Lay the system out for the device solve (mh_big_solve_prepare): dense diagonal blocks
@@ -355,8 +502,14 @@ class DeviceBigSystem:
         nz = np.zeros(self.rowval.size)
         self.ctx.check(lib.mh_big_assemble(self.ctx.h, dptr(hv) if hv.size else None, dptr(nz)), "big assemble")
         M = sp.csc_matrix((nz, self.rowval.copy(), self.colptr.copy()), shape=(self.nv, self.nv))
-        M.eliminate_zeros()   # as the host path: structural entries that are zero leave the LU's pattern
-        return M
+        return sparser(M)   # as the host path: sparser!(cLvv, Lvv, 1e-20) before the LU
+
+
+class _NoDeviceKinds:
+    """The per-step assembler of a model with no device element kind: nothing to add."""
+
+    def assemble(self, state, Λ, mission="matrices", Δt=1.0, dbg=None):
+        return {}, {}
 
 
 class _DirectXUAMeta(type):
@@ -374,7 +527,7 @@ class DirectXUA(metaclass=_DirectXUAMeta):
     @classmethod
     def run(cls, pstate, verbose, dbg, *, time, initialstate, maxiter=50, maxΔλ=1e-5, maxΔx=1e-5, maxΔu=1e-5,
             maxΔa=1e-5, saveiter=False, device=0, Assembly=AssemblyDirect, batch_steps=False, big="host", linsolve="host",
-            stats=None):
+            stats=None, Λgradient=True):
         """`Assembly` is the per-step assembler of the device element kinds (prepare(AssemblyDirect{OX,OU,IA}),
         src/DirectXUA.jl:22-56). batch_steps: all steps of an experiment in one device launch per
         Newton iteration (StepBatchAssembly; beams only). big: "host" assembles the all-steps
@@ -383,7 +536,12 @@ class DirectXUA(metaclass=_DirectXUAMeta):
         factorizes the system with SciPy's sparse LU (lu(Lvv), DirectXUA.jl:489-490), "device"
         factorizes it where it was assembled (big="device"; DeviceBigSystem.solve, a dense block
         LU of the steps). stats (a dict) receives the iteration count and the seconds spent in
-        device assembly, host assembly and the linear solve (lu_s)."""
+        device assembly, host assembly and the linear solve (lu_s).
+
+        Λgradient: add Λᵀ∂R/∂(X,U) of the residual-path elements (the device kinds, and host
+        residual elements with no_second_order) to the gradient, so that Λ converges to the
+        multiplier (the default). Λgradient=False leaves it out, as DirectXUA.jl:104-119 does: the
+        reference's own iterates."""
         clock = _time.perf_counter
         st_ = stats if stats is not None else {}
         st_.update(iterations=0, device_assembly_s=0., host_assembly_s=0., lu_s=0.)
@@ -426,7 +584,10 @@ class DirectXUA(metaclass=_DirectXUAMeta):
                        for n in nstep]
             provider = None
         else:
-            out = Assembly(OX, OU, IA, model, dis, device=device, skip=tuple(host.skip))
+            if Assembly is AssemblyDirect and len(host.skip) == len(model.eletypes):
+                out = _NoDeviceKinds()          # every element is evaluated on the host
+            else:
+                out = Assembly(OX, OU, IA, model, dis, device=device, skip=tuple(host.skip))
             provider = getattr(out, "gauge_derivatives", None)
         if host.costed:
             if provider is None:
@@ -437,6 +598,11 @@ class DirectXUA(metaclass=_DirectXUAMeta):
         exp0 = np.cumsum([0] + [n * per_step for n in nstep])
         nv = int(exp0[-1]) + (nA if IA else 0)   # A-dofs are unknowns only when IA = 1
         Aoff = int(exp0[-1])
+        # Muscade's all-steps pattern (makepattern + preparebig, DirectXUA.jl:245-304)
+        bigpat = BigPattern(IA, nstep, class_patterns(model, dis, OX, OU, IA))
+        if bigpat.nv != nv:
+            raise MuscadeError(f"all-steps pattern of order {bigpat.nv}, expected {nv}")
+        st_["pattern"] = bigpat
         coff = {"Λ": 0, "X": nX, "U": 2 * nX}
 
         def block(iexp, istep, c):   # istep 1-based
@@ -476,11 +642,11 @@ class DirectXUA(metaclass=_DirectXUAMeta):
                         t0 = clock()
                         L1d, L2d = out.assemble(st, st.Λ[0], "matrices", 1.0, dbg=dict(dbg, iter=it, step=istep))
                         t_dev += clock() - t0
-                    L1h, L2h = host.step(st)
+                    L1h, L2h = host.step(st, Λgradient)
                     # the device kinds take the residual path, which leaves Λᵀ∂R/∂(X,U) out of L1
                     # (src/DirectXUA.jl:104-119); add it from the same Jacobian blocks
                     for b, pair in (("X", ("Λ", "X")), ("U", ("Λ", "U"))):
-                        if pair in L2d and L1h[b]:
+                        if Λgradient and pair in L2d and L1h[b]:
                             for bder, M in enumerate(L2d[pair][0][:nder[b]]):
                                 L1h[b][bder] = L1h[b][bder] + M.T @ st.Λ[0]
                     for b in classes:
@@ -523,7 +689,8 @@ class DirectXUA(metaclass=_DirectXUAMeta):
             t_lin = 0.
             if big == "device":
                 if dbig is None or not dbig.matches(hcontrib):
-                    dbig = DeviceBigSystem(batched[0], nv, lambda s_, c_: block(0, s_, c_), Δt[0], hcontrib)
+                    dbig = DeviceBigSystem(batched[0], nv, lambda s_, c_: block(0, s_, c_), Δt[0], hcontrib,
+                                           pattern=bigpat)
                     if linsolve == "device":
                         st_["block_order"], st_["blocks"] = dbig.prepare_solve(per_step, nstep[0], nv - Aoff)
                 t0 = clock()
@@ -535,10 +702,9 @@ class DirectXUA(metaclass=_DirectXUAMeta):
                     Lvv = dbig.assemble(hcontrib)
                     t_dev += clock() - t0
             else:
-                Lvv = sp.coo_matrix((np.concatenate(vals) if vals else np.zeros(0),
-                                     (np.concatenate(rows) if rows else np.zeros(0, int),
-                                      np.concatenate(cols) if cols else np.zeros(0, int))), shape=(nv, nv)).tocsc()
-                Lvv.eliminate_zeros()
+                Lvv = sparser(bigpat.matrix(np.concatenate(rows) if rows else np.zeros(0, np.int64),
+                                            np.concatenate(cols) if cols else np.zeros(0, np.int64),
+                                            np.concatenate(vals) if vals else np.zeros(0)))
             st_["device_assembly_s"] += t_dev
             st_["host_assembly_s"] += clock() - t_iter - t_dev - t_lin
             st_["iterations"] = it

@@ -305,7 +305,8 @@ class DofConstraint:
     — src/BasicElements.jl:349-460. Gap g ≥ 0, multiplier λ ≥ 0; mode(t) ∈ equal, positive, off.
       λclass "X": residual R = (−∂g/∂x·λ, −g | −S(λ,g,γ) | −λ for off), gap(x, t, gargs...);
       λclass "U": L = −g·λ | −KKT(λ,g,γ) | −½λ², gap(x, u, a, t, gargs...), λ a U-dof;
-      λclass "A": the same over A-dofs only, gap(a, gargs...), once-off.
+      λclass "A": the same over A-dofs only, gap(a, gargs...), at every step (like any element
+      with a Lagrangian; assembleA! takes Acost elements only).
     The gap is a user function, so the element is evaluated on the host: in SweepX as a
     host-evaluated group (R and ∂R/∂X handed to the device scatter, MH_KIND_HOST), in DirectXUA
     as a host Lagrangian (the X class by the residual path: R, ∂R/∂X, no X-X block, like the
@@ -334,10 +335,6 @@ class DofConstraint:
                getattr(mode, "__name__", str(mode)))
         return key, (inod, dclass, field), {"gap": gap, "gargs": tuple(gargs), "mode": mode, "λclass": λclass,
                                             "N": (Nx, Nu, Na)}
-
-    @staticmethod
-    def once_off_of(p):
-        return p["λclass"] == "A"
 
     @staticmethod
     def residual_blocks(p, Xv, t, γ=0.):
@@ -389,6 +386,9 @@ class DofConstraint:
                 L = L + Λ[k] * Rk
             return L
         mode = p["mode"](t)
+        if p["λclass"] == "A":
+            # gap(a, gargs...), λ the last A-dof (BasicElements.jl:449-458), at every step's t
+            return DofConstraint._mode_lagrangian(mode, A[Na], call_cost(p["gap"], (A[:Na],), p["gargs"]), γ)
         x = X[0][:Nx]
         u = U[0][:Nu]
         a = A[:Na] if Na else []
@@ -406,13 +406,123 @@ class DofConstraint:
             return -0.5 * (lam * lam)
         raise MuscadeError(f"unknown constraint mode {mode!r}")
 
+
+
+class ResidualElement:
+    """Base of element types whose residual is a host function: Muscade's element API
+    (src/ElementAPI.jl:79-173: `doflist`, `residual`, `no_second_order`) for elements that have no
+    device kernel, such as user and test elements. A subclass defines
+
+      doflist(cls) -> (inod, class, field)    as Muscade.doflist (ElementAPI.jl:79-99);
+      make(cls, model, nodes0, **kw) -> dict  the parameters of the elements of one addelement
+                                              call (the element constructor E(nod; kwargs...),
+                                              ModelDescription.jl:3-15, 190-199);
+      type_key(cls, **kw) -> tuple            what makes a distinct concrete type (a Julia type
+                                              parameter such as Spring{D}'s D; default none);
+      residual(p, X, U, A, t) -> sequence     the Nx residual components, as Muscade.residual
+                                              (ElementAPI.jl:144-173). X[d][k] is time derivative
+                                              d (0..2) of the element's k-th X-dof, U[d][k]
+                                              likewise, A[k] its k-th A-dof; each is a Dual2 or a
+                                              float array over the elements of the call, and the
+                                              derivatives the solver does not carry are zeros.
+
+    no_second_order = False (ElementAPI.jl:110): DirectXUA differentiates L = Λ·R to second
+    order (the Lagrangian path, DirectXUA.jl:152-171); True: the residual path (:85-120).
+    In SweepX the host evaluates R and its derivative along the solver's trial path
+    (SweepX.jl:39-96) and the device scatters them with everything else (MH_KIND_HOST)."""
+    kind = _lib.MH_KIND_HOST
+    host = True
+    residual_host = True
+    no_second_order = False
+
+    @classmethod
+    def doflist(cls):
+        raise MuscadeError(f"method 'doflist' must be provided for elements of type '{cls.__name__}'")
+
+    @classmethod
+    def make(cls, model, nodes0, **kw):
+        return dict(kw)
+
+    @classmethod
+    def type_key(cls, **kw):
+        return ()
+
+    @classmethod
+    def construct(cls, model, nodes0, **kw):
+        return (cls.__name__,) + tuple(cls.type_key(**kw)), cls.doflist(), cls.make(model, nodes0, **kw)
+
     @staticmethod
-    def acost(p, A, SP=None):
-        """λclass A, once-off: L = −g·λ | −KKT(λ,g,γ) | −½λ² with gap(a, gargs...)."""
-        γ = float((SP or {}).get("γ", 0.))
-        Na = p["N"][2]
-        g = call_cost(p["gap"], (A[:Na],), p["gargs"])
-        return DofConstraint._mode_lagrangian(p["mode"](0.), A[Na], g, γ)
+    def residual(p, X, U, A, t):
+        raise MuscadeError("method 'residual' must be provided")
+
+    @classmethod
+    def counts(cls):
+        _, dclass, _ = cls.doflist()
+        return dclass.count("X"), dclass.count("U"), dclass.count("A")
+
+    @classmethod
+    def _residual(cls, p, X, U, A, t):
+        m = np.shape(X[0][0].v if isinstance(X[0][0], Dual2) else X[0][0])[0] if X and X[0] else 0
+        R = list(cls.residual(p, X, U, A, t))
+        Nx = cls.counts()[0]
+        if len(R) != Nx:
+            raise MuscadeError(f"residual of {cls.__name__} returned {len(R)} components, expected {Nx}")
+        return [r if isinstance(r, Dual2) else np.broadcast_to(np.asarray(r, float), (m,)) for r in R]
+
+    @classmethod
+    def sweepx_blocks(cls, p, Xv, Uv, Av, t, c, mission, OX):
+        """R (m, Nx), ∂R/∂δX (m, Nx, Nx) and, on a :step of OX ≥ 1, ∂R/∂δr (m, Nx), unscaled, at the
+        element values Xv[d] (m, Nx), Uv (m, Nu), Av (m, Na): the trial path of SweepX{OX}'s addin!
+        (SweepX.jl:45-96) vx = x + δX, vx′ = x′ + a₁δX + a·δr, vx″ = x″ + b₁δX + b·δr, with
+        a = a₂x′ + a₃x″, b = b₂x′ + b₃x″ on :step."""
+        m, Nx = Xv[0].shape
+        step = mission == "step" and OX > 0
+        P = Nx + (1 if step else 0)
+        zero = np.zeros(m)
+        X = [[None] * Nx for _ in range(3)]
+        x1 = Xv[1] if OX >= 1 else np.zeros((m, Nx))
+        x2 = Xv[2] if OX >= 2 else np.zeros((m, Nx))
+        a = (c.a2 * x1 + c.a3 * x2) if step else None
+        b = (c.b2 * x1 + c.b3 * x2) if step else None
+        for k in range(Nx):
+            g0 = np.zeros((m, P)); g0[:, k] = 1.
+            X[0][k] = Dual2(Xv[0][:, k].astype(float), g0, np.zeros((m, P, P)))
+            if OX >= 1:
+                g1 = np.zeros((m, P)); g1[:, k] = c.a1
+                if step:
+                    g1[:, Nx] = a[:, k]
+                X[1][k] = Dual2(x1[:, k].astype(float), g1, np.zeros((m, P, P)))
+            else:
+                X[1][k] = zero
+            if OX >= 2:
+                g2 = np.zeros((m, P)); g2[:, k] = c.b1
+                if step:
+                    g2[:, Nx] = b[:, k]
+                X[2][k] = Dual2(x2[:, k].astype(float), g2, np.zeros((m, P, P)))
+            else:
+                X[2][k] = zero
+        U = [[Uv[:, k] for k in range(Uv.shape[1])], [zero] * Uv.shape[1], [zero] * Uv.shape[1]]
+        A = [Av[:, k] for k in range(Av.shape[1])]
+        R = cls._residual(p, X, U, A, t)
+        Rv = np.stack([r.v if isinstance(r, Dual2) else r for r in R], axis=1)
+        G = np.stack([r.g if isinstance(r, Dual2) else np.zeros((m, P)) for r in R], axis=1)   # m, Nx, P
+        return Rv, G[:, :, :Nx].copy(), (G[:, :, Nx].copy() if step else None)
+
+    @classmethod
+    def lagrangian(cls, p, Λ, X, U, A, t):
+        """L = Λ∘₁R (getlagrangian, src/Assemble.jl:707-736) over the solver's partials. With
+        no_second_order the residual is first order (revariate{1}, DirectXUA.jl:96-102): its
+        second derivatives are dropped, so L keeps only the Λ-β cross terms."""
+        zero = 0. * (X[0][0].v if isinstance(X[0][0], Dual2) else np.asarray(X[0][0], float))
+        Xp = [list(X[d]) if d < len(X) else [zero] * len(X[0]) for d in range(3)]
+        Up = [list(U[d]) if d < len(U) else [zero] * len(U[0]) for d in range(3)] if U and len(U[0]) else [[], [], []]
+        R = cls._residual(p, Xp, Up, list(A), t)
+        if cls.no_second_order:
+            R = [Dual2(r.v, r.g, np.zeros_like(r.h)) if isinstance(r, Dual2) else r for r in R]
+        L = Λ[0] * R[0]
+        for k in range(1, len(R)):
+            L = L + Λ[k] * R[k]
+        return L
 
 
 class DofLoad:
@@ -501,8 +611,10 @@ class SingleUdof:
 
 
 class Acost:
-    """Acost(nod; inod, field, cost, costargs) — src/BasicElements.jl:77-86: a once-off cost
-    cost(A, costargs...) on A-dofs, assembled once (assembleA!, src/Assemble.jl:507-519), not per step."""
+    """Acost(nod; inod, field, cost, costargs) — src/BasicElements.jl:77-86: a cost cost(A, costargs...)
+    on A-dofs. DirectXUA adds it once through assembleA! (src/Assemble.jl:507-519, unscaled partials),
+    and, like the reference's per-step element loop, again at every step through the Lagrangian path
+    (scaled partials; see HostLagrangians.step)."""
     kind = _lib.MH_KIND_ZERO
     host = True
     once_off = True

@@ -95,6 +95,22 @@ template <int N, class T> static bool has_nan(const Dual<N, T>& v) {
 
 extern "C" {
 
+// ---------------------------------------------------------------- the power rule
+// x^b (Adiff.jl:230) on a three-level nested dual x = W + W′ε₃, W = V + V′ε₂, W′ = Q + Q′ε₂,
+// V = v + v′ε₁, V′ = p + p′ε₁, Q = q + q′ε₁, Q′ = r + r′ε₁ (ε₁ innermost): in and out are
+// {v, v′, p, p′, q, q′, r, r′}. A probe of the rule for the tests.
+void orc_ipow_probe(const double* in, int b, double* out) {
+    using D1 = Dual<1, double>;
+    using D2 = Dual<1, D1>;
+    using D3 = Dual<1, D2>;
+    D3 x;
+    x.x.x.x = in[0]; x.x.x.dx[0] = in[1]; x.x.dx[0].x = in[2]; x.x.dx[0].dx[0] = in[3];
+    x.dx[0].x.x = in[4]; x.dx[0].x.dx[0] = in[5]; x.dx[0].dx[0].x = in[6]; x.dx[0].dx[0].dx[0] = in[7];
+    const D3 y = ipow(x, b);
+    out[0] = y.x.x.x; out[1] = y.x.x.dx[0]; out[2] = y.x.dx[0].x; out[3] = y.x.dx[0].dx[0];
+    out[4] = y.dx[0].x.x; out[5] = y.dx[0].x.dx[0]; out[6] = y.dx[0].dx[0].x; out[7] = y.dx[0].dx[0].dx[0];
+}
+
 // ---------------------------------------------------------------- rotations
 double orc_sinc1_k(int k, double x) {
     switch (k) { case 0: return sinc1_0(x); case 1: return sinc1_1(x); case 2: return sinc1_2(x);

@@ -19,6 +19,7 @@ _V = ctypes.c_void_p
 K_BEAM, K_HOLD, K_DOFLOAD, K_ZERO, K_TURBINE, K_ANCHOR, K_BAR = 0, 1, 2, 3, 4, 5, 6
 
 _SIG = {
+    "orc_ipow_probe": (None, [_D, ctypes.c_int, _D]),
     "orc_sinc1_k": (ctypes.c_double, [ctypes.c_int, ctypes.c_double]),
     "orc_scac_d": (ctypes.c_double, [ctypes.c_int, ctypes.c_double]),
     "orc_rodrigues": (None, [_D, _D]),
@@ -77,6 +78,12 @@ def ip64(a):
 
 def ip(a):
     return None if a is None else a.ctypes.data_as(_IP)
+
+
+def ipow_probe(x8, b):
+    out = np.zeros(8)
+    lib.orc_ipow_probe(dp(f64(x8)), int(b), dp(out))
+    return out
 
 
 def beam_construct(c1, c2, orient2):

@@ -156,3 +156,76 @@ class OracleAssemblyDirect:
                     blocks.append(row)
                 L2[(cls[a], cls[b])] = blocks
         return L1, L2
+
+
+class HostSweepXAssembly:
+    """AssemblySweepX{OX} for models whose elements are all host residual elements
+    (muscade_amd.ResidualElement): the product's host evaluation of R, ∂R/∂δX and ∂R/∂δr along the
+    trial path, scattered on the CPU with add_value!/add_∂! (src/Assemble.jl:559-608) in element
+    order. It lets the product's SweepX loop run on a CPU against the reference's solver goldens
+    (test infrastructure only)."""
+
+    def __init__(self, OX, model, dis, device=0):
+        import scipy.sparse as sp
+        from muscade_amd.assemble import Newmarkβcoefficients
+        self._sp = sp
+        self.OX, self.model, self.dis = OX, model, dis
+        self.nX = model.ndof["X"]
+        self.c = Newmarkβcoefficients(OX)
+        from muscade_amd import _lib
+        self._point = {_lib.MH_KIND_HOLD: "hold", _lib.MH_KIND_DOFLOAD: "load", _lib.MH_KIND_ZERO: "zero"}
+        for E in model.eletypes:
+            if not getattr(E.cls, "residual_host", False) and E.cls.kind not in self._point:
+                raise TypeError(f"{E.key[0]} is neither a host residual element nor Hold, DofLoad or a cost")
+        self.Lλ = np.zeros(self.nX)
+        self._K = None
+
+    def assemble(self, state, mission, dbg=None, fetch=True):
+        nX = self.nX
+        rhs = np.zeros(nX)
+        rows, cols, vals = [], [], []
+        for it, E in enumerate(self.model.eletypes):
+            d = self.dis.dis[it]
+            s = d.scaleX
+            e0 = 0
+            point = self._point.get(E.cls.kind) if not getattr(E.cls, "residual_host", False) else None
+            for nodes, p in zip(E.nodes, E.params):
+                m = nodes.shape[0]
+                iX = d.X[e0:e0 + m] - 1
+                if point is not None:   # Hold (BasicElements.jl:426-438), DofLoad (:275-284), cost elements
+                    e0 += m
+                    if point == "zero" or iX.shape[1] == 0:
+                        continue
+                    if point == "hold":
+                        x, lam = state.X[0][iX[:, 0]], state.X[0][iX[:, 1]]
+                        R = np.stack([-lam, -x], axis=1)
+                        K = np.broadcast_to(np.array([[0., -1.], [-1., 0.]]), (m, 2, 2))
+                    else:
+                        R = np.full((m, 1), -float(p["value"](state.time, **p["args"])))
+                        K = np.zeros((m, 1, 1))
+                    for e in range(m):
+                        np.add.at(rhs, iX[e], R[e] * s)
+                        rows.append(np.repeat(iX[e], iX.shape[1]))
+                        cols.append(np.tile(iX[e], iX.shape[1]))
+                        vals.append((K[e] * s[:, None] * s[None, :]).ravel())
+                    continue
+                Xv = [state.X[k][iX] if k < len(state.X) else np.zeros(iX.shape) for k in range(self.OX + 1)]
+                Uv = state.U[0][d.U[e0:e0 + m] - 1] if d.U.shape[1] else np.zeros((m, 0))
+                Av = state.A[d.A[e0:e0 + m] - 1] if d.A.shape[1] else np.zeros((m, 0))
+                R, K, dR = E.cls.sweepx_blocks(p, Xv, Uv, Av, state.time, self.c, mission, self.OX)
+                for e in range(m):
+                    np.add.at(rhs, iX[e], R[e] * s)
+                    if dR is not None:
+                        np.add.at(rhs, iX[e], -(dR[e] * s))
+                    rows.append(np.repeat(iX[e], iX.shape[1]))
+                    cols.append(np.tile(iX[e], iX.shape[1]))
+                    vals.append((K[e] * s[:, None] * s[None, :]).ravel())
+                e0 += m
+        cat = (lambda a: np.concatenate(a)) if rows else (lambda a: np.zeros(0))
+        self._K = self._sp.coo_matrix((cat(vals), (cat(rows).astype(int), cat(cols).astype(int))),
+                                      shape=(nX, nX)).tocsc()
+        self.Lλ = rhs
+        return rhs, self._K.data
+
+    def Lλx(self):
+        return self._K.copy()

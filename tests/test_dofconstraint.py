@@ -63,3 +63,39 @@ def test_X_class_in_directxua_is_residual_path():
     assert np.allclose(L.g[0, :3], R[0]) and np.allclose(L.g[0, 3:], K[0].T @ lam)
     assert np.allclose(L.h[0, :3, 3:], K[0]) and np.allclose(L.h[0, 3:, :3], K[0].T)
     assert np.all(L.h[0, 3:, 3:] == 0.)
+
+
+def test_prescribed_motion_is_not_a_hold():
+    """A DofConstraint{:X} whose gap is not Hold's (the prescribed motion horizMove of
+    examples/DynamicBeamAnalysis.jl:114-116, gap = x − (1 − e^min(t,0))·181 − xMotion(t)) is never
+    routed to the Hold kernel: it is a host-evaluated group, and its λ-row residual is −gap(x, t)
+    at the element's own t. Even a user gap equal to x[1] stays a host group: only Hold's own
+    constructor gives the Hold kernel (its gap, BasicElements.jl:484-488)."""
+    import math
+    from muscade_amd import _lib
+    import muscade_amd as mc
+
+    def x_motion(t):
+        return 0.5 * math.sin(0.7 * t)
+
+    def horiz_move(x, t):
+        return x[0] - ((1.0 - math.exp(min(t, 0.))) * 181.0 + x_motion(t))
+
+    def same_as_hold(x, t):
+        return x[0]
+
+    model = mc.Model("prescribed")
+    n = mc.addnode(model, [0., 0., 0.])
+    mc.addelement(model, mc.DofConstraint, [n], xinod=(1,), xfield=("t1",), λinod=1, λclass="X", λfield="λt1",
+                  gap=horiz_move, mode=mc.equal)
+    mc.addelement(model, mc.DofConstraint, [n], xinod=(1,), xfield=("t3",), λinod=1, λclass="X", λfield="λt3",
+                  gap=same_as_hold, mode=mc.equal)
+    mc.addelement(model, mc.Hold, [n], field="t2")
+    kinds = [E.cls.kind for E in model.eletypes]
+    assert kinds == [_lib.MH_KIND_HOST, _lib.MH_KIND_HOST, _lib.MH_KIND_HOLD]
+    E = model.eletypes[0]
+    for t in (-3., 0.4, 2.):
+        R, K = E.cls.residual_blocks(E.params[0], np.array([[0.25, 3.]]), t)
+        assert R[0, 1] == pytest.approx(-horiz_move([0.25], t), rel=1e-15)
+        assert R[0, 0] == pytest.approx(-3., rel=1e-15)            # −∂gap/∂x·λ
+        assert np.allclose(K[0], [[0., -1.], [-1., 0.]])

@@ -239,7 +239,9 @@ def test_constrained_load_identification():
 
 def test_off_constraint_leaves_lambda_zero():
     target = gauge_target(3e4)
-    inv = gauge_cantilever_constrained(target=target, mode=mc.off)
+    # the load's prior must survive sparser!(·, 1e-20) before the LU (DirectXUA.jl:486): with the
+    # constraint off nothing else holds u, and 2·cu must exceed 1e-20·max|Lvv|
+    inv = gauge_cantilever_constrained(target=target, mode=mc.off, cu=1e-9)
     out = mc.solve(mc.DirectXUA[0, 0, 0], initialstate=[mc.initialize(inv, time=0.)], Assembly=OracleAssemblyDirect,
                    **SOLVE_KW)
     st = out[0][0]

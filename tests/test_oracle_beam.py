@@ -224,3 +224,103 @@ def test_rigid_body_motion_is_stress_free():
     X = np.concatenate([u1, v, u2, v])
     R, _, _ = orc.beam_residual_plain(c1, c2, [0., 0., 1.], m, [X])
     assert np.abs(R).max() <= 1e-13
+
+
+# ---------------------------------------------------------------- the power rule of the time jets
+def test_ipow_three_level_square_hand_derived():
+    """Adiff.jl:230, a^b = b==0 ? zero(a) : ∂ℝ(a.x^b, a.dx·b·a.x^(b−1)), worked by hand on
+    x = W + W′ε₃, W = V + V′ε₂, W′ = Q + Q′ε₂, V = v + v′ε₁, V′ = p + p′ε₁, Q = q + q′ε₁, Q′ = r + r′ε₁:
+    x² = (W², W′·2·W¹) with W² = (V², V′·2·V¹), V¹ = (v, v′) (v is a Float64: v⁰ = 1), but
+    W¹ = (V¹, V′·1·V⁰) = (V, 0) because V is a dual and V⁰ = zero(V). Hence
+    x² = [v², 2vv′, 2pv, 2(pv′ + p′v), 2qv, 2(qv′ + q′v), 2rv, 2(rv′ + r′v)]: against the exact
+    square, the ε₃ε₂ and ε₃ε₂ε₁ parts lack 2qp and 2(qp′ + q′p). Small integers keep every
+    product exact."""
+    v, v1, p, p1, q, q1, r, r1 = 3., -2., 5., 7., -4., 6., 2., -3.
+    got = orc.ipow_probe([v, v1, p, p1, q, q1, r, r1], 2)
+    muscade = [v * v, 2 * v * v1, 2 * p * v, 2 * (p * v1 + p1 * v), 2 * q * v, 2 * (q * v1 + q1 * v), 2 * r * v,
+               2 * (r * v1 + r1 * v)]
+    exact_last = [2 * (r * v + q * p), 2 * (r * v1 + r1 * v + q * p1 + q1 * p)]
+    assert list(got) == muscade
+    assert list(got[6:]) != exact_last
+    # x¹ = (W¹, W′·1·W⁰) = ((v, v′), 0): the rule drops every outer part of a first power
+    assert list(orc.ipow_probe([v, v1, p, p1, q, q1, r, r1], 1)) == [v, v1, 0., 0., 0., 0., 0., 0.]
+    # x³ = (W³, W′·3·W²) with W² exact: the cube equals the exact product x·x·x
+    x = np.array([v, v1, p, p1, q, q1, r, r1])
+
+    def mul(a, b):   # exact product of two such three-level duals (multilinear in ε₁, ε₂, ε₃)
+        A = {(i, j, k): a[4 * k + 2 * j + i] for i in (0, 1) for j in (0, 1) for k in (0, 1)}
+        B = {(i, j, k): b[4 * k + 2 * j + i] for i in (0, 1) for j in (0, 1) for k in (0, 1)}
+        c = np.zeros(8)
+        for (i, j, k), av in A.items():
+            for (i2, j2, k2), bv in B.items():
+                if i + i2 <= 1 and j + j2 <= 1 and k + k2 <= 1:
+                    c[4 * (k + k2) + 2 * (j + j2) + (i + i2)] += av * bv
+        return c
+    assert np.array_equal(orc.ipow_probe(x, 3), mul(mul(x, x), x))
+    assert np.array_equal(orc.ipow_probe(x, 2)[:6], mul(x, x)[:6])
+
+
+def _moving_state(rng, udof):
+    c1 = rng.normal(size=3)
+    c2 = c1 + np.array([2.0, 0.3, -0.2]) + 0.2 * rng.normal(size=3)
+    o2 = np.array([0.1, 1.0, 0.2])
+    mat = np.zeros(16)
+    mat[:4] = rng.uniform(1., 10., size=4)
+    mat[4] = rng.uniform(0.5, 2.)          # μ
+    mat[5] = rng.uniform(0.1, 1.)          # ι₁
+    mat[6] = rng.uniform(-2., 2.)          # w
+    mat[7:16] = rng.uniform(0., 1., size=9)
+    X = [rng.normal(size=12) * np.r_[[1e-2] * 3, [0.2] * 3, [1e-2] * 3, [0.2] * 3],
+         rng.normal(size=12) * 0.5, rng.normal(size=12) * 0.5]
+    U = rng.normal(size=3) if udof else None
+    return c1, c2, o2, mat, X, U
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_dynamic_tangent_matches_central_differences(seed):
+    """∂R/∂X, ∂R/∂X′, ∂R/∂X″ and ∂R/∂U of the dynamic beam (ND = 3: inertia, added mass, linear
+    and quadratic drag, roll inertia, weight, a distributed U-load) at random moving states,
+    against central differences of R. R is taken as the oracle evaluates it on the solver's duals,
+    power rule included: the rule drops the same terms from the value and from its partials
+    (zero(a) zeroes every level below), so the tangent is the exact derivative of that R and
+    central differences pin it; the rule itself is pinned by the hand-derived test above."""
+    rng = np.random.default_rng(100 + seed)
+    c1, c2, o2, mat, X, U = _moving_state(rng, udof=True)
+    R0, Ks, KU = orc.beam_diffed_residual(c1, c2, o2, mat, X, U)
+    for d in range(3):
+        h = 1e-6
+        FD = np.zeros((12, 12))
+        for k in range(12):
+            Xp = [x.copy() for x in X]
+            Xm = [x.copy() for x in X]
+            Xp[d][k] += h
+            Xm[d][k] -= h
+            FD[:, k] = (orc.beam_diffed_residual(c1, c2, o2, mat, Xp, U)[0]
+                        - orc.beam_diffed_residual(c1, c2, o2, mat, Xm, U)[0]) / (2 * h)
+        scale = np.abs(Ks[d]).max()
+        assert np.abs(FD - Ks[d]).max() <= 1e-6 * scale, (d, np.abs(FD - Ks[d]).max() / scale)
+    FDU = np.zeros((12, 3))
+    for c in range(3):
+        Up, Um = U.copy(), U.copy()
+        Up[c] += 1e-6
+        Um[c] -= 1e-6
+        FDU[:, c] = (orc.beam_diffed_residual(c1, c2, o2, mat, X, Up)[0]
+                     - orc.beam_diffed_residual(c1, c2, o2, mat, X, Um)[0]) / 2e-6
+    assert np.abs(FDU - KU).max() <= 1e-6 * np.abs(KU).max()
+    assert np.abs(Ks[1]).max() > 0. and np.abs(Ks[2]).max() > 0.
+
+
+def test_power_rule_changes_the_dynamic_residual_only():
+    """Where the rule acts on the beam: the residual on the solver's duals (three levels) and the
+    residual on plain X (two levels, where the dropped V⁰ is a Float64 power and equals 1) agree
+    at rest in time (X′ = X″ = 0) and at constant rotation rate zero; with moving rotations they
+    differ, the difference being the 2ṡ² term of sinc1(θ/2)² that the rule removes."""
+    rng = np.random.default_rng(7)
+    c1, c2, o2, mat, X, _ = _moving_state(rng, udof=False)
+    still = [X[0], np.zeros(12), np.zeros(12)]
+    Rd = orc.beam_diffed_residual(c1, c2, o2, mat, still)[0]
+    Rp = orc.beam_residual_plain(c1, c2, o2, mat, still)[0]
+    assert np.abs(Rd - Rp).max() <= 1e-12 * np.abs(Rp).max()
+    Rd = orc.beam_diffed_residual(c1, c2, o2, mat, X)[0]
+    Rp = orc.beam_residual_plain(c1, c2, o2, mat, X)[0]
+    assert np.abs(Rd - Rp).max() > 1e-9 * np.abs(Rp).max()
