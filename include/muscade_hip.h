@@ -186,6 +186,14 @@ int mh_beam_results(mh_ctx* ctx, int igroup, int ND, int ngauge, const double* g
  * unscaled model coordinates, exactly as revariate{2} (src/Taylor.jl:143-151) forms them.
  * out: 364×n. The gauge strains are linear in these (E, K1, K2, K3). */
 int mh_beam_gauge_derivs(mh_ctx* ctx, int igroup, int64_t nsel, const int64_t* sel, double* out);
+/* mh_beam_gauge_derivs that allocates nothing. It has no counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments and results. The selection and the
+ * output staging live in a fixed device array of 11.9 MB that each GPU the library runs on
+ * holds until the process ends, so a call takes at most 4096 elements (MH_ERR_SIZE beyond).
+ * Every call on one GPU shares that array: calls that overlap on one GPU (contexts driven
+ * from several host threads) race. */
+int mh_beam_gauge_derivs_noalloc(mh_ctx* ctx, int igroup, int64_t nsel, const int64_t* sel, double* out);
 
 /* ---- AssemblyDirect{OX,OU,IA}: one DirectXUA step (src/DirectXUA.jl:18-150) ----
  * prepare(AssemblyDirect{OX,OU,IA},model,dis) (DirectXUA.jl:22-56) builds the class-pair

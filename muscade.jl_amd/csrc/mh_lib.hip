@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "../../include/muscade_hip.h"
@@ -102,6 +103,7 @@ struct Ctx {
     uint32_t* d_rsrc = nullptr;
     unsigned long long* d_flag = nullptr;
     unsigned long long* h_flag = nullptr;   // pinned host copy of d_flag (copied on the stream)
+    void* d_scratch = nullptr;   // this device's address of the fixed `scratch` array (set at create)
     // adjoint (VJP) of the residual Lagrangian
     int64_t nRU = 0;             // per-element ∂L/∂U entries
     int32_t* d_useg = nullptr;
@@ -1604,6 +1606,41 @@ static int launch_direct(Ctx* c, const Group& g, int nd, double dt, unsigned lon
     return check_launch(c, "direct element kernels");
 }
 
+// ------------------------------------------------------------------ fixed scratch
+// Device memory of the entry points that allocate nothing (mh_beam_gauge_derivs_noalloc): a
+// file-scope array that every GPU the library runs on holds until the process ends. Host
+// code never passes the array's name to a kernel or a copy (on the host it is only a
+// stand-in): it uses the per-device address that hipGetSymbolAddress gives, looked up once
+// per device when the first context on that device is created.
+namespace {
+constexpr int64_t kGaugeNoAllocMax = 4096;                         // elements per call
+constexpr size_t kScratchSelBytes = sizeof(int64_t) * kGaugeNoAllocMax;
+constexpr size_t kScratchBytes = kScratchSelBytes + sizeof(double) * kGaugeOut * kGaugeNoAllocMax;   // 11.9 MB
+static_assert(kScratchSelBytes % 256 == 0, "output staging must stay 256-byte aligned");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kScratchMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchMaxDevices] = {};
+
+// The calling thread's current device must be `device`.
+int scratch_address(int device, void** p) {
+    *p = nullptr;
+    if (device < 0 || device >= kScratchMaxDevices) return MH_OK;   // no fixed scratch: the no-alloc entries refuse
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* q = nullptr;
+        if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q) return MH_ERR_HIP;
+        g_scratch_ptr[device] = q;
+    }
+    *p = g_scratch_ptr[device];
+    return MH_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char* mh_strerror(int code) {
@@ -1635,6 +1672,9 @@ int mh_ctx_create(int device, mh_ctx** out) {
     if (hipMalloc(&c->d_flag, sizeof(unsigned long long)) != hipSuccess) { delete c; return MH_ERR_HIP; }
     if (hipHostMalloc((void**)&c->h_flag, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
         delete c; return MH_ERR_HIP;
+    }
+    if (scratch_address(device, &c->d_scratch) != MH_OK) {
+        (void)hipHostFree(c->h_flag); (void)hipFree(c->d_flag); delete c; return MH_ERR_HIP;
     }
     *out = c;
     return MH_OK;
@@ -2745,6 +2785,45 @@ int mh_beam_gauge_derivs(mh_ctx* c, int igroup, int64_t nsel, const int64_t* sel
     if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, sizeof(double) * n * kGaugeOut, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     dfree(d_out); dfree(d_sel);
+    if (e != hipSuccess) { c->err = std::string("gauge derivatives: ") + hipGetErrorString(e); return MH_ERR_HIP; }
+    return MH_OK;
+}
+
+// mh_beam_gauge_derivs without any allocation: the selection and the output staging live in
+// this device's fixed scratch array (kGaugeNoAllocMax elements at most); the kernel is the same.
+int mh_beam_gauge_derivs_noalloc(mh_ctx* c, int igroup, int64_t nsel, const int64_t* sel, double* out) {
+    if (!c || igroup < 0 || igroup >= (int)c->groups.size() || nsel < 0 || !out) return MH_ERR_ARG;
+    const Group& g = c->groups[igroup];
+    if (g.kind != MH_KIND_BEAM && g.kind != MH_KIND_BEAM_U) { c->err = "group is not an EulerBeam3D group"; return MH_ERR_ARG; }
+    const int64_t n = sel ? nsel : g.nele;
+    if (n > kGaugeNoAllocMax) {
+        c->err = "gauge derivatives without allocation: " + std::to_string(n) + " elements requested, at most " +
+                 std::to_string(kGaugeNoAllocMax) + " per call (the fixed scratch)";
+        return MH_ERR_SIZE;
+    }
+    if (!c->d_scratch) { c->err = "no fixed scratch on this device"; return MH_ERR_ARG; }
+    if (sel)
+        for (int64_t k = 0; k < nsel; ++k)
+            if (sel[k] < 1 || sel[k] > g.nele) { c->err = "element number out of range"; return MH_ERR_ARG; }
+    HIPCHK(c, hipSetDevice(c->device));
+    if (n == 0) return MH_OK;
+    int64_t* d_sel = static_cast<int64_t*>(c->d_scratch);
+    double* d_out = reinterpret_cast<double*>(static_cast<unsigned char*>(c->d_scratch) + kScratchSelBytes);
+    BeamDynArgs b;
+    b.nele = g.nele; b.idx = g.d_idx; b.uidx = g.d_uidx; b.nodes = g.d_nodes; b.xyz = c->d_xyz; b.o2 = g.d_o2;
+    b.sec = g.d_sec;
+    for (int d = 0; d < 3; ++d) b.X[d] = c->d_X[d];
+    b.U = c->d_U;
+    hipStream_t s = c->stream;
+    hipError_t e = hipSuccess;
+    if (sel) e = hipMemcpyAsync(d_sel, sel, sizeof(int64_t) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL((beam_gauge_derivs_kernel<int64_t, 1>), dim3((unsigned)((n * kPairs + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, s, b, sel ? d_sel : nullptr, n, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, sizeof(double) * n * kGaugeOut, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) { c->err = std::string("gauge derivatives: ") + hipGetErrorString(e); return MH_ERR_HIP; }
     return MH_OK;
 }

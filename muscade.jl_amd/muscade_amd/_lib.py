@@ -75,6 +75,7 @@ SIGNATURES = {
     "mh_big_solve_prepare": (_I, [_P, _L, _L, _L, _I64, _I64]),
     "mh_big_solve": (_I, [_P, _D, _D]),
     "mh_beam_gauge_derivs": (_I, [_P, _I, _L, _I64, _D]),
+    "mh_beam_gauge_derivs_noalloc": (_I, [_P, _I, _L, _I64, _D]),
     "mh_last_timings": (_I, [_P, _D]),
     "mh_calib_copy": (_I, [_P, _L, _D]),
     "mh_sync": (_I, [_P]),
