@@ -30,7 +30,7 @@ HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
 FP64_PEAK = 78.6e12        # MI355X FP64 vector, FLOP/s
 # FP64 operations per beam of the static kernel (add/sub/mul/div/sqrt = 1, fma = 2),
 # from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters of one assembly of this
-# workload divided by the number of beams (profiles/beam_static_pmc.md).
+# workload divided by the number of beams (profiles/space_frame_100x100x33_pmc.md).
 F_ELEM = 6299
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "space_frame_100x100x33_traffic.json")
 
@@ -92,6 +92,56 @@ def cpu_baseline(model, state, nsample, nthreads=1):
     return n / dt, n, dt
 
 
+def cpu_baseline_dynamic(model, state, nsample, c6):
+    """The oracle's dynamic assembly (SweepX{2} :iter: three time levels of nested duals around the
+    solver's partials, then add_value!/add_∂! in element order) on the first `nsample` beams of
+    the same model, 1 thread."""
+    sys.path.insert(0, ROOT)
+    from tests import _oracle as orc
+    E = model.eletypes[0]
+    d = state.dis.dis[0]
+    n = min(nsample, d.X.shape[0])
+    xyz = model.coords()
+    nodes = E.cat_nodes()[:n]
+    o2 = np.concatenate([p["orient2"] for p in E.params])[:n]
+    mat = np.concatenate([p["mat"] for p in E.params])[:n]
+    params = np.concatenate([xyz[nodes[:, 0], :3], xyz[nodes[:, 1], :3], o2, mat], axis=1)
+    om = orc.OracleModel(model.ndof["X"])
+    om.add_type(orc.K_BEAM, d.X[:n], None, None, d.scaleX, params)
+    om.prepare()
+    t0 = time.perf_counter()
+    r, rhs, nz, _ = om.assemble(2, 1, c6, state.X[:3])
+    dt = time.perf_counter() - t0
+    assert r == 0
+    return n / dt, n, dt
+
+
+def cpu_baseline_gradient(model, state, lam, nsample):
+    """The reference's way to a gradient of L = Σ Λ_eᵀR_e: per element the forward-mode Jacobian
+    (diffed_residual, nested duals) contracted with Λ_e, on the first `nsample` beams, 1 thread.
+    Returns seconds per beam."""
+    sys.path.insert(0, ROOT)
+    from tests import _oracle as orc
+    E = model.eletypes[0]
+    d = state.dis.dis[0]
+    n = min(nsample, d.X.shape[0])
+    xyz = model.coords()
+    nodes = E.cat_nodes()
+    o2 = np.concatenate([p["orient2"] for p in E.params])
+    mat = np.concatenate([p["mat"] for p in E.params])
+    amask = E.params[0].get("amask", 0)
+    if amask:
+        mat = E.cls.scaled_section(mat, state.A[d.A[:, 0] - 1], amask)
+    udof = d.U.shape[1] > 0
+    t0 = time.perf_counter()
+    for e in range(n):
+        R, Ks, KU = orc.beam_diffed_residual(xyz[nodes[e, 0], :3], xyz[nodes[e, 1], :3], o2[e], mat[e],
+                                             [state.X[0][d.X[e] - 1]], state.U[0][d.U[e] - 1] if udof else None)
+        g = lam[d.X[e] - 1] @ Ks[0]
+    dt = time.perf_counter() - t0
+    return dt / n, n, dt
+
+
 def host_threads():
     """Threads of the all-cores CPU baseline: OMP_NUM_THREADS if set (the host share of one
     GPU), else the visible cores, at most 16."""
@@ -142,6 +192,12 @@ def dynamic_bench(args):
         log(f"dynamic {mission}: {el / nst * 1e3:.3f} ms/step, kernels {acc / nst}")
     if not args.no_chain_solve:
         res["newton_device"] = device_newton_iteration(args, out, state, c6)
+    if not args.no_cpu_baseline:
+        eps, ns, dts = cpu_baseline_dynamic(model, state, args.cpu_sample_dynamic, c6)
+        res["cpu_baseline"] = {"value": eps, "unit": "elements/s", "cores": 1, "kind": "port",
+                               "sample": f"first {ns} beams of the same chains, one sequential assemble!{{:iter}} of "
+                                         f"AssemblySweepX{{2}} ({dts:.1f} s)"}
+        log(f"dynamic cpu baseline: {eps:.0f} beams/s")
     return {"workload": f"catenary chains x{args.chains}, {nbeam} EulerBeam3D with inertia/added mass/drag/roll "
                         f"inertia, AssemblySweepX{{2}} (Newmark-beta, dt=0.05)",
             "nbeam": nbeam, "nX": out.nX, "nnz": out.nnz, "prepare_s": t_prep, **res}
@@ -306,13 +362,136 @@ def adjoint_bench(args):
         nd = 6 * (specs[k]["ne"] + 1)
         worst = max(worst, float(np.abs(gXj[xo:xo + nd] - g[:nd]).max()))
         xo += nd
+    cpu = None
+    if not args.no_cpu_baseline:
+        spb, ns, dts = cpu_baseline_gradient(mj, sj, lj, args.cpu_sample_gradient)
+        cpu = {"value": len(specs) / (spb * nbeam), "unit": "gradient sets/s", "cores": 1, "kind": "port",
+               "sample": f"first {ns} beams of the batch ({dts:.1f} s): per beam the nested-dual Jacobian "
+                         f"(diffed_residual) contracted with Lambda, scaled to the {nbeam} beams"}
+        log(f"adjoint cpu baseline: {cpu['value']:.3g} gradient sets/s")
     return {"workload": f"{len(specs)} cantilevers of U{{{args.model_min}..{args.model_max}}} EulerBeam3D{{true}} "
                         f"({nbeam} beams), 8 U-loads each; gradient set = dL/dX, dL/dU of L = sum Lambda.R",
+            "cpu_baseline": cpu,
             "metric": "batched adjoint grads/s vs plain",
             "batched": res["batched"], "plain": res["plain"],
             "speedup": res["batched"]["grads_per_s"] / res["plain"]["grads_per_s"],
             "batched_kernels_ms": dict(zip(("elements", "scatter_X", "scatter_U"), ms.tolist())),
             "batched_vs_plain_max_abs_diff": worst}
+
+
+def catenary_motion_bench(args):
+    """The catenary chains in time (SweepX{2}, Newmark-β with β = 1/4, γ = 1/2, dt = 0.05 s): each
+    chain's top moved horizontally, a·sin(2πt/T) with a ~ U(0.5, 3) m, T ~ U(6, 15) s, from rest
+    in the as-meshed shape. Newton iterations with graph-replayed assembly, the device chain solve
+    and the device Newmark decrement; X stays on the device through the steps. `--motion-steps`
+    steps are timed (the reference runs 1000)."""
+    import muscade_amd as mc
+    from muscade_amd.workloads import catenary_chains
+    t0 = time.perf_counter()
+    model, state = catenary_chains(args.chains, args.chain_min, args.chain_max, disp=0., rot=0., vel=0., acc=0.,
+                                   motion=True)
+    state.time = 0.
+    nbeam = model.eletypes[0].nele
+    log(f"catenary motion: {args.chains} chains, {nbeam} beams ({time.perf_counter() - t0:.1f}s)")
+    dt = 0.05
+    times = dt * np.arange(1, args.motion_steps + 1)
+    st = {}
+    t0 = time.perf_counter()
+    out = mc.solve(mc.SweepX[2], initialstate=state, time=times, maxΔx=1e-5, maxiter=20, verbose=False,
+                   catcherror=False, linsolve="device", graph=True, stats=st)
+    el = time.perf_counter() - t0
+    top_nodes = np.concatenate([E.cat_nodes()[:, 0] + 1 for E in model.eletypes if E.cls is mc.DofMotion])
+    tops = model.getdofID("X", "t1", nodID=top_nodes)
+    last = out[-1]
+    return {"workload": f"catenary chains x{args.chains} ({nbeam} EulerBeam3D), top moved a*sin(2*pi*t/T), "
+                        f"SweepX{{2}} dt=0.05 s from rest, device chain solve, graph-replayed assembly",
+            "steps": int(args.motion_steps), "seconds_incl_prepare": el,
+            "steps_per_s": args.motion_steps / el, "newton_iterations": st.get("iterations"),
+            "iterations_per_step": st.get("iterations", 0) / args.motion_steps,
+            "projected_1000_steps_s": el / args.motion_steps * 1000,
+            "top_t1_at_last_step_m": {"min": float(last.X[0][tops - 1].min()), "max": float(last.X[0][tops - 1].max())}}
+
+
+def design_point_bench(args):
+    """Batched design-point search (the limit-state gradients of many models): `--models`
+    uneven cantilevers of 10 m (U{model_min..model_max} EulerBeam3DA{true}, 16 stiffness
+    parameters a as 10^a on EA, EI₂, EI₃, GJ per sixteenth, 8 U-loads each), at their static
+    equilibrium (SweepX{0} with the device chain solve, timed apart). A gradient set is dg/dA
+    (16) and dg/dU (24) of one model's limit state g = tip displacement along z: assembly of
+    the tangent at X*, the adjoint chain solve K·μ = ∂g/∂X and the adjoint launch with Λ = −μ.
+    Batched: all models as one model (one of each launch). Plain: one context per model, in a
+    loop, at the same equilibrium."""
+    import muscade_amd as mc
+    from muscade_amd.workloads import design_point_batch, design_point_specs
+    specs = design_point_specs(args.models, args.model_min, args.model_max)
+    t0 = time.perf_counter()
+    [(mj, sj, tipj)] = design_point_batch(specs)
+    sep = design_point_batch(specs, joined=False)
+    nbeam = mj.eletypes[0].nele
+    log(f"design point: {len(specs)} models, {nbeam} beams ({time.perf_counter() - t0:.1f}s)")
+    st = {}
+    t0 = time.perf_counter()
+    eq = mc.solve(mc.SweepX[0], initialstate=sj, time=[1.], maxΔx=1e-10, verbose=False, catcherror=False,
+                  linsolve="device", stats=st)[-1]
+    t_eq = time.perf_counter() - t0
+    log(f"design point: equilibrium {t_eq:.2f}s ({st.get('iterations')} Newton iterations)")
+    # the plain models at the same equilibrium: node dofs model after model, then the six λ fields
+    nodal = np.cumsum([0] + [6 * (s["ne"] + 1) for s in specs])
+    nm = len(specs)
+    plain = []
+    for m, (mm, sm, tm) in enumerate(sep):
+        s_ = sm.copy()
+        s_.X[0][:6 * (specs[m]["ne"] + 1)] = eq.X[0][nodal[m]:nodal[m + 1]]
+        s_.X[0][6 * (specs[m]["ne"] + 1):] = eq.X[0][nodal[-1] + m + nm * np.arange(6)]
+        d_ = np.zeros(mm.ndof["X"])
+        d_[tm - 1] = 1.
+        plain.append((mc.LimitStateGradient(mm, s_.dis), s_, d_))
+    dgj = np.zeros(mj.ndof["X"])
+    dgj[tipj - 1] = 1.
+    lsg = mc.LimitStateGradient(mj, eq.dis)
+    res = {}
+
+    def batched():
+        return lsg(eq, dgj)
+
+    def per_model():
+        return [f(s_, d_) for f, s_, d_ in plain]
+
+    for name, fn in (("batched", batched), ("plain", per_model)):
+        for _ in range(max(1, args.warmup)):
+            fn()
+        n = max(2, args.steps // 4)
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        el = (time.perf_counter() - t0) / n
+        res[name] = {"grads_per_s": nm / el, "ms_per_batch": el * 1e3}
+        log(f"design point {name}: {el * 1e3:.2f} ms for {nm} gradient sets")
+    gA, gU, gX = batched()
+    worst, worst_x = 0., float(np.abs(gX + dgj).max())
+    for m in range(min(8, nm)):
+        a1, u1, _ = plain[m][0](plain[m][1], plain[m][2])
+        worst = max(worst, float(np.abs(gA[16 * m:16 * m + 16] - a1).max() / np.abs(a1).max()),
+                    float(np.abs(gU[24 * m:24 * m + 24] - u1).max() / np.abs(u1).max()))
+    cpu = None
+    if not args.no_cpu_baseline:
+        lam = np.zeros(mj.ndof["X"])
+        lam[:] = np.random.default_rng(0).normal(size=mj.ndof["X"])
+        spb, ns, dts = cpu_baseline_gradient(mj, eq, lam, args.cpu_sample_gradient)
+        cpu = {"value": nm / (spb * nbeam), "unit": "gradient sets/s", "cores": 1, "kind": "port",
+               "sample": f"first {ns} beams of the batch ({dts:.1f} s): the element part only (per beam the "
+                         f"nested-dual Jacobian contracted with Lambda, scaled to the {nbeam} beams); the adjoint "
+                         f"solve is not included, so this bounds the CPU rate from above"}
+        log(f"design point cpu baseline: {cpu['value']:.3g} gradient sets/s (element part)")
+    return {"workload": f"{nm} cantilevers of 10 m, U{{{args.model_min}..{args.model_max}}} EulerBeam3DA{{true}} "
+                        f"({nbeam} beams), 16 stiffness parameters and 8 U-loads each; gradient set = dg/dA, dg/dU "
+                        f"of the tip displacement at equilibrium",
+            "metric": "batched adjoint grads/s vs plain",
+            "batched": res["batched"], "plain": res["plain"],
+            "speedup": res["batched"]["grads_per_s"] / res["plain"]["grads_per_s"],
+            "equilibrium_s": t_eq, "equilibrium_iterations": st.get("iterations"),
+            "batched_vs_plain_max_rel_diff": worst, "adjoint_check_max_abs": worst_x, "cpu_baseline": cpu,
+            "tip_displacement_m": {"min": float(eq.X[0][tipj - 1].min()), "max": float(eq.X[0][tipj - 1].max())}}
 
 
 def graph_bench(args):
@@ -436,6 +615,8 @@ def main():
     ap.add_argument("--nz", type=int, default=33)
     ap.add_argument("--cpu-sample", type=int, default=40000)
     ap.add_argument("--cpu-sample-parallel", type=int, default=200000)
+    ap.add_argument("--cpu-sample-dynamic", type=int, default=4000)
+    ap.add_argument("--cpu-sample-gradient", type=int, default=20000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="run only the counter-calibration copy kernel")
     ap.add_argument("--no-dynamic", action="store_true", help="skip the dynamic (SweepX{2}) assembly measurement")
@@ -450,6 +631,11 @@ def main():
                     help="skip the device linear solve measurements (chain topologies)")
     ap.add_argument("--only-chain-solve", action="store_true", help="measure only the device vs host chain solve")
     ap.add_argument("--only-adjoint", action="store_true", help="measure only the batched adjoint gradients")
+    ap.add_argument("--no-design-point", action="store_true", help="skip the batched limit-state gradients")
+    ap.add_argument("--no-motion", action="store_true", help="skip the catenary chains run in time")
+    ap.add_argument("--only-motion", action="store_true", help="measure only the catenary chains run in time")
+    ap.add_argument("--motion-steps", type=int, default=10)
+    ap.add_argument("--only-design-point", action="store_true", help="measure only the batched limit-state gradients")
     ap.add_argument("--models", type=int, default=512)
     ap.add_argument("--model-min", type=int, default=500)
     ap.add_argument("--model-max", type=int, default=20000)
@@ -470,6 +656,12 @@ def main():
         return
     if args.only_adjoint:
         print(json.dumps({"adjoint": adjoint_bench(args)}))
+        return
+    if args.only_motion:
+        print(json.dumps({"catenary_motion": catenary_motion_bench(args)}))
+        return
+    if args.only_design_point:
+        print(json.dumps({"design_point": design_point_bench(args)}))
         return
     if args.only_graph:
         print(json.dumps({"graph": graph_bench(args)}))
@@ -607,6 +799,7 @@ def main():
                    "nbeam": nbeam, "nX": out.nX, "nnz": out.nnz},
         "roofline": dict(dominant[1], kernel=dominant[0]),
         "kernels_ms": {"eval": t_eval * 1e3, "scatter_nz": t_sk * 1e3, "scatter_rhs": t_sr * 1e3},
+        "roofline_eval": rl_eval,
         "roofline_eval_hbm": rl_eval_hbm,
         "roofline_scatter": rl_scatter,
         "prepare_s": t_prep,
@@ -621,6 +814,10 @@ def main():
         line["dynamic"] = dynamic_bench(args)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_bench(args)
+    if not args.no_design_point:
+        line["design_point"] = design_point_bench(args)
+    if not args.no_motion:
+        line["catenary_motion"] = catenary_motion_bench(args)
     if not args.no_graph:
         line["graph"] = graph_bench(args)
     if not args.no_newton:

@@ -87,6 +87,14 @@ int mh_model_begin(mh_ctx* ctx, int64_t nXdof, int64_t nUdof, int64_t nAdof, int
 int mh_group_add(mh_ctx* ctx, int kind, int64_t nele, int nX, const int64_t* idxX, const double* scaleX, int nU,
                  const int64_t* idxU, const int64_t* nodes, const double* orient2, const double* mat, int* igroup);
 
+/* EulerBeam3DA: the stiffnesses of mask bits k (1 EA, 2 EI₂, 4 EI₃, 8 GJ) of beam group igroup
+ * become stiffness·10^a, a = A[idxA[e]] (idxA: 1×nele, 1-based model A-dofs). A wrapper in the
+ * style of the reference's Spring, which scales its L₀ and EA by exp10 of its A-dofs
+ * (test/SomeElements.jl:166-186), so that A-dofs parametrize the beam's section
+ * (BeamElement.jl:151-174 with the scaled BeamCrossSection). Every kernel then evaluates the
+ * scaled section at the device A (mh_set_A); mh_vjp_lagrangian adds ∂L/∂A. Call before prepare. */
+int mh_group_set_adofs(mh_ctx* ctx, int igroup, const int64_t* idxA, int mask);
+
 /* ---- prepare(AssemblySweepX{OX}) (src/SweepX.jl:24-33) ------------------
  * Builds on the device the CSC pattern of Lλx and the sorted
  * contribution→nzval maps that replace asm[1,:] / asm[2,:]
@@ -118,9 +126,10 @@ int mh_set_time(mh_ctx* ctx, double t);
  * ∂L/∂U on the device, in unscaled model coordinates, recomputing each element's
  * internal state (tangent, time jets) instead of storing it; the per-element products
  * are reduced into the model vectors in Muscade's element order (no atomics).
- * mh_get_vjp copies them out: gX is (OX+1)×nX (der-major), gU nU, gA nA; gA is zero,
- * since no device element kind has A-dofs (BeamElement.jl:109-118, BarElement.jl:107-114,
- * BasicElements.jl:275-284,406-438: A-dofs belong to host-evaluated cost elements). There is no
+ * mh_get_vjp copies them out: gX is (OX+1)×nX (der-major), gU nU, gA nA. gA gathers
+ * ∂L/∂a of the EulerBeam3DA groups (mh_group_set_adofs); it is zero without them (EulerBeam3D,
+ * Bar3D, Hold and DofLoad have no A-dofs: BeamElement.jl:109-118, BarElement.jl:107-114,
+ * BasicElements.jl:275-284,406-438). There is no
  * counterpart in the reference (which forms such gradients with forward-mode duals,
  * src/Diagnostic.jl:851-911); it serves the gradient (rrule) of a solve's Lagrangian. */
 int mh_set_lambda(mh_ctx* ctx, const double* lam);
@@ -243,6 +252,13 @@ int mh_get_dx(mh_ctx* ctx, double* dx);
 int mh_newton_prepare(mh_ctx* ctx, const double* scaleX);
 int mh_newton_update(mh_ctx* ctx, int OX, int first, const double* c6, double* norms);
 int mh_get_X(mh_ctx* ctx, int ider, double* x);
+/* The adjoint solve of a limit state g(X) at an equilibrium (no counterpart in the reference;
+ * the step before the adjoint gradient of a reliability or design-point search): with the
+ * tangent of the last assembly (SweepX{0} :iter, symmetric), solve K·μ = rhs (rhs = ∂g/∂X, nX
+ * values in unscaled model coordinates) by the chain solve and set Λ := alpha·μ on the device,
+ * ready for mh_vjp_lagrangian (alpha = −1 gives dg/dA = Λᵀ∂R/∂A, dg/dU = Λᵀ∂R/∂U, since
+ * R(X*(A,U),A,U) = 0). Needs mh_chain_prepare and mh_newton_prepare (the dof scales). */
+int mh_adjoint_solve(mh_ctx* ctx, const double* rhs, double alpha);
 
 /* ---- DirectXUA's all-steps system on the device (src/DirectXUA.jl:245-383:
  *      makepattern / preparebig / assemblebig!, with SparseTools.prepare / addin!,

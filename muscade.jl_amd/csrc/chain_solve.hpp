@@ -107,7 +107,15 @@ __device__ inline void lu_solve(const double (&a)[B][B], const int (&piv)[B], do
     }
 }
 
-__global__ void clear_flag(unsigned* f) { *f = 0u; }
+// The status words of a solve: flag (the k* bits) at word 0, then, as 64-bit values at
+// byte 8 and 16, the lowest block with a bad pivot and the lowest bad λ pair (all ones: none).
+constexpr int kStatusBytes = 24;
+__device__ inline unsigned long long* status_where(unsigned* f) { return reinterpret_cast<unsigned long long*>(f + 2); }
+__global__ void clear_flag(unsigned* f) {
+    f[0] = 0u;
+    status_where(f)[0] = ~0ull;
+    status_where(f)[1] = ~0ull;
+}
 
 __global__ __launch_bounds__(kCB) void fill_f64(int64_t n, double* __restrict__ p, double v) {
     const int64_t i = (int64_t)blockIdx.x * kCB + threadIdx.x;
@@ -220,7 +228,10 @@ __global__ __launch_bounds__(kCB) void pair_kernel(int64_t np, const int32_t* __
     const int64_t p = (int64_t)blockIdx.x * kCB + threadIdx.x;
     if (p >= np) return;
     const double cL = pc[4 * p], cU = pc[4 * p + 1], dg = pc[4 * p + 2], rl = pc[4 * p + 3];
-    if (dg != 0.0 || cL == 0.0 || cU == 0.0) atomicOr(flag, kBadPair);
+    if (dg != 0.0 || cL == 0.0 || cU == 0.0) {
+        atomicOr(flag, kBadPair);
+        atomicMin(status_where(flag) + 1, (unsigned long long)p);
+    }
     const int64_t h = hpos[p], blk = h / B, i = h - blk * B;
     double* sv = saved + p * (3 * B + 1);
     const int64_t o = blk * B * B + i;    // row i of the blocks: entries o + j·B
@@ -252,7 +263,10 @@ __global__ __launch_bounds__(kCB) void factor_kernel(int64_t nb, int64_t s, doub
     for (int c = 0; c < B; ++c)
 #pragma unroll
         for (int r = 0; r < B; ++r) a[r][c] = Di[c * B + r];
-    if (!lu_factor<B>(a, piv)) atomicOr(flag, kSingular);
+    if (!lu_factor<B>(a, piv)) {
+        atomicOr(flag, kSingular);
+        atomicMin(status_where(flag), (unsigned long long)i);
+    }
     double v[B];
     double* bi = b + i * B;
 #pragma unroll
@@ -340,7 +354,10 @@ __global__ void top_kernel(double* __restrict__ D, double* __restrict__ b, unsig
 #pragma unroll
         for (int c = 0; c < B; ++c) a[r][c] = D[c * B + r];
     }
-    if (!lu_factor<B>(a, piv)) atomicOr(flag, kSingular);
+    if (!lu_factor<B>(a, piv)) {
+        atomicOr(flag, kSingular);
+        atomicMin(status_where(flag), 0ull);
+    }
     lu_solve<B>(a, piv, v);
 #pragma unroll
     for (int r = 0; r < B; ++r) b[r] = v[r];

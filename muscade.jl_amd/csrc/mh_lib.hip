@@ -60,6 +60,11 @@ struct Group {
     int64_t ubase = 0;           // U-dof groups: offset of the per-element ∂L/∂U entries
     int64_t kdbase = 0, kubase = 0;   // AssemblyDirect: offsets of the ∂R/∂X⁽ᵈ⁾ and ∂R/∂U blocks
     bool compact = false;
+    // EulerBeam3DA: the stiffnesses of mask bits k (EA, EI₂, EI₃, GJ) are scaled by 10^A[aidx[e]]
+    int amask = 0;
+    int32_t* d_aidx = nullptr;   // 1 × nele, 0-based A-dof of each element
+    double* d_sec_base = nullptr;   // 16 × nele: the section before the scaling (d_sec holds the scaled one)
+    int64_t abase = 0;           // offset of the per-element ∂L/∂A entries
 };
 
 // One CSC pattern with its contribution maps (seg: first contribution of each nz; src:
@@ -113,6 +118,12 @@ struct Ctx {
     double* d_gX = nullptr;      // 3 × nX
     double* d_gubuf = nullptr;   // nRU
     double* d_gU = nullptr;      // nU
+    int64_t nRA = 0;             // per-element ∂L/∂A entries (EulerBeam3DA elements)
+    int32_t* d_aseg = nullptr;   // nA + 1: first entry of each A-dof in the sorted A map
+    uint32_t* d_asrc = nullptr;  // nRA: per-element entry of each contribution, by A-dof then element
+    double* d_gabuf = nullptr;   // nRA
+    double* d_gA = nullptr;      // nA
+    double* d_crhs = nullptr;    // nX: right-hand side of an adjoint chain solve
     int vjp_nd = 0;              // derivative orders of the last VJP
     // AssemblyDirect{OX,OU,IA} (DirectXUA, one step)
     bool direct = false;
@@ -234,6 +245,8 @@ void free_prepared(Ctx* c) {
     dfree(c->d_Kbuf); dfree(c->d_Rbuf); dfree(c->d_dRbuf); dfree(c->d_nzval); dfree(c->d_rhs); dfree(c->d_Kc);
     dfree(c->d_colptr); dfree(c->d_rowval); dfree(c->d_kseg); dfree(c->d_ksrc); dfree(c->d_rseg); dfree(c->d_rsrc);
     dfree(c->d_useg); dfree(c->d_usrc); dfree(c->d_gbuf); dfree(c->d_gX); dfree(c->d_gubuf); dfree(c->d_gU);
+    dfree(c->d_aseg); dfree(c->d_asrc); dfree(c->d_gabuf); dfree(c->d_gA); dfree(c->d_crhs);
+    c->nRA = 0;
     c->vjp_nd = 0;
     for (DMap* M : {&c->dXX, &c->dXU, &c->dUX, &c->dUU}) {
         dfree(M->colptr); dfree(M->rowval); dfree(M->seg); dfree(M->src); dfree(M->srcT);
@@ -245,7 +258,10 @@ void free_prepared(Ctx* c) {
 }
 void free_model(Ctx* c) {
     free_prepared(c);
-    for (auto& g : c->groups) { dfree(g.d_idx); dfree(g.d_uidx); dfree(g.d_nodes); dfree(g.d_o2); dfree(g.d_sec); dfree(g.d_vals); }
+    for (auto& g : c->groups) {
+        dfree(g.d_idx); dfree(g.d_uidx); dfree(g.d_nodes); dfree(g.d_o2); dfree(g.d_sec); dfree(g.d_vals);
+        dfree(g.d_aidx); dfree(g.d_sec_base);
+    }
     c->groups.clear();
     dfree(c->d_xyz);
     for (auto& x : c->d_X) dfree(x);
@@ -355,9 +371,10 @@ __device__ __forceinline__ void point_element(const PointArgs& a, int64_t e) {
     const int64_t n = a.nele;
     const int nX = a.nX;
     if (a.kind == MH_KIND_HOLD) {
-        // DofConstraint{:X} equal, gap(x)=x: R = (-λ, -x), ∂R = [[0,-1],[-1,0]]
+        // DofConstraint{:X} equal, gap(x,t) = x - g(t) with g the host value (0 for Hold):
+        // R = (-λ, -gap), ∂R = [[0,-1],[-1,0]]
         double x = a.X[a.idx[e]], lam = a.X[a.idx[n + e]];
-        double R0 = -lam, R1 = -x;
+        double R0 = -lam, R1 = -(x - a.vals[e]);
         a.R[e] = R0 * a.s[0];
         a.R[n + e] = R1 * a.s[1];
         double* K = a.K + 4 * e;
@@ -728,6 +745,49 @@ __global__ __launch_bounds__(kBeamBlock) void beam_vjp_static_kernel(VjpArgs a) 
         a.gu[e] = gu.x; a.gu[n + e] = gu.y; a.gu[2 * n + e] = gu.z;
     }
     if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+}
+
+// EulerBeam3DA: the scaled section, sec = base · 10^A[aidx[e]] on the stiffnesses of the mask.
+__global__ __launch_bounds__(kBlock) void scale_section_kernel(int64_t n, const int32_t* __restrict__ aidx,
+                                                               const double* __restrict__ A, int mask,
+                                                               const double* __restrict__ base, double* __restrict__ sec) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double f = exp10(A[aidx[e]]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (mask >> k & 1) sec[k * n + e] = base[k * n + e] * f;
+}
+
+// ∂L/∂a of EulerBeam3DA elements, L = Λᵀ R: R depends on a only through the scaled
+// stiffnesses, linearly, so ∂R/∂a = ln 10 · R(X; the scaled stiffnesses alone, no loads), the
+// internal force of those stiffnesses. One element per lane; the internal force is the
+// static closed form at X (for the dynamic beam too: the loads do not depend on stiffness).
+__global__ __launch_bounds__(kBeamBlock) void beam_vjp_a_kernel(VjpArgs a, int mask, double* __restrict__ ga) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nele) return;
+    const int64_t n = a.nele;
+    double X[12], lam[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) { const int32_t q = a.idx[k * n + e]; X[k] = a.X[0][q]; lam[k] = a.lam[q]; }
+    const int32_t n1 = a.nodes[e], n2 = a.nodes[n + e];
+    mh::v3 c1 = mh::mk(a.xyz[3 * (int64_t)n1], a.xyz[3 * (int64_t)n1 + 1], a.xyz[3 * (int64_t)n1 + 2]);
+    mh::v3 c2 = mh::mk(a.xyz[3 * (int64_t)n2], a.xyz[3 * (int64_t)n2 + 1], a.xyz[3 * (int64_t)n2 + 2]);
+    const mh::BeamGeo G = mh::beam_geometry(c1, c2, mh::mk(a.o2[e], a.o2[n + e], a.o2[2 * n + e]));
+    mh::BeamSection S;
+    S.EA = (mask & 1) ? a.sec[e] : 0.;
+    S.EI2 = (mask & 2) ? a.sec[n + e] : 0.;
+    S.EI3 = (mask & 4) ? a.sec[2 * n + e] : 0.;
+    S.GJ = (mask & 8) ? a.sec[3 * n + e] : 0.;
+    S.w = 0.;
+    mh::BeamOut out;
+    mh::beam_static(G, S, mh::mk(0., 0., 0.), X, out);
+    double g = 0.;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) g = g + lam[i] * out.R[i];
+    g = g * 2.302585092994045684;   // ln 10
+    ga[e] = g;
+    if (g != g) atomicMin(a.flag, a.gtag | (unsigned long long)e);
 }
 
 // Dynamic beams, 16 lanes per element: lane 12 forms y = J_x Λ, J_v Λ (shared through LDS)
@@ -1149,6 +1209,13 @@ __global__ __launch_bounds__(kTile) void scatter_long_kernel(int64_t nseg, const
 
 __global__ void fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
 
+// y = alpha · s ⊙ x (x and y may alias)
+__global__ __launch_bounds__(kBlock) void scaled_copy_kernel(int64_t n, double alpha, const double* __restrict__ sc,
+                                                             const double* x, double* y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = alpha * (sc[i] * x[i]);
+}
+
 // Plain streaming copy of n doubles: the reference kernel for HBM counter calibration.
 __global__ __launch_bounds__(kBlock) void copy_kernel(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = a[i];
@@ -1396,7 +1463,21 @@ int launch_vjp(Ctx* c, const Group& g, int nd, unsigned long long gtag) {
     } else {
         hipLaunchKernelGGL(point_vjp_kernel, dim3(grid(g.nele)), dim3(kBlock), 0, s, a, nd);
     }
+    if (beam && g.amask)
+        hipLaunchKernelGGL(beam_vjp_a_kernel, dim3((unsigned)((g.nele + kBeamBlock - 1) / kBeamBlock)), dim3(kBeamBlock),
+                           0, s, a, g.amask, c->d_gabuf + g.abase);
     return check_launch(c, "adjoint kernels");
+}
+
+// The scaled sections of the EulerBeam3DA groups at the device A (after mh_set_A and
+// mh_group_set_adofs).
+int apply_ascale(Ctx* c) {
+    for (auto& g : c->groups) {
+        if (!g.amask || g.nele == 0) continue;
+        hipLaunchKernelGGL(scale_section_kernel, dim3(grid(g.nele)), dim3(kBlock), 0, c->stream, g.nele, g.d_aidx, c->d_A,
+                           g.amask, g.d_sec_base, g.d_sec);
+    }
+    return check_launch(c, "section scaling");
 }
 
 // ============================================================================ C ABI
@@ -1780,12 +1861,29 @@ int mh_group_add(mh_ctx* c, int kind, int64_t nele, int nX, const int64_t* idxX,
         if ((r = upload_soa<double>(c, &g.d_o2, orient2, 3, nele))) return r;
         if ((r = upload_soa<double>(c, &g.d_sec, mat, 16, nele))) return r;
     }
-    if (kind == MH_KIND_DOFLOAD) {
+    if (kind == MH_KIND_DOFLOAD || kind == MH_KIND_HOLD) {   // DofLoad values, prescribed-motion gaps
         HIPCHK(c, hipMalloc(&g.d_vals, sizeof(double) * std::max<int64_t>(nele, 1)));
         HIPCHK(c, hipMemset(g.d_vals, 0, sizeof(double) * std::max<int64_t>(nele, 1)));
     }
     c->groups.push_back(std::move(g));
     if (igroup) *igroup = (int)c->groups.size() - 1;
+    return MH_OK;
+}
+
+int mh_group_set_adofs(mh_ctx* c, int igroup, const int64_t* idxA, int mask) {
+    if (!c || igroup < 0 || igroup >= (int)c->groups.size() || mask < 1 || mask > 15) return MH_ERR_ARG;
+    Group& g = c->groups[igroup];
+    if (g.kind != MH_KIND_BEAM && g.kind != MH_KIND_BEAM_U) { c->err = "A-scaled stiffness needs an EulerBeam3D group"; return MH_ERR_ARG; }
+    if (g.amask) { c->err = "the group's A-dofs are already set"; return MH_ERR_ARG; }
+    if (g.nele > 0 && !idxA) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    free_prepared(c);
+    if (int r = upload_idx(c, &g.d_aidx, idxA, 1, g.nele, c->nA, false)) return r;
+    HIPCHK(c, hipMalloc(&g.d_sec_base, sizeof(double) * std::max<int64_t>(16 * g.nele, 1)));
+    if (g.nele) HIPCHK(c, hipMemcpy(g.d_sec_base, g.d_sec, sizeof(double) * 16 * g.nele, hipMemcpyDeviceToDevice));
+    g.amask = mask;
+    if (int r = apply_ascale(c)) return r;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return MH_OK;
 }
 
@@ -1795,11 +1893,13 @@ int mh_prepare_sweepx(mh_ctx* c, int OX, int64_t* nnz_out) {
     free_prepared(c);
     // contribution counts and buffer offsets, in eletype order. Static beams store the
     // compact tangent; dynamic beams (K + a₁C + b₁M is not symmetric) the full block.
-    int64_t nK = 0, nP = 0, nR = 0, nKc = 0, nRU = 0;
+    int64_t nK = 0, nP = 0, nR = 0, nKc = 0, nRU = 0, nRA = 0;
     bool coded = false;
     for (auto& g : c->groups) {
         g.ubase = nRU;
         nRU += g.nele * g.nU;
+        g.abase = nRA;
+        if (g.amask) nRA += g.nele;
         const bool beam = g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U;
         g.compact = beam && OX == 0;
         coded = coded || g.compact;
@@ -1815,7 +1915,7 @@ int mh_prepare_sweepx(mh_ctx* c, int OX, int64_t* nnz_out) {
     if (nP >= INT32_MAX || nK >= klim || nR >= (int64_t)UINT32_MAX) {
         c->err = "more than 2^31 contributions or too many element outputs for 32-bit maps"; return MH_ERR_SIZE;
     }
-    c->OX = OX; c->coded = coded; c->nRU = nRU;
+    c->OX = OX; c->coded = coded; c->nRU = nRU; c->nRA = nRA;
     c->nK = nK; c->nP = nP; c->nR = nR;
     const int64_t nX = c->nX;
     hipStream_t s = c->stream;
@@ -1903,6 +2003,30 @@ int mh_prepare_sweepx(mh_ctx* c, int OX, int64_t* nnz_out) {
         HIPCHK(c, hipStreamSynchronize(s));
         dfree(rkin); dfree(rkout); dfree(rvin);
     }
+    // ---- A-dof map (gradients ∂L/∂A of the EulerBeam3DA elements), one entry per element
+    if (nRA > 0) {
+        HIPCHK(c, hipMalloc(&rkin, sizeof(unsigned int) * nRA));
+        HIPCHK(c, hipMalloc(&rkout, sizeof(unsigned int) * nRA));
+        HIPCHK(c, hipMalloc(&rvin, sizeof(uint32_t) * nRA));
+        HIPCHK(c, hipMalloc(&c->d_asrc, sizeof(uint32_t) * nRA));
+        HIPCHK(c, hipMalloc(&c->d_aseg, sizeof(int32_t) * (c->nA + 1)));
+        for (auto& g : c->groups) {
+            if (!g.amask || g.nele == 0) continue;
+            hipLaunchKernelGGL(gen_rpairs_kernel, dim3(grid(g.nele)), dim3(kBlock), 0, s, g.nele, 1, g.d_aidx, g.abase,
+                               rkin + g.abase, rvin + g.abase);
+        }
+        int abit = 1;
+        while (abit < 32 && (unsigned long long)c->nA > (1ull << abit)) ++abit;
+        tmp_bytes = 0;
+        HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, rkin, rkout, rvin, c->d_asrc, (int)nRA, 0, abit, s));
+        HIPCHK(c, hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
+        HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, rkin, rkout, rvin, c->d_asrc, (int)nRA, 0, abit, s));
+        dfree(tmp);
+        hipLaunchKernelGGL(rseg_kernel, dim3(grid(nRA + 1)), dim3(kBlock), 0, s, nRA, rkout, c->nA, c->d_aseg);
+        if (int r = check_launch(c, "A pattern")) return r;
+        HIPCHK(c, hipStreamSynchronize(s));
+        dfree(rkin); dfree(rkout); dfree(rvin);
+    }
     // ---- tuner: time each shape of the nz scatter on these maps and keep the fastest.
     // The sums are identical for every shape (each segment is summed in the same order).
     if (nnz > 0) {
@@ -1978,13 +2102,14 @@ int mh_set_A(mh_ctx* c, const double* a) {
     if (!c || (c->nA && !a)) return MH_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     if (c->nA) HIPCHK(c, hipMemcpyAsync(c->d_A, a, sizeof(double) * c->nA, hipMemcpyHostToDevice, c->stream));
+    if (int r = apply_ascale(c)) return r;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MH_OK;
 }
 int mh_set_point_values(mh_ctx* c, int igroup, const double* v) {
     if (!c || igroup < 0 || igroup >= (int)c->groups.size() || !v) return MH_ERR_ARG;
     Group& g = c->groups[igroup];
-    if (g.kind != MH_KIND_DOFLOAD) { c->err = "group has no host-evaluated values"; return MH_ERR_ARG; }
+    if (g.kind != MH_KIND_DOFLOAD && g.kind != MH_KIND_HOLD) { c->err = "group has no host-evaluated values"; return MH_ERR_ARG; }
     HIPCHK(c, hipSetDevice(c->device));
     if (g.nele) HIPCHK(c, hipMemcpyAsync(g.d_vals, v, sizeof(double) * g.nele, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -2165,6 +2290,10 @@ int mh_vjp_lagrangian(mh_ctx* c, int OX, mh_err* err) {
         HIPCHK(c, hipMalloc(&c->d_gX, sizeof(double) * std::max<int64_t>(3 * c->nX, 1)));
         HIPCHK(c, hipMalloc(&c->d_gubuf, sizeof(double) * std::max<int64_t>(c->nRU, 1)));
         HIPCHK(c, hipMalloc(&c->d_gU, sizeof(double) * std::max<int64_t>(c->nU, 1)));
+        if (c->nRA) {
+            HIPCHK(c, hipMalloc(&c->d_gabuf, sizeof(double) * c->nRA));
+            HIPCHK(c, hipMalloc(&c->d_gA, sizeof(double) * std::max<int64_t>(c->nA, 1)));
+        }
     }
     if (OX > 0 && !c->d_Kc) {
         int64_t nKc = 0;
@@ -2199,6 +2328,10 @@ int mh_vjp_lagrangian(mh_ctx* c, int OX, mh_err* err) {
             HIPCHK(c, hipMemsetAsync(c->d_gU, 0, sizeof(double) * c->nU, s));
         }
     }
+    if (c->nRA) {   // ∂L/∂A: few A-dofs shared by many elements, a workgroup per A-dof
+        hipLaunchKernelGGL(scatter_long_kernel, dim3((unsigned)c->nA), dim3(kTile), 0, s, c->nA, c->d_aseg, c->d_asrc,
+                           c->d_gabuf, c->d_gA);
+    }
     HIPCHK(c, hipEventRecord(c->ev[3], s));
     if (int r = check_launch(c, "adjoint scatter")) return r;
     unsigned long long flag = 0;
@@ -2223,7 +2356,12 @@ int mh_get_vjp(mh_ctx* c, double* gX, double* gU, double* gA) {
         HIPCHK(c, hipMemcpyAsync(gX, c->d_gX, sizeof(double) * c->nX * c->vjp_nd, hipMemcpyDeviceToHost, c->stream));
     if (gU && c->nU) HIPCHK(c, hipMemcpyAsync(gU, c->d_gU, sizeof(double) * c->nU, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (gA) for (int64_t i = 0; i < c->nA; ++i) gA[i] = 0.;   // no element kind here depends on A
+    if (gA && c->nA && c->nRA) {
+        HIPCHK(c, hipMemcpyAsync(gA, c->d_gA, sizeof(double) * c->nA, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else if (gA) {
+        for (int64_t i = 0; i < c->nA; ++i) gA[i] = 0.;   // no element here depends on A
+    }
     return MH_OK;
 }
 
@@ -2502,7 +2640,7 @@ namespace {
 int64_t cgrid(int64_t n) { return (n + chain::kCB - 1) / chain::kCB; }
 
 template <int B>
-int enqueue_chain_solve(Ctx* c) {
+int enqueue_chain_solve(Ctx* c, const double* rhs) {
     using namespace chain;
     hipStream_t s = c->stream;
     const int64_t nb = c->cnb, BB = (int64_t)B * B;
@@ -2517,7 +2655,7 @@ int enqueue_chain_solve(Ctx* c) {
     if (c->cnu) hipLaunchKernelGGL(zero_slots_kernel, dim3(cgrid(c->cnu)), dim3(kCB), 0, s, c->cnu, c->d_cunc, D);
     if (c->cnd) hipLaunchKernelGGL(dummy_kernel<B>, dim3(cgrid(c->cnd)), dim3(kCB), 0, s, c->cnd, c->d_cdum, D, b);
     if (c->nnz) hipLaunchKernelGGL(pack_kernel, dim3(cgrid(c->nnz)), dim3(kCB), 0, s, c->nnz, c->d_cslot, c->d_nzval, D, pc);
-    hipLaunchKernelGGL(pack_rhs_kernel, dim3(cgrid(c->nX)), dim3(kCB), 0, s, c->nX, c->d_cpos, c->d_rhs, b, pc);
+    hipLaunchKernelGGL(pack_rhs_kernel, dim3(cgrid(c->nX)), dim3(kCB), 0, s, c->nX, c->d_cpos, rhs, b, pc);
     if (c->cnp) hipLaunchKernelGGL(pair_kernel<B>, dim3(cgrid(c->cnp)), dim3(kCB), 0, s, c->cnp, c->d_chpos, pc, D, L,
                                    U, b, c->d_csaved, c->d_cflag);
     HIPCHK(c, hipEventRecord(c->cev[1], s));
@@ -2620,8 +2758,8 @@ int mh_chain_prepare(mh_ctx* c, int B, int64_t nblock, const int64_t* pos, int64
     HIPCHK(c, hipMalloc(&c->d_cbuf, sizeof(double) * ntot));
     HIPCHK(c, hipMalloc(&c->d_csaved, sizeof(double) * std::max<int64_t>(npair * (3 * B + 1), 1)));
     HIPCHK(c, hipMalloc(&c->d_dx, sizeof(double) * std::max<int64_t>(c->nX, 1)));
-    HIPCHK(c, hipMalloc(&c->d_cflag, sizeof(unsigned)));
-    HIPCHK(c, hipHostMalloc((void**)&c->h_cflag, sizeof(unsigned), hipHostMallocDefault));
+    HIPCHK(c, hipMalloc(&c->d_cflag, chain::kStatusBytes));
+    HIPCHK(c, hipHostMalloc((void**)&c->h_cflag, chain::kStatusBytes, hipHostMallocDefault));
     for (auto& e : c->cev) HIPCHK(c, hipEventCreate(&e));
     if (c->nX) HIPCHK(c, hipMemcpy(c->d_cpos, p32.data(), sizeof(int32_t) * c->nX, hipMemcpyHostToDevice));
     if (npair) HIPCHK(c, hipMemcpy(c->d_chpos, h32.data(), sizeof(int32_t) * npair, hipMemcpyHostToDevice));
@@ -2636,16 +2774,43 @@ int mh_chain_prepare(mh_ctx* c, int B, int64_t nblock, const int64_t* pos, int64
     return r;
 }
 
+// Reads the status words of the last chain solve (stream synchronized by the caller's copy).
+static int chain_status(Ctx* c) {
+    HIPCHK(c, hipMemcpyAsync(c->h_cflag, c->d_cflag, chain::kStatusBytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const unsigned f = c->h_cflag[0];
+    const unsigned long long* where = reinterpret_cast<const unsigned long long*>(c->h_cflag + 2);
+    if (f & chain::kBadPair) {
+        c->err = "λ row with a nonzero diagonal or a zero coupling (pair " + std::to_string(where[1] + 1) + ")";
+        return MH_ERR_SOLVE;
+    }
+    if (f & chain::kSingular) {
+        c->err = "zero or non-finite pivot in diagonal block " + std::to_string(where[0]) + " (0-based, chain order)";
+        return MH_ERR_SOLVE;
+    }
+    return MH_OK;
+}
+
 int mh_chain_solve(mh_ctx* c) {
     if (!c || !c->prepared || !c->cB) return MH_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
-    if (int r = (c->cB == 6 ? enqueue_chain_solve<6>(c) : enqueue_chain_solve<3>(c))) return r;
-    HIPCHK(c, hipMemcpyAsync(c->h_cflag, c->d_cflag, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    const unsigned f = *c->h_cflag;
-    if (f & chain::kBadPair) { c->err = "λ row with a nonzero diagonal or a zero coupling"; return MH_ERR_SOLVE; }
-    if (f & chain::kSingular) { c->err = "zero or non-finite pivot in a diagonal block"; return MH_ERR_SOLVE; }
-    return MH_OK;
+    if (int r = (c->cB == 6 ? enqueue_chain_solve<6>(c, c->d_rhs) : enqueue_chain_solve<3>(c, c->d_rhs))) return r;
+    return chain_status(c);
+}
+
+int mh_adjoint_solve(mh_ctx* c, const double* rhs, double alpha) {
+    if (!c || !c->prepared || !c->cB || !rhs || !c->d_scaleX) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = std::max<int64_t>(c->nX, 1);
+    if (!c->d_crhs) HIPCHK(c, hipMalloc(&c->d_crhs, sizeof(double) * n));
+    if (!c->d_lam) HIPCHK(c, hipMalloc(&c->d_lam, sizeof(double) * n));
+    hipStream_t s = c->stream;
+    if (c->nX) HIPCHK(c, hipMemcpyAsync(c->d_crhs, rhs, sizeof(double) * c->nX, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(scaled_copy_kernel, dim3(grid(n)), dim3(kBlock), 0, s, c->nX, 1., c->d_scaleX, c->d_crhs, c->d_crhs);
+    if (int r = (c->cB == 6 ? enqueue_chain_solve<6>(c, c->d_crhs) : enqueue_chain_solve<3>(c, c->d_crhs))) return r;
+    hipLaunchKernelGGL(scaled_copy_kernel, dim3(grid(n)), dim3(kBlock), 0, s, c->nX, alpha, c->d_scaleX, c->d_dx, c->d_lam);
+    if (int r = check_launch(c, "adjoint solve")) return r;
+    return chain_status(c);
 }
 
 int mh_chain_timings(mh_ctx* c, double* ms) {

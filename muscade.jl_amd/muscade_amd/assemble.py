@@ -85,6 +85,13 @@ def upload_model(ctx, model, dis, skip=(), nA=None):
         g = ctypes.c_int()
         ctx.check(lib.mh_group_add(ctx.h, kind, nele, nX, i64ptr(idx) if nX else None, dptr(sX), nU, i64ptr(idxU),
                                    i64ptr(nodes), dptr(orient), dptr(mat), ctypes.byref(g)), f"element type {it + 1}")
+        amask = E.params[0].get("amask", 0) if E.params else 0
+        if amask:   # EulerBeam3DA: stiffnesses scaled by 10^A
+            if nA == 0:
+                raise MuscadeError(f"element type {it + 1} ({E.key[0]}) is scaled by A-dofs, which this "
+                                   "assembler does not carry")
+            ctx.check(lib.mh_group_set_adofs(ctx.h, g.value, i64ptr(as_i64(d.A[:, :1])), int(amask)),
+                      f"A-dofs of element type {it + 1}")
         groups.append(g.value)
     return groups
 
@@ -138,10 +145,11 @@ class AssemblySweepX:
         if self.model.ndof["A"]:
             self.ctx.check(lib.mh_set_A(h, dptr(as_f64(state.A))), "set A")
         for it, E in enumerate(self.model.eletypes):
-            if E.cls.kind == _lib.MH_KIND_DOFLOAD and E.nele:
+            if E.cls.kind in (_lib.MH_KIND_DOFLOAD, _lib.MH_KIND_HOLD) and E.nele and "value" in E.params[0]:
+                # DofLoad's value(t), DofMotion's prescribed value
                 vals = np.concatenate([np.full(n.shape[0], float(p["value"](t, **p["args"])))
                                        for n, p in zip(E.nodes, E.params)])
-                self.ctx.check(lib.mh_set_point_values(h, self.groups[it], dptr(as_f64(vals))), "DofLoad values")
+                self.ctx.check(lib.mh_set_point_values(h, self.groups[it], dptr(as_f64(vals))), "point values")
 
     def assemble(self, state, mission, dbg=None, fetch=True, upload=True):
         """assemble!{:step|:iter}(out,asm,dis,model,state,Δt,dbg). mission: 'step' or 'iter'.

@@ -20,6 +20,8 @@ closed loop, is not a chain: `chain_layout` raises NotAChain and the host factor
 remains the way to solve it.
 """
 
+import re
+
 import numpy as np
 import scipy.sparse as sp
 from scipy.sparse.csgraph import connected_components, depth_first_order
@@ -165,8 +167,18 @@ class ChainSolver:
         r = lib.mh_chain_solve(self.out.ctx.h)
         if r == _lib.MH_ERR_SOLVE:
             d = dict(dbg or {})
+            d["reason"] = self.failure()
             raise MuscadeError("matrix factorization failed at step=%s, iiter=%s" % (d.get("step"), d.get("iiter")), d)
         self.out.ctx.check(r, "chain solve")
+
+    def failure(self):
+        """The library's reason for the last failed solve, with the model node of a singular
+        diagonal block (1-based) where it names one."""
+        msg = lib.mh_last_error(self.out.ctx.h).decode()
+        m = re.search(r"diagonal block (\d+)", msg)
+        if m and int(m.group(1)) < self.layout.node_order.size:
+            msg += f", node {int(self.layout.node_order[int(m.group(1))]) + 1}"
+        return msg
 
     def dx(self):
         x = np.zeros(self.out.nX)

@@ -87,6 +87,52 @@ class EulerBeam3DU(EulerBeam3D):
     kind = _lib.MH_KIND_BEAM_U
 
 
+STIFFNESS_FIELDS = ("EA", "EI₂", "EI₃", "GJ")
+
+
+class EulerBeam3DA(EulerBeam3D):
+    """An EulerBeam3D whose section stiffnesses `scaled` (a subset of EA, EI₂, EI₃, GJ) are
+    multiplied by 10^a, a being one A-dof of field `afield` at the element's last node: the
+    parameter-identification wrapper in the style of the reference's Spring, which scales its L₀
+    and EA by exp10 of its A-dofs (test/SomeElements.jl:166-186). Elements of a segment share
+    one A node, hence one parameter. The residual is EulerBeam3D's with the scaled section
+    (toolbox/BeamElement.jl:151-174); R is linear in each stiffness, so
+    ∂R/∂a = ln 10 · (R of the scaled stiffnesses alone, without loads). The device evaluates it
+    like any beam (mh_group_set_adofs)."""
+    Udof = False
+    kind = _lib.MH_KIND_BEAM
+
+    @classmethod
+    def doflist(cls, afield="ΞEI"):
+        inod, dclass, field = EulerBeam3D.doflist.__func__(cls)
+        return inod + (max(inod) + 1,), dclass + ("A",), field + (afield,)
+
+    @classmethod
+    def construct(cls, model, nodes0, mat, orient2=(0., 1., 0.), scaled=("EI₂", "EI₃"), afield="ΞEI"):
+        scaled = tuple(scaled)
+        if not scaled or any(f not in STIFFNESS_FIELDS for f in scaled):
+            raise MuscadeError(f"scaled must name section stiffnesses among {STIFFNESS_FIELDS}")
+        key, _, params = EulerBeam3D.construct.__func__(cls, model, nodes0, mat, orient2)
+        params["amask"] = sum(1 << STIFFNESS_FIELDS.index(f) for f in scaled)
+        return key + ("A", params["amask"], afield), cls.doflist(afield), params
+
+    @staticmethod
+    def scaled_section(mat, a, amask):
+        """The section with its scaled stiffnesses × 10^a (mat (nele,16), a (nele,))."""
+        m = np.array(mat, dtype=float, copy=True)
+        for k in range(4):
+            if amask >> k & 1:
+                m[:, k] = m[:, k] * 10. ** np.asarray(a, float)
+        return m
+
+
+class EulerBeam3DAU(EulerBeam3DA):
+    """EulerBeam3DA with 3 U-dofs on a third node, a distributed load per length (EulerBeam3D{true});
+    the A-dof is on the fourth node."""
+    Udof = True
+    kind = _lib.MH_KIND_BEAM_U
+
+
 class StrainGaugeOnEulerBeam3D(EulerBeam3D):
     """StrainGaugeOnEulerBeam3D(nod; P, D, elementkwargs=(mat, orient2)) —
     toolbox/StrainGaugeOnBeamElement.jl:55-76. An EulerBeam3D (same dofs, same residual: the
@@ -283,6 +329,35 @@ class Hold:
         λf = λfield if λfield is not None else "λ" + field
         doflist = ((1, 1), ("X", "X"), (field, λf))
         return ("DofConstraint", "X", (field,), λf, "equal"), doflist, {}
+
+
+class DofMotion(Hold):
+    """DofMotion(nod; field, value, λfield=Symbol(:λ,field), args...): a prescribed motion of one
+    X-dof, the DofConstraint{:X} with gap(x, t) = x[1] − value(t, args...) and mode equal
+    (src/BasicElements.jl:406-438; examples/DynamicBeamAnalysis.jl:114-117 prescribes the top of
+    a riser this way, with horizMove and vertMove). R = (−λ, −(x − value(t))), K = [[0,−1],[−1,0]]:
+    the device runs it as a Hold whose gap carries the value, which the host evaluates at each
+    time like DofLoad's (the Julia binding keeps a generic DofConstraint on the host kind)."""
+    kind = _lib.MH_KIND_HOLD
+    direct_host = True
+
+    @staticmethod
+    def lagrangian(p, Λ, X, U, A, t):
+        return Λ[0] * (-X[0][1]) + Λ[1] * (-(X[0][0] - float(p["value"](t, **p["args"]))))
+
+    @staticmethod
+    def values(E, t):
+        """value(t, args...) of every element of type E, in element order."""
+        return np.concatenate([np.full(n.shape[0], float(p["value"](t, **p["args"])))
+                               for n, p in zip(E.nodes, E.params)])
+
+    @classmethod
+    def construct(cls, model, nodes0, field, value, λfield=None, **args):
+        λf = λfield if λfield is not None else "λ" + field
+        name = getattr(value, "__name__", type(value).__name__)
+        doflist = ((1, 1), ("X", "X"), (field, λf))
+        return ("DofConstraint", "X", (field,), λf, "equal", "motion", name, tuple(sorted(args))), doflist, \
+            {"value": value, "args": args}
 
 
 def equal(t):

@@ -330,3 +330,131 @@ def setdof(state, value, cls="X", field=None, nodID=None, order=0):
         sc.append(np.zeros(n))
     sc[order][ids - 1] = value
     return state
+
+
+class DofGroup:
+    """DofGroup (src/Assemble.jl:167-188): a selection of model dofs stacked class after class in a
+    solver vector, y[j] ↔ state[i]·scale. i*: model dofs (1-based) of each class in the group,
+    j*: their positions in y (1-based), scale*, field* per group entry. nX, nU, nA are the
+    model's dof counts."""
+
+    def __init__(self, nX, nU, nA, iΛ, iX, iU, iA, jΛ, jX, jU, jA, sΛ, sX, sU, sA, fΛ, fX, fU, fA):
+        self.nX, self.nU, self.nA = nX, nU, nA
+        self.iΛ, self.iX, self.iU, self.iA = [np.asarray(v, np.int64) for v in (iΛ, iX, iU, iA)]
+        self.jΛ, self.jX, self.jU, self.jA = jΛ, jX, jU, jA
+        self.scaleΛ, self.scaleX, self.scaleU, self.scaleA = sΛ, sX, sU, sA
+        self.fieldΛ, self.fieldX, self.fieldU, self.fieldA = fΛ, fX, fU, fA
+
+    @classmethod
+    def from_dis(cls, dis, iΛ, iX, iU, iA):
+        """DofGroup(dis, iΛ, iX, iU, iA) (Assemble.jl:190-200): classes stacked, no permutation."""
+        iΛ, iX, iU, iA = [np.asarray(list(v), np.int64) for v in (iΛ, iX, iU, iA)]
+        nX, nU, nA = len(dis.scaleX), len(dis.scaleU), len(dis.scaleA)
+        jΛ, jX, jU, jA = gradientpartition(len(iΛ), len(iX), len(iU), len(iA))
+        pick = lambda a, i: a[i - 1] if len(i) else a[:0]   # noqa: E731
+        pickf = lambda f, i: [f[k - 1] for k in i]          # noqa: E731
+        return cls(nX, nU, nA, iΛ, iX, iU, iA, jΛ, jX, jU, jA, pick(dis.scaleΛ, iΛ), pick(dis.scaleX, iX),
+                   pick(dis.scaleU, iU), pick(dis.scaleA, iA), pickf(dis.fieldX, iΛ), pickf(dis.fieldX, iX),
+                   pickf(dis.fieldU, iU), pickf(dis.fieldA, iA))
+
+    def getndof(self):
+        return len(self.iΛ) + len(self.iX) + len(self.iU) + len(self.iA)
+
+
+def gradientpartition(nΛ, nX, nU, nA, nXder=1, nUder=None):
+    """Indices (1-based ranges) of the class partitions of a gradient (Assemble.jl:327-335)."""
+    nUder = nXder if nUder is None else nUder
+    return (range(1, nΛ + 1), range(nΛ + 1, nΛ + nX + 1), range(nΛ + nX * nXder + 1, nΛ + nX * nXder + nU + 1),
+            range(nΛ + nX * nXder + nU * nUder + 1, nΛ + nX * nXder + nU * nUder + nA + 1))
+
+
+def gradientstructure(dofgr, eledis):
+    """Number of dofs of each class in the gradient an element returns (Assemble.jl:321-326)."""
+    return (0 if len(dofgr.iΛ) == 0 else eledis.X.shape[1], 0 if len(dofgr.iX) == 0 else eledis.X.shape[1],
+            0 if len(dofgr.iU) == 0 else eledis.U.shape[1], 0 if len(dofgr.iA) == 0 else eledis.A.shape[1])
+
+
+def indexedstate(dofgr):
+    """(Λ, X, U, A): for each model dof its position in the group (1-based), 0 if not in it
+    (Assemble.jl:246-256)."""
+    out = []
+    for n, i, j in ((dofgr.nX, dofgr.iΛ, dofgr.jΛ), (dofgr.nX, dofgr.iX, dofgr.jX), (dofgr.nU, dofgr.iU, dofgr.jU),
+                    (dofgr.nA, dofgr.iA, dofgr.jA)):
+        v = np.zeros(n, np.int64)
+        if len(i):
+            v[i - 1] = np.asarray(list(j), np.int64)
+        out.append(v)
+    return tuple(out)
+
+
+def _group(model, dis, Λ=False, X=False, U=False, A=False):
+    r = lambda c, on: range(1, model.ndof[c] + 1) if on else range(0)   # noqa: E731
+    return DofGroup.from_dis(dis, r("X", Λ), r("X", X), r("U", U), r("A", A))
+
+
+def nodofs(model, dis):
+    return _group(model, dis)
+
+
+def allΛdofs(model, dis):
+    return _group(model, dis, Λ=True)
+
+
+def allXdofs(model, dis):
+    """allXdofs(model, dis) (Assemble.jl:261): the group of SweepX."""
+    return _group(model, dis, X=True)
+
+
+def allUdofs(model, dis):
+    return _group(model, dis, U=True)
+
+
+def allAdofs(model, dis):
+    return _group(model, dis, A=True)
+
+
+def allXUdofs(model, dis):
+    return _group(model, dis, X=True, U=True)
+
+
+def allΛXUdofs(model, dis):
+    return _group(model, dis, Λ=True, X=True, U=True)
+
+
+def allΛXUAdofs(model, dis):
+    return _group(model, dis, Λ=True, X=True, U=True, A=True)
+
+
+def selecteddofs(model, dis, classes):
+    """selecteddofs(model, dis, classes) (Assemble.jl:268-274)."""
+    return _group(model, dis, Λ="Λ" in classes, X="X" in classes, U="U" in classes, A="A" in classes)
+
+
+def decrement(state, ider, y, gr):
+    """decrement!(state, ider, y, gr) (Assemble.jl:201-206), ider 1-based like the reference."""
+    y = np.asarray(y, float)
+    jl = lambda j: np.asarray(list(j), np.int64) - 1   # noqa: E731
+    if ider <= len(state.Λ) and len(gr.iΛ):
+        state.Λ[ider - 1][gr.iΛ - 1] -= y[jl(gr.jΛ)] * gr.scaleΛ
+    if ider <= len(state.X) and len(gr.iX):
+        state.X[ider - 1][gr.iX - 1] -= y[jl(gr.jX)] * gr.scaleX
+    if ider <= len(state.U) and len(gr.iU):
+        state.U[ider - 1][gr.iU - 1] -= y[jl(gr.jU)] * gr.scaleU
+    if ider == 1 and len(gr.iA):
+        state.A[gr.iA - 1] -= y[jl(gr.jA)] * gr.scaleA
+
+
+def getdofs(state, ider, gr):
+    """getdof!(state, ider, y, gr) (Assemble.jl:232-237): y[j] = state[i]/scale, ider 0-based as
+    in the reference (the derivative ider+1)."""
+    y = np.zeros(gr.getndof())
+    jl = lambda j: np.asarray(list(j), np.int64) - 1   # noqa: E731
+    if len(gr.iΛ):
+        y[jl(gr.jΛ)] = state.Λ[ider][gr.iΛ - 1] / gr.scaleΛ
+    if len(gr.iX):
+        y[jl(gr.jX)] = state.X[ider][gr.iX - 1] / gr.scaleX
+    if len(gr.iU):
+        y[jl(gr.jU)] = state.U[ider][gr.iU - 1] / gr.scaleU
+    if len(gr.iA):
+        y[jl(gr.jA)] = state.A[gr.iA - 1] / gr.scaleA
+    return y

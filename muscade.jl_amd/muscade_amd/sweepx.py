@@ -33,12 +33,12 @@ class SweepX(metaclass=_SweepXMeta):
 
     @classmethod
     def run(cls, pstate, verbose, dbg, *, time, initialstate, β=0.25, γ=0.5, maxiter=50, maxΔx=1e-5,
-            maxLλ=math.inf, saveiter=False, device=0, Assembly=AssemblySweepX, graph=True, linsolve="host"):
+            maxLλ=math.inf, saveiter=False, device=0, Assembly=AssemblySweepX, graph=True, linsolve="host", stats=None):
         """`Assembly` is the assembler type prepared for the model, as the type argument of
         prepare(AssemblySweepX{OX},model,dis) (SweepX.jl:24,188). graph: the device assembler
         replays captured HIP graphs for the Newton iterations of each time step.
         linsolve: "host" (SuperLU) or "device" (chain topologies only; raises
-        chainsolve.NotAChain otherwise)."""
+        chainsolve.NotAChain otherwise). stats (a dict) receives the Newton iteration count."""
         OX = cls.OX
         model, dis = initialstate.model, initialstate.dis
         if linsolve not in ("host", "device"):
@@ -69,6 +69,8 @@ class SweepX(metaclass=_SweepXMeta):
                 out.set_state(state, state.time, X=(step == 1))
             for iiter in range(1, maxiter + 1):
                 citer += 1
+                if stats is not None:
+                    stats["iterations"] = citer
                 first = iiter == 1
                 d = dict(dbg, solver="SweepX", step=step, iiter=iiter)
                 if dev:

@@ -25,10 +25,18 @@ cantilever_batch: the batched-adjoint case — many uneven cantilevers
 N(0, 1) kN/m), with a state of small random displacements and rotations and a
 random adjoint vector Λ. The same models are built either one Model each (the
 plain path) or all into one Model as disconnected parts (the batch).
+
+design_point_batch: the batched design-point search — uneven cantilevers of L = 10 m
+(n_e ~ U{nmin..nmax} EulerBeam3DA{true} elements), whose section stiffnesses (EA, EI₂, EI₃,
+GJ) are scaled per sixteenth of the length by 10^a, a ~ N(0, 0.1/ln 10) (a lognormal
+stiffness factor of σ = 0.1), loaded by 8 distributed U-loads (one U-node per eighth, t3
+~ N(1, 0.2)·q₀, t1 and t2 ~ N(0, 0.1)·q₀, q₀ = 1 kN/m); the limit state is the tip
+displacement along z against a threshold.
 """
 import numpy as np
 
-from .elements import BeamCrossSection, DofLoad, EulerBeam3D, EulerBeam3DU, Hold
+from .elements import (STIFFNESS_FIELDS, BeamCrossSection, DofLoad, DofMotion, EulerBeam3D, EulerBeam3DAU,
+                       EulerBeam3DU, Hold)
 from .model import Model, addelement, addnode, initialize, setdof
 
 E_STEEL = 2.1e11
@@ -117,7 +125,17 @@ def topload(t, F0=1e5, T=10.):
     return F0 * np.sin(2. * np.pi * t / T)
 
 
-def catenary_chains(nchain=256, nmin=1000, nmax=50000, seed=1002, disp=1e-3, rot=1e-3, vel=0.1, acc=0.1):
+def top_motion(t, a=1., T=10.):
+    """Prescribed horizontal motion of a chain's top: a·sin(2πt/T)."""
+    return a * np.sin(2. * np.pi * t / T)
+
+
+def catenary_chains(nchain=256, nmin=1000, nmax=50000, seed=1002, disp=1e-3, rot=1e-3, vel=0.1, acc=0.1,
+                    motion=False):
+    """motion=True: the top of each chain held vertically and transversally and moved
+    horizontally, a·sin(2πt/T) with a ~ U(0.5, 3) m and T ~ U(6, 15) s per chain (DofMotion, the
+    prescribed-motion DofConstraint of examples/DynamicBeamAnalysis.jl:114-117), instead of the
+    horizontal top force."""
     rng = np.random.default_rng(seed)
     model = Model("catenary_chains")
     ne = rng.integers(nmin, nmax + 1, size=nchain)
@@ -144,7 +162,15 @@ def catenary_chains(nchain=256, nmin=1000, nmax=50000, seed=1002, disp=1e-3, rot
     tops = np.array(first) + ne + 1
     for f in ("t1", "t2", "t3"):
         addelement(model, Hold, anchors.reshape(-1, 1), field=f)
-    addelement(model, DofLoad, tops.reshape(-1, 1), field="t1", value=topload)
+    if motion:
+        amp = rng.uniform(0.5, 3., size=nchain)
+        per = rng.uniform(6., 15., size=nchain)
+        for c in range(nchain):
+            addelement(model, DofMotion, [int(tops[c])], field="t1", value=top_motion, a=float(amp[c]), T=float(per[c]))
+        for f in ("t2", "t3"):
+            addelement(model, Hold, tops.reshape(-1, 1), field=f)
+    else:
+        addelement(model, DofLoad, tops.reshape(-1, 1), field="t1", value=topload)
     state = initialize(model, nXder=3)
     nX = model.ndof["X"]
     state.X[0][:] = rng.normal(0., disp, nX)
@@ -224,3 +250,59 @@ def cantilever_batch(specs, joined=True):
         state.U[0][:] = u
         out.append((model, state, lam))
     return out
+
+
+def design_point_specs(nmodel=512, nmin=500, nmax=20000, seed=1005):
+    """Per-model element counts and seeds of the design-point batch."""
+    rng = np.random.default_rng(seed)
+    ne = rng.integers(nmin, nmax + 1, size=nmodel)
+    return [dict(ne=int(ne[m]), seed=int(seed * 1000 + m)) for m in range(nmodel)]
+
+
+def _add_cantilever_a(model, spec, L=10., yoff=0.):
+    """One cantilever of the design-point batch; returns (first node, tip node)."""
+    n = spec["ne"]
+    x = np.arange(n + 1) * (L / n)
+    base = model.nnod
+    xyz = np.stack([x, np.full_like(x, yoff), np.zeros_like(x)], axis=1)
+    useg = np.minimum((np.arange(n) * 8) // n, 7)
+    aseg = np.minimum((np.arange(n) * 16) // n, 15)
+    uxyz = np.stack([(np.arange(8) + 0.5) * L / 8, np.full(8, yoff + 1.), np.zeros(8)], axis=1)
+    axyz = np.stack([(np.arange(16) + 0.5) * L / 16, np.full(16, yoff + 2.), np.zeros(16)], axis=1)
+    addnode(model, np.concatenate([xyz, uxyz, axyz]))
+    nodes = np.stack([base + 1 + np.arange(n), base + 2 + np.arange(n), base + n + 2 + useg, base + n + 10 + aseg],
+                     axis=1)
+    addelement(model, EulerBeam3DAU, nodes, mat=base_section(), orient2=np.array([0., 1., 0.]),
+               scaled=STIFFNESS_FIELDS, afield="Ξk")
+    return base + 1, base + n + 1
+
+
+def _design_vectors(spec, q0):
+    """U (8 U-nodes × t1, t2, t3) and A (16) of one cantilever."""
+    rng = np.random.default_rng(spec["seed"])
+    u = np.zeros((8, 3))
+    u[:, :2] = rng.normal(0., 0.1, (8, 2)) * q0
+    u[:, 2] = rng.normal(1., 0.2, 8) * q0
+    a = rng.normal(0., 0.1 / np.log(10.), 16)
+    return u.ravel(), a
+
+
+def design_point_batch(specs, joined=True, q0=1e3):
+    """The batch as one Model (joined) or one Model per spec. Returns a list of (model, state,
+    tip) with the state's U and A set and tip the 1-based X-dofs of the tip displacements
+    along z (one per cantilever)."""
+    def build(sp_list, name):
+        model = Model(name)
+        ends = [_add_cantilever_a(model, s_, yoff=3. * m) for m, s_ in enumerate(sp_list)]
+        for fld in ("t1", "t2", "t3", "r1", "r2", "r3"):
+            addelement(model, Hold, np.array([f for f, _ in ends]).reshape(-1, 1), field=fld)
+        state = initialize(model, time=0.)
+        for m, s_ in enumerate(sp_list):
+            u, a = _design_vectors(s_, q0)
+            state.U[0][24 * m:24 * m + 24] = u
+            state.A[16 * m:16 * m + 16] = a
+        tip = np.array([model.getdofID("X", "t3", nodID=[t])[0] for _, t in ends])
+        return model, state, tip
+    if joined:
+        return [build(specs, "design_point_batch")]
+    return [build([s_], "cantilever") for s_ in specs]

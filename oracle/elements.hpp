@@ -11,12 +11,14 @@ namespace orc {
 
 // DofConstraint{:X} with gap(x,t)=x[1] and mode `equal` — what Hold builds
 // (src/BasicElements.jl:426-438,484-488). X = (x, λ):  R = (-g∂x·λ, -gap).
-template <class TX> inline void hold_residual(const TX* X0, TX* R) {
+// g: the prescribed value of a DofConstraint{:X} with gap(x,t) = x[1] - g(t) (0 for Hold,
+// whose gap is x[1], BasicElements.jl:484-488).
+template <class TX> inline void hold_residual(const TX* X0, double g, TX* R) {
     // P = constants(∂0(X),t); x∂ = variate{P,1}(x); gap,g∂x = value_∂{P,1}(gap(x∂))
     TX x = X0[0], lam = X0[1];
     double gdx = 1.;
     R[0] = -gdx * lam;
-    R[1] = -x;
+    R[1] = -(x - g);
 }
 
 // DofLoad (BasicElements.jl:281-284): R = [-F], F = value(t) evaluated by the caller.

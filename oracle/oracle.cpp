@@ -77,7 +77,7 @@ static int elem_residual(const EType& ty, int e, const TX (*X)[12], const double
         }
         return 0;
     }
-    case K_HOLD: hold_residual<TX>(X[0], R); return 0;
+    case K_HOLD: hold_residual<TX>(X[0], ty.F.empty() ? 0. : ty.F[e], R); return 0;
     case K_DOFLOAD: dofload_residual<TX>(ty.F[e], R); return 0;
     case K_ZERO: for (int i = 0; i < ty.nX; ++i) R[i] = TX(0.); return 0;
     case K_TURBINE: turbine_residual<TX>(ty.turbines[e], X[0], A, R); return 0;
@@ -426,6 +426,7 @@ void orc_model_free(void* h) { delete (OModel*)h; }
 //   K_BEAM    : nele × 25 = c1(3) c2(3) orient2(3) mat(16)
 //   K_BAR     : nele × 16 = c1(3) c2(3) mat(9: EA μ w Caₜ Clₜ Cqₜ Caₙ Clₙ Cqₙ) ϵₛ
 //   K_DOFLOAD : nele × 1  = F
+//   K_HOLD    : nele × 1  = g, the prescribed value (gap x - g), or none (Hold: g = 0)
 //   K_TURBINE : nele × 5  = xm(2) z seadrag skydrag
 //   K_ANCHOR  : nele × 10 = xmtop(3) dxmtop(3) xmbot(2) L buoyancy
 int orc_model_add_type(void* h, int kind, int nele, int nX, int nU, int nA, const int64_t* iX, const int64_t* iU,
@@ -453,7 +454,7 @@ int orc_model_add_type(void* h, int kind, int nele, int nX, int nU, int nA, cons
             AxisymmetricBarCrossSection mat; std::memcpy(&mat, p + 6, sizeof(mat));
             make_bar(p, p + 3, mat, p[15], udof, &t.bars[e]);
         }
-    } else if (kind == K_DOFLOAD) {
+    } else if (kind == K_DOFLOAD || (kind == K_HOLD && params)) {   // loads; prescribed values of a held dof
         t.F.assign(params, params + nele);
     } else if (kind == K_TURBINE) {
         t.turbines.resize(nele);
@@ -752,7 +753,7 @@ static void direct_residual(const EType& t, int e, const TX (*X)[12], const TX* 
     } else if (t.kind == K_BAR) {
         bar_residual<TX, ND>(t.bars[e], X, t.udof ? U : nullptr, time, R);
     } else if (t.kind == K_HOLD) {
-        hold_residual<TX>(X[0], R);
+        hold_residual<TX>(X[0], t.F.empty() ? 0. : t.F[e], R);
     } else if (t.kind == K_DOFLOAD) {
         dofload_residual<TX>(t.F[e], R);
     }

@@ -49,7 +49,7 @@ def oracle_model(model, dis, t=0.0, skip=(), nA=None):
             mat = np.concatenate([p["mat"] for p in E.params])
             eps = np.concatenate([p["eps_s"] for p in E.params])[:, None]
             params = np.concatenate([xyz[nodes[:, 0], :3], xyz[nodes[:, 1], :3], mat, eps], axis=1)
-        elif kind == orc.K_DOFLOAD:
+        elif kind == orc.K_DOFLOAD or (kind == orc.K_HOLD and E.nele and "value" in E.params[0]):
             params = np.concatenate([np.full(n.shape[0], float(p["value"](t, **p["args"])))
                                      for n, p in zip(E.nodes, E.params)])
         om.add_type(kind, d.X, d.U if d.U.shape[1] else None, d.A if d.A.shape[1] else None, d.scaleX, params,
@@ -198,7 +198,8 @@ class HostSweepXAssembly:
                         continue
                     if point == "hold":
                         x, lam = state.X[0][iX[:, 0]], state.X[0][iX[:, 1]]
-                        R = np.stack([-lam, -x], axis=1)
+                        g = float(p["value"](state.time, **p["args"])) if "value" in p else 0.
+                        R = np.stack([-lam, -(x - g)], axis=1)
                         K = np.broadcast_to(np.array([[0., -1.], [-1., 0.]]), (m, 2, 2))
                     else:
                         R = np.full((m, 1), -float(p["value"](state.time, **p["args"])))

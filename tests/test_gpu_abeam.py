@@ -1,0 +1,148 @@
+"""EulerBeam3DA (section stiffnesses scaled by 10^a, a an A-dof) on the device and the adjoint
+gradients of a limit state (mh_group_set_adofs, mh_vjp_lagrangian's ∂L/∂A, mh_adjoint_solve):
+
+- element R and K against the oracle's EulerBeam3D with the scaled section, 1e-12;
+- ∂L/∂A against the oracle's ln 10 · Λᵀ R(scaled stiffnesses alone, no loads), 1e-12, and
+  against central differences of Λ·R(A) assembled on the device (h = 1e-6), 1e-6;
+- the limit-state gradient dg/dA, dg/dU of the tip displacement at equilibrium against central
+  differences of re-solved equilibria, 1e-5; the adjoint solve's ∂L/∂X = −∂g/∂X; the batch's
+  gradients against each model's own."""
+import numpy as np
+import pytest
+
+from tests import _oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mc():
+    import muscade_amd
+    return muscade_amd
+
+
+def batch(mc, ne=(20, 31, 45), joined=True):
+    from muscade_amd.workloads import design_point_batch, design_point_specs
+    specs = design_point_specs(len(ne), 1, 2)
+    for s, n in zip(specs, ne):
+        s["ne"] = n
+    return design_point_batch(specs, joined=joined)
+
+
+def random_state(model, state, seed):
+    rng = np.random.default_rng(seed)
+    st = state.copy()
+    st.X[0][:] = rng.normal(0., 1e-2, model.ndof["X"])
+    rd = np.concatenate([model.getdofID("X", f) for f in ("r1", "r2", "r3")]) - 1
+    st.X[0][rd] = rng.normal(0., 0.1, rd.size)
+    st.A[:] = rng.normal(0., 0.2, model.ndof["A"])
+    return st
+
+
+def oracle_elements(model, st, it, loads=True, masked_only=False):
+    E = model.eletypes[it]
+    d = st.dis.dis[it]
+    xyz = model.coords()
+    nodes = E.cat_nodes()
+    o2 = np.concatenate([p["orient2"] for p in E.params])
+    mat = np.concatenate([p["mat"] for p in E.params])
+    amask = E.params[0]["amask"]
+    mat = E.cls.scaled_section(mat, st.A[d.A[:, 0] - 1], amask)
+    if masked_only:
+        mat = mat.copy()
+        for k in range(4):
+            if not amask >> k & 1:
+                mat[:, k] = 0.
+        mat[:, 4:] = 0.
+    R, K = np.zeros((E.nele, 12)), np.zeros((E.nele, 12, 12))
+    for e in range(E.nele):
+        U = st.U[0][d.U[e] - 1] if (loads and not masked_only) else np.zeros(3)
+        r, Ks, _ = orc.beam_diffed_residual(xyz[nodes[e, 0], :3], xyz[nodes[e, 1], :3], o2[e], mat[e],
+                                            [st.X[0][d.X[e] - 1]], U)
+        R[e], K[e] = r, Ks[0]
+    return R, K
+
+
+def test_scaled_beam_element_blocks(mc):
+    [(model, s0, _)] = batch(mc)
+    st = random_state(model, s0, 1)
+    out = mc.AssemblySweepX(0, model, st.dis)
+    out.assemble(st, "iter")
+    R, K = out.element_blocks(0)
+    Ro, Ko = oracle_elements(model, st, 0)
+    assert np.abs(R - Ro).max() <= 1e-12 * np.abs(Ro).max()
+    assert np.abs(K - Ko).max() <= 1e-12 * np.abs(Ko).max()
+    # a zero A gives the plain beam: the base section is kept apart from the scaled one
+    st0 = st.copy()
+    st0.A[:] = 0.
+    out.assemble(st0, "iter")
+    R0, _ = out.element_blocks(0)
+    Rb, _ = oracle_elements(model, st0, 0)
+    assert np.abs(R0 - Rb).max() <= 1e-12 * np.abs(Rb).max()
+
+
+def test_dLdA_against_oracle_and_differences(mc):
+    [(model, s0, _)] = batch(mc)
+    st = random_state(model, s0, 2)
+    rng = np.random.default_rng(3)
+    lam = rng.normal(size=model.ndof["X"])
+    out = mc.AssemblySweepX(0, model, st.dis)
+    gX, gU, gA = out.vjp(st, lam)
+    d = st.dis.dis[0]
+    Rm, _ = oracle_elements(model, st, 0, masked_only=True)
+    want = np.zeros(model.ndof["A"])
+    np.add.at(want, d.A[:, 0] - 1, np.log(10.) * np.sum(lam[d.X - 1] * Rm, axis=1))
+    assert np.abs(gA - want).max() <= 1e-12 * np.abs(want).max()
+    h = 1e-6
+    for a in (0, 7, model.ndof["A"] - 1):
+        sp, sm = st.copy(), st.copy()
+        sp.A[a] += h
+        sm.A[a] -= h
+        Lp = lam @ (out.assemble(sp, "iter")[0].copy() / st.dis.scaleX)
+        Lm = lam @ (out.assemble(sm, "iter")[0].copy() / st.dis.scaleX)
+        assert (Lp - Lm) / (2 * h) == pytest.approx(gA[a], rel=1e-6, abs=1e-9 * np.abs(gA).max())
+
+
+def equilibrium(mc, model, s0):
+    st = mc.solve(mc.SweepX[0], initialstate=s0, time=[1.], maxΔx=1e-9, verbose=False, catcherror=False,
+                  linsolve="device")
+    return st[-1]
+
+
+def test_limit_state_gradient_against_differences(mc):
+    [(model, s0, tip)] = batch(mc)
+    st = equilibrium(mc, model, s0)
+    lsg = mc.LimitStateGradient(model, st.dis)
+    dg = np.zeros(model.ndof["X"])
+    dg[tip[1] - 1] = 1.                     # g = tip displacement of the second cantilever
+    gA, gU, gX = lsg(st, dg)
+    assert np.abs(gX + dg).max() <= 1e-10
+    h = 1e-4
+    for a in (16, 20, 31):                  # the second cantilever's A-dofs
+        sp, sm = s0.copy(), s0.copy()
+        sp.A[a] += h
+        sm.A[a] -= h
+        fd = (equilibrium(mc, model, sp).X[0][tip[1] - 1] - equilibrium(mc, model, sm).X[0][tip[1] - 1]) / (2 * h)
+        assert gA[a] == pytest.approx(fd, rel=1e-5)
+    for u in (24 + 2, 24 + 14):             # t3 loads of the second cantilever
+        sp, sm = s0.copy(), s0.copy()
+        sp.U[0][u] += 1.
+        sm.U[0][u] -= 1.
+        fd = (equilibrium(mc, model, sp).X[0][tip[1] - 1] - equilibrium(mc, model, sm).X[0][tip[1] - 1]) / 2.
+        assert gU[u] == pytest.approx(fd, rel=1e-5)
+    assert np.abs(gA[:16]).max() == 0. and np.abs(gA[32:]).max() == 0.
+
+
+def test_limit_state_gradient_batched_equals_per_model(mc):
+    [(model, s0, tip)] = batch(mc)
+    st = equilibrium(mc, model, s0)
+    dg = np.zeros(model.ndof["X"])
+    dg[tip - 1] = 1.
+    gA, gU, _ = mc.LimitStateGradient(model, st.dis)(st, dg)
+    for m, (mm, sm0, tm) in enumerate(batch(mc, joined=False)):
+        sm = equilibrium(mc, mm, sm0)
+        d1 = np.zeros(mm.ndof["X"])
+        d1[tm - 1] = 1.
+        a1, u1, _ = mc.LimitStateGradient(mm, sm.dis)(sm, d1)
+        assert np.abs(gA[16 * m:16 * m + 16] - a1).max() <= 1e-9 * np.abs(a1).max()
+        assert np.abs(gU[24 * m:24 * m + 24] - u1).max() <= 1e-9 * np.abs(u1).max()

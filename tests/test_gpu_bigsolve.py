@@ -27,7 +27,9 @@ def with_A(mc, model, nel=6):
     """An A-dof a scaling a target tip motion, the same for every step (IA = 1)."""
     mc.addelement(model, mc.DofCost, [nel + 1], xinod=(1,), xfield=("t2",), ainod=(1,), afield=("a",),
                   cost=lambda X, U, A, t: 1e2 * (X[0][0] - 1e-3 * A[0] * np.sin(t)) ** 2)
-    mc.addelement(model, mc.SingleAcost, [nel + 1], field="a", cost=lambda a: 1e-2 * (a - 1.) ** 2)
+    # the prior enters once (assembleA!) and at every step (the reference's element loop reaches
+    # Acost elements too), so its weight is kept small
+    mc.addelement(model, mc.SingleAcost, [nel + 1], field="a", cost=lambda a: 1e-3 * (a - 1.) ** 2)
     return model
 
 

@@ -177,3 +177,46 @@ def test_space_frame_numbering_matches_oracle():
     got = np.concatenate([np.concatenate(E.dofid).ravel() for E in model.eletypes])
     assert np.array_equal(got, did)
     assert model.ndof["X"] == nd[0] == 6 * 36 + 6 * 12
+
+
+def test_dofgroup_golden():   # TestAssemble.jl:69-108
+    model = mm.Model("TestModel")
+    n1 = mm.addnode(model, [0, 0, 100])
+    n2 = mm.addnode(model, [0., 0, 0])
+    n3 = mm.addnode(model, [0., 0, 0])
+    mm.addelement(model, Turbine, [n1, n2])
+    mm.addelement(model, AnchorLine, [n1, n3])
+    dis = mm.Disassembler(model)
+    dofgr = mm.allXdofs(model, dis)
+    Λ, X, U, A = mm.indexedstate(dofgr)
+    nΛ, nX, nU, nA = mm.gradientstructure(dofgr, dis.dis[0])
+    iΛ, iX, iU, iA = mm.gradientpartition(nΛ, nX, nU, nA)
+    assert (dofgr.nX, dofgr.nU, dofgr.nA) == (3, 0, 4)
+    assert dofgr.iΛ.tolist() == [] and dofgr.iX.tolist() == [1, 2, 3] and dofgr.iU.tolist() == [] and dofgr.iA.tolist() == []
+    assert dofgr.jΛ == range(1, 1) and dofgr.jX == range(1, 4) and dofgr.jU == range(4, 4) and dofgr.jA == range(4, 4)
+    assert dofgr.scaleΛ.size == 0 and np.allclose(dofgr.scaleX, [1, 1, 1]) and dofgr.scaleU.size == 0
+    assert dofgr.scaleA.size == 0
+    assert dofgr.fieldΛ == [] and dofgr.fieldX == ["tx1", "tx2", "rx3"] and dofgr.fieldU == [] and dofgr.fieldA == []
+    assert Λ.tolist() == [0, 0, 0] and X.tolist() == [1, 2, 3] and U.tolist() == [] and A.tolist() == [0, 0, 0, 0]
+    assert (nΛ, nX, nU, nA) == (0, 2, 0, 0)
+    assert iΛ == range(1, 1) and iX == range(1, 3) and iU == range(3, 3) and iA == range(3, 3)
+
+
+def test_dofgroup_decrement_and_getdof():
+    """decrement!/getdof! (Assemble.jl:201-237) round trip on a scaled model, and the group the
+    SweepX loop decrements with (its scaled Newton increment)."""
+    model = mm.Model("TestModel")
+    n1 = mm.addnode(model, [0, 0, 100])
+    n2 = mm.addnode(model, [0., 0, 0])
+    n3 = mm.addnode(model, [0., 0, 0])
+    mm.addelement(model, Turbine, [n1, n2])
+    mm.addelement(model, AnchorLine, [n1, n3])
+    mm.setscale(model, scale={"X": {"tx1": 2., "rx3": 5.}, "A": {"ΔL": 3.}}, Λscale=7.)
+    st = mm.initialize(model)
+    gr = mm.allΛXUAdofs(model, st.dis)
+    y = np.arange(1., gr.getndof() + 1)
+    mm.decrement(st, 1, y, gr)
+    assert np.allclose(mm.getdofs(st, 0, gr), -y)
+    assert np.allclose(st.Λ[0], -y[:3] * np.array([2., 1., 5.]) * 7.)
+    assert np.allclose(st.X[0], -y[3:6] * np.array([2., 1., 5.]))
+    assert np.allclose(st.A, -y[6:] * np.array([1., 1., 3., 1.]))
