@@ -463,12 +463,28 @@ template <int NT, bool E> MH_HD void dyn_seed(const double (*X)[12], const doubl
 
 // Value pass: the frozen loads of the dynamic element, eg[gp] = dL_gp·fₑ(gp) and
 // mtot = Σ_gp dL_gp·mₑ, for beam_static<true>. X: ND rows of 12 element dofs.
+// kb (optional, stride ks): the ε-free kinematics that dyn_translation_direction reads, the
+// kDynBase numbers ∂ⁿrₛₘ(i,j)/∂tⁿ at n·9 + i + 3j (n < 3) and ẋ_gp,i at 27 + 3·gp + i.
+constexpr int kDynBase = 39;
 template <int ND> MH_HD void dyn_frozen_loads(const BeamGeo& G, const BeamDynSection& D, const double (*X)[12],
-                                              const double* U, v3* eg, v3& mtot) {
+                                              const double* U, v3* eg, v3& mtot, double* kb = nullptr,
+                                              int64_t ks = 1) {
     Jet<ND, false> Xj[12];
     dyn_seed<ND, false>(X, nullptr, nullptr, -1, Xj);
     DynKin<false> k;
     dyn_kinematics<ND, false>(G, Xj, k);
+    if (kb) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            kb[q * ks] = k.r0[q].v[0];
+            kb[(9 + q) * ks] = k.r1[q].v[0];
+            kb[(18 + q) * ks] = k.r2[q].v[0];
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) kb[(27 + 3 * g + i) * ks] = k.xd[g][i].v[0];
+    }
     Jet<1, false> fe[4][3], me[3];
     dyn_loads<false>(k, D, U, fe, me);
     mtot = mk(0., 0., 0.);
@@ -615,6 +631,68 @@ MH_HD void dyn_direction(const BeamGeo& G, const BeamDynSection& D, const double
     } else {
         o.Jv[0] = 0.; o.Jv[1] = 0.; o.Jv[2] = 0.;
     }
+}
+// The δX direction along a translation dof (dir 0..2, 6..8) in closed form, from the
+// kinematics jets k of any direction of the same element (only their ε-free parts are read).
+// Along a translation rₛₘ and vₛₘ carry no ε, so with the seed σ = (w0·s, a₁s, b₁s) on dof
+// component c of node 1 (η = −½) or node 2 (η = +½):
+//   x_gp,ε = (η·M_gp(:,c) + ½·e_c) ⊛ σ,   M_gp = rₛₘ·diag(ya, yu, yu)·rₛₘᵀ   (Leibniz products
+// of time jets), the loads vary through ẋ and ẍ alone, and the roll couple mₑ not at all:
+//   DFᵢ = μ·ẍᵢ,ε + Σⱼ r₀(i,j)·(Caⱼ·(r₀ᵀẍ_ε)ⱼ + (Clⱼ + 2Cqⱼ|(r₀ᵀẋ)ⱼ|)·(r₀ᵀẋ_ε)ⱼ).
+// The same numbers as dyn_direction<ND, false>(…, dir, …) up to rounding.
+template <int ND>
+MH_HD void dyn_translation_direction(const BeamGeo& G, const BeamDynSection& D, const double* kb, int64_t ks,
+                                     const double* scale, const double* c, int dir, DynDir& o, double w0 = 1.) {
+    const bool node2 = dir >= 6;
+    const int cc = node2 ? dir - 6 : dir;
+    const double eta = node2 ? 0.5 : -0.5;
+    const double s = scale[dir];
+    const double sg0 = w0 * s, sg1 = ND > 1 ? c[0] * s : 0., sg2 = ND > 2 ? c[3] * s : 0.;
+    // R[n][i + 3j]: ∂ⁿrₛₘ(i,j)/∂tⁿ
+    double R[3][9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) { R[0][q] = kb[q * ks]; R[1][q] = kb[(9 + q) * ks]; R[2][q] = kb[(18 + q) * ks]; }
+    // Wᵢⱼ = rₛₘ(i,j) ⊛ rₛₘ(c,j): the gp-independent products of M_gp(i,c) = Σⱼ Yⱼ Wᵢⱼ
+    double W[3][3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int a = i + 3 * j, b = cc + 3 * j;
+            W[i][j][0] = R[0][a] * R[0][b];
+            W[i][j][1] = R[1][a] * R[0][b] + R[0][a] * R[1][b];
+            W[i][j][2] = (R[2][a] * R[0][b] + 2. * (R[1][a] * R[1][b])) + R[0][a] * R[2][b];
+        }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const double z = G.zgp[g];
+        const double Y[3] = {2. * z, -4. * (z * z * z) + 3. * z, -4. * (z * z * z) + 3. * z};
+        double xd[3], xdd[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            double m[3];
+#pragma unroll
+            for (int n = 0; n < 3; ++n) m[n] = eta * ((Y[0] * W[i][0][n] + Y[1] * W[i][1][n]) + Y[2] * W[i][2][n]);
+            if (i == cc) m[0] = m[0] + 0.5;
+            o.Jx[g][i] = m[0] * sg0;
+            xd[i] = ND > 1 ? m[1] * sg0 + m[0] * sg1 : 0.;
+            xdd[i] = ND > 2 ? (m[2] * sg0 + 2. * (m[1] * sg1)) + m[0] * sg2 : 0.;
+        }
+        double fl[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double l1 = (kb[(27 + 3 * g) * ks] * R[0][3 * j] + kb[(28 + 3 * g) * ks] * R[0][3 * j + 1]) +
+                              kb[(29 + 3 * g) * ks] * R[0][3 * j + 2];
+            const double l1e = (xd[0] * R[0][3 * j] + xd[1] * R[0][3 * j + 1]) + xd[2] * R[0][3 * j + 2];
+            const double l2e = (xdd[0] * R[0][3 * j] + xdd[1] * R[0][3 * j + 1]) + xdd[2] * R[0][3 * j + 2];
+            fl[j] = D.Ca[j] * l2e + (D.Cl[j] + 2. * (D.Cq[j] * fabs(l1))) * l1e;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            o.DF[g][i] = D.mu * xdd[i] + ((R[0][i] * fl[0] + R[0][i + 3] * fl[1]) + R[0][i + 6] * fl[2]);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) { o.Dm[q] = 0.; o.Jv[q] = 0.; }
 }
 // Row i of the load-variation part of the tangent along direction d:
 // Σ_gp dL_gp·(∂x_gp/∂X₀ᵢ · DF_gp + ∂vₛₘ/∂X₀ᵢ · Dm)   (Jx_i, Jv_i: column i of the Jacobians,

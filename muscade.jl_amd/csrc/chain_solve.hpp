@@ -108,7 +108,9 @@ __device__ inline void lu_solve(const double (&a)[B][B], const int (&piv)[B], do
 }
 
 // The status words of a solve: flag (the k* bits) at word 0, then, as 64-bit values at
-// byte 8 and 16, the lowest block with a bad pivot and the lowest bad λ pair (all ones: none).
+// byte 8 and 16, the first block found with a bad pivot (the earliest level; a block
+// i = s + 2s·t of level s, or 0 for the last remaining block) and the first bad λ pair
+// (all ones: none).
 constexpr int kStatusBytes = 24;
 __device__ inline unsigned long long* status_where(unsigned* f) { return reinterpret_cast<unsigned long long*>(f + 2); }
 __global__ void clear_flag(unsigned* f) {
@@ -230,7 +232,7 @@ __global__ __launch_bounds__(kCB) void pair_kernel(int64_t np, const int32_t* __
     const double cL = pc[4 * p], cU = pc[4 * p + 1], dg = pc[4 * p + 2], rl = pc[4 * p + 3];
     if (dg != 0.0 || cL == 0.0 || cU == 0.0) {
         atomicOr(flag, kBadPair);
-        atomicMin(status_where(flag) + 1, (unsigned long long)p);
+        atomicCAS(status_where(flag) + 1, ~0ull, (unsigned long long)p);
     }
     const int64_t h = hpos[p], blk = h / B, i = h - blk * B;
     double* sv = saved + p * (3 * B + 1);
@@ -265,7 +267,7 @@ __global__ __launch_bounds__(kCB) void factor_kernel(int64_t nb, int64_t s, doub
         for (int r = 0; r < B; ++r) a[r][c] = Di[c * B + r];
     if (!lu_factor<B>(a, piv)) {
         atomicOr(flag, kSingular);
-        atomicMin(status_where(flag), (unsigned long long)i);
+        atomicCAS(status_where(flag), ~0ull, (unsigned long long)i);
     }
     double v[B];
     double* bi = b + i * B;
@@ -356,7 +358,7 @@ __global__ void top_kernel(double* __restrict__ D, double* __restrict__ b, unsig
     }
     if (!lu_factor<B>(a, piv)) {
         atomicOr(flag, kSingular);
-        atomicMin(status_where(flag), 0ull);
+        atomicCAS(status_where(flag), ~0ull, 0ull);
     }
     lu_solve<B>(a, piv, v);
 #pragma unroll

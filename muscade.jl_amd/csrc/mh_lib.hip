@@ -95,6 +95,7 @@ struct Ctx {
     bool coded = false;          // ksrc carries multiplier codes (compact groups present)
     int64_t nK = 0, nP = 0, nR = 0, nnz = 0;
     double* d_Kc = nullptr;      // dynamic beams: scaled compact static part, 45 per element
+    double* d_Kb = nullptr;      // dynamic beams (SweepX): ε-free kinematics, mh::kDynBase per element
     double* d_Kbuf = nullptr;
     double* d_Rbuf = nullptr;
     double* d_dRbuf = nullptr;
@@ -242,7 +243,7 @@ void free_prepared(Ctx* c) {
     free_graph(c);
     free_chain(c);
     free_big(c);
-    dfree(c->d_Kbuf); dfree(c->d_Rbuf); dfree(c->d_dRbuf); dfree(c->d_nzval); dfree(c->d_rhs); dfree(c->d_Kc);
+    dfree(c->d_Kbuf); dfree(c->d_Rbuf); dfree(c->d_dRbuf); dfree(c->d_nzval); dfree(c->d_rhs); dfree(c->d_Kc); dfree(c->d_Kb);
     dfree(c->d_colptr); dfree(c->d_rowval); dfree(c->d_kseg); dfree(c->d_ksrc); dfree(c->d_rseg); dfree(c->d_rsrc);
     dfree(c->d_useg); dfree(c->d_usrc); dfree(c->d_gbuf); dfree(c->d_gX); dfree(c->d_gubuf); dfree(c->d_gU);
     dfree(c->d_aseg); dfree(c->d_asrc); dfree(c->d_gabuf); dfree(c->d_gA); dfree(c->d_crhs);
@@ -431,6 +432,7 @@ struct BeamDynArgs {
     double* dR;         // ∂R/∂δr (:step)
     unsigned long long* flag;
     unsigned long long gtag;
+    double* Kb = nullptr;   // value pass → direction pass: ε-free kinematics, mh::kDynBase × nele (SoA)
 };
 
 template <int ND, bool UDOF>
@@ -473,7 +475,7 @@ __global__ __launch_bounds__(kBeamBlock) void beam_dyn_value_kernel(BeamDynArgs 
         double X[ND][12], U[3];
         dyn_load<ND, UDOF>(a, e, G, S, D, X, U);
         mh::v3 eg[4], mtot;
-        mh::dyn_frozen_loads<ND>(G, D, X, U, eg, mtot);
+        mh::dyn_frozen_loads<ND>(G, D, X, U, eg, mtot, a.Kb ? a.Kb + e : nullptr, n);
         mh::BeamOut out;
         mh::beam_static<true>(G, S, mh::mk(0., 0., 0.), X[0], out, eg, mtot);
         if (a.R) {
@@ -551,34 +553,34 @@ __global__ __launch_bounds__(kBlock) void beam_gauge_derivs_kernel(BeamDynArgs a
     }
 }
 
-// Direction pass: one lane per trial direction of an element (δX of dof k, k < 12; δr on
-// :step). Each lane propagates the time jets with its own ε seed; the lanes of an element
-// then exchange the Jacobian columns of x_gp and vₛₘ through LDS, and the lane of direction k
-// writes column k of K = (compact static part) + Σ_gp dL·(J_xᵀ ∂fₑ/∂ε + J_vᵀ ∂mₑ/∂ε).
-// A translation direction leaves the rotation chain free of ε (about half the work), so a
-// workgroup of kDirWElems elements runs them in a wave of their own: wave 0 = the 6
-// translation directions of each element, wave 1 = the 6 rotation directions and δr. Each
-// wave then follows one code path.
+// Direction pass, one wave per workgroup of kDirElemsW(STEP) elements, after the value pass
+// (which leaves the compact static part in Kc and the ε-free kinematics in Kb). One lane per
+// rotation dof of an element (δX of dofs 3..5, 9..11) and, on :step, one for δr. A lane first
+// evaluates the δX direction of one translation dof (0..2, 6..8) in closed form from Kb
+// (mh::dyn_translation_direction: along a translation the rotation chain carries no ε), then
+// propagates the time jets with its own ε seed. The lanes of an element exchange the Jacobian
+// columns of x_gp and vₛₘ (and the translation load derivatives) through LDS, and each lane
+// writes the columns k of its directions, K[:,k] = (compact static part) +
+// Σ_gp dL·(J_xᵀ ∂fₑ/∂ε + J_vᵀ ∂mₑ/∂ε), or ∂R/∂δr.
 constexpr int kDirLanes = 16;              // the adjoint and DirectXUA kernels: 16 lanes per element
 constexpr int kDirElems = 16;
 constexpr int kDirBlock = kDirLanes * kDirElems;
-constexpr int kDirWElems = 9;              // SweepX direction pass: elements per 2-wave workgroup
-constexpr int kDirWBlock = 128;
+constexpr int kDirWBlock = 64;             // SweepX direction pass: one wave
+template <bool STEP> constexpr int kDirPerW = STEP ? 7 : 6;             // lanes per element
+template <bool STEP> constexpr int kDirElemsW = kDirWBlock / kDirPerW<STEP>;   // 9 on :step, 10 on :iter
 template <int ND, bool UDOF, bool STEP>
 __global__ __launch_bounds__(kDirWBlock) void beam_dyn_dir_kernel(BeamDynArgs a) {
-    __shared__ double Jsh[kDirWElems][12][15];
-    const int64_t e0 = xcd_tile(a.per_xcd) * kDirWElems;
+    constexpr int per = kDirPerW<STEP>, NE = kDirElemsW<STEP>;
+    __shared__ double Jsh[NE][12][15];      // per dof: Jx (4 gp × 3), Jv (3)
+    __shared__ double DFt[NE][6][12];       // per translation dof: ∂fₑ/∂ε (4 gp × 3); ∂mₑ/∂ε = 0
+    const int64_t e0 = xcd_tile(a.per_xcd) * NE;
     if (e0 >= a.nele) return;   // whole workgroup leaves together
-    const bool rot = threadIdx.x >= 64;   // uniform within each wave
-    const int l = rot ? threadIdx.x - 64 : threadIdx.x;
-    const int per = rot ? 7 : 6;
-    const int el = l / per, j = l - el * per;
-    int dir;
-    if (!rot) dir = j < 3 ? j : j + 3;                          // t of node 1, t of node 2
-    else dir = j < 3 ? 3 + j : (j < 6 ? 6 + j : (STEP ? 12 : -1));   // r of node 1, r of node 2, δr
+    const int el = threadIdx.x / per, j = threadIdx.x - el * per;
+    const int dir = j < 3 ? 3 + j : (j < 6 ? 6 + j : 12);        // r of node 1, r of node 2, δr
+    const int dir_t = j < 3 ? j : j + 3;                          // t of node 1, t of node 2 (j < 6)
     const int64_t e = e0 + el;
     const int64_t n = a.nele;
-    const bool live = el < kDirWElems && e < n && dir >= 0;
+    const bool live = el < NE && e < n;
     mh::BeamGeo G;
     mh::DynDir d;
     if (live) {
@@ -586,8 +588,17 @@ __global__ __launch_bounds__(kDirWBlock) void beam_dyn_dir_kernel(BeamDynArgs a)
         mh::BeamDynSection D;
         double X[ND][12], U[3];
         dyn_load<ND, UDOF>(a, e, G, S, D, X, U);
-        if (rot) mh::dyn_direction<ND, true>(G, D, X, U, a.scale, a.c, dir, d);
-        else mh::dyn_direction<ND, false>(G, D, X, U, a.scale, a.c, dir, d);
+        if (j < 6) {
+            mh::DynDir dt;
+            mh::dyn_translation_direction<ND>(G, D, a.Kb + e, n, a.scale, a.c, dir_t, dt);
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) { Jsh[el][dir_t][3 * g + q] = dt.Jx[g][q]; DFt[el][j][3 * g + q] = dt.DF[g][q]; }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) Jsh[el][dir_t][12 + q] = 0.;
+        }
+        mh::dyn_direction<ND, true>(G, D, X, U, a.scale, a.c, dir, d);
         if (dir < 12) {
 #pragma unroll
             for (int g = 0; g < 4; ++g)
@@ -599,22 +610,33 @@ __global__ __launch_bounds__(kDirWBlock) void beam_dyn_dir_kernel(BeamDynArgs a)
     }
     __syncthreads();
     if (!live) return;
-    double col[12];
+    double col[12], colt[12];
     int nan = 0;
 #pragma unroll
     for (int i = 0; i < 12; ++i) {
-        col[i] = mh::dyn_row(G, (const double(*)[3])(&Jsh[el][i][0]), &Jsh[el][i][12], d);
+        const double(*Jx)[3] = (const double(*)[3])(&Jsh[el][i][0]);
+        col[i] = mh::dyn_row(G, Jx, &Jsh[el][i][12], d);
         if (dir < 12) {
+            double t = 0.;
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                t = t + ((Jx[g][0] * DFt[el][j][3 * g] + Jx[g][1] * DFt[el][j][3 * g + 1]) +
+                         Jx[g][2] * DFt[el][j][3 * g + 2]) * G.dL[g];
             double m;
-            const int slot = mh::compact_slot(i, dir, &m);
+            int slot = mh::compact_slot(i, dir, &m);
             col[i] = m * a.Kc[e * mh::kCompact + slot] + col[i];
+            slot = mh::compact_slot(i, dir_t, &m);
+            colt[i] = m * a.Kc[e * mh::kCompact + slot] + t;
+            nan |= colt[i] != colt[i];
         }
         nan |= col[i] != col[i];
     }
     if (dir < 12) {
-        double* dst = a.K + e * 144 + 12 * dir;
+        double* dst = a.K + e * 144;
 #pragma unroll
-        for (int i = 0; i < 12; ++i) dst[i] = col[i];
+        for (int i = 0; i < 12; ++i) dst[12 * dir + i] = col[i];
+#pragma unroll
+        for (int i = 0; i < 12; ++i) dst[12 * dir_t + i] = colt[i];
     } else {
 #pragma unroll
         for (int i = 0; i < 12; ++i) a.dR[i * n + e] = col[i];
@@ -1347,7 +1369,8 @@ void launch_dyn(BeamDynArgs& a, bool step, hipStream_t s) {
         a.per_xcd = (ntile + kXCD - 1) / kXCD;
         hipLaunchKernelGGL((beam_dyn_value_kernel<ND, UDOF>), dim3((unsigned)(a.per_xcd * kXCD)), dim3(kBeamBlock), 0, s, a);
     }
-    const int64_t ntile = (a.nele + kDirWElems - 1) / kDirWElems;
+    const int NE = step ? kDirElemsW<true> : kDirElemsW<false>;
+    const int64_t ntile = (a.nele + NE - 1) / NE;
     a.per_xcd = (ntile + kXCD - 1) / kXCD;
     const dim3 gr((unsigned)(a.per_xcd * kXCD));
     if (step) hipLaunchKernelGGL((beam_dyn_dir_kernel<ND, UDOF, true>), gr, dim3(kDirWBlock), 0, s, a);
@@ -1367,6 +1390,7 @@ int launch_dyn_beams(Ctx* c, const Group& g, int OX, bool step, const double* c6
     for (int i = 0; i < 12; ++i) { a.sR[i] = b.sR[i]; a.scale[i] = g.scaleX[i]; }
     for (int i = 0; i < 6; ++i) a.c[i] = c6[i];
     a.Kc = c->d_Kc + g.kcbase; a.K = c->d_Kbuf + g.kbase; a.R = c->d_Rbuf + g.rbase;
+    a.Kb = c->d_Kb + g.kcbase / mh::kCompact * mh::kDynBase;
     a.dR = step ? c->d_dRbuf + g.rbase : nullptr;
     a.flag = c->d_flag; a.gtag = gtag;
     const bool u = g.kind == MH_KIND_BEAM_U;
@@ -1923,6 +1947,7 @@ int mh_prepare_sweepx(mh_ctx* c, int OX, int64_t* nnz_out) {
     HIPCHK(c, hipMalloc(&c->d_Rbuf, sizeof(double) * std::max<int64_t>(nR, 1)));
     HIPCHK(c, hipMalloc(&c->d_dRbuf, sizeof(double) * std::max<int64_t>(nR, 1)));
     if (nKc) HIPCHK(c, hipMalloc(&c->d_Kc, sizeof(double) * nKc));
+    if (nKc) HIPCHK(c, hipMalloc(&c->d_Kb, sizeof(double) * (nKc / mh::kCompact * mh::kDynBase)));
     HIPCHK(c, hipMalloc(&c->d_rhs, sizeof(double) * std::max<int64_t>(nX, 1)));
     HIPCHK(c, hipMalloc(&c->d_colptr, sizeof(int32_t) * (nX + 1)));
     // ---- matrix pairs
@@ -2785,7 +2810,9 @@ static int chain_status(Ctx* c) {
         return MH_ERR_SOLVE;
     }
     if (f & chain::kSingular) {
-        c->err = "zero or non-finite pivot in diagonal block " + std::to_string(where[0]) + " (0-based, chain order)";
+        const unsigned long long b = where[0];
+        c->err = "zero or non-finite pivot in diagonal block " + std::to_string(b) + " (0-based, chain order; " +
+                 (b ? "reduction level " + std::to_string(b & (~b + 1)) : std::string("the last block")) + ")";
         return MH_ERR_SOLVE;
     }
     return MH_OK;

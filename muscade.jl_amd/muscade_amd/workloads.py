@@ -259,49 +259,81 @@ def design_point_specs(nmodel=512, nmin=500, nmax=20000, seed=1005):
     return [dict(ne=int(ne[m]), seed=int(seed * 1000 + m)) for m in range(nmodel)]
 
 
-def _add_cantilever_a(model, spec, L=10., yoff=0.):
-    """One cantilever of the design-point batch; returns (first node, tip node)."""
+def riser_section():
+    """Section of the design-point risers: the catenary chains' stiffnesses, no weight."""
+    return BeamCrossSection(EA=5e9, EI2=2e7, EI3=2e7, GJ=1.5e7, mu=1e3 / 9.81, iota1=0.5)
+
+
+def riser_tension(t, T=1e6):
+    """Top tension of a design-point riser (a dead vertical load)."""
+    return T
+
+
+def _add_riser_a(model, spec, h=0.1, xoff=0.):
+    """One riser of the design-point batch: a vertical EulerBeam3DAU cantilever of ne elements
+    of h, clamped at the bottom (first node), the section's EA, EI₂, EI₃, GJ scaled per
+    sixteenth by 10^a, 8 segments of U-loads (per length, global axes). Returns (bottom node,
+    top node)."""
     n = spec["ne"]
-    x = np.arange(n + 1) * (L / n)
+    L = n * h
+    z = np.arange(n + 1) * h
     base = model.nnod
-    xyz = np.stack([x, np.full_like(x, yoff), np.zeros_like(x)], axis=1)
+    xyz = np.stack([np.full_like(z, xoff), np.zeros_like(z), z], axis=1)
     useg = np.minimum((np.arange(n) * 8) // n, 7)
     aseg = np.minimum((np.arange(n) * 16) // n, 15)
-    uxyz = np.stack([(np.arange(8) + 0.5) * L / 8, np.full(8, yoff + 1.), np.zeros(8)], axis=1)
-    axyz = np.stack([(np.arange(16) + 0.5) * L / 16, np.full(16, yoff + 2.), np.zeros(16)], axis=1)
+    uxyz = np.stack([np.full(8, xoff + 1.), np.zeros(8), (np.arange(8) + 0.5) * L / 8], axis=1)
+    axyz = np.stack([np.full(16, xoff + 2.), np.zeros(16), (np.arange(16) + 0.5) * L / 16], axis=1)
     addnode(model, np.concatenate([xyz, uxyz, axyz]))
     nodes = np.stack([base + 1 + np.arange(n), base + 2 + np.arange(n), base + n + 2 + useg, base + n + 10 + aseg],
                      axis=1)
-    addelement(model, EulerBeam3DAU, nodes, mat=base_section(), orient2=np.array([0., 1., 0.]),
+    addelement(model, EulerBeam3DAU, nodes, mat=riser_section(), orient2=np.array([0., 1., 0.]),
                scaled=STIFFNESS_FIELDS, afield="Ξk")
     return base + 1, base + n + 1
 
 
-def _design_vectors(spec, q0):
-    """U (8 U-nodes × t1, t2, t3) and A (16) of one cantilever."""
+def _design_vectors(spec, T, h=0.1, drift=0.02):
+    """U (8 U-nodes × t1, t2, t3) and A (16) of one riser: a lateral current along x,
+    u ~ N(1, 0.2)·q with q = 2·drift·T/L (the taut-string tip drift q·L/(2T) = drift·L), small
+    loads along y and z (N(0, 0.1)·q); a ~ N(0, 0.1/ln 10), so that 10^a is lognormal with σ = 0.1."""
     rng = np.random.default_rng(spec["seed"])
+    q = 2. * drift * T / (spec["ne"] * h)
     u = np.zeros((8, 3))
-    u[:, :2] = rng.normal(0., 0.1, (8, 2)) * q0
-    u[:, 2] = rng.normal(1., 0.2, 8) * q0
+    u[:, 0] = rng.normal(1., 0.2, 8) * q
+    u[:, 1:] = rng.normal(0., 0.1, (8, 2)) * q
     a = rng.normal(0., 0.1 / np.log(10.), 16)
     return u.ravel(), a
 
 
-def design_point_batch(specs, joined=True, q0=1e3):
-    """The batch as one Model (joined) or one Model per spec. Returns a list of (model, state,
-    tip) with the state's U and A set and tip the 1-based X-dofs of the tip displacements
-    along z (one per cantilever)."""
+def design_point_batch(specs, joined=True, T=1e6):
+    """The batch as one Model (joined) or one Model per spec. Each model is a tensioned riser:
+    a vertical cantilever clamped at the seabed with the dead top tension T (DofLoad along z),
+    so that its lateral stiffness comes mostly from the tension, as for a production riser.
+    (A horizontal 10 m cantilever cut into 20k elements is not solvable in FP64: the
+    condition of its bending stiffness grows as ne⁴.) Returns a list of (model, state, tip)
+    with the state's U and A set and tip the 1-based X-dofs of the lateral top displacements
+    (t1, one per riser)."""
     def build(sp_list, name):
         model = Model(name)
-        ends = [_add_cantilever_a(model, s_, yoff=3. * m) for m, s_ in enumerate(sp_list)]
+        ends = [_add_riser_a(model, s_, xoff=10. * m) for m, s_ in enumerate(sp_list)]
         for fld in ("t1", "t2", "t3", "r1", "r2", "r3"):
             addelement(model, Hold, np.array([f for f, _ in ends]).reshape(-1, 1), field=fld)
+        addelement(model, DofLoad, np.array([t for _, t in ends]).reshape(-1, 1), field="t3", value=riser_tension,
+                   T=T)
         state = initialize(model, time=0.)
+        # start from the taut riser (axial strain T/EA), so that the first Newton iteration
+        # already has the tension's lateral stiffness
+        EA = riser_section().values[0]
         for m, s_ in enumerate(sp_list):
-            u, a = _design_vectors(s_, q0)
+            u, a = _design_vectors(s_, T)
             state.U[0][24 * m:24 * m + 24] = u
             state.A[16 * m:16 * m + 16] = a
-        tip = np.array([model.getdofID("X", "t3", nodID=[t])[0] for _, t in ends])
+        dn = model.dofnode("X")
+        t3 = model.getdofID("X", "t3")
+        state.X[0][t3 - 1] = model.coords()[dn[t3 - 1], 2] * (T / EA)
+        t1 = model.getdofID("X", "t1")
+        at = np.full(model.nnod, -1, np.int64)
+        at[dn[t1 - 1]] = t1
+        tip = at[np.array([t for _, t in ends]) - 1]
         return model, state, tip
     if joined:
         return [build(specs, "design_point_batch")]
