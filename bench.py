@@ -32,6 +32,11 @@ FP64_PEAK = 78.6e12        # MI355X FP64 vector, FLOP/s
 # from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters of one assembly of this
 # workload divided by the number of beams (profiles/space_frame_100x100x33_pmc.md).
 F_ELEM = 6299
+# The same for the dynamic beam of the catenary chains (OX = 2): value pass + direction pass
+# per beam, and their HBM bytes per assembly from FETCH_SIZE/WRITE_SIZE
+# (profiles/catenary_chains_256_kernels.md), by mission.
+F_DYN = {"iter": 10565 + 69935, "step": 10565 + 78130}
+DYN_TRAFFIC = {"iter": 13.37e9 + 28.57e9, "step": 13.37e9 + 32.53e9}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "space_frame_100x100x33_traffic.json")
 
 
@@ -189,6 +194,13 @@ def dynamic_bench(args):
         el = time.perf_counter() - t0
         res[mission] = {"elements_per_s": nbeam * nst / el, "ms_per_step": el / nst * 1e3,
                         "kernels_ms": dict(zip(("eval", "scatter_nz", "scatter_rhs"), (acc / nst).tolist()))}
+        t_eval = acc[0] / nst * 1e-3
+        if t_eval > 0:
+            ach = F_DYN[mission] * nbeam / t_eval / 1e12
+            res[mission]["roofline_eval"] = {
+                "bound": "fp64", "achieved": ach, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                "frac": ach / (FP64_PEAK / 1e12),
+                "traffic": DYN_TRAFFIC[mission] if args.chains == 256 else None}
         log(f"dynamic {mission}: {el / nst * 1e3:.3f} ms/step, kernels {acc / nst}")
     if not args.no_chain_solve:
         res["newton_device"] = device_newton_iteration(args, out, state, c6)
@@ -415,15 +427,15 @@ def catenary_motion_bench(args):
 def design_point_bench(args):
     """Batched design-point search (the limit-state gradients of many models): `--models`
     tensioned risers, vertical cantilevers clamped at the bottom with a 1 MN dead top tension
-    (U{model_min..model_max} EulerBeam3DAU elements of 0.1 m, 16 stiffness parameters a as
+    and the top held in twist (U{model_min..model_max} EulerBeam3DAU elements of 0.1 m, 16 stiffness parameters a as
     10^a on EA, EI₂, EI₃, GJ per sixteenth, 8 segments of U-loads each), at their static
-    equilibrium (SweepX{0} with the device chain solve, timed apart). A gradient set is dg/dA
+    equilibrium (SweepX{0} with the device chain solve in 4 load steps, timed apart). A gradient set is dg/dA
     (16) and dg/dU (24) of one model's limit state g = lateral top displacement: assembly of
     the tangent at X*, the adjoint chain solve K·μ = ∂g/∂X and the adjoint launch with Λ = −μ.
     Batched: all models as one model (one of each launch). Plain: one context per model, in a
     loop, at the same equilibrium."""
     import muscade_amd as mc
-    from muscade_amd.workloads import design_point_batch, design_point_specs
+    from muscade_amd.workloads import design_point_batch, design_point_equilibrium, design_point_specs
     specs = design_point_specs(args.models, args.model_min, args.model_max)
     t0 = time.perf_counter()
     [(mj, sj, tipj)] = design_point_batch(specs)
@@ -432,46 +444,52 @@ def design_point_bench(args):
     log(f"design point: {len(specs)} models, {nbeam} beams ({time.perf_counter() - t0:.1f}s)")
     st = {}
     t0 = time.perf_counter()
-    eq = mc.solve(mc.SweepX[0], initialstate=sj, time=[1.], maxΔx=1e-6, verbose=False, catcherror=False,
-                  linsolve="device", stats=st)[-1]
+    eq = design_point_equilibrium(sj, stats=st)
     t_eq = time.perf_counter() - t0
     log(f"design point: equilibrium {t_eq:.2f}s ({st.get('iterations')} Newton iterations)")
-    # the plain models at the same equilibrium: node dofs model after model, then the six λ fields
+    # the plain models at the same equilibrium: node dofs model after model, then the λ fields
+    # (one λ of each Hold per model, field after field)
     nodal = np.cumsum([0] + [6 * (s["ne"] + 1) for s in specs])
     nm = len(specs)
+    nlam = (mj.ndof["X"] - nodal[-1]) // nm
     plain = []
     for m, (mm, sm, tm) in enumerate(sep):
         s_ = sm.copy()
         s_.X[0][:6 * (specs[m]["ne"] + 1)] = eq.X[0][nodal[m]:nodal[m + 1]]
-        s_.X[0][6 * (specs[m]["ne"] + 1):] = eq.X[0][nodal[-1] + m + nm * np.arange(6)]
-        d_ = np.zeros(mm.ndof["X"])
-        d_[tm - 1] = 1.
-        plain.append((mc.LimitStateGradient(mm, s_.dis), s_, d_))
+        s_.X[0][6 * (specs[m]["ne"] + 1):] = eq.X[0][nodal[-1] + m + nm * np.arange(nlam)]
+        plain.append((mc.LimitStateGradient(mm, s_.dis), s_, (np.asarray(tm), np.ones(1))))
     dgj = np.zeros(mj.ndof["X"])
     dgj[tipj - 1] = 1.
+    sparse_j = (tipj, np.ones(tipj.size))       # ∂g/∂X by its nonzeros: one per model
     lsg = mc.LimitStateGradient(mj, eq.dis)
     res = {}
 
-    def batched():
-        return lsg(eq, dgj)
+    # timed with the equilibrium state resident on the device (uploaded by the warm-up call),
+    # and once more with the state's upload (X, U, A from the host) in each call
+    def batched(upload=False):
+        return lsg(eq, sparse_j, upload=upload, check=False)
 
-    def per_model():
-        return [f(s_, d_) for f, s_, d_ in plain]
+    def per_model(upload=False):
+        return [f(s_, d_, upload=upload, check=False) for f, s_, d_ in plain]
 
     for name, fn in (("batched", batched), ("plain", per_model)):
-        for _ in range(max(1, args.warmup)):
+        fn(upload=True)
+        for _ in range(max(0, args.warmup - 1)):
             fn()
         n = max(2, args.steps // 4)
         t0 = time.perf_counter()
         for _ in range(n):
             fn()
         el = (time.perf_counter() - t0) / n
-        res[name] = {"grads_per_s": nm / el, "ms_per_batch": el * 1e3}
-        log(f"design point {name}: {el * 1e3:.2f} ms for {nm} gradient sets")
-    gA, gU, gX = batched()
+        t0 = time.perf_counter()
+        fn(upload=True)
+        elu = time.perf_counter() - t0
+        res[name] = {"grads_per_s": nm / el, "ms_per_batch": el * 1e3, "ms_per_batch_with_state_upload": elu * 1e3}
+        log(f"design point {name}: {el * 1e3:.2f} ms for {nm} gradient sets ({elu * 1e3:.2f} ms with the state upload)")
+    gA, gU, gX = lsg(eq, dgj)
     worst, worst_x = 0., float(np.abs(gX + dgj).max())
     for m in range(min(8, nm)):
-        a1, u1, _ = plain[m][0](plain[m][1], plain[m][2])
+        a1, u1, _ = plain[m][0](plain[m][1], plain[m][2], upload=True, check=False)
         worst = max(worst, float(np.abs(gA[16 * m:16 * m + 16] - a1).max() / np.abs(a1).max()),
                     float(np.abs(gU[24 * m:24 * m + 24] - u1).max() / np.abs(u1).max()))
     cpu = None
@@ -581,7 +599,9 @@ def directxua_bench(args):
                   mat=chain_section(), orient2=np.array([0., 1., 0.]))
     for f in ("t1", "t2", "t3", "r1", "r2", "r3"):
         mc.addelement(model, mc.Hold, [1], field=f)
-    mc.addelement(model, mc.SingleUdof, [nel + 1], Xfield="t2", Ufield="ut2", cost=lambda u, t: 1e-10 * u * u)
+    # the load prior's Hessian (2e-6) must stay above DirectXUA's sparser! threshold, 1e-20 of the
+    # largest entry of the all-steps matrix (about 3e10 here), or the U block loses its diagonal
+    mc.addelement(model, mc.SingleUdof, [nel + 1], Xfield="t2", Ufield="ut2", cost=lambda u, t: 1e-6 * u * u)
     mc.addelement(model, mc.SingleDofCost, [nel + 1], dofclass="X", field="t2",
                   cost=lambda x, t: 1e3 * (x - 1e-2 * np.sin(t)) ** 2)
     st0 = mc.initialize(model, time=0.)

@@ -259,6 +259,9 @@ int mh_get_X(mh_ctx* ctx, int ider, double* x);
  * ready for mh_vjp_lagrangian (alpha = −1 gives dg/dA = Λᵀ∂R/∂A, dg/dU = Λᵀ∂R/∂U, since
  * R(X*(A,U),A,U) = 0). Needs mh_chain_prepare and mh_newton_prepare (the dof scales). */
 int mh_adjoint_solve(mh_ctx* ctx, const double* rhs, double alpha);
+/* The same with ∂g/∂X given by its n nonzeros (idx: 1-based model X-dofs, repeated dofs
+ * summed; val: their values): a limit state reads few dofs, and nothing of size nX crosses PCIe. */
+int mh_adjoint_solve_sparse(mh_ctx* ctx, int64_t n, const int64_t* idx, const double* val, double alpha);
 
 /* ---- DirectXUA's all-steps system on the device (src/DirectXUA.jl:245-383:
  *      makepattern / preparebig / assemblebig!, with SparseTools.prepare / addin!,

@@ -125,6 +125,9 @@ struct Ctx {
     double* d_gabuf = nullptr;   // nRA
     double* d_gA = nullptr;      // nA
     double* d_crhs = nullptr;    // nX: right-hand side of an adjoint chain solve
+    int64_t* d_spi = nullptr;    // sparse right-hand side: 0-based dofs and values (sp_cap each)
+    double* d_spv = nullptr;
+    int64_t sp_cap = 0;
     int vjp_nd = 0;              // derivative orders of the last VJP
     // AssemblyDirect{OX,OU,IA} (DirectXUA, one step)
     bool direct = false;
@@ -247,6 +250,7 @@ void free_prepared(Ctx* c) {
     dfree(c->d_colptr); dfree(c->d_rowval); dfree(c->d_kseg); dfree(c->d_ksrc); dfree(c->d_rseg); dfree(c->d_rsrc);
     dfree(c->d_useg); dfree(c->d_usrc); dfree(c->d_gbuf); dfree(c->d_gX); dfree(c->d_gubuf); dfree(c->d_gU);
     dfree(c->d_aseg); dfree(c->d_asrc); dfree(c->d_gabuf); dfree(c->d_gA); dfree(c->d_crhs);
+    dfree(c->d_spi); dfree(c->d_spv); c->sp_cap = 0;
     c->nRA = 0;
     c->vjp_nd = 0;
     for (DMap* M : {&c->dXX, &c->dXU, &c->dUX, &c->dUU}) {
@@ -1239,6 +1243,11 @@ __global__ __launch_bounds__(kBlock) void scaled_copy_kernel(int64_t n, double a
 }
 
 // Plain streaming copy of n doubles: the reference kernel for HBM counter calibration.
+__global__ __launch_bounds__(kBlock) void sparse_set_kernel(int64_t n, const int64_t* __restrict__ idx,
+                                                            const double* __restrict__ val, double* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[idx[i]] = val[i];
+}
 __global__ __launch_bounds__(kBlock) void copy_kernel(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = a[i];
 }
@@ -1507,6 +1516,21 @@ int apply_ascale(Ctx* c) {
 // ============================================================================ C ABI
 struct mh_ctx : Ctx {};
 
+// A copy between host and device memory (or device to device) in the context's stream
+// order, returning when it is complete. The context's stream is non-blocking, so copies on
+// the null stream (hipMemcpy) would not be ordered with its kernels.
+static hipError_t copy_sync(Ctx* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+// Zero device memory at set-up, in the context's stream order (never inside a captured graph).
+static hipError_t zero_sync(Ctx* c, void* dst, size_t bytes) {
+    const hipError_t e = hipMemsetAsync(dst, 0, bytes, c->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
 static int upload_idx(Ctx* c, int32_t** dst, const int64_t* src, int n, int64_t nele, int64_t ndof, bool allow_zero) {
     std::vector<int32_t> h((size_t)n * nele);
     for (int64_t e = 0; e < nele; ++e)
@@ -1516,7 +1540,7 @@ static int upload_idx(Ctx* c, int32_t** dst, const int64_t* src, int n, int64_t 
             h[(size_t)k * nele + e] = (int32_t)(v - 1);
         }
     HIPCHK(c, hipMalloc(dst, sizeof(int32_t) * std::max<size_t>(h.size(), 1)));
-    if (!h.empty()) HIPCHK(c, hipMemcpy(*dst, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
+    if (!h.empty()) HIPCHK(c, copy_sync(c, *dst, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
     return MH_OK;
 }
 template <class T>
@@ -1525,7 +1549,7 @@ static int upload_soa(Ctx* c, T** dst, const T* src, int n, int64_t nele) {
     for (int64_t e = 0; e < nele; ++e)
         for (int k = 0; k < n; ++k) h[(size_t)k * nele + e] = src[e * n + k];
     HIPCHK(c, hipMalloc(dst, sizeof(T) * std::max<size_t>(h.size(), 1)));
-    if (!h.empty()) HIPCHK(c, hipMemcpy(*dst, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    if (!h.empty()) HIPCHK(c, copy_sync(c, *dst, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
     return MH_OK;
 }
 
@@ -1808,16 +1832,16 @@ int mh_model_begin(mh_ctx* c, int64_t nX, int64_t nU, int64_t nA, int64_t nnod, 
     c->nX = nX; c->nU = nU; c->nA = nA; c->nnod = nnod;
     if (nnod) {
         HIPCHK(c, hipMalloc(&c->d_xyz, sizeof(double) * 3 * nnod));
-        HIPCHK(c, hipMemcpy(c->d_xyz, xyz, sizeof(double) * 3 * nnod, hipMemcpyHostToDevice));
+        HIPCHK(c, copy_sync(c, c->d_xyz, xyz, sizeof(double) * 3 * nnod, hipMemcpyHostToDevice));
     }
     for (int d = 0; d < 3; ++d) {
         HIPCHK(c, hipMalloc(&c->d_X[d], sizeof(double) * std::max<int64_t>(nX, 1)));
-        HIPCHK(c, hipMemset(c->d_X[d], 0, sizeof(double) * std::max<int64_t>(nX, 1)));
+        HIPCHK(c, zero_sync(c, c->d_X[d], sizeof(double) * std::max<int64_t>(nX, 1)));
     }
     HIPCHK(c, hipMalloc(&c->d_U, sizeof(double) * std::max<int64_t>(nU, 1)));
-    HIPCHK(c, hipMemset(c->d_U, 0, sizeof(double) * std::max<int64_t>(nU, 1)));
+    HIPCHK(c, zero_sync(c, c->d_U, sizeof(double) * std::max<int64_t>(nU, 1)));
     HIPCHK(c, hipMalloc(&c->d_A, sizeof(double) * std::max<int64_t>(nA, 1)));
-    HIPCHK(c, hipMemset(c->d_A, 0, sizeof(double) * std::max<int64_t>(nA, 1)));
+    HIPCHK(c, zero_sync(c, c->d_A, sizeof(double) * std::max<int64_t>(nA, 1)));
     return MH_OK;
 }
 
@@ -1871,7 +1895,7 @@ int mh_group_add(mh_ctx* c, int kind, int64_t nele, int nX, const int64_t* idxX,
             }
         // orient2 check of the constructor (BeamElement.jl:132-135), on the host
         std::vector<double> xyz((size_t)3 * c->nnod);
-        if (c->nnod) HIPCHK(c, hipMemcpy(xyz.data(), c->d_xyz, sizeof(double) * xyz.size(), hipMemcpyDeviceToHost));
+        if (c->nnod) HIPCHK(c, copy_sync(c, xyz.data(), c->d_xyz, sizeof(double) * xyz.size(), hipMemcpyDeviceToHost));
         for (int64_t e = 0; e < nele; ++e) {
             const double* a = &xyz[3 * nd[e * 2]];
             const double* b = &xyz[3 * nd[e * 2 + 1]];
@@ -1887,7 +1911,7 @@ int mh_group_add(mh_ctx* c, int kind, int64_t nele, int nX, const int64_t* idxX,
     }
     if (kind == MH_KIND_DOFLOAD || kind == MH_KIND_HOLD) {   // DofLoad values, prescribed-motion gaps
         HIPCHK(c, hipMalloc(&g.d_vals, sizeof(double) * std::max<int64_t>(nele, 1)));
-        HIPCHK(c, hipMemset(g.d_vals, 0, sizeof(double) * std::max<int64_t>(nele, 1)));
+        HIPCHK(c, zero_sync(c, g.d_vals, sizeof(double) * std::max<int64_t>(nele, 1)));
     }
     c->groups.push_back(std::move(g));
     if (igroup) *igroup = (int)c->groups.size() - 1;
@@ -1904,7 +1928,7 @@ int mh_group_set_adofs(mh_ctx* c, int igroup, const int64_t* idxA, int mask) {
     free_prepared(c);
     if (int r = upload_idx(c, &g.d_aidx, idxA, 1, g.nele, c->nA, false)) return r;
     HIPCHK(c, hipMalloc(&g.d_sec_base, sizeof(double) * std::max<int64_t>(16 * g.nele, 1)));
-    if (g.nele) HIPCHK(c, hipMemcpy(g.d_sec_base, g.d_sec, sizeof(double) * 16 * g.nele, hipMemcpyDeviceToDevice));
+    if (g.nele) HIPCHK(c, copy_sync(c, g.d_sec_base, g.d_sec, sizeof(double) * 16 * g.nele, hipMemcpyDeviceToDevice));
     g.amask = mask;
     if (int r = apply_ascale(c)) return r;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -2082,8 +2106,8 @@ int mh_get_pattern(mh_ctx* c, int64_t* colptr, int64_t* rowval) {
     if (!c || !c->prepared) return MH_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     std::vector<int32_t> cp(c->nX + 1), rv(c->nnz);
-    HIPCHK(c, hipMemcpy(cp.data(), c->d_colptr, sizeof(int32_t) * cp.size(), hipMemcpyDeviceToHost));
-    if (c->nnz) HIPCHK(c, hipMemcpy(rv.data(), c->d_rowval, sizeof(int32_t) * rv.size(), hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c, cp.data(), c->d_colptr, sizeof(int32_t) * cp.size(), hipMemcpyDeviceToHost));
+    if (c->nnz) HIPCHK(c, copy_sync(c, rv.data(), c->d_rowval, sizeof(int32_t) * rv.size(), hipMemcpyDeviceToHost));
     if (colptr) for (size_t i = 0; i < cp.size(); ++i) colptr[i] = (int64_t)cp[i] + 1;
     if (rowval) for (size_t i = 0; i < rv.size(); ++i) rowval[i] = (int64_t)rv[i] + 1;
     return MH_OK;
@@ -2555,8 +2579,8 @@ int mh_get_direct_pattern(mh_ctx* c, int which, int64_t* colptr, int64_t* rowval
     HIPCHK(c, hipSetDevice(c->device));
     const DMap& M = which == 0 ? c->dXX : which == 1 ? c->dXU : which == 2 ? c->dUX : c->dUU;
     std::vector<int32_t> cp(M.ncol + 1, 0), rv(M.nnz);
-    if (M.colptr) HIPCHK(c, hipMemcpy(cp.data(), M.colptr, sizeof(int32_t) * cp.size(), hipMemcpyDeviceToHost));
-    if (M.nnz) HIPCHK(c, hipMemcpy(rv.data(), M.rowval, sizeof(int32_t) * rv.size(), hipMemcpyDeviceToHost));
+    if (M.colptr) HIPCHK(c, copy_sync(c, cp.data(), M.colptr, sizeof(int32_t) * cp.size(), hipMemcpyDeviceToHost));
+    if (M.nnz) HIPCHK(c, copy_sync(c, rv.data(), M.rowval, sizeof(int32_t) * rv.size(), hipMemcpyDeviceToHost));
     if (colptr) for (size_t i = 0; i < cp.size(); ++i) colptr[i] = (int64_t)cp[i] + 1;
     if (rowval) for (size_t i = 0; i < rv.size(); ++i) rowval[i] = (int64_t)rv[i] + 1;
     return MH_OK;
@@ -2628,8 +2652,8 @@ int mh_get_direct(mh_ctx* c, double* L1, double* L2) {
         const int64_t n1 = nX + (int64_t)ndX * nX + (int64_t)ndU * nU;
         std::fill(L1, L1 + n1, 0.);
         if (nX) {
-            HIPCHK(c, hipMemcpy(L1, c->d_dL1, sizeof(double) * nX, hipMemcpyDeviceToHost));
-            HIPCHK(c, hipMemcpy(L1 + nX, c->d_dL1 + nX, sizeof(double) * nX, hipMemcpyDeviceToHost));
+            HIPCHK(c, copy_sync(c, L1, c->d_dL1, sizeof(double) * nX, hipMemcpyDeviceToHost));
+            HIPCHK(c, copy_sync(c, L1 + nX, c->d_dL1 + nX, sizeof(double) * nX, hipMemcpyDeviceToHost));
         }
     }
     if (L2) {
@@ -2649,7 +2673,7 @@ int mh_get_direct(mh_ctx* c, double* L1, double* L2) {
                         if (a == 1 && b == 0 && j == 0) src = c->d_dL2 + (ndX + i) * nxx;
                         if (a == 0 && b == 2 && i == 0 && j == 0) src = c->d_dL2 + 2 * ndX * nxx;
                         if (a == 2 && b == 0 && i == 0 && j == 0) src = c->d_dL2 + 2 * ndX * nxx + nxu;
-                        if (src && blk) HIPCHK(c, hipMemcpy(dst, src, sizeof(double) * blk, hipMemcpyDeviceToHost));
+                        if (src && blk) HIPCHK(c, copy_sync(c, dst, src, sizeof(double) * blk, hipMemcpyDeviceToHost));
                         else std::fill(dst, dst + blk, 0.);
                     }
                 off += (int64_t)nder[a] * nder[b] * blk;
@@ -2786,9 +2810,9 @@ int mh_chain_prepare(mh_ctx* c, int B, int64_t nblock, const int64_t* pos, int64
     HIPCHK(c, hipMalloc(&c->d_cflag, chain::kStatusBytes));
     HIPCHK(c, hipHostMalloc((void**)&c->h_cflag, chain::kStatusBytes, hipHostMallocDefault));
     for (auto& e : c->cev) HIPCHK(c, hipEventCreate(&e));
-    if (c->nX) HIPCHK(c, hipMemcpy(c->d_cpos, p32.data(), sizeof(int32_t) * c->nX, hipMemcpyHostToDevice));
-    if (npair) HIPCHK(c, hipMemcpy(c->d_chpos, h32.data(), sizeof(int32_t) * npair, hipMemcpyHostToDevice));
-    if (ndummy) HIPCHK(c, hipMemcpy(c->d_cdum, d32.data(), sizeof(int32_t) * ndummy, hipMemcpyHostToDevice));
+    if (c->nX) HIPCHK(c, copy_sync(c, c->d_cpos, p32.data(), sizeof(int32_t) * c->nX, hipMemcpyHostToDevice));
+    if (npair) HIPCHK(c, copy_sync(c, c->d_chpos, h32.data(), sizeof(int32_t) * npair, hipMemcpyHostToDevice));
+    if (ndummy) HIPCHK(c, copy_sync(c, c->d_cdum, d32.data(), sizeof(int32_t) * ndummy, hipMemcpyHostToDevice));
     c->cB = B; c->cnb = nblock; c->cnp = npair; c->cnd = ndummy;
     const int r = B == 6 ? build_chain_slots<6>(c) : build_chain_slots<3>(c);
     if (r != MH_OK) {
@@ -2825,19 +2849,64 @@ int mh_chain_solve(mh_ctx* c) {
     return chain_status(c);
 }
 
-int mh_adjoint_solve(mh_ctx* c, const double* rhs, double alpha) {
-    if (!c || !c->prepared || !c->cB || !rhs || !c->d_scaleX) return MH_ERR_ARG;
-    HIPCHK(c, hipSetDevice(c->device));
-    const int64_t n = std::max<int64_t>(c->nX, 1);
-    if (!c->d_crhs) HIPCHK(c, hipMalloc(&c->d_crhs, sizeof(double) * n));
-    if (!c->d_lam) HIPCHK(c, hipMalloc(&c->d_lam, sizeof(double) * n));
+// The adjoint solve once the unscaled ∂g/∂X is in d_crhs: scale, solve, Λ := alpha·scale ⊙ Δx.
+static int adjoint_solve_tail(Ctx* c, double alpha) {
     hipStream_t s = c->stream;
-    if (c->nX) HIPCHK(c, hipMemcpyAsync(c->d_crhs, rhs, sizeof(double) * c->nX, hipMemcpyHostToDevice, s));
+    const int64_t n = std::max<int64_t>(c->nX, 1);
     hipLaunchKernelGGL(scaled_copy_kernel, dim3(grid(n)), dim3(kBlock), 0, s, c->nX, 1., c->d_scaleX, c->d_crhs, c->d_crhs);
     if (int r = (c->cB == 6 ? enqueue_chain_solve<6>(c, c->d_crhs) : enqueue_chain_solve<3>(c, c->d_crhs))) return r;
     hipLaunchKernelGGL(scaled_copy_kernel, dim3(grid(n)), dim3(kBlock), 0, s, c->nX, alpha, c->d_scaleX, c->d_dx, c->d_lam);
     if (int r = check_launch(c, "adjoint solve")) return r;
     return chain_status(c);
+}
+static int adjoint_buffers(Ctx* c) {
+    const int64_t n = std::max<int64_t>(c->nX, 1);
+    if (!c->d_crhs) HIPCHK(c, hipMalloc(&c->d_crhs, sizeof(double) * n));
+    if (!c->d_lam) HIPCHK(c, hipMalloc(&c->d_lam, sizeof(double) * n));
+    return MH_OK;
+}
+
+int mh_adjoint_solve(mh_ctx* c, const double* rhs, double alpha) {
+    if (!c || !c->prepared || !c->cB || !rhs || !c->d_scaleX) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (int r = adjoint_buffers(c)) return r;
+    if (c->nX) HIPCHK(c, hipMemcpyAsync(c->d_crhs, rhs, sizeof(double) * c->nX, hipMemcpyHostToDevice, c->stream));
+    return adjoint_solve_tail(c, alpha);
+}
+
+int mh_adjoint_solve_sparse(mh_ctx* c, int64_t n, const int64_t* idx, const double* val, double alpha) {
+    if (!c || !c->prepared || !c->cB || !c->d_scaleX || n < 0 || (n && (!idx || !val))) return MH_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    // nonzeros by dof, repeated dofs summed in their given order
+    std::vector<std::pair<int64_t, int64_t>> order((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        if (idx[i] < 1 || idx[i] > c->nX) { c->err = "adjoint right-hand side: dof index out of range"; return MH_ERR_ARG; }
+        order[(size_t)i] = {idx[i] - 1, i};
+    }
+    std::sort(order.begin(), order.end());
+    std::vector<int64_t> hi;
+    std::vector<double> hv;
+    for (size_t k = 0; k < order.size(); ++k) {
+        if (k && order[k].first == order[k - 1].first) hv.back() += val[order[k].second];
+        else { hi.push_back(order[k].first); hv.push_back(val[order[k].second]); }
+    }
+    const int64_t m = (int64_t)hi.size();
+    if (int r = adjoint_buffers(c)) return r;
+    if (m > c->sp_cap) {
+        dfree(c->d_spi); dfree(c->d_spv);
+        c->sp_cap = std::max<int64_t>(m, 256);
+        HIPCHK(c, hipMalloc(&c->d_spi, sizeof(int64_t) * c->sp_cap));
+        HIPCHK(c, hipMalloc(&c->d_spv, sizeof(double) * c->sp_cap));
+    }
+    hipStream_t s = c->stream;
+    const int64_t nx = std::max<int64_t>(c->nX, 1);
+    hipLaunchKernelGGL(chain::fill_f64, dim3(cgrid(nx)), dim3(chain::kCB), 0, s, c->nX, c->d_crhs, 0.0);
+    if (m) {
+        HIPCHK(c, hipMemcpyAsync(c->d_spi, hi.data(), sizeof(int64_t) * m, hipMemcpyHostToDevice, s));
+        HIPCHK(c, hipMemcpyAsync(c->d_spv, hv.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(sparse_set_kernel, dim3(grid(m)), dim3(kBlock), 0, s, m, c->d_spi, c->d_spv, c->d_crhs);
+    }
+    return adjoint_solve_tail(c, alpha);
 }
 
 int mh_chain_timings(mh_ctx* c, double* ms) {
@@ -2853,7 +2922,7 @@ int mh_chain_timings(mh_ctx* c, double* ms) {
 int mh_get_dx(mh_ctx* c, double* dx) {
     if (!c || !c->cB || !dx) return MH_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
-    if (c->nX) HIPCHK(c, hipMemcpy(dx, c->d_dx, sizeof(double) * c->nX, hipMemcpyDeviceToHost));
+    if (c->nX) HIPCHK(c, copy_sync(c, dx, c->d_dx, sizeof(double) * c->nX, hipMemcpyDeviceToHost));
     return MH_OK;
 }
 
@@ -2868,7 +2937,7 @@ int mh_newton_prepare(mh_ctx* c, const double* scaleX) {
     HIPCHK(c, hipMalloc(&c->d_part, sizeof(double) * 2 * cgrid(n)));
     HIPCHK(c, hipMalloc(&c->d_norm, sizeof(double) * 2));
     HIPCHK(c, hipHostMalloc((void**)&c->h_norm, sizeof(double) * 2, hipHostMallocDefault));
-    if (c->nX) HIPCHK(c, hipMemcpy(c->d_scaleX, scaleX, sizeof(double) * c->nX, hipMemcpyHostToDevice));
+    if (c->nX) HIPCHK(c, copy_sync(c, c->d_scaleX, scaleX, sizeof(double) * c->nX, hipMemcpyHostToDevice));
     return MH_OK;
 }
 
@@ -2894,7 +2963,7 @@ int mh_newton_update(mh_ctx* c, int OX, int first, const double* c6, double* nor
 int mh_get_X(mh_ctx* c, int ider, double* x) {
     if (!c || ider < 0 || ider > 2 || !x || (c->nX && !c->d_X[ider])) return MH_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
-    if (c->nX) HIPCHK(c, hipMemcpy(x, c->d_X[ider], sizeof(double) * c->nX, hipMemcpyDeviceToHost));
+    if (c->nX) HIPCHK(c, copy_sync(c, x, c->d_X[ider], sizeof(double) * c->nX, hipMemcpyDeviceToHost));
     return MH_OK;
 }
 
@@ -3185,8 +3254,8 @@ int mh_big_get_pattern(mh_ctx* c, int64_t* colptr, int64_t* rowval) {
     if (!c || !c->d_bcolptr || !colptr || (c->bnnz && !rowval)) return MH_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     std::vector<int32_t> cp(c->bnv + 1), rv(c->bnnz);
-    HIPCHK(c, hipMemcpy(cp.data(), c->d_bcolptr, sizeof(int32_t) * cp.size(), hipMemcpyDeviceToHost));
-    if (c->bnnz) HIPCHK(c, hipMemcpy(rv.data(), c->d_browval, sizeof(int32_t) * rv.size(), hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c, cp.data(), c->d_bcolptr, sizeof(int32_t) * cp.size(), hipMemcpyDeviceToHost));
+    if (c->bnnz) HIPCHK(c, copy_sync(c, rv.data(), c->d_browval, sizeof(int32_t) * rv.size(), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < cp.size(); ++i) colptr[i] = (int64_t)cp[i] + 1;
     for (size_t i = 0; i < rv.size(); ++i) rowval[i] = (int64_t)rv[i] + 1;
     return MH_OK;
@@ -3559,8 +3628,8 @@ int mh_big_solve_prepare(mh_ctx* c, int64_t per_step, int64_t nstep, int64_t na,
     free_bsolve(c);
     const int64_t nv = c->bnv, nnz = c->bnnz, nsp = per_step * nstep;
     std::vector<int32_t> cp(nv + 1), rv(std::max<int64_t>(nnz, 1));
-    HIPCHK(c, hipMemcpy(cp.data(), c->d_bcolptr, sizeof(int32_t) * (nv + 1), hipMemcpyDeviceToHost));
-    if (nnz) HIPCHK(c, hipMemcpy(rv.data(), c->d_browval, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c, cp.data(), c->d_bcolptr, sizeof(int32_t) * (nv + 1), hipMemcpyDeviceToHost));
+    if (nnz) HIPCHK(c, copy_sync(c, rv.data(), c->d_browval, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
     // the widest coupling between steps fixes how many steps one diagonal block holds, so that the
     // blocks are tridiagonal (finite differences of order OX couple steps up to two apart)
     int64_t bw = 0;
@@ -3609,7 +3678,7 @@ int mh_big_solve_prepare(mh_ctx* c, int64_t per_step, int64_t nstep, int64_t na,
     HIPCHK(c, hipMalloc(&c->d_sipiv, sizeof(int32_t) * (nbm + std::max<int64_t>(na, 1))));
     HIPCHK(c, hipMalloc(&c->d_sinfo, sizeof(int32_t) * (nb + 1)));
     HIPCHK(c, hipMalloc(&c->d_sprm, sizeof(int32_t) * (nb + 1) * 2 * m));   // + one for the recursive LU
-    HIPCHK(c, hipMemcpy(c->d_sslot, slot.data(), sizeof(int64_t) * slot.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_sync(c, c->d_sslot, slot.data(), sizeof(int64_t) * slot.size(), hipMemcpyHostToDevice));
     {   // the pattern by rows, for the residuals of the refinement steps
         std::vector<int32_t> rp(nv + 1, 0), ci(std::max<int64_t>(nnz, 1)), pm(std::max<int64_t>(nnz, 1));
         for (int64_t p = 0; p < nnz; ++p) ++rp[rv[p] + 1];
@@ -3628,12 +3697,12 @@ int mh_big_solve_prepare(mh_ctx* c, int64_t per_step, int64_t nstep, int64_t na,
         for (int64_t j = 0; j < nv; ++j)
             for (int32_t p = cp[j]; p < cp[j + 1]; ++p) cv[p] = (int32_t)j;
         HIPCHK(c, hipMalloc(&c->d_scol, sizeof(int32_t) * cv.size()));
-        HIPCHK(c, hipMemcpy(c->d_scol, cv.data(), sizeof(int32_t) * cv.size(), hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(c->d_srowptr, rp.data(), sizeof(int32_t) * (nv + 1), hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(c->d_scolidx, ci.data(), sizeof(int32_t) * ci.size(), hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(c->d_sperm, pm.data(), sizeof(int32_t) * pm.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, copy_sync(c, c->d_scol, cv.data(), sizeof(int32_t) * cv.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, copy_sync(c, c->d_srowptr, rp.data(), sizeof(int32_t) * (nv + 1), hipMemcpyHostToDevice));
+        HIPCHK(c, copy_sync(c, c->d_scolidx, ci.data(), sizeof(int32_t) * ci.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, copy_sync(c, c->d_sperm, pm.data(), sizeof(int32_t) * pm.size(), hipMemcpyHostToDevice));
     }
-    if (!pad.empty()) HIPCHK(c, hipMemcpy(c->d_spad, pad.data(), sizeof(int64_t) * pad.size(), hipMemcpyHostToDevice));
+    if (!pad.empty()) HIPCHK(c, copy_sync(c, c->d_spad, pad.data(), sizeof(int64_t) * pad.size(), hipMemcpyHostToDevice));
     if (!c->blas) BLASCHK(c, rocblas_create_handle(&c->blas));
     BLASCHK(c, rocblas_set_stream(c->blas, s));
     c->sps = per_step; c->snst = nstep; c->sna = na; c->sg = g; c->sm = m; c->snb = nb; c->stot = tot;

@@ -36,6 +36,7 @@ SIGNATURES = {
     "mh_group_add": (_I, [_P, _I, _L, _I, _I64, _D, _I, _I64, _I64, _D, _D, ctypes.POINTER(_I)]),
     "mh_group_set_adofs": (_I, [_P, _I, _I64, _I]),
     "mh_adjoint_solve": (_I, [_P, _D, ctypes.c_double]),
+    "mh_adjoint_solve_sparse": (_I, [_P, _L, _I64, _D, ctypes.c_double]),
     "mh_prepare_sweepx": (_I, [_P, _I, _I64]),
     "mh_get_pattern": (_I, [_P, _I64, _I64]),
     "mh_get_asm": (_I, [_P, _I, _I64, _I64]),

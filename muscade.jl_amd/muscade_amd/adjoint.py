@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import as_f64, dptr, lib
+from ._lib import as_f64, dptr, i64ptr, lib
 from .assemble import AssemblySweepX
 from .chainsolve import ChainSolver
 from .model import MuscadeError
@@ -28,13 +28,20 @@ class LimitStateGradient:
         self.out = AssemblySweepX(0, model, dis, device=device, graph=False)
         self.solver = ChainSolver(self.out)
 
-    def __call__(self, state, dgdX, dbg=None):
-        """Returns (dg/dA (nA), dg/dU (nU), ∂L/∂X (nX)) for ∂g/∂X = dgdX at the equilibrium
-        `state` (X, U, A). The last is −(∂g/∂X)ᵀ up to the solve's accuracy (K symmetric), a
-        check of the adjoint solve."""
+    def __call__(self, state, dgdX, dbg=None, upload=True, check=True):
+        """Returns (dg/dA (nA), dg/dU (nU), ∂L/∂X (nX) or None) for ∂g/∂X = dgdX at the
+        equilibrium `state` (X, U, A). dgdX: a dense vector (nX) or its nonzeros as
+        (1-based dofs, values). upload=False assembles at the state already on the device (the
+        last call's). ∂L/∂X (check=True) is −(∂g/∂X)ᵀ up to the solve's accuracy (K
+        symmetric), a check of the adjoint solve."""
         out = self.out
-        out.assemble(state, "iter", dbg=dbg, fetch=False)
-        r = lib.mh_adjoint_solve(out.ctx.h, dptr(as_f64(dgdX)), -1.0)
+        out.assemble(state, "iter", dbg=dbg, fetch=False, upload=upload)
+        if isinstance(dgdX, tuple):
+            idx = np.ascontiguousarray(dgdX[0], dtype=np.int64)
+            val = as_f64(np.asarray(dgdX[1], dtype=np.float64))
+            r = lib.mh_adjoint_solve_sparse(out.ctx.h, idx.size, i64ptr(idx), dptr(val), -1.0)
+        else:
+            r = lib.mh_adjoint_solve(out.ctx.h, dptr(as_f64(dgdX)), -1.0)
         if r == _lib.MH_ERR_SOLVE:
             raise MuscadeError("adjoint solve failed: " + self.solver.failure(), dbg)
         out.ctx.check(r, "adjoint solve")
@@ -43,9 +50,9 @@ class LimitStateGradient:
         if r == _lib.MH_ERR_NAN:
             raise MuscadeError("adjoint returned NaN", dict(dbg or {}, ieletyp=err.igroup + 1, iele=err.iele + 1))
         out.ctx.check(r, "adjoint")
-        gX = np.zeros(self.model.ndof["X"])
+        gX = np.zeros(self.model.ndof["X"]) if check else None
         gU = np.zeros(self.model.ndof["U"])
         gA = np.zeros(self.model.ndof["A"])
-        out.ctx.check(lib.mh_get_vjp(out.ctx.h, dptr(gX), dptr(gU) if gU.size else None, dptr(gA) if gA.size else None),
-                      "adjoint results")
+        out.ctx.check(lib.mh_get_vjp(out.ctx.h, dptr(gX) if check else None, dptr(gU) if gU.size else None,
+                                     dptr(gA) if gA.size else None), "adjoint results")
         return gA, gU, gX

@@ -306,8 +306,9 @@ def _design_vectors(spec, T, h=0.1, drift=0.02):
 
 def design_point_batch(specs, joined=True, T=1e6):
     """The batch as one Model (joined) or one Model per spec. Each model is a tensioned riser:
-    a vertical cantilever clamped at the seabed with the dead top tension T (DofLoad along z),
-    so that its lateral stiffness comes mostly from the tension, as for a production riser.
+    a vertical cantilever clamped at the seabed with the dead top tension T (DofLoad along z)
+    and its top held in twist, so that its lateral stiffness comes mostly from the tension,
+    as for a production riser.
     (A horizontal 10 m cantilever cut into 20k elements is not solvable in FP64: the
     condition of its bending stiffness grows as ne⁴.) Returns a list of (model, state, tip)
     with the state's U and A set and tip the 1-based X-dofs of the lateral top displacements
@@ -317,6 +318,9 @@ def design_point_batch(specs, joined=True, T=1e6):
         ends = [_add_riser_a(model, s_, xoff=10. * m) for m, s_ in enumerate(sp_list)]
         for fld in ("t1", "t2", "t3", "r1", "r2", "r3"):
             addelement(model, Hold, np.array([f for f, _ in ends]).reshape(-1, 1), field=fld)
+        # the top is held in twist (a riser hangs from its vessel): without it the free twist
+        # of a long riser is the softest mode and Newton wanders in it
+        addelement(model, Hold, np.array([t for _, t in ends]).reshape(-1, 1), field="r3")
         addelement(model, DofLoad, np.array([t for _, t in ends]).reshape(-1, 1), field="t3", value=riser_tension,
                    T=T)
         state = initialize(model, time=0.)
@@ -337,4 +341,26 @@ def design_point_batch(specs, joined=True, T=1e6):
         return model, state, tip
     if joined:
         return [build(specs, "design_point_batch")]
-    return [build([s_], "cantilever") for s_ in specs]
+    return [build([s_], "riser") for s_ in specs]
+
+
+def design_point_equilibrium(state, nload=4, maxΔx=1e-6, maxiter=30, linsolve="device", stats=None, **kw):
+    """The static equilibrium of a design-point batch under its U-loads, by load stepping: the
+    U-loads at 1/nload, 2/nload, … of their value, each SweepX{0} solve started from the last
+    equilibrium (a taut line under its full lateral load in one Newton solve from the straight
+    state overshoots: its axial force doubles in the first iterate). Returns the final state;
+    stats["iterations"] totals the Newton iterations."""
+    from .sweepx import SweepX, solve
+    U = state.U[0].copy()
+    st = state.copy()
+    total = 0
+    for k in range(1, nload + 1):
+        st.U[0][:] = U * (k / nload)
+        st.time = 0.
+        sub = {}
+        st = solve(SweepX[0], initialstate=st, time=[1.], maxΔx=maxΔx, maxiter=maxiter, verbose=False,
+                   catcherror=False, linsolve=linsolve, stats=sub, **kw)[-1]
+        total += sub.get("iterations", 0)
+    if stats is not None:
+        stats["iterations"] = total
+    return st

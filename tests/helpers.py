@@ -27,8 +27,10 @@ def build_lattice(nx, ny, nz, seed=7, disp=1e-3, rot=1e-2, sigma=0.2):
 KIND_TO_ORACLE = {1: orc.K_BEAM, 2: orc.K_BEAM, 3: orc.K_HOLD, 4: orc.K_DOFLOAD, 5: orc.K_ZERO, 6: orc.K_BAR, 7: orc.K_BAR}
 
 
-def oracle_model(model, dis, t=0.0, skip=(), nA=None):
-    """The same model for the oracle's sequential assembler (element types in `skip` left out)."""
+def oracle_model(model, dis, t=0.0, skip=(), nA=None, A=None):
+    """The same model for the oracle's sequential assembler (element types in `skip` left out).
+    A: the A-dof values, which give A-scaled beams (EulerBeam3DA) their scaled sections (the
+    oracle's beam has no A-dofs; it evaluates the scaled section it is given)."""
     om = orc.OracleModel(model.ndof["X"], model.ndof["U"], model.ndof["A"] if nA is None else nA)
     xyz = model.coords()
     io = -1
@@ -43,6 +45,8 @@ def oracle_model(model, dis, t=0.0, skip=(), nA=None):
             nodes = E.cat_nodes()
             o2 = np.concatenate([p["orient2"] for p in E.params])
             mat = np.concatenate([p["mat"] for p in E.params])
+            if E.params[0].get("amask") and A is not None:
+                mat = E.cls.scaled_section(mat, np.asarray(A)[d.A[:, 0] - 1], E.params[0]["amask"])
             params = np.concatenate([xyz[nodes[:, 0], :3], xyz[nodes[:, 1], :3], o2, mat], axis=1)
         elif kind == orc.K_BAR:
             nodes = E.cat_nodes()
@@ -90,7 +94,7 @@ class OracleAssembly:
         self.Lλ = self.nzval = None
 
     def assemble(self, state, mission, dbg=None, fetch=True):
-        self.om = oracle_model(self.model, self.dis, t=state.time)
+        self.om = oracle_model(self.model, self.dis, t=state.time, A=state.A)
         r, rhs, nz, err = self.om.assemble(self.OX, 0 if mission == "step" else 1, self.c.array(),
                                            state.X[:self.OX + 1], state.U[0] if self.model.ndof["U"] else None,
                                            state.A if self.model.ndof["A"] else None)

@@ -40,7 +40,7 @@ def random_state(model, state, seed):
     return st
 
 
-def oracle_elements(model, st, it, loads=True, masked_only=False):
+def oracle_elements(model, st, it, loads=True, masked_only=False, elements=None):
     E = model.eletypes[it]
     d = st.dis.dis[it]
     xyz = model.coords()
@@ -55,12 +55,13 @@ def oracle_elements(model, st, it, loads=True, masked_only=False):
             if not amask >> k & 1:
                 mat[:, k] = 0.
         mat[:, 4:] = 0.
-    R, K = np.zeros((E.nele, 12)), np.zeros((E.nele, 12, 12))
-    for e in range(E.nele):
+    els = np.arange(E.nele) if elements is None else np.asarray(elements)
+    R, K = np.zeros((els.size, 12)), np.zeros((els.size, 12, 12))
+    for q, e in enumerate(els):
         U = st.U[0][d.U[e] - 1] if (loads and not masked_only) else np.zeros(3)
         r, Ks, _ = orc.beam_diffed_residual(xyz[nodes[e, 0], :3], xyz[nodes[e, 1], :3], o2[e], mat[e],
                                             [st.X[0][d.X[e] - 1]], U)
-        R[e], K[e] = r, Ks[0]
+        R[q], K[q] = r, Ks[0]
     return R, K
 
 
@@ -80,6 +81,23 @@ def test_scaled_beam_element_blocks(mc):
     R0, _ = out.element_blocks(0)
     Rb, _ = oracle_elements(model, st0, 0)
     assert np.abs(R0 - Rb).max() <= 1e-12 * np.abs(Rb).max()
+
+
+def test_scaled_beam_large_group(mc):
+    """A group of 1.5M A-scaled beams: the base section copy and the scaling kernel run in
+    stream order (a null-stream copy racing the scaling left later section fields at zero
+    at this size); sampled elements, the last ones included, against the oracle."""
+    from muscade_amd.workloads import design_point_batch
+    [(model, s0, _)] = design_point_batch([dict(ne=15000, seed=9 + m) for m in range(100)])
+    st = random_state(model, s0, 4)
+    out = mc.AssemblySweepX(0, model, st.dis, graph=False)
+    out.assemble(st, "iter", fetch=False)
+    R, K = out.element_blocks(0)
+    n = R.shape[0]
+    sample = np.unique(np.concatenate([np.arange(0, n, n // 40), np.arange(n - 20, n)]))
+    Ro, Ko = oracle_elements(model, st, 0, elements=sample)
+    assert np.abs(R[sample] - Ro).max() <= 1e-12 * np.abs(Ro).max()
+    assert np.abs(K[sample] - Ko).max() <= 1e-12 * np.abs(Ko).max()
 
 
 def test_dLdA_against_oracle_and_differences(mc):
@@ -105,9 +123,8 @@ def test_dLdA_against_oracle_and_differences(mc):
 
 
 def equilibrium(mc, model, s0):
-    st = mc.solve(mc.SweepX[0], initialstate=s0, time=[1.], maxΔx=1e-9, verbose=False, catcherror=False,
-                  linsolve="device")
-    return st[-1]
+    from muscade_amd.workloads import design_point_equilibrium
+    return design_point_equilibrium(s0, maxΔx=1e-9)
 
 
 def richardson(g, h):
@@ -147,7 +164,11 @@ def test_limit_state_gradient_batched_equals_per_model(mc):
     st = equilibrium(mc, model, s0)
     dg = np.zeros(model.ndof["X"])
     dg[tip - 1] = 1.
-    gA, gU, _ = mc.LimitStateGradient(model, st.dis)(st, dg)
+    lsg = mc.LimitStateGradient(model, st.dis)
+    gA, gU, _ = lsg(st, dg)
+    # ∂g/∂X by its nonzeros gives the same right-hand side, so the same gradients bit for bit
+    gAs, gUs, gXs = lsg(st, (tip, np.ones(tip.size)), upload=False, check=False)
+    assert gXs is None and np.array_equal(gAs, gA) and np.array_equal(gUs, gU)
     for m, (mm, sm0, tm) in enumerate(batch(mc, joined=False)):
         sm = equilibrium(mc, mm, sm0)
         d1 = np.zeros(mm.ndof["X"])
