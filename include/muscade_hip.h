@@ -94,6 +94,12 @@ int mh_group_add(mh_ctx* ctx, int kind, int64_t nele, int nX, const int64_t* idx
  * (BeamElement.jl:151-174 with the scaled BeamCrossSection). Every kernel then evaluates the
  * scaled section at the device A (mh_set_A); mh_vjp_lagrangian adds ∂L/∂A. Call before prepare. */
 int mh_group_set_adofs(mh_ctx* ctx, int igroup, const int64_t* idxA, int mask);
+/* ∂R/∂a of every element of such a group at the device X (mh_set_X, ider 0) and A: 12 numbers
+ * per element, element-major, unscaled; ∂R/∂a = ln 10 · R(X; the masked stiffnesses alone,
+ * no loads), R being linear in the scaled stiffnesses. The A column of the residual path's
+ * Jacobian in a DirectXUA step (L2[Λ,A], src/DirectXUA.jl:85-120 with an element whose R
+ * reads A, as the reference's test Spring does). MH_ERR_NAN names the first NaN element. */
+int mh_beam_dRdA(mh_ctx* ctx, int igroup, double* out);
 
 /* ---- prepare(AssemblySweepX{OX}) (src/SweepX.jl:24-33) ------------------
  * Builds on the device the CSC pattern of Lλx and the sorted
@@ -214,7 +220,9 @@ int mh_beam_gauge_derivs_noalloc(mh_ctx* ctx, int igroup, int64_t nsel, const in
  * mission: 0 = :vectors (L1 only), 1 = :matrices. mh_get_direct returns L1 =
  * [Λ (nX) | X (OX+1)·nX | U (OU+1)·nU] and L2 = the pairs (α,β) ∈ {Λ,X,U}² in row-major order
  * without (Λ,Λ), each nder(α)×nder(β) blocks of nnz(α,β) values (nder = 1, OX+1, OU+1).
- * Elements with user cost functions (DofCost) and A-dofs are not evaluated on the device. */
+ * Elements with user cost functions (DofCost) are not evaluated on the device. A-dofs enter only
+ * through A-scaled beams (mh_group_set_adofs): their section follows the device A (mh_set_A),
+ * and their A column is mh_beam_dRdA's, outside these blocks. */
 int mh_set_group_scales(mh_ctx* ctx, int igroup, const double* scaleL, const double* scaleU);
 int mh_prepare_direct(mh_ctx* ctx, int OX, int OU, int IA, int64_t* nnz4);
 int mh_get_direct_pattern(mh_ctx* ctx, int which, int64_t* colptr, int64_t* rowval);
@@ -309,6 +317,11 @@ int mh_calib_copy(mh_ctx* ctx, int64_t n, double* ms);
  * launch_scatter) and each shape's measured time in ms (16).
  * A measurement utility; no counterpart in the reference. */
 int mh_get_tuning(mh_ctx* ctx, int* variant, float* ms);
+/* The static beam kernel's LDS tile (elements per workgroup: 64, 128 or 256; every tile gives
+ * the same numbers), picked by a tuner at the context's first SweepX assembly on its state, and
+ * each tile's measured time in ms (3); MH_ERR_ARG before that assembly. A measurement utility;
+ * no counterpart in the reference. */
+int mh_get_beam_tuning(mh_ctx* ctx, int* block, float* ms);
 /* Override the tuner's pick (a testing and measurement utility: every shape gives bitwise the
  * same sums). */
 int mh_set_scatter_variant(mh_ctx* ctx, int variant);

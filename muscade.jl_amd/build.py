@@ -16,13 +16,32 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
 LIBS = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
 
 
+def _sources_digest():
+    """Digest of every source the library is built from: hipcc reads them once for the device
+    pass and again for the host pass, so a change in between gives host launches of kernels
+    that the code object does not hold."""
+    import hashlib
+    h = hashlib.sha256()
+    csrc = os.path.join(HERE, "csrc")
+    inc = os.path.join(HERE, "..", "include")
+    for d in (csrc, inc):
+        for f in sorted(os.listdir(d)):
+            with open(os.path.join(d, f), "rb") as fh:
+                h.update(f.encode() + fh.read())
+    return h.hexdigest()
+
+
 def build(verbose=False):
     cmd = [HIPCC] + FLAGS + ["-o", OUT + ".tmp"] + SRC + LIBS
     if verbose:
         print(" ".join(cmd), file=sys.stderr)
+    before = _sources_digest()
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stderr[-6000:])
+    if _sources_digest() != before:
+        os.remove(OUT + ".tmp")
+        raise RuntimeError("the sources changed while hipcc ran: build again")
     os.replace(OUT + ".tmp", OUT)
     return OUT
 

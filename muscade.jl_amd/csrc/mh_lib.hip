@@ -141,6 +141,9 @@ struct Ctx {
     double* d_dL1 = nullptr;     // 2·nX: L1[Λ][1], L1[X][1]
     double* d_dL2 = nullptr;     // 2(OX+1)·nnz(XX) + 2·nnz(XU): L2[Λ,X][1,d], L2[X,Λ][d,1], L2[Λ,U][1,1], L2[U,Λ][1,1]
     int scatter_variant = 0;     // nz scatter shape picked at prepare (launch_scatter)
+    int beam_variant = 1;        // static beam kernel shape picked by tune_beams (launch_beam_static)
+    float beam_ms[3] = {0.f, 0.f, 0.f};
+    bool beam_tuned = false;
     float scatter_ms[16] = {};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timed = false;
@@ -306,10 +309,10 @@ struct BeamArgs {
 // One element per lane. Static EulerBeam3D residual + compact tangent (OX=0).
 // The 45 tangent numbers of the block's elements are staged in LDS and written
 // as one contiguous, coalesced range.
-template <bool UDOF>
-__global__ __launch_bounds__(kBeamBlock) void beam_static_kernel(BeamArgs a) {
-    __shared__ double tile[kBeamBlock * mh::kCompact];
-    const int64_t e0 = xcd_tile(a.per_xcd) * kBeamBlock;
+template <bool UDOF, int BLK>
+__global__ __launch_bounds__(BLK) void beam_static_kernel(BeamArgs a) {
+    __shared__ double tile[BLK * mh::kCompact];
+    const int64_t e0 = xcd_tile(a.per_xcd) * BLK;
     if (e0 >= a.nele) return;   // whole workgroup leaves together
     const int tid = threadIdx.x;
     const int64_t e = e0 + tid;
@@ -351,9 +354,27 @@ __global__ __launch_bounds__(kBeamBlock) void beam_static_kernel(BeamArgs a) {
         if (out.nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
     }
     __syncthreads();
-    const int64_t nv = (n - e0 < kBeamBlock ? n - e0 : kBeamBlock) * mh::kCompact;
+    const int64_t nv = (n - e0 < BLK ? n - e0 : BLK) * mh::kCompact;
     double* dst = a.K + e0 * mh::kCompact;
-    for (int64_t f = tid; f < nv; f += kBeamBlock) dst[f] = tile[f];
+    for (int64_t f = tid; f < nv; f += BLK) dst[f] = tile[f];
+}
+// The static beam kernel's shapes for the tuner: elements per workgroup, i.e. the LDS tile of
+// compact tangents staged for one contiguous write (64: one wave per workgroup).
+constexpr int kBeamVariants = 3;
+constexpr int kBeamVariantBlock[kBeamVariants] = {64, 128, 256};
+void launch_beam_static(int variant, BeamArgs& a, bool udof, hipStream_t s) {
+    const int blk = kBeamVariantBlock[variant];
+    const int64_t ntile = (a.nele + blk - 1) / blk;
+    a.per_xcd = (ntile + kXCD - 1) / kXCD;
+    const dim3 gr((unsigned)(a.per_xcd * kXCD));
+    switch (variant) {
+        case 0: if (udof) hipLaunchKernelGGL((beam_static_kernel<true, 64>), gr, dim3(64), 0, s, a);
+                else hipLaunchKernelGGL((beam_static_kernel<false, 64>), gr, dim3(64), 0, s, a); break;
+        case 1: if (udof) hipLaunchKernelGGL((beam_static_kernel<true, 128>), gr, dim3(128), 0, s, a);
+                else hipLaunchKernelGGL((beam_static_kernel<false, 128>), gr, dim3(128), 0, s, a); break;
+        default: if (udof) hipLaunchKernelGGL((beam_static_kernel<true, 256>), gr, dim3(256), 0, s, a);
+                 else hipLaunchKernelGGL((beam_static_kernel<false, 256>), gr, dim3(256), 0, s, a); break;
+    }
 }
 
 struct PointArgs {
@@ -789,13 +810,19 @@ __global__ __launch_bounds__(kBlock) void scale_section_kernel(int64_t n, const 
 // stiffnesses, linearly, so ∂R/∂a = ln 10 · R(X; the scaled stiffnesses alone, no loads), the
 // internal force of those stiffnesses. One element per lane; the internal force is the
 // static closed form at X (for the dynamic beam too: the loads do not depend on stiffness).
+// VEC: the vector ∂R/∂a itself, 12 per element (element-major) into ga, instead of Λ·∂R/∂a.
+template <bool VEC>
 __global__ __launch_bounds__(kBeamBlock) void beam_vjp_a_kernel(VjpArgs a, int mask, double* __restrict__ ga) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= a.nele) return;
     const int64_t n = a.nele;
     double X[12], lam[12];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) { const int32_t q = a.idx[k * n + e]; X[k] = a.X[0][q]; lam[k] = a.lam[q]; }
+    for (int k = 0; k < 12; ++k) {
+        const int32_t q = a.idx[k * n + e];
+        X[k] = a.X[0][q];
+        lam[k] = VEC ? 0. : a.lam[q];
+    }
     const int32_t n1 = a.nodes[e], n2 = a.nodes[n + e];
     mh::v3 c1 = mh::mk(a.xyz[3 * (int64_t)n1], a.xyz[3 * (int64_t)n1 + 1], a.xyz[3 * (int64_t)n1 + 2]);
     mh::v3 c2 = mh::mk(a.xyz[3 * (int64_t)n2], a.xyz[3 * (int64_t)n2 + 1], a.xyz[3 * (int64_t)n2 + 2]);
@@ -808,12 +835,24 @@ __global__ __launch_bounds__(kBeamBlock) void beam_vjp_a_kernel(VjpArgs a, int m
     S.w = 0.;
     mh::BeamOut out;
     mh::beam_static(G, S, mh::mk(0., 0., 0.), X, out);
-    double g = 0.;
+    constexpr double kLn10 = 2.302585092994045684;
+    if constexpr (VEC) {
+        int nan = 0;
 #pragma unroll
-    for (int i = 0; i < 12; ++i) g = g + lam[i] * out.R[i];
-    g = g * 2.302585092994045684;   // ln 10
-    ga[e] = g;
-    if (g != g) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+        for (int i = 0; i < 12; ++i) {
+            const double v = kLn10 * out.R[i];
+            ga[e * 12 + i] = v;
+            nan |= v != v;
+        }
+        if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+    } else {
+        double g = 0.;
+#pragma unroll
+        for (int i = 0; i < 12; ++i) g = g + lam[i] * out.R[i];
+        g = g * kLn10;
+        ga[e] = g;
+        if (g != g) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+    }
 }
 
 // Dynamic beams, 16 lanes per element: lane 12 forms y = J_x Λ, J_v Λ (shared through LDS)
@@ -1370,6 +1409,16 @@ void beam_scales(const double* sX, BeamArgs& a) {
         for (int q = r; q < 6; ++q) a.sS[mh::sym6(r, q)] = sX[r < 3 ? 3 + r : 6 + r] * sX[q < 3 ? 3 + q : 6 + q];
 }
 
+// The static beam kernel's arguments for group g (compact tangent and R into the group's slots).
+BeamArgs static_beam_args(Ctx* c, const Group& g, unsigned long long gtag) {
+    BeamArgs a;
+    a.nele = g.nele; a.idx = g.d_idx; a.uidx = g.d_uidx; a.nodes = g.d_nodes; a.xyz = c->d_xyz; a.o2 = g.d_o2;
+    a.sec = g.d_sec; a.X = c->d_X[0]; a.U = c->d_U;
+    beam_scales(g.scaleX, a);
+    a.K = c->d_Kbuf + g.kbase; a.R = c->d_Rbuf + g.rbase; a.flag = c->d_flag; a.gtag = gtag;
+    return a;
+}
+
 // Value pass then direction pass of one dynamic beam group.
 template <int ND, bool UDOF>
 void launch_dyn(BeamDynArgs& a, bool step, hipStream_t s) {
@@ -1497,7 +1546,7 @@ int launch_vjp(Ctx* c, const Group& g, int nd, unsigned long long gtag) {
         hipLaunchKernelGGL(point_vjp_kernel, dim3(grid(g.nele)), dim3(kBlock), 0, s, a, nd);
     }
     if (beam && g.amask)
-        hipLaunchKernelGGL(beam_vjp_a_kernel, dim3((unsigned)((g.nele + kBeamBlock - 1) / kBeamBlock)), dim3(kBeamBlock),
+        hipLaunchKernelGGL(beam_vjp_a_kernel<false>, dim3((unsigned)((g.nele + kBeamBlock - 1) / kBeamBlock)), dim3(kBeamBlock),
                            0, s, a, g.amask, c->d_gabuf + g.abase);
     return check_launch(c, "adjoint kernels");
 }
@@ -2097,6 +2146,7 @@ int mh_prepare_sweepx(mh_ctx* c, int OX, int64_t* nnz_out) {
             if (tmin < best) { best = tmin; c->scatter_variant = v; }
         }
     }
+    c->beam_tuned = false;
     c->prepared = true;
     if (nnz_out) *nnz_out = nnz;
     return MH_OK;
@@ -2176,6 +2226,40 @@ int mh_set_time(mh_ctx* c, double t) {
 // Every launch of one assemble!{mission} of AssemblySweepX{OX}: the NaN-flag reset, the
 // element kernels of all groups, the nz scatter and the rhs scatter, on the context's
 // stream. With timed, HIP events bracket the three phases (not inside a graph capture).
+// The tuner of the static beam kernel's LDS tile (elements per workgroup), run once per
+// prepared context at its first assembly (the device state is then the caller's), before any
+// graph capture. Every shape gives the same numbers.
+static int tune_beams(Ctx* c) {
+    c->beam_tuned = true;
+    c->beam_variant = 1;
+    bool any_static = false;
+    for (auto& g : c->groups) any_static = any_static || (g.compact && g.nele > 0);
+    if (!any_static) return MH_OK;
+    hipStream_t s = c->stream;
+    float best = 1e30f;
+    for (int v = 0; v < kBeamVariants; ++v) {
+        float tmin = 1e30f;
+        for (int rep = 0; rep < 3; ++rep) {
+            HIPCHK(c, hipEventRecord(c->ev[0], s));
+            for (size_t gi = 0; gi < c->groups.size(); ++gi) {
+                const Group& g = c->groups[gi];
+                if (!g.compact || g.nele == 0) continue;
+                BeamArgs a = static_beam_args(c, g, (unsigned long long)gi << 40);
+                launch_beam_static(v, a, g.kind == MH_KIND_BEAM_U, s);
+            }
+            HIPCHK(c, hipEventRecord(c->ev[1], s));
+            HIPCHK(c, hipEventSynchronize(c->ev[1]));
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            if (rep > 0 && ms < tmin) tmin = ms;
+        }
+        if (int r = check_launch(c, "beam kernel tuning")) return r;
+        c->beam_ms[v] = tmin;
+        if (tmin < best) { best = tmin; c->beam_variant = v; }
+    }
+    return MH_OK;
+}
+
 static int enqueue_sweepx(Ctx* c, int OX, bool step, const double* c6, bool timed) {
     hipStream_t s = c->stream;
     hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, s, c->d_flag, ~0ull);
@@ -2197,16 +2281,8 @@ static int enqueue_sweepx(Ctx* c, int OX, bool step, const double* c6, bool time
         } else if ((g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U) && OX > 0) {
             if (int r = launch_dyn_beams(c, g, OX, step, c6, gtag)) return r;
         } else if (g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U) {
-            BeamArgs a;
-            a.nele = g.nele; a.idx = g.d_idx; a.uidx = g.d_uidx; a.nodes = g.d_nodes; a.xyz = c->d_xyz; a.o2 = g.d_o2;
-            a.sec = g.d_sec; a.X = c->d_X[0]; a.U = c->d_U;
-            beam_scales(g.scaleX, a);
-            a.K = c->d_Kbuf + g.kbase; a.R = c->d_Rbuf + g.rbase; a.flag = c->d_flag; a.gtag = gtag;
-            const int64_t ntile = (g.nele + kBeamBlock - 1) / kBeamBlock;
-            a.per_xcd = (ntile + kXCD - 1) / kXCD;
-            const dim3 gr((unsigned)(a.per_xcd * kXCD));
-            if (g.kind == MH_KIND_BEAM_U) hipLaunchKernelGGL(beam_static_kernel<true>, gr, dim3(kBeamBlock), 0, s, a);
-            else hipLaunchKernelGGL(beam_static_kernel<false>, gr, dim3(kBeamBlock), 0, s, a);
+            BeamArgs a = static_beam_args(c, g, gtag);
+            launch_beam_static(c->beam_variant, a, g.kind == MH_KIND_BEAM_U, s);
         } else {
             PointArgs& a = pb.g[pb.ngroup];
             a.kind = g.kind; a.nele = g.nele; a.nX = g.nX; a.idx = g.d_idx; a.X = c->d_X[0]; a.vals = g.d_vals;
@@ -2269,6 +2345,8 @@ extern "C" {
 int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err* err) {
     if (int r = check_sweepx_args(c, OX, mission, c6, err)) return r;
     HIPCHK(c, hipSetDevice(c->device));
+    if (!c->beam_tuned)
+        if (int r = tune_beams(c)) return r;
     const bool step = (mission == 0) && OX > 0;
     if (int r = enqueue_sweepx(c, OX, step, c6, true)) return r;
     c->timed = true;
@@ -2279,6 +2357,10 @@ int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err*
 int mh_assemble_sweepx_graph(mh_ctx* c, int OX, int mission, const double* c6, mh_err* err) {
     if (int r = check_sweepx_args(c, OX, mission, c6, err)) return r;
     HIPCHK(c, hipSetDevice(c->device));
+    if (!c->beam_tuned) {   // before any capture: the tuner times launches
+        if (int r = tune_beams(c)) return r;
+        ++c->gen;
+    }
     const bool step = (mission == 0) && OX > 0;
     double key[9] = {0, 0, 0, 0, 0, 0, 0, 0, c->t};
     for (int i = 0; i < 6 && OX > 0; ++i) key[i] = c6[i];
@@ -2507,6 +2589,13 @@ int mh_get_tuning(mh_ctx* c, int* variant, float* ms) {
     return MH_OK;
 }
 
+int mh_get_beam_tuning(mh_ctx* c, int* block, float* ms) {
+    if (!c || !c->prepared || !c->beam_tuned) return MH_ERR_ARG;
+    if (block) *block = kBeamVariantBlock[c->beam_variant];
+    if (ms) for (int v = 0; v < kBeamVariants; ++v) ms[v] = c->beam_ms[v];
+    return MH_OK;
+}
+
 int mh_sync(mh_ctx* c) {
     if (!c) return MH_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
@@ -2529,7 +2618,8 @@ int mh_prepare_direct(mh_ctx* c, int OX, int OU, int IA, int64_t* nnz4) {
     if (!c || OX < 0 || OX > 2 || OU < 0 || OU > 2 || IA < 0 || IA > 1) return MH_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     free_prepared(c);
-    if (c->nA) { c->err = "A-dofs are not evaluated on the device in AssemblyDirect"; return MH_ERR_ARG; }
+    // A-dofs reach the device only through A-scaled beams: their section follows A (mh_set_A)
+    // and their A column comes from mh_beam_dRdA, outside these blocks
     int64_t nR = 0, nKd = 0, nKU = 0, nKc = 0;
     for (auto& g : c->groups) {
         if (g.kind == MH_KIND_ZERO || g.kind == MH_KIND_HOST) {
@@ -3048,6 +3138,32 @@ int mh_beam_gauge_derivs(mh_ctx* c, int igroup, int64_t nsel, const int64_t* sel
     dfree(d_out); dfree(d_sel);
     if (e != hipSuccess) { c->err = std::string("gauge derivatives: ") + hipGetErrorString(e); return MH_ERR_HIP; }
     return MH_OK;
+}
+
+int mh_beam_dRdA(mh_ctx* c, int igroup, double* out) {
+    if (!c || igroup < 0 || igroup >= (int)c->groups.size() || !out) return MH_ERR_ARG;
+    const Group& g = c->groups[igroup];
+    if (!g.amask) { c->err = "group has no A-scaled stiffness (mh_group_set_adofs)"; return MH_ERR_ARG; }
+    HIPCHK(c, hipSetDevice(c->device));
+    if (g.nele == 0) return MH_OK;
+    double* d_out = nullptr;
+    HIPCHK(c, hipMalloc(&d_out, sizeof(double) * 12 * g.nele));
+    VjpArgs a{};
+    a.nele = g.nele; a.nX = g.nX; a.kind = g.kind; a.idx = g.d_idx; a.uidx = g.d_uidx; a.nodes = g.d_nodes;
+    a.xyz = c->d_xyz; a.o2 = g.d_o2; a.sec = g.d_sec;
+    for (int d = 0; d < 3; ++d) a.X[d] = c->d_X[d];
+    a.U = c->d_U; a.lam = nullptr; a.flag = c->d_flag; a.gtag = (unsigned long long)igroup << 40;
+    hipStream_t s = c->stream;
+    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, s, c->d_flag, ~0ull);
+    hipLaunchKernelGGL(beam_vjp_a_kernel<true>, dim3((unsigned)((g.nele + kBeamBlock - 1) / kBeamBlock)), dim3(kBeamBlock),
+                       0, s, a, g.amask, d_out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, sizeof(double) * 12 * g.nele, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dfree(d_out);
+    if (e != hipSuccess) { c->err = std::string("dR/dA: ") + hipGetErrorString(e); return MH_ERR_HIP; }
+    return finish_sweepx(c, nullptr);
 }
 
 // mh_beam_gauge_derivs without any allocation: the selection and the output staging live in

@@ -35,6 +35,7 @@ SIGNATURES = {
     "mh_model_begin": (_I, [_P, _L, _L, _L, _L, _D]),
     "mh_group_add": (_I, [_P, _I, _L, _I, _I64, _D, _I, _I64, _I64, _D, _D, ctypes.POINTER(_I)]),
     "mh_group_set_adofs": (_I, [_P, _I, _I64, _I]),
+    "mh_beam_dRdA": (_I, [_P, _I, _D]),
     "mh_adjoint_solve": (_I, [_P, _D, ctypes.c_double]),
     "mh_adjoint_solve_sparse": (_I, [_P, _L, _I64, _D, ctypes.c_double]),
     "mh_prepare_sweepx": (_I, [_P, _I, _I64]),
@@ -49,6 +50,7 @@ SIGNATURES = {
     "mh_vjp_lagrangian": (_I, [_P, _I, ctypes.POINTER(mh_err)]),
     "mh_get_vjp": (_I, [_P, _D, _D, _D]),
     "mh_get_tuning": (_I, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)]),
+    "mh_get_beam_tuning": (_I, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)]),
     "mh_set_scatter_variant": (_I, [_P, _I]),
     "mh_set_group_scales": (_I, [_P, _I, _D, _D]),
     "mh_prepare_direct": (_I, [_P, _I, _I, _I, _I64]),

@@ -278,7 +278,12 @@ class AssemblyDirect:
         self.OX, self.OU, self.IA = OX, OU, IA
         self.model, self.dis = model, dis
         self.ctx = DeviceContext(device)
-        self.groups = upload_model(self.ctx, model, dis, skip=skip, nA=0 if (skip or nA == 0) else None)
+        # A-scaled beams (EulerBeam3DA) on the device read A; otherwise the device needs no A-dofs
+        self.ascaled = [it for it, E in enumerate(model.eletypes)
+                        if it not in skip and E.params and E.params[0].get("amask")]
+        if nA is None and skip and not self.ascaled:
+            nA = 0
+        self.groups = upload_model(self.ctx, model, dis, skip=skip, nA=nA)
         for it, g in enumerate(self.groups):
             d = dis.dis[it]
             if g is not None and d.X.shape[1]:
@@ -295,6 +300,17 @@ class AssemblyDirect:
             self.ctx.check(lib.mh_get_direct_pattern(self.ctx.h, w, i64ptr(cp), i64ptr(rv)), "pattern")
             self.patterns.append((cp, rv, (nr, nc)))
         self.nder = {"Λ": 1, "X": OX + 1, "U": OU + 1}
+
+    def beam_dRdA(self, it):
+        """∂R/∂a (nele × 12, unscaled) of the A-scaled beam type it at the state of the last
+        `assemble` (mh_beam_dRdA)."""
+        E = self.model.eletypes[it]
+        out = np.zeros((E.nele, 12))
+        r = lib.mh_beam_dRdA(self.ctx.h, self.groups[it], dptr(out))
+        if r == _lib.MH_ERR_NAN:
+            raise MuscadeError(f"residual({E.key[0]},...) returned NaN in ∂R/∂A", dict(ieletyp=it + 1))
+        self.ctx.check(r, "dR/dA")
+        return out
 
     def gauge_derivatives(self, it, state):
         """Values, gradients and Hessians of the gauge requestables of beam type it at
@@ -313,6 +329,8 @@ class AssemblyDirect:
         if self.nU:
             self.ctx.check(lib.mh_set_U(h, dptr(as_f64(state.U[0]))), "U")
         self.ctx.check(lib.mh_set_lambda(h, dptr(as_f64(Λ))), "Λ")
+        if self.ascaled:
+            self.ctx.check(lib.mh_set_A(h, dptr(as_f64(state.A))), "A")
         for it, E in enumerate(self.model.eletypes):
             if E.cls.kind == _lib.MH_KIND_DOFLOAD and E.nele and self.groups[it] is not None:
                 vals = np.concatenate([np.full(n.shape[0], float(p["value"](state.time, **p["args"])))

@@ -16,9 +16,9 @@ of src/FiniteDifferences.jl for the time derivatives (assemblebig!, :322-357). T
 of that system stays on the host (UMFPACK `lu` in the reference, SuperLU here). The
 increments are applied back through the same kernels (decrementbig!, :358-383).
 
-The unknowns are laid out step after step, [Λ (nX) | X (nX) | U (nU)] per step, then A. This
-is the block order of makepattern (:245-296); the sparsity pattern of the big matrix itself
-is built by scipy, not bit for bit as SparseTools.prepare does.
+The unknowns are laid out step after step, [Λ (nX) | X (nX) | U (nU)] per step, then A: the
+block order of makepattern (:245-296). The big matrix has Muscade's pattern (BigPattern:
+makepattern + SparseTools.prepare), bit for bit.
 
 One deliberate difference. For residual elements (beams, bars: no_second_order), the
 reference's step assembly adds L1[Λ] = R and the Λ-row/column Jacobian blocks, but not the
@@ -315,6 +315,20 @@ class HostLagrangians:
         return gA, pieces
 
 
+def _add_beam_dRdA(out, it, d, Λ, L1, L2, Λgradient):
+    """The A column of the residual path for an A-scaled beam type (R reads A, as the
+    reference's test Spring does, test/SomeElements.jl:162-186): L2[Λ,A] = ∂R/∂A·sΛ·sA and its
+    transpose L2[A,Λ], and, with Λgradient, L1[A] += (L2[Λ,A])ᵀΛ as for X and U."""
+    D = out.beam_dRdA(it)                                   # nele × 12, unscaled
+    rows = d.X - 1                                          # the Λ dofs of the element (its X dofs)
+    col = np.repeat(d.A[:, 0] - 1, rows.shape[1])
+    vals = (D * d.scaleΛ[None, :] * d.scaleA[0]).ravel()
+    L2.append(("Λ", 0, "A", 0, rows.ravel(), col, vals))
+    L2.append(("A", 0, "Λ", 0, col, rows.ravel(), vals))
+    if Λgradient:
+        np.add.at(L1["A"][0], col, vals * Λ[rows.ravel()])
+
+
 class _Replica:
     """n disjoint copies of a model's device element types, numbered copy after copy: copy s
     owns nodes s·nnod.., X-dofs s·nX.. and U-dofs s·nU.. (just what upload_model reads)."""
@@ -579,7 +593,11 @@ class DirectXUA(metaclass=_DirectXUAMeta):
         if linsolve == "device" and big != "device":
             raise MuscadeError("linsolve='device' solves the system where it is assembled (big='device')")
         host = HostLagrangians(model, dis, OX, OU, IA)
+        ascaled = [it for it, E in enumerate(model.eletypes) if it not in host.skip and E.params
+                   and E.params[0].get("amask")]
         if batch_steps:
+            if ascaled:
+                raise MuscadeError("A-scaled beams (EulerBeam3DA) are assembled one step per launch (batch_steps=False)")
             batched = [StepBatchAssembly(OX, OU, IA, model, dis, n, skip=tuple(host.skip), device=device)
                        for n in nstep]
             provider = None
@@ -602,7 +620,7 @@ class DirectXUA(metaclass=_DirectXUAMeta):
         bigpat = BigPattern(IA, nstep, class_patterns(model, dis, OX, OU, IA))
         if bigpat.nv != nv:
             raise MuscadeError(f"all-steps pattern of order {bigpat.nv}, expected {nv}")
-        st_["pattern"] = bigpat
+        st_["nnz"] = bigpat.nnz
         coff = {"Λ": 0, "X": nX, "U": 2 * nX}
 
         def block(iexp, istep, c):   # istep 1-based
@@ -649,6 +667,9 @@ class DirectXUA(metaclass=_DirectXUAMeta):
                         if Λgradient and pair in L2d and L1h[b]:
                             for bder, M in enumerate(L2d[pair][0][:nder[b]]):
                                 L1h[b][bder] = L1h[b][bder] + M.T @ st.Λ[0]
+                    if IA:
+                        for it in ascaled:
+                            _add_beam_dRdA(out, it, dis.dis[it], st.Λ[0], L1h, L2h, Λgradient)
                     for b in classes:
                         for bder in range(nder[b]):
                             v = L1h[b][bder]

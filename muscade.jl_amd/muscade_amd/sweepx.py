@@ -38,7 +38,8 @@ class SweepX(metaclass=_SweepXMeta):
         prepare(AssemblySweepX{OX},model,dis) (SweepX.jl:24,188). graph: the device assembler
         replays captured HIP graphs for the Newton iterations of each time step.
         linsolve: "host" (SuperLU) or "device" (chain topologies only; raises
-        chainsolve.NotAChain otherwise). stats (a dict) receives the Newton iteration count."""
+        chainsolve.NotAChain otherwise). stats (a dict) receives the Newton iteration count and
+        the seconds spent preparing the assembler (and the chain layout)."""
         OX = cls.OX
         model, dis = initialstate.model, initialstate.dis
         if linsolve not in ("host", "device"):
@@ -47,8 +48,11 @@ class SweepX(metaclass=_SweepXMeta):
         if dev and Assembly is not AssemblySweepX:
             raise MuscadeError("linsolve='device' solves the device assembler's system (Assembly=AssemblySweepX)")
         kw = {"graph": graph} if Assembly is AssemblySweepX else {}
+        t_prep = _time.perf_counter()
         out = Assembly(OX, model, dis, device=device, **kw)
         solver = ChainSolver(out) if dev else None
+        if stats is not None:
+            stats["prepare_s"] = _time.perf_counter() - t_prep
         nX = out.nX
         scaleX = dis.scaleX
         cΔx2, cLλ2 = maxΔx ** 2, maxLλ ** 2

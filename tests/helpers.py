@@ -115,6 +115,28 @@ class OracleAssemblyDirect:
         self.OX, self.OU, self.IA, self.model, self.dis, self.skip = OX, OU, IA, model, dis, tuple(skip)
         self.nX, self.nU = model.ndof["X"], model.ndof["U"]
         self.nder = {"Λ": 1, "X": OX + 1, "U": OU + 1}
+        self._state = None
+
+    def beam_dRdA(self, it):
+        """∂R/∂a of an A-scaled beam type at the last assembled state: ln 10 · R(X; the masked
+        stiffnesses of the scaled section alone, no loads), element by element with the oracle's
+        beam."""
+        E, d, st = self.model.eletypes[it], self.dis.dis[it], self._state
+        xyz = self.model.coords()
+        nodes = E.cat_nodes()
+        o2 = np.concatenate([p["orient2"] for p in E.params])
+        amask = E.params[0]["amask"]
+        mat = E.cls.scaled_section(np.concatenate([p["mat"] for p in E.params]), st.A[d.A[:, 0] - 1], amask).copy()
+        for k in range(4):
+            if not amask >> k & 1:
+                mat[:, k] = 0.
+        mat[:, 4:] = 0.
+        out = np.zeros((E.nele, 12))
+        for e in range(E.nele):
+            r, _, _ = orc.beam_diffed_residual(xyz[nodes[e, 0], :3], xyz[nodes[e, 1], :3], o2[e], mat[e],
+                                               [st.X[0][d.X[e] - 1]], np.zeros(3))
+            out[e] = np.log(10.) * r
+        return out
 
     def gauge_derivatives(self, it, state):
         """The oracle's (εₐₓ, κ) values, gradients and Hessians (orc_beam_gauge2) per element."""
@@ -132,7 +154,8 @@ class OracleAssemblyDirect:
 
     def assemble(self, state, Λ, mission="matrices", Δt=1.0, dbg=None):
         import scipy.sparse as sp
-        om = oracle_model(self.model, self.dis, t=state.time, skip=self.skip, nA=0)
+        self._state = state
+        om = oracle_model(self.model, self.dis, t=state.time, skip=self.skip, nA=0, A=state.A)
         om.prepare_direct(self.OX, self.OU, 0)
         U = [state.U[d] for d in range(self.OU + 1)] if self.nU else None
         r, L1f, L2f = om.assemble_direct(1 if mission == "matrices" else 0, Δt, Λ, state.X[:self.OX + 1], U,

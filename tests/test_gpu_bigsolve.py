@@ -99,6 +99,43 @@ def test_each_solve_against_splu(mc):
         assert be <= 1e-13 and be <= 10. * bs + 1e-20, (be, bs)
 
 
+@pytest.mark.parametrize("nel", [48, 123])
+def test_large_blocks_against_splu(mc, nel):
+    """Block orders past one panel row slot of the device LU: nel = 48 gives m = 1,200 (rows
+    1024..2400 of a block fall in the panel kernel's register slots 1 and 2), nel = 123 gives
+    m = 3,000 (above 2,560: the rocSOLVER getrf path). Backward error of every Newton system."""
+    from muscade_amd import directxua as dx
+    model = beam_model(mc, nel=nel, dyn_load=True)
+    time = np.arange(1., 8.) * 0.1
+    errs, orders = [], []
+    orig = dx.DeviceBigSystem.solve
+
+    def spy(self, hcontrib, rhs, dbg=None):
+        x = orig(self, hcontrib, rhs, dbg)
+        M = self.assemble(hcontrib)
+
+        def berr(y):
+            return np.abs(M @ y - rhs).max() / (abs(M).max() * np.abs(y).sum() + np.abs(rhs).max())
+        xs = spla.spsolve(M.tocsc(), rhs)
+        errs.append((berr(x), berr(xs)))
+        return x
+    dx.DeviceBigSystem.solve = spy
+    try:
+        st0 = mc.initialize(model, time=0.)
+        sd = {}
+        mc.solve(mc.DirectXUA[2, 0, 0], initialstate=[st0], time=[time], maxΔx=1e-9, maxΔλ=1e-6, maxiter=6,
+                 verbose=False, catcherror=True, silenterror=True, batch_steps=True, big="device", linsolve="device",
+                 stats=sd)
+    finally:
+        dx.DeviceBigSystem.solve = orig
+    print("nel", nel, "block order", sd.get("block_order"), "solves", len(errs), flush=True)
+    assert sd["block_order"] == 2 * (2 * model.ndof["X"] + model.ndof["U"])
+    assert errs
+    for be, bs in errs:
+        print("backward error device", be, "SuperLU", bs, flush=True)
+        assert be <= 1e-13 and be <= 10. * bs + 1e-20, (be, bs)
+
+
 def test_singular_system_raises(mc):
     """An X-dof that only a zero cost touches: its column (and its Λ row) is exactly zero."""
     model = beam_model(mc, dyn_load=True)
