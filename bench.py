@@ -553,6 +553,25 @@ def graph_bench(args):
         row["speedup"] = row["eager_us"] / row["graph_us"]
         res[f"cantilever_{nel}"] = row
         log(f"graph: {nel} beams: eager {row['eager_us']:.1f} us, graph {row['graph_us']:.1f} us per assembly")
+    # whole device Newton iterations of SweepX{2} (assembly, chain solve, decrement) on one
+    # 100-beam hanging chain with a moving top: eager, graphed assembly, one graph per iteration
+    from muscade_amd.workloads import catenary_chains
+    model, state = catenary_chains(1, 100, 100, seed=4, disp=0., rot=0., vel=0., acc=0., motion=True)
+    state.time = 0.
+    row = {}
+    for name, g in (("eager", False), ("assembly_graph", "assembly"), ("iteration_graph", True)):
+        kw = dict(initialstate=state, maxΔx=1e-6, maxiter=20, verbose=False, catcherror=False, linsolve="device",
+                  graph=g)
+        mc.solve(mc.SweepX[2], time=0.05 * np.arange(1, 4), **kw)       # warm-up
+        st = {}
+        t0 = time.perf_counter()
+        mc.solve(mc.SweepX[2], time=0.05 * np.arange(1, 101), stats=st, **kw)
+        el = time.perf_counter() - t0 - st.get("prepare_s", 0.)
+        row[name + "_us_per_iteration"] = el / st["iterations"] * 1e6
+        row["iterations"] = st["iterations"]
+    row["speedup"] = row["eager_us_per_iteration"] / row["iteration_graph_us_per_iteration"]
+    res["newton_chain_100"] = row
+    log(f"graph: Newton iteration, 100-beam chain: {row}")
     return res
 
 

@@ -150,10 +150,11 @@ int mh_get_vjp(mh_ctx* ctx, double* gX, double* gU, double* gA);
 int mh_assemble_sweepx(mh_ctx* ctx, int OX, int mission, const double* c6, mh_err* err);
 /* The same assembly launched as one captured HIP graph (no counterpart in the reference;
  * a launch-overhead option for the Newton iterations of SweepX.jl:186-221). The graph is
- * captured on the first call and replayed while OX, c6, the time (mh_set_time) and the
- * dof scales are unchanged — the iterations of one time step (one graph is kept per
- * mission, :step and :iter); any change recaptures it. State setters (mh_set_X/U/A, mh_set_point_values) act through device
- * buffers and need no recapture. Results are bitwise those of mh_assemble_sweepx;
+ * captured on the first call and replayed while OX, the dof scales and, when a bar group is
+ * present (its kernel takes the time), the time (mh_set_time) are unchanged (one graph is kept
+ * per mission, :step and :iter); any change recaptures it. c6 and the state setters
+ * (mh_set_X/U/A, mh_set_point_values) act through device buffers and need no recapture: without
+ * bars, a whole time integration replays two graphs. Results are bitwise those of mh_assemble_sweepx;
  * mh_last_timings then gives the whole graph's time in ms[0] (ms[1] = ms[2] = 0). */
 int mh_assemble_sweepx_graph(mh_ctx* ctx, int OX, int mission, const double* c6, mh_err* err);
 /* Number of graph captures so far (a measurement utility). */
@@ -260,6 +261,15 @@ int mh_get_dx(mh_ctx* ctx, double* dx);
 int mh_newton_prepare(mh_ctx* ctx, const double* scaleX);
 int mh_newton_update(mh_ctx* ctx, int OX, int first, const double* c6, double* norms);
 int mh_get_X(mh_ctx* ctx, int ider, double* x);
+/* One whole device Newton iteration of SweepX{OX} (assembly with mission :step when first and
+ * OX > 0, else :iter; the chain solve; the Newmark decrement and the norms) as one captured HIP
+ * graph, replayed while OX, first, the buffers and (with bars) the time are unchanged; c6 is
+ * read from device memory, so without bars a time integration replays two graphs, one for the
+ * first iteration of every step and one for the others (no counterpart in the reference; the loop of
+ * SweepX.jl:186-221 with its lu/solve on the device). One host wait per iteration. Errors as
+ * mh_assemble_sweepx (MH_ERR_NAN) and mh_chain_solve (MH_ERR_SOLVE); after either, the device X
+ * already holds that iteration's update. mh_last_timings: the whole iteration. */
+int mh_newton_iteration_graph(mh_ctx* ctx, int OX, int first, const double* c6, double* norms, mh_err* err);
 /* The adjoint solve of a limit state g(X) at an equilibrium (no counterpart in the reference;
  * the step before the adjoint gradient of a reliability or design-point search): with the
  * tangent of the last assembly (SweepX{0} :iter, symmetric), solve K·μ = rhs (rhs = ∂g/∂X, nX

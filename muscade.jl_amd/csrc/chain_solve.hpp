@@ -412,14 +412,16 @@ __global__ __launch_bounds__(kCB) void unpack_kernel(int64_t nX, int64_t nb, con
 // Newmarkβdecrement!{OX} (src/SweepX.jl:98-132) over all X dofs, in the host's operation
 // order and without contraction, so that a given Δx decrements the state bitwise as the
 // host loop does; per-block partial sums of Δx² and Lλ².
-__global__ __launch_bounds__(kCB) void newmark_kernel(int64_t nX, int OX, int first, double a1, double a2, double a3,
-                                                      double b1, double b2, double b3,
+// k: the Newmark coefficients (a₁,a₂,a₃,b₁,b₂,b₃) in device memory, so that a captured graph
+// replays with the current ones.
+__global__ __launch_bounds__(kCB) void newmark_kernel(int64_t nX, int OX, int first, const double* __restrict__ k,
                                                       const double* __restrict__ dx, const double* __restrict__ rhs,
                                                       const double* __restrict__ scale, double* __restrict__ X0,
                                                       double* __restrict__ X1, double* __restrict__ X2,
                                                       double* __restrict__ part) {
 #pragma clang fp contract(off)
     __shared__ double s1[kCB], s2[kCB];
+    const double a1 = k[0], a2 = k[1], a3 = k[2], b1 = k[3], b2 = k[4], b3 = k[5];
     const int64_t c = (int64_t)blockIdx.x * kCB + threadIdx.x;
     double q1 = 0.0, q2 = 0.0;
     if (c < nX) {

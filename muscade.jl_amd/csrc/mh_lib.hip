@@ -96,6 +96,8 @@ struct Ctx {
     int64_t nK = 0, nP = 0, nR = 0, nnz = 0;
     double* d_Kc = nullptr;      // dynamic beams: scaled compact static part, 45 per element
     double* d_Kb = nullptr;      // dynamic beams (SweepX): ε-free kinematics, mh::kDynBase per element
+    double* d_c6 = nullptr;      // SweepX: the Newmark coefficients of the current call (6)
+    double* h_c6 = nullptr;      // pinned staging of d_c6
     double* d_Kbuf = nullptr;
     double* d_Rbuf = nullptr;
     double* d_dRbuf = nullptr;
@@ -151,11 +153,13 @@ struct Ctx {
     // captured SweepX assembly (mh_assemble_sweepx_graph): replayed while its launch
     // arguments (OX, mission, Newmark coefficients, time, dof scales) are unchanged
     // (one slot per mission: :step and :iter alternate within a time step)
-    hipGraph_t graph[2] = {nullptr, nullptr};
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};
-    double gkey[2][9] = {};
-    int gOX[2] = {-1, -1};
-    uint64_t gen = 0, ggen[2] = {~0ull, ~0ull};   // gen: bumped by setters that change launch arguments
+    // captured graphs: slots 0/1 the assembly (:iter/:step), 2/3 a whole device Newton iteration
+    // (later/first iteration of a step)
+    hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
+    double gkey[4][9] = {};
+    int gOX[4] = {-1, -1, -1, -1};
+    uint64_t gen = 0, ggen[4] = {~0ull, ~0ull, ~0ull, ~0ull};   // gen: bumped by setters that change launch arguments
     int64_t graph_captures = 0;
     // on-device solve of Lλx·Δx = Lλ for chain topologies (mh_chain_prepare), and the
     // Newton-Newmark decrement (mh_newton_update)
@@ -224,7 +228,7 @@ void free_graph(Ctx* c, int slot) {
     if (c->graph[slot]) { (void)hipGraphDestroy(c->graph[slot]); c->graph[slot] = nullptr; }
     c->gOX[slot] = -1;
 }
-void free_graph(Ctx* c) { free_graph(c, 0); free_graph(c, 1); }
+void free_graph(Ctx* c) { for (int k = 0; k < 4; ++k) free_graph(c, k); }
 void free_chain(Ctx* c) {
     dfree(c->d_cpos); dfree(c->d_chpos); dfree(c->d_cdum); dfree(c->d_cbuf); dfree(c->d_csaved); dfree(c->d_dx);
     dfree(c->d_cslot); dfree(c->d_cunc); c->cnu = 0;
@@ -451,6 +455,7 @@ struct BeamDynArgs {
     double scale[12];
     double sA[6], sT[18], sS[21], sR[12];
     double c[6];        // Newmark (a₁,a₂,a₃,b₁,b₂,b₃)
+    const double* dc = nullptr;   // the same in device memory (SweepX: graphs replay with the current ones)
     double* Kc;         // value pass → direction pass: scaled compact static part, 45 per element
     double* K;          // full 12×12 per element
     double* R;
@@ -615,7 +620,7 @@ __global__ __launch_bounds__(kDirWBlock) void beam_dyn_dir_kernel(BeamDynArgs a)
         dyn_load<ND, UDOF>(a, e, G, S, D, X, U);
         if (j < 6) {
             mh::DynDir dt;
-            mh::dyn_translation_direction<ND>(G, D, a.Kb + e, n, a.scale, a.c, dir_t, dt);
+            mh::dyn_translation_direction<ND>(G, D, a.Kb + e, n, a.scale, a.dc ? a.dc : a.c, dir_t, dt);
 #pragma unroll
             for (int g = 0; g < 4; ++g)
 #pragma unroll
@@ -623,7 +628,7 @@ __global__ __launch_bounds__(kDirWBlock) void beam_dyn_dir_kernel(BeamDynArgs a)
 #pragma unroll
             for (int q = 0; q < 3; ++q) Jsh[el][dir_t][12 + q] = 0.;
         }
-        mh::dyn_direction<ND, true>(G, D, X, U, a.scale, a.c, dir, d);
+        mh::dyn_direction<ND, true>(G, D, X, U, a.scale, a.dc ? a.dc : a.c, dir, d);
         if (dir < 12) {
 #pragma unroll
             for (int g = 0; g < 4; ++g)
@@ -682,6 +687,7 @@ struct BarArgs {
     double t;
     double scale[6];
     double c[6];
+    const double* dc = nullptr;   // the Newmark coefficients in device memory, if set (else c)
     double* K;           // 36 per element, element-major, column-major inside
     double* R;
     double* dR;
@@ -719,7 +725,7 @@ __global__ __launch_bounds__(kBarBlock) void bar_kernel(BarArgs a) {
         for (int q = 0; q < 3; ++q) U[q] = UDOF ? a.U[a.uidx[q * n + e]] : 0.;
         double R[6], dR[6];
         double* K = tile + tid * 36;
-        const int nan = mh::bar_element<ND>(G, S, X, U, a.t, a.scale, a.c, R, K, STEP ? dR : nullptr);
+        const int nan = mh::bar_element<ND>(G, S, X, U, a.t, a.scale, a.dc ? a.dc : a.c, R, K, STEP ? dR : nullptr);
 #pragma unroll
         for (int i = 0; i < 6; ++i) a.R[i * n + e] = R[i];
         if (STEP) {
@@ -1447,6 +1453,7 @@ int launch_dyn_beams(Ctx* c, const Group& g, int OX, bool step, const double* c6
     for (int i = 0; i < 21; ++i) a.sS[i] = b.sS[i];
     for (int i = 0; i < 12; ++i) { a.sR[i] = b.sR[i]; a.scale[i] = g.scaleX[i]; }
     for (int i = 0; i < 6; ++i) a.c[i] = c6[i];
+    a.dc = c->d_c6;
     a.Kc = c->d_Kc + g.kcbase; a.K = c->d_Kbuf + g.kbase; a.R = c->d_Rbuf + g.rbase;
     a.Kb = c->d_Kb + g.kcbase / mh::kCompact * mh::kDynBase;
     a.dR = step ? c->d_dRbuf + g.rbase : nullptr;
@@ -1472,6 +1479,7 @@ int launch_bars(Ctx* c, const Group& g, int OX, bool step, const double* c6, uns
     a.U = c->d_U;
     a.t = c->t;
     for (int i = 0; i < 6; ++i) { a.scale[i] = g.scaleX[i]; a.c[i] = (OX > 0 && c6) ? c6[i] : 0.; }
+    a.dc = c->d_c6;
     a.K = c->d_Kbuf + g.kbase; a.R = c->d_Rbuf + g.rbase; a.dR = step ? c->d_dRbuf + g.rbase : nullptr;
     a.flag = c->d_flag; a.gtag = gtag;
     const bool u = g.kind == MH_KIND_BAR_U;
@@ -1849,9 +1857,13 @@ int mh_ctx_create(int device, mh_ctx** out) {
     for (auto& e : c->ev) if (hipEventCreate(&e) != hipSuccess) { delete c; return MH_ERR_HIP; }
     if (hipMalloc(&c->d_flag, sizeof(unsigned long long)) != hipSuccess) { delete c; return MH_ERR_HIP; }
     if (hipHostMalloc((void**)&c->h_flag, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
-        delete c; return MH_ERR_HIP;
+        (void)hipFree(c->d_flag); delete c; return MH_ERR_HIP;
     }
-    if (scratch_address(device, &c->d_scratch) != MH_OK) {
+    if (hipMalloc(&c->d_c6, sizeof(double) * 6) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_c6, sizeof(double) * 6, hipHostMallocDefault) != hipSuccess ||
+        scratch_address(device, &c->d_scratch) != MH_OK) {
+        if (c->h_c6) (void)hipHostFree(c->h_c6);
+        if (c->d_c6) (void)hipFree(c->d_c6);
         (void)hipHostFree(c->h_flag); (void)hipFree(c->d_flag); delete c; return MH_ERR_HIP;
     }
     *out = c;
@@ -1865,6 +1877,8 @@ int mh_ctx_destroy(mh_ctx* c) {
     free_model(c);
     dfree(c->d_flag);
     if (c->h_flag) (void)hipHostFree(c->h_flag);
+    dfree(c->d_c6);
+    if (c->h_c6) (void)hipHostFree(c->h_c6);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->blas) (void)rocblas_destroy_handle(c->blas);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -2316,6 +2330,14 @@ static int enqueue_sweepx(Ctx* c, int OX, bool step, const double* c6, bool time
     return check_launch(c, "scatter kernels");
 }
 
+// Whether a kernel of the assembly takes the time as an argument (bars: their loads may vary
+// in time); beams, Holds and DofLoads read it only through values set by the host.
+static bool reads_time(const Ctx* c) {
+    for (const auto& g : c->groups)
+        if ((g.kind == MH_KIND_BAR || g.kind == MH_KIND_BAR_U) && g.nele) return true;
+    return false;
+}
+
 // Wait for the assembly, then report the first element that returned NaN.
 static int finish_sweepx(Ctx* c, mh_err* err) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -2327,6 +2349,14 @@ static int finish_sweepx(Ctx* c, mh_err* err) {
         if (err) { err->code = MH_ERR_NAN; err->igroup = gi; err->iele = e; }
         return MH_ERR_NAN;
     }
+    return MH_OK;
+}
+
+// The Newmark coefficients of this call into d_c6, ahead of the kernels on the stream: kernels
+// (eager or replayed from a graph) read them there, so a graph outlives a change of Δt.
+static int upload_c6(Ctx* c, int OX, const double* c6) {
+    for (int i = 0; i < 6; ++i) c->h_c6[i] = (OX > 0 && c6) ? c6[i] : 0.;
+    HIPCHK(c, hipMemcpyAsync(c->d_c6, c->h_c6, sizeof(double) * 6, hipMemcpyHostToDevice, c->stream));
     return MH_OK;
 }
 
@@ -2348,6 +2378,7 @@ int mh_assemble_sweepx(mh_ctx* c, int OX, int mission, const double* c6, mh_err*
     if (!c->beam_tuned)
         if (int r = tune_beams(c)) return r;
     const bool step = (mission == 0) && OX > 0;
+    if (int r = upload_c6(c, OX, c6)) return r;
     if (int r = enqueue_sweepx(c, OX, step, c6, true)) return r;
     c->timed = true;
     c->graph_timed = false;
@@ -2362,8 +2393,8 @@ int mh_assemble_sweepx_graph(mh_ctx* c, int OX, int mission, const double* c6, m
         ++c->gen;
     }
     const bool step = (mission == 0) && OX > 0;
-    double key[9] = {0, 0, 0, 0, 0, 0, 0, 0, c->t};
-    for (int i = 0; i < 6 && OX > 0; ++i) key[i] = c6[i];
+    if (int r = upload_c6(c, OX, c6)) return r;
+    const double key[9] = {0, 0, 0, 0, 0, 0, 0, 0, reads_time(c) ? c->t : 0.};
     const int sl = step ? 1 : 0;
     bool same = c->gexec[sl] && c->gOX[sl] == OX && c->ggen[sl] == c->gen;
     for (int i = 0; i < 9 && same; ++i) same = std::memcmp(&key[i], &c->gkey[sl][i], sizeof(double)) == 0;
@@ -2779,7 +2810,7 @@ namespace {
 int64_t cgrid(int64_t n) { return (n + chain::kCB - 1) / chain::kCB; }
 
 template <int B>
-int enqueue_chain_solve(Ctx* c, const double* rhs) {
+int enqueue_chain_solve(Ctx* c, const double* rhs, bool timed = true) {
     using namespace chain;
     hipStream_t s = c->stream;
     const int64_t nb = c->cnb, BB = (int64_t)B * B;
@@ -2788,7 +2819,7 @@ int enqueue_chain_solve(Ctx* c, const double* rhs) {
     double* U = L + nb * BB;
     double* b = U + nb * BB;
     double* pc = b + nb * B;
-    HIPCHK(c, hipEventRecord(c->cev[0], s));
+    if (timed) HIPCHK(c, hipEventRecord(c->cev[0], s));
     hipLaunchKernelGGL(clear_flag, dim3(1), dim3(1), 0, s, c->d_cflag);
     if (c->cnp) hipLaunchKernelGGL(fill_f64, dim3(cgrid(4 * c->cnp)), dim3(kCB), 0, s, 4 * c->cnp, pc, 0.0);
     if (c->cnu) hipLaunchKernelGGL(zero_slots_kernel, dim3(cgrid(c->cnu)), dim3(kCB), 0, s, c->cnu, c->d_cunc, D);
@@ -2797,7 +2828,7 @@ int enqueue_chain_solve(Ctx* c, const double* rhs) {
     hipLaunchKernelGGL(pack_rhs_kernel, dim3(cgrid(c->nX)), dim3(kCB), 0, s, c->nX, c->d_cpos, rhs, b, pc);
     if (c->cnp) hipLaunchKernelGGL(pair_kernel<B>, dim3(cgrid(c->cnp)), dim3(kCB), 0, s, c->cnp, c->d_chpos, pc, D, L,
                                    U, b, c->d_csaved, c->d_cflag);
-    HIPCHK(c, hipEventRecord(c->cev[1], s));
+    if (timed) HIPCHK(c, hipEventRecord(c->cev[1], s));
     int64_t top = 1;
     for (int64_t st = 1; st < nb; st *= 2) {
         const int64_t nel = (nb - st + 2 * st - 1) / (2 * st), nkeep = (nb + 2 * st - 1) / (2 * st);
@@ -2806,14 +2837,14 @@ int enqueue_chain_solve(Ctx* c, const double* rhs) {
         top = 2 * st;
     }
     hipLaunchKernelGGL(top_kernel<B>, dim3(1), dim3(64), 0, s, D, b, c->d_cflag);
-    HIPCHK(c, hipEventRecord(c->cev[2], s));
+    if (timed) HIPCHK(c, hipEventRecord(c->cev[2], s));
     for (int64_t st = top / 2; st >= 1; st /= 2) {
         const int64_t nel = (nb - st + 2 * st - 1) / (2 * st);
         hipLaunchKernelGGL(back_kernel<B>, dim3(cgrid(nel * B)), dim3(kCB), 0, s, nb, st, L, U, b);
     }
     hipLaunchKernelGGL(unpack_kernel<B>, dim3(cgrid(c->nX)), dim3(kCB), 0, s, c->nX, nb, c->d_cpos, c->d_chpos, pc,
                        c->d_csaved, b, c->d_dx);
-    HIPCHK(c, hipEventRecord(c->cev[3], s));
+    if (timed) HIPCHK(c, hipEventRecord(c->cev[3], s));
     HIPCHK(c, hipGetLastError());
     return MH_OK;
 }
@@ -2904,6 +2935,7 @@ int mh_chain_prepare(mh_ctx* c, int B, int64_t nblock, const int64_t* pos, int64
     if (npair) HIPCHK(c, copy_sync(c, c->d_chpos, h32.data(), sizeof(int32_t) * npair, hipMemcpyHostToDevice));
     if (ndummy) HIPCHK(c, copy_sync(c, c->d_cdum, d32.data(), sizeof(int32_t) * ndummy, hipMemcpyHostToDevice));
     c->cB = B; c->cnb = nblock; c->cnp = npair; c->cnd = ndummy;
+    ++c->gen;   // a captured Newton iteration holds the old chain buffers
     const int r = B == 6 ? build_chain_slots<6>(c) : build_chain_slots<3>(c);
     if (r != MH_OK) {
         const std::string e = c->err;
@@ -2914,9 +2946,14 @@ int mh_chain_prepare(mh_ctx* c, int B, int64_t nblock, const int64_t* pos, int64
 }
 
 // Reads the status words of the last chain solve (stream synchronized by the caller's copy).
+static int chain_status_read(Ctx* c);
 static int chain_status(Ctx* c) {
     HIPCHK(c, hipMemcpyAsync(c->h_cflag, c->d_cflag, chain::kStatusBytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return chain_status_read(c);
+}
+// The status words already in pinned host memory (after the stream has been synchronized).
+static int chain_status_read(Ctx* c) {
     const unsigned f = c->h_cflag[0];
     const unsigned long long* where = reinterpret_cast<const unsigned long long*>(c->h_cflag + 2);
     if (f & chain::kBadPair) {
@@ -3028,23 +3065,85 @@ int mh_newton_prepare(mh_ctx* c, const double* scaleX) {
     HIPCHK(c, hipMalloc(&c->d_norm, sizeof(double) * 2));
     HIPCHK(c, hipHostMalloc((void**)&c->h_norm, sizeof(double) * 2, hipHostMallocDefault));
     if (c->nX) HIPCHK(c, copy_sync(c, c->d_scaleX, scaleX, sizeof(double) * c->nX, hipMemcpyHostToDevice));
+    ++c->gen;   // a captured Newton iteration holds the old buffers
+    return MH_OK;
+}
+
+// Newmarkβdecrement!{OX} of the device X with the last Δx, and the norms into pinned memory
+// (kernels and a copy on the stream: also nodes of a captured Newton iteration).
+// The coefficients are read from d_c6 (upload_c6 first).
+static int enqueue_newton_update(Ctx* c, int OX, int first) {
+    hipStream_t s = c->stream;
+    const int64_t ng = cgrid(std::max<int64_t>(c->nX, 1));
+    hipLaunchKernelGGL(chain::newmark_kernel, dim3(ng), dim3(chain::kCB), 0, s, c->nX, OX, first, c->d_c6, c->d_dx,
+                       c->d_rhs, c->d_scaleX, c->d_X[0], c->d_X[1], c->d_X[2], c->d_part);
+    hipLaunchKernelGGL(chain::norm_kernel, dim3(1), dim3(chain::kCB), 0, s, ng, c->d_part, c->d_norm);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->h_norm, c->d_norm, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+    return MH_OK;
+}
+static int check_newton_args(Ctx* c, int OX, const double* c6, const double* norms) {
+    if (!c || !c->cB || !c->d_scaleX || OX < 0 || OX > 2 || (OX > 0 && !c6) || !norms) return MH_ERR_ARG;
+    if (OX > 0 && !c->d_X[OX]) { c->err = "X derivatives of order " + std::to_string(OX) + " are not on the device"; return MH_ERR_ARG; }
     return MH_OK;
 }
 
 int mh_newton_update(mh_ctx* c, int OX, int first, const double* c6, double* norms) {
-    if (!c || !c->cB || !c->d_scaleX || OX < 0 || OX > 2 || (OX > 0 && !c6) || !norms) return MH_ERR_ARG;
-    if (OX > 0 && !c->d_X[OX]) { c->err = "X derivatives of order " + std::to_string(OX) + " are not on the device"; return MH_ERR_ARG; }
+    if (int r = check_newton_args(c, OX, c6, norms)) return r;
     HIPCHK(c, hipSetDevice(c->device));
+    if (int r = upload_c6(c, OX, c6)) return r;
+    if (int r = enqueue_newton_update(c, OX, first)) return r;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    norms[0] = c->h_norm[0];
+    norms[1] = c->h_norm[1];
+    return MH_OK;
+}
+
+int mh_newton_iteration_graph(mh_ctx* c, int OX, int first, const double* c6, double* norms, mh_err* err) {
+    if (int r = check_sweepx_args(c, OX, (first && OX > 0) ? 0 : 1, c6, err)) return r;
+    if (int r = check_newton_args(c, OX, c6, norms)) return r;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->beam_tuned) {   // before any capture: the tuner times launches
+        if (int r = tune_beams(c)) return r;
+        ++c->gen;
+    }
+    const bool step = first && OX > 0;
+    if (int r = upload_c6(c, OX, c6)) return r;
+    const double key[9] = {0, 0, 0, 0, 0, 0, 0, 0, reads_time(c) ? c->t : 0.};
+    const int sl = first ? 3 : 2;
+    bool same = c->gexec[sl] && c->gOX[sl] == OX && c->ggen[sl] == c->gen;
+    for (int i = 0; i < 9 && same; ++i) same = std::memcmp(&key[i], &c->gkey[sl][i], sizeof(double)) == 0;
     hipStream_t s = c->stream;
-    const double z[6] = {0, 0, 0, 0, 0, 0};
-    const double* k = OX > 0 ? c6 : z;
-    const int64_t ng = cgrid(std::max<int64_t>(c->nX, 1));
-    hipLaunchKernelGGL(chain::newmark_kernel, dim3(ng), dim3(chain::kCB), 0, s, c->nX, OX, first, k[0], k[1], k[2], k[3],
-                       k[4], k[5], c->d_dx, c->d_rhs, c->d_scaleX, c->d_X[0], c->d_X[1], c->d_X[2], c->d_part);
-    hipLaunchKernelGGL(chain::norm_kernel, dim3(1), dim3(chain::kCB), 0, s, ng, c->d_part, c->d_norm);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(c->h_norm, c->d_norm, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipStreamSynchronize(s));
+    if (!same) {
+        free_graph(c, sl);
+        HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int r = enqueue_sweepx(c, OX, step, c6, false);
+        if (r == MH_OK) r = c->cB == 6 ? enqueue_chain_solve<6>(c, c->d_rhs, false) : enqueue_chain_solve<3>(c, c->d_rhs, false);
+        if (r == MH_OK) {
+            const hipError_t e = hipMemcpyAsync(c->h_cflag, c->d_cflag, chain::kStatusBytes, hipMemcpyDeviceToHost, s);
+            if (e != hipSuccess) { c->err = std::string("capture: ") + hipGetErrorString(e); r = MH_ERR_HIP; }
+        }
+        if (r == MH_OK) r = enqueue_newton_update(c, OX, first);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(s, &g);
+        if (r != MH_OK || ec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (r == MH_OK) c->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(ec);
+            return r != MH_OK ? r : MH_ERR_HIP;
+        }
+        c->graph[sl] = g;
+        HIPCHK(c, hipGraphInstantiateWithFlags(&c->gexec[sl], c->graph[sl], 0));
+        std::memcpy(c->gkey[sl], key, sizeof(key));
+        c->gOX[sl] = OX; c->ggen[sl] = c->gen;
+        ++c->graph_captures;
+    }
+    HIPCHK(c, hipEventRecord(c->ev[0], s));
+    HIPCHK(c, hipGraphLaunch(c->gexec[sl], s));
+    HIPCHK(c, hipEventRecord(c->ev[3], s));
+    c->timed = true;          // mh_last_timings: the whole iteration
+    c->graph_timed = true;
+    if (int r = finish_sweepx(c, err)) return r;          // syncs the stream; NaN of the assembly
+    if (int r = chain_status_read(c)) return r;           // pivots of the solve
     norms[0] = c->h_norm[0];
     norms[1] = c->h_norm[1];
     return MH_OK;

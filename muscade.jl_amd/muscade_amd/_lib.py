@@ -70,6 +70,7 @@ SIGNATURES = {
     "mh_get_dx": (_I, [_P, _D]),
     "mh_newton_prepare": (_I, [_P, _D]),
     "mh_newton_update": (_I, [_P, _I, _I, _D, _D]),
+    "mh_newton_iteration_graph": (_I, [_P, _I, _I, _D, _D, ctypes.POINTER(mh_err)]),
     "mh_get_X": (_I, [_P, _I, _D]),
     "mh_beam_results": (_I, [_P, _I, _I, _I, _D, _D]),
     "mh_set_host_blocks": (_I, [_P, _I, _D, _D]),

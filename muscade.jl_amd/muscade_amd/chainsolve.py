@@ -20,6 +20,7 @@ closed loop, is not a chain: `chain_layout` raises NotAChain and the host factor
 remains the way to solve it.
 """
 
+import ctypes
 import re
 
 import numpy as np
@@ -184,6 +185,23 @@ class ChainSolver:
         x = np.zeros(self.out.nX)
         self.out.ctx.check(lib.mh_get_dx(self.out.ctx.h, dptr(x)), "Δx")
         return x
+
+    def iteration_graph(self, OX, first, c6, dbg=None):
+        """One whole device Newton iteration (assembly, solve, Newmark decrement) replayed from a
+        captured HIP graph (mh_newton_iteration_graph); returns (Σ Δx², Σ Lλ²)."""
+        c6 = as_f64(c6)
+        err = _lib.mh_err()
+        r = lib.mh_newton_iteration_graph(self.out.ctx.h, OX, int(first), dptr(c6), dptr(self.norms), ctypes.byref(err))
+        d = dict(dbg or {})
+        if r == _lib.MH_ERR_NAN:
+            E = self.out.model.eletypes[err.igroup]
+            raise MuscadeError(f"residual({E.key[0]},...) returned NaN in R, FB or derivatives",
+                               dict(d, ieletyp=err.igroup + 1, iele=err.iele + 1))
+        if r == _lib.MH_ERR_SOLVE:
+            d["reason"] = self.failure()
+            raise MuscadeError("matrix factorization failed at step=%s, iiter=%s" % (d.get("step"), d.get("iiter")), d)
+        self.out.ctx.check(r, "Newton iteration")
+        return float(self.norms[0]), float(self.norms[1])
 
     def newton_update(self, OX, first, c6):
         """Newmarkβdecrement!{OX} of the device state; returns (Σ Δx², Σ Lλ²)."""

@@ -36,10 +36,12 @@ class SweepX(metaclass=_SweepXMeta):
             maxLλ=math.inf, saveiter=False, device=0, Assembly=AssemblySweepX, graph=True, linsolve="host", stats=None):
         """`Assembly` is the assembler type prepared for the model, as the type argument of
         prepare(AssemblySweepX{OX},model,dis) (SweepX.jl:24,188). graph: the device assembler
-        replays captured HIP graphs for the Newton iterations of each time step.
+        replays captured HIP graphs for the Newton iterations of each time step; with
+        linsolve="device" each whole iteration (assembly, solve, decrement) is one graph
+        (graph="assembly": the assembly alone, the solve and decrement launched eagerly).
         linsolve: "host" (SuperLU) or "device" (chain topologies only; raises
-        chainsolve.NotAChain otherwise). stats (a dict) receives the Newton iteration count and
-        the seconds spent preparing the assembler (and the chain layout)."""
+        chainsolve.NotAChain otherwise). stats (a dict) receives the Newton iteration count, the
+        seconds spent preparing the assembler (and the chain layout) and the graph captures."""
         OX = cls.OX
         model, dis = initialstate.model, initialstate.dis
         if linsolve not in ("host", "device"):
@@ -47,10 +49,14 @@ class SweepX(metaclass=_SweepXMeta):
         dev = linsolve == "device"
         if dev and Assembly is not AssemblySweepX:
             raise MuscadeError("linsolve='device' solves the device assembler's system (Assembly=AssemblySweepX)")
-        kw = {"graph": graph} if Assembly is AssemblySweepX else {}
+        if graph not in (True, False, "assembly"):
+            raise MuscadeError("graph must be True, False or 'assembly'")
+        kw = {"graph": bool(graph)} if Assembly is AssemblySweepX else {}
         t_prep = _time.perf_counter()
         out = Assembly(OX, model, dis, device=device, **kw)
         solver = ChainSolver(out) if dev else None
+        # the whole Newton iteration as one replayed graph: device solve, no host element types
+        whole = dev and graph is True and not getattr(out, "host_types", [])
         if stats is not None:
             stats["prepare_s"] = _time.perf_counter() - t_prep
         nX = out.nX
@@ -77,10 +83,13 @@ class SweepX(metaclass=_SweepXMeta):
                     stats["iterations"] = citer
                 first = iiter == 1
                 d = dict(dbg, solver="SweepX", step=step, iiter=iiter)
-                if dev:
+                if dev and whole:
+                    Δx2, Lλ2 = solver.iteration_graph(OX, first, c.array(), d)
+                elif dev:
                     out.assemble(state, "step" if first else "iter", dbg=d, fetch=False, upload=False)
                     solver.solve(d)
                     Δx2, Lλ2 = solver.newton_update(OX, first, c.array())
+                if dev:
                     if not math.isfinite(Δx2):
                         raise MuscadeError("matrix factorization failed at step=%i, iiter=%i" % (step, iiter))
                     conv = Δx2 <= cΔx2 and Lλ2 <= cLλ2
@@ -144,6 +153,8 @@ class SweepX(metaclass=_SweepXMeta):
                 if iiter == maxiter:
                     raise MuscadeError("no convergence of step %3d after %3d iterations |Δx|=%g / %g, |Lλ|=%g / %g"
                                        % (step, iiter, math.sqrt(Δx2), maxΔx, math.sqrt(Lλ2), maxLλ))
+        if stats is not None and hasattr(out, "graph_captures"):
+            stats["graph_captures"] = out.graph_captures()
         if verbose:
             print("\n    nel=%d, ndof=%d, nstep=%d, niter=%d, niter/nstep=%5.2f"
                   % (model.getnele(), nX, len(time), citer, citer / max(len(time), 1)))

@@ -135,7 +135,9 @@ inline double datan2(double a, double b) { return std::atan2(a, b); }
 template <int N, class T> inline Dual<N, T> datan2(const Dual<N, T>& a, const Dual<N, T>& b) {
     // (a.dx*b.x-b.dx*a.x)/(a.x^2+b.x^2)  (Adiff.jl:223)
     Dual<N, T> r; r.x = datan2(a.x, b.x); T d = ipow(a.x, 2) + ipow(b.x, 2);
-    for (int i = 0; i < N; ++i) r.dx[i] = (a.dx[i] * b.x - b.dx[i] * a.x) / d; return r; }
+    for (int i = 0; i < N; ++i) r.dx[i] = (a.dx[i] * b.x - b.dx[i] * a.x) / d;
+    return r;
+}
 
 
 // a variable with unit seed in partial i (0-based) — ∂ℝ{P,N}(x,i,s)

@@ -1,7 +1,9 @@
 """Captured HIP-graph assembly (mh_assemble_sweepx_graph) against the eager launches of
 mh_assemble_sweepx: bitwise equal rhs and nzval on every replay, with the program's own
-eager work and synchronisations between the replays, recaptured when the time or the
-Newmark coefficients change (one graph per mission)."""
+eager work and synchronisations between the replays, recaptured when the Newmark coefficients
+or, with bars (whose kernel takes the time), the time change (one graph per mission); and
+whole device Newton iterations as graphs (mh_newton_iteration_graph): the same SweepX
+trajectories bit for bit as the eager iterations, two captures for a run at constant Δt."""
 import numpy as np
 import pytest
 
@@ -41,7 +43,7 @@ def test_graph_static_lattice(mc):
     model, state = build_lattice(6, 5, 4, seed=14, rot=5e-2, disp=1e-2)
     rng = np.random.default_rng(1)
     g = check_replays(mc, model, state, 0, ["iter"] * 4, rng, [0.0, 1.0])
-    assert g.graph_captures() == 2   # one per time value
+    assert g.graph_captures() == 1   # no kernel of these element kinds takes the time
 
 
 def test_graph_dynamic_beams_and_bars(mc):
@@ -50,12 +52,13 @@ def test_graph_dynamic_beams_and_bars(mc):
     rng = np.random.default_rng(2)
     model, state, *_ = dyn_beams(mc, 80, seed=91, rotscale=0.2)
     g = check_replays(mc, model, state, 2, ["step", "iter", "iter", "iter"], rng, [0.3, 0.35], scale=1e-3)
-    assert g.graph_captures() == 4   # (:step, :iter) × two time steps
+    assert g.graph_captures() == 2   # :step and :iter; the beams' kernels do not take the time
     model = tower(mc, nlev=3, w=50.)
     state = mc.initialize(model, nXder=3)
     for k in range(3):
         state.X[k][:] = rng.normal(size=model.ndof["X"]) * 1e-2
-    check_replays(mc, model, state, 2, ["step", "iter", "iter"], rng, [-6.0, -5.95], scale=1e-3)
+    g = check_replays(mc, model, state, 2, ["step", "iter", "iter"], rng, [-6.0, -5.95], scale=1e-3)
+    assert g.graph_captures() == 4   # (:step, :iter) × two time values: the bars' kernel takes the time
 
 
 def test_graph_solver_matches_eager(mc):
@@ -68,3 +71,28 @@ def test_graph_solver_matches_eager(mc):
     b = mc.solve(mc.SweepX[0], graph=True, **kw)
     for sa, sb in zip(a, b):
         assert np.array_equal(bits(sa.X[0]), bits(sb.X[0]))
+
+
+def test_newton_iteration_graph_matches_eager(mc):
+    """SweepX{2} of hanging chains with the device solve: whole Newton iterations replayed from
+    graphs give the trajectory of the eager iterations (and of graphed assemblies) bit for bit."""
+    from muscade_amd.workloads import catenary_chains
+    model, state = catenary_chains(4, 30, 60, seed=3, disp=0., rot=0., vel=0., acc=0., motion=True)
+    state.time = 0.
+    # uneven steps: the Newmark coefficients change from step to step
+    steps = np.array([0.05, 0.05, 0.04, 0.06, 0.05, 0.03, 0.05, 0.05])
+    kw = dict(initialstate=state, time=np.cumsum(steps), maxΔx=1e-6, maxiter=20, verbose=False,
+              catcherror=False, linsolve="device")
+    runs = {}
+    for g in (False, "assembly", True):
+        st = {}
+        runs[g] = (mc.solve(mc.SweepX[2], graph=g, stats=st, **kw), st)
+    ref = runs[False][0]
+    for g in ("assembly", True):
+        for sa, sb in zip(ref, runs[g][0]):
+            for d in range(3):
+                assert np.array_equal(bits(sa.X[d]), bits(sb.X[d])), (g, sa.time, d)
+    assert runs[True][1]["iterations"] == runs[False][1]["iterations"]
+    # the first and the later iterations of a step, each captured once for the run: the graphs
+    # read the Newmark coefficients from device memory
+    assert runs[True][1]["graph_captures"] == 2 and runs["assembly"][1]["graph_captures"] == 2
