@@ -399,9 +399,9 @@ def adjoint_bench(args):
 def catenary_motion_bench(args):
     """The catenary chains in time (SweepX{2}, Newmark-β with β = 1/4, γ = 1/2, dt = 0.05 s): each
     chain's top moved horizontally, a·sin(2πt/T) with a ~ U(0.5, 3) m, T ~ U(6, 15) s, from rest
-    in the as-meshed shape. Newton iterations with graph-replayed assembly, the device chain solve
-    and the device Newmark decrement; X stays on the device through the steps. `--motion-steps`
-    steps are timed (the reference runs 1000)."""
+    in the as-meshed shape. Whole Newton iterations (assembly, device chain solve, device Newmark
+    decrement) replayed from HIP graphs; X stays on the device through the steps and is fetched
+    at the last one (savestep). `--motion-steps` steps are timed (the reference runs 1000)."""
     import muscade_amd as mc
     from muscade_amd.workloads import catenary_chains
     t0 = time.perf_counter()
@@ -414,17 +414,17 @@ def catenary_motion_bench(args):
     times = dt * np.arange(1, args.motion_steps + 1)
     st = {}
     t0 = time.perf_counter()
-    out = mc.solve(mc.SweepX[2], initialstate=state, time=times, maxΔx=1e-5, maxiter=20, verbose=False,
-                   catcherror=False, linsolve="device", graph=True, stats=st)
+    out = mc.solve(mc.SweepX[2], initialstate=state, time=times, maxΔx=1e-5, maxiter=20, verbose=args.motion_progress,
+                   catcherror=False, linsolve="device", graph=True, stats=st, savestep=args.motion_steps)
     el = time.perf_counter() - t0
     top_nodes = np.concatenate([E.cat_nodes()[:, 0] + 1 for E in model.eletypes if E.cls is mc.DofMotion])
     tops = model.getdofID("X", "t1", nodID=top_nodes)
     last = out[-1]
     return {"workload": f"catenary chains x{args.chains} ({nbeam} EulerBeam3D), top moved a*sin(2*pi*t/T), "
-                        f"SweepX{{2}} dt=0.05 s from rest, device chain solve, graph-replayed assembly",
+                        f"SweepX{{2}} dt=0.05 s from rest, device chain solve, whole Newton iterations as HIP graphs",
             "steps": int(args.motion_steps), "seconds_incl_prepare": el, "prepare_s": st.get("prepare_s"),
             "steps_per_s": args.motion_steps / (el - st.get("prepare_s", 0.)),
-            "newton_iterations": st.get("iterations"),
+            "newton_iterations": st.get("iterations"), "graph_captures": st.get("graph_captures"),
             "iterations_per_step": st.get("iterations", 0) / args.motion_steps,
             "projected_1000_steps_s": st.get("prepare_s", 0.) + (el - st.get("prepare_s", 0.)) / args.motion_steps * 1000,
             "top_t1_at_last_step_m": {"min": float(last.X[0][tops - 1].min()), "max": float(last.X[0][tops - 1].max())}}
@@ -682,6 +682,8 @@ def main():
     ap.add_argument("--no-motion", action="store_true", help="skip the catenary chains run in time")
     ap.add_argument("--only-motion", action="store_true", help="measure only the catenary chains run in time")
     ap.add_argument("--motion-steps", type=int, default=10)
+    ap.add_argument("--motion-progress", action="store_true",
+                    help="print the solver's line per converged step (long motion runs)")
     ap.add_argument("--only-design-point", action="store_true", help="measure only the batched limit-state gradients")
     ap.add_argument("--models", type=int, default=512)
     ap.add_argument("--model-min", type=int, default=500)

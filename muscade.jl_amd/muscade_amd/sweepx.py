@@ -33,7 +33,8 @@ class SweepX(metaclass=_SweepXMeta):
 
     @classmethod
     def run(cls, pstate, verbose, dbg, *, time, initialstate, β=0.25, γ=0.5, maxiter=50, maxΔx=1e-5,
-            maxLλ=math.inf, saveiter=False, device=0, Assembly=AssemblySweepX, graph=True, linsolve="host", stats=None):
+            maxLλ=math.inf, saveiter=False, device=0, Assembly=AssemblySweepX, graph=True, linsolve="host", stats=None,
+            savestep=1):
         """`Assembly` is the assembler type prepared for the model, as the type argument of
         prepare(AssemblySweepX{OX},model,dis) (SweepX.jl:24,188). graph: the device assembler
         replays captured HIP graphs for the Newton iterations of each time step; with
@@ -41,9 +42,19 @@ class SweepX(metaclass=_SweepXMeta):
         (graph="assembly": the assembly alone, the solve and decrement launched eagerly).
         linsolve: "host" (SuperLU) or "device" (chain topologies only; raises
         chainsolve.NotAChain otherwise). stats (a dict) receives the Newton iteration count, the
-        seconds spent preparing the assembler (and the chain layout) and the graph captures."""
+        seconds spent preparing the assembler (and the chain layout) and the graph captures.
+        savestep (an extension; the reference keeps every step): keep the states of every
+        savestep-th step and of the last one, the others left None. A long run of a large model
+        then holds a few states on the host, and with linsolve="device" X leaves the device only
+        at the kept steps."""
         OX = cls.OX
         model, dis = initialstate.model, initialstate.dis
+        if int(savestep) != savestep or savestep < 1:
+            raise MuscadeError("savestep must be a positive integer")
+        nstep = len(time)
+
+        def kept(step):
+            return step % savestep == 0 or step == nstep
         if linsolve not in ("host", "device"):
             raise MuscadeError(f"linsolve must be 'host' or 'device', not {linsolve!r}")
         dev = linsolve == "device"
@@ -93,7 +104,7 @@ class SweepX(metaclass=_SweepXMeta):
                     if not math.isfinite(Δx2):
                         raise MuscadeError("matrix factorization failed at step=%i, iiter=%i" % (step, iiter))
                     conv = Δx2 <= cΔx2 and Lλ2 <= cLλ2
-                    if saveiter or conv:
+                    if saveiter or (conv and kept(step)):
                         solver.fetch_X(state, OX)
                     if saveiter:
                         states[iiter - 1] = state.copy()
@@ -101,7 +112,7 @@ class SweepX(metaclass=_SweepXMeta):
                         if verbose:
                             print("    step %3d converged in %3d iterations. |Δx|=%7.1e |Lλ|=%7.1e"
                                   % (step, iiter, math.sqrt(Δx2), math.sqrt(Lλ2)))
-                        if not saveiter:
+                        if not saveiter and kept(step):
                             states[step - 1] = state.copy()
                         break
                     if iiter == maxiter:
@@ -147,7 +158,7 @@ class SweepX(metaclass=_SweepXMeta):
                     if verbose:
                         print("    step %3d converged in %3d iterations. |Δx|=%7.1e |Lλ|=%7.1e"
                               % (step, iiter, math.sqrt(Δx2), math.sqrt(Lλ2)))
-                    if not saveiter:
+                    if not saveiter and kept(step):
                         states[step - 1] = state.copy()
                     break
                 if iiter == maxiter:

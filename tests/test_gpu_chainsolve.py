@@ -151,6 +151,13 @@ def test_sweepx_dynamic_device_linsolve(mc):
         for k in range(3):
             assert relerr(sd.X[k][disp], sh.X[k][disp]) <= 1e-5, k
     assert np.abs(devs[-1].X[0][disp]).max() > 1e-2   # the chains moved
+    # kept steps only: X leaves the device at steps 2, 4 and 5, bitwise the same states
+    some = mc.solve(mc.SweepX[2], linsolve="device", savestep=2, **kw)
+    for k, (a, b) in enumerate(zip(devs, some), start=1):
+        if k in (2, 4, 5):
+            assert all(np.array_equal(a.X[d], b.X[d]) for d in range(3)), k
+        else:
+            assert b is None
 
 
 def test_refuses_lattice(mc):

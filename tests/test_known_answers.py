@@ -64,6 +64,18 @@ def test_cantilever_c0_elastica_oracle(mc):
     check_elastica(mc, states)
 
 
+def test_savestep_keeps_every_kth_and_the_last_step(mc):
+    _, every = cantilever_steps(mc, OracleAssembly)
+    _, some = cantilever_steps(mc, OracleAssembly, savestep=4)
+    for k, (a, b) in enumerate(zip(every, some), start=1):
+        if k % 4 == 0 or k == 10:
+            assert b.time == a.time and np.array_equal(b.X[0], a.X[0])
+        else:
+            assert b is None
+    with pytest.raises(mc.MuscadeError):
+        cantilever_steps(mc, OracleAssembly, savestep=0)
+
+
 def bend_model(mc):
     """examples/StaticBeamAnalysis.jl:10-33."""
     R, nel = 100., 8
