@@ -520,6 +520,111 @@ def design_point_bench(args):
             "tip_displacement_m": {"min": float(eq.X[0][tipj - 1].min()), "max": float(eq.X[0][tipj - 1].max())}}
 
 
+def gauge_identification_bench(args):
+    """The all-steps adjoint gradient of a load and stiffness identification from strain
+    gauges, DirectXUA{2,0,1}: one catenary of `--gauge-nel` A-scaled beams (the chains' section
+    and shape, EI scaled by 10^a per segment, 64 segments), a distributed load per length on
+    every node (3 U-dofs per element), 50 bending-strain gauges (ElementCost, σ = 1e-6) over
+    `--gauge-steps` steps of 0.05 s, priors on loads and parameters. The measured strains come
+    from a forward SweepX{2} run with the true parameters and load (device chain solve), plus
+    N(0, 1e-6) noise. Timed: L and ∂L/∂Λ, ∂L/∂X, ∂L/∂U, ∂L/∂A over all steps (AllStepsGradient:
+    one assembly and one adjoint launch for every step's beams, the host Lagrangians per step),
+    with central differences (h = 1e-6) at a few entries of each class."""
+    import muscade_amd as mc
+    from muscade_amd.results import ElementResults
+    from muscade_amd.workloads import gauge_identification, gauge_identification_truth, gauge_strains
+    nel, nstep, dt, nseg, ng = args.gauge_nel, args.gauge_steps, 0.05, 64, 50
+    t0 = time.perf_counter()
+    fwd, A, U, states = gauge_identification_truth(nel, nseg, nstep, dt, linsolve="device", graph=True)
+    t_fwd = time.perf_counter() - t0
+    log(f"gauge identification: forward SweepX{{2}} of {nel} beams over {nstep} steps {t_fwd:.1f}s")
+    model0, gidx = gauge_identification(nel, nseg, ng, targets=np.zeros((nstep, ng)))
+    st = mc.initialize(model0)
+    it = next(i for i, E in enumerate(model0.eletypes) if E.cls is mc.ElementCost)
+    G = np.concatenate([p["gauges"] for p in model0.eletypes[it].params])
+    res_ = ElementResults(model0, st.dis)
+    meas = []
+    for s_ in states:
+        st.X[0][:] = s_.X[0]
+        v, _, _ = res_.gauge_derivatives(it, st)
+        meas.append(gauge_strains(v, G)[:, 0])
+    rng = np.random.default_rng(1003)
+    meas = np.array(meas) + rng.normal(0., 1e-6, (nstep, ng))
+    model, _ = gauge_identification(nel, nseg, ng, targets=meas)
+    dis = mc.initialize(model).dis
+    nX = model.ndof["X"]
+    time_ = dt * np.arange(1, nstep + 1)
+    X = np.array([s_.X[0] for s_ in states])
+    Λ = rng.normal(0., 1e-3, (nstep, nX))
+    Uv = np.tile(0.9 * U, (nstep, 1))
+    Av = np.zeros(model.ndof["A"])
+    t0 = time.perf_counter()
+    grad = mc.AllStepsGradient(2, 0, 1, model, dis, nstep, dt)
+    t_prep = time.perf_counter() - t0
+    L, g = grad(Λ, X, Uv, Av, time_)
+    n = max(2, args.steps // 8)
+    t0 = time.perf_counter()
+    dev = host = 0.
+    for _ in range(n):
+        L, g = grad(Λ, X, Uv, Av, time_)
+        dev += grad.timings["device_s"]
+        host += grad.timings["host_s"]
+    el = (time.perf_counter() - t0) / n
+    log(f"gauge identification: gradient {el:.3f}s (device part {dev / n:.3f}s, host Lagrangians {host / n:.3f}s)")
+    # h = 1e-6 relative to the entry (at least 1e-6); the rounding of L bounds what a difference
+    # quotient can resolve: about eps·|L|/h, reported beside the error
+    fd_err = {}
+    for cls, k, i in (("X", nstep // 2, 6 * (nel // 3) + 4), ("U", nstep // 3, 3 * (nel // 2) + 1), ("A", None, nseg // 2)):
+        base = Av[i] if cls == "A" else {"X": X, "U": Uv}[cls][k, i]
+        h = 1e-6 * max(1., abs(base))
+        fd = 0.
+        for sgn in (1., -1.):
+            v = {"Λ": Λ, "X": X.copy(), "U": Uv.copy(), "A": Av.copy()}
+            if cls == "A":
+                v["A"][i] += sgn * h
+            else:
+                v[cls][k, i] += sgn * h
+            fd += sgn * grad(v["Λ"], v["X"], v["U"], v["A"], time_)[0]
+        fd /= 2 * h
+        got = g["A"][i] if cls == "A" else g[cls][k, i]
+        cmax = np.abs(g[cls]).max()
+        fd_err[cls] = {"h": h, "adjoint": float(got), "central_difference": float(fd),
+                       "err_vs_class_max": float(abs(got - fd) / cmax),
+                       "rounding_floor_vs_class_max": float(np.finfo(float).eps * abs(L) / h / cmax)}
+    log(f"gauge identification: central differences {fd_err}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, ROOT)
+        from tests import _oracle as orc
+        E, d = model.eletypes[0], dis.dis[0]
+        xyz, nodes = model.coords(), E.cat_nodes()
+        o2 = np.concatenate([p["orient2"] for p in E.params])
+        mat = E.cls.scaled_section(np.concatenate([p["mat"] for p in E.params]), Av[d.A[:, 0] - 1],
+                                   E.params[0]["amask"])
+        Xd = mc.directxua.step_derivatives(X, 2, dt)
+        k = nstep // 2
+        ns = min(args.cpu_sample_dynamic, nel)
+        t0 = time.perf_counter()
+        for e in range(ns):
+            idx = d.X[e] - 1
+            R, Ks, KU = orc.beam_diffed_residual(xyz[nodes[e, 0], :3], xyz[nodes[e, 1], :3], o2[e], mat[e],
+                                                 [x[k][idx] for x in Xd], Uv[k][d.U[e] - 1])
+            gx = [Λ[k][idx] @ K_ for K_ in Ks]
+        dts = time.perf_counter() - t0
+        per = dts / ns
+        cpu = {"value": 1. / (per * nel * nstep), "unit": "gradients/s", "cores": 1, "kind": "port",
+               "sample": f"{ns} beams of step {k} ({dts:.1f} s): per beam the nested-dual Jacobians (X, X', X'', U) "
+                         f"contracted with Lambda, scaled to {nel} beams x {nstep} steps; the host Lagrangians are "
+                         f"not included, so this bounds the CPU rate from above"}
+        log(f"gauge identification cpu baseline: {cpu['value']:.3g} gradients/s (element part)")
+    return {"workload": f"load and stiffness identification from 50 strain gauges, DirectXUA{{2,0,1}}: "
+                        f"{nel} EulerBeam3DAU catenary, {nseg} EI parameters, 3 load dofs per node, {nstep} steps",
+            "unknowns": nstep * (2 * nX + model.ndof["U"]) + model.ndof["A"],
+            "forward_sweepx_s": t_fwd, "prepare_s": t_prep, "gradients": n, "s_per_gradient": el,
+            "device_s_per_gradient": dev / n, "host_s_per_gradient": host / n, "L": L,
+            "central_differences": fd_err, "cpu_baseline": cpu}
+
+
 def graph_bench(args):
     """Per-assembly time of small models (launch-bound), eager launches vs one captured HIP
     graph replayed (the Newton iterations of a time step): the 100-beam cantilever of the
@@ -685,6 +790,10 @@ def main():
     ap.add_argument("--motion-progress", action="store_true",
                     help="print the solver's line per converged step (long motion runs)")
     ap.add_argument("--only-design-point", action="store_true", help="measure only the batched limit-state gradients")
+    ap.add_argument("--no-gauge", action="store_true", help="skip the gauge identification gradient")
+    ap.add_argument("--only-gauge", action="store_true", help="measure only the gauge identification gradient")
+    ap.add_argument("--gauge-nel", type=int, default=5000)
+    ap.add_argument("--gauge-steps", type=int, default=200)
     ap.add_argument("--models", type=int, default=512)
     ap.add_argument("--model-min", type=int, default=500)
     ap.add_argument("--model-max", type=int, default=20000)
@@ -711,6 +820,9 @@ def main():
         return
     if args.only_design_point:
         print(json.dumps({"design_point": design_point_bench(args)}))
+        return
+    if args.only_gauge:
+        print(json.dumps({"gauge_identification": gauge_identification_bench(args)}))
         return
     if args.only_graph:
         print(json.dumps({"graph": graph_bench(args)}))
@@ -868,6 +980,7 @@ def main():
     # secondary sections: a solver error in one is reported in its place and does not stop the line
     for flag, key, fn in (("no_dynamic", "dynamic", dynamic_bench), ("no_adjoint", "adjoint", adjoint_bench),
                           ("no_design_point", "design_point", design_point_bench),
+                          ("no_gauge", "gauge_identification", gauge_identification_bench),
                           ("no_motion", "catenary_motion", catenary_motion_bench), ("no_graph", "graph", graph_bench),
                           ("no_newton", "newton_step", newton_bench), ("no_chain_solve", "chain_solve", chain_solve_bench),
                           ("no_directxua", "directxua", directxua_bench)):

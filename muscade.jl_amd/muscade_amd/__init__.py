@@ -14,7 +14,7 @@ from .elements import (Acost, AxisymmetricBarCrossSection, Bar3D, Bar3DU, BeamCr
                        positive, off)
 from .assemble import AssemblyDirect, AssemblySweepX, DeviceContext, Newmarkβcoefficients
 from .sweepx import SweepX, solve
-from .directxua import DirectXUA
+from .directxua import AllStepsGradient, DirectXUA
 from .hostcost import Dual2, exp10
 from .results import getresult, request
 from .adjoint import LimitStateGradient
@@ -26,5 +26,5 @@ __all__ = ["LIBPATH", "lib", "Model", "MuscadeError", "State", "Disassembler", "
            "AxisymmetricBarCrossSection", "Bar3D", "Bar3DU",
            "DofCost", "SingleDofCost", "SingleUdof", "Acost", "SingleAcost", "DirectXUA", "Dual2", "exp10",
            "StrainGaugeOnEulerBeam3D", "ResidualElement", "ElementCost", "ElementConstraint", "DofConstraint", "DofMotion", "equal", "positive", "off",
-           "getresult", "request", "LimitStateGradient",
+           "getresult", "request", "LimitStateGradient", "AllStepsGradient",
            "AssemblyDirect", "AssemblySweepX", "DeviceContext", "Newmarkβcoefficients", "SweepX", "solve"]

@@ -101,8 +101,9 @@ class AssemblySweepX:
 
     def __init__(self, OX, model, dis, device=0, graph=False):
         """graph=True: each assemble! replays one captured HIP graph of the element kernels and
-        scatters (recaptured when the Newmark coefficients or the time change, i.e. once per
-        time step of the Newton loop); results are bitwise those of the eager launches."""
+        scatters, one per mission (recaptured when the dof scales change, or the time when
+        bars are present: their kernel takes it); results are bitwise those of the eager
+        launches."""
         self.OX = OX
         self.graph = graph
         self.model, self.dis = model, dis
@@ -222,12 +223,14 @@ class AssemblySweepX:
         self.ctx.check(lib.mh_graph_captures(self.ctx.h, ctypes.byref(n)), "graph captures")
         return n.value
 
-    def vjp(self, state, Λ, OX=None, dbg=None):
+    def vjp(self, state, Λ, OX=None, dbg=None, upload=True):
         """Gradients of L = Σ_e Λ_eᵀ R_e(X,U,A,t) (getlagrangian, Assemble.jl:707-736) at `state`:
         returns ([∂L/∂X, ∂L/∂X′, …] up to order OX, ∂L/∂U, ∂L/∂A), unscaled model coordinates.
-        Computed by the device adjoint kernels (include/muscade_hip.h, mh_vjp_lagrangian)."""
+        Computed by the device adjoint kernels (include/muscade_hip.h, mh_vjp_lagrangian).
+        upload=False: at the state already on the device (the last assembly's)."""
         OX = self.OX if OX is None else OX
-        self.set_state(state, state.time)
+        if upload:
+            self.set_state(state, state.time)
         self.ctx.check(lib.mh_set_lambda(self.ctx.h, dptr(as_f64(Λ))), "set Λ")
         err = _lib.mh_err()
         r = lib.mh_vjp_lagrangian(self.ctx.h, OX, ctypes.byref(err))

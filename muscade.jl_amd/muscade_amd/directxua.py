@@ -224,13 +224,16 @@ class HostLagrangians:
             yield p, d, slice(e0, e0 + m)
             e0 += m
 
-    def step(self, state, Λgradient=True):
+    def step(self, state, Λgradient=True, pieces=True):
         """Per-step contributions: L1[α][der] dense vectors, and a list of COO pieces
-        (α, i, β, j, rows, cols, vals) of L2 (0-based model dofs). Λgradient=False: host residual
-        elements with no_second_order add only L1[Λ] = R to the gradient, as the reference's
-        residual path does (DirectXUA.jl:104-119)."""
+        (α, i, β, j, rows, cols, vals) of L2 (0-based model dofs; none with pieces=False).
+        Λgradient=False: host residual elements with no_second_order add only L1[Λ] = R to the
+        gradient, as the reference's residual path does (DirectXUA.jl:104-119). With
+        pieces=False the partials are first order (no Hessians are formed). self.value is left
+        holding the step's summed Lagrangian value."""
         L1 = {c: [np.zeros(self.ndof[c]) for _ in range(self.nder[c])] for c in CLASSES}
         L2 = []
+        value = 0.
         for it in self.types:
             E = self.model.eletypes[it]
             once = self._once_off(E)
@@ -255,7 +258,8 @@ class HostLagrangians:
                 def seeded(c, der, idx, vals, scale):
                     for (cc, dd, s0, _) in layout:
                         if cc == c and dd == der:
-                            return [Dual2.seed(vals[:, k], s0 + k, npar, scale[k]) for k in range(idx.shape[1])]
+                            return [Dual2.seed(vals[:, k], s0 + k, npar, scale[k], first_order=not pieces)
+                                    for k in range(idx.shape[1])]
                     return [vals[:, k] for k in range(idx.shape[1])]
 
                 Λ = seeded("Λ", 0, iX, state.Λ[0][iX], d.scaleΛ)
@@ -273,15 +277,19 @@ class HostLagrangians:
                 else:
                     L = E.cls.lagrangian(p, Λ, X, U, A, state.time)
                 if not isinstance(L, Dual2):
+                    value += float(np.sum(L))
                     continue
-                if not np.all(np.isfinite(L.g)) or not np.all(np.isfinite(L.h)):
+                if not np.all(np.isfinite(L.g)) or (L.h is not None and not np.all(np.isfinite(L.h))):
                     raise MuscadeError(f"lagrangian({E.key[0]},...) returned NaN",
                                        dict(ieletyp=it + 1, t=state.time))
+                value += float(np.sum(L.v))
                 rpath = (not Λgradient and getattr(E.cls, "residual_host", False) and E.cls.no_second_order)
                 for (a, i, sa, ia) in layout:
                     na = ia.shape[1]
                     if not rpath or a == "Λ":
                         np.add.at(L1[a][i], ia, L.g[:, sa:sa + na])
+                    if not pieces:
+                        continue
                     for (b, j, sb, ib) in layout:
                         if a == "Λ" and b == "Λ":
                             continue
@@ -290,6 +298,7 @@ class HostLagrangians:
                         rows = np.broadcast_to(ia[:, :, None], (m, na, nb)).ravel()
                         cols = np.broadcast_to(ib[:, None, :], (m, na, nb)).ravel()
                         L2.append((a, i, b, j, rows, cols, H.ravel()))
+        self.value = value
         return L1, L2
 
     def once(self, state):
@@ -297,6 +306,7 @@ class HostLagrangians:
         gradient and Hessian over the unscaled A-dofs."""
         gA = np.zeros(self.ndof["A"])
         pieces = []
+        self.once_value = 0.
         for it in self.types:
             E = self.model.eletypes[it]
             if not self._once_off(E):
@@ -307,7 +317,9 @@ class HostLagrangians:
                 A = [Dual2.seed(state.A[iA[:, k]], k, Na) for k in range(Na)]
                 C = E.cls.acost(p, A)
                 if not isinstance(C, Dual2):
+                    self.once_value += float(np.sum(C))
                     continue
+                self.once_value += float(np.sum(C.v))
                 np.add.at(gA, iA, C.g)
                 rows = np.broadcast_to(iA[:, :, None], (m, Na, Na)).ravel()
                 cols = np.broadcast_to(iA[:, None, :], (m, Na, Na)).ravel()
@@ -315,11 +327,11 @@ class HostLagrangians:
         return gA, pieces
 
 
-def _add_beam_dRdA(out, it, d, Λ, L1, L2, Λgradient):
+def _add_beam_dRdA(D, d, Λ, L1, L2, Λgradient):
     """The A column of the residual path for an A-scaled beam type (R reads A, as the
     reference's test Spring does, test/SomeElements.jl:162-186): L2[Λ,A] = ∂R/∂A·sΛ·sA and its
-    transpose L2[A,Λ], and, with Λgradient, L1[A] += (L2[Λ,A])ᵀΛ as for X and U."""
-    D = out.beam_dRdA(it)                                   # nele × 12, unscaled
+    transpose L2[A,Λ], and, with Λgradient, L1[A] += (L2[Λ,A])ᵀΛ as for X and U. D: ∂R/∂a of
+    the type's elements at this step (nele × 12, unscaled; mh_beam_dRdA)."""
     rows = d.X - 1                                          # the Λ dofs of the element (its X dofs)
     col = np.repeat(d.A[:, 0] - 1, rows.shape[1])
     vals = (D * d.scaleΛ[None, :] * d.scaleA[0]).ravel()
@@ -339,24 +351,27 @@ class _Replica:
             self._nodes = np.concatenate([E.cat_nodes() + s * nnod for s in range(n)])
             self.params = [p for s in range(n) for p in E.params]
             self.nodes = [q for s in range(n) for q in E.nodes]
+            self.nele = E.nele * n
 
         def cat_nodes(self):
             return self._nodes
 
     class _Dis:
-        def __init__(self, d, n, nX, nU):
+        def __init__(self, d, n, nX, nU, shareA):
             self.X = np.concatenate([d.X + s * nX for s in range(n)])
             self.U = np.concatenate([d.U + s * nU for s in range(n)])
-            self.A = np.zeros((self.X.shape[0], 0), np.int64)
+            # A-dofs are the model's, shared by every copy (one A for all steps)
+            self.A = np.tile(d.A, (n, 1)) if shareA else np.zeros((self.X.shape[0], 0), np.int64)
             self.scaleX, self.scaleU, self.scaleA, self.scaleΛ = d.scaleX, d.scaleU, d.scaleA, d.scaleΛ
 
     def __init__(self, model, dis, types, n):
         nX, nU = model.ndof["X"], model.ndof["U"]
+        shareA = any(model.eletypes[it].params and model.eletypes[it].params[0].get("amask") for it in types)
         self.nnod = model.nnod * n
         self._xyz = np.tile(model.coords(), (n, 1))
-        self.ndof = {"X": nX * n, "U": nU * n, "A": 0}
+        self.ndof = {"X": nX * n, "U": nU * n, "A": model.ndof["A"] if shareA else 0}
         self.eletypes = [self._Type(model.eletypes[it], n, model.nnod) for it in types]
-        self.dis = [self._Dis(dis.dis[it], n, nX, nU) for it in types]
+        self.dis = [self._Dis(dis.dis[it], n, nX, nU, shareA) for it in types]
 
     def coords(self):
         return self._xyz
@@ -381,7 +396,8 @@ class StepBatchAssembly:
         rdis.dis = self.rep.dis
         self.nstep, self.nX, self.nU = nstep, model.ndof["X"], model.ndof["U"]
         self.OX, self.OU = OX, OU
-        self.out = AssemblyDirect(OX, OU, IA, self.rep, rdis, device=device, skip=(), nA=0)
+        self.rindex = {it: k for k, it in enumerate(types)}
+        self.out = AssemblyDirect(OX, OU, IA, self.rep, rdis, device=device, skip=())
 
     def assemble_steps(self, states, dbg=None):
         """L1, L2 of every step (lists over steps, as AssemblyDirect.assemble returns them)."""
@@ -391,6 +407,7 @@ class StepBatchAssembly:
         st.time = states[0].time
         st.X = [np.concatenate([s.X[d] for s in states]) for d in range(self.OX + 1)]
         st.U = [np.concatenate([s.U[d] for s in states]) for d in range(self.OU + 1)]
+        st.A = states[0].A
         lam = np.concatenate([s.Λ[0] for s in states])
         L1, L2 = self.out.assemble(st, lam, "matrices", 1.0, dbg=dbg)
         n = {"Λ": self.nX, "X": self.nX, "U": self.nU}
@@ -403,6 +420,12 @@ class StepBatchAssembly:
                 l2[(a, b)] = [[M[ra, rb] for M in row] for row in blocks]
             res.append((l1, l2))
         return res
+
+    def beam_dRdA(self, it):
+        """∂R/∂a of the A-scaled beam type it (a model index) at every step of the last
+        `assemble_steps`: a list over steps of nele × 12 arrays (mh_beam_dRdA)."""
+        D = self.out.beam_dRdA(self.rindex[it])
+        return np.split(D, self.nstep)
 
 
 class DeviceBigSystem:
@@ -519,6 +542,132 @@ class DeviceBigSystem:
         return sparser(M)   # as the host path: sparser!(cLvv, Lvv, 1e-20) before the LU
 
 
+def step_derivatives(V, order, Δt):
+    """V⁽ᵈ⁾ of every step for d = 0..order (V: nstep × n), by the finite-difference kernels of
+    src/FiniteDifferences.jl:8-34 (the derivatives DirectXUA's steps hold)."""
+    n = V.shape[0]
+    out = [V]
+    for d in range(1, order + 1):
+        W = np.zeros_like(V)
+        s = Δt ** (-d)
+        for k in range(1, n + 1):
+            for Δs, w in finitediff(d, n, k):
+                W[k - 1] += (w * s) * V[k - 1 + Δs]
+        out.append(W)
+    return out
+
+
+def step_pullback(G, Δt):
+    """The transpose of step_derivatives: Σ_d of the derivative-level gradients G[d] (nstep × n)
+    mapped onto the steps, as assemblebig! adds L1 into Lv (src/DirectXUA.jl:336-357)."""
+    n = G[0].shape[0]
+    out = np.zeros_like(G[0])
+    for d, Gd in enumerate(G):
+        s = Δt ** (-d)
+        for k in range(1, n + 1):
+            for Δs, w in finitediff(d, n, k):
+                out[k - 1 + Δs] += (w * s) * Gd[k - 1]
+    return out
+
+
+class _StepState:
+    """The per-step state HostLagrangians reads (model, dis, time, Λ, X, U, A, SP)."""
+
+    def __init__(self, model, dis, time, Λ, X, U, A, SP):
+        self.model, self.dis, self.time = model, dis, time
+        self.Λ, self.X, self.U, self.A, self.SP = Λ, X, U, A, SP
+
+
+class AllStepsGradient:
+    """The gradient of DirectXUA{OX,OU,IA}'s all-steps Lagrangian at given steps, by the device
+    adjoint kernels, without the all-steps matrix:
+
+        L = Σₖ [Λₖ·Rₖ(Xₖ, X′ₖ, X″ₖ, Uₖ, A) + host Lagrangians of step k] (+ once-off A costs),
+
+    with X′, X″ (and U′, U″) the finite differences of the steps, as assemblebig! applies them
+    (src/DirectXUA.jl:336-383, src/FiniteDifferences.jl:2-34). The residual elements (beams)
+    of all steps are disjoint copies in one SweepX context whose A-dofs are shared: one
+    assembly gives every Rₖ, one mh_vjp_lagrangian every Λₖ·∂Rₖ/∂X⁽ᵈ⁾ and Λₖ·∂Rₖ/∂Uₖ, and
+    Σₖ Λₖ·∂Rₖ/∂A, recomputing each element's internal state instead of storing it. The host
+    elements (constraints, loads, costs, gauge costs, A priors) add their gradients from
+    HostLagrangians. The derivative-level gradients are mapped onto the steps' Xₖ through the
+    finite-difference weights, the chain rule assemblebig! applies to Lv.
+
+    Returns unscaled gradients: at unit dof scales, the all-steps Lv of solve(DirectXUA) with
+    Λgradient=True, class by class. There is no counterpart in the reference (which forms Lv
+    with forward-mode duals in its Newton loop)."""
+
+    def __init__(self, OX, OU, IA, model, dis, nstep, Δt, device=0):
+        from .assemble import AssemblySweepX
+        self.OX, self.OU, self.IA = OX, OU, IA
+        self.model, self.dis, self.nstep, self.Δt = model, dis, int(nstep), float(Δt)
+        self.nX, self.nU, self.nA = model.ndof["X"], model.ndof["U"], model.ndof["A"]
+        if OX > 0 and self.nstep < 6:
+            raise MuscadeError("Number of steps must be ≥6")
+        self.host = HostLagrangians(model, dis, OX, OU, IA)
+        types = [it for it in range(len(model.eletypes)) if it not in self.host.skip]
+        for it in types:
+            if model.eletypes[it].cls.kind not in (_lib.MH_KIND_BEAM, _lib.MH_KIND_BEAM_U):
+                raise MuscadeError("the all-steps adjoint takes beams as its residual elements")
+        self.rep = _Replica(model, dis, types, self.nstep)
+
+        class _RDis:
+            pass
+        rdis = _RDis()
+        rdis.dis = self.rep.dis
+        self.out = AssemblySweepX(OX, self.rep, rdis, device=device)
+        if self.host.costed:
+            from .results import ElementResults
+            self.host.gauge_provider = ElementResults(model, dis, device).gauge_derivatives
+        self.timings = {}
+
+    def __call__(self, Λ, X, U, A, time):
+        """Λ, X (nstep × nX), U (nstep × nU): each step's multipliers, X and U; A (nA); time
+        (nstep). Returns (L, dict of ∂L/∂Λ, ∂L/∂X, ∂L/∂U (nstep × n each) and ∂L/∂A (nA))."""
+        clock = _time.perf_counter
+        t0 = clock()
+        n, nX, nU, dis = self.nstep, self.nX, self.nU, self.dis
+        Λ, X = np.asarray(Λ, float).reshape(n, nX), np.asarray(X, float).reshape(n, nX)
+        U = np.asarray(U, float).reshape(n, nU)
+        A = np.asarray(A, float)
+        Xd = step_derivatives(X, self.OX, self.Δt)
+        Ud = step_derivatives(U, self.OU, self.Δt)
+        st = _StepState(self.rep, None, float(time[0]), [Λ.ravel()], [x.ravel() for x in Xd], [Ud[0].ravel()], A, None)
+        self.out.assemble(st, "iter", fetch=False)
+        R = np.zeros(n * nX)
+        self.out.ctx.check(lib.mh_get_rhs(self.out.ctx.h, dptr(R)), "rhs")
+        gXd, gU0, gA = self.out.vjp(st, st.Λ[0], self.OX, upload=False)
+        t1 = clock()
+        R = R.reshape(n, nX)
+        L = float(np.sum(Λ * R))
+        gΛ = R.copy()
+        gXd = [g.reshape(n, nX).copy() for g in gXd]
+        gUd = [gU0.reshape(n, nU).copy()] + [np.zeros((n, nU)) for _ in range(self.OU)]
+        gA = gA.copy()
+        sc = {"Λ": dis.scaleΛ, "X": dis.scaleX, "U": dis.scaleU, "A": dis.scaleA}
+        for k in range(n):
+            sk = _StepState(self.model, dis, float(time[k]), [Λ[k]], [x[k] for x in Xd], [u[k] for u in Ud], A,
+                            dict(γ=0., iter=1))
+            L1, _ = self.host.step(sk, Λgradient=True, pieces=False)
+            L += self.host.value
+            gΛ[k] += L1["Λ"][0] / sc["Λ"]
+            for d in range(self.OX + 1):
+                gXd[d][k] += L1["X"][d] / sc["X"]
+            for d in range(self.OU + 1):
+                if nU:
+                    gUd[d][k] += L1["U"][d] / sc["U"]
+            if self.IA and self.nA:
+                gA += L1["A"][0] / sc["A"]
+        if self.IA and self.nA:
+            g1, _ = self.host.once(_StepState(self.model, dis, float(time[0]), [Λ[0]], [X[0]], [U[0]], A, None))
+            gA += g1
+            L += self.host.once_value
+        grad = {"Λ": gΛ, "X": step_pullback(gXd, self.Δt), "U": step_pullback(gUd, self.Δt) if nU else np.zeros((n, 0)),
+                "A": gA}
+        self.timings = {"device_s": t1 - t0, "host_s": clock() - t1}
+        return L, grad
+
+
 class _NoDeviceKinds:
     """The per-step assembler of a model with no device element kind: nothing to add."""
 
@@ -596,8 +745,6 @@ class DirectXUA(metaclass=_DirectXUAMeta):
         ascaled = [it for it, E in enumerate(model.eletypes) if it not in host.skip and E.params
                    and E.params[0].get("amask")]
         if batch_steps:
-            if ascaled:
-                raise MuscadeError("A-scaled beams (EulerBeam3DA) are assembled one step per launch (batch_steps=False)")
             batched = [StepBatchAssembly(OX, OU, IA, model, dis, n, skip=tuple(host.skip), device=device)
                        for n in nstep]
             provider = None
@@ -650,6 +797,7 @@ class DirectXUA(metaclass=_DirectXUAMeta):
                 if batch_steps:
                     t0 = clock()
                     stepblocks = batched[iexp].assemble_steps(state[iexp], dbg=dict(dbg, iter=it))
+                    dRdA = {ia: batched[iexp].beam_dRdA(ia) for ia in ascaled} if IA else {}
                     t_dev += clock() - t0
                 for istep in range(1, nstep[iexp] + 1):
                     st = state[iexp][istep - 1]
@@ -668,8 +816,9 @@ class DirectXUA(metaclass=_DirectXUAMeta):
                             for bder, M in enumerate(L2d[pair][0][:nder[b]]):
                                 L1h[b][bder] = L1h[b][bder] + M.T @ st.Λ[0]
                     if IA:
-                        for it in ascaled:
-                            _add_beam_dRdA(out, it, dis.dis[it], st.Λ[0], L1h, L2h, Λgradient)
+                        for ia in ascaled:
+                            D = dRdA[ia][istep - 1] if batch_steps else out.beam_dRdA(ia)
+                            _add_beam_dRdA(D, dis.dis[ia], st.Λ[0], L1h, L2h, Λgradient)
                     for b in classes:
                         for bder in range(nder[b]):
                             v = L1h[b][bder]

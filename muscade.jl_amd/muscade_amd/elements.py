@@ -192,9 +192,8 @@ class ElementCost(EulerBeam3D):
         p["_gd"] = (values (m,4), gradients (m,4,12), Hessians (m,4,12,12)) of (εₐₓ, κ₁, κ₂, κ₃)
         w.r.t. the element's X-dofs at this state (the Hessians dropped unless p["exact"])."""
         v, g, H = p["_gd"]
-        if not p.get("exact", False):
-            H = np.zeros_like(H)
-        q = [compose(v[:, k], g[:, k], H[:, k], X[0][:12]) for k in range(4)]
+        exact = p.get("exact", False)
+        q = [compose(v[:, k], g[:, k], H[:, k] if exact else None, X[0][:12]) for k in range(4)]
         G = p["gauges"]                                   # m, ngauge, 4
         eps = [((G[:, k, 0] * q[0] + G[:, k, 1] * q[1]) + G[:, k, 2] * q[2]) + G[:, k, 3] * q[3]
                for k in range(G.shape[1])]
@@ -457,7 +456,7 @@ class DofConstraint:
             R, K = DofConstraint.residual_blocks(p, vals, t, γ)
             L = 0.
             for k in range(Nx + 1):
-                Rk = compose(R[:, k], K[:, k, :], np.zeros((R.shape[0], Nx + 1, Nx + 1)), xs)
+                Rk = compose(R[:, k], K[:, k, :], None, xs)
                 L = L + Λ[k] * Rk
             return L
         mode = p["mode"](t)
@@ -593,7 +592,8 @@ class ResidualElement:
         Up = [list(U[d]) if d < len(U) else [zero] * len(U[0]) for d in range(3)] if U and len(U[0]) else [[], [], []]
         R = cls._residual(p, Xp, Up, list(A), t)
         if cls.no_second_order:
-            R = [Dual2(r.v, r.g, np.zeros_like(r.h)) if isinstance(r, Dual2) else r for r in R]
+            R = [Dual2(r.v, r.g, None if r.h is None else np.zeros_like(r.h)) if isinstance(r, Dual2) else r
+                 for r in R]
         L = Λ[0] * R[0]
         for k in range(1, len(R)):
             L = L + Λ[k] * R[k]

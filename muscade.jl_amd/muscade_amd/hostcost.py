@@ -16,21 +16,25 @@ import numpy as np
 class Dual2:
     __array_priority__ = 100
 
-    def __init__(self, v, g, h):
+    def __init__(self, v, g, h, hzero=False):
         self.v, self.g, self.h = v, g, h
+        self.hzero = hzero          # h is known to be zero (a seeded partial)
 
     @staticmethod
-    def seed(value, slot, n, scale=1.0):
-        """The variable `value` (m,) as partial number `slot` of n, scaled: x = value + scale·δ."""
+    def seed(value, slot, n, scale=1.0, first_order=False):
+        """The variable `value` (m,) as partial number `slot` of n, scaled: x = value + scale·δ.
+        first_order: no second derivatives are carried (h = None through the arithmetic), for
+        callers that want the gradient only."""
         v = np.asarray(value, dtype=float)
         g = np.zeros(v.shape + (n,))
         g[..., slot] = scale
-        return Dual2(v, g, np.zeros(v.shape + (n, n)))
+        return Dual2(v, g, None if first_order else np.zeros(v.shape + (n, n)), hzero=True)
 
     # ---- helpers
     def _const(self, c):
         c = np.asarray(c, dtype=float)
-        return Dual2(np.broadcast_to(c, self.v.shape).astype(float), np.zeros_like(self.g), np.zeros_like(self.h))
+        return Dual2(np.broadcast_to(c, self.v.shape).astype(float), np.zeros_like(self.g),
+                     None if self.h is None else np.zeros_like(self.h), hzero=True)
 
     def _lift(self, o):
         return o if isinstance(o, Dual2) else self._const(o)
@@ -38,19 +42,22 @@ class Dual2:
     def chain(self, f, f1, f2):
         """f(x) with f, f′, f″ evaluated at x.v."""
         g1 = f1[..., None] * self.g
+        if self.h is None:
+            return Dual2(f, g1, None)
         h = f1[..., None, None] * self.h + f2[..., None, None] * (self.g[..., :, None] * self.g[..., None, :])
         return Dual2(f, g1, h)
 
     # ---- arithmetic
     def __add__(self, o):
         if not isinstance(o, Dual2):
-            return Dual2(self.v + o, self.g, self.h)
-        return Dual2(self.v + o.v, self.g + o.g, self.h + o.h)
+            return Dual2(self.v + o, self.g, self.h, self.hzero)
+        h = None if self.h is None or o.h is None else self.h + o.h
+        return Dual2(self.v + o.v, self.g + o.g, h, self.hzero and o.hzero)
 
     __radd__ = __add__
 
     def __neg__(self):
-        return Dual2(-self.v, -self.g, -self.h)
+        return Dual2(-self.v, -self.g, None if self.h is None else -self.h, self.hzero)
 
     def __sub__(self, o):
         return self + (-o)
@@ -61,9 +68,13 @@ class Dual2:
     def __mul__(self, o):
         if not isinstance(o, Dual2):
             o = np.asarray(o, dtype=float)
-            return Dual2(self.v * o, self.g * o[..., None], self.h * o[..., None, None])
+            return Dual2(self.v * o, self.g * o[..., None], None if self.h is None else self.h * o[..., None, None],
+                         self.hzero)
+        g = self.g * o.v[..., None] + o.g * self.v[..., None]
+        if self.h is None or o.h is None:
+            return Dual2(self.v * o.v, g, None)
         gg = self.g[..., :, None] * o.g[..., None, :]
-        return Dual2(self.v * o.v, self.g * o.v[..., None] + o.g * self.v[..., None],
+        return Dual2(self.v * o.v, g,
                      self.h * o.v[..., None, None] + o.h * self.v[..., None, None] + gg + np.swapaxes(gg, -1, -2))
 
     __rmul__ = __mul__
@@ -124,16 +135,23 @@ class Dual2:
 def compose(v, g, H, x):
     """y = f(x) as a Dual2 from f's value v (m,), gradient g (m, k) and Hessian H (m, k, k)
     over k inputs x (a list of Dual2 or arrays): the chain rule to second order,
-    ∇y = gᵀ∇x and ∇²y = ∇xᵀ H ∇x + Σ_k g_k ∇²x_k."""
+    ∇y = gᵀ∇x and ∇²y = ∇xᵀ H ∇x + Σ_k g_k ∇²x_k. H = None: f's Hessian left out (a
+    Gauss-Newton composition)."""
     dx = [xi for xi in x if isinstance(xi, Dual2)]
     if not dx:
         return np.asarray(v, dtype=float)
     n = dx[0].g.shape[-1]
     m = np.shape(v)[0]
     G = np.stack([xi.g if isinstance(xi, Dual2) else np.zeros((m, n)) for xi in x], axis=1)          # m,k,n
-    Hx = np.stack([xi.h if isinstance(xi, Dual2) else np.zeros((m, n, n)) for xi in x], axis=1)      # m,k,n,n
     yg = np.einsum("mk,mkn->mn", g, G)
-    yh = np.einsum("mkl,mkn,mlp->mnp", H, G, G) + np.einsum("mk,mknp->mnp", g, Hx)
+    if any(xi.h is None for xi in dx):   # first-order partials
+        return Dual2(np.asarray(v, dtype=float), yg, None)
+    # ∇xᵀ H ∇x as two contractions (H is zero for a Gauss-Newton composition), then the
+    # inputs' own Hessians (zero for seeded partials)
+    yh = np.einsum("mkn,mkp->mnp", G, np.einsum("mkl,mlp->mkp", H, G)) if H is not None else np.zeros((m, n, n))
+    for k, xi in enumerate(x):
+        if isinstance(xi, Dual2) and not xi.hzero:
+            yh += g[:, k, None, None] * xi.h
     return Dual2(np.asarray(v, dtype=float), yg, yh)
 
 
@@ -151,15 +169,18 @@ def KKT(λ, g, γ):
         return 0.
     ref = λ if lg else g
     m, n = ref.g.shape
+    first = (lg and λ.h is None) or (gg and g.h is None)
     lv = λ.v if lg else np.broadcast_to(np.asarray(λ, float), (m,))
     gv = g.v if gg else np.broadcast_to(np.asarray(g, float), (m,))
     lgr = λ.g if lg else np.zeros((m, n))
     ggr = g.g if gg else np.zeros((m, n))
-    lh = λ.h if lg else np.zeros((m, n, n))
-    gh = g.h if gg else np.zeros((m, n, n))
     s = S(lv, gv, γ)
     grad = lv[:, None] * ggr + s[:, None] * lgr
+    if first:
+        return Dual2(np.zeros(m), grad, None)
     dS = gv[:, None] * lgr + lv[:, None] * ggr
+    lh = λ.h if lg else np.zeros((m, n, n))
+    gh = g.h if gg else np.zeros((m, n, n))
     H = (ggr[:, :, None] * lgr[:, None, :] + lv[:, None, None] * gh + lgr[:, :, None] * dS[:, None, :]
          + s[:, None, None] * lh)
     return Dual2(np.zeros(m), grad, H)

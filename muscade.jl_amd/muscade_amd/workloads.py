@@ -26,6 +26,16 @@ N(0, 1) kN/m), with a state of small random displacements and rotations and a
 random adjoint vector Λ. The same models are built either one Model each (the
 plain path) or all into one Model as disconnected parts (the batch).
 
+gauge_identification: load and stiffness identification from strain gauges (DirectXUA{2,0,1}) —
+one catenary of the chains' section and shape, n_e EulerBeam3DA{true} elements whose EI₂ and
+EI₃ are scaled by 10^a per segment (nseg A-dofs), a distributed load per length on every node
+(3 U-dofs, at each element's first node), anchor and top held, and ngauge bending-strain
+gauges (offset 0.15 m, axial direction) on massless, stiffness-free sensors overlaid on beams
+along the line, each an ElementCost ((ε − ε_measured)/σ)², σ = 1e-6, with priors 1e-6·|u|² on
+the loads and a² on the parameters. The measured strains are those of a forward SweepX{2} run
+with a ~ N(0, 0.1/ln 10) and a steady load (0.1, 0.3, 0) kN/m from rest in the as-meshed
+shape, plus N(0, 1e-6) noise.
+
 design_point_batch: the batched design-point search — uneven cantilevers of L = 10 m
 (n_e ~ U{nmin..nmax} EulerBeam3DA{true} elements), whose section stiffnesses (EA, EI₂, EI₃,
 GJ) are scaled per sixteenth of the length by 10^a, a ~ N(0, 0.1/ln 10) (a lognormal
@@ -35,9 +45,10 @@ displacement along z against a threshold.
 """
 import numpy as np
 
-from .elements import (STIFFNESS_FIELDS, BeamCrossSection, DofLoad, DofMotion, EulerBeam3D, EulerBeam3DAU,
-                       EulerBeam3DU, Hold)
+from .elements import (STIFFNESS_FIELDS, BeamCrossSection, DofCost, DofLoad, DofMotion, ElementCost, EulerBeam3D,
+                       EulerBeam3DAU, EulerBeam3DU, Hold, SingleAcost, StrainGaugeOnEulerBeam3D)
 from .model import Model, addelement, addnode, initialize, setdof
+from .results import request
 
 E_STEEL = 2.1e11
 SECTION = dict(A=1e-2, I=1e-5, J=2e-5)
@@ -364,3 +375,78 @@ def design_point_equilibrium(state, nload=4, maxΔx=1e-6, maxiter=30, linsolve="
     if stats is not None:
         stats["iterations"] = total
     return st
+
+
+GAUGE_SIGMA = 1e-6
+
+
+def _gauge_cost(eleres, t, targets, dt, w):
+    """((ε − ε_measured)/σ)² of each gauge sensor at the step of time t (steps at dt, 2dt, …)."""
+    k = int(round(t / dt)) - 1
+    e = eleres["ε"][0] - targets[k]
+    return w * (e * e)
+
+
+def _load_prior(X, U, A, t, w):
+    return w * (U[0][0] * U[0][0] + U[0][1] * U[0][1] + U[0][2] * U[0][2])
+
+
+def _a_prior(a, w):
+    return w * a * a
+
+
+def gauge_identification(nel=5000, nseg=64, ngauge=50, length=1000., targets=None, dt=0.05, wU=1e-6, wA=1.):
+    """The identification model (targets: measured strains, nstep × ngauge; None: the forward
+    model, without sensors and priors). Beams, then Holds, then sensors and priors, so that
+    the X-, U- and A-dofs are numbered alike in both. Returns (model, gauge beam indices)."""
+    model = Model("gauge_identification")
+    s_ = np.linspace(0., 1., nel + 1)
+    span, depth = 0.8 * length, 0.5 * length
+    x = span * s_
+    z = depth * (np.cosh(1.5 * s_) - 1.) / (np.cosh(1.5) - 1.)
+    addnode(model, np.stack([x, np.zeros_like(s_), z], axis=1))
+    seg = np.minimum((np.arange(nel) * nseg) // nel, nseg - 1)
+    mid = (np.arange(nseg) + 0.5) / nseg
+    addnode(model, np.stack([span * mid, np.full(nseg, 10.), np.zeros(nseg)], axis=1))
+    e = np.arange(nel)
+    nodes = np.stack([e + 1, e + 2, e + 1, nel + 2 + seg], axis=1)
+    addelement(model, EulerBeam3DAU, nodes, mat=chain_section(), orient2=np.array([0., 1., 0.]),
+               scaled=("EI₂", "EI₃"), afield="ΞEI")
+    for fld in ("t1", "t2", "t3"):
+        addelement(model, Hold, np.array([[1], [nel + 1]]), field=fld)
+    gidx = np.round(np.linspace(0, nel - 1, ngauge + 2)[1:-1]).astype(np.int64)
+    if targets is not None:
+        targets = np.asarray(targets, float)
+        kw = dict(P=np.array([[0.], [0.], [0.15]]), D=np.array([[1.], [0.], [0.]]),
+                  elementkwargs=dict(mat=np.zeros(16), orient2=np.array([0., 1., 0.])))
+        addelement(model, ElementCost, np.stack([gidx + 1, gidx + 2], axis=1), req=request("ε"), cost=_gauge_cost,
+                   ElementType=StrainGaugeOnEulerBeam3D, elementkwargs=kw,
+                   costargs=(targets, dt, 1. / GAUGE_SIGMA ** 2))
+        addelement(model, DofCost, (e + 1).reshape(-1, 1), cost=_load_prior, uinod=(1, 1, 1),
+                   ufield=("t1", "t2", "t3"), costargs=(wU,))
+        addelement(model, SingleAcost, (nel + 2 + np.arange(nseg)).reshape(-1, 1), field="ΞEI", cost=_a_prior,
+                   costargs=(wA,))
+    return model, gidx
+
+
+def gauge_identification_truth(nel=5000, nseg=64, nstep=200, dt=0.05, seed=1003, load=(100., 300., 0.), **solve_kw):
+    """The truth of the identification: a ~ N(0, 0.1/ln 10) per segment, the steady load per
+    length `load` on every element, and the forward SweepX{2} run from rest in the as-meshed
+    shape (solve_kw: the assembler and linear solver). Returns (model, A, U, states)."""
+    from .sweepx import SweepX, solve
+    rng = np.random.default_rng(seed)
+    model, _ = gauge_identification(nel, nseg, targets=None)
+    st = initialize(model, time=0., nXder=3)
+    A = rng.normal(0., 0.1 / np.log(10.), nseg)
+    st.A[:] = A
+    st.U[0][:] = np.tile(np.asarray(load, float), nel)
+    times = dt * np.arange(1, nstep + 1)
+    out = solve(SweepX[2], initialstate=st, time=times, maxΔx=1e-6, maxiter=20, verbose=False, catcherror=False,
+                **solve_kw)
+    return model, A, st.U[0].copy(), out
+
+
+def gauge_strains(values, gauges):
+    """Gauge strains ε = G·(εₐₓ, κ₁, κ₂, κ₃) of the sensors: values (m, 4) of their requestables
+    at a state, gauges (m, ngauge, 4) their gauge factors (StrainGaugeOnEulerBeam3D's G)."""
+    return np.einsum("mgk,mk->mg", gauges, values)

@@ -257,3 +257,46 @@ class HostSweepXAssembly:
 
     def Lλx(self):
         return self._K.copy()
+
+
+def allsteps_lagrangian_blocks(model, dis, Λ, X, U, A, time, dt, Assembly, OX=2, OU=0, IA=1):
+    """The all-steps DirectXUA Lagrangian and its gradient by the step blocks of `Assembly`
+    (AssemblyDirect or the oracle's): the route of solve(DirectXUA)'s Lv with Λgradient,
+    L1 + (L2[Λ,·])ᵀΛ per step plus the A column ∂R/∂A, at unit dof scales. L = Σₖ Λₖ·Rₖ + the
+    host Lagrangians. Returns (L, {"Λ","X","U" (nstep × n), "A"}), as AllStepsGradient."""
+    from muscade_amd.directxua import HostLagrangians, _StepState, step_derivatives, step_pullback
+    n, nX, nU, nA = len(time), model.ndof["X"], model.ndof["U"], model.ndof["A"]
+    host = HostLagrangians(model, dis, OX, OU, IA)
+    out = Assembly(OX, OU, IA, model, dis, skip=tuple(host.skip))
+    if host.costed:
+        host.gauge_provider = out.gauge_derivatives
+    ascaled = [it for it, E in enumerate(model.eletypes) if it not in host.skip and E.params
+               and E.params[0].get("amask")]
+    Xd, Ud = step_derivatives(X, OX, dt), step_derivatives(U, OU, dt)
+    gΛ = np.zeros((n, nX))
+    gXd = [np.zeros((n, nX)) for _ in range(OX + 1)]
+    gUd = [np.zeros((n, nU)) for _ in range(OU + 1)]
+    gA = np.zeros(nA)
+    L = 0.
+    for k in range(n):
+        st = _StepState(model, dis, float(time[k]), [Λ[k]], [x[k] for x in Xd], [u[k] for u in Ud], A,
+                        dict(γ=0., iter=1))
+        L1d, L2d = out.assemble(st, Λ[k], "matrices", 1.0)
+        L1h, _ = host.step(st, Λgradient=True, pieces=False)
+        L += float(Λ[k] @ L1d["Λ"][0]) + host.value
+        gΛ[k] = L1d["Λ"][0] + L1h["Λ"][0]
+        for d in range(OX + 1):
+            gXd[d][k] = L1d["X"][d] + L2d[("Λ", "X")][0][d].T @ Λ[k] + L1h["X"][d]
+        for d in range(OU + 1):
+            if nU:
+                gUd[d][k] = L1d["U"][d] + L2d[("Λ", "U")][0][d].T @ Λ[k] + L1h["U"][d]
+        if IA:
+            gA += L1h["A"][0]
+            for it in ascaled:
+                D = out.beam_dRdA(it)
+                np.add.at(gA, dis.dis[it].A[:, 0] - 1, np.einsum("ei,ei->e", D, Λ[k][dis.dis[it].X - 1]))
+    if IA and nA:
+        g1, _ = host.once(_StepState(model, dis, float(time[0]), [Λ[0]], [X[0]], [U[0]], A, None))
+        gA += g1
+        L += host.once_value
+    return L, {"Λ": gΛ, "X": step_pullback(gXd, dt), "U": step_pullback(gUd, dt), "A": gA}

@@ -58,11 +58,12 @@ def truth(nel=6):
     return {n: np.array([s.X[0][d] for s in out]) for n, d in dofs.items()}
 
 
-def identify(meas, Assembly=OracleAssemblyDirect, **kw):
+def identify(meas, Assembly=OracleAssemblyDirect, solve_kw=None, **kw):
     model = abeam_model(measurements=meas, **kw)
     st0 = mc.initialize(model, time=0.)
-    out = mc.solve(mc.DirectXUA[0, 0, 1], initialstate=[st0], time=[TIMES], maxΔx=1e-10, maxΔλ=1e-6, maxΔa=1e-9,
-                   maxiter=40, verbose=False, catcherror=False, Assembly=Assembly)
+    skw = dict(maxΔx=1e-10, maxΔλ=1e-6, maxΔa=1e-9, maxiter=40, verbose=False, catcherror=False, Assembly=Assembly)
+    skw.update(solve_kw or {})
+    out = mc.solve(mc.DirectXUA[0, 0, 1], initialstate=[st0], time=[TIMES], **skw)
     return out[0][0].A.copy(), out
 
 
@@ -109,5 +110,6 @@ class _NoDRdA(OracleAssemblyDirect):
 
 
 def test_without_dRdA_nothing_is_identified(measurements):
-    A, _ = identify(measurements, Assembly=_NoDRdA)
+    # Λ (of order 1e6 here) then stagnates at a few 1e-6: the tolerance on it is relaxed
+    A, _ = identify(measurements, Assembly=_NoDRdA, solve_kw=dict(maxΔλ=1e-4))
     assert np.abs(A).max() <= 1e-12

@@ -1,0 +1,105 @@
+"""The gauge identification workload (DirectXUA{2,0,1}: an A-scaled catenary with a distributed
+load per length on every node, bending-strain sensors with ElementCost, priors on loads and
+parameters) on the CPU, with the oracle's step blocks:
+
+- the model's dofs: 6 per chain node plus the Holds' multipliers, 3 U per element in element
+  order, one A per segment; the forward model numbers them alike;
+- the all-steps gradient of the step-block route (solve(DirectXUA)'s Lv with Λgradient, the
+  oracle's Jacobians, stencils, host Lagrangians, ∂R/∂A) against central differences of the
+  all-steps Lagrangian, h = 1e-6, for Λ, X at the first, inner and last steps, U and A;
+- the forward truth run moves the line and the sensors read its strains.
+"""
+import numpy as np
+import pytest
+
+import muscade_amd as mc
+from muscade_amd.workloads import gauge_identification, gauge_identification_truth, gauge_strains
+from tests import _oracle as orc
+from tests.helpers import OracleAssembly, OracleAssemblyDirect, allsteps_lagrangian_blocks
+
+NEL, NSEG, NG, NSTEP, DT = 12, 3, 3, 8, 0.05
+
+
+def sensor_strains(model, state):
+    """Gauge strains of the sensors at state.X[0], by the oracle's requestables."""
+    it = next(i for i, E in enumerate(model.eletypes) if E.cls is mc.ElementCost)
+    E, d = model.eletypes[it], state.dis.dis[it]
+    xyz, nodes = model.coords(), E.cat_nodes()
+    o2 = np.concatenate([p["orient2"] for p in E.params])
+    mat = np.concatenate([p["mat"] for p in E.params])
+    v = np.zeros((E.nele, 4))
+    for e in range(E.nele):
+        v[e], _, _ = orc.beam_gauge2(xyz[nodes[e, 0], :3], xyz[nodes[e, 1], :3], o2[e], mat[e],
+                                     state.X[0][d.X[e] - 1])
+    return gauge_strains(v, np.concatenate([p["gauges"] for p in E.params]))[:, 0]
+
+
+@pytest.fixture(scope="module")
+def truth():
+    return gauge_identification_truth(NEL, NSEG, NSTEP, DT, Assembly=OracleAssembly)
+
+
+@pytest.fixture(scope="module")
+def problem(truth):
+    fwd, A, U, states = truth
+    model0, gidx = gauge_identification(NEL, NSEG, NG, targets=np.zeros((NSTEP, NG)))
+    st = mc.initialize(model0)
+    meas = []
+    for s in states:
+        st.X[0][:] = s.X[0]
+        meas.append(sensor_strains(model0, st))
+    rng = np.random.default_rng(7)
+    meas = np.array(meas) + rng.normal(0., 1e-6, (NSTEP, NG))
+    model, _ = gauge_identification(NEL, NSEG, NG, targets=meas)
+    return fwd, model, A, U, states, meas
+
+
+def test_dofs_numbered_alike(truth):
+    fwd, A, U, states = truth
+    model, gidx = gauge_identification(NEL, NSEG, NG, targets=np.zeros((NSTEP, NG)))
+    assert model.ndof == fwd.ndof == {"X": 6 * (NEL + 1) + 6, "U": 3 * NEL, "A": NSEG}
+    for f in ("t1", "t2", "t3", "r1", "r2", "r3"):
+        assert np.array_equal(model.getdofID("X", f), fwd.getdofID("X", f))
+    assert np.array_equal(model.getdofID("U", "t2"), 3 * np.arange(NEL) + 2)
+    assert gidx.size == NG and np.all(np.diff(gidx) > 0)
+
+
+def test_truth_moves_and_is_read(problem):
+    fwd, model, A, U, states, meas = problem
+    t3 = fwd.getdofID("X", "t3") - 1
+    assert np.abs(states[-1].X[0][t3]).max() > 1e-2
+    assert np.abs(meas).max() > 1e-5
+
+
+def test_block_route_gradient_is_the_difference_quotient(problem):
+    fwd, model, A, U, states, meas = problem
+    rng = np.random.default_rng(11)
+    nX, nU = model.ndof["X"], model.ndof["U"]
+    time = DT * np.arange(1, NSTEP + 1)
+    X = np.array([s.X[0] for s in states]) + rng.normal(0., 1e-4, (NSTEP, nX))
+    Λ = rng.normal(0., 1e-3, (NSTEP, nX))
+    Uv = np.tile(0.9 * U, (NSTEP, 1))
+    Av = A + rng.normal(0., 0.01, A.size)
+    st0 = mc.initialize(model)
+
+    def lag(Λ_, X_, U_, A_):
+        return allsteps_lagrangian_blocks(model, st0.dis, Λ_, X_, U_, A_, time, DT, OracleAssemblyDirect)
+
+    L, g = lag(Λ, X, Uv, Av)
+    h = 1e-6
+    cases = [("X", 0, 30), ("X", 3, 40), ("X", NSTEP - 1, 50), ("X", 1, 9), ("Λ", 2, 31), ("U", 4, 7),
+             ("A", None, 1)]
+    for cls, k, i in cases:
+        vals = {"Λ": Λ, "X": X, "U": Uv, "A": Av}
+        fd = 0.
+        for sgn in (1., -1.):
+            v = {c: a.copy() for c, a in vals.items()}
+            if cls == "A":
+                v["A"][i] += sgn * h
+            else:
+                v[cls][k, i] += sgn * h
+            fd += sgn * lag(v["Λ"], v["X"], v["U"], v["A"])[0]
+        fd /= 2 * h
+        got = g["A"][i] if cls == "A" else g[cls][k, i]
+        scale = np.abs(g[cls]).max()
+        assert abs(got - fd) <= 1e-7 * scale, (cls, k, i, got, fd, scale)
