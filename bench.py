@@ -415,7 +415,7 @@ def catenary_motion_bench(args):
     st = {}
     t0 = time.perf_counter()
     out = mc.solve(mc.SweepX[2], initialstate=state, time=times, maxΔx=1e-5, maxiter=20, verbose=args.motion_progress,
-                   catcherror=False, linsolve="device", graph=True, stats=st, savestep=args.motion_steps)
+                   catcherror=False, linsolve="device", graph=not args.eager, stats=st, savestep=args.motion_steps)
     el = time.perf_counter() - t0
     top_nodes = np.concatenate([E.cat_nodes()[:, 0] + 1 for E in model.eletypes if E.cls is mc.DofMotion])
     tops = model.getdofID("X", "t1", nodID=top_nodes)
@@ -535,7 +535,7 @@ def gauge_identification_bench(args):
     from muscade_amd.workloads import gauge_identification, gauge_identification_truth, gauge_strains
     nel, nstep, dt, nseg, ng = args.gauge_nel, args.gauge_steps, 0.05, 64, 50
     t0 = time.perf_counter()
-    fwd, A, U, states = gauge_identification_truth(nel, nseg, nstep, dt, linsolve="device", graph=True)
+    fwd, A, U, states = gauge_identification_truth(nel, nseg, nstep, dt, linsolve="device", graph=not args.eager)
     t_fwd = time.perf_counter() - t0
     log(f"gauge identification: forward SweepX{{2}} of {nel} beams over {nstep} steps {t_fwd:.1f}s")
     model0, gidx = gauge_identification(nel, nseg, ng, targets=np.zeros((nstep, ng)))
@@ -790,6 +790,8 @@ def main():
     ap.add_argument("--motion-progress", action="store_true",
                     help="print the solver's line per converged step (long motion runs)")
     ap.add_argument("--only-design-point", action="store_true", help="measure only the batched limit-state gradients")
+    ap.add_argument("--eager", action="store_true",
+                    help="SweepX solves with eager launches instead of replayed HIP graphs (for tracing tools)")
     ap.add_argument("--no-gauge", action="store_true", help="skip the gauge identification gradient")
     ap.add_argument("--only-gauge", action="store_true", help="measure only the gauge identification gradient")
     ap.add_argument("--gauge-nel", type=int, default=5000)

@@ -756,6 +756,7 @@ struct VjpArgs {
     const double* lam;
     double t;
     const double* Kc;    // dynamic beams: unscaled compact static part (value pass)
+    const double* Kb;    // dynamic beams: ε-free kinematics (value pass), mh::kDynBase per element
     double* g;           // g[der*gstride + i*nele + e]
     int64_t gstride;
     double* gu;          // gu[q*nele + e] (U-dof groups)
@@ -861,59 +862,64 @@ __global__ __launch_bounds__(kBeamBlock) void beam_vjp_a_kernel(VjpArgs a, int m
     }
 }
 
-// Dynamic beams, 16 lanes per element: lane 12 forms y = J_x Λ, J_v Λ (shared through LDS)
-// and ∂L/∂U; lane k < 12 differentiates the loads along X⁽ᵈᵉʳ⁾ₖ for every order and
-// contracts with y, adding the static part (Kc Λ)ₖ for der = 0.
+// Dynamic beams, after the value pass with unit scales (the compact static part in Kc, the
+// ε-free kinematics in Kb): one wave per workgroup of kDirElemsW<false> elements, 6 lanes per
+// element, lane j owning one translation dof (0..2, 6..8) and one rotation dof (3..5, 9..11)
+// as in the direction pass. A lane forms y = (J_x Λ, J_v Λ), the derivatives of x_gp and vₛₘ
+// along Λ (a first-order jet seeded with Λ), then differentiates the loads along X⁽ᵈᵉʳ⁾ₖ of
+// its two dofs for every order — the translation in closed form, the rotation on time jets —
+// and contracts: ∂L/∂X⁽ᵈᵉʳ⁾ₖ = Σ_gp dL (y_gp·∂fₑ/∂ε + y_v·∂mₑ/∂ε), plus (Kc Λ)ₖ for der = 0.
+// Lane 0 also writes ∂L/∂U.
 template <int ND, bool UDOF>
-__global__ __launch_bounds__(kDirBlock) void beam_dyn_vjp_kernel(VjpArgs a) {
-    __shared__ double Ysh[kDirElems][15];
-    const int64_t e0 = xcd_tile(a.per_xcd) * kDirElems;
-    if (e0 >= a.nele) return;   // whole workgroup leaves together
-    const int el = threadIdx.x / kDirLanes, k = threadIdx.x % kDirLanes;
+__global__ __launch_bounds__(kDirWBlock) void beam_dyn_vjp_kernel(VjpArgs a) {
+    constexpr int per = kDirPerW<false>, NE = kDirElemsW<false>;
+    const int64_t e0 = xcd_tile(a.per_xcd) * NE;
+    const int el = threadIdx.x / per, j = threadIdx.x - el * per;
     const int64_t e = e0 + el;
     const int64_t n = a.nele;
-    const bool live = e < n && k <= 12;
+    if (el >= NE || e >= n) return;
+    const int dir_t = j < 3 ? j : j + 3;     // t of node 1, t of node 2
+    const int dir_r = j < 3 ? 3 + j : 6 + j; // r of node 1, r of node 2
     mh::BeamGeo G;
+    mh::BeamSection S;
     mh::BeamDynSection D;
     double X[ND][12], U[3], lam[12];
-    if (live) {
-        mh::BeamSection S;
+    {
         BeamDynArgs b;
         b.nele = a.nele; b.idx = a.idx; b.uidx = a.uidx; b.nodes = a.nodes; b.xyz = a.xyz; b.o2 = a.o2; b.sec = a.sec;
         for (int d = 0; d < 3; ++d) b.X[d] = a.X[d];
         b.U = a.U;
         dyn_load<ND, UDOF>(b, e, G, S, D, X, U);
-#pragma unroll
-        for (int i = 0; i < 12; ++i) lam[i] = a.lam[a.idx[i * n + e]];
-        if (k == 12) {
-            mh::DynY y;
-            mh::dyn_y(G, X, lam, y);
-            for (int g = 0; g < 4; ++g)
-                for (int q = 0; q < 3; ++q) Ysh[el][3 * g + q] = y.y[g][q];
-            for (int q = 0; q < 3; ++q) Ysh[el][12 + q] = y.yv[q];
-            if (UDOF) {
-                const mh::v3 gu = mh::dyn_vjp_u(G, y);
-                a.gu[e] = gu.x; a.gu[n + e] = gu.y; a.gu[2 * n + e] = gu.z;
-            }
-        }
     }
-    __syncthreads();
-    if (!live || k == 12) return;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) lam[i] = a.lam[a.idx[i * n + e]];
     mh::DynY y;
-    for (int g = 0; g < 4; ++g)
-        for (int q = 0; q < 3; ++q) y.y[g][q] = Ysh[el][3 * g + q];
-    for (int q = 0; q < 3; ++q) y.yv[q] = Ysh[el][12 + q];
+    mh::dyn_y(G, X, lam, y);
+    if (UDOF && j == 0) {
+        const mh::v3 gu = mh::dyn_vjp_u(G, y);
+        a.gu[e] = gu.x; a.gu[n + e] = gu.y; a.gu[2 * n + e] = gu.z;
+    }
     const double one[12] = {1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1.};
+    const double* kc = a.Kc + e * mh::kCompact;
     int nan = 0;
     for (int der = 0; der < ND; ++der) {
         const double c[6] = {der == 1 ? 1. : 0., 0., 0., der == 2 ? 1. : 0., 0., 0.};
+        const double w0 = der == 0 ? 1. : 0.;
         mh::DynDir d;
-        mh::dyn_direction<ND>(G, D, X, U, one, c, k, d, der == 0 ? 1. : 0.);
-        double g = mh::dyn_ycontract(G, y, d);
-        if (der == 0)
-            for (int i = 0; i < 12; ++i) g = g + mh::compact_entry(a.Kc + e * mh::kCompact, k, i) * lam[i];
-        a.g[der * a.gstride + k * n + e] = g;
-        nan |= g != g;
+        mh::dyn_translation_direction<ND>(G, D, a.Kb + e, n, one, c, dir_t, d, w0);
+        double gt = mh::dyn_ycontract(G, y, d);
+        mh::dyn_direction<ND, true>(G, D, X, U, one, c, dir_r, d, w0);
+        double gr = mh::dyn_ycontract(G, y, d);
+        if (der == 0) {
+#pragma unroll
+            for (int i = 0; i < 12; ++i) {
+                gt = gt + mh::compact_entry(kc, dir_t, i) * lam[i];
+                gr = gr + mh::compact_entry(kc, dir_r, i) * lam[i];
+            }
+        }
+        a.g[der * a.gstride + dir_t * n + e] = gt;
+        a.g[der * a.gstride + dir_r * n + e] = gr;
+        nan |= (gt != gt) | (gr != gr);
     }
     if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
 }
@@ -1520,6 +1526,7 @@ int launch_vjp(Ctx* c, const Group& g, int nd, unsigned long long gtag) {
         for (int i = 0; i < 12; ++i) { b.sR[i] = 1.; b.scale[i] = 1.; }
         for (int i = 0; i < 6; ++i) b.c[i] = 0.;
         b.Kc = c->d_Kc + g.kcbase; b.K = nullptr; b.R = nullptr; b.dR = nullptr;
+        b.Kb = c->d_Kb + g.kcbase / mh::kCompact * mh::kDynBase;
         b.flag = c->d_flag; b.gtag = gtag;
         const int64_t ntile = (g.nele + kBeamBlock - 1) / kBeamBlock;
         b.per_xcd = (ntile + kXCD - 1) / kXCD;
@@ -1532,15 +1539,16 @@ int launch_vjp(Ctx* c, const Group& g, int nd, unsigned long long gtag) {
             else hipLaunchKernelGGL((beam_dyn_value_kernel<3, false>), grv, dim3(kBeamBlock), 0, s, b);
         }
         a.Kc = b.Kc;
-        const int64_t nt = (g.nele + kDirElems - 1) / kDirElems;
+        a.Kb = b.Kb;
+        const int64_t nt = (g.nele + kDirElemsW<false> - 1) / kDirElemsW<false>;
         a.per_xcd = (nt + kXCD - 1) / kXCD;
         const dim3 gr((unsigned)(a.per_xcd * kXCD));
         if (nd == 2) {
-            if (u) hipLaunchKernelGGL((beam_dyn_vjp_kernel<2, true>), gr, dim3(kDirBlock), 0, s, a);
-            else hipLaunchKernelGGL((beam_dyn_vjp_kernel<2, false>), gr, dim3(kDirBlock), 0, s, a);
+            if (u) hipLaunchKernelGGL((beam_dyn_vjp_kernel<2, true>), gr, dim3(kDirWBlock), 0, s, a);
+            else hipLaunchKernelGGL((beam_dyn_vjp_kernel<2, false>), gr, dim3(kDirWBlock), 0, s, a);
         } else {
-            if (u) hipLaunchKernelGGL((beam_dyn_vjp_kernel<3, true>), gr, dim3(kDirBlock), 0, s, a);
-            else hipLaunchKernelGGL((beam_dyn_vjp_kernel<3, false>), gr, dim3(kDirBlock), 0, s, a);
+            if (u) hipLaunchKernelGGL((beam_dyn_vjp_kernel<3, true>), gr, dim3(kDirWBlock), 0, s, a);
+            else hipLaunchKernelGGL((beam_dyn_vjp_kernel<3, false>), gr, dim3(kDirWBlock), 0, s, a);
         }
     } else if (bar) {
         const dim3 gr((unsigned)((g.nele + kBarBlock - 1) / kBarBlock));
@@ -2462,6 +2470,7 @@ int mh_vjp_lagrangian(mh_ctx* c, int OX, mh_err* err) {
         for (auto& g : c->groups)
             if (g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U) { g.kcbase = nKc; nKc += g.nele * mh::kCompact; }
         if (nKc) HIPCHK(c, hipMalloc(&c->d_Kc, sizeof(double) * nKc));
+        if (nKc && !c->d_Kb) HIPCHK(c, hipMalloc(&c->d_Kb, sizeof(double) * (nKc / mh::kCompact * mh::kDynBase)));
     }
     hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, s, c->d_flag, ~0ull);
     HIPCHK(c, hipEventRecord(c->ev[0], s));
