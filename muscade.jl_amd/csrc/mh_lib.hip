@@ -96,6 +96,7 @@ struct Ctx {
     int64_t nK = 0, nP = 0, nR = 0, nnz = 0;
     double* d_Kc = nullptr;      // dynamic beams: scaled compact static part, 45 per element
     double* d_Kb = nullptr;      // dynamic beams (SweepX): ε-free kinematics, mh::kDynBase per element
+    double* d_ybuf = nullptr;    // dynamic beams (adjoint): y = (J_x Λ, J_v Λ), 15 per element
     double* d_c6 = nullptr;      // SweepX: the Newmark coefficients of the current call (6)
     double* h_c6 = nullptr;      // pinned staging of d_c6
     double* d_Kbuf = nullptr;
@@ -253,7 +254,7 @@ void free_prepared(Ctx* c) {
     free_graph(c);
     free_chain(c);
     free_big(c);
-    dfree(c->d_Kbuf); dfree(c->d_Rbuf); dfree(c->d_dRbuf); dfree(c->d_nzval); dfree(c->d_rhs); dfree(c->d_Kc); dfree(c->d_Kb);
+    dfree(c->d_Kbuf); dfree(c->d_Rbuf); dfree(c->d_dRbuf); dfree(c->d_nzval); dfree(c->d_rhs); dfree(c->d_Kc); dfree(c->d_Kb); dfree(c->d_ybuf);
     dfree(c->d_colptr); dfree(c->d_rowval); dfree(c->d_kseg); dfree(c->d_ksrc); dfree(c->d_rseg); dfree(c->d_rsrc);
     dfree(c->d_useg); dfree(c->d_usrc); dfree(c->d_gbuf); dfree(c->d_gX); dfree(c->d_gubuf); dfree(c->d_gU);
     dfree(c->d_aseg); dfree(c->d_asrc); dfree(c->d_gabuf); dfree(c->d_gA); dfree(c->d_crhs);
@@ -757,6 +758,8 @@ struct VjpArgs {
     double t;
     const double* Kc;    // dynamic beams: unscaled compact static part (value pass)
     const double* Kb;    // dynamic beams: ε-free kinematics (value pass), mh::kDynBase per element
+    double* yb;          // dynamic beams: y = (J_x Λ, J_v Λ), 15 per element (yb[q*nele + e])
+    double one[12] = {1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1.};   // unit dof scales
     double* g;           // g[der*gstride + i*nele + e]
     int64_t gstride;
     double* gu;          // gu[q*nele + e] (U-dof groups)
@@ -863,23 +866,14 @@ __global__ __launch_bounds__(kBeamBlock) void beam_vjp_a_kernel(VjpArgs a, int m
 }
 
 // Dynamic beams, after the value pass with unit scales (the compact static part in Kc, the
-// ε-free kinematics in Kb): one wave per workgroup of kDirElemsW<false> elements, 6 lanes per
-// element, lane j owning one translation dof (0..2, 6..8) and one rotation dof (3..5, 9..11)
-// as in the direction pass. A lane forms y = (J_x Λ, J_v Λ), the derivatives of x_gp and vₛₘ
-// along Λ (a first-order jet seeded with Λ), then differentiates the loads along X⁽ᵈᵉʳ⁾ₖ of
-// its two dofs for every order — the translation in closed form, the rotation on time jets —
-// and contracts: ∂L/∂X⁽ᵈᵉʳ⁾ₖ = Σ_gp dL (y_gp·∂fₑ/∂ε + y_v·∂mₑ/∂ε), plus (Kc Λ)ₖ for der = 0.
-// Lane 0 also writes ∂L/∂U.
+// ε-free kinematics in Kb), one element per lane: y = (J_x Λ, J_v Λ), the derivatives of x_gp
+// (4 × 3) and vₛₘ (3) along Λ (a first-order jet seeded with Λ), 15 numbers per element into
+// yb (yb[q·n + e]); the static part (Kc Λ)ₖ of ∂L/∂Xₖ into g; ∂L/∂U = −Σ_gp dL y_gp with U-dofs.
 template <int ND, bool UDOF>
-__global__ __launch_bounds__(kDirWBlock) void beam_dyn_vjp_kernel(VjpArgs a) {
-    constexpr int per = kDirPerW<false>, NE = kDirElemsW<false>;
-    const int64_t e0 = xcd_tile(a.per_xcd) * NE;
-    const int el = threadIdx.x / per, j = threadIdx.x - el * per;
-    const int64_t e = e0 + el;
+__global__ __launch_bounds__(kBlock) void beam_dyn_y_kernel(VjpArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t n = a.nele;
-    if (el >= NE || e >= n) return;
-    const int dir_t = j < 3 ? j : j + 3;     // t of node 1, t of node 2
-    const int dir_r = j < 3 ? 3 + j : 6 + j; // r of node 1, r of node 2
+    if (e >= n) return;
     mh::BeamGeo G;
     mh::BeamSection S;
     mh::BeamDynSection D;
@@ -895,33 +889,76 @@ __global__ __launch_bounds__(kDirWBlock) void beam_dyn_vjp_kernel(VjpArgs a) {
     for (int i = 0; i < 12; ++i) lam[i] = a.lam[a.idx[i * n + e]];
     mh::DynY y;
     mh::dyn_y(G, X, lam, y);
-    if (UDOF && j == 0) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) a.yb[(3 * g + q) * n + e] = y.y[g][q];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) a.yb[(12 + q) * n + e] = y.yv[q];
+    if (UDOF) {
         const mh::v3 gu = mh::dyn_vjp_u(G, y);
         a.gu[e] = gu.x; a.gu[n + e] = gu.y; a.gu[2 * n + e] = gu.z;
     }
-    const double one[12] = {1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1.};
     const double* kc = a.Kc + e * mh::kCompact;
-    int nan = 0;
-    for (int der = 0; der < ND; ++der) {
-        const double c[6] = {der == 1 ? 1. : 0., 0., 0., der == 2 ? 1. : 0., 0., 0.};
-        const double w0 = der == 0 ? 1. : 0.;
-        mh::DynDir d;
-        mh::dyn_translation_direction<ND>(G, D, a.Kb + e, n, one, c, dir_t, d, w0);
-        double gt = mh::dyn_ycontract(G, y, d);
-        mh::dyn_direction<ND, true>(G, D, X, U, one, c, dir_r, d, w0);
-        double gr = mh::dyn_ycontract(G, y, d);
-        if (der == 0) {
 #pragma unroll
-            for (int i = 0; i < 12; ++i) {
-                gt = gt + mh::compact_entry(kc, dir_t, i) * lam[i];
-                gr = gr + mh::compact_entry(kc, dir_r, i) * lam[i];
-            }
-        }
-        a.g[der * a.gstride + dir_t * n + e] = gt;
-        a.g[der * a.gstride + dir_r * n + e] = gr;
-        nan |= (gt != gt) | (gr != gr);
+    for (int k = 0; k < 12; ++k) {
+        double g = 0.;
+#pragma unroll
+        for (int i = 0; i < 12; ++i) g = g + mh::compact_entry(kc, k, i) * lam[i];
+        a.g[k * n + e] = g;
     }
-    if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+}
+
+// Then one launch per time-derivative order DER: one wave per workgroup of kDirElemsW<false>
+// elements, 6 lanes per element, lane j owning one translation dof (0..2, 6..8) and one
+// rotation dof (3..5, 9..11) as in the direction pass. A lane differentiates the loads along
+// X⁽ᴰᴱᴿ⁾ₖ of its two dofs — the rotation on time jets, then the translation in closed form — and
+// contracts with y: ∂L/∂X⁽ᴰᴱᴿ⁾ₖ = Σ_gp dL (y_gp·∂fₑ/∂ε + y_v·∂mₑ/∂ε), added to the static part
+// for DER = 0. (All orders in one launch keep three jet passes' state live at once and spill
+// about 2 KB per lane.)
+template <int ND, bool UDOF, int DER>
+__global__ __launch_bounds__(kDirWBlock) void beam_dyn_vjp_kernel(VjpArgs a) {
+    constexpr int per = kDirPerW<false>, NE = kDirElemsW<false>;
+    const int64_t e0 = xcd_tile(a.per_xcd) * NE;
+    const int el = threadIdx.x / per, j = threadIdx.x - el * per;
+    const int64_t e = e0 + el;
+    const int64_t n = a.nele;
+    if (el >= NE || e >= n) return;
+    const int dir_t = j < 3 ? j : j + 3;     // t of node 1, t of node 2
+    const int dir_r = j < 3 ? 3 + j : 6 + j; // r of node 1, r of node 2
+    mh::BeamGeo G;
+    mh::BeamSection S;
+    mh::BeamDynSection D;
+    double X[ND][12], U[3];
+    {
+        BeamDynArgs b;
+        b.nele = a.nele; b.idx = a.idx; b.uidx = a.uidx; b.nodes = a.nodes; b.xyz = a.xyz; b.o2 = a.o2; b.sec = a.sec;
+        for (int d = 0; d < 3; ++d) b.X[d] = a.X[d];
+        b.U = a.U;
+        dyn_load<ND, UDOF>(b, e, G, S, D, X, U);
+    }
+    // the rotation direction first, while nothing else is live
+    const double c[6] = {DER == 1 ? 1. : 0., 0., 0., DER == 2 ? 1. : 0., 0., 0.};
+    constexpr double w0 = DER == 0 ? 1. : 0.;
+    mh::DynDir d;
+    mh::DynY y;
+    mh::dyn_direction<ND, true>(G, D, X, U, a.one, c, dir_r, d, w0);
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) y.y[g][q] = a.yb[(3 * g + q) * n + e];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) y.yv[q] = a.yb[(12 + q) * n + e];
+    double gr = mh::dyn_ycontract(G, y, d);
+    mh::dyn_translation_direction<ND>(G, D, a.Kb + e, n, a.one, c, dir_t, d, w0);
+    double gt = mh::dyn_ycontract(G, y, d);
+    if constexpr (DER == 0) {   // the static part, from the y kernel
+        gt = a.g[dir_t * n + e] + gt;
+        gr = a.g[dir_r * n + e] + gr;
+    }
+    a.g[DER * a.gstride + dir_t * n + e] = gt;
+    a.g[DER * a.gstride + dir_r * n + e] = gr;
+    if ((gt != gt) | (gr != gr)) atomicMin(a.flag, a.gtag | (unsigned long long)e);
 }
 
 template <int ND, bool UDOF>
@@ -1540,15 +1577,38 @@ int launch_vjp(Ctx* c, const Group& g, int nd, unsigned long long gtag) {
         }
         a.Kc = b.Kc;
         a.Kb = b.Kb;
+        a.yb = c->d_ybuf + g.kcbase / mh::kCompact * 15;
+        {
+            const dim3 gy((unsigned)((g.nele + kBlock - 1) / kBlock));
+            if (nd == 2) {
+                if (u) hipLaunchKernelGGL((beam_dyn_y_kernel<2, true>), gy, dim3(kBlock), 0, s, a);
+                else hipLaunchKernelGGL((beam_dyn_y_kernel<2, false>), gy, dim3(kBlock), 0, s, a);
+            } else {
+                if (u) hipLaunchKernelGGL((beam_dyn_y_kernel<3, true>), gy, dim3(kBlock), 0, s, a);
+                else hipLaunchKernelGGL((beam_dyn_y_kernel<3, false>), gy, dim3(kBlock), 0, s, a);
+            }
+        }
         const int64_t nt = (g.nele + kDirElemsW<false> - 1) / kDirElemsW<false>;
         a.per_xcd = (nt + kXCD - 1) / kXCD;
         const dim3 gr((unsigned)(a.per_xcd * kXCD));
         if (nd == 2) {
-            if (u) hipLaunchKernelGGL((beam_dyn_vjp_kernel<2, true>), gr, dim3(kDirWBlock), 0, s, a);
-            else hipLaunchKernelGGL((beam_dyn_vjp_kernel<2, false>), gr, dim3(kDirWBlock), 0, s, a);
+            if (u) {
+                hipLaunchKernelGGL((beam_dyn_vjp_kernel<2, true, 0>), gr, dim3(kDirWBlock), 0, s, a);
+                hipLaunchKernelGGL((beam_dyn_vjp_kernel<2, true, 1>), gr, dim3(kDirWBlock), 0, s, a);
+            } else {
+                hipLaunchKernelGGL((beam_dyn_vjp_kernel<2, false, 0>), gr, dim3(kDirWBlock), 0, s, a);
+                hipLaunchKernelGGL((beam_dyn_vjp_kernel<2, false, 1>), gr, dim3(kDirWBlock), 0, s, a);
+            }
         } else {
-            if (u) hipLaunchKernelGGL((beam_dyn_vjp_kernel<3, true>), gr, dim3(kDirWBlock), 0, s, a);
-            else hipLaunchKernelGGL((beam_dyn_vjp_kernel<3, false>), gr, dim3(kDirWBlock), 0, s, a);
+            if (u) {
+                hipLaunchKernelGGL((beam_dyn_vjp_kernel<3, true, 0>), gr, dim3(kDirWBlock), 0, s, a);
+                hipLaunchKernelGGL((beam_dyn_vjp_kernel<3, true, 1>), gr, dim3(kDirWBlock), 0, s, a);
+                hipLaunchKernelGGL((beam_dyn_vjp_kernel<3, true, 2>), gr, dim3(kDirWBlock), 0, s, a);
+            } else {
+                hipLaunchKernelGGL((beam_dyn_vjp_kernel<3, false, 0>), gr, dim3(kDirWBlock), 0, s, a);
+                hipLaunchKernelGGL((beam_dyn_vjp_kernel<3, false, 1>), gr, dim3(kDirWBlock), 0, s, a);
+                hipLaunchKernelGGL((beam_dyn_vjp_kernel<3, false, 2>), gr, dim3(kDirWBlock), 0, s, a);
+            }
         }
     } else if (bar) {
         const dim3 gr((unsigned)((g.nele + kBarBlock - 1) / kBarBlock));
@@ -2471,6 +2531,12 @@ int mh_vjp_lagrangian(mh_ctx* c, int OX, mh_err* err) {
             if (g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U) { g.kcbase = nKc; nKc += g.nele * mh::kCompact; }
         if (nKc) HIPCHK(c, hipMalloc(&c->d_Kc, sizeof(double) * nKc));
         if (nKc && !c->d_Kb) HIPCHK(c, hipMalloc(&c->d_Kb, sizeof(double) * (nKc / mh::kCompact * mh::kDynBase)));
+    }
+    if (OX > 0 && !c->d_ybuf) {
+        int64_t nb = 0;   // beams, in the layout of Kc
+        for (auto& g : c->groups)
+            if (g.kind == MH_KIND_BEAM || g.kind == MH_KIND_BEAM_U) nb = std::max(nb, g.kcbase / mh::kCompact + g.nele);
+        if (nb) HIPCHK(c, hipMalloc(&c->d_ybuf, sizeof(double) * 15 * nb));
     }
     hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, s, c->d_flag, ~0ull);
     HIPCHK(c, hipEventRecord(c->ev[0], s));
