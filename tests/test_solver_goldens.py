@@ -109,9 +109,10 @@ def acost(a):
     return 0.5 * (a / .1) ** 2
 
 
-def directxua001(mc, SweepAssembly, DirectAssembly, times=None, **kw):
+def directxua001(mc, SweepAssembly, DirectAssembly, times=None, scale=None, Λscale=None, sweep_times=(0.,), **kw):
     """TestDirectXUA001.jl:7-31: two springs sharing their A-dofs, loads, holds, position
-    measurements and A priors; SweepX{0} at t = 0, then DirectXUA{0,0,1} over 0:0.1:1."""
+    measurements and A priors; SweepX{0} at t = 0, then DirectXUA{0,0,1} over 0:0.1:1.
+    scale, Λscale: setscale! before initialize (TestScale.jl:31)."""
     from tests.fixture_elements import Spring2
     model = mc.Model("TestModel")
     n1 = mc.addnode(model, [0., 0.])
@@ -130,8 +131,10 @@ def directxua001(mc, SweepAssembly, DirectAssembly, times=None, **kw):
     mc.addelement(model, mc.SingleDofCost, [n1], dofclass="X", field="tx2", cost=position_meas)
     mc.addelement(model, mc.SingleAcost, [n4], field="ΞL₀", cost=acost)
     mc.addelement(model, mc.SingleAcost, [n4], field="ΞEI", cost=acost)
+    if scale is not None or Λscale is not None:
+        mc.setscale(model, scale=scale, Λscale=Λscale)
     s0 = mc.initialize(model)
-    stateX = mc.solve(mc.SweepX[0], initialstate=s0, time=[0.], verbose=False, catcherror=False,
+    stateX = mc.solve(mc.SweepX[0], initialstate=s0, time=list(sweep_times), verbose=False, catcherror=False,
                       Assembly=SweepAssembly)
     times = times or [np.array([k / 10 for k in range(11)])]
     return mc.solve(mc.DirectXUA[0, 0, 1], initialstate=[stateX[0]] * len(times), time=times, verbose=False,
@@ -148,6 +151,24 @@ def check_directxua001(out):
 def test_directxua001_solution_golden(mc):
     from tests.helpers import OracleAssemblyDirect
     check_directxua001(directxua001(mc, HostSweepXAssembly, OracleAssemblyDirect))
+
+
+def test_scale_invariance(mc):
+    """TestScale.jl:28-48: the same model with setscale!(X: tx1 = 1, tx2 = 10, rx3 = 2; A: three
+    fields it does not have; Λscale = 2) gives the same SweepX{0} state and the same DirectXUA{0,0,1}
+    X and A (≈, Julia's default rtol √eps). The reference runs SweepX over [0, 1] unscaled and
+    [0] scaled and compares the first step."""
+    from tests.helpers import OracleAssemblyDirect
+    kw = dict(maxΔλ=.5, maxΔa=1e-4, maxΔx=1e-4)
+    sc = {"X": {"tx1": 1., "tx2": 10., "rx3": 2.}, "A": {"Δseadrag": 3., "Δskydrag": 4., "ΔL": 5.}}
+    out1 = directxua001(mc, HostSweepXAssembly, OracleAssemblyDirect, sweep_times=(0., 1.), **kw)
+    out2 = directxua001(mc, HostSweepXAssembly, OracleAssemblyDirect, scale=sc, Λscale=2., **kw)
+    rtol = np.sqrt(np.finfo(float).eps)
+    s1, s2 = out1[0][0], out2[0][0]
+    assert isapprox(s1.X[0], s2.X[0], rtol)
+    assert isapprox(s1.A, s2.A, rtol)
+    d1, d2 = out1[0][0].dis, out2[0][0].dis
+    assert np.any(np.asarray(d2.scaleX) == 10.) and not np.any(np.asarray(d1.scaleX) == 10.)   # the scaling took
 
 
 def test_directxua001_multiple_experiments_share_A(mc):
