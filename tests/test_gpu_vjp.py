@@ -77,8 +77,9 @@ def test_vjp_static_beams(mc, udof):
 
 @pytest.mark.parametrize("OX", [1, 2])
 @pytest.mark.parametrize("udof", [False, True])
-def test_vjp_dynamic_beams(mc, OX, udof):
-    model, state, *_ = dyn_beams(mc, 150, seed=30 + OX, rotscale=0.4, udof=udof)
+@pytest.mark.parametrize("nbeam", [150, 7, 253])   # whole and partial tiles of 10 elements
+def test_vjp_dynamic_beams(mc, OX, udof, nbeam):
+    model, state, *_ = dyn_beams(mc, nbeam, seed=30 + OX, rotscale=0.4, udof=udof)
     lam = np.random.default_rng(2).normal(size=model.ndof["X"])
     out = mc.AssemblySweepX(OX, model, state.dis)
     gX, gU, _ = out.vjp(state, lam)
