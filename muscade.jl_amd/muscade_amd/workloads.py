@@ -47,7 +47,7 @@ import numpy as np
 
 from .elements import (STIFFNESS_FIELDS, BeamCrossSection, DofCost, DofLoad, DofMotion, ElementCost, EulerBeam3D,
                        EulerBeam3DAU, EulerBeam3DU, Hold, SingleAcost, StrainGaugeOnEulerBeam3D)
-from .model import Model, addelement, addnode, initialize, setdof
+from .model import Model, MuscadeError, addelement, addnode, initialize, setdof
 from .results import request
 
 E_STEEL = 2.1e11
@@ -382,7 +382,10 @@ GAUGE_SIGMA = 1e-6
 
 def _gauge_cost(eleres, t, targets, dt, w):
     """((ε − ε_measured)/σ)² of each gauge sensor at the step of time t (steps at dt, 2dt, …)."""
-    k = int(round(t / dt)) - 1
+    q = t / dt
+    k = int(round(q)) - 1
+    if abs(q - round(q)) > 1e-6 or not 0 <= k < len(targets):
+        raise MuscadeError(f"no measured strains at t = {t} (steps of {dt} s, {len(targets)} of them)")
     e = eleres["ε"][0] - targets[k]
     return w * (e * e)
 
