@@ -268,7 +268,9 @@ int mh_get_X(mh_ctx* ctx, int ider, double* x);
  * first iteration of every step and one for the others (no counterpart in the reference; the loop of
  * SweepX.jl:186-221 with its lu/solve on the device). One host wait per iteration. Errors as
  * mh_assemble_sweepx (MH_ERR_NAN) and mh_chain_solve (MH_ERR_SOLVE); after either, the device X
- * already holds that iteration's update. mh_last_timings: the whole iteration. */
+ * already holds that iteration's update. mh_last_timings: the whole iteration. The assembly
+ * scatters Lλx straight into the chain solve's blocks: out.Lλx (mh_get_nzval) is not formed;
+ * Lλ (mh_get_rhs) is. */
 int mh_newton_iteration_graph(mh_ctx* ctx, int OX, int first, const double* c6, double* norms, mh_err* err);
 /* The adjoint solve of a limit state g(X) at an equilibrium (no counterpart in the reference;
  * the step before the adjoint gradient of a reliability or design-point search): with the

@@ -196,17 +196,24 @@ __global__ __launch_bounds__(kCB) void zero_slots_kernel(int64_t n, const int32_
     if (q < n) DLU[list[q]] = 0.0;
 }
 
-// One thread per nz: nzval into D|L|U by slot (coalesced reads; the rows of one column
-// are contiguous slots), λ couplings into pc[4p + w].
+// Prepare: slots as offsets into the whole chain buffer D|L|U|b|pc (a λ coupling −(4p+w+1)
+// becomes pcbase + 4p + w), so that one map serves the pack and an assembly that scatters
+// straight into the blocks.
+__global__ __launch_bounds__(kCB) void absolute_slot_kernel(int64_t nnz, int64_t pcbase, int32_t* __restrict__ slot) {
+    const int64_t k = (int64_t)blockIdx.x * kCB + threadIdx.x;
+    if (k >= nnz) return;
+    const int32_t q = slot[k];
+    if (q < 0 && q != INT32_MIN) slot[k] = (int32_t)(pcbase - (int64_t)q - 1);
+}
+
+// One thread per nz: nzval into the chain buffer by slot (coalesced reads; the rows of one
+// column are contiguous slots of D|L|U; λ couplings go to pc).
 __global__ __launch_bounds__(kCB) void pack_kernel(int64_t nnz, const int32_t* __restrict__ slot,
-                                                   const double* __restrict__ nzval, double* __restrict__ DLU,
-                                                   double* __restrict__ pc) {
+                                                   const double* __restrict__ nzval, double* __restrict__ cbuf) {
     const int64_t k = (int64_t)blockIdx.x * kCB + threadIdx.x;
     if (k >= nnz) return;
     const int32_t q = __builtin_nontemporal_load(slot + k);
-    const double v = __builtin_nontemporal_load(nzval + k);
-    if (q >= 0) DLU[q] = v;
-    else pc[-(int64_t)q - 1] = v;
+    cbuf[q] = __builtin_nontemporal_load(nzval + k);
 }
 
 // One thread per dof: Lλ into b, or into the λ pair's r_λ.

@@ -169,7 +169,7 @@ struct Ctx {
     int32_t* d_cpos = nullptr;   // nX: block position of each dof, or −(p+1) for the λ of pair p
     int32_t* d_chpos = nullptr;  // cnp: block position of each pair's held dof
     int32_t* d_cdum = nullptr;   // cnd: padding positions
-    int32_t* d_cslot = nullptr;  // nnz: slot of each nz in D|L|U, or −(4p+w+1) for a λ coupling
+    int32_t* d_cslot = nullptr;  // nnz: offset of each nz in the chain buffer (D|L|U, or pc for a λ coupling)
     int32_t* d_cunc = nullptr;   // cnu: the slots no nz covers (zeroed by each solve)
     int64_t cnu = 0;
     double* d_cbuf = nullptr;    // D | L | U (3·cnb·B²) | b (cnb·B) | pair couplings (4·cnp)
@@ -1213,10 +1213,13 @@ constexpr int kChunk = 1024;
 // plain 32-bit index. SPL segments per lane (lane t owns k0 + t + j·kTile, so the
 // writes stay coalesced) and CH contributions per LDS chunk are the shape the
 // prepare-time tuner picks (scatter_variant).
-template <bool STEP, bool CODED, int SPL = 1, int CH = kChunk, bool NTM = false>
+// IDX: segment k is written to out[dst[k]] instead of out[k] (the nz straight into the
+// chain solve's blocks).
+template <bool STEP, bool CODED, int SPL = 1, int CH = kChunk, bool NTM = false, bool IDX = false>
 __global__ __launch_bounds__(kTile) void scatter_kernel(int64_t nseg, int64_t per_xcd, const int32_t* __restrict__ seg,
                                                         const uint32_t* __restrict__ src, const double* __restrict__ buf,
-                                                        const double* __restrict__ dbuf, double* __restrict__ out) {
+                                                        const double* __restrict__ dbuf, double* __restrict__ out,
+                                                        const int32_t* __restrict__ dst = nullptr) {
     __shared__ double vals[CH];
     __shared__ double dvals[STEP ? CH : 1];
     constexpr int kSeg = kTile * SPL;
@@ -1259,8 +1262,9 @@ __global__ __launch_bounds__(kTile) void scatter_kernel(int64_t nseg, int64_t pe
     for (int j = 0; j < SPL; ++j) {
         const int64_t k = k0 + t + j * kTile;
         if (k < nseg) {
-            if (NTM) __builtin_nontemporal_store(acc[j], out + k);
-            else out[k] = acc[j];
+            const int64_t o = IDX ? (int64_t)(NTM ? __builtin_nontemporal_load(dst + k) : dst[k]) : k;
+            if (NTM) __builtin_nontemporal_store(acc[j], out + o);
+            else out[o] = acc[j];
         }
     }
 }
@@ -1269,31 +1273,38 @@ __global__ __launch_bounds__(kTile) void scatter_kernel(int64_t nseg, int64_t pe
 // out of the caches, which then hold more of the gathered element outputs).
 constexpr int kScatterVariants = 16;
 constexpr int kScatterSPL[kScatterVariants] = {1, 2, 2, 3, 4, 4, 6, 8, 1, 2, 2, 3, 4, 4, 6, 8};
-template <bool CODED>
-void launch_scatter(int variant, hipStream_t s, int64_t nseg, const int32_t* seg, const uint32_t* src, const double* buf,
-                    double* out) {
+template <bool CODED, bool IDX>
+void launch_scatter_shape(int variant, hipStream_t s, int64_t nseg, const int32_t* seg, const uint32_t* src,
+                          const double* buf, double* out, const int32_t* dst) {
     const int spl = kScatterSPL[variant];
     const int64_t per = ((nseg + kTile * spl - 1) / (kTile * spl) + kXCD - 1) / kXCD;
     const dim3 gr((unsigned)(per * kXCD));
     const double* nul = nullptr;
     switch (variant) {
-    case 0: hipLaunchKernelGGL((scatter_kernel<false, CODED, 1, 1024>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
-    case 1: hipLaunchKernelGGL((scatter_kernel<false, CODED, 2, 1024>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
-    case 2: hipLaunchKernelGGL((scatter_kernel<false, CODED, 2, 2048>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
-    case 3: hipLaunchKernelGGL((scatter_kernel<false, CODED, 3, 2048>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
-    case 4: hipLaunchKernelGGL((scatter_kernel<false, CODED, 4, 1024>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
-    case 5: hipLaunchKernelGGL((scatter_kernel<false, CODED, 4, 2048>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
-    case 6: hipLaunchKernelGGL((scatter_kernel<false, CODED, 6, 2048>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
-    case 7: hipLaunchKernelGGL((scatter_kernel<false, CODED, 8, 2048>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
-    case 8: hipLaunchKernelGGL((scatter_kernel<false, CODED, 1, 1024, true>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
-    case 9: hipLaunchKernelGGL((scatter_kernel<false, CODED, 2, 1024, true>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
-    case 10: hipLaunchKernelGGL((scatter_kernel<false, CODED, 2, 2048, true>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
-    case 11: hipLaunchKernelGGL((scatter_kernel<false, CODED, 3, 2048, true>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
-    case 12: hipLaunchKernelGGL((scatter_kernel<false, CODED, 4, 1024, true>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
-    case 13: hipLaunchKernelGGL((scatter_kernel<false, CODED, 4, 2048, true>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
-    case 14: hipLaunchKernelGGL((scatter_kernel<false, CODED, 6, 2048, true>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
-    default: hipLaunchKernelGGL((scatter_kernel<false, CODED, 8, 2048, true>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out); break;
+    case 0: hipLaunchKernelGGL((scatter_kernel<false, CODED, 1, 1024, false, IDX>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out, dst); break;
+    case 1: hipLaunchKernelGGL((scatter_kernel<false, CODED, 2, 1024, false, IDX>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out, dst); break;
+    case 2: hipLaunchKernelGGL((scatter_kernel<false, CODED, 2, 2048, false, IDX>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out, dst); break;
+    case 3: hipLaunchKernelGGL((scatter_kernel<false, CODED, 3, 2048, false, IDX>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out, dst); break;
+    case 4: hipLaunchKernelGGL((scatter_kernel<false, CODED, 4, 1024, false, IDX>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out, dst); break;
+    case 5: hipLaunchKernelGGL((scatter_kernel<false, CODED, 4, 2048, false, IDX>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out, dst); break;
+    case 6: hipLaunchKernelGGL((scatter_kernel<false, CODED, 6, 2048, false, IDX>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out, dst); break;
+    case 7: hipLaunchKernelGGL((scatter_kernel<false, CODED, 8, 2048, false, IDX>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out, dst); break;
+    case 8: hipLaunchKernelGGL((scatter_kernel<false, CODED, 1, 1024, true, IDX>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out, dst); break;
+    case 9: hipLaunchKernelGGL((scatter_kernel<false, CODED, 2, 1024, true, IDX>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out, dst); break;
+    case 10: hipLaunchKernelGGL((scatter_kernel<false, CODED, 2, 2048, true, IDX>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out, dst); break;
+    case 11: hipLaunchKernelGGL((scatter_kernel<false, CODED, 3, 2048, true, IDX>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out, dst); break;
+    case 12: hipLaunchKernelGGL((scatter_kernel<false, CODED, 4, 1024, true, IDX>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out, dst); break;
+    case 13: hipLaunchKernelGGL((scatter_kernel<false, CODED, 4, 2048, true, IDX>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out, dst); break;
+    case 14: hipLaunchKernelGGL((scatter_kernel<false, CODED, 6, 2048, true, IDX>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out, dst); break;
+    default: hipLaunchKernelGGL((scatter_kernel<false, CODED, 8, 2048, true, IDX>), gr, dim3(kTile), 0, s, nseg, per, seg, src, buf, nul, out, dst); break;
     }
+}
+// Launch one of the tuned shapes of the nz scatter; dst (optional): write segment k to out[dst[k]].
+template <bool CODED>
+void launch_scatter(int variant, hipStream_t s, int64_t nseg, const int32_t* seg, const uint32_t* src, const double* buf,
+                    double* out, const int32_t* dst = nullptr) {
+    if (dst) launch_scatter_shape<CODED, true>(variant, s, nseg, seg, src, buf, out, dst);
+    else launch_scatter_shape<CODED, false>(variant, s, nseg, seg, src, buf, out, nullptr);
 }
 
 // Long segments (many elements share a dof, e.g. a U-node loading a whole stretch of
@@ -1330,12 +1341,13 @@ __global__ __launch_bounds__(kBlock) void scaled_copy_kernel(int64_t n, double a
     if (i < n) y[i] = alpha * (sc[i] * x[i]);
 }
 
-// Plain streaming copy of n doubles: the reference kernel for HBM counter calibration.
+// y[idx[i]] = val[i]
 __global__ __launch_bounds__(kBlock) void sparse_set_kernel(int64_t n, const int64_t* __restrict__ idx,
                                                             const double* __restrict__ val, double* __restrict__ y) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) y[idx[i]] = val[i];
 }
+// Plain streaming copy of n doubles: the reference kernel for HBM counter calibration.
 __global__ __launch_bounds__(kBlock) void copy_kernel(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = a[i];
 }
@@ -2342,7 +2354,9 @@ static int tune_beams(Ctx* c) {
     return MH_OK;
 }
 
-static int enqueue_sweepx(Ctx* c, int OX, bool step, const double* c6, bool timed) {
+// fused: Lλx goes straight into the chain solve's blocks (each nz to its slot; the pc part
+// cleared first) instead of nzval, for a device Newton iteration that solves right after.
+static int enqueue_sweepx(Ctx* c, int OX, bool step, const double* c6, bool timed, bool fused = false) {
     hipStream_t s = c->stream;
     hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, s, c->d_flag, ~0ull);
     if (timed) HIPCHK(c, hipEventRecord(c->ev[0], s));
@@ -2379,8 +2393,18 @@ static int enqueue_sweepx(Ctx* c, int OX, bool step, const double* c6, bool time
     if (int r = check_launch(c, "element kernels")) return r;
     if (timed) HIPCHK(c, hipEventRecord(c->ev[1], s));
     if (c->nnz) {
-        if (c->coded) launch_scatter<true>(c->scatter_variant, s, c->nnz, c->d_kseg, c->d_ksrc, c->d_Kbuf, c->d_nzval);
-        else launch_scatter<false>(c->scatter_variant, s, c->nnz, c->d_kseg, c->d_ksrc, c->d_Kbuf, c->d_nzval);
+        double* out = c->d_nzval;
+        const int32_t* dst = nullptr;
+        if (fused) {
+            const int64_t pcbase = 3 * c->cnb * c->cB * c->cB + c->cnb * c->cB;
+            if (c->cnp)
+                hipLaunchKernelGGL(chain::fill_f64, dim3((unsigned)((4 * c->cnp + chain::kCB - 1) / chain::kCB)),
+                                   dim3(chain::kCB), 0, s, 4 * c->cnp, c->d_cbuf + pcbase, 0.0);
+            out = c->d_cbuf;
+            dst = c->d_cslot;
+        }
+        if (c->coded) launch_scatter<true>(c->scatter_variant, s, c->nnz, c->d_kseg, c->d_ksrc, c->d_Kbuf, out, dst);
+        else launch_scatter<false>(c->scatter_variant, s, c->nnz, c->d_kseg, c->d_ksrc, c->d_Kbuf, out, dst);
     }
     if (timed) HIPCHK(c, hipEventRecord(c->ev[2], s));
     if (c->nX) {
@@ -2884,8 +2908,10 @@ namespace {
 
 int64_t cgrid(int64_t n) { return (n + chain::kCB - 1) / chain::kCB; }
 
+// fused: the assembly already scattered Lλx into the blocks (enqueue_sweepx with fused),
+// after clearing pc; the pack is skipped.
 template <int B>
-int enqueue_chain_solve(Ctx* c, const double* rhs, bool timed = true) {
+int enqueue_chain_solve(Ctx* c, const double* rhs, bool timed = true, bool fused = false) {
     using namespace chain;
     hipStream_t s = c->stream;
     const int64_t nb = c->cnb, BB = (int64_t)B * B;
@@ -2896,10 +2922,11 @@ int enqueue_chain_solve(Ctx* c, const double* rhs, bool timed = true) {
     double* pc = b + nb * B;
     if (timed) HIPCHK(c, hipEventRecord(c->cev[0], s));
     hipLaunchKernelGGL(clear_flag, dim3(1), dim3(1), 0, s, c->d_cflag);
-    if (c->cnp) hipLaunchKernelGGL(fill_f64, dim3(cgrid(4 * c->cnp)), dim3(kCB), 0, s, 4 * c->cnp, pc, 0.0);
+    if (c->cnp && !fused) hipLaunchKernelGGL(fill_f64, dim3(cgrid(4 * c->cnp)), dim3(kCB), 0, s, 4 * c->cnp, pc, 0.0);
     if (c->cnu) hipLaunchKernelGGL(zero_slots_kernel, dim3(cgrid(c->cnu)), dim3(kCB), 0, s, c->cnu, c->d_cunc, D);
     if (c->cnd) hipLaunchKernelGGL(dummy_kernel<B>, dim3(cgrid(c->cnd)), dim3(kCB), 0, s, c->cnd, c->d_cdum, D, b);
-    if (c->nnz) hipLaunchKernelGGL(pack_kernel, dim3(cgrid(c->nnz)), dim3(kCB), 0, s, c->nnz, c->d_cslot, c->d_nzval, D, pc);
+    if (c->nnz && !fused)
+        hipLaunchKernelGGL(pack_kernel, dim3(cgrid(c->nnz)), dim3(kCB), 0, s, c->nnz, c->d_cslot, c->d_nzval, c->d_cbuf);
     hipLaunchKernelGGL(pack_rhs_kernel, dim3(cgrid(c->nX)), dim3(kCB), 0, s, c->nX, c->d_cpos, rhs, b, pc);
     if (c->cnp) hipLaunchKernelGGL(pair_kernel<B>, dim3(cgrid(c->cnp)), dim3(kCB), 0, s, c->cnp, c->d_chpos, pc, D, L,
                                    U, b, c->d_csaved, c->d_cflag);
@@ -2938,6 +2965,9 @@ int build_chain_slots(Ctx* c) {
     hipLaunchKernelGGL(fill_i32, dim3(cgrid(nslot + 1)), dim3(kCB), 0, s, nslot + 1, unc, 1);
     hipLaunchKernelGGL(slot_kernel<B>, dim3(cgrid(c->nX)), dim3(kCB), 0, s, c->nX, c->cnb, c->d_colptr, c->d_rowval,
                        c->d_cpos, c->d_chpos, c->d_cslot, unc, c->d_cflag);
+    if (c->nnz)
+        hipLaunchKernelGGL(absolute_slot_kernel, dim3(cgrid(c->nnz)), dim3(kCB), 0, s, c->nnz,
+                           nslot + c->cnb * B, c->d_cslot);
     size_t tmp_bytes = 0;
     void* tmp = nullptr;
     HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, unc, at, (int)(nslot + 1), s));
@@ -3192,8 +3222,9 @@ int mh_newton_iteration_graph(mh_ctx* c, int OX, int first, const double* c6, do
     if (!same) {
         free_graph(c, sl);
         HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int r = enqueue_sweepx(c, OX, step, c6, false);
-        if (r == MH_OK) r = c->cB == 6 ? enqueue_chain_solve<6>(c, c->d_rhs, false) : enqueue_chain_solve<3>(c, c->d_rhs, false);
+        int r = enqueue_sweepx(c, OX, step, c6, false, true);
+        if (r == MH_OK)
+            r = c->cB == 6 ? enqueue_chain_solve<6>(c, c->d_rhs, false, true) : enqueue_chain_solve<3>(c, c->d_rhs, false, true);
         if (r == MH_OK) {
             const hipError_t e = hipMemcpyAsync(c->h_cflag, c->d_cflag, chain::kStatusBytes, hipMemcpyDeviceToHost, s);
             if (e != hipSuccess) { c->err = std::string("capture: ") + hipGetErrorString(e); r = MH_ERR_HIP; }
