@@ -96,3 +96,24 @@ def test_newton_iteration_graph_matches_eager(mc):
     # the first and the later iterations of a step, each captured once for the run: the graphs
     # read the Newmark coefficients from device memory
     assert runs[True][1]["graph_captures"] == 2 and runs["assembly"][1]["graph_captures"] == 2
+
+
+def test_newton_iteration_graph_bar_chain(mc):
+    """A sagging bar chain (3-dof blocks) in SweepX{2} with the device solve: whole iterations
+    replayed from graphs, the assembly writing the chain blocks directly, give the eager
+    trajectory bit for bit. The bars' kernel takes the time, so each step captures its two
+    graphs anew."""
+    from tests.test_chain_layout import bar_chain
+    model = bar_chain(n=11)
+    st0 = mc.initialize(model, nXder=3, time=0.)
+    kw = dict(initialstate=st0, time=0.02 * np.arange(1, 7), maxΔx=1e-8, maxiter=30, verbose=False,
+              catcherror=False, linsolve="device")
+    runs = {}
+    for g in (False, True):
+        st = {}
+        runs[g] = (mc.solve(mc.SweepX[2], graph=g, stats=st, **kw), st)
+    for sa, sb in zip(runs[False][0], runs[True][0]):
+        for d in range(3):
+            assert np.array_equal(bits(sa.X[d]), bits(sb.X[d])), (sa.time, d)
+    assert np.abs(runs[True][0][-1].X[0]).max() > 1e-4   # the chain moved
+    assert runs[True][1]["graph_captures"] == 2 * 6
