@@ -103,3 +103,17 @@ def test_block_route_gradient_is_the_difference_quotient(problem):
         got = g["A"][i] if cls == "A" else g[cls][k, i]
         scale = np.abs(g[cls]).max()
         assert abs(got - fd) <= 1e-7 * scale, (cls, k, i, got, fd, scale)
+
+
+def test_allsteps_gradient_refuses_bad_arguments():
+    """AllStepsGradient refuses, before touching a device: fewer than 6 steps with time
+    derivatives (the finite-difference kernels need them, FiniteDifferences.jl:8-34), and residual
+    elements other than beams (bars read the launch's single time)."""
+    model, _ = gauge_identification(NEL, NSEG, NG, targets=np.zeros((NSTEP, NG)))
+    dis = mc.initialize(model).dis
+    with pytest.raises(mc.MuscadeError, match="≥6"):
+        mc.AllStepsGradient(2, 0, 1, model, dis, 5, DT)
+    from tests.test_chain_layout import bar_chain
+    bars = bar_chain(n=4)
+    with pytest.raises(mc.MuscadeError, match="beams"):
+        mc.AllStepsGradient(2, 0, 0, bars, mc.initialize(bars).dis, 8, DT)
