@@ -591,8 +591,9 @@ __global__ __launch_bounds__(kBlock) void beam_gauge_derivs_kernel(BeamDynArgs a
 // (mh::dyn_translation_direction: along a translation the rotation chain carries no ε), then
 // propagates the time jets with its own ε seed. The lanes of an element exchange the Jacobian
 // columns of x_gp and vₛₘ (and the translation load derivatives) through LDS, and each lane
-// writes the columns k of its directions, K[:,k] = (compact static part) +
-// Σ_gp dL·(J_xᵀ ∂fₑ/∂ε + J_vᵀ ∂mₑ/∂ε), or ∂R/∂δr.
+// forms the columns k of its directions, K[:,k] = (compact static part) +
+// Σ_gp dL·(J_xᵀ ∂fₑ/∂ε + J_vᵀ ∂mₑ/∂ε), in LDS (the workgroup then writes its elements' K blocks
+// as one contiguous range), or ∂R/∂δr.
 constexpr int kDirLanes = 16;              // the adjoint and DirectXUA kernels: 16 lanes per element
 constexpr int kDirElems = 16;
 constexpr int kDirBlock = kDirLanes * kDirElems;
@@ -604,6 +605,7 @@ __global__ __launch_bounds__(kDirWBlock) void beam_dyn_dir_kernel(BeamDynArgs a)
     constexpr int per = kDirPerW<STEP>, NE = kDirElemsW<STEP>;
     __shared__ double Jsh[NE][12][15];      // per dof: Jx (4 gp × 3), Jv (3)
     __shared__ double DFt[NE][6][12];       // per translation dof: ∂fₑ/∂ε (4 gp × 3); ∂mₑ/∂ε = 0
+    __shared__ double Ksh[NE][144];         // the elements' K blocks, written out as one contiguous range
     const int64_t e0 = xcd_tile(a.per_xcd) * NE;
     if (e0 >= a.nele) return;   // whole workgroup leaves together
     const int el = threadIdx.x / per, j = threadIdx.x - el * per;
@@ -640,39 +642,45 @@ __global__ __launch_bounds__(kDirWBlock) void beam_dyn_dir_kernel(BeamDynArgs a)
         }
     }
     __syncthreads();
-    if (!live) return;
-    double col[12], colt[12];
-    int nan = 0;
+    if (live) {
+        double col[12], colt[12];
+        int nan = 0;
 #pragma unroll
-    for (int i = 0; i < 12; ++i) {
-        const double(*Jx)[3] = (const double(*)[3])(&Jsh[el][i][0]);
-        col[i] = mh::dyn_row(G, Jx, &Jsh[el][i][12], d);
-        if (dir < 12) {
-            double t = 0.;
+        for (int i = 0; i < 12; ++i) {
+            const double(*Jx)[3] = (const double(*)[3])(&Jsh[el][i][0]);
+            col[i] = mh::dyn_row(G, Jx, &Jsh[el][i][12], d);
+            if (dir < 12) {
+                double t = 0.;
 #pragma unroll
-            for (int g = 0; g < 4; ++g)
-                t = t + ((Jx[g][0] * DFt[el][j][3 * g] + Jx[g][1] * DFt[el][j][3 * g + 1]) +
-                         Jx[g][2] * DFt[el][j][3 * g + 2]) * G.dL[g];
-            double m;
-            int slot = mh::compact_slot(i, dir, &m);
-            col[i] = m * a.Kc[e * mh::kCompact + slot] + col[i];
-            slot = mh::compact_slot(i, dir_t, &m);
-            colt[i] = m * a.Kc[e * mh::kCompact + slot] + t;
-            nan |= colt[i] != colt[i];
+                for (int g = 0; g < 4; ++g)
+                    t = t + ((Jx[g][0] * DFt[el][j][3 * g] + Jx[g][1] * DFt[el][j][3 * g + 1]) +
+                             Jx[g][2] * DFt[el][j][3 * g + 2]) * G.dL[g];
+                double m;
+                int slot = mh::compact_slot(i, dir, &m);
+                col[i] = m * a.Kc[e * mh::kCompact + slot] + col[i];
+                slot = mh::compact_slot(i, dir_t, &m);
+                colt[i] = m * a.Kc[e * mh::kCompact + slot] + t;
+                nan |= colt[i] != colt[i];
+            }
+            nan |= col[i] != col[i];
         }
-        nan |= col[i] != col[i];
+        if (dir < 12) {
+#pragma unroll
+            for (int i = 0; i < 12; ++i) Ksh[el][12 * dir + i] = col[i];
+#pragma unroll
+            for (int i = 0; i < 12; ++i) Ksh[el][12 * dir_t + i] = colt[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 12; ++i) a.dR[i * n + e] = col[i];
+        }
+        if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
     }
-    if (dir < 12) {
-        double* dst = a.K + e * 144;
-#pragma unroll
-        for (int i = 0; i < 12; ++i) dst[12 * dir + i] = col[i];
-#pragma unroll
-        for (int i = 0; i < 12; ++i) dst[12 * dir_t + i] = colt[i];
-    } else {
-#pragma unroll
-        for (int i = 0; i < 12; ++i) a.dR[i * n + e] = col[i];
-    }
-    if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
+    __syncthreads();
+    // the workgroup's elements are consecutive: their K blocks form one contiguous range
+    const int64_t nv = (n - e0 < NE ? n - e0 : NE) * 144;
+    double* dst = a.K + e0 * 144;
+    const double* src = &Ksh[0][0];
+    for (int64_t f = threadIdx.x; f < nv; f += kDirWBlock) dst[f] = src[f];
 }
 
 // ---- Bar3D -----------------------------------------------------------------
