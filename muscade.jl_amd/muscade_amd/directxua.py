@@ -244,7 +244,7 @@ class HostLagrangians:
                 gd = self.gauge_provider(it, state)
             for p, d, sl in self._chunks(it):
                 if gd is not None:
-                    p = dict(p, _gd=tuple(a[sl] for a in gd))
+                    p = dict(p, _gd=tuple(None if a is None else a[sl] for a in gd))
                 iX, iU, iA = d.X[sl] - 1, d.U[sl] - 1, d.A[sl] - 1
                 m, Nx, Nu, Na = iX.shape[0], iX.shape[1], iU.shape[1], iA.shape[1]
                 # partial layout and global dofs of each (class, derivative)
@@ -610,15 +610,18 @@ class AllStepsGradient:
             if model.eletypes[it].cls.kind not in (_lib.MH_KIND_BEAM, _lib.MH_KIND_BEAM_U):
                 raise MuscadeError("the all-steps adjoint takes beams as its residual elements")
         self.rep = _Replica(model, dis, types, self.nstep)
+        self._rindex = {it: k for k, it in enumerate(types)}
 
         class _RDis:
             pass
         rdis = _RDis()
         rdis.dis = self.rep.dis
         self.out = AssemblySweepX(OX, self.rep, rdis, device=device)
+        # the gauge requestables' derivatives of every step in one launch on the copies (X of all
+        # steps is on the device after the assembly), handed to the host Lagrangians step by step
+        self._gauge, self._step = {}, 0
         if self.host.costed:
-            from .results import ElementResults
-            self.host.gauge_provider = ElementResults(model, dis, device).gauge_derivatives
+            self.host.gauge_provider = lambda it, state: tuple(a[self._step] for a in self._gauge[it])
         self.timings = {}
 
     def __call__(self, Λ, X, U, A, time):
@@ -637,6 +640,16 @@ class AllStepsGradient:
         R = np.zeros(n * nX)
         self.out.ctx.check(lib.mh_get_rhs(self.out.ctx.h, dptr(R)), "rhs")
         gXd, gU0, gA = self.out.vjp(st, st.Λ[0], self.OX, upload=False)
+        if self.host.costed:
+            from .results import gauge_derivatives_on
+            for it in self.host.costed:
+                m = self.model.eletypes[it].nele
+                # the Hessians are read only by costs built with exact=True
+                hess = any(p.get("exact", False) for p, _, _ in self.host._chunks(it))
+                v, g_, H = gauge_derivatives_on(self.out.ctx, self.out.groups[self._rindex[it]], None, n * m,
+                                                upload=False, hessians=hess)
+                self._gauge[it] = (v.reshape(n, m, 4), g_.reshape(n, m, 4, 12),
+                                   H.reshape(n, m, 4, 12, 12) if hess else [None] * n)
         t1 = clock()
         R = R.reshape(n, nX)
         L = float(np.sum(Λ * R))
@@ -648,6 +661,7 @@ class AllStepsGradient:
         for k in range(n):
             sk = _StepState(self.model, dis, float(time[k]), [Λ[k]], [x[k] for x in Xd], [u[k] for u in Ud], A,
                             dict(γ=0., iter=1))
+            self._step = k
             L1, _ = self.host.step(sk, Λgradient=True, pieces=False)
             L += self.host.value
             gΛ[k] += L1["Λ"][0] / sc["Λ"]

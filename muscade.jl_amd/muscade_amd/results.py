@@ -88,12 +88,17 @@ class ElementResults:
         return gauge_derivatives_on(self.ctx, self.groups[ieletyp], state, self.model.eletypes[ieletyp].nele)
 
 
-def gauge_derivatives_on(ctx, group, state, nele):
+def gauge_derivatives_on(ctx, group, state, nele, upload=True, hessians=True):
     """(values (nele,4), gradients (nele,4,12), Hessians (nele,4,12,12)) of εₐₓ, κ₁, κ₂, κ₃
-    w.r.t. the element X-dofs of a beam group at state.X[0] (mh_beam_gauge_derivs)."""
-    ctx.check(lib.mh_set_X(ctx.h, 0, dptr(as_f64(state.X[0]))), "set X")
+    w.r.t. the element X-dofs of a beam group at state.X[0] (mh_beam_gauge_derivs);
+    upload=False: at the X already on the device (state unused); hessians=False: None in
+    their place (for first-order partials)."""
+    if upload:
+        ctx.check(lib.mh_set_X(ctx.h, 0, dptr(as_f64(state.X[0]))), "set X")
     raw = np.zeros((nele, 364))
     ctx.check(lib.mh_beam_gauge_derivs(ctx.h, group, 0, None, dptr(raw)), "gauge derivatives")
+    if not hessians:
+        return raw[:, :4], raw[:, 4:52].reshape(nele, 4, 12), None
     H = np.zeros((nele, 4, 12, 12))
     iu = np.triu_indices(12)
     h78 = raw[:, 52:].reshape(nele, 4, 78)
