@@ -186,34 +186,50 @@ template <int NT, bool E> MH_HD Jet<NT, E> jscac(const Jet<NT, E>& a) {
 template <int NT, bool E> struct JM3 { Jet<NT, E> a[9]; };   // column-major 3×3
 template <int NT, bool E> struct JV3 { Jet<NT, E> a[3]; };
 
-// Rodrigues(v) = I + sinc1(θ) S + (sinc1(θ/2)²/2) spin²(S), θ = norm3(v) (Rotations.jl:106-135).
+// Rodrigues(v) = I + sinc1(θ) S + (sinc1(θ/2)²/2) spin²(S), θ = norm3(v) (Rotations.jl:106-135):
+// the coefficient jets a = sinc1(θ), b = sinc1(θ/2)²/2.
 // XSQ = false: the square by Muscade's power rule on jets over solver partials (the
 // assembly); XSQ = true: the exact square (jets over plain values, as getresult evaluates).
-template <int NT, bool E, bool XSQ = false> MH_HD JM3<NT, E> jrodrigues(const JV3<NT, E>& v) {
+template <int NT, bool E, bool XSQ = false>
+MH_HD void jrodrigues_coef(const JV3<NT, E>& v, Jet<NT, E>& a, Jet<NT, E>& b) {
     using J = Jet<NT, E>;
     const J& x = v.a[0]; const J& y = v.a[1]; const J& z = v.a[2];
     J ss = (x * x + y * y) + z * z;
     J th;
     if (sqrt(ss.v[0]) < 1e-14) th = jzero<NT, E>();
     else th = jsqrt(ss);
-    J a = jsinc1(th);
+    a = jsinc1(th);
     J s = jsinc1(0.5 * th);
-    J b;
     if constexpr (XSQ) b = 0.5 * (s * s);
     else b = 0.5 * jsq_muscade(s);
-    // spin²(S) with S = spin(v) (Rotations.jl:114-122)
-    J ab = x * y, ca = z * x, bc = y * z;
-    J ma2 = -(x * x), mb2 = -(y * y), mc2 = -(z * z);
+}
+// Column k of Rodrigues(v) from its coefficients, spin²(S) entrywise (Rotations.jl:114-122).
+template <int K, int NT, bool E>
+MH_HD JV3<NT, E> jrodrigues_col(const JV3<NT, E>& v, const Jet<NT, E>& a, const Jet<NT, E>& b) {
+    const Jet<NT, E>& x = v.a[0]; const Jet<NT, E>& y = v.a[1]; const Jet<NT, E>& z = v.a[2];
+    JV3<NT, E> r;
+    if constexpr (K == 0) {
+        r.a[0] = b * (-(z * z) + -(y * y)) + 1.;
+        r.a[1] = a * z + b * (x * y);
+        r.a[2] = -(a * y) + b * (z * x);
+    } else if constexpr (K == 1) {
+        r.a[0] = -(a * z) + b * (x * y);
+        r.a[1] = b * (-(x * x) + -(z * z)) + 1.;
+        r.a[2] = a * x + b * (y * z);
+    } else {
+        r.a[0] = a * y + b * (z * x);
+        r.a[1] = -(a * x) + b * (y * z);
+        r.a[2] = b * (-(x * x) + -(y * y)) + 1.;
+    }
+    return r;
+}
+template <int NT, bool E, bool XSQ = false> MH_HD JM3<NT, E> jrodrigues(const JV3<NT, E>& v) {
+    Jet<NT, E> a, b;
+    jrodrigues_coef<NT, E, XSQ>(v, a, b);
+    JV3<NT, E> c0 = jrodrigues_col<0>(v, a, b), c1 = jrodrigues_col<1>(v, a, b), c2 = jrodrigues_col<2>(v, a, b);
     JM3<NT, E> r;
-    r.a[0] = b * (mc2 + mb2) + 1.;
-    r.a[1] = a * z + b * ab;
-    r.a[2] = -(a * y) + b * ca;
-    r.a[3] = -(a * z) + b * ab;
-    r.a[4] = b * (ma2 + mc2) + 1.;
-    r.a[5] = a * x + b * bc;
-    r.a[6] = a * y + b * ca;
-    r.a[7] = -(a * x) + b * bc;
-    r.a[8] = b * (ma2 + mb2) + 1.;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { r.a[i] = c0.a[i]; r.a[3 + i] = c1.a[i]; r.a[6 + i] = c2.a[i]; }
     return r;
 }
 // Rodrigues⁻¹(m) = spin⁻¹(m)/scac((tr m - 1)/2) (Rotations.jl:101)
@@ -284,15 +300,51 @@ template <int NT, bool E> MH_HD Jet<1, E> jslice(const Jet<NT, E>& a, int n) {
 }
 
 // The rotation chain (v₁,v₂) ↦ (rₛₘ, Δv) of BeamElement.jl:192-208 on time jets.
+// Neither R₂ = Rodrigues(v₂) nor the relative rotation M = R₂R₁ᵀ is held whole: φ = Rodrigues⁻¹(M)
+// reads only tr M = Σₖ R₁eₖ·R₂eₖ and the axial vector of M − Mᵀ, Σₖ R₁eₖ × R₂eₖ, so R₂ is
+// formed a column at a time; rₛₘ = Rodrigues(Δv)·(R₁ rₘ) applies Rodrigues(Δv) to one column
+// of R₁rₘ at a time, w ↦ w + a Δv×w + b (Δv (Δv·w) − |Δv|² w). This keeps the live jets
+// of the chain to about 21 instead of the 27 of three whole matrices.
 template <int NT, bool E, bool XSQ = false>
 MH_HD void dyn_rotation_chain(const BeamGeo& G, const JV3<NT, E>& v1, const JV3<NT, E>& v2, JM3<NT, E>& rsm,
                               JV3<NT, E>& dv) {
+    using J = Jet<NT, E>;
     JM3<NT, E> R1 = jrodrigues<NT, E, XSQ>(v1);
-    JM3<NT, E> M = jmulABt(jrodrigues<NT, E, XSQ>(v2), R1);
-    JV3<NT, E> phi = jrodrigues_inv(M);
+    J tr, ax[3];
+    {
+        J a2, b2;
+        jrodrigues_coef<NT, E, XSQ>(v2, a2, b2);
+        auto add = [&](const JV3<NT, E>& c2, int k, bool first) {
+            const J* c1 = &R1.a[3 * k];
+            J d = (c1[0] * c2.a[0] + c1[1] * c2.a[1]) + c1[2] * c2.a[2];
+            J x = c1[1] * c2.a[2] - c1[2] * c2.a[1];
+            J y = c1[2] * c2.a[0] - c1[0] * c2.a[2];
+            J z = c1[0] * c2.a[1] - c1[1] * c2.a[0];
+            if (first) { tr = d; ax[0] = x; ax[1] = y; ax[2] = z; }
+            else { tr = tr + d; ax[0] = ax[0] + x; ax[1] = ax[1] + y; ax[2] = ax[2] + z; }
+        };
+        add(jrodrigues_col<0>(v2, a2, b2), 0, true);
+        add(jrodrigues_col<1>(v2, a2, b2), 1, false);
+        add(jrodrigues_col<2>(v2, a2, b2), 2, false);
+    }
+    // Rodrigues⁻¹(M) = spin⁻¹(M)/scac((tr M - 1)/2) (Rotations.jl:101), Δv = φ/2
+    J rs = jrecip(jscac(0.5 * (tr + (-1.))));
 #pragma unroll
-    for (int c = 0; c < 3; ++c) dv.a[c] = 0.5 * phi.a[c];
-    rsm = jmul_const(jmul(jrodrigues<NT, E, XSQ>(dv), R1), G.rm);
+    for (int c = 0; c < 3; ++c) dv.a[c] = 0.5 * ((0.5 * ax[c]) * rs);
+    JM3<NT, E> R1m = jmul_const(R1, G.rm);
+    J ah, bh;
+    jrodrigues_coef<NT, E, XSQ>(dv, ah, bh);
+    const J& x = dv.a[0]; const J& y = dv.a[1]; const J& z = dv.a[2];
+    J n2 = (x * x + y * y) + z * z;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const J* w = &R1m.a[3 * j];
+        J cx = y * w[2] - z * w[1], cy = z * w[0] - x * w[2], cz = x * w[1] - y * w[0];
+        J s = (x * w[0] + y * w[1]) + z * w[2];
+        rsm.a[3 * j] = (w[0] + ah * cx) + bh * (x * s - n2 * w[0]);
+        rsm.a[3 * j + 1] = (w[1] + ah * cy) + bh * (y * s - n2 * w[1]);
+        rsm.a[3 * j + 2] = (w[2] + ah * cz) + bh * (z * s - n2 * w[2]);
+    }
 }
 // A time jet with no ε part as one whose ε part is zero.
 template <int NT, bool E> MH_HD Jet<NT, E> jlift(const Jet<NT, false>& a) {
@@ -653,15 +705,21 @@ MH_HD void dyn_translation_direction(const BeamGeo& G, const BeamDynSection& D, 
 #pragma unroll
     for (int q = 0; q < 9; ++q) { R[0][q] = kb[q * ks]; R[1][q] = kb[(9 + q) * ks]; R[2][q] = kb[(18 + q) * ks]; }
     // Wᵢⱼ = rₛₘ(i,j) ⊛ rₛₘ(c,j): the gp-independent products of M_gp(i,c) = Σⱼ Yⱼ Wᵢⱼ
+    // row cc of R by selects (an index known only at run time would put R in scratch memory)
+    double Rc[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Rc[m][j] = cc == 0 ? R[m][3 * j] : (cc == 1 ? R[m][3 * j + 1] : R[m][3 * j + 2]);
     double W[3][3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const int a = i + 3 * j, b = cc + 3 * j;
-            W[i][j][0] = R[0][a] * R[0][b];
-            W[i][j][1] = R[1][a] * R[0][b] + R[0][a] * R[1][b];
-            W[i][j][2] = (R[2][a] * R[0][b] + 2. * (R[1][a] * R[1][b])) + R[0][a] * R[2][b];
+            const int a = i + 3 * j;
+            W[i][j][0] = R[0][a] * Rc[0][j];
+            W[i][j][1] = R[1][a] * Rc[0][j] + R[0][a] * Rc[1][j];
+            W[i][j][2] = (R[2][a] * Rc[0][j] + 2. * (R[1][a] * Rc[1][j])) + R[0][a] * Rc[2][j];
         }
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
