@@ -35,8 +35,8 @@ F_ELEM = 6299
 # The same for the dynamic beam of the catenary chains (OX = 2): value pass + direction pass
 # per beam, and their HBM bytes per assembly from FETCH_SIZE/WRITE_SIZE
 # (profiles/catenary_chains_256_kernels.md), by mission.
-F_DYN = {"iter": 10565 + 69935, "step": 10565 + 78130}
-DYN_TRAFFIC = {"iter": 13.37e9 + 28.57e9, "step": 13.37e9 + 32.53e9}
+F_DYN = {"iter": 10890 + 75981, "step": 10890 + 84845}
+DYN_TRAFFIC = {"iter": 12.85e9 + 13.77e9, "step": 12.85e9 + 14.39e9}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "space_frame_100x100x33_traffic.json")
 
 
