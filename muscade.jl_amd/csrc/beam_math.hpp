@@ -273,7 +273,7 @@ struct BeamOut { double R[12]; double Kc[kCompact]; int nan; };
 //                damping, roll inertia). vₛₘ = Rodrigues⁻¹(rₛₘ) (BeamElement.jl:203).
 template <bool GEN = false>
 MH_HD void beam_static(const BeamGeo& G, const BeamSection& S, v3 fe, const double* X, BeamOut& out,
-                       const v3* eg = nullptr, v3 mtot = v3{0., 0., 0.}) {
+                       const v3* eg = nullptr, v3 mtot = v3{0., 0., 0.}, double* hms = nullptr, int hst = 0) {
     const double L = G.L;
     v3 u1 = mk(X[0], X[1], X[2]), v1 = mk(X[3], X[4], X[5]);
     v3 u2 = mk(X[6], X[7], X[8]), v2 = mk(X[9], X[10], X[11]);
@@ -384,6 +384,9 @@ MH_HD void beam_static(const BeamGeo& G, const BeamSection& S, v3 fe, const doub
         v3 BJ0 = smul(B, J0), BJ1 = smul(B, J1), BJ2 = smul(B, J2);
         Hm.xx = -dot(J0, BJ0); Hm.yy = -dot(J1, BJ1); Hm.zz = -dot(J2, BJ2);
         Hm.xy = -dot(J0, BJ1); Hm.xz = -dot(J0, BJ2); Hm.yz = -dot(J1, BJ2);
+        // parked (LDS, stride hst) until Hφφ' is formed: fewer registers live through the
+        // rotation Jacobians
+        if (hms) { hms[0] = Hm.xx; hms[hst] = Hm.yy; hms[2 * hst] = Hm.zz; hms[3 * hst] = Hm.xy; hms[4 * hst] = Hm.xz; hms[5 * hst] = Hm.yz; }
     }
     v3 gD = Fv;
     // translational residual R = Aᵀ g_z on u₁, u₂
@@ -459,6 +462,7 @@ MH_HD void beam_static(const BeamGeo& G, const BeamSection& S, v3 fe, const doub
         Hpp.xy = 0.5 * (Fu.x * d.y + d.x * Fu.y) + 0.5 * (Fv.x * dv.y + dv.x * Fv.y) + Hee.xy;
         Hpp.xz = 0.5 * (Fu.x * d.z + d.x * Fu.z) + 0.5 * (Fv.x * dv.z + dv.x * Fv.z) + Hee.xz;
         Hpp.yz = 0.5 * (Fu.y * d.z + d.y * Fu.z) + 0.5 * (Fv.y * dv.z + dv.y * Fv.z) + Hee.yz;
+        if (GEN && hms) { Hm.xx = hms[0]; Hm.yy = hms[hst]; Hm.zz = hms[2 * hst]; Hm.xy = hms[3 * hst]; Hm.xz = hms[4 * hst]; Hm.yz = hms[5 * hst]; }
         if (GEN) {
             Hpp.xx += Hm.xx; Hpp.yy += Hm.yy; Hpp.zz += Hm.zz;
             Hpp.xy += Hm.xy; Hpp.xz += Hm.xz; Hpp.yz += Hm.yz;

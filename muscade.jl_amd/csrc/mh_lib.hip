@@ -494,6 +494,7 @@ __device__ __forceinline__ void dyn_load(const BeamDynArgs& a, int64_t e, mh::Be
 template <int ND, bool UDOF>
 __global__ __launch_bounds__(kBeamBlock) void beam_dyn_value_kernel(BeamDynArgs a) {
     __shared__ double tile[kBeamBlock * mh::kCompact];
+    __shared__ double hmsh[6 * kBeamBlock];   // each lane's couple Hessian, parked (stride kBeamBlock)
     const int64_t e0 = xcd_tile(a.per_xcd) * kBeamBlock;
     if (e0 >= a.nele) return;   // whole workgroup leaves together
     const int tid = threadIdx.x;
@@ -508,7 +509,7 @@ __global__ __launch_bounds__(kBeamBlock) void beam_dyn_value_kernel(BeamDynArgs 
         mh::v3 eg[4], mtot;
         mh::dyn_frozen_loads<ND>(G, D, X, U, eg, mtot, a.Kb ? a.Kb + e : nullptr, n);
         mh::BeamOut out;
-        mh::beam_static<true>(G, S, mh::mk(0., 0., 0.), X[0], out, eg, mtot);
+        mh::beam_static<true>(G, S, mh::mk(0., 0., 0.), X[0], out, eg, mtot, hmsh + tid, kBeamBlock);
         if (a.R) {
 #pragma unroll
             for (int i = 0; i < 12; ++i) a.R[i * n + e] = out.R[i] * a.sR[i];
