@@ -36,7 +36,7 @@ F_ELEM = 6299
 # per beam, and their HBM bytes per assembly from FETCH_SIZE/WRITE_SIZE
 # (profiles/catenary_chains_256_kernels.md), by mission.
 F_DYN = {"iter": 10890 + 75981, "step": 10890 + 84845}
-DYN_TRAFFIC = {"iter": 12.85e9 + 13.77e9, "step": 12.85e9 + 14.39e9}
+DYN_TRAFFIC = {"iter": 9.17e9 + 13.77e9, "step": 9.17e9 + 14.39e9}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "space_frame_100x100x33_traffic.json")
 
 
