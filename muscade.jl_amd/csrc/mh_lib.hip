@@ -793,22 +793,24 @@ __global__ __launch_bounds__(kBeamBlock) void beam_vjp_static_kernel(VjpArgs a) 
     mh::BeamSection S;
     S.EA = a.sec[e]; S.EI2 = a.sec[n + e]; S.EI3 = a.sec[2 * n + e]; S.GJ = a.sec[3 * n + e]; S.w = a.sec[6 * n + e];
     mh::v3 fe = mh::mk(0., 0., S.w);
-    if (UDOF) fe = fe - mh::mk(a.U[a.uidx[e]], a.U[a.uidx[n + e]], a.U[a.uidx[2 * n + e]]);
+    int nan = 0;
+    if (UDOF) {
+        fe = fe - mh::mk(a.U[a.uidx[e]], a.U[a.uidx[n + e]], a.U[a.uidx[2 * n + e]]);
+        // ∂L/∂U first, so that nothing but λ stays live through the closed form
+        mh::DynY y;
+        mh::dyn_y(G, X, lam, y);
+        const mh::v3 gu = mh::dyn_vjp_u(G, y);
+        a.gu[e] = gu.x; a.gu[n + e] = gu.y; a.gu[2 * n + e] = gu.z;
+    }
     mh::BeamOut out;
     mh::beam_static(G, S, fe, X[0], out);
-    int nan = out.nan;
+    nan |= out.nan;
 #pragma unroll
     for (int i = 0; i < 12; ++i) {
         double g = 0.;
         for (int j = 0; j < 12; ++j) g = g + mh::compact_entry(out.Kc, i, j) * lam[j];
         a.g[i * n + e] = g;
         nan |= g != g;
-    }
-    if (UDOF) {
-        mh::DynY y;
-        mh::dyn_y(G, X, lam, y);
-        const mh::v3 gu = mh::dyn_vjp_u(G, y);
-        a.gu[e] = gu.x; a.gu[n + e] = gu.y; a.gu[2 * n + e] = gu.z;
     }
     if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
 }
