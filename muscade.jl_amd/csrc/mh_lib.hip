@@ -1062,17 +1062,22 @@ __global__ __launch_bounds__(kDirBlock) void beam_direct_kernel(DirectArgs a) {
     const int64_t n = a.nele;
     const bool live = e < n && k <= 12;
     mh::BeamGeo G;
-    mh::BeamDynSection D;
-    double X[ND][12], U[3];
     const double one[12] = {1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1., 1.};
     mh::DynDir d0;
-    if (live) {
+    // the element's data from memory (again for the higher time orders, rather than held
+    // live across the barrier)
+    auto load = [&](mh::BeamGeo& Gl, mh::BeamDynSection& Dl, double (*Xl)[12], double* Ul) {
         mh::BeamSection S;
         BeamDynArgs b;
         b.nele = a.nele; b.idx = a.idx; b.uidx = a.uidx; b.nodes = a.nodes; b.xyz = a.xyz; b.o2 = a.o2; b.sec = a.sec;
         for (int d = 0; d < 3; ++d) b.X[d] = a.X[d];
         b.U = a.U;
-        dyn_load<ND, UDOF>(b, e, G, S, D, X, U);
+        dyn_load<ND, UDOF>(b, e, Gl, S, Dl, Xl, Ul);
+    };
+    if (live) {
+        mh::BeamDynSection D;
+        double X[ND][12], U[3];
+        load(G, D, X, U);
         if (k < 12) {
             const double c0[6] = {0., 0., 0., 0., 0., 0.};
             mh::dyn_direction<ND>(G, D, X, U, one, c0, k, d0, 1.);
@@ -1099,15 +1104,12 @@ __global__ __launch_bounds__(kDirBlock) void beam_direct_kernel(DirectArgs a) {
                 }
         }
     } else {
-        for (int der = 0; der < ND; ++der) {
-            mh::DynDir d = d0;
-            if (der > 0) {
-                const double c[6] = {der == 1 ? 1. : 0., 0., 0., der == 2 ? 1. : 0., 0., 0.};
-                mh::dyn_direction<ND>(G, D, X, U, one, c, k, d, 0.);
-            }
+        // column k of the block of each time order; d0 (order 0) is consumed first, so that it is
+        // not live through the directions of the higher orders
+        auto rows = [&](int der, const mh::DynDir& d, const mh::BeamGeo& Gx) {
             double* dst = a.Kd + der * a.nKd + e * 144 + 12 * k;
             for (int i = 0; i < 12; ++i) {
-                double v = mh::dyn_row(G, (const double(*)[3])(&Jsh[el][i][0]), &Jsh[el][i][12], d);
+                double v = mh::dyn_row(Gx, (const double(*)[3])(&Jsh[el][i][0]), &Jsh[el][i][12], d);
                 if (der == 0) {
                     double m;
                     const int slot = mh::compact_slot(i, k, &m);
@@ -1116,6 +1118,17 @@ __global__ __launch_bounds__(kDirBlock) void beam_direct_kernel(DirectArgs a) {
                 dst[i] = (v * a.sX[k]) * a.dt;
                 nan |= dst[i] != dst[i];
             }
+        };
+        rows(0, d0, G);
+        for (int der = 1; der < ND; ++der) {
+            mh::BeamGeo Gd;
+            mh::BeamDynSection D;
+            double X[ND][12], U[3];
+            load(Gd, D, X, U);
+            mh::DynDir d;
+            const double c[6] = {der == 1 ? 1. : 0., 0., 0., der == 2 ? 1. : 0., 0., 0.};
+            mh::dyn_direction<ND>(Gd, D, X, U, one, c, k, d, 0.);
+            rows(der, d, Gd);
         }
     }
     if (nan) atomicMin(a.flag, a.gtag | (unsigned long long)e);
