@@ -166,7 +166,10 @@ MH_HD int bar_vjp_element(const BarGeo& G, const BarSection& S, const double (*X
         for (int k = 0; k < 6; ++k) {
             double g = 0.;
             for (int i = 0; i < 6; ++i) g = g + lam[i] * K[i + 6 * k];
-            gX[der][k] = g;
+            // constant row indices (der is a loop counter: an index by it would put gX in scratch)
+            if (der == 0) gX[0][k] = g;
+            else if (ND > 1 && der == 1) gX[ND > 1 ? 1 : 0][k] = g;
+            else if (ND > 2) gX[ND > 2 ? 2 : 0][k] = g;
         }
     }
     // ∂Rⱼ/∂U_q = -Σ_gp w ψⱼ δ(j mod 3, q)
